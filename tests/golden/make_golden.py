@@ -1,0 +1,224 @@
+"""Write the reference-produced golden fixtures under tests/golden/*.npz.
+
+Runs ONLY in the build container, where the reference is mounted read-only at
+/root/reference. It imports the reference's NumPy core (`libs.core`, never
+`main`) for the functions that no closed form or recorded value pins to
+round-off: the composed step (1-3 steps on tiny grids, and 200 steps of the
+128x4x20 lock exchange), `tke_step` on stratified partial-cell columns,
+`momentum_advection_up3` next to walls, the pressure gradient with partial
+cells, the masked Coriolis averages at coasts, the Helmholtz solvers'
+trajectories on arbitrary coefficients, and `Domain` geometry. Only inputs
+and outputs are stored; the GPU box never sees the reference.
+
+Usage:  python tests/golden/make_golden.py [--out tests/golden]
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+import zlib
+from pathlib import Path
+
+import numpy as np
+
+REF = Path("/root/reference")
+sys.path.insert(0, str(REF))
+sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+
+from libs.core.bathymetry import build_bathymetry  # noqa: E402
+from libs.core.closure import TKEParams, tke_step  # noqa: E402
+from libs.core.advection_v06 import momentum_advection_up3  # noqa: E402
+from libs.core.domain import Domain  # noqa: E402
+from libs.core.grid import CGrid  # noqa: E402
+from libs.core.model3d import Physics3D, State3D  # noqa: E402
+from libs.core.model3d_v05 import PhysicsV05, Stepper3DV05  # noqa: E402
+from libs.core.operators_masked import avg_u_to_v_m, avg_v_to_u_m  # noqa: E402
+from libs.core.schemes import SchemeParams  # noqa: E402
+from libs.core.solvers_var import HelmholtzVar  # noqa: E402
+
+
+def _domain(nx, ny, nz, Lx, Ly, H0, topo, bc_x, bc_y, island=False, seed=7):
+    grid = CGrid(nx=nx, ny=ny, Lx=Lx, Ly=Ly, nz=nz, depth=H0)
+    H, mask = build_bathymetry(grid, topo, H0, r_target=0.25, seed=seed, island=island,
+                               h_rel=0.6)
+    return Domain(grid=grid, H=H, mask=mask, vcoord="zlevel", bc_x=bc_x, bc_y=bc_y)
+
+
+def _geometry(d: Domain, prefix: str) -> dict:
+    dzu, dzv = d.face_thickness(d.dz3_ref)
+    return {f"{prefix}H": d.H, f"{prefix}mask": d.mask, f"{prefix}dz3": d.dz3_ref,
+            f"{prefix}mask3": d.mask3, f"{prefix}mask3u": d.mask3u, f"{prefix}mask3v": d.mask3v,
+            f"{prefix}mask_u": d.mask_u, f"{prefix}mask_v": d.mask_v,
+            f"{prefix}dz3u": dzu, f"{prefix}dz3v": dzv, f"{prefix}zc": d.depth_at_centres()}
+
+
+def _random_state(d: Domain, rng, tracers, e=True):
+    g = d.grid
+    nz, ny, nx = g.shape3d
+    depth = d.depth_at_centres()
+    u = 0.05 * rng.standard_normal(g.shape3d) * d.mask3u
+    v = 0.05 * rng.standard_normal(g.shape3d) * d.mask3v
+    eta = 0.02 * rng.standard_normal(g.shape) * d.mask
+    T = (4.0 + 14.0 * np.exp(-depth / 300.0) + 0.05 * rng.standard_normal(g.shape3d)) * d.mask3
+    S = (35.0 + 0.1 * rng.standard_normal(g.shape3d)) * d.mask3
+    ee = (1e-6 + 1e-4 * rng.random((nz + 1, ny, nx))) if e else None
+    b = 0.01 * rng.standard_normal(g.shape3d) * d.mask3
+    if tracers != "TS":
+        T = S = None
+    return dict(u=u, v=v, eta=eta, T=T, S=S, e=ee, b=b)
+
+
+from golden_cases import COMPOSED  # noqa: E402  (tests/golden_cases.py)
+
+
+def composed_fixtures() -> dict:
+    out = {}
+    for name, (dk, pk, vk, sk, dt, steps) in COMPOSED.items():
+        d = _domain(**dk)
+        phys = PhysicsV05(base=Physics3D(**pk), **vk)
+        params = SchemeParams(**sk)
+        rng = np.random.default_rng(zlib.crc32(name.encode()))
+        st0 = _random_state(d, rng, pk["tracers"], e=vk.get("closure") == "tke")
+        stepper = Stepper3DV05(d, phys, params, dt)
+        if pk["tracers"] == "TS":
+            st0["b"] = stepper.buoyancy_of(st0["T"], st0["S"]) * d.mask3
+        s = State3D(u=st0["u"], v=st0["v"], b=st0["b"], eta=st0["eta"], t=0.0,
+                    T=st0["T"], S=st0["S"], e=st0["e"])
+        rec = _geometry(d, "geo_")
+        for k, v in st0.items():
+            if v is not None:
+                rec[f"in_{k}"] = v
+        for n in range(1, steps + 1):
+            s = stepper.step(s)
+            for k in ("u", "v", "b", "eta", "T", "S", "e"):
+                val = getattr(s, k)
+                if val is not None:
+                    rec[f"s{n}_{k}"] = val
+            rec[f"s{n}_nu"] = stepper.nu
+            rec[f"s{n}_kappa"] = stepper.kappa
+            rec[f"s{n}_iters"] = np.array(stepper.solver_iterations)
+        rec["counters"] = np.array([stepper.solver_iterations, stepper.solver_failures,
+                                    stepper.solver_rebuilds, stepper.tridiagonal_solves,
+                                    stepper.eos_cells])
+        out[name] = rec
+    return out
+
+
+def lock_exchange_fixture() -> dict:
+    """The 128x4x20 lock exchange: flat 20 m, Lx=64 km, Ly=4 km, closed in x
+    and periodic in y, linear EOS rho0=1000 alpha_T=2e-4 T0=5, T = 5|35,
+    S = 35, up3_tvd, TKE (integral), theta 0.5 PCG-Jacobi rtol 1e-12, dt=1 s."""
+    grid = CGrid(nx=128, ny=4, Lx=6.4e4, Ly=4.0e3, nz=20, depth=20.0)
+    H = np.full(grid.shape, 20.0)
+    mask = np.ones(grid.shape)
+    d = Domain(grid=grid, H=H, mask=mask, vcoord="zlevel", bc_x="closed", bc_y="periodic")
+    base = Physics3D(g=9.80616, f=0.0, H=20.0, N2=0.0, nu=1e-5, kappa=1e-6, rho0=1000.0,
+                     tau_x=0.05, tau_y=0.0, tracers="TS", alpha_T=2e-4, beta_S=7.4e-4,
+                     T0=5.0, S0=35.0, A_h=10.0, K_h=10.0)
+    phys = PhysicsV05(base=base, eos="linear", pgf="keep", closure="tke", mxl="integral")
+    params = SchemeParams(name="theta", theta=0.5, theta_cor=0.5, n_picard=2, theta_v=0.5,
+                          advection="up3_tvd", solver="pcg_jacobi", rtol=1e-12, max_iter=2000)
+    st = Stepper3DV05(d, phys, params, 1.0)
+    x, _ = grid.coords("eta")
+    warm = (x > 0.5 * grid.Lx).astype(np.float64)
+    T = (5.0 + 30.0 * warm)[None] * np.ones(grid.shape3d)
+    S = np.full(grid.shape3d, 35.0)
+    b = st.buoyancy_of(T, S) * d.mask3
+    zero = np.zeros(grid.shape3d)
+    s = State3D(u=zero, v=zero, b=b, eta=np.zeros(grid.shape), t=0.0,
+                T=T * d.mask3, S=S * d.mask3)
+    rec = {"in_T": s.T, "in_S": s.S, "in_b": s.b}
+    iters = []
+    for n in range(1, 201):
+        before = st.solver_iterations
+        s = st.step(s)
+        iters.append(st.solver_iterations - before)
+        if n in (1, 10, 200):
+            for k in ("u", "v", "b", "eta", "T", "S", "e"):
+                rec[f"s{n}_{k}"] = getattr(s, k)
+            rec[f"s{n}_nu"] = st.nu
+            rec[f"s{n}_kappa"] = st.kappa
+    rec["iters_per_step"] = np.array(iters)
+    rec["counters"] = np.array([st.solver_iterations, st.solver_failures, st.solver_rebuilds,
+                                st.tridiagonal_solves, st.eos_cells])
+    return rec
+
+
+def operator_fixtures() -> dict:
+    rng = np.random.default_rng(20261202)
+    d = _domain(nx=24, ny=16, nz=10, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
+                bc_x="closed", bc_y="periodic", island=True, seed=11)
+    g = d.grid
+    rec = _geometry(d, "geo_")
+    dzu, dzv = d.face_thickness(d.dz3_ref)
+    # tke_step on stratified partial-cell columns, both mixing-length forms
+    nz1 = g.nz + 1
+    depth = d.depth_at_centres()
+    e = 1e-6 + 3e-4 * rng.random((nz1,) + g.shape)
+    u = 0.2 * rng.standard_normal(g.shape3d) * d.mask3u
+    v = 0.2 * rng.standard_normal(g.shape3d) * d.mask3v
+    b = (2e-3 * np.exp(-depth / 400.0) + 2e-5 * rng.standard_normal(g.shape3d)) * d.mask3
+    Km = 1e-4 + 1e-2 * rng.random((nz1,) + g.shape)
+    Kh = 1e-5 + 1e-2 * rng.random((nz1,) + g.shape)
+    ustar2 = 1e-4 * (1.0 + rng.random(g.shape))
+    rec.update(tke_e=e, tke_u=u, tke_v=v, tke_b=b, tke_Km=Km, tke_Kh=Kh, tke_ustar2=ustar2)
+    for mxl in ("integral", "recursive"):
+        p = TKEParams(mxl=mxl)
+        en, km, kh = tke_step(e, u, v, b, d.dz3_ref, d.mask3, ustar2, 300.0, p, 1e-4, 1e-5,
+                              Km, Kh)
+        rec[f"tke_{mxl}_e"], rec[f"tke_{mxl}_Km"], rec[f"tke_{mxl}_Kh"] = en, km, kh
+    # momentum advection next to walls and the island
+    w = np.zeros((nz1,) + g.shape)
+    w[1:g.nz] = 1e-3 * rng.standard_normal((g.nz - 1,) + g.shape)
+    rec.update(adv_u=u, adv_v=v, adv_w=w)
+    for lim in (False, True):
+        au, av = momentum_advection_up3(u, v, w, g.dx, g.dy, g.dz, d.mask3u, d.mask3v, lim)
+        rec[f"adv_{'tvd' if lim else 'up3'}_u"], rec[f"adv_{'tvd' if lim else 'up3'}_v"] = au, av
+    # masked Coriolis averages
+    rec["cor_v_to_u"] = avg_v_to_u_m(v, d.mask3v, d.mask3u)
+    rec["cor_u_to_v"] = avg_u_to_v_m(u, d.mask3u, d.mask3v)
+    # pressure gradient with partial cells, both pgf modes
+    for pgf in ("keep", "remove"):
+        phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0), pgf=pgf)
+        st = Stepper3DV05(d, phys, SchemeParams(name="fb"), 10.0)
+        px, py = st._pressure_gradients(b)
+        rec[f"pg_{pgf}_x"], rec[f"pg_{pgf}_y"] = px, py
+    rec["pg_b"] = b
+    # Helmholtz solver trajectories on arbitrary coefficients (nx, ny even)
+    Ku = (200.0 + 800.0 * rng.random(g.shape)) * d.mask_u
+    Kv = (200.0 + 800.0 * rng.random(g.shape)) * d.mask_v
+    rhs = rng.standard_normal(g.shape) * d.mask
+    coef = 9.80616 * 0.25 * 600.0 ** 2
+    rec.update(helm_Ku=Ku, helm_Kv=Kv, helm_rhs=rhs, helm_coef=np.array(coef))
+    for kind in ("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid"):
+        solver = HelmholtzVar(g, coef, Ku, Kv, d.mask, kind=kind, rtol=1e-11, max_iter=5000)
+        x, rep = solver.solve(rhs)
+        rec[f"helm_{kind}_x"] = x
+        rec[f"helm_{kind}_iters"] = np.array(rep.iterations)
+    # bathymetry bits: rough (spectral) and seamount at a non-square size
+    grid2 = CGrid(nx=40, ny=24, Lx=4.0e5, Ly=2.4e5, nz=6, depth=800.0)
+    for topo in ("rough", "seamount", "slope", "ridge"):
+        H, m = build_bathymetry(grid2, topo, 800.0, seed=20260911)
+        rec[f"bathy_{topo}"] = H
+    return rec
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", type=Path, default=Path(__file__).resolve().parent)
+    args = ap.parse_args()
+    args.out.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(args.out / "operators.npz", **operator_fixtures())
+    print("operators.npz written")
+    for name, rec in composed_fixtures().items():
+        np.savez_compressed(args.out / f"step_{name}.npz", **rec)
+        print(f"step_{name}.npz written (iterations {int(rec['counters'][0])})")
+    rec = lock_exchange_fixture()
+    np.savez_compressed(args.out / "lock_exchange_128x4x20.npz", **rec)
+    print("lock_exchange_128x4x20.npz written, iterations", rec["iters_per_step"].sum())
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

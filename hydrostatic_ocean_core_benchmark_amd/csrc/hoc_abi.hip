@@ -1,0 +1,730 @@
+// Host side of libhoc.so: the C-ABI declared in include/hoc.h, the stepper
+// handle (geometry, workspaces, counters) and the launch sequence of one step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hoc.h"
+#include "hoc_kernels.h"
+
+using namespace hoc;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(HOC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                                   \
+  do {                                                                                       \
+    hipError_t _e = hipGetLastError();                                                       \
+    if (_e != hipSuccess)                                                                    \
+      return fail(HOC_ERR_DEVICE, std::string("launch ") + name + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+constexpr int TPB = 256;
+inline unsigned blocks(long long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+// ---------------------------------------------------- pairwise-sum plans
+struct TreeBuild {
+  std::vector<short> leaf_off, leaf_len;
+  struct Node {
+    int l, r, level;  // child ids: >= 0 leaf index, < 0 -(node index + 1)
+  };
+  std::vector<Node> nodes;
+  // returns (id, level): id >= 0 leaf, < 0 node
+  std::pair<int, int> build(int off, int n) {
+    if (n <= 128) {
+      leaf_off.push_back((short)off);
+      leaf_len.push_back((short)n);
+      return {(int)leaf_off.size() - 1, 0};
+    }
+    int n2 = n / 2;
+    n2 -= n2 % 8;
+    auto a = build(off, n2);
+    auto b = build(off + n2, n - n2);
+    const int lv = std::max(a.second, b.second) + 1;
+    nodes.push_back({a.first, b.first, lv});
+    return {-(int)nodes.size(), lv};
+  }
+};
+
+PwPlan make_plan(int n) {
+  PwPlan p;
+  std::memset(&p, 0, sizeof(p));
+  if (n <= 0) return p;
+  TreeBuild t;
+  t.build(0, n);
+  const int L = (int)t.leaf_off.size();
+  const int N = (int)t.nodes.size();
+  p.nleaves = L;
+  p.nnodes = N;
+  for (int i = 0; i < L; ++i) {
+    p.leaf_off[i] = t.leaf_off[i];
+    p.leaf_len[i] = t.leaf_len[i];
+  }
+  // order nodes by level (stable); the root has the highest level and is last
+  std::vector<int> order(N);
+  for (int i = 0; i < N; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return t.nodes[a].level < t.nodes[b].level; });
+  std::vector<int> rank(N);
+  for (int i = 0; i < N; ++i) rank[order[i]] = i;
+  auto value_index = [&](int id) { return id >= 0 ? id : L + rank[-id - 1]; };
+  int maxlv = 0;
+  for (int i = 0; i < N; ++i) {
+    const auto& nd = t.nodes[order[i]];
+    p.node_l[i] = (short)value_index(nd.l);
+    p.node_r[i] = (short)value_index(nd.r);
+    maxlv = std::max(maxlv, nd.level);
+  }
+  p.nlevels = maxlv;
+  for (int lv = 1; lv <= maxlv; ++lv) {
+    int cnt = 0;
+    for (int i = 0; i < N; ++i)
+      if (t.nodes[i].level <= lv) ++cnt;
+    p.level_end[lv - 1] = cnt;
+  }
+  return p;
+}
+
+struct Reducer {
+  long long n = 0, nfull = 0, nchunks = 0;
+  PwPlan* plans = nullptr;  // device [2]
+  int init(long long n_) {
+    n = n_;
+    nfull = n / PW_CHUNK;
+    const long long tail = n % PW_CHUNK;
+    nchunks = nfull + (tail ? 1 : 0);
+    PwPlan h[2];
+    h[0] = make_plan((int)PW_CHUNK);
+    h[1] = make_plan((int)(tail ? tail : 0));
+    HIP_TRY(hipMalloc(&plans, sizeof(h)));
+    HIP_TRY(hipMemcpy(plans, h, sizeof(h), hipMemcpyHostToDevice));
+    return HOC_OK;
+  }
+  void release() {
+    if (plans) (void)hipFree(plans);
+    plans = nullptr;
+  }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------ the handle
+struct hoc_stepper {
+  hoc_config cfg{};
+  Dims d{};
+  StepParams p{};
+  Geo geo{};
+  std::vector<void*> allocs;
+  // geometry owned on device
+  uint8_t *m2 = nullptr, *mu2 = nullptr, *mv2 = nullptr, *m3 = nullptr, *m3u = nullptr,
+          *m3v = nullptr;
+  double *dz3 = nullptr, *dz3u = nullptr, *dz3v = nullptr, *zc = nullptr, *dzi = nullptr;
+  double *tau_x = nullptr, *tau_y = nullptr, *tau_u = nullptr, *tau_v = nullptr;
+  double *ustar2 = nullptr, *kappa_b = nullptr;
+  // mixing (caller-owned)
+  double *Km = nullptr, *Kh = nullptr;
+  // coefficients
+  double *sub_m, *den_m, *cst_m, *sub_b, *den_b, *cst_b, *q, *Ku, *Kv, *inv_diag;
+  // step work arrays
+  double *phi, *w, *pgx, *pgy, *avu, *avv, *dzu, *dzv, *stu, *stv;
+  double *U0, *V0, *div0, *Ug, *Vg, *rhs;
+  double *gh_u[2], *gh_v[2];
+  double *s_n2, *s_sh, *s_l, *s_c;
+  // Helmholtz
+  double *r, *z, *pp, *Ap, *part_a, *part_b;
+  PcgScalars* sc = nullptr;
+  long long* dcount = nullptr;  // [0] solver iterations, [1] failures
+  int* host_flag = nullptr;     // pinned: done, iters
+  Reducer red;
+  int last_iters = 8;
+  bool coeffs_built = false;
+  // host counters
+  long long solver_rebuilds = 0, tridiagonal_solves = 0, eos_cells = 0;
+
+  template <class T>
+  int alloc(T** ptr, long long count) {
+    void* v = nullptr;
+    HIP_TRY(hipMalloc(&v, std::max<long long>(count, 1) * (long long)sizeof(T)));
+    allocs.push_back(v);
+    *ptr = (T*)v;
+    return HOC_OK;
+  }
+  ~hoc_stepper() {
+    for (void* v : allocs) (void)hipFree(v);
+    if (host_flag) (void)hipHostFree(host_flag);
+    red.release();
+  }
+};
+
+namespace {
+
+int derive_params(const hoc_config& c, StepParams& p) {
+  std::memset(&p, 0, sizeof(p));
+  p.scheme = c.scheme;
+  p.advection = c.advection;
+  p.tracers_ts = c.tracers_ts;
+  p.eos = c.eos;
+  p.pgf_remove = c.pgf_remove;
+  p.pgf_correction = c.pgf_correction;
+  p.closure_tke = c.closure_tke;
+  p.mxl_recursive = c.mxl_recursive;
+  p.n_picard = c.n_picard;
+  p.solver = c.solver;
+  p.max_iter = c.max_iter;
+  p.has_A_h = c.A_h > 0.0;
+  p.has_K_h = c.K_h > 0.0;
+  p.dx = c.dx;
+  p.dy = c.dy;
+  p.dz_nom = c.dz_nominal;
+  p.dt = c.dt;
+  p.theta = c.theta;
+  p.theta_cor = c.theta_cor;
+  p.theta_v = c.theta_v;
+  p.g = c.g;
+  p.f = c.f;
+  p.rho0 = c.rho0;
+  p.nu_b = c.nu_b;
+  p.bottom_drag = c.bottom_drag;
+  p.N2 = c.N2;
+  p.alpha_T = c.alpha_T;
+  p.beta_S = c.beta_S;
+  p.T0 = c.T0;
+  p.S0 = c.S0;
+  p.A_h = c.A_h;
+  p.K_h = c.K_h;
+  p.c_k = c.c_k;
+  p.c_eps = c.c_eps;
+  p.pr_t = c.pr_t;
+  p.z_0 = c.z_0;
+  p.e_min = c.e_min;
+  p.n2_min = c.n2_min;
+  p.l_min = c.l_min;
+  p.e_bb = c.e_bb;
+  p.rtol = c.rtol;
+  // Derived constants in the reference's (Python) evaluation order; `x**2`
+  // in Python is libm pow(x, 2.0), used identically here.
+  const double th = (c.scheme == 0) ? c.theta : 0.0;
+  p.th = th;
+  p.thv_dt = c.theta_v * c.dt;
+  p.thv_dt_drag = c.theta_v * c.dt * c.bottom_drag;
+  p.one_m_tc = 1.0 - c.theta_cor;
+  p.c_gx = c.g * c.dt * (1.0 - th);
+  p.c_dzd = (1.0 - c.theta_v) * c.dt;
+  p.c_bs = c.g * c.dt * th;
+  p.one_m_th = 1.0 - th;
+  p.helm_coef = c.g * std::pow(c.theta, 2.0) * std::pow(c.dt, 2.0);
+  p.dx2 = std::pow(c.dx, 2.0);
+  p.dy2 = std::pow(c.dy, 2.0);
+  p.helm_cx = p.helm_coef * (1.0 / p.dx2);
+  p.helm_cy = p.helm_coef * (1.0 / p.dy2);
+  p.flux_T = c.q_heat / (c.rho0 * c.cp);
+  p.flux_S = c.q_salt;
+  p.neg_alpha_T = -c.alpha_T;
+  p.neg_g = -c.g;
+  p.neg_f = -c.f;
+  p.neg_N2 = -c.N2;
+  return HOC_OK;
+}
+
+int validate(const hoc_config& c) {
+  if (c.nx < 2 || c.ny < 2 || c.nz < 1)
+    return fail(HOC_ERR_INVALID, "grid must have nx >= 2, ny >= 2, nz >= 1");
+  if (!(c.dx > 0.0) || !(c.dy > 0.0) || !(c.dt > 0.0))
+    return fail(HOC_ERR_INVALID, "dx, dy and dt must be positive");
+  if (c.scheme != 0 && c.scheme != 1)
+    return fail(HOC_ERR_UNSUPPORTED, "scheme must be theta (0) or fb (1)");
+  if (c.scheme == 0 && !(c.theta > 0.0 && c.theta <= 1.0))
+    return fail(HOC_ERR_INVALID, "theta must lie in (0, 1]");
+  if (c.n_picard < 1) return fail(HOC_ERR_INVALID, "n_picard must be >= 1");
+  if (c.advection < 0 || c.advection > 4) return fail(HOC_ERR_INVALID, "unknown advection");
+  if (c.eos < 0 || c.eos > 2)
+    return fail(HOC_ERR_INVALID, "unknown eos (expected linear|seos|teos10)");
+  if (c.scheme == 0 && c.solver != SOLVER_PCG_JACOBI)
+    return fail(HOC_ERR_UNSUPPORTED, "this build solves the Helmholtz problem with pcg_jacobi");
+  if (c.max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
+  return HOC_OK;
+}
+
+template <class T>
+int upload(T* dst, const T* src, long long n, const char* name) {
+  if (!src) return fail(HOC_ERR_INVALID, std::string("geometry array missing: ") + name);
+  HIP_TRY(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return HOC_OK;
+}
+
+#define TRY(expr)             \
+  do {                        \
+    int _rc = (expr);         \
+    if (_rc != HOC_OK) return _rc; \
+  } while (0)
+
+int build_coefficients(hoc_stepper* h, hipStream_t s) {
+  if (!h->Km || !h->Kh) return fail(HOC_ERR_INVALID, "bind K_m / K_h before building");
+  const Dims d = h->d;
+  k_coefficients<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->Km, h->Kh, h->sub_m,
+                                              h->den_m, h->cst_m, h->sub_b, h->den_b,
+                                              h->cst_b, h->q);
+  LAUNCH_CHECK("k_coefficients");
+  k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, h->q, h->Ku, h->Kv);
+  LAUNCH_CHECK("k_kukv");
+  h->tridiagonal_solves += 1;
+  if (h->cfg.scheme == 0) {
+    HelmOp op{h->Ku, h->Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2,
+              h->p.dy2};
+    k_helm_diag<<<blocks(d.n2), TPB, 0, s>>>(d, op, h->inv_diag);
+    LAUNCH_CHECK("k_helm_diag");
+    if (h->coeffs_built) h->solver_rebuilds += 1;
+  }
+  h->coeffs_built = true;
+  return HOC_OK;
+}
+
+// PCG-Jacobi on A = I - coef D[K G .]; x is the output (device).
+int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const double* inv_diag,
+                    const double* rhs, double* x, hipStream_t s, int* iters_out,
+                    int* conv_out) {
+  const Dims d = h->d;
+  const Reducer& R = h->red;
+  HelmOp op{Ku, Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2, h->p.dy2};
+  PcgVec vv{x, h->r, h->z, h->pp, h->Ap, inv_diag, h->part_a, h->part_b};
+  const unsigned nch = (unsigned)R.nchunks;
+  if (nch == 0) return fail(HOC_ERR_INVALID, "empty grid");
+  k_pcg_init<<<nch, 512, 0, s>>>(d, op, vv, rhs, R.plans, R.nfull);
+  LAUNCH_CHECK("k_pcg_init");
+  k_pcg_start<<<1, 1, 0, s>>>(vv, R.nchunks, h->p.rtol, h->sc);
+  LAUNCH_CHECK("k_pcg_start");
+  k_pcg_init_z<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc);
+  LAUNCH_CHECK("k_pcg_init_z");
+  k_pcg_rz0<<<1, 1, 0, s>>>(vv, R.nchunks, h->sc);
+  LAUNCH_CHECK("k_pcg_rz0");
+  const int max_iter = h->p.max_iter;
+  int launched = 0;
+  int batch = std::max(1, std::min(h->last_iters - 1, max_iter));
+  while (true) {
+    for (int b = 0; b < batch; ++b) {
+      k_pcg_ap<<<nch, 512, 0, s>>>(d, op, vv, R.plans, R.nfull, h->sc);
+      k_pcg_alpha<<<1, 1, 0, s>>>(vv, R.nchunks, h->sc);
+      k_pcg_update<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc);
+      k_pcg_check<<<1, 1, 0, s>>>(vv, R.nchunks, max_iter, h->sc, h->dcount);
+      k_pcg_p<<<blocks(d.n2), TPB, 0, s>>>(d, vv, h->sc);
+    }
+    LAUNCH_CHECK("pcg iteration");
+    launched += batch;
+    HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h->host_flag[0]) break;
+    if (launched > max_iter + 8)
+      return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
+    batch = 4;
+  }
+  // PcgScalars ends with it, iters, done, converged
+  int tail[4];
+  HIP_TRY(hipMemcpy(tail, &h->sc->it, sizeof(tail), hipMemcpyDeviceToHost));
+  if (iters_out) *iters_out = tail[1];
+  if (conv_out) *conv_out = tail[3];
+  if (tail[1] > 0) h->last_iters = tail[1];
+  return HOC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hoc_last_error(void) { return g_error.c_str(); }
+
+int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper** out) {
+  if (!cfg || !g || !out) return fail(HOC_ERR_INVALID, "null argument");
+  TRY(validate(*cfg));
+  hoc_stepper* h = new (std::nothrow) hoc_stepper();
+  if (!h) return fail(HOC_ERR_DEVICE, "out of host memory");
+  h->cfg = *cfg;
+  Dims& d = h->d;
+  d.nx = cfg->nx;
+  d.ny = cfg->ny;
+  d.nz = cfg->nz;
+  d.n2 = (long long)cfg->nx * cfg->ny;
+  d.n3 = d.n2 * cfg->nz;
+  d.ni = d.n2 * (cfg->nz + 1);
+  derive_params(*cfg, h->p);
+  auto cleanup = [&](int rc) {
+    delete h;
+    return rc;
+  };
+#define A(ptr, n)                        \
+  do {                                   \
+    int _rc = h->alloc(&(ptr), (n));     \
+    if (_rc != HOC_OK) return cleanup(_rc); \
+  } while (0)
+  A(h->m2, d.n2);
+  A(h->mu2, d.n2);
+  A(h->mv2, d.n2);
+  A(h->m3, d.n3);
+  A(h->m3u, d.n3);
+  A(h->m3v, d.n3);
+  A(h->dz3, d.n3);
+  A(h->dz3u, d.n3);
+  A(h->dz3v, d.n3);
+  A(h->zc, d.n3);
+  A(h->dzi, d.ni);
+  A(h->tau_x, d.n2);
+  A(h->tau_y, d.n2);
+  A(h->tau_u, d.n2);
+  A(h->tau_v, d.n2);
+  A(h->ustar2, d.n2);
+  A(h->kappa_b, d.n2);
+  A(h->sub_m, d.n3);
+  A(h->den_m, d.n3);
+  A(h->cst_m, d.n3);
+  A(h->sub_b, d.n3);
+  A(h->den_b, d.n3);
+  A(h->cst_b, d.n3);
+  A(h->q, d.n3);
+  A(h->Ku, d.n2);
+  A(h->Kv, d.n2);
+  A(h->inv_diag, d.n2);
+  A(h->phi, d.n3);
+  A(h->w, d.ni);
+  A(h->pgx, d.n3);
+  A(h->pgy, d.n3);
+  A(h->avu, d.n3);
+  A(h->avv, d.n3);
+  A(h->dzu, d.n3);
+  A(h->dzv, d.n3);
+  A(h->stu, d.n3);
+  A(h->stv, d.n3);
+  A(h->U0, d.n2);
+  A(h->V0, d.n2);
+  A(h->div0, d.n2);
+  A(h->Ug, d.n2);
+  A(h->Vg, d.n2);
+  A(h->rhs, d.n2);
+  A(h->gh_u[0], d.n3);
+  A(h->gh_v[0], d.n3);
+  A(h->gh_u[1], d.n3);
+  A(h->gh_v[1], d.n3);
+  A(h->s_n2, d.ni);
+  A(h->s_sh, d.ni);
+  A(h->s_l, d.ni);
+  A(h->s_c, d.ni);
+  A(h->r, d.n2);
+  A(h->z, d.n2);
+  A(h->pp, d.n2);
+  A(h->Ap, d.n2);
+  A(h->sc, 1);
+  A(h->dcount, 4);
+#undef A
+  {
+    int rc = h->red.init(d.n2);
+    if (rc != HOC_OK) return cleanup(rc);
+    double* pa = nullptr;
+    double* pb = nullptr;
+    rc = h->alloc(&pa, h->red.nchunks);
+    if (rc == HOC_OK) rc = h->alloc(&pb, h->red.nchunks);
+    if (rc != HOC_OK) return cleanup(rc);
+    h->part_a = pa;
+    h->part_b = pb;
+    if (hipHostMalloc((void**)&h->host_flag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(HOC_ERR_DEVICE, "hipHostMalloc failed"));
+  }
+  int rc = HOC_OK;
+  rc = rc ? rc : upload(h->m2, g->mask, d.n2, "mask");
+  rc = rc ? rc : upload(h->mu2, g->mask_u, d.n2, "mask_u");
+  rc = rc ? rc : upload(h->mv2, g->mask_v, d.n2, "mask_v");
+  rc = rc ? rc : upload(h->m3, g->mask3, d.n3, "mask3");
+  rc = rc ? rc : upload(h->m3u, g->mask3u, d.n3, "mask3u");
+  rc = rc ? rc : upload(h->m3v, g->mask3v, d.n3, "mask3v");
+  rc = rc ? rc : upload(h->dz3, g->dz3, d.n3, "dz3");
+  rc = rc ? rc : upload(h->dz3u, g->dz3u, d.n3, "dz3u");
+  rc = rc ? rc : upload(h->dz3v, g->dz3v, d.n3, "dz3v");
+  rc = rc ? rc : upload(h->zc, g->z_centre, d.n3, "z_centre");
+  rc = rc ? rc : upload(h->tau_x, g->tau_x, d.n2, "tau_x");
+  rc = rc ? rc : upload(h->tau_y, g->tau_y, d.n2, "tau_y");
+  rc = rc ? rc : upload(h->ustar2, g->ustar2, d.n2, "ustar2");
+  rc = rc ? rc : upload(h->kappa_b, g->kappa_b, d.n2, "kappa_b");
+  if (rc != HOC_OK) return cleanup(rc);
+  if (hipMemset(h->dcount, 0, 4 * sizeof(long long)) != hipSuccess)
+    return cleanup(fail(HOC_ERR_DEVICE, "hipMemset failed"));
+  k_geometry<<<blocks(d.n2), TPB>>>(d, h->p, h->dz3, h->m3u, h->m3v, h->tau_x, h->tau_y, h->dzi,
+                                    h->tau_u, h->tau_v);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return cleanup(fail(HOC_ERR_DEVICE, "geometry kernel failed"));
+  Geo& geo = h->geo;
+  geo.m2 = h->m2;
+  geo.mu2 = h->mu2;
+  geo.mv2 = h->mv2;
+  geo.m3 = h->m3;
+  geo.m3u = h->m3u;
+  geo.m3v = h->m3v;
+  geo.dz3 = h->dz3;
+  geo.dz3u = h->dz3u;
+  geo.dz3v = h->dz3v;
+  geo.zc = h->zc;
+  geo.dzi = h->dzi;
+  geo.tau_u = h->tau_u;
+  geo.tau_v = h->tau_v;
+  geo.ustar2 = h->ustar2;
+  geo.kappa_b = h->kappa_b;
+  *out = h;
+  return HOC_OK;
+}
+
+int hoc_stepper_destroy(hoc_stepper* h) {
+  delete h;
+  return HOC_OK;
+}
+
+int hoc_stepper_bind_mixing(hoc_stepper* h, double* K_m, double* K_h) {
+  if (!h || !K_m || !K_h) return fail(HOC_ERR_INVALID, "null argument");
+  h->Km = K_m;
+  h->Kh = K_h;
+  return HOC_OK;
+}
+
+int hoc_stepper_build_coefficients(hoc_stepper* h, void* stream) {
+  if (!h) return fail(HOC_ERR_INVALID, "null handle");
+  return build_coefficients(h, (hipStream_t)stream);
+}
+
+int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, const double* v,
+                         const double* b, double* e_out, void* stream) {
+  if (!h || !e_in || !u || !v || !b || !e_out) return fail(HOC_ERR_INVALID, "null argument");
+  if (!h->Km || !h->Kh) return fail(HOC_ERR_INVALID, "bind K_m / K_h first");
+  hipStream_t s = (hipStream_t)stream;
+  const Dims d = h->d;
+  if (h->p.mxl_recursive) {
+    k_closure_n2<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, u, v, b, h->s_n2);
+    k_mixing_recursive<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->s_n2, h->s_l);
+  }
+  k_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, u, v, b, h->Km, h->Kh, e_out,
+                                         h->s_n2, h->s_sh, h->s_l, h->s_c);
+  LAUNCH_CHECK("k_closure");
+  return HOC_OK;
+}
+
+int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream) {
+  if (!h || !in || !out) return fail(HOC_ERR_INVALID, "null argument");
+  if (!in->u || !in->v || !in->b || !in->eta || !out->u || !out->v || !out->b || !out->eta)
+    return fail(HOC_ERR_INVALID, "state arrays u, v, b, eta are required");
+  const bool ts = h->cfg.tracers_ts != 0;
+  if (ts && (!in->T || !in->S || !out->T || !out->S))
+    return fail(HOC_ERR_INVALID, "tracers_ts needs T and S");
+  const bool tke = h->cfg.closure_tke != 0;
+  if (tke && (!in->e || !out->e)) return fail(HOC_ERR_INVALID, "closure_tke needs e");
+  if (!h->coeffs_built) TRY(build_coefficients(h, (hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  const Dims d = h->d;
+  const StepParams& p = h->p;
+  const Geo& geo = h->geo;
+  const bool theta = h->cfg.scheme == 0;
+
+  if (tke) {
+    TRY(hoc_stepper_tke_step(h, in->e, in->u, in->v, in->b, out->e, stream));
+    h->tridiagonal_solves += 1;
+    TRY(build_coefficients(h, s));
+  } else if (in->e && out->e && in->e != out->e) {
+    HIP_TRY(hipMemcpyAsync(out->e, in->e, d.ni * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->b, h->phi);
+  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w);
+  k_momentum_explicit<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->b, h->phi, h->w,
+                                                   h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu,
+                                                   h->dzv, h->stu, h->stv);
+  LAUNCH_CHECK("explicit momentum");
+  if (theta) {
+    k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
+    k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, h->div0);
+  }
+  const double* u_it = in->u;
+  const double* v_it = in->v;
+  for (int it = 0; it < h->cfg.n_picard; ++it) {
+    double* gu = h->gh_u[it & 1];
+    double* gv = h->gh_v[it & 1];
+    k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
+                                             h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
+                                             h->stu, h->stv, h->sub_m, h->den_m, h->cst_m, gu, gv,
+                                             h->Ug, h->Vg);
+    LAUNCH_CHECK("k_predictor");
+    h->tridiagonal_solves += 2;
+    if (theta) {
+      k_rhs_eta<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->div0, h->Ug, h->Vg, h->rhs);
+      LAUNCH_CHECK("k_rhs_eta");
+      int iters = 0, conv = 0;
+      TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
+      k_backsub<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u, out->v);
+      LAUNCH_CHECK("k_backsub");
+      u_it = out->u;
+      v_it = out->v;
+    } else {
+      u_it = gu;
+      v_it = gv;
+    }
+  }
+  if (!theta) {
+    HIP_TRY(hipMemcpyAsync(out->u, u_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(out->v, v_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, out->u, out->v, h->U0, h->V0);
+    k_eta_fb<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->U0, h->V0, out->eta);
+    LAUNCH_CHECK("k_eta_fb");
+  }
+  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w);
+  k_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->T, in->S, in->b, out->u, out->v, h->w,
+                                         h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S,
+                                         out->b);
+  LAUNCH_CHECK("k_tracers");
+  h->tridiagonal_solves += ts ? 2 : 1;
+  if (ts) h->eos_cells += d.n3;
+  return HOC_OK;
+}
+
+int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out) {
+  if (!h || !out) return fail(HOC_ERR_INVALID, "null argument");
+  long long dc[4];
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(dc, h->dcount, sizeof(dc), hipMemcpyDeviceToHost));
+  out->solver_iterations = dc[0];
+  out->solver_failures = dc[1];
+  out->solver_rebuilds = h->solver_rebuilds;
+  out->tridiagonal_solves = h->tridiagonal_solves;
+  out->barotropic_substeps = 0;
+  out->eos_cells = h->eos_cells;
+  return HOC_OK;
+}
+
+int hoc_stepper_momentum_advection(hoc_stepper* h, const double* u, const double* v,
+                                   const double* w, double* au, double* av, void* stream) {
+  if (!h || !u || !v || !w || !au || !av) return fail(HOC_ERR_INVALID, "null argument");
+  const Dims d = h->d;
+  StepParams p = h->p;
+  if (p.advection == ADV_UPWIND1) p.advection = ADV_CENTRED2;
+  if (p.advection == ADV_NONE) return fail(HOC_ERR_INVALID, "advection is 'none'");
+  k_momentum_advection<<<blocks(d.n3), TPB, 0, (hipStream_t)stream>>>(d, p, h->geo, u, v, w, au,
+                                                                      av);
+  LAUNCH_CHECK("k_momentum_advection");
+  return HOC_OK;
+}
+
+int hoc_stepper_tracer_advection(hoc_stepper* h, const double* c, const double* u,
+                                 const double* v, const double* w, double* out, void* stream) {
+  if (!h || !c || !u || !v || !w || !out) return fail(HOC_ERR_INVALID, "null argument");
+  const Dims d = h->d;
+  k_tracer_advection<<<blocks(d.n3), TPB, 0, (hipStream_t)stream>>>(d, h->p, h->geo, c, u, v, w,
+                                                                    out);
+  LAUNCH_CHECK("k_tracer_advection");
+  return HOC_OK;
+}
+
+int hoc_stepper_pressure_gradients(hoc_stepper* h, const double* b, double* pgx, double* pgy,
+                                   void* stream) {
+  if (!h || !b || !pgx || !pgy) return fail(HOC_ERR_INVALID, "null argument");
+  if (!h->Km) return fail(HOC_ERR_INVALID, "bind K_m / K_h first");
+  hipStream_t s = (hipStream_t)stream;
+  const Dims d = h->d;
+  // The explicit-momentum kernel computes the gradient along with the other
+  // start-of-step terms; u = v = 0 and w = 0 make those terms harmless here.
+  HIP_TRY(hipMemsetAsync(h->gh_u[0], 0, d.n3 * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(h->s_n2, 0, d.ni * sizeof(double), s));
+  k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, b, h->phi);
+  k_momentum_explicit<<<blocks(d.n3), TPB, 0, s>>>(d, h->p, h->geo, h->gh_u[0], h->gh_u[0], b,
+                                                   h->phi, h->s_n2, h->Km, pgx, pgy, h->avu,
+                                                   h->avv, h->dzu, h->dzv, h->stu, h->stv);
+  LAUNCH_CHECK("pressure gradients");
+  return HOC_OK;
+}
+
+int hoc_stepper_w_diag(hoc_stepper* h, const double* u, const double* v, double* w,
+                       void* stream) {
+  if (!h || !u || !v || !w) return fail(HOC_ERR_INVALID, "null argument");
+  const Dims d = h->d;
+  k_w_diag<<<blocks(d.n2), TPB, 0, (hipStream_t)stream>>>(d, h->p, h->geo, u, v, w);
+  LAUNCH_CHECK("k_w_diag");
+  return HOC_OK;
+}
+
+int hoc_stepper_helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv,
+                                const double* rhs, double* x, int32_t* iterations,
+                                int32_t* converged, void* stream) {
+  if (!h || !Ku || !Kv || !rhs || !x) return fail(HOC_ERR_INVALID, "null argument");
+  if (h->cfg.scheme != 0) return fail(HOC_ERR_INVALID, "the fb scheme has no Helmholtz solve");
+  hipStream_t s = (hipStream_t)stream;
+  const Dims d = h->d;
+  HelmOp op{Ku, Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2, h->p.dy2};
+  double* inv = h->gh_u[1];  // n3-sized work array, unused by the solve itself
+  k_helm_diag<<<blocks(d.n2), TPB, 0, s>>>(d, op, inv);
+  LAUNCH_CHECK("k_helm_diag");
+  int it = 0, cv = 0;
+  TRY(helmholtz_solve(h, Ku, Kv, inv, rhs, x, s, &it, &cv));
+  if (iterations) *iterations = it;
+  if (converged) *converged = cv;
+  return HOC_OK;
+}
+
+int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
+                       const double* rhs, double* x, double* scratch, int64_t n, int64_t ncol,
+                       void* stream) {
+  if (!sub || !diag || !sup || !rhs || !x || !scratch) return fail(HOC_ERR_INVALID, "null argument");
+  if (n < 1 || ncol < 1) return fail(HOC_ERR_INVALID, "n and ncol must be positive");
+  k_thomas<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(ncol, (int)n, sub, diag, sup, rhs, x,
+                                                          scratch);
+  LAUNCH_CHECK("k_thomas");
+  return HOC_OK;
+}
+
+int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double beta_S,
+                     double T0, double S0, const double* T, const double* S, const double* z,
+                     double* b, int64_t n, void* stream) {
+  if (!T || !S || !z || !b) return fail(HOC_ERR_INVALID, "null argument");
+  if (eos < 0 || eos > 2) return fail(HOC_ERR_INVALID, "unknown eos (expected linear|seos|teos10)");
+  if (n <= 0) return HOC_OK;
+  StepParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.eos = eos;
+  p.g = g;
+  p.rho0 = rho0;
+  p.alpha_T = alpha_T;
+  p.beta_S = beta_S;
+  p.T0 = T0;
+  p.S0 = S0;
+  p.neg_alpha_T = -alpha_T;
+  p.neg_g = -g;
+  k_eos<<<blocks(n), TPB, 0, (hipStream_t)stream>>>(n, p, T, S, z, b);
+  LAUNCH_CHECK("k_eos");
+  return HOC_OK;
+}
+
+int hoc_pairwise_dot(const double* a, const double* b, int64_t n, double* out, double* scratch,
+                     void* stream) {
+  if (!a || !b || !out || !scratch) return fail(HOC_ERR_INVALID, "null argument");
+  if (n < 0) return fail(HOC_ERR_INVALID, "negative length");
+  hipStream_t s = (hipStream_t)stream;
+  Reducer R;
+  TRY(R.init(n));
+  if (R.nchunks > 0) {
+    k_pw_dot<<<(unsigned)R.nchunks, 512, 0, s>>>(a, b, R.plans, R.nfull, scratch);
+  }
+  k_seq_sum<<<1, 1, 0, s>>>(scratch, R.nchunks, out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  R.release();
+  if (e != hipSuccess) return fail(HOC_ERR_DEVICE, hipGetErrorString(e));
+  return HOC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// Device-side helpers shared by every kernel of the ocean-core step.
+//
+// Layout: fp64, C order. A 3D layer field is [k][j][i] (nz x ny x nx), an
+// interface field [k][j][i] with nz+1 levels, a 2D field [j][i]. i is the
+// fastest index, so one lane per (j, i) column gives coalesced access at
+// every level k. Masks are uint8 (0/1); multiplying by (double)m is bitwise
+// the same as multiplying by the reference's 0.0/1.0 float masks.
+//
+// Every arithmetic expression keeps the operand order of the reference's
+// NumPy code and the library is built with -ffp-contract=off, so no
+// multiply-add is fused and results are bitwise those of the oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hoc_params.h"
+
+namespace hoc {
+
+__device__ __forceinline__ long long at3(const Dims& d, int k, int j, int i) {
+  return ((long long)k * d.ny + j) * d.nx + i;
+}
+__device__ __forceinline__ long long at2(const Dims& d, int j, int i) {
+  return (long long)j * d.nx + i;
+}
+// Periodic neighbour index, |s| <= n (every shift of the step is within 2).
+__device__ __forceinline__ int wrap(int i, int s, int n) {
+  int t = i + s;
+  if (t < 0) t += n;
+  if (t >= n) t -= n;
+  return t;
+}
+
+// NumPy's maximum/minimum: NaN in either operand propagates.
+__device__ __forceinline__ double nmax(double a, double b) {
+  return (a >= b || a != a) ? a : b;
+}
+__device__ __forceinline__ double nmin(double a, double b) {
+  return (a <= b || a != a) ? a : b;
+}
+// 1/x where x > 0, else 0 (the reference's `_safe`).
+__device__ __forceinline__ double safe_inv(double x) {
+  return x > 0.0 ? 1.0 / x : 0.0;
+}
+__device__ __forceinline__ double mk(uint8_t m) { return m ? 1.0 : 0.0; }
+
+// Superbee limiter phi(r) = max(0, max(min(2r, 1), min(r, 2))).
+__device__ __forceinline__ double superbee(double r) {
+  return nmax(0.0, nmax(nmin(2.0 * r, 1.0), nmin(r, 2.0)));
+}
+
+constexpr double KAPPA = 1.0 / 3.0;
+constexpr double K1M = 1.0 - KAPPA;
+constexpr double K1P = 1.0 + KAPPA;
+
+// kappa = 1/3 face value between c0 = c[i] and c1 = c[i+1] for the face
+// velocity vel; cm = c[i-1], c2 = c[i+2]; wm / w2 tell whether the far
+// upstream cell (i-1 for vel > 0, i+2 for vel < 0) is wet, else upwind1.
+// (libs/core/advection_v06.py:21-47)
+__device__ __forceinline__ double kappa_face(double cm, double c0, double c1, double c2,
+                                             double vel, bool wm, bool w2, bool limited) {
+  double fp, fn;
+  if (limited) {
+    const double dc = c1 - c0;
+    const bool z = dc == 0.0;
+    const double eps = z ? 1.0 : dc;
+    const double rp = z ? 0.0 : (c0 - cm) / eps;
+    const double rn = z ? 0.0 : (c2 - c1) / eps;
+    fp = c0 + 0.5 * superbee(rp) * dc;
+    fn = c1 - 0.5 * superbee(rn) * dc;
+  } else {
+    fp = c0 + 0.25 * (K1M * (c0 - cm) + K1P * (c1 - c0));
+    fn = c1 - 0.25 * (K1M * (c2 - c1) + K1P * (c1 - c0));
+  }
+  if (!wm) fp = c0;
+  if (!w2) fn = c1;
+  return vel > 0.0 ? fp : fn;
+}
+
+// Interior interface value between above = c[k-1] and below = c[k], w > 0
+// upward; far_up = c[k-2] (0 when k < 2), far_dn = c[k+1] (0 when k > nz-2);
+// has_up / has_dn are the wet flags of those far cells.
+// (libs/core/advection_v06.py:50-74)
+__device__ __forceinline__ double kappa_face_z(double far_up, double above, double below,
+                                               double far_dn, double w, bool has_up,
+                                               bool has_dn, bool limited) {
+  const double d = above - below;
+  double fu, fd;
+  if (limited) {
+    const bool z = d == 0.0;
+    const double eps = z ? 1.0 : d;
+    const double ru = z ? 0.0 : (below - far_dn) / eps;
+    const double rd = z ? 0.0 : (far_up - above) / eps;
+    fu = below + 0.5 * superbee(ru) * d;
+    fd = above - 0.5 * superbee(rd) * d;
+  } else {
+    fu = below + 0.25 * (K1M * (below - far_dn) + K1P * (above - below));
+    fd = above - 0.25 * (K1M * (far_up - above) + K1P * (above - below));
+  }
+  if (!has_dn) fu = below;
+  if (!has_up) fd = above;
+  return w > 0.0 ? fu : fd;
+}
+
+// ------------------------------------------------------------------ EOS
+// libs/core/eos.py:68-133; coefficients of Roquet et al. (2015).
+__device__ __forceinline__ double rho_teos10(double CT, double SA, double z) {
+  const double R00 = 4.6494977072e+01, R01 = -5.2099962525e+00, R02 = 2.2601900708e-01,
+               R03 = 6.4326772569e-02, R04 = 1.5616995503e-02, R05 = -1.7243708991e-03;
+  const double E000 = 8.0189615746e+02, E100 = 8.6672408165e+02, E200 = -1.7864682637e+03,
+               E300 = 2.0375295546e+03, E400 = -1.2849161071e+03, E500 = 4.3227585684e+02,
+               E600 = -6.0579916612e+01;
+  const double E010 = 2.6010145068e+01, E110 = -6.5281885265e+01, E210 = 8.1770425108e+01,
+               E310 = -5.6888046321e+01, E410 = 1.7681814114e+01, E510 = -1.9193502195e+00;
+  const double E020 = -3.7074170417e+01, E120 = 6.1548258127e+01, E220 = -6.0362551501e+01,
+               E320 = 2.9130021253e+01, E420 = -5.4723692739e+00;
+  const double E030 = 2.1661789529e+01, E130 = -3.3449108469e+01, E230 = 1.9717078466e+01,
+               E330 = -3.1742946532e+00;
+  const double E040 = -8.3627885467e+00, E140 = 1.1311538584e+01, E240 = -5.3563304045e+00;
+  const double E050 = 5.4048723791e-01, E150 = 4.5111434961e-01;
+  const double E060 = -1.9098268277e-01;
+  const double E001 = 1.9681925209e+01, E101 = -4.2549998214e+01, E201 = 5.0774768218e+01,
+               E301 = -3.0938076334e+01, E401 = 6.6051753097e+00;
+  const double E011 = -1.3336301113e+01, E111 = -4.4870114575e+00, E211 = 5.0042598061e+00,
+               E311 = -6.5399043664e-01;
+  const double E021 = 6.7080479603e+00, E121 = 3.5063081279e+00, E221 = -1.8795372996e+00;
+  const double E031 = -2.4649669534e+00, E131 = -5.5077101279e-01;
+  const double E041 = 5.5927935970e-01;
+  const double E002 = 2.0660924175e+00, E102 = -4.9527603989e+00, E202 = 2.5019633803e+00;
+  const double E012 = 2.0564311499e+00, E112 = -2.1311365518e-01;
+  const double E022 = -1.2419983026e+00;
+  const double E003 = -2.3342758797e-02, E103 = -1.8507636718e-02;
+  const double E013 = 3.7969820455e-01;
+  const double RDELTA_S = 32.0;
+  const double R1_S0 = 0.875 / 35.16504;
+  const double R1_T0 = 1.0 / 40.0;
+  const double R1_Z0 = 1.0e-4;
+
+  const double zh = z * R1_Z0;
+  const double ss = sqrt((SA + RDELTA_S) * R1_S0);
+  const double tt = CT * R1_T0;
+  const double r0 = (((((R05 * zh + R04) * zh + R03) * zh + R02) * zh + R01) * zh + R00) * zh;
+  const double rz3 = E013 * tt + E103 * ss + E003;
+  const double rz2 = ((E022 * tt + E112 * ss + E012) * tt + (E202 * ss + E102) * ss + E002);
+  const double rz1 =
+      ((((E041 * tt + E131 * ss + E031) * tt + (E221 * ss + E121) * ss + E021) * tt +
+        ((E311 * ss + E211) * ss + E111) * ss + E011) * tt +
+       (((E401 * ss + E301) * ss + E201) * ss + E101) * ss + E001);
+  const double rz0 =
+      ((((((E060 * tt + E150 * ss + E050) * tt + (E240 * ss + E140) * ss + E040) * tt +
+          ((E330 * ss + E230) * ss + E130) * ss + E030) * tt +
+         (((E420 * ss + E320) * ss + E220) * ss + E120) * ss + E020) * tt +
+        ((((E510 * ss + E410) * ss + E310) * ss + E210) * ss + E110) * ss + E010) * tt +
+       (((((E600 * ss + E500) * ss + E400) * ss + E300) * ss + E200) * ss + E100) * ss + E000);
+  return ((rz3 * zh + rz2) * zh + rz1) * zh + rz0 + r0;
+}
+
+// b = -g rho'/rho0 for the configured EOS (libs/core/eos.py:103-133).
+__device__ __forceinline__ double buoyancy(const StepParams& p, double T, double S, double z) {
+  double rp;
+  if (p.eos == EOS_LINEAR) {
+    rp = p.rho0 * (p.neg_alpha_T * (T - p.T0) + p.beta_S * (S - p.S0));
+  } else if (p.eos == EOS_SEOS) {
+    const double a0 = 1.6550e-1, b0 = 7.6554e-1, lam1 = 5.9520e-2, lam2 = 7.4914e-4,
+                 mu1 = 1.4970e-4, mu2 = 1.1090e-5, nu_ts = 2.4341e-3;
+    const double Ta = T - 10.0, Sa = S - 35.0;
+    rp = (-a0 * (1.0 + 0.5 * lam1 * Ta + mu1 * z) * Ta +
+          b0 * (1.0 - 0.5 * lam2 * Sa - mu2 * z) * Sa - nu_ts * Ta * Sa);
+  } else {
+    rp = rho_teos10(T, S, z) - p.rho0;
+  }
+  return p.neg_g * rp / p.rho0;
+}
+
+}  // namespace hoc

@@ -1,0 +1,290 @@
+// Barotropic Helmholtz solve  eta - coef D[K G eta] = rhs  on the masked
+// C-grid (libs/core/solvers_var.py:22-197), PCG with device-resident
+// scalars: no host synchronisation inside an iteration; the host only polls
+// a `done` flag between batches of iterations, and every kernel of an
+// iteration past convergence returns immediately, so the iterate and the
+// iteration count are exactly those of the sequential algorithm.
+//
+// Dot products reproduce NumPy's float64 `sum` bit for bit: the vector is cut
+// into 8192-element chunks; each chunk is summed pairwise (recursive halving
+// to blocks of <= 128 elements, each block with eight interleaved
+// accumulators combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then its
+// remainder); the chunk sums are then added in order. One workgroup owns one
+// chunk: eight lanes per block accumulate, a 3-step xor-shuffle combines the
+// eight accumulators in exactly that order, and the blocks' tree is walked
+// level by level in LDS from a host-built plan.
+#include "hoc_device.h"
+#include "hoc_kernels.h"
+
+namespace hoc {
+
+constexpr int PW_THREADS = 512;  // 64 groups of 8 lanes
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 8); }
+
+struct Two {
+  double a, b;
+};
+
+// Pairwise sum(s) of one chunk. `elem(e)` returns the product(s) summed at
+// flat index e and may also store per-element results; every index of the
+// chunk is visited exactly once. Returns the chunk sums in lane 0 of the
+// block (valid for threadIdx.x == 0).
+template <int NS, class F>
+__device__ Two chunk_sum(const PwPlan* __restrict__ plan, long long base, F elem,
+                         double* lds /* [2][256] */) {
+  const int tid = threadIdx.x;
+  const int g = tid >> 3, lane = tid & 7;
+  const int L = plan->nleaves;
+  for (int leaf = g; leaf < L; leaf += PW_THREADS / 8) {
+    const int off = plan->leaf_off[leaf], len = plan->leaf_len[leaf];
+    Two res{0.0, 0.0};
+    if (len >= 8) {
+      const int full = len - (len % 8);
+      Two acc = elem(base + off + lane);
+      for (int t = 8; t < full; t += 8) {
+        const Two v = elem(base + off + t + lane);
+        acc.a = acc.a + v.a;
+        if (NS > 1) acc.b = acc.b + v.b;
+      }
+      for (int m = 1; m < 8; m <<= 1) {
+        acc.a = acc.a + shfl_xor_d(acc.a, m);
+        if (NS > 1) acc.b = acc.b + shfl_xor_d(acc.b, m);
+      }
+      res = acc;
+      if (lane == 0) {
+        for (int t = full; t < len; ++t) {
+          const Two v = elem(base + off + t);
+          res.a = res.a + v.a;
+          if (NS > 1) res.b = res.b + v.b;
+        }
+      }
+    } else if (lane == 0) {
+      for (int t = 0; t < len; ++t) {
+        const Two v = elem(base + off + t);
+        res.a = res.a + v.a;
+        if (NS > 1) res.b = res.b + v.b;
+      }
+    }
+    if (lane == 0) {
+      lds[leaf] = res.a;
+      if (NS > 1) lds[256 + leaf] = res.b;
+    }
+  }
+  __syncthreads();
+  int start = 0;
+  for (int lv = 0; lv < plan->nlevels; ++lv) {
+    const int end = plan->level_end[lv];
+    for (int n = start + tid; n < end; n += PW_THREADS) {
+      const int l = plan->node_l[n], r = plan->node_r[n];
+      lds[L + n] = lds[l] + lds[r];
+      if (NS > 1) lds[256 + L + n] = lds[256 + l] + lds[256 + r];
+    }
+    start = end;
+    __syncthreads();
+  }
+  const int root = plan->nnodes > 0 ? L + plan->nnodes - 1 : 0;
+  Two out{lds[root], NS > 1 ? lds[256 + root] : 0.0};
+  return out;
+}
+
+__device__ __forceinline__ const PwPlan* plan_of(const PwPlan* plans, long long chunk,
+                                                 long long nfull) {
+  return chunk < nfull ? &plans[0] : &plans[1];
+}
+__device__ __forceinline__ long long chunk_len(long long chunk, long long n) {
+  const long long b = chunk * PW_CHUNK;
+  return (n - b) < PW_CHUNK ? (n - b) : PW_CHUNK;
+}
+
+// Masked Helmholtz operator A x at flat index c (libs/core/solvers_var.py:47-53).
+__device__ __forceinline__ double apply_A(const Dims& d, const HelmOp& op, const double* x,
+                                          long long c) {
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
+  const double mc = mk(op.mask[c]);
+  const double xc = x[c] * mc;
+  const double xe = x[ce] * mk(op.mask[ce]);
+  const double xw = x[cw] * mk(op.mask[cw]);
+  const double xn = x[cn] * mk(op.mask[cn]);
+  const double xs = x[cs] * mk(op.mask[cs]);
+  const double fx = op.Ku[c] * (xe - xc);
+  const double fxw = op.Ku[cw] * (xc - xw);
+  const double fy = op.Kv[c] * (xn - xc);
+  const double fys = op.Kv[cs] * (xc - xs);
+  const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+  return (xc - op.coef * lap) * mc + xc * (1.0 - mc);
+}
+
+// diag = 1 + cx (Ku + Ku_w) + cy (Kv + Kv_s), 1 on dry cells; inv = 1/diag.
+__global__ void k_helm_diag(Dims d, HelmOp op, double* __restrict__ inv_diag) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
+  double dg = 1.0 + op.cx * (op.Ku[c] + op.Ku[cw]) + op.cy * (op.Kv[c] + op.Kv[cs]);
+  dg = op.mask[c] ? dg : 1.0;
+  inv_diag[c] = 1.0 / dg;
+}
+
+// ---------------------------------------------------------------- PCG
+// init 1: r = rhs * mask, x = 0, partial sums of r*r (the norm of rhs)
+__global__ __launch_bounds__(PW_THREADS) void k_pcg_init(Dims d, HelmOp op, PcgVec vv,
+                                                         const double* __restrict__ rhs,
+                                                         const PwPlan* __restrict__ plans,
+                                                         long long nfull) {
+  __shared__ double lds[512];
+  const long long chunk = blockIdx.x;
+  const long long base = chunk * PW_CHUNK;
+  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
+    const double r = rhs[e] * mk(op.mask[e]);
+    vv.r[e] = r;
+    vv.x[e] = 0.0;
+    return Two{r * r, 0.0};
+  }, lds);
+  if (threadIdx.x == 0) vv.part_a[chunk] = s.a;
+}
+
+// init 2: z = inv_diag r, p = z, partial sums of r*z
+__global__ __launch_bounds__(PW_THREADS) void k_pcg_init_z(Dims d, PcgVec vv,
+                                                           const PwPlan* __restrict__ plans,
+                                                           long long nfull,
+                                                           const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double lds[512];
+  const long long chunk = blockIdx.x;
+  const long long base = chunk * PW_CHUNK;
+  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
+    const double z = vv.inv_diag[e] * vv.r[e];
+    vv.z[e] = z;
+    vv.p[e] = z;
+    return Two{vv.r[e] * z, 0.0};
+  }, lds);
+  if (threadIdx.x == 0) vv.part_b[chunk] = s.a;
+}
+
+__device__ __forceinline__ double seq_sum(const double* part, long long nchunks) {
+  double s = 0.0;
+  for (long long c = 0; c < nchunks; ++c) s = s + part[c];
+  return s;
+}
+
+// norm_b = sqrt(sum rhs^2); a zero right-hand side finishes in 0 iterations.
+__global__ void k_pcg_start(PcgVec vv, long long nchunks, double rtol, PcgScalars* sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double nb = sqrt(seq_sum(vv.part_a, nchunks));
+  sc->norm_b = nb;
+  sc->tol = rtol * nb;
+  sc->it = 0;
+  sc->iters = 0;
+  sc->converged = 1;
+  sc->done = (nb == 0.0) ? 1 : 0;
+  sc->res = 0.0;
+}
+
+__global__ void k_pcg_rz0(PcgVec vv, long long nchunks, PcgScalars* sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
+  sc->rz = seq_sum(vv.part_b, nchunks);
+}
+
+// A: Ap = A p and partial sums of p*Ap
+__global__ __launch_bounds__(PW_THREADS) void k_pcg_ap(Dims d, HelmOp op, PcgVec vv,
+                                                       const PwPlan* __restrict__ plans,
+                                                       long long nfull,
+                                                       const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double lds[512];
+  const long long chunk = blockIdx.x;
+  const long long base = chunk * PW_CHUNK;
+  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
+    const double ap = apply_A(d, op, vv.p, e);
+    vv.Ap[e] = ap;
+    return Two{vv.p[e] * ap, 0.0};
+  }, lds);
+  if (threadIdx.x == 0) vv.part_a[chunk] = s.a;
+}
+
+// B: alpha = rz / (p.Ap), one more iteration
+__global__ void k_pcg_alpha(PcgVec vv, long long nchunks, PcgScalars* sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
+  sc->alpha = sc->rz / seq_sum(vv.part_a, nchunks);
+  sc->it = sc->it + 1;
+}
+
+// C: x += alpha p, r -= alpha Ap, z = inv_diag r; partial sums of r*r, r*z
+__global__ __launch_bounds__(PW_THREADS) void k_pcg_update(Dims d, PcgVec vv,
+                                                           const PwPlan* __restrict__ plans,
+                                                           long long nfull,
+                                                           const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double lds[512];
+  const double alpha = sc->alpha;
+  const long long chunk = blockIdx.x;
+  const long long base = chunk * PW_CHUNK;
+  const Two s = chunk_sum<2>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
+    vv.x[e] = vv.x[e] + alpha * vv.p[e];
+    const double r = vv.r[e] - alpha * vv.Ap[e];
+    vv.r[e] = r;
+    const double z = vv.inv_diag[e] * r;
+    vv.z[e] = z;
+    return Two{r * r, r * z};
+  }, lds);
+  if (threadIdx.x == 0) {
+    vv.part_a[chunk] = s.a;
+    vv.part_b[chunk] = s.b;
+  }
+}
+
+// D: convergence test ||r|| <= rtol ||rhs||, else beta = rz_new / rz
+__global__ void k_pcg_check(PcgVec vv, long long nchunks, int max_iter, PcgScalars* sc,
+                            long long* counters) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
+  const double res = sqrt(seq_sum(vv.part_a, nchunks));
+  sc->res = res;
+  if (res <= sc->tol) {
+    sc->done = 1;
+    sc->converged = 1;
+    sc->iters = sc->it;
+    counters[0] += sc->it;
+    return;
+  }
+  const double rz_new = seq_sum(vv.part_b, nchunks);
+  sc->beta = rz_new / sc->rz;
+  sc->rz = rz_new;
+  if (sc->it >= max_iter) {
+    sc->done = 1;
+    sc->converged = 0;
+    sc->iters = max_iter;
+    counters[0] += max_iter;
+    counters[1] += 1;
+  }
+}
+
+// E: p = z + beta p
+__global__ void k_pcg_p(Dims d, PcgVec vv, const PcgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  vv.p[c] = vv.z[c] + sc->beta * vv.p[c];
+}
+
+// Generic pairwise dot (used by tests of the reduction): part[chunk] = sum(a*b)
+__global__ __launch_bounds__(PW_THREADS) void k_pw_dot(const double* __restrict__ a,
+                                                       const double* __restrict__ b,
+                                                       const PwPlan* __restrict__ plans,
+                                                       long long nfull, double* __restrict__ part) {
+  __shared__ double lds[512];
+  const long long chunk = blockIdx.x;
+  const long long base = chunk * PW_CHUNK;
+  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
+    return Two{a[e] * b[e], 0.0};
+  }, lds);
+  if (threadIdx.x == 0) part[chunk] = s.a;
+}
+
+__global__ void k_seq_sum(const double* __restrict__ part, long long nchunks, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = seq_sum(part, nchunks);
+}
+
+}  // namespace hoc

@@ -1,0 +1,123 @@
+// Kernel declarations and the small device structs shared between the
+// kernels and the host orchestration in hoc_abi.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hoc_params.h"
+
+namespace hoc {
+
+constexpr long long PW_CHUNK = 8192;  // NumPy's reduction buffer length
+constexpr int PW_MAX_LEAVES = 128;
+
+// Pairwise-summation plan of one chunk length: the leaf blocks (<= 128
+// elements each, in order) and the internal nodes of the halving tree,
+// level by level from the leaves up. Values are indexed leaves 0..L-1 then
+// nodes L..L+N-1; the root is the last node.
+struct PwPlan {
+  int nleaves, nnodes, nlevels, pad;
+  short leaf_off[PW_MAX_LEAVES];
+  short leaf_len[PW_MAX_LEAVES];
+  short node_l[PW_MAX_LEAVES];
+  short node_r[PW_MAX_LEAVES];
+  int level_end[16];
+};
+
+struct HelmOp {
+  const double *Ku, *Kv;
+  const uint8_t* mask;
+  double coef, cx, cy, dx2, dy2;
+};
+
+struct PcgVec {
+  double *x, *r, *z, *p, *Ap;
+  const double* inv_diag;
+  double *part_a, *part_b;  // per-chunk partial sums
+};
+
+struct PcgScalars {
+  double rz, alpha, beta, tol, norm_b, res;
+  int it, iters, done, converged;
+};
+
+// hoc_step_kernels.hip
+__global__ void k_geometry(Dims d, StepParams p, const double* dz3, const uint8_t* m3u,
+                           const uint8_t* m3v, const double* tau_x, const double* tau_y,
+                           double* dzi, double* tau_u, double* tau_v);
+__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* e_in, const double* u,
+                          const double* v, const double* b, double* Km, double* Kh,
+                          double* e_out, double* s_n2, double* s_sh, double* s_l,
+                          double* s_c);
+__global__ void k_closure_n2(Dims d, Geo geo, const double* u, const double* v,
+                             const double* b, double* s_n2);
+__global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* e_in,
+                                   const double* s_n2, double* s_l);
+__global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* Km,
+                               const double* Kh, double* sub_m, double* den_m, double* cst_m,
+                               double* sub_b, double* den_b, double* cst_b, double* q);
+__global__ void k_kukv(Dims d, Geo geo, const double* q, double* Ku, double* Kv);
+__global__ void k_phi(Dims d, StepParams p, Geo geo, const double* b, double* phi);
+__global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* u, const double* v,
+                         double* w);
+__global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double* u,
+                                    const double* v, const double* b, const double* phi,
+                                    const double* w, const double* Km, double* pgx,
+                                    double* pgy, double* avu, double* avv, double* dzu,
+                                    double* dzv, double* stu, double* stv);
+__global__ void k_depth_sum(Dims d, Geo geo, const double* u, const double* v, double* U,
+                            double* V);
+__global__ void k_div2(Dims d, StepParams p, Geo geo, const double* U, const double* V,
+                       double* out);
+__global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* eta,
+                          const double* div_old, const double* Ug, const double* Vg,
+                          double* rhs);
+__global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* eta, const double* U,
+                         const double* V, double* eta_new);
+__global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* u, const double* v,
+                            const double* eta, const double* u_it, const double* v_it,
+                            const double* pgx, const double* pgy, const double* avu,
+                            const double* avv, const double* dzu, const double* dzv,
+                            const double* stu, const double* stv, const double* sub,
+                            const double* den, const double* cst, double* gh_u, double* gh_v,
+                            double* Ug, double* Vg);
+__global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* gh_u,
+                          const double* gh_v, const double* q, const double* eta_n,
+                          double* u_it, double* v_it);
+__global__ void k_tracers(Dims d, StepParams p, Geo geo, const double* T, const double* S,
+                          const double* b, const double* u, const double* v, const double* w,
+                          const double* Kh, const double* sub, const double* den,
+                          const double* cst, double* T_out, double* S_out, double* b_out);
+__global__ void k_thomas(long long ncol, int n, const double* sub, const double* diag,
+                         const double* sup, const double* rhs, double* x, double* cs);
+__global__ void k_eos(long long n, StepParams p, const double* T, const double* S,
+                      const double* z, double* b);
+__global__ void k_tracer_advection(Dims d, StepParams p, Geo geo, const double* cc,
+                                   const double* u, const double* v, const double* w,
+                                   double* out);
+__global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double* u,
+                                     const double* v, const double* w, double* au,
+                                     double* av);
+
+// hoc_helmholtz.hip
+__global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
+__global__ void k_pcg_init(Dims d, HelmOp op, PcgVec vv, const double* rhs,
+                           const PwPlan* plans, long long nfull);
+__global__ void k_pcg_init_z(Dims d, PcgVec vv, const PwPlan* plans, long long nfull,
+                             const PcgScalars* sc);
+__global__ void k_pcg_start(PcgVec vv, long long nchunks, double rtol, PcgScalars* sc);
+__global__ void k_pcg_rz0(PcgVec vv, long long nchunks, PcgScalars* sc);
+__global__ void k_pcg_ap(Dims d, HelmOp op, PcgVec vv, const PwPlan* plans, long long nfull,
+                         const PcgScalars* sc);
+__global__ void k_pcg_alpha(PcgVec vv, long long nchunks, PcgScalars* sc);
+__global__ void k_pcg_update(Dims d, PcgVec vv, const PwPlan* plans, long long nfull,
+                             const PcgScalars* sc);
+__global__ void k_pcg_check(PcgVec vv, long long nchunks, int max_iter, PcgScalars* sc,
+                            long long* counters);
+__global__ void k_pcg_p(Dims d, PcgVec vv, const PcgScalars* sc);
+__global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,
+                         long long nfull, double* part);
+__global__ void k_seq_sum(const double* part, long long nchunks, double* out);
+
+}  // namespace hoc

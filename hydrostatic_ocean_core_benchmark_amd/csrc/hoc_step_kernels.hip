@@ -1,0 +1,1045 @@
+// Kernels of the ocean-core time step (everything except the 2D Helmholtz
+// solve, which lives in hoc_helmholtz.hip).
+//
+// Two launch shapes:
+//   column kernels: one lane per (j, i) column, marching k; lanes are
+//     consecutive in i, so every level's load is one coalesced row segment.
+//     Used where the work is a recurrence in k (tridiagonal sweeps, prefix
+//     sums, the TKE mixing-length march) and the horizontal stencil at each
+//     level is fused into the same pass.
+//   cell kernels: one lane per (k, j, i) for the pure 3D stencils.
+//
+// Column recurrences keep their forward-sweep values (c*, d*) in global
+// scratch rather than registers so that any nz runs through the same code.
+#include "hoc_device.h"
+#include "hoc_kernels.h"
+
+namespace hoc {
+
+// ------------------------------------------------------------ geometry
+// Interface thickness of dz3 (libs/core/vertical_var.py:21-32) and the
+// surface stress per unit density on faces, (tau / rho0) * mask3[u|v][0]
+// (libs/core/model3d_v05.py:355-356).
+__global__ void k_geometry(Dims d, StepParams p, const double* __restrict__ dz3,
+                           const uint8_t* __restrict__ m3u, const uint8_t* __restrict__ m3v,
+                           const double* __restrict__ tau_x, const double* __restrict__ tau_y,
+                           double* __restrict__ dzi, double* __restrict__ tau_u,
+                           double* __restrict__ tau_v) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  dzi[c] = dz3[c];
+  for (int k = 1; k < nz; ++k) dzi[k * n2 + c] = 0.5 * (dz3[(k - 1) * n2 + c] + dz3[k * n2 + c]);
+  dzi[nz * n2 + c] = dz3[(nz - 1) * n2 + c];
+  tau_u[c] = (tau_x[c] / p.rho0) * mk(m3u[c]);
+  tau_v[c] = (tau_y[c] / p.rho0) * mk(m3v[c]);
+}
+
+// ---------------------------------------------------------- TKE closure
+// One lane per column: N^2 and shear on interfaces, Gaspar (integral) or
+// recursive mixing lengths, the implicit interface tridiagonal for e, then
+// K_m and K_h (libs/core/closure.py:141-205). The march of the integral
+// length stops exactly where the reference's full loop stops changing
+// anything (past the column end, or once the energy budget is spent).
+// Scratch (each [nz+1][ny][nx]): s_n2 = N^2, s_sh = shear^2, s_l = l_k,
+// s_c = Thomas c*. e_out receives d* during the forward sweep.
+__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
+                          const double* __restrict__ u, const double* __restrict__ v,
+                          const double* __restrict__ b, double* __restrict__ Km,
+                          double* __restrict__ Kh, double* __restrict__ e_out,
+                          double* __restrict__ s_n2, double* __restrict__ s_sh,
+                          double* __restrict__ s_l, double* __restrict__ s_c) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  const long long cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cs = at2(d, wrap(j, -1, d.ny), i);
+  const uint8_t* m3 = geo.m3;
+
+  // phase A: N^2, shear^2 on interfaces; depth of the wet column
+  double h = 0.0;
+  for (int k = 0; k < nz; ++k) h = h + geo.dz3[k * n2 + c];
+  double uc_prev = 0.0, vc_prev = 0.0, b_prev = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    double uc = 0.0, vc = 0.0, bk = 0.0;
+    if (k < nz) {
+      uc = 0.5 * (u[k * n2 + c] + u[k * n2 + cw]);
+      vc = 0.5 * (v[k * n2 + c] + v[k * n2 + cs]);
+      bk = b[k * n2 + c];
+    }
+    double n2k = 0.0, sh = 0.0;
+    if (k >= 1 && k < nz) {
+      const double wet = mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]);
+      const double idz = safe_inv(geo.dzi[k * n2 + c]);
+      const double du = (uc_prev - uc) * idz;
+      const double dv = (vc_prev - vc) * idz;
+      const double db = (b_prev - bk) * idz;
+      sh = (du * du + dv * dv) * wet;
+      n2k = db * wet;
+    }
+    s_n2[k * n2 + c] = n2k;
+    s_sh[k * n2 + c] = sh;
+    uc_prev = uc;
+    vc_prev = vc;
+    b_prev = bk;
+  }
+
+  // phase B+C: mixing lengths and the forward sweep, interface by interface
+  const double kb = geo.kappa_b[c];
+  const double dt = p.dt;
+  double zi = 0.0;              // depth of interface k below the surface
+  double c_prev = 0.0, d_prev = 0.0;
+  double rhs0 = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
+    const double wet = (k >= 1 && k < nz) ? mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]) : 0.0;
+    const double ek = e_in[k * n2 + c];
+    const double n2k = s_n2[k * n2 + c];
+    double l_up, l_dn;
+    if (!p.mxl_recursive) {
+      const double budget0 = nmax(ek, p.e_min);
+      // upward march: interface k crosses k-1, k-2, ..., 0
+      double l = 0.0, bud = budget0;
+      for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
+        const double n2j = nmax(s_n2[kk * n2 + c], 0.0);
+        const double dzj = geo.dzi[kk * n2 + c];
+        const double cost = n2j * dzj * (l + 0.5 * dzj);
+        double step;
+        if (cost <= bud) {
+          step = dzj;
+          bud = bud - cost;
+        } else {
+          if (n2j > 0.0) {
+            const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
+            step = nmin(nmax(s, 0.0), dzj);
+          } else {
+            step = dzj;
+          }
+          bud = 0.0;
+        }
+        l = l + step;
+      }
+      l_up = nmin(l, zi + p.z_0) * wet;
+      // downward march: interface k crosses k+1, ..., nz
+      l = 0.0;
+      bud = budget0;
+      for (int kk = k + 1; kk <= nz && bud > 0.0; ++kk) {
+        const double n2j = nmax(s_n2[kk * n2 + c], 0.0);
+        const double dzj = geo.dzi[kk * n2 + c];
+        const double cost = n2j * dzj * (l + 0.5 * dzj);
+        double step;
+        if (cost <= bud) {
+          step = dzj;
+          bud = bud - cost;
+        } else {
+          if (n2j > 0.0) {
+            const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
+            step = nmin(nmax(s, 0.0), dzj);
+          } else {
+            step = dzj;
+          }
+          bud = 0.0;
+        }
+        l = l + step;
+      }
+      const double zb = nmax(h - zi, 0.0);
+      l_dn = nmin(l, zb + p.z_0) * wet;
+    } else {
+      // recursive form: lengths precomputed into s_l by k_mixing_recursive
+      l_up = s_l[k * n2 + c];
+      l_dn = l_up;
+    }
+    const double l_k = nmax(p.l_min, sqrt(l_up * l_dn));
+    const double l_eps = nmax(p.l_min, nmin(l_up, l_dn));
+    s_l[k * n2 + c] = l_k;
+
+    const double kmo = Km[k * n2 + c];
+    const double kho = Kh[k * n2 + c];
+    const double prod = (kmo - p.nu_b) * s_sh[k * n2 + c] - (kho - kb) * n2k;
+    const double diss = p.c_eps * sqrt(nmax(ek, p.e_min)) / l_eps;
+    double a_up = 0.0, a_dn = 0.0;
+    if (k >= 1 && k < nz) {
+      const double idzi = safe_inv(geo.dzi[k * n2 + c]);
+      const double klay_up = 0.5 * (Km[(k - 1) * n2 + c] + kmo);
+      const double klay_dn = 0.5 * (kmo + Km[(k + 1) * n2 + c]);
+      a_up = dt * klay_up * safe_inv(geo.dz3[(k - 1) * n2 + c]) * idzi;
+      a_dn = dt * klay_dn * safe_inv(geo.dz3[k * n2 + c]) * idzi;
+    }
+    double sub = -a_up, sup = -a_dn;
+    double diag = 1.0 + a_up + a_dn + dt * diss;
+    double rhs = ek + dt * prod;
+    if (wet <= 0.0) {
+      diag = 1.0;
+      sub = 0.0;
+      sup = 0.0;
+      rhs = p.e_min;
+    }
+    if (k == 0) {
+      rhs = m3[c] ? p.e_bb * geo.ustar2[c] : p.e_min;
+      rhs0 = rhs;
+    }
+    double cst, dst;
+    if (k == 0) {
+      cst = sup / diag;
+      dst = rhs / diag;
+    } else {
+      const double den = diag - sub * c_prev;
+      cst = sup / den;
+      dst = (rhs - sub * d_prev) / den;
+    }
+    s_c[k * n2 + c] = cst;
+    e_out[k * n2 + c] = dst;
+    c_prev = cst;
+    d_prev = dst;
+  }
+
+  // phase D: back-substitution, floors, eddy coefficients
+  double x_next = 0.0;
+  for (int k = nz; k >= 0; --k) {
+    const double dst = e_out[k * n2 + c];
+    const double x = (k == nz) ? dst : dst - s_c[k * n2 + c] * x_next;
+    x_next = x;
+    const double wet = (k >= 1 && k < nz) ? mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]) : 0.0;
+    double en = nmax(x, p.e_min);
+    en = wet > 0.0 ? en : p.e_min;
+    if (k == 0) en = rhs0;
+    const double km = p.c_k * s_l[k * n2 + c] * sqrt(en) * wet + p.nu_b;
+    const double kh = (km - p.nu_b) / p.pr_t + kb;
+    e_out[k * n2 + c] = en;
+    Km[k * n2 + c] = km;
+    Kh[k * n2 + c] = kh;
+  }
+}
+
+// Recursive (NEMO nn_mxl=2) mixing length into s_l (libs/core/closure.py:106-129).
+// Runs after the N^2 pass of k_closure_n2 and before k_closure's solve.
+__global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
+                                   const double* __restrict__ s_n2, double* __restrict__ s_l) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  double h = 0.0;
+  for (int k = 0; k < nz; ++k) h = h + geo.dz3[k * n2 + c];
+  double zi = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
+    const double n2c = nmax(s_n2[k * n2 + c], p.n2_min);
+    const double ek = nmax(e_in[k * n2 + c], p.e_min);
+    double l = sqrt(2.0 * ek / n2c);
+    const double zb = nmax(h - zi, 0.0);
+    l = nmin(l, nmin(zi + p.z_0, zb + p.z_0));
+    if (k > 0) l = nmin(l, s_l[(k - 1) * n2 + c] + geo.dzi[k * n2 + c]);
+    s_l[k * n2 + c] = l;
+  }
+  for (int k = nz - 1; k >= 0; --k) {
+    const double l = nmin(s_l[k * n2 + c], s_l[(k + 1) * n2 + c] + geo.dzi[(k + 1) * n2 + c]);
+    s_l[k * n2 + c] = l;
+  }
+  for (int k = 0; k <= nz; ++k) {
+    const double wet = (k >= 1 && k < nz)
+                           ? mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]) : 0.0;
+    s_l[k * n2 + c] = s_l[k * n2 + c] * wet;
+  }
+}
+
+// N^2 / shear pass alone (the recursive form needs N^2 before the solve).
+__global__ void k_closure_n2(Dims d, Geo geo, const double* __restrict__ u,
+                             const double* __restrict__ v, const double* __restrict__ b,
+                             double* __restrict__ s_n2) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  (void)u;
+  (void)v;
+  double b_prev = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    const double bk = k < nz ? b[k * n2 + c] : 0.0;
+    double n2k = 0.0;
+    if (k >= 1 && k < nz) {
+      const double wet = mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]);
+      const double idz = safe_inv(geo.dzi[k * n2 + c]);
+      n2k = ((b_prev - bk) * idz) * wet;
+    }
+    s_n2[k * n2 + c] = n2k;
+    b_prev = bk;
+  }
+}
+
+// ------------------------------------------------ vertical coefficients
+// Rows of A = I - theta_v dt Dz for viscosity nu on the layer grid with the
+// surface, bottom-drag and dry-cell rows of libs/core/vertical_var.py:35-77,
+// stored factored as (sub, den, c*) of the Thomas forward sweep
+// (libs/core/vertical.py:33-38), which depends on the matrix alone. Each
+// later solve with this matrix then runs d*_k = (r_k - sub_k d*_{k-1}) / den_k
+// and x_k = d*_k - c*_k x_{k+1}, bitwise the unfactored sweep.
+__device__ __forceinline__ void factor_column(const Dims& d, const StepParams& p, const Geo& geo,
+                                              long long c, const double* __restrict__ nu,
+                                              double drag_term, double* __restrict__ sub_o,
+                                              double* __restrict__ den_o,
+                                              double* __restrict__ cst_o) {
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  double c_prev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const double inv_dz = safe_inv(geo.dz3[k * n2 + c]);
+    const double fac = p.thv_dt * inv_dz;
+    const double a_top = fac * nu[k * n2 + c] * safe_inv(geo.dzi[k * n2 + c]);
+    const double a_bot = fac * nu[(k + 1) * n2 + c] * safe_inv(geo.dzi[(k + 1) * n2 + c]);
+    double sub = -a_top, sup = -a_bot;
+    double diag = 1.0 + a_top + a_bot;
+    if (k == 0) {
+      diag = 1.0 + a_bot;
+      sub = 0.0;
+    }
+    const bool wet = geo.m3[k * n2 + c] != 0;
+    const bool below = (k + 1 < nz) && geo.m3[(k + 1) * n2 + c] != 0;
+    if (wet && !below) {
+      diag = 1.0 + a_top + drag_term * inv_dz;
+      sup = 0.0;
+    }
+    if (!wet) {
+      diag = 1.0;
+      sub = 0.0;
+      sup = 0.0;
+    }
+    const double den = (k == 0) ? diag : diag - sub * c_prev;
+    const double cst = sup / den;
+    sub_o[k * n2 + c] = sub;
+    den_o[k * n2 + c] = den;
+    cst_o[k * n2 + c] = cst;
+    c_prev = cst;
+  }
+}
+
+// tri_m (viscosity Km, bottom drag) and tri_b (diffusivity Kh, no drag),
+// then q = A_m^-1 1 (libs/core/model3d_v05.py:141-153).
+__global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* __restrict__ Km,
+                               const double* __restrict__ Kh, double* __restrict__ sub_m,
+                               double* __restrict__ den_m, double* __restrict__ cst_m,
+                               double* __restrict__ sub_b, double* __restrict__ den_b,
+                               double* __restrict__ cst_b, double* __restrict__ q) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  factor_column(d, p, geo, c, Km, p.thv_dt_drag, sub_m, den_m, cst_m);
+  factor_column(d, p, geo, c, Kh, p.thv_dt * 0.0, sub_b, den_b, cst_b);
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  double d_prev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long o = k * n2 + c;
+    const double dst = (k == 0) ? 1.0 / den_m[o] : (1.0 - sub_m[o] * d_prev) / den_m[o];
+    q[o] = dst;
+    d_prev = dst;
+  }
+  double x_next = q[(nz - 1) * n2 + c];
+  for (int k = nz - 2; k >= 0; --k) {
+    const long long o = k * n2 + c;
+    const double x = q[o] - cst_m[o] * x_next;
+    q[o] = x;
+    x_next = x;
+  }
+}
+
+// Helmholtz face coefficients Ku = sum_k dz3u min(q, q_east) mask_u and Kv
+// (libs/core/model3d_v05.py:154-157), summed in k order like NumPy's
+// axis-0 reduction.
+__global__ void k_kukv(Dims d, Geo geo, const double* __restrict__ q, double* __restrict__ Ku,
+                       double* __restrict__ Kv) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i);
+  const long long n2 = d.n2;
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    const double qc = q[k * n2 + c];
+    const double tu = geo.dz3u[k * n2 + c] * nmin(qc, q[k * n2 + ce]);
+    const double tv = geo.dz3v[k * n2 + c] * nmin(qc, q[k * n2 + cn]);
+    su = (k == 0) ? tu : su + tu;
+    sv = (k == 0) ? tv : sv + tv;
+  }
+  Ku[c] = su * mk(geo.mu2[c]);
+  Kv[c] = sv * mk(geo.mv2[c]);
+}
+
+// --------------------------------------------------- pressure potential
+// Phi_k = 1/2 dz_k b_k + sum_{k'<k} dz b (libs/core/vertical_var.py:132-136);
+// optionally the thickness-weighted depth mean removed (:139-145).
+__global__ void k_phi(Dims d, StepParams p, Geo geo, const double* __restrict__ b,
+                      double* __restrict__ phi) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  double above = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    const double dz = geo.dz3[k * n2 + c];
+    const double bk = b[k * n2 + c];
+    phi[k * n2 + c] = 0.5 * dz * bk + above;
+    const double t = dz * bk;
+    above = (k == 0) ? t : above + t;
+  }
+  if (p.pgf_remove) {
+    double tot = 0.0, wsum = 0.0;
+    for (int k = 0; k < d.nz; ++k) {
+      const double dz = geo.dz3[k * n2 + c];
+      const double t = dz * phi[k * n2 + c];
+      tot = (k == 0) ? dz : tot + dz;
+      wsum = (k == 0) ? t : wsum + t;
+    }
+    const double mean = wsum * safe_inv(tot);
+    for (int k = 0; k < d.nz; ++k) phi[k * n2 + c] = phi[k * n2 + c] - mean;
+  }
+}
+
+// ------------------------------------------------- vertical velocity
+// w_k = -sum_{k'>=k} div(dz3u u, dz3v v), w_nz = 0, summed from the bottom
+// (libs/core/vertical_var.py:109-121 with model3d_v05.py:232-235).
+__global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
+                         const double* __restrict__ v, double* __restrict__ w) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cs = at2(d, wrap(j, -1, d.ny), i);
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  double s = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long o = k * n2;
+    const double U = geo.dz3u[o + c] * u[o + c];
+    const double Uw = geo.dz3u[o + cw] * u[o + cw];
+    const double V = geo.dz3v[o + c] * v[o + c];
+    const double Vs = geo.dz3v[o + cs] * v[o + cs];
+    const double dv = mk(geo.m3[o + c]) * ((U - Uw) / p.dx + (V - Vs) / p.dy);
+    s = (k == nz - 1) ? dv : s + dv;
+    w[o + c] = -s;
+  }
+  w[(long long)nz * n2 + c] = 0.0;
+}
+
+// ------------------------------------------------ explicit momentum terms
+struct Nb {  // periodic neighbour offsets of a column
+  long long c, e, w, ee, ww, n, s, nn, ss, ne, nw, se, sw;
+};
+__device__ __forceinline__ Nb neighbours(const Dims& d, int j, int i) {
+  Nb q;
+  const int ie = wrap(i, 1, d.nx), iw = wrap(i, -1, d.nx);
+  const int iee = wrap(i, 2, d.nx), iww = wrap(i, -2, d.nx);
+  const int jn = wrap(j, 1, d.ny), js = wrap(j, -1, d.ny);
+  const int jnn = wrap(j, 2, d.ny), jss = wrap(j, -2, d.ny);
+  q.c = at2(d, j, i);
+  q.e = at2(d, j, ie);
+  q.w = at2(d, j, iw);
+  q.ee = at2(d, j, iee);
+  q.ww = at2(d, j, iww);
+  q.n = at2(d, jn, i);
+  q.s = at2(d, js, i);
+  q.nn = at2(d, jnn, i);
+  q.ss = at2(d, jss, i);
+  q.ne = at2(d, jn, ie);
+  q.nw = at2(d, jn, iw);
+  q.se = at2(d, js, ie);
+  q.sw = at2(d, js, iw);
+  return q;
+}
+
+// Coriolis partner of u: v averaged over the wet v faces around the u point
+// (libs/core/operators_masked.py:45-52).
+__device__ __forceinline__ double avg_v_to_u(const double* v, const uint8_t* mv, const uint8_t* mu,
+                                             long long o, const Nb& q) {
+  const double w0 = mk(mv[o + q.c]), w1 = mk(mv[o + q.s]), w2 = mk(mv[o + q.e]),
+               w3 = mk(mv[o + q.se]);
+  const double num = 0.0 + w0 * v[o + q.c] + w1 * v[o + q.s] + w2 * v[o + q.e] + w3 * v[o + q.se];
+  const double den = 0.0 + w0 + w1 + w2 + w3;
+  const double out = den > 0.0 ? num / den : 0.0;
+  return out * mk(mu[o + q.c]);
+}
+// (libs/core/operators_masked.py:55-61)
+__device__ __forceinline__ double avg_u_to_v(const double* u, const uint8_t* mu, const uint8_t* mv,
+                                             long long o, const Nb& q) {
+  const double w0 = mk(mu[o + q.c]), w1 = mk(mu[o + q.w]), w2 = mk(mu[o + q.n]),
+               w3 = mk(mu[o + q.nw]);
+  const double num = 0.0 + w0 * u[o + q.c] + w1 * u[o + q.w] + w2 * u[o + q.n] + w3 * u[o + q.nw];
+  const double den = 0.0 + w0 + w1 + w2 + w3;
+  const double out = den > 0.0 ? num / den : 0.0;
+  return out * mk(mv[o + q.c]);
+}
+
+// Explicit vertical diffusion Dz[f] at layer k with surface flux sflux, no
+// flux through a column's bottom interface and linear drag on the bottom
+// cell (libs/core/vertical_var.py:80-106). dzf is the thickness of f's
+// points (dz3u for u, dz3v for v, dz3 for tracers); cell mask3 decides the
+// bottom.
+__device__ __forceinline__ double explicit_diffusion(const Dims& d, const uint8_t* m3,
+                                                     const double* f, const double* nu,
+                                                     const double* dzf, double sflux,
+                                                     double drag, long long c, int k) {
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  auto dzi_f = [&](int kk) -> double {  // interface_thickness(dzf)[kk]
+    if (kk == 0) return dzf[c];
+    if (kk == nz) return dzf[(nz - 1) * n2 + c];
+    return 0.5 * (dzf[(kk - 1) * n2 + c] + dzf[kk * n2 + c]);
+  };
+  auto is_bottom = [&](int kk) -> bool {
+    return m3[kk * n2 + c] != 0 && !((kk + 1 < nz) && m3[(kk + 1) * n2 + c] != 0);
+  };
+  // flux through the top (interface k) and bottom (interface k+1) of cell k
+  double ft;
+  if (k == 0) {
+    ft = sflux;
+  } else {
+    ft = nu[k * n2 + c] * (f[(k - 1) * n2 + c] - f[k * n2 + c]) * safe_inv(dzi_f(k));
+    if (is_bottom(k - 1)) ft = 0.0;
+  }
+  double fb = 0.0;
+  if (k + 1 < nz) {
+    fb = nu[(k + 1) * n2 + c] * (f[k * n2 + c] - f[(k + 1) * n2 + c]) * safe_inv(dzi_f(k + 1));
+    if (is_bottom(k)) fb = 0.0;
+  }
+  const double inv = safe_inv(dzf[k * n2 + c]);
+  const double out = (ft - fb) * inv;
+  const double dr = drag * f[k * n2 + c] * (is_bottom(k) ? 1.0 : 0.0);
+  return (out - dr * inv) * (m3[k * n2 + c] != 0 ? 1.0 : 0.0);
+}
+
+// u-momentum advection at (k, j, i) (libs/core/advection_v06.py:77-99 for
+// up3/up3_tvd, libs/core/advection.py:62-79 for centred2).
+__device__ double advect_u(const Dims& d, const StepParams& p, const uint8_t* m3u,
+                           const double* u, const double* v, const double* w, long long o,
+                           int k, const Nb& q) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  const double* U = u + o;
+  const double* V = v + o;
+  double ax, ay, az;
+  if (p.advection == ADV_CENTRED2) {
+    const double ub = 0.5 * (U[q.w] + U[q.c]);
+    const double ube = 0.5 * (U[q.c] + U[q.e]);
+    ax = (ube * ube - ub * ub) / p.dx;
+    const double vb = 0.5 * (V[q.c] + V[q.e]);
+    const double vbs = 0.5 * (V[q.s] + V[q.se]);
+    const double fy = vb * (0.5 * (U[q.c] + U[q.n]));
+    const double fys = vbs * (0.5 * (U[q.s] + U[q.c]));
+    ay = (fy - fys) / p.dy;
+  } else {
+    const uint8_t* M = m3u + o;
+    // x flux at centres i (faces i-1|i) and i+1 (faces i|i+1)
+    const double ub = 0.5 * (U[q.w] + U[q.c]);
+    const double ube = 0.5 * (U[q.c] + U[q.e]);
+    const double uc = kappa_face(U[q.ww], U[q.w], U[q.c], U[q.e], ub, M[q.ww] != 0,
+                                 M[q.e] != 0, lim);
+    const double uce = kappa_face(U[q.w], U[q.c], U[q.e], U[q.ee], ube, M[q.w] != 0,
+                                  M[q.ee] != 0, lim);
+    ax = (ube * uce - ub * uc) / p.dx;
+    // y flux at corners j (between u[j], u[j+1]) and j-1
+    const double vb = 0.5 * (V[q.c] + V[q.e]);
+    const double vbs = 0.5 * (V[q.s] + V[q.se]);
+    const double uy = kappa_face(U[q.s], U[q.c], U[q.n], U[q.nn], vb, M[q.s] != 0,
+                                 M[q.nn] != 0, lim);
+    const double uys = kappa_face(U[q.ss], U[q.s], U[q.c], U[q.n], vbs, M[q.ss] != 0,
+                                  M[q.n] != 0, lim);
+    ay = (vb * uy - vbs * uys) / p.dy;
+  }
+  // vertical flux on interfaces k (top) and k+1 (bottom)
+  auto fz = [&](int kk) -> double {
+    if (kk <= 0 || kk >= nz) return 0.0;
+    const long long a = (long long)kk * n2;
+    const double wb = 0.5 * (w[a + q.c] + w[a + q.e]);
+    const double above = u[a - n2 + q.c], below = u[a + q.c];
+    if (p.advection == ADV_CENTRED2) return wb * 0.5 * (above + below);
+    const double far_up = kk >= 2 ? u[a - 2 * n2 + q.c] : 0.0;
+    const double far_dn = kk <= nz - 2 ? u[a + n2 + q.c] : 0.0;
+    const bool hu = kk >= 2 && m3u[a - 2 * n2 + q.c] != 0;
+    const bool hd = kk <= nz - 2 && m3u[a + n2 + q.c] != 0;
+    return wb * kappa_face_z(far_up, above, below, far_dn, wb, hu, hd, lim);
+  };
+  az = (fz(k) - fz(k + 1)) / p.dz_nom;
+  return ax + ay + az;
+}
+
+// v-momentum advection (libs/core/advection_v06.py:100-108, advection.py:81-95).
+__device__ double advect_v(const Dims& d, const StepParams& p, const uint8_t* m3v,
+                           const double* u, const double* v, const double* w, long long o,
+                           int k, const Nb& q) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  const double* U = u + o;
+  const double* V = v + o;
+  double bx, by, bz;
+  if (p.advection == ADV_CENTRED2) {
+    const double vb = 0.5 * (V[q.s] + V[q.c]);
+    const double vbn = 0.5 * (V[q.c] + V[q.n]);
+    by = (vbn * vbn - vb * vb) / p.dy;
+    const double ub = 0.5 * (U[q.c] + U[q.n]);
+    const double ubw = 0.5 * (U[q.w] + U[q.nw]);
+    const double fx = ub * (0.5 * (V[q.c] + V[q.e]));
+    const double fxw = ubw * (0.5 * (V[q.w] + V[q.c]));
+    bx = (fx - fxw) / p.dx;
+  } else {
+    const uint8_t* M = m3v + o;
+    const double vb = 0.5 * (V[q.s] + V[q.c]);
+    const double vbn = 0.5 * (V[q.c] + V[q.n]);
+    const double vc = kappa_face(V[q.ss], V[q.s], V[q.c], V[q.n], vb, M[q.ss] != 0,
+                                 M[q.n] != 0, lim);
+    const double vcn = kappa_face(V[q.s], V[q.c], V[q.n], V[q.nn], vbn, M[q.s] != 0,
+                                  M[q.nn] != 0, lim);
+    by = (vbn * vcn - vb * vc) / p.dy;
+    const double ub = 0.5 * (U[q.c] + U[q.n]);
+    const double ubw = 0.5 * (U[q.w] + U[q.nw]);
+    const double vx = kappa_face(V[q.w], V[q.c], V[q.e], V[q.ee], ub, M[q.w] != 0,
+                                 M[q.ee] != 0, lim);
+    const double vxw = kappa_face(V[q.ww], V[q.w], V[q.c], V[q.e], ubw, M[q.ww] != 0,
+                                  M[q.e] != 0, lim);
+    bx = (ub * vx - ubw * vxw) / p.dx;
+  }
+  auto fz = [&](int kk) -> double {
+    if (kk <= 0 || kk >= nz) return 0.0;
+    const long long a = (long long)kk * n2;
+    const double wb = 0.5 * (w[a + q.c] + w[a + q.n]);
+    const double above = v[a - n2 + q.c], below = v[a + q.c];
+    if (p.advection == ADV_CENTRED2) return wb * 0.5 * (above + below);
+    const double far_up = kk >= 2 ? v[a - 2 * n2 + q.c] : 0.0;
+    const double far_dn = kk <= nz - 2 ? v[a + n2 + q.c] : 0.0;
+    const bool hu = kk >= 2 && m3v[a - 2 * n2 + q.c] != 0;
+    const bool hd = kk <= nz - 2 && m3v[a + n2 + q.c] != 0;
+    return wb * kappa_face_z(far_up, above, below, far_dn, wb, hu, hd, lim);
+  };
+  bz = (fz(k) - fz(k + 1)) / p.dz_nom;
+  return bx + by + bz;
+}
+
+// Masked horizontal Laplacian D o G of a at (o + q.c) with gradient masks
+// gmu (x faces), gmv (y faces) and divergence mask dm
+// (libs/core/operators_masked.py:64-68).
+__device__ __forceinline__ double laplacian(const StepParams& p, const double* a,
+                                            const uint8_t* gmu, const uint8_t* gmv,
+                                            const uint8_t* dm, long long o, const Nb& q) {
+  const double* A = a + o;
+  const double gx = mk(gmu[o + q.c]) * (A[q.e] - A[q.c]) / p.dx;
+  const double gxw = mk(gmu[o + q.w]) * (A[q.c] - A[q.w]) / p.dx;
+  const double gy = mk(gmv[o + q.c]) * (A[q.n] - A[q.c]) / p.dy;
+  const double gys = mk(gmv[o + q.s]) * (A[q.c] - A[q.s]) / p.dy;
+  return mk(dm[o + q.c]) * ((gx - gxw) / p.dx + (gy - gys) / p.dy);
+}
+
+// One lane per cell: the step's explicit momentum terms from the state at
+// the start of the step (libs/core/model3d_v05.py:352-370):
+//   pgx, pgy   baroclinic pressure gradient at the face depth (:195-230)
+//   avu, avv   Coriolis partners avg_u_to_v(u), avg_v_to_u(v)
+//   dzu, dzv   explicit vertical diffusion with surface stress and drag
+//   stu, stv   theta_v dt tau / dz (k = 0) + dt * (-advection + A_h lap)
+__global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
+                                    const double* __restrict__ v, const double* __restrict__ b,
+                                    const double* __restrict__ phi, const double* __restrict__ w,
+                                    const double* __restrict__ Km, double* __restrict__ pgx,
+                                    double* __restrict__ pgy, double* __restrict__ avu,
+                                    double* __restrict__ avv, double* __restrict__ dzu,
+                                    double* __restrict__ dzv, double* __restrict__ stu,
+                                    double* __restrict__ stv) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const int k = (int)(t / d.n2);
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const long long o = (long long)k * d.n2;
+
+  // pressure gradient
+  double gx, gy;
+  if (p.pgf_correction) {
+    const double zc = geo.zc[o + c];
+    {
+      const double zr = geo.zc[o + q.e];
+      const double zf = 0.5 * (zc + zr);
+      const double left = phi[o + c] + b[o + c] * (zf - zc);
+      const double right = phi[o + q.e] + b[o + q.e] * (zf - zr);
+      gx = mk(geo.m3u[o + c]) * (right - left) / p.dx;
+    }
+    {
+      const double zr = geo.zc[o + q.n];
+      const double zf = 0.5 * (zc + zr);
+      const double left = phi[o + c] + b[o + c] * (zf - zc);
+      const double right = phi[o + q.n] + b[o + q.n] * (zf - zr);
+      gy = mk(geo.m3v[o + c]) * (right - left) / p.dy;
+    }
+  } else {
+    gx = mk(geo.m3u[o + c]) * (phi[o + q.e] - phi[o + c]) / p.dx * mk(geo.m3u[o + c]);
+    gy = mk(geo.m3v[o + c]) * (phi[o + q.n] - phi[o + c]) / p.dy * mk(geo.m3v[o + c]);
+  }
+  pgx[t] = gx;
+  pgy[t] = gy;
+
+  avv[t] = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
+  avu[t] = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
+
+  dzu[t] = explicit_diffusion(d, geo.m3, u, Km, geo.dz3u, k == 0 ? geo.tau_u[c] : 0.0,
+                              p.bottom_drag, c, k);
+  dzv[t] = explicit_diffusion(d, geo.m3, v, Km, geo.dz3v, k == 0 ? geo.tau_v[c] : 0.0,
+                              p.bottom_drag, c, k);
+
+  double exu = 0.0, exv = 0.0;
+  if (p.advection != ADV_NONE) {
+    double au, av;
+    if (p.advection == ADV_UPWIND1) {
+      // upwind1 momentum uses the centred2 form (libs/core/model3d_v05.py:251-254)
+      StepParams pc = p;
+      pc.advection = ADV_CENTRED2;
+      au = advect_u(d, pc, geo.m3u, u, v, w, o, k, q);
+      av = advect_v(d, pc, geo.m3v, u, v, w, o, k, q);
+    } else {
+      au = advect_u(d, p, geo.m3u, u, v, w, o, k, q);
+      av = advect_v(d, p, geo.m3v, u, v, w, o, k, q);
+    }
+    exu = exu - au * mk(geo.m3u[o + c]);
+    exv = exv - av * mk(geo.m3v[o + c]);
+  }
+  if (p.has_A_h) {
+    exu = exu + p.A_h * laplacian(p, u, geo.m3u, geo.m3v, geo.m3u, o, q);
+    exv = exv + p.A_h * laplacian(p, v, geo.m3u, geo.m3v, geo.m3v, o, q);
+  }
+  double s0u = 0.0, s0v = 0.0;
+  if (k == 0) {
+    s0u = p.thv_dt * geo.tau_u[c] * safe_inv(geo.dz3u[c]);
+    s0v = p.thv_dt * geo.tau_v[c] * safe_inv(geo.dz3v[c]);
+  }
+  stu[t] = s0u + p.dt * exu;
+  stv[t] = s0v + p.dt * exv;
+}
+
+// Depth-integrated transports U = sum_k dz3u u, V = sum_k dz3v v (k order).
+__global__ void k_depth_sum(Dims d, Geo geo, const double* __restrict__ u,
+                            const double* __restrict__ v, double* __restrict__ U,
+                            double* __restrict__ V) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    const double tu = geo.dz3u[k * n2 + c] * u[k * n2 + c];
+    const double tv = geo.dz3v[k * n2 + c] * v[k * n2 + c];
+    su = (k == 0) ? tu : su + tu;
+    sv = (k == 0) ? tv : sv + tv;
+  }
+  U[c] = su;
+  V[c] = sv;
+}
+
+// div = mask ((U - U_w)/dx + (V - V_s)/dy) (libs/core/operators_masked.py:27-32).
+__device__ __forceinline__ double div2(const Dims& d, const StepParams& p, const Geo& geo,
+                                       const double* U, const double* V, long long c) {
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cs = at2(d, wrap(j, -1, d.ny), i);
+  return mk(geo.m2[c]) * ((U[c] - U[cw]) / p.dx + (V[c] - V[cs]) / p.dy);
+}
+
+__global__ void k_div2(Dims d, StepParams p, Geo geo, const double* __restrict__ U,
+                       const double* __restrict__ V, double* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  out[c] = div2(d, p, geo, U, V, c);
+}
+
+// rhs = eta - dt ((1 - th) div_old + th div(U_g, V_g))  (model3d_v05.py:398-400)
+__global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* __restrict__ eta,
+                          const double* __restrict__ div_old, const double* __restrict__ Ug,
+                          const double* __restrict__ Vg, double* __restrict__ rhs) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const double dg = div2(d, p, geo, Ug, Vg, c);
+  rhs[c] = eta[c] - p.dt * (p.one_m_th * div_old[c] + p.th * dg);
+}
+
+// fb: eta_new = eta - dt div(U, V)  (model3d_v05.py:410-413)
+__global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* __restrict__ eta,
+                         const double* __restrict__ U, const double* __restrict__ V,
+                         double* __restrict__ eta_new) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  eta_new[c] = eta[c] - p.dt * div2(d, p, geo, U, V, c);
+}
+
+// ------------------------------------------- predictor and column solves
+// One lane per column, one Picard pass (model3d_v05.py:387-396):
+//   v_cor = (1-tc) avv + tc avg_v_to_u(v_it),  u_cor likewise
+//   gu = u + dt (f v_cor + pgx) - g dt (1-th) Gx(eta) + (1-thv) dt dzu
+//   gh_u = A_m^-1 (gu + stu) * mask3, and the same for v,
+// then the depth sums Ug = sum dz3u gh_u, Vg = sum dz3v gh_v for the
+// Helmholtz right-hand side. gh_u / gh_v hold d* between the sweeps.
+__global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
+                            const double* __restrict__ v, const double* __restrict__ eta,
+                            const double* __restrict__ u_it, const double* __restrict__ v_it,
+                            const double* __restrict__ pgx, const double* __restrict__ pgy,
+                            const double* __restrict__ avu, const double* __restrict__ avv,
+                            const double* __restrict__ dzu, const double* __restrict__ dzv,
+                            const double* __restrict__ stu, const double* __restrict__ stv,
+                            const double* __restrict__ sub, const double* __restrict__ den,
+                            const double* __restrict__ cst, double* __restrict__ gh_u,
+                            double* __restrict__ gh_v, double* __restrict__ Ug,
+                            double* __restrict__ Vg) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const double gx2 = mk(geo.mu2[c]) * (eta[q.e] - eta[c]) / p.dx;
+  const double gy2 = mk(geo.mv2[c]) * (eta[q.n] - eta[c]) / p.dy;
+  double du_prev = 0.0, dv_prev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long o = k * n2;
+    const long long t = o + c;
+    const double vcor = p.one_m_tc * avv[t] + p.theta_cor * avg_v_to_u(v_it, geo.m3v, geo.m3u, o, q);
+    const double ucor = p.one_m_tc * avu[t] + p.theta_cor * avg_u_to_v(u_it, geo.m3u, geo.m3v, o, q);
+    const double gxe = gx2 * mk(geo.m3u[t]);
+    const double gye = gy2 * mk(geo.m3v[t]);
+    const double gu = u[t] + p.dt * (p.f * vcor + pgx[t]) - p.c_gx * gxe + p.c_dzd * dzu[t];
+    const double gv = v[t] + p.dt * (p.neg_f * ucor + pgy[t]) - p.c_gx * gye + p.c_dzd * dzv[t];
+    const double ru = gu + stu[t];
+    const double rv = gv + stv[t];
+    double du, dv;
+    if (k == 0) {
+      du = ru / den[t];
+      dv = rv / den[t];
+    } else {
+      du = (ru - sub[t] * du_prev) / den[t];
+      dv = (rv - sub[t] * dv_prev) / den[t];
+    }
+    gh_u[t] = du;
+    gh_v[t] = dv;
+    du_prev = du;
+    dv_prev = dv;
+  }
+  double xu = 0.0, xv = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long t = k * n2 + c;
+    if (k == nz - 1) {
+      xu = gh_u[t];
+      xv = gh_v[t];
+    } else {
+      xu = gh_u[t] - cst[t] * xu;
+      xv = gh_v[t] - cst[t] * xv;
+    }
+    const double mm = mk(geo.m3[t]);
+    gh_u[t] = xu * mm;
+    gh_v[t] = xv * mm;
+  }
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = k * n2 + c;
+    const double tu = geo.dz3u[t] * gh_u[t];
+    const double tv = geo.dz3v[t] * gh_v[t];
+    su = (k == 0) ? tu : su + tu;
+    sv = (k == 0) ? tv : sv + tv;
+  }
+  Ug[c] = su;
+  Vg[c] = sv;
+}
+
+// u_it = (gh_u - g dt th q Gx(eta_new) mask3u) mask3 (model3d_v05.py:404-406)
+__global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* __restrict__ gh_u,
+                          const double* __restrict__ gh_v, const double* __restrict__ q,
+                          const double* __restrict__ eta_n, double* __restrict__ u_it,
+                          double* __restrict__ v_it) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i);
+  const double gx = mk(geo.mu2[c]) * (eta_n[ce] - eta_n[c]) / p.dx;
+  const double gy = mk(geo.mv2[c]) * (eta_n[cn] - eta_n[c]) / p.dy;
+  const double mm = mk(geo.m3[t]);
+  u_it[t] = (gh_u[t] - p.c_bs * q[t] * gx * mk(geo.m3u[t])) * mm;
+  v_it[t] = (gh_v[t] - p.c_bs * q[t] * gy * mk(geo.m3v[t])) * mm;
+}
+
+// ------------------------------------------------------------- tracers
+// Flux-form tracer advection divergence at (k, j, i) with face thickness
+// (libs/core/model3d_v05.py:482-512).
+__device__ double tracer_advection(const Dims& d, const StepParams& p, const Geo& geo,
+                                   const double* cc, const double* u, const double* v,
+                                   const double* w, long long c, int k, const Nb& q) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const long long o = (long long)k * n2;
+  const double* C = cc + o;
+  const uint8_t* M = geo.m3 + o;
+  const bool up3 = p.advection == ADV_UP3 || p.advection == ADV_UP3_TVD;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  auto face = [&](double cm, double c0, double c1, double c2, double vel, bool wm,
+                  bool w2) -> double {
+    if (up3) return kappa_face(cm, c0, c1, c2, vel, wm, w2, lim);
+    if (p.advection == ADV_CENTRED2) return 0.5 * (c0 + c1);
+    return vel > 0.0 ? c0 : c1;
+  };
+  const double ue = u[o + q.c], uw = u[o + q.w];
+  const double vn = v[o + q.c], vs = v[o + q.s];
+  const double cx = face(C[q.w], C[q.c], C[q.e], C[q.ee], ue, M[q.w] != 0, M[q.ee] != 0);
+  const double cxw = face(C[q.ww], C[q.w], C[q.c], C[q.e], uw, M[q.ww] != 0, M[q.e] != 0);
+  const double cy = face(C[q.s], C[q.c], C[q.n], C[q.nn], vn, M[q.s] != 0, M[q.nn] != 0);
+  const double cys = face(C[q.ss], C[q.s], C[q.c], C[q.n], vs, M[q.ss] != 0, M[q.n] != 0);
+  const double fx = geo.dz3u[o + q.c] * ue * cx;
+  const double fxw = geo.dz3u[o + q.w] * uw * cxw;
+  const double fy = geo.dz3v[o + q.c] * vn * cy;
+  const double fys = geo.dz3v[o + q.s] * vs * cys;
+  const double inv = safe_inv(geo.dz3[o + c]);
+  const double hdiv = ((fx - fxw) / p.dx + (fy - fys) / p.dy) * inv;
+  auto cf = [&](int kk) -> double {  // interface value, kk in 0..nz
+    if (kk == 0) return cc[c];
+    if (kk == nz) return 0.0;
+    const long long a = (long long)kk * n2;
+    const double above = cc[a - n2 + c], below = cc[a + c];
+    if (!up3) return 0.5 * (above + below);
+    const double far_up = kk >= 2 ? cc[a - 2 * n2 + c] : 0.0;
+    const double far_dn = kk <= nz - 2 ? cc[a + n2 + c] : 0.0;
+    const bool hu = kk >= 2 && geo.m3[a - 2 * n2 + c] != 0;
+    const bool hd = kk <= nz - 2 && geo.m3[a + n2 + c] != 0;
+    return kappa_face_z(far_up, above, below, far_dn, w[a + c], hu, hd, lim);
+  };
+  const double vdiv = (w[o + c] * cf(k) - w[o + n2 + c] * cf(k + 1)) * inv;
+  return (hdiv + vdiv) * mk(M[q.c]);
+}
+
+// Right-hand side of one tracer at layer k (model3d_v05.py:466-479):
+// c + dt tend + (1-thv) dt Dz[c], plus theta_v dt flux / dz at the surface.
+__device__ double tracer_rhs(const Dims& d, const StepParams& p, const Geo& geo,
+                             const double* cc, const double* u, const double* v, const double* w,
+                             const double* Kh, double sflux, long long c, int k, const Nb& q,
+                             bool buoyancy_tracer) {
+  const long long o = (long long)k * d.n2;
+  double tend = 0.0;
+  if (buoyancy_tracer) tend = p.neg_N2 * (0.5 * (w[o + c] + w[o + d.n2 + c]));
+  if (p.advection != ADV_NONE) tend = tend - tracer_advection(d, p, geo, cc, u, v, w, c, k, q);
+  if (p.has_K_h) tend = tend + p.K_h * laplacian(p, cc, geo.m3u, geo.m3v, geo.m3, o, q);
+  const double dzd = explicit_diffusion(d, geo.m3, cc, Kh, geo.dz3, k == 0 ? sflux : 0.0, 0.0, c, k);
+  double r = cc[o + c] + p.dt * tend + p.c_dzd * dzd;
+  if (k == 0) r = r + p.thv_dt * sflux * safe_inv(geo.dz3[c]);
+  return r;
+}
+
+// Both tracers of a column solved on tri_b, then b from the EOS
+// (model3d_v05.py:514-533); buoyancy-only runs advance b with the -N^2 w
+// source instead. T_out/S_out (or b_out) hold d* between the sweeps.
+__global__ void k_tracers(Dims d, StepParams p, Geo geo, const double* __restrict__ T,
+                          const double* __restrict__ S, const double* __restrict__ b,
+                          const double* __restrict__ u, const double* __restrict__ v,
+                          const double* __restrict__ w, const double* __restrict__ Kh,
+                          const double* __restrict__ sub, const double* __restrict__ den,
+                          const double* __restrict__ cst, double* __restrict__ T_out,
+                          double* __restrict__ S_out, double* __restrict__ b_out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const bool ts = p.tracers_ts != 0;
+  const double* A = ts ? T : b;
+  double* Ao = ts ? T_out : b_out;
+  const double fa = ts ? p.flux_T : 0.0;
+  double da = 0.0, db = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = k * n2 + c;
+    const double ra = tracer_rhs(d, p, geo, A, u, v, w, Kh, fa, c, k, q, !ts);
+    da = (k == 0) ? ra / den[t] : (ra - sub[t] * da) / den[t];
+    Ao[t] = da;
+    if (ts) {
+      const double rb = tracer_rhs(d, p, geo, S, u, v, w, Kh, p.flux_S, c, k, q, false);
+      db = (k == 0) ? rb / den[t] : (rb - sub[t] * db) / den[t];
+      S_out[t] = db;
+    }
+  }
+  double xa = 0.0, xb = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long t = k * n2 + c;
+    const double mm = mk(geo.m3[t]);
+    xa = (k == nz - 1) ? Ao[t] : Ao[t] - cst[t] * xa;
+    Ao[t] = xa * mm;
+    if (ts) {
+      xb = (k == nz - 1) ? S_out[t] : S_out[t] - cst[t] * xb;
+      S_out[t] = xb * mm;
+      b_out[t] = buoyancy(p, xa * mm, xb * mm, geo.zc[t]) * mm;
+    }
+  }
+}
+
+// ---------------------------------------------- standalone operator kernels
+// Batched tridiagonal solve, one lane per column (libs/core/vertical.py:17-44);
+// x holds d* between the sweeps, cs holds c*.
+__global__ void k_thomas(long long ncol, int n, const double* __restrict__ sub,
+                         const double* __restrict__ diag, const double* __restrict__ sup,
+                         const double* __restrict__ rhs, double* __restrict__ x,
+                         double* __restrict__ cs) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  double cp = 0.0, dp = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const long long t = k * ncol + c;
+    double cc, dd;
+    if (k == 0) {
+      cc = sup[t] / diag[t];
+      dd = rhs[t] / diag[t];
+    } else {
+      const double den = diag[t] - sub[t] * cp;
+      cc = sup[t] / den;
+      dd = (rhs[t] - sub[t] * dp) / den;
+    }
+    cs[t] = cc;
+    x[t] = dd;
+    cp = cc;
+    dp = dd;
+  }
+  double xn = x[(long long)(n - 1) * ncol + c];
+  for (int k = n - 2; k >= 0; --k) {
+    const long long t = k * ncol + c;
+    xn = x[t] - cs[t] * xn;
+    x[t] = xn;
+  }
+}
+
+// Buoyancy from T, S at depth z for n points (libs/core/eos.py:130-133).
+__global__ void k_eos(long long n, StepParams p, const double* __restrict__ T,
+                      const double* __restrict__ S, const double* __restrict__ z,
+                      double* __restrict__ b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  b[t] = buoyancy(p, T[t], S[t], z[t]);
+}
+
+// Tracer advection divergence over the whole 3D field (the operator alone).
+__global__ void k_tracer_advection(Dims d, StepParams p, Geo geo, const double* __restrict__ cc,
+                                   const double* __restrict__ u, const double* __restrict__ v,
+                                   const double* __restrict__ w, double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const int k = (int)(t / d.n2);
+  const Nb q = neighbours(d, (int)(c / d.nx), (int)(c % d.nx));
+  out[t] = tracer_advection(d, p, geo, cc, u, v, w, c, k, q);
+}
+
+// Momentum advection (u, v) over the whole field (the operator alone).
+__global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
+                                     const double* __restrict__ v, const double* __restrict__ w,
+                                     double* __restrict__ au, double* __restrict__ av) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const int k = (int)(t / d.n2);
+  const Nb q = neighbours(d, (int)(c / d.nx), (int)(c % d.nx));
+  const long long o = (long long)k * d.n2;
+  au[t] = advect_u(d, p, geo.m3u, u, v, w, o, k, q);
+  av[t] = advect_v(d, p, geo.m3v, u, v, w, o, k, q);
+}
+
+}  // namespace hoc

@@ -1,0 +1,163 @@
+/* C-ABI of the MI355X ocean-core step (libhoc.so).
+ *
+ * The library replaces the hot path of the reference's NumPy core behind its
+ * step API; every entry point names the reference interface it stands in for.
+ * Conventions:
+ *   - Arrays are fp64 (masks uint8 0/1), C order: layers [nz][ny][nx],
+ *     interfaces [nz+1][ny][nx], columns [ny][nx]; i (x) is fastest.
+ *   - `hoc_geometry` arrays are HOST pointers, copied once at create.
+ *   - State, mixing and operator arrays are DEVICE pointers owned by the
+ *     caller (torch tensors in the Python binding).
+ *   - `stream` is a hipStream_t (NULL = the default stream). Calls are
+ *     asynchronous on it except where noted ("synchronises").
+ *   - Every entry point returns HOC_OK or an error code; hoc_last_error()
+ *     gives the message of the last failure on the calling thread. No C++
+ *     exception crosses the ABI.
+ *   - One handle per stream; a handle is not thread-safe.
+ */
+#ifndef HOC_H
+#define HOC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HOC_OK 0
+#define HOC_ERR_INVALID 1     /* invalid argument or configuration (ValueError) */
+#define HOC_ERR_UNSUPPORTED 2 /* a configuration this backend does not run     */
+#define HOC_ERR_DEVICE 3      /* HIP runtime failure (RuntimeError)            */
+
+/* Scheme, physics and solver settings. Field meanings are those of
+ * SchemeParams (libs/core/schemes.py:54-67), Physics3D (libs/core/model3d.py:57-81)
+ * and PhysicsV05 (libs/core/model3d_v05.py:40-72). */
+typedef struct hoc_config {
+  int32_t nx, ny, nz;
+  double dx, dy;
+  double dz_nominal;       /* grid.dz = physics.H / nz, used by momentum vertical advection */
+  double dt;
+  int32_t scheme;          /* 0 theta, 1 fb */
+  double theta, theta_cor, theta_v;
+  int32_t n_picard;
+  int32_t advection;       /* 0 none, 1 centered2, 2 upwind1, 3 up3, 4 up3_tvd */
+  int32_t solver;          /* 0 pcg_jacobi (others: HOC_ERR_UNSUPPORTED for now) */
+  double rtol;
+  int32_t max_iter;
+  double g, f, rho0, nu_b, bottom_drag, N2;
+  int32_t tracers_ts;      /* 1: T, S prognostic and b diagnosed; 0: b prognostic */
+  double alpha_T, beta_S, T0, S0, cp, A_h, K_h, q_heat, q_salt;
+  int32_t eos;             /* 0 linear, 1 seos, 2 teos10 */
+  int32_t pgf_remove, pgf_correction;
+  int32_t closure_tke, mxl_recursive;
+  double c_k, c_eps, pr_t, kappa_vk, z_0, e_min, n2_min, l_min, e_bb;
+} hoc_config;
+
+/* Static z-level geometry and surface forcing (host arrays). Built like
+ * Domain (libs/core/domain.py:24-195); the forcing fields are the array form
+ * of the reference's scalar tau_x, tau_y (model3d_v05.py:176,355-356) and
+ * kappa (model3d_v05.py:98, closure.py:143). */
+typedef struct hoc_geometry {
+  const uint8_t *mask, *mask_u, *mask_v;       /* [ny][nx] */
+  const uint8_t *mask3, *mask3u, *mask3v;      /* [nz][ny][nx] */
+  const double *dz3, *dz3u, *dz3v, *z_centre;  /* [nz][ny][nx] */
+  const double *tau_x, *tau_y;                 /* [ny][nx] wind stress, N m-2 */
+  const double *ustar2;                        /* [ny][nx] hypot(tau_x, tau_y) / rho0 */
+  const double *kappa_b;                       /* [ny][nx] background diffusivity */
+} hoc_geometry;
+
+/* Prognostic state (device). T and S may be NULL when tracers_ts == 0;
+ * e may be NULL when closure_tke == 0. Mirrors State3D (libs/core/model3d.py:34-47). */
+typedef struct hoc_state {
+  double *u, *v, *b, *eta, *T, *S, *e;
+} hoc_state;
+
+/* Stepper counters (libs/core/model3d_v05.py:89-94). */
+typedef struct hoc_counters {
+  int64_t solver_iterations, solver_failures, solver_rebuilds, tridiagonal_solves;
+  int64_t barotropic_substeps, eos_cells;
+} hoc_counters;
+
+typedef struct hoc_stepper hoc_stepper;
+
+/* Stepper3DV05.__init__ (libs/core/model3d_v05.py:81-125): validates, uploads
+ * the geometry and allocates the workspaces. */
+int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* geo, hoc_stepper** out);
+int hoc_stepper_destroy(hoc_stepper* h);
+
+/* The stepper's vertical viscosity / diffusivity on interfaces (the
+ * reference's self.nu / self.kappa, model3d_v05.py:97-98,179), as
+ * caller-owned device buffers [nz+1][ny][nx] that the TKE closure updates in
+ * place every step. Must be bound before the first build or step. */
+int hoc_stepper_bind_mixing(hoc_stepper* h, double* K_m, double* K_h);
+
+/* Stepper3DV05._build_coefficients (model3d_v05.py:141-167): tri_m, tri_b,
+ * q = A_m^-1 1, Ku, Kv and the Helmholtz diagonal from the bound K_m, K_h. */
+int hoc_stepper_build_coefficients(hoc_stepper* h, void* stream);
+
+/* Stepper3DV05.step (model3d_v05.py:337-415): reads `in`, writes `out`
+ * (distinct buffers). Synchronises on `stream` once per Helmholtz solve to
+ * poll convergence. */
+int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream);
+
+/* Counters accumulated since create (synchronises). */
+int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out);
+
+/* --- single operators, for parity tests and for callers of the reference's
+ *     module-level functions; all use the handle's geometry and settings --- */
+
+/* closure.tke_step (libs/core/closure.py:141-205) with K_m, K_h the bound
+ * buffers (old values in, new values out). e_out is [nz+1][ny][nx]. */
+int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, const double* v,
+                         const double* b, double* e_out, void* stream);
+
+/* advection_v06.momentum_advection_up3 / advection.momentum_advection
+ * (libs/core/advection_v06.py:77-108, libs/core/advection.py:62-95) per the
+ * configured scheme. w is [nz+1][ny][nx]. */
+int hoc_stepper_momentum_advection(hoc_stepper* h, const double* u, const double* v,
+                                   const double* w, double* adv_u, double* adv_v, void* stream);
+
+/* Stepper3DV05._tracer_advection (model3d_v05.py:482-512). */
+int hoc_stepper_tracer_advection(hoc_stepper* h, const double* c, const double* u,
+                                 const double* v, const double* w, double* out, void* stream);
+
+/* Stepper3DV05._pressure_gradients (model3d_v05.py:195-230). */
+int hoc_stepper_pressure_gradients(hoc_stepper* h, const double* b, double* pgx, double* pgy,
+                                   void* stream);
+
+/* Stepper3DV05._w_diag (model3d_v05.py:232-235). */
+int hoc_stepper_w_diag(hoc_stepper* h, const double* u, const double* v, double* w,
+                       void* stream);
+
+/* HelmholtzVar(..., kind=configured solver).solve (libs/core/solvers_var.py:162-197)
+ * on face coefficients Ku, Kv [ny][nx] (device). Writes x; iterations and
+ * converged are host outputs (synchronises). */
+int hoc_stepper_helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv,
+                                const double* rhs, double* x, int32_t* iterations,
+                                int32_t* converged, void* stream);
+
+/* vertical.thomas_batched (libs/core/vertical.py:17-44): n rows per column,
+ * ncol columns, arrays [n][ncol]; scratch is n*ncol doubles (device). */
+int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
+                       const double* rhs, double* x, double* scratch, int64_t n, int64_t ncol,
+                       void* stream);
+
+/* eos.buoyancy (libs/core/eos.py:130-133) for n points; eos 0/1/2 = linear,
+ * seos, teos10; g, rho0, alpha_T, beta_S, T0, S0 as in Physics3D. */
+int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double beta_S,
+                     double T0, double S0, const double* T, const double* S, const double* z,
+                     double* b, int64_t n, void* stream);
+
+/* numpy.sum(a * b) for float64, bit for bit (sequential over 8192-element
+ * chunks, pairwise inside each). out is one device double; scratch holds
+ * ceil(n / 8192) doubles. */
+int hoc_pairwise_dot(const double* a, const double* b, int64_t n, double* out, double* scratch,
+                     void* stream);
+
+const char* hoc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOC_H */

@@ -1,0 +1,223 @@
+"""HIP path against the reference's golden fixtures and the oracle, through
+the C-ABI. Bitwise agreement is the expectation (the kernels keep NumPy's
+operation order, no FMA contraction, and NumPy's summation order in the
+solver's dot products); the gates below are the reference's own: relative L2
+< 1e-12 for one step, < 1e-9 for a full run, solver iteration counts equal."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_l2
+from golden_cases import COMPOSED, build_case, lock_exchange_problem
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def hoc(gpu_available):
+    from hydrostatic_ocean_core_benchmark_amd import stepper
+    return stepper
+
+
+def test_pairwise_dot_matches_numpy_sum(hoc):
+    rng = np.random.default_rng(3)
+    for n in (1, 5, 8, 37, 128, 129, 384, 512, 1000, 8192, 8193, 36864, 90000, 262144):
+        a = rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, size=n)
+        b = rng.standard_normal(n)
+        got = hoc.pairwise_dot(torch.as_tensor(a, device="cuda"), torch.as_tensor(b, device="cuda"))
+        assert got == float(np.sum(a * b)), n
+
+
+def test_thomas_matches_oracle(hoc):
+    from oracle.column import thomas
+    rng = np.random.default_rng(4)
+    for n, ncol in ((1, 7), (2, 5), (20, 512), (61, 300)):
+        sub = rng.standard_normal((n, ncol))
+        sup = rng.standard_normal((n, ncol))
+        diag = 4.0 + np.abs(sub) + np.abs(sup) + rng.random((n, ncol))
+        rhs = rng.standard_normal((n, ncol))
+        want = thomas(sub, diag, sup, rhs)
+        got = hoc.thomas_batched(*(torch.as_tensor(a, device="cuda") for a in (sub, diag, sup, rhs)))
+        assert np.array_equal(_np(got), want), (n, ncol)
+
+
+@pytest.mark.parametrize("kind", ["linear", "seos", "teos10"])
+def test_eos_matches_oracle(hoc, kind):
+    from oracle.eos import buoyancy
+    rng = np.random.default_rng(5)
+    T = 30.0 * rng.random(4099) - 2.0
+    S = 30.0 + 8.0 * rng.random(4099)
+    z = 5000.0 * rng.random(4099)
+    kw = dict(g=9.80616, rho0=1025.0, alpha_T=2e-4, beta_S=7.4e-4, T0=10.0, S0=35.0)
+    want = buoyancy(kind, T, S, z, **kw)
+    got = hoc.eos_buoyancy(kind, *(torch.as_tensor(a, device="cuda") for a in (T, S, z)), **kw)
+    assert np.array_equal(_np(got), want)
+
+
+def test_teos10_check_value(hoc):
+    """rho(SA=30, CT=10, z=1000) = 1027.45140 (Roquet et al. 2015), and the
+    reference's computed value 1027.4513640377502
+    (expr/E07_v05_physics/data/v05_verification_20260923.json, V5-4)."""
+    t = lambda x: torch.tensor([x], dtype=torch.float64, device="cuda")  # noqa: E731
+    # b = -g (rho - rho0)/rho0; with g = -1, rho0 = 1: b = rho - 1
+    b = hoc.eos_buoyancy("teos10", t(10.0), t(30.0), t(1000.0), g=-1.0, rho0=1.0, alpha_T=0.0,
+                         beta_S=0.0, T0=0.0, S0=0.0)
+    rho = float(b.item()) + 1.0
+    assert abs(rho - 1027.45140) < 1e-4
+    assert abs(rho - 1027.4513640377502) < 1e-9
+
+
+def _stepper_for(hoc, name, golden, solver=None):
+    dom, phys, params, dt, steps, s0 = build_case(name, golden)
+    if solver is not None:
+        params.solver = solver
+    return hoc.HipStepper(dom, phys, params, dt), dom, phys, params, dt, steps, s0
+
+
+@pytest.mark.parametrize("mxl", ["integral", "recursive"])
+def test_tke_step_matches_reference(hoc, mxl):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    f = load_golden("operators.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=10, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0, nu=1e-4, kappa=1e-5),
+                      closure="tke", mxl=mxl)
+    st = hoc.HipStepper(dom, phys, SchemeParams(name="fb"), 300.0, ustar2=f["tke_ustar2"])
+    st.nu.copy_(torch.as_tensor(f["tke_Km"], device="cuda"))
+    st.kappa.copy_(torch.as_tensor(f["tke_Kh"], device="cuda"))
+    e = st.tke_step(f["tke_e"], f["tke_u"], f["tke_v"], f["tke_b"])
+    assert np.array_equal(_np(e), f[f"tke_{mxl}_e"])
+    assert np.array_equal(_np(st.nu), f[f"tke_{mxl}_Km"])
+    assert np.array_equal(_np(st.kappa), f[f"tke_{mxl}_Kh"])
+
+
+@pytest.mark.parametrize("limited", [False, True])
+def test_momentum_advection_walls_matches_reference(hoc, limited):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    f = load_golden("operators.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=10, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0))
+    st = hoc.HipStepper(dom, phys, SchemeParams(name="fb", advection="up3_tvd" if limited
+                                                else "up3"), 10.0)
+    au, av = st.momentum_advection(f["adv_u"], f["adv_v"], f["adv_w"])
+    tag = "tvd" if limited else "up3"
+    assert np.array_equal(_np(au), f[f"adv_{tag}_u"])
+    assert np.array_equal(_np(av), f[f"adv_{tag}_v"])
+
+
+@pytest.mark.parametrize("pgf", ["keep", "remove"])
+def test_pressure_gradient_matches_reference(hoc, pgf):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    f = load_golden("operators.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=10, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0), pgf=pgf)
+    st = hoc.HipStepper(dom, phys, SchemeParams(name="fb"), 10.0)
+    px, py = st.pressure_gradients(f["pg_b"])
+    assert np.array_equal(_np(px), f[f"pg_{pgf}_x"])
+    assert np.array_equal(_np(py), f[f"pg_{pgf}_y"])
+
+
+@pytest.mark.parametrize("scheme", ["centered2", "upwind1", "up3", "up3_tvd"])
+def test_tracer_advection_matches_oracle(hoc, scheme):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    from oracle.stepper import OracleStepper
+    f = load_golden("operators.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=10, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0))
+    params = SchemeParams(name="fb", advection=scheme)
+    rng = np.random.default_rng(6)
+    c = (10.0 + rng.standard_normal(grid.shape3d)) * dom.mask3
+    want = OracleStepper(dom, phys, params, 10.0).tracer_advection(c, f["adv_u"], f["adv_v"],
+                                                                   f["adv_w"])
+    got = hoc.HipStepper(dom, phys, params, 10.0).tracer_advection(c, f["adv_u"], f["adv_v"],
+                                                                    f["adv_w"])
+    assert np.array_equal(_np(got), want)
+
+
+def test_helmholtz_pcg_matches_reference(hoc):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    f = load_golden("operators.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=10, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
+    # coef = g theta^2 dt^2 with theta = 0.5, dt = 600 (the fixture's coef)
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0))
+    params = SchemeParams(name="theta", solver="pcg_jacobi", rtol=1e-11, max_iter=5000)
+    st = hoc.HipStepper(dom, phys, params, 600.0)
+    assert st.cfg.g * 0.25 * 600.0 ** 2 == float(f["helm_coef"])
+    x, iters, conv = st.helmholtz_solve(f["helm_Ku"], f["helm_Kv"], f["helm_rhs"])
+    assert conv and iters == int(f["helm_pcg_jacobi_iters"])
+    assert np.array_equal(_np(x), f["helm_pcg_jacobi_x"])
+
+
+FIELDS = ("u", "v", "b", "eta", "T", "S", "e")
+
+
+@pytest.mark.parametrize("name", list(COMPOSED))
+def test_composed_step_matches_reference(hoc, name):
+    """1-3 steps of the reference stepper; the solver is switched to
+    pcg_jacobi where the fixture used another, then the oracle is the check."""
+    f = load_golden(f"step_{name}.npz")
+    dom, phys, params, dt, steps, s0 = build_case(name, f)
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    use_ref = params.name == "fb" or params.solver == "pcg_jacobi"
+    if not use_ref:
+        params.solver = "pcg_jacobi"
+        from oracle.stepper import OracleStepper, OState
+        ost = OracleStepper(dom, phys, params, dt)
+        os_ = OState(**s0, t=0.0)
+    st = hoc.HipStepper(dom, phys, params, dt)
+    s = State3D(**s0, t=0.0)
+    for n in range(1, steps + 1):
+        s = st.step(s)
+        if use_ref:
+            want = {k: f[f"s{n}_{k}"] for k in FIELDS if f"s{n}_{k}" in f}
+            want_iters = int(f[f"s{n}_iters"])
+        else:
+            os_ = ost.step(os_)
+            want = {k: getattr(os_, k) for k in FIELDS if getattr(os_, k) is not None}
+            want_iters = ost.solver_iterations
+        for k, w in want.items():
+            got = _np(getattr(s, k))
+            assert rel_l2(got, w) < 1e-12, (n, k, rel_l2(got, w))
+            assert np.array_equal(got, w), (n, k, "not bitwise", rel_l2(got, w))
+        assert st.solver_iterations == want_iters, n
+
+
+def test_lock_exchange_200_steps_matches_reference(hoc):
+    """The 128x4x20 lock exchange for 200 steps against the reference
+    (full-run gate 1e-9; iteration counts step by step)."""
+    f = load_golden("lock_exchange_128x4x20.npz")
+    dom, phys, params, dt = lock_exchange_problem()
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    st = hoc.HipStepper(dom, phys, params, dt)
+    g = dom.grid
+    s = State3D(u=np.zeros(g.shape3d), v=np.zeros(g.shape3d), b=f["in_b"],
+                eta=np.zeros(g.shape), T=f["in_T"], S=f["in_S"])
+    iters = []
+    for n in range(1, 201):
+        before = st.solver_iterations
+        s = st.step(s)
+        iters.append(st.solver_iterations - before)
+        if n in (1, 10, 200):
+            tol = 1e-12 if n == 1 else 1e-9
+            for k in FIELDS:
+                got = _np(getattr(s, k))
+                assert rel_l2(got, f[f"s{n}_{k}"]) < tol, (n, k)
+    assert iters == [int(x) for x in f["iters_per_step"]]
+    assert np.array_equal(_np(s.T), f["s200_T"])

@@ -1,0 +1,178 @@
+"""Benchmark: FP64 cell-steps/s of the ocean-core time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel NAME]
+
+Workload: the closed basin at 512 x 512 x 40 (dx = 10 km, dt = 600 s),
+Gaussian-random-field bathymetry (correlation 200 km, 500-4000 m,
+z-level partial cells), T(z) = 2 + 18 exp(z/800) + N(0, 0.05), S with a weak
+linear gradient, tau_x = 0.1 cos(pi y / Ly), polyTEOS10, up3_tvd, TKE
+(integral mixing length), theta = 0.5 with a PCG-Jacobi barotropic solve at
+rtol 1e-12. Inputs are resident in HBM before the timed region; W untimed
+steps, then exactly K timed steps bracketed by device synchronisation.
+A "step" is one full model time step. cells = nx*ny*nz (the reference's
+convention, dry cells included).
+
+The JSON line also carries
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average launch time (HIP events on the launch stream, inside
+                the timed region), against 8 TB/s;
+  cpu_baseline  the NumPy oracle of the same step on one host core, timed on
+                a bounded sample (the 128 x 128 x 40 basin from the same
+                generator: 1 warm-up step, then 3 timed steps).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cell-steps/s FP64 plain vs mixed at stated L2 error; adjoint:forward ratio"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) HBM bytes of one launch per layer cell of each
+# kernel: every array it must read or write touched once, fp64 = 8 B, masks
+# 1 B; interface arrays counted per layer cell (the (nz+1)/nz excess is
+# ignored). Stage figures follow the stage model of the step:
+#   k_tracers      T, S, u, v, w, K_h, dz3 read; T, S, b written     (72 B)
+#   k_closure      e, K_m, K_h, u, v, b, dz3 read; e, K_m, K_h written (80 B)
+#   k_predictor    u, v, (u, v)_iter, 4 explicit terms x 2, K_m, dz3u, dz3v
+#                  read; gh_u, gh_v written                        (128 B)
+#   k_momentum_explicit  u, v, b, w, K_m read; 8 explicit terms written (104 B)
+ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 128.0,
+                      "k_momentum_explicit": 104.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--kernel", default="k_tracers",
+                    help="kernel timed with HIP events for the roofline figure")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(nz: int) -> dict:
+    """The oracle on one core: 128 x 128 x nz basin, 1 warm-up + 3 timed steps."""
+    import numpy as np
+
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    from oracle.stepper import OracleStepper, OState
+    p = basin(nx=128, nz=nz)
+    st = OracleStepper(p.domain, p.physics, p.params, p.dt)
+    g = p.domain.grid
+    b = st.buoyancy_of(p.initial_T, p.initial_S) * p.domain.mask3
+    s = OState(u=np.zeros(g.shape3d), v=np.zeros(g.shape3d), b=b, eta=np.zeros(g.shape),
+               T=p.initial_T, S=p.initial_S)
+    s = st.step(s)
+    n = 3
+    t0 = time.perf_counter()
+    for _ in range(n):
+        s = st.step(s)
+    wall = time.perf_counter() - t0
+    cells = g.nx * g.ny * g.nz
+    return {"value": cells * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy oracle, 128x128x{nz} basin from the same generator, 3 steps "
+                      f"after 1 warm-up step ({wall:.1f} s; {platform.processor() or 'cpu'}, "
+                      f"numpy {np.__version__}, OMP_NUM_THREADS="
+                      f"{os.environ.get('OMP_NUM_THREADS', 'unset')})"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU slab decomposition is design-only)"}))
+        return 2
+    import numpy as np
+    import torch
+
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        return 2
+    prob = basin(nx=args.nx, nz=args.nz)
+    g = prob.domain.grid
+    st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+    T = torch.as_tensor(prob.initial_T, device="cuda")
+    S = torch.as_tensor(prob.initial_S, device="cuda")
+    mask3 = torch.as_tensor(prob.domain.mask3, device="cuda")
+    b = st.buoyancy_of(T, S) * mask3
+    zero3 = torch.zeros(g.shape3d, dtype=torch.float64, device="cuda")
+    cur = {"u": zero3.clone(), "v": zero3.clone(), "b": b, "T": T, "S": S,
+           "eta": torch.zeros(g.shape, dtype=torch.float64, device="cuda"),
+           "e": torch.full((g.nz + 1,) + g.shape, float(prob.physics.e_min),
+                           dtype=torch.float64, device="cuda")}
+    bufs = [st._new_state_buffers(), st._new_state_buffers()]
+    n = 0
+
+    def one_step():
+        nonlocal cur, n
+        out = bufs[n % 2]
+        st._step_into(cur, out)
+        cur = out
+        n += 1
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    it0 = st.solver_iterations
+    st.profile_kernel(args.kernel)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launches, kms = st.profile_read()
+    st.profile_kernel(None)
+    iters = st.solver_iterations - it0
+    fails = st.solver_failures
+    finite = bool(torch.isfinite(cur["eta"]).all() and torch.isfinite(cur["T"]).all())
+
+    cells = g.nx * g.ny * g.nz
+    value = cells * args.steps / wall
+    roofline = None
+    if launches > 0 and args.kernel in ALG_BYTES_PER_CELL:
+        avg_s = kms / launches / 1e3
+        alg = ALG_BYTES_PER_CELL[args.kernel] * cells
+        achieved = alg / avg_s / 1e9
+        roofline = {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None, "alg_bytes_per_launch": alg,
+                    "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
+    rec = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"fp64 plain step, closed GRF basin {g.nx}x{g.ny}x{g.nz}, "
+                               "dx 10 km, dt 600 s, up3_tvd, TKE, polyTEOS10, theta 0.5 "
+                               "PCG-Jacobi rtol 1e-12",
+                   "cells": cells, "wet_fraction": round(float(np.mean(prob.domain.mask3)), 4),
+                   "solver": prob.params.solver,
+                   "solver_iterations_per_step": iters / args.steps,
+                   "solver_failures": fails, "state_finite": finite},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(args.nz)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -158,6 +158,12 @@ struct hoc_stepper {
   bool coeffs_built = false;
   // host counters
   long long solver_rebuilds = 0, tridiagonal_solves = 0, eos_cells = 0;
+  // per-kernel event timing (one kernel name at a time)
+  std::string prof_name;
+  std::vector<hipEvent_t> prof_ev;  // start/stop pairs
+  int prof_used = 0;
+  long long prof_launches = 0;
+  double prof_ms = 0.0;
 
   template <class T>
   int alloc(T** ptr, long long count) {
@@ -168,6 +174,7 @@ struct hoc_stepper {
     return HOC_OK;
   }
   ~hoc_stepper() {
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void* v : allocs) (void)hipFree(v);
     if (host_flag) (void)hipHostFree(host_flag);
     red.release();
@@ -276,14 +283,47 @@ int upload(T* dst, const T* src, long long n, const char* name) {
     if (_rc != HOC_OK) return _rc; \
   } while (0)
 
+// Fold the recorded event pairs into the running total.
+int prof_drain(hoc_stepper* h) {
+  for (int i = 0; i < h->prof_used; ++i) {
+    HIP_TRY(hipEventSynchronize(h->prof_ev[2 * i + 1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]));
+    h->prof_ms += ms;
+  }
+  h->prof_used = 0;
+  return HOC_OK;
+}
+
+// Event pair around one launch of `name` when that kernel is being timed.
+struct ProfScope {
+  hoc_stepper* h;
+  hipStream_t s;
+  bool on;
+  ProfScope(hoc_stepper* h_, const char* name, hipStream_t s_) : h(h_), s(s_) {
+    on = !h->prof_name.empty() && h->prof_name == name;
+    if (!on) return;
+    if (2 * (h->prof_used + 1) > (int)h->prof_ev.size()) (void)prof_drain(h);
+    (void)hipEventRecord(h->prof_ev[2 * h->prof_used], s);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(h->prof_ev[2 * h->prof_used + 1], s);
+    h->prof_used += 1;
+    h->prof_launches += 1;
+  }
+};
+
 int build_coefficients(hoc_stepper* h, hipStream_t s) {
   if (!h->Km || !h->Kh) return fail(HOC_ERR_INVALID, "bind K_m / K_h before building");
   const Dims d = h->d;
+  { ProfScope _ps(h, "k_coefficients", s);
   k_coefficients<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->Km, h->Kh, h->sub_m,
                                               h->den_m, h->cst_m, h->sub_b, h->den_b,
-                                              h->cst_b, h->q);
+                                              h->cst_b, h->q); }
   LAUNCH_CHECK("k_coefficients");
-  k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, h->q, h->Ku, h->Kv);
+  { ProfScope _ps(h, "k_kukv", s);
+  k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, h->q, h->Ku, h->Kv); }
   LAUNCH_CHECK("k_kukv");
   h->tridiagonal_solves += 1;
   if (h->cfg.scheme == 0) {
@@ -320,9 +360,11 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   int batch = std::max(1, std::min(h->last_iters - 1, max_iter));
   while (true) {
     for (int b = 0; b < batch; ++b) {
-      k_pcg_ap<<<nch, 512, 0, s>>>(d, op, vv, R.plans, R.nfull, h->sc);
+      { ProfScope _ps(h, "k_pcg_ap", s);
+        k_pcg_ap<<<nch, 512, 0, s>>>(d, op, vv, R.plans, R.nfull, h->sc); }
       k_pcg_alpha<<<1, 1, 0, s>>>(vv, R.nchunks, h->sc);
-      k_pcg_update<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc);
+      { ProfScope _ps(h, "k_pcg_update", s);
+        k_pcg_update<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc); }
       k_pcg_check<<<1, 1, 0, s>>>(vv, R.nchunks, max_iter, h->sc, h->dcount);
       k_pcg_p<<<blocks(d.n2), TPB, 0, s>>>(d, vv, h->sc);
     }
@@ -513,8 +555,9 @@ int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, co
     k_closure_n2<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, u, v, b, h->s_n2);
     k_mixing_recursive<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->s_n2, h->s_l);
   }
+  { ProfScope _ps(h, "k_closure", s);
   k_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, u, v, b, h->Km, h->Kh, e_out,
-                                         h->s_n2, h->s_sh, h->s_l, h->s_c);
+                                         h->s_n2, h->s_sh, h->s_l, h->s_c); }
   LAUNCH_CHECK("k_closure");
   return HOC_OK;
 }
@@ -542,11 +585,14 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   } else if (in->e && out->e && in->e != out->e) {
     HIP_TRY(hipMemcpyAsync(out->e, in->e, d.ni * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->b, h->phi);
-  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w);
+  { ProfScope _ps(h, "k_phi", s);
+  k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->b, h->phi); }
+  { ProfScope _ps(h, "k_w_diag", s);
+  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
+  { ProfScope _ps(h, "k_momentum_explicit", s);
   k_momentum_explicit<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->b, h->phi, h->w,
                                                    h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu,
-                                                   h->dzv, h->stu, h->stv);
+                                                   h->dzv, h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
   if (theta) {
     k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
@@ -557,10 +603,11 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   for (int it = 0; it < h->cfg.n_picard; ++it) {
     double* gu = h->gh_u[it & 1];
     double* gv = h->gh_v[it & 1];
-    k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
+    { ProfScope _ps(h, "k_predictor", s);
+  k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
                                              h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
                                              h->stu, h->stv, h->sub_m, h->den_m, h->cst_m, gu, gv,
-                                             h->Ug, h->Vg);
+                                             h->Ug, h->Vg); }
     LAUNCH_CHECK("k_predictor");
     h->tridiagonal_solves += 2;
     if (theta) {
@@ -568,7 +615,8 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       LAUNCH_CHECK("k_rhs_eta");
       int iters = 0, conv = 0;
       TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
-      k_backsub<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u, out->v);
+      { ProfScope _ps(h, "k_backsub", s);
+  k_backsub<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u, out->v); }
       LAUNCH_CHECK("k_backsub");
       u_it = out->u;
       v_it = out->v;
@@ -584,10 +632,12 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     k_eta_fb<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->U0, h->V0, out->eta);
     LAUNCH_CHECK("k_eta_fb");
   }
-  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w);
+  { ProfScope _ps(h, "k_w_diag", s);
+  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
+  { ProfScope _ps(h, "k_tracers", s);
   k_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->T, in->S, in->b, out->u, out->v, h->w,
                                          h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S,
-                                         out->b);
+                                         out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;
@@ -605,6 +655,29 @@ int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out) {
   out->tridiagonal_solves = h->tridiagonal_solves;
   out->barotropic_substeps = 0;
   out->eos_cells = h->eos_cells;
+  return HOC_OK;
+}
+
+int hoc_stepper_profile_kernel(hoc_stepper* h, const char* name) {
+  if (!h) return fail(HOC_ERR_INVALID, "null handle");
+  TRY(prof_drain(h));
+  h->prof_name = name ? name : "";
+  h->prof_launches = 0;
+  h->prof_ms = 0.0;
+  if (!h->prof_name.empty() && h->prof_ev.empty()) {
+    h->prof_ev.resize(2 * 512);
+    for (auto& e : h->prof_ev) HIP_TRY(hipEventCreate(&e));
+  }
+  return HOC_OK;
+}
+
+int hoc_stepper_profile_read(hoc_stepper* h, int64_t* launches, double* total_ms) {
+  if (!h || !launches || !total_ms) return fail(HOC_ERR_INVALID, "null argument");
+  TRY(prof_drain(h));
+  *launches = h->prof_launches;
+  *total_ms = h->prof_ms;
+  h->prof_launches = 0;
+  h->prof_ms = 0.0;
   return HOC_OK;
 }
 

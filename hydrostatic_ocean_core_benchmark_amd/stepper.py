@@ -127,8 +127,9 @@ class HipStepper:
 
         opts = dict(dtype=torch.float64, device=self.device)
         self.nu = torch.full((nz + 1, ny, nx), float(p.nu), **opts)
-        self.kappa = torch.as_tensor(np.broadcast_to(self._host["kappa_b"], (nz + 1, ny, nx)),
-                                     **opts).contiguous()
+        self.kappa = torch.as_tensor(
+            np.ascontiguousarray(np.broadcast_to(self._host["kappa_b"], (nz + 1, ny, nx))),
+            **opts)
         self.z_centre = torch.as_tensor(self._host["z_centre"], **opts)
         _lib.check(self.lib.hoc_stepper_bind_mixing(self._h, _lib.ptr(self.nu),
                                                     _lib.ptr(self.kappa)))
@@ -283,6 +284,17 @@ class HipStepper:
         c = self._counters()
         return {k: int(getattr(c, k)) for k, _ in _lib.HocCounters._fields_} | {
             "eos_cells": int(c.eos_cells) + self._eos_host}
+
+    # --------------------------------------------------------- measurement
+    def profile_kernel(self, name: str | None) -> None:
+        """Time every launch of one step kernel with HIP events on its stream."""
+        _lib.check(self.lib.hoc_stepper_profile_kernel(self._h, (name or "").encode()))
+
+    def profile_read(self) -> tuple[int, float]:
+        """(launches, total ms) of the timed kernel since the last read."""
+        n, ms = C.c_int64(), C.c_double()
+        _lib.check(self.lib.hoc_stepper_profile_read(self._h, C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
 
     # ----------------------------------------------------- single operators
     def tke_step(self, e, u, v, b):

@@ -104,6 +104,15 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
 /* Counters accumulated since create (synchronises). */
 int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out);
 
+/* Time every launch of one step kernel (e.g. "k_tracers") with HIP events
+ * recorded on the launch stream; name NULL or "" stops timing. No reference
+ * counterpart: measurement support for the roofline figures. */
+int hoc_stepper_profile_kernel(hoc_stepper* h, const char* name);
+
+/* Launches and total milliseconds of the timed kernel since the last read
+ * (synchronises). */
+int hoc_stepper_profile_read(hoc_stepper* h, int64_t* launches, double* total_ms);
+
 /* --- single operators, for parity tests and for callers of the reference's
  *     module-level functions; all use the handle's geometry and settings --- */
 

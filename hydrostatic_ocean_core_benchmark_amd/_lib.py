@@ -72,6 +72,8 @@ PROTOTYPES = {
     "hoc_thomas_batched": ([VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_eos_buoyancy": ([i32, f64, f64, f64, f64, f64, f64, VP, VP, VP, VP, i64, VP], i32),
     "hoc_pairwise_dot": ([VP, VP, i64, VP, VP, VP], i32),
+    "hoc_stepper_set_solver_path": ([VP, i32], i32),
+    "hoc_stepper_solver_path": ([VP], i32),
     "hoc_stepper_profile_kernel": ([VP, C.c_char_p], i32),
     "hoc_stepper_profile_read": ([VP, C.POINTER(i64), C.POINTER(f64)], i32),
     "hoc_last_error": ([], C.c_char_p),

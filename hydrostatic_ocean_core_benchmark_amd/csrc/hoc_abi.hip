@@ -149,12 +149,16 @@ struct hoc_stepper {
   double *gh_u[2], *gh_v[2];
   double *s_n2, *s_sh, *s_l, *s_c;
   // Helmholtz
-  double *r, *z, *pp, *Ap, *part_a, *part_b;
+  double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
+  unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
+  int* bar_err = nullptr;
+  int pcg_sub = 0, pcg_parts = 0;  // persistent-solver partition (0: not usable)
   PcgScalars* sc = nullptr;
   long long* dcount = nullptr;  // [0] solver iterations, [1] failures
   int* host_flag = nullptr;     // pinned: done, iters
   Reducer red;
   int last_iters = 8;
+  bool force_multikernel = false;  // test hook: use the multi-launch PCG
   bool coeffs_built = false;
   // host counters
   long long solver_rebuilds = 0, tridiagonal_solves = 0, eos_cells = 0;
@@ -337,6 +341,56 @@ int build_coefficients(hoc_stepper* h, hipStream_t s) {
   return HOC_OK;
 }
 
+// PCG-Jacobi in one cooperative launch (k_pcg_persistent).
+int helmholtz_solve_persistent(hoc_stepper* h, const HelmOp& op, const double* inv_diag,
+                               const double* rhs, double* x, hipStream_t s, int* iters_out,
+                               int* conv_out) {
+  PcgPersistArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.d = h->d;
+  a.op = op;
+  a.rhs = rhs;
+  a.inv_diag = inv_diag;
+  a.x = x;
+  a.r = h->r;
+  a.z = h->z;
+  a.pb0 = h->pp;
+  a.pb1 = h->pp2;
+  a.Ap = h->Ap;
+  a.part_a = h->part_a;
+  a.part_b = h->part_b;
+  a.part_c = h->part_c;
+  a.plans = h->red.plans;
+  a.nfull = h->red.nfull;
+  a.sub = h->pcg_sub;
+  a.tail = (int)(h->red.nchunks - h->red.nfull);
+  a.nparts = h->pcg_parts;
+  a.nchunks = (int)h->red.nchunks;
+  a.rtol = h->p.rtol;
+  a.max_iter = h->p.max_iter;
+  a.bar = h->bar;
+  a.err = h->bar_err;
+  a.sc = h->sc;
+  a.counters = h->dcount;
+  void* args[] = {&a};
+  HIP_TRY(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(h->pcg_parts),
+                                     dim3(PCG_WG), args, 0, s));
+  int tail[4];
+  HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->it, sizeof(tail), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h->host_flag + 4, h->bar_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::memcpy(tail, h->host_flag, sizeof(tail));
+  if (h->host_flag[4]) {
+    (void)hipMemset(h->bar_err, 0, sizeof(int));
+    (void)hipMemset(h->bar, 0, 2 * sizeof(unsigned));
+    return fail(HOC_ERR_DEVICE, "persistent Helmholtz solve: grid barrier timed out");
+  }
+  if (iters_out) *iters_out = tail[1];
+  if (conv_out) *conv_out = tail[3];
+  if (tail[1] > 0) h->last_iters = tail[1];
+  return HOC_OK;
+}
+
 // PCG-Jacobi on A = I - coef D[K G .]; x is the output (device).
 int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const double* inv_diag,
                     const double* rhs, double* x, hipStream_t s, int* iters_out,
@@ -344,6 +398,8 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   const Dims d = h->d;
   const Reducer& R = h->red;
   HelmOp op{Ku, Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2, h->p.dy2};
+  if (h->pcg_parts > 0 && !h->force_multikernel)
+    return helmholtz_solve_persistent(h, op, inv_diag, rhs, x, s, iters_out, conv_out);
   PcgVec vv{x, h->r, h->z, h->pp, h->Ap, inv_diag, h->part_a, h->part_b};
   const unsigned nch = (unsigned)R.nchunks;
   if (nch == 0) return fail(HOC_ERR_INVALID, "empty grid");
@@ -469,7 +525,10 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   A(h->r, d.n2);
   A(h->z, d.n2);
   A(h->pp, d.n2);
+  A(h->pp2, d.n2);
   A(h->Ap, d.n2);
+  A(h->bar, 2);
+  A(h->bar_err, 1);
   A(h->sc, 1);
   A(h->dcount, 4);
 #undef A
@@ -483,7 +542,35 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     if (rc != HOC_OK) return cleanup(rc);
     h->part_a = pa;
     h->part_b = pb;
-    if (hipHostMalloc((void**)&h->host_flag, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    // persistent solver: split each full chunk into `sub` subtrees so that
+    // about 256-512 single-wave workgroups share the work
+    const long long nfull = h->red.nfull;
+    int sub = 8;
+    while (sub > 1 && nfull * sub > 512) sub >>= 1;
+    const long long parts = nfull * sub + (h->red.nchunks - nfull);
+    int dev = 0, ncu = 0, coop = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pcg_persistent,
+                                                     PCG_WG, 0) == hipSuccess &&
+        coop && h->red.nchunks <= PCG_MAX_CHUNKS && parts <= (long long)ncu * per_cu) {
+      h->pcg_sub = sub;
+      h->pcg_parts = (int)parts;
+    }
+    double* pc = nullptr;
+    rc = h->alloc(&pc, std::max<long long>(parts, h->red.nchunks));
+    if (rc != HOC_OK) return cleanup(rc);
+    h->part_c = pc;
+    if (parts > h->red.nchunks) {  // part_a / part_b must hold one value per part
+      double *pa2 = nullptr, *pb2 = nullptr;
+      rc = h->alloc(&pa2, parts);
+      if (rc == HOC_OK) rc = h->alloc(&pb2, parts);
+      if (rc != HOC_OK) return cleanup(rc);
+      h->part_a = pa2;
+      h->part_b = pb2;
+    }
+    if (hipHostMalloc((void**)&h->host_flag, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       return cleanup(fail(HOC_ERR_DEVICE, "hipHostMalloc failed"));
   }
   int rc = HOC_OK;
@@ -502,6 +589,9 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   rc = rc ? rc : upload(h->ustar2, g->ustar2, d.n2, "ustar2");
   rc = rc ? rc : upload(h->kappa_b, g->kappa_b, d.n2, "kappa_b");
   if (rc != HOC_OK) return cleanup(rc);
+  if (hipMemset(h->bar, 0, 2 * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(h->bar_err, 0, sizeof(int)) != hipSuccess)
+    return cleanup(fail(HOC_ERR_DEVICE, "hipMemset failed"));
   if (hipMemset(h->dcount, 0, 4 * sizeof(long long)) != hipSuccess)
     return cleanup(fail(HOC_ERR_DEVICE, "hipMemset failed"));
   k_geometry<<<blocks(d.n2), TPB>>>(d, h->p, h->dz3, h->m3u, h->m3v, h->tau_x, h->tau_y, h->dzi,
@@ -656,6 +746,18 @@ int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out) {
   out->barotropic_substeps = 0;
   out->eos_cells = h->eos_cells;
   return HOC_OK;
+}
+
+int hoc_stepper_set_solver_path(hoc_stepper* h, int32_t path) {
+  if (!h) return fail(HOC_ERR_INVALID, "null handle");
+  if (path != 0 && path != 1) return fail(HOC_ERR_INVALID, "path must be 0 or 1");
+  h->force_multikernel = path == 1;
+  return HOC_OK;
+}
+
+int32_t hoc_stepper_solver_path(hoc_stepper* h) {
+  if (!h) return -1;
+  return (h->pcg_parts > 0 && !h->force_multikernel) ? 0 : 1;
 }
 
 int hoc_stepper_profile_kernel(hoc_stepper* h, const char* name) {

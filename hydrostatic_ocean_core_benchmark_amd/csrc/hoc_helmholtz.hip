@@ -288,3 +288,297 @@ __global__ void k_seq_sum(const double* __restrict__ part, long long nchunks, do
 }
 
 }  // namespace hoc
+
+// ====================================================== persistent PCG
+namespace hoc {
+
+constexpr long long SPIN_LIMIT = 1ll << 22;
+
+// Grid barrier over all workgroups of the cooperative launch. Bounded: a
+// wait longer than SPIN_LIMIT polls raises the error flag and every
+// workgroup leaves, so the kernel always finishes. Returns false on error.
+__device__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    if (prev == nblocks - 1) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long long spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > SPIN_LIMIT) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        if ((spins & 255) == 0 &&
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+// This workgroup's part: element range and the pairwise sum(s) over it in
+// NumPy's order. Full-chunk parts are perfect subtrees (leaves of 128);
+// the trailing chunk follows its plan. `elem(e)` may store per-element
+// results; it is called once per index of the part.
+template <int NS, class F>
+__device__ Two part_sum(const PcgPersistArgs& a, int part, F elem, double* lds) {
+  const int tid = threadIdx.x;
+  const int g = tid >> 3, lane = tid & 7;
+  const bool full = part < a.nfull * a.sub;
+  const PwPlan* plan = full ? &a.plans[0] : &a.plans[1];
+  long long base;
+  int leaf0, nleaves;
+  if (full) {
+    const long long c = part / a.sub;
+    const int s = part % a.sub;
+    nleaves = plan->nleaves / a.sub;
+    leaf0 = s * nleaves;
+    base = c * PW_CHUNK;
+  } else {
+    base = a.nfull * PW_CHUNK;
+    leaf0 = 0;
+    nleaves = plan->nleaves;
+  }
+  for (int li = g; li < nleaves; li += PCG_WG / 8) {
+    const int leaf = leaf0 + li;
+    const int off = plan->leaf_off[leaf], len = plan->leaf_len[leaf];
+    Two res{0.0, 0.0};
+    if (len >= 8) {
+      const int fl = len - (len % 8);
+      Two acc = elem(base + off + lane);
+      for (int t = 8; t < fl; t += 8) {
+        const Two v = elem(base + off + t + lane);
+        acc.a = acc.a + v.a;
+        if (NS > 1) acc.b = acc.b + v.b;
+      }
+      for (int m = 1; m < 8; m <<= 1) {
+        acc.a = acc.a + shfl_xor_d(acc.a, m);
+        if (NS > 1) acc.b = acc.b + shfl_xor_d(acc.b, m);
+      }
+      res = acc;
+      if (lane == 0) {
+        for (int t = fl; t < len; ++t) {
+          const Two v = elem(base + off + t);
+          res.a = res.a + v.a;
+          if (NS > 1) res.b = res.b + v.b;
+        }
+      }
+    } else if (lane == 0) {
+      for (int t = 0; t < len; ++t) {
+        const Two v = elem(base + off + t);
+        res.a = res.a + v.a;
+        if (NS > 1) res.b = res.b + v.b;
+      }
+    }
+    if (lane == 0) {
+      lds[li] = res.a;
+      if (NS > 1) lds[256 + li] = res.b;
+    }
+  }
+  __syncthreads();
+  if (full) {
+    // perfect pairwise tree over this subtree's leaves (a power of two)
+    for (int w = 1; w < nleaves; w <<= 1) {
+      for (int t = tid; t < nleaves / (2 * w); t += PCG_WG) {
+        lds[2 * w * t] = lds[2 * w * t] + lds[2 * w * t + w];
+        if (NS > 1) lds[256 + 2 * w * t] = lds[256 + 2 * w * t] + lds[256 + 2 * w * t + w];
+      }
+      __syncthreads();
+    }
+    Two out{lds[0], NS > 1 ? lds[256] : 0.0};
+    __syncthreads();
+    return out;
+  }
+  const int L = plan->nleaves;
+  int start = 0;
+  for (int lv = 0; lv < plan->nlevels; ++lv) {
+    const int end = plan->level_end[lv];
+    for (int n = start + tid; n < end; n += PCG_WG) {
+      const int l = plan->node_l[n], r = plan->node_r[n];
+      lds[L + n] = lds[l] + lds[r];
+      if (NS > 1) lds[256 + L + n] = lds[256 + l] + lds[256 + r];
+    }
+    start = end;
+    __syncthreads();
+  }
+  const int root = plan->nnodes > 0 ? L + plan->nnodes - 1 : 0;
+  Two out{lds[root], NS > 1 ? lds[256 + root] : 0.0};
+  __syncthreads();
+  return out;
+}
+
+// In-order float64 sum of a whole vector from the per-part partials: each
+// chunk's top tree levels (perfect, over `sub` subtree sums), then the
+// chunk sums added sequentially. Every workgroup computes it identically.
+__device__ double total_from_parts(const PcgPersistArgs& a, const double* part, double* cs) {
+  const int tid = threadIdx.x;
+  for (int c = tid; c < a.nchunks; c += PCG_WG) {
+    double v;
+    if (c < a.nfull) {
+      double t[8];
+      for (int s = 0; s < a.sub; ++s) t[s] = part[c * a.sub + s];
+      for (int w = 1; w < a.sub; w <<= 1)
+        for (int s = 0; s + w < a.sub; s += 2 * w) t[s] = t[s] + t[s + w];
+      v = t[0];
+    } else {
+      v = part[a.nfull * a.sub];
+    }
+    cs[c] = v;
+  }
+  __syncthreads();
+  __shared__ double s_tot;
+  if (tid == 0) {
+    double s = 0.0;
+    for (int c = 0; c < a.nchunks; ++c) s = s + cs[c];
+    s_tot = s;
+  }
+  __syncthreads();
+  return s_tot;
+}
+
+__device__ __forceinline__ int logical_block(int nb) {
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous range of parts so its L2 holds one slab of the vectors.
+  const int w = blockIdx.x;
+  if (nb % 8 != 0) return w;
+  return (w % 8) * (nb / 8) + (w / 8);
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgPersistArgs a) {
+  __shared__ double lds[512];
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const unsigned nb = gridDim.x;
+  const int part = logical_block((int)nb);
+  const HelmOp& op = a.op;
+  const Dims& d = a.d;
+  bool ok = true;
+  int it = 0, done = 0, conv = 1;
+  double rz = 0.0, alpha = 0.0, beta = 0.0, tol = 0.0, res = 0.0;
+
+  // r = rhs * mask, x = 0; ||rhs||
+  {
+    const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+      const double r = a.rhs[e] * mk(op.mask[e]);
+      a.r[e] = r;
+      a.x[e] = 0.0;
+      return Two{r * r, 0.0};
+    }, lds);
+    if (threadIdx.x == 0) a.part_a[part] = s.a;
+  }
+  ok = grid_barrier(a.bar, a.err, nb);
+  if (ok) {
+    const double nb2 = total_from_parts(a, a.part_a, cs);
+    const double norm_b = sqrt(nb2);
+    tol = a.rtol * norm_b;
+    if (norm_b == 0.0) done = 1;
+  }
+  if (ok && !done) {
+    // z = M^-1 r, p = z (into pb1, the buffer iteration 1 uses); r.z
+    const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+      const double z = a.inv_diag[e] * a.r[e];
+      a.z[e] = z;
+      a.pb1[e] = z;
+      return Two{a.r[e] * z, 0.0};
+    }, lds);
+    if (threadIdx.x == 0) a.part_b[part] = s.a;
+    ok = grid_barrier(a.bar, a.err, nb);
+    if (ok) rz = total_from_parts(a, a.part_b, cs);
+  }
+  while (ok && !done) {
+    it += 1;
+    double* pnew = (it & 1) ? a.pb1 : a.pb0;
+    const double* pold = (it & 1) ? a.pb0 : a.pb1;
+    const bool first = it == 1;
+    const double bt = beta;
+    {
+      const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+        const int i = (int)(e % d.nx), j = (int)(e / d.nx);
+        const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+        const long long cn = at2(d, wrap(j, 1, d.ny), i), cs2 = at2(d, wrap(j, -1, d.ny), i);
+        auto pv = [&](long long q) -> double {
+          return first ? pnew[q] : a.z[q] + bt * pold[q];
+        };
+        const double pc = pv(e);
+        const double mc = mk(op.mask[e]);
+        const double xc = pc * mc;
+        const double xe = pv(ce) * mk(op.mask[ce]);
+        const double xw = pv(cw) * mk(op.mask[cw]);
+        const double xn = pv(cn) * mk(op.mask[cn]);
+        const double xs = pv(cs2) * mk(op.mask[cs2]);
+        const double fx = op.Ku[e] * (xe - xc);
+        const double fxw = op.Ku[cw] * (xc - xw);
+        const double fy = op.Kv[e] * (xn - xc);
+        const double fys = op.Kv[cs2] * (xc - xs);
+        const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+        const double ap = (xc - op.coef * lap) * mc + xc * (1.0 - mc);
+        if (!first) pnew[e] = pc;
+        a.Ap[e] = ap;
+        return Two{pc * ap, 0.0};
+      }, lds);
+      if (threadIdx.x == 0) a.part_a[part] = s.a;
+    }
+    ok = grid_barrier(a.bar, a.err, nb);
+    if (!ok) break;
+    alpha = rz / total_from_parts(a, a.part_a, cs);
+    {
+      const double al = alpha;
+      const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
+        a.x[e] = a.x[e] + al * pnew[e];
+        const double r = a.r[e] - al * a.Ap[e];
+        a.r[e] = r;
+        const double z = a.inv_diag[e] * r;
+        a.z[e] = z;
+        return Two{r * r, r * z};
+      }, lds);
+      if (threadIdx.x == 0) {
+        a.part_b[part] = s.a;
+        a.part_c[part] = s.b;
+      }
+    }
+    ok = grid_barrier(a.bar, a.err, nb);
+    if (!ok) break;
+    res = sqrt(total_from_parts(a, a.part_b, cs));
+    if (res <= tol) {
+      done = 1;
+      conv = 1;
+      break;
+    }
+    const double rz_new = total_from_parts(a, a.part_c, cs);
+    beta = rz_new / rz;
+    rz = rz_new;
+    if (it >= a.max_iter) {
+      done = 1;
+      conv = 0;
+      break;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc->it = it;
+    a.sc->iters = ok ? it : 0;
+    a.sc->done = 1;
+    a.sc->converged = ok ? conv : 0;
+    a.sc->res = res;
+    if (ok) {
+      a.counters[0] += it;
+      if (!conv) a.counters[1] += 1;
+    }
+  }
+}
+
+}  // namespace hoc

@@ -121,3 +121,35 @@ __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,
 __global__ void k_seq_sum(const double* part, long long nchunks, double* out);
 
 }  // namespace hoc
+
+namespace hoc {
+
+// Persistent PCG (one cooperative launch per solve). Full 8192-element
+// chunks are cut into `sub` aligned subtrees of the pairwise tree, one
+// single-wave workgroup each; a trailing partial chunk, if any, is one more
+// workgroup. Partial sums land in part_*[part]; after a grid barrier every
+// workgroup rebuilds each chunk's top levels and the in-order chunk sum.
+struct PcgPersistArgs {
+  Dims d;
+  HelmOp op;
+  const double* rhs;
+  const double* inv_diag;
+  double *x, *r, *z, *pb0, *pb1, *Ap;
+  double *part_a, *part_b, *part_c;
+  const PwPlan* plans;  // [0] full chunk, [1] trailing chunk
+  long long nfull;
+  int sub, tail, nparts, nchunks;
+  double rtol;
+  int max_iter;
+  unsigned* bar;        // [0] arrivals, [1] generation
+  int* err;
+  PcgScalars* sc;
+  long long* counters;
+};
+
+constexpr int PCG_WG = 64;          // one wave per workgroup
+constexpr int PCG_MAX_CHUNKS = 1024;
+
+__global__ void k_pcg_persistent(PcgPersistArgs a);
+
+}  // namespace hoc

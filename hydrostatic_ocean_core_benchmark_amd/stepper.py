@@ -285,6 +285,16 @@ class HipStepper:
         return {k: int(getattr(c, k)) for k, _ in _lib.HocCounters._fields_} | {
             "eos_cells": int(c.eos_cells) + self._eos_host}
 
+    @property
+    def solver_path(self) -> str:
+        """'persistent' (one cooperative launch per solve) or 'multikernel'."""
+        return "persistent" if self.lib.hoc_stepper_solver_path(self._h) == 0 else "multikernel"
+
+    @solver_path.setter
+    def solver_path(self, path: str) -> None:
+        _lib.check(self.lib.hoc_stepper_set_solver_path(self._h, 0 if path == "persistent"
+                                                        else 1))
+
     # --------------------------------------------------------- measurement
     def profile_kernel(self, name: str | None) -> None:
         """Time every launch of one step kernel with HIP events on its stream."""

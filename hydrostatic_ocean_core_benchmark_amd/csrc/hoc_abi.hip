@@ -152,7 +152,9 @@ struct hoc_stepper {
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
   unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
   int* bar_err = nullptr;
-  int pcg_sub = 0, pcg_parts = 0;  // persistent-solver partition (0: not usable)
+  int pcg_sub = 1;        // subtrees per full 8192-element chunk
+  int pcg_parts_all = 0;  // workgroups of one PCG phase
+  int pcg_parts = 0;      // == pcg_parts_all when the cooperative launch is usable
   PcgScalars* sc = nullptr;
   long long* dcount = nullptr;  // [0] solver iterations, [1] failures
   int* host_flag = nullptr;     // pinned: done, iters
@@ -341,11 +343,9 @@ int build_coefficients(hoc_stepper* h, hipStream_t s) {
   return HOC_OK;
 }
 
-// PCG-Jacobi in one cooperative launch (k_pcg_persistent).
-int helmholtz_solve_persistent(hoc_stepper* h, const HelmOp& op, const double* inv_diag,
-                               const double* rhs, double* x, hipStream_t s, int* iters_out,
-                               int* conv_out) {
-  PcgPersistArgs a;
+PcgArgs pcg_args(hoc_stepper* h, const HelmOp& op, const double* inv_diag, const double* rhs,
+                 double* x) {
+  PcgArgs a;
   std::memset(&a, 0, sizeof(a));
   a.d = h->d;
   a.op = op;
@@ -372,74 +372,68 @@ int helmholtz_solve_persistent(hoc_stepper* h, const HelmOp& op, const double* i
   a.err = h->bar_err;
   a.sc = h->sc;
   a.counters = h->dcount;
-  void* args[] = {&a};
-  HIP_TRY(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(h->pcg_parts),
-                                     dim3(PCG_WG), args, 0, s));
-  int tail[4];
-  HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->it, sizeof(tail), hipMemcpyDeviceToHost, s));
+  return a;
+}
+
+// Read back (iterations, converged) after a solve; synchronises.
+int pcg_finish(hoc_stepper* h, hipStream_t s, bool persistent, int* iters_out, int* conv_out) {
+  HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->it, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(h->host_flag + 4, h->bar_err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  std::memcpy(tail, h->host_flag, sizeof(tail));
-  if (h->host_flag[4]) {
+  if (persistent && h->host_flag[4]) {
     (void)hipMemset(h->bar_err, 0, sizeof(int));
     (void)hipMemset(h->bar, 0, 2 * sizeof(unsigned));
     return fail(HOC_ERR_DEVICE, "persistent Helmholtz solve: grid barrier timed out");
   }
-  if (iters_out) *iters_out = tail[1];
-  if (conv_out) *conv_out = tail[3];
-  if (tail[1] > 0) h->last_iters = tail[1];
+  // PcgScalars ends with it, iters, done, converged
+  if (iters_out) *iters_out = h->host_flag[1];
+  if (conv_out) *conv_out = h->host_flag[3];
+  if (h->host_flag[1] > 0) h->last_iters = h->host_flag[1];
   return HOC_OK;
 }
 
-// PCG-Jacobi on A = I - coef D[K G .]; x is the output (device).
+// PCG-Jacobi on A = I - coef D[K G .]; x is the output (device). Path 0: one
+// cooperative launch; path 1: one launch per phase, the host polling the
+// device `done` flag between batches of iterations.
 int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const double* inv_diag,
                     const double* rhs, double* x, hipStream_t s, int* iters_out,
                     int* conv_out) {
-  const Dims d = h->d;
-  const Reducer& R = h->red;
   HelmOp op{Ku, Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2, h->p.dy2};
-  if (h->pcg_parts > 0 && !h->force_multikernel)
-    return helmholtz_solve_persistent(h, op, inv_diag, rhs, x, s, iters_out, conv_out);
-  PcgVec vv{x, h->r, h->z, h->pp, h->Ap, inv_diag, h->part_a, h->part_b};
-  const unsigned nch = (unsigned)R.nchunks;
-  if (nch == 0) return fail(HOC_ERR_INVALID, "empty grid");
-  k_pcg_init<<<nch, 512, 0, s>>>(d, op, vv, rhs, R.plans, R.nfull);
-  LAUNCH_CHECK("k_pcg_init");
-  k_pcg_start<<<1, 1, 0, s>>>(vv, R.nchunks, h->p.rtol, h->sc);
-  LAUNCH_CHECK("k_pcg_start");
-  k_pcg_init_z<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc);
-  LAUNCH_CHECK("k_pcg_init_z");
-  k_pcg_rz0<<<1, 1, 0, s>>>(vv, R.nchunks, h->sc);
-  LAUNCH_CHECK("k_pcg_rz0");
+  PcgArgs a = pcg_args(h, op, inv_diag, rhs, x);
+  const unsigned np = (unsigned)h->pcg_parts_all;
+  if (np == 0) return fail(HOC_ERR_INVALID, "empty grid");
+  if (h->pcg_parts > 0 && !h->force_multikernel) {
+    void* args[] = {&a};
+    { ProfScope _ps(h, "k_pcg_persistent", s);
+      HIP_TRY(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(h->pcg_parts),
+                                         dim3(PCG_WG), args, 0, s)); }
+    return pcg_finish(h, s, true, iters_out, conv_out);
+  }
+  k_pcg_init<<<np, PCG_WG, 0, s>>>(a);
+  k_pcg_start<<<1, PCG_WG, 0, s>>>(a);
+  k_pcg_init_z<<<np, PCG_WG, 0, s>>>(a);
+  k_pcg_rz0<<<1, PCG_WG, 0, s>>>(a);
+  LAUNCH_CHECK("pcg init");
   const int max_iter = h->p.max_iter;
   int launched = 0;
   int batch = std::max(1, std::min(h->last_iters - 1, max_iter));
   while (true) {
     for (int b = 0; b < batch; ++b) {
-      { ProfScope _ps(h, "k_pcg_ap", s);
-        k_pcg_ap<<<nch, 512, 0, s>>>(d, op, vv, R.plans, R.nfull, h->sc); }
-      k_pcg_alpha<<<1, 1, 0, s>>>(vv, R.nchunks, h->sc);
-      { ProfScope _ps(h, "k_pcg_update", s);
-        k_pcg_update<<<nch, 512, 0, s>>>(d, vv, R.plans, R.nfull, h->sc); }
-      k_pcg_check<<<1, 1, 0, s>>>(vv, R.nchunks, max_iter, h->sc, h->dcount);
-      k_pcg_p<<<blocks(d.n2), TPB, 0, s>>>(d, vv, h->sc);
+      { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a); }
+      k_pcg_alpha<<<1, PCG_WG, 0, s>>>(a);
+      { ProfScope _ps(h, "k_pcg_update", s); k_pcg_update<<<np, PCG_WG, 0, s>>>(a); }
+      k_pcg_check<<<1, PCG_WG, 0, s>>>(a);
     }
     LAUNCH_CHECK("pcg iteration");
     launched += batch;
-    HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h->host_flag[0]) break;
     if (launched > max_iter + 8)
       return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
     batch = 4;
   }
-  // PcgScalars ends with it, iters, done, converged
-  int tail[4];
-  HIP_TRY(hipMemcpy(tail, &h->sc->it, sizeof(tail), hipMemcpyDeviceToHost));
-  if (iters_out) *iters_out = tail[1];
-  if (conv_out) *conv_out = tail[3];
-  if (tail[1] > 0) h->last_iters = tail[1];
-  return HOC_OK;
+  return pcg_finish(h, s, false, iters_out, conv_out);
 }
 
 }  // namespace
@@ -548,16 +542,20 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     int sub = 8;
     while (sub > 1 && nfull * sub > 512) sub >>= 1;
     const long long parts = nfull * sub + (h->red.nchunks - nfull);
+    h->pcg_sub = sub;
+    h->pcg_parts_all = (int)parts;
     int dev = 0, ncu = 0, coop = 0, per_cu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pcg_persistent,
                                                      PCG_WG, 0) == hipSuccess &&
-        coop && h->red.nchunks <= PCG_MAX_CHUNKS && parts <= (long long)ncu * per_cu) {
-      h->pcg_sub = sub;
+        coop && parts <= (long long)ncu * per_cu) {
       h->pcg_parts = (int)parts;
     }
+    if (h->red.nchunks > PCG_MAX_CHUNKS)
+      return cleanup(fail(HOC_ERR_UNSUPPORTED, "grid too large for the Helmholtz solver "
+                                               "(more than 1024 x 8192 columns)"));
     double* pc = nullptr;
     rc = h->alloc(&pc, std::max<long long>(parts, h->red.nchunks));
     if (rc != HOC_OK) return cleanup(rc);

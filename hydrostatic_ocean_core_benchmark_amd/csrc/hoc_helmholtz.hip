@@ -39,7 +39,24 @@ __device__ Two chunk_sum(const PwPlan* __restrict__ plan, long long base, F elem
   for (int leaf = g; leaf < L; leaf += PW_THREADS / 8) {
     const int off = plan->leaf_off[leaf], len = plan->leaf_len[leaf];
     Two res{0.0, 0.0};
-    if (len >= 8) {
+    if (len == 128) {
+      // 16 elements per lane: evaluate them all first so their loads are in
+      // flight together, then accumulate in NumPy's order
+      Two v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[t] = elem(base + off + 8 * t + lane);
+      Two acc = v[0];
+#pragma unroll
+      for (int t = 1; t < 16; ++t) {
+        acc.a = acc.a + v[t].a;
+        if (NS > 1) acc.b = acc.b + v[t].b;
+      }
+      for (int m = 1; m < 8; m <<= 1) {
+        acc.a = acc.a + shfl_xor_d(acc.a, m);
+        if (NS > 1) acc.b = acc.b + shfl_xor_d(acc.b, m);
+      }
+      res = acc;
+    } else if (len >= 8) {
       const int full = len - (len % 8);
       Two acc = elem(base + off + lane);
       for (int t = 8; t < full; t += 8) {
@@ -128,147 +145,6 @@ __global__ void k_helm_diag(Dims d, HelmOp op, double* __restrict__ inv_diag) {
   inv_diag[c] = 1.0 / dg;
 }
 
-// ---------------------------------------------------------------- PCG
-// init 1: r = rhs * mask, x = 0, partial sums of r*r (the norm of rhs)
-__global__ __launch_bounds__(PW_THREADS) void k_pcg_init(Dims d, HelmOp op, PcgVec vv,
-                                                         const double* __restrict__ rhs,
-                                                         const PwPlan* __restrict__ plans,
-                                                         long long nfull) {
-  __shared__ double lds[512];
-  const long long chunk = blockIdx.x;
-  const long long base = chunk * PW_CHUNK;
-  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
-    const double r = rhs[e] * mk(op.mask[e]);
-    vv.r[e] = r;
-    vv.x[e] = 0.0;
-    return Two{r * r, 0.0};
-  }, lds);
-  if (threadIdx.x == 0) vv.part_a[chunk] = s.a;
-}
-
-// init 2: z = inv_diag r, p = z, partial sums of r*z
-__global__ __launch_bounds__(PW_THREADS) void k_pcg_init_z(Dims d, PcgVec vv,
-                                                           const PwPlan* __restrict__ plans,
-                                                           long long nfull,
-                                                           const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double lds[512];
-  const long long chunk = blockIdx.x;
-  const long long base = chunk * PW_CHUNK;
-  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
-    const double z = vv.inv_diag[e] * vv.r[e];
-    vv.z[e] = z;
-    vv.p[e] = z;
-    return Two{vv.r[e] * z, 0.0};
-  }, lds);
-  if (threadIdx.x == 0) vv.part_b[chunk] = s.a;
-}
-
-__device__ __forceinline__ double seq_sum(const double* part, long long nchunks) {
-  double s = 0.0;
-  for (long long c = 0; c < nchunks; ++c) s = s + part[c];
-  return s;
-}
-
-// norm_b = sqrt(sum rhs^2); a zero right-hand side finishes in 0 iterations.
-__global__ void k_pcg_start(PcgVec vv, long long nchunks, double rtol, PcgScalars* sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double nb = sqrt(seq_sum(vv.part_a, nchunks));
-  sc->norm_b = nb;
-  sc->tol = rtol * nb;
-  sc->it = 0;
-  sc->iters = 0;
-  sc->converged = 1;
-  sc->done = (nb == 0.0) ? 1 : 0;
-  sc->res = 0.0;
-}
-
-__global__ void k_pcg_rz0(PcgVec vv, long long nchunks, PcgScalars* sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
-  sc->rz = seq_sum(vv.part_b, nchunks);
-}
-
-// A: Ap = A p and partial sums of p*Ap
-__global__ __launch_bounds__(PW_THREADS) void k_pcg_ap(Dims d, HelmOp op, PcgVec vv,
-                                                       const PwPlan* __restrict__ plans,
-                                                       long long nfull,
-                                                       const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double lds[512];
-  const long long chunk = blockIdx.x;
-  const long long base = chunk * PW_CHUNK;
-  const Two s = chunk_sum<1>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
-    const double ap = apply_A(d, op, vv.p, e);
-    vv.Ap[e] = ap;
-    return Two{vv.p[e] * ap, 0.0};
-  }, lds);
-  if (threadIdx.x == 0) vv.part_a[chunk] = s.a;
-}
-
-// B: alpha = rz / (p.Ap), one more iteration
-__global__ void k_pcg_alpha(PcgVec vv, long long nchunks, PcgScalars* sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
-  sc->alpha = sc->rz / seq_sum(vv.part_a, nchunks);
-  sc->it = sc->it + 1;
-}
-
-// C: x += alpha p, r -= alpha Ap, z = inv_diag r; partial sums of r*r, r*z
-__global__ __launch_bounds__(PW_THREADS) void k_pcg_update(Dims d, PcgVec vv,
-                                                           const PwPlan* __restrict__ plans,
-                                                           long long nfull,
-                                                           const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  __shared__ double lds[512];
-  const double alpha = sc->alpha;
-  const long long chunk = blockIdx.x;
-  const long long base = chunk * PW_CHUNK;
-  const Two s = chunk_sum<2>(plan_of(plans, chunk, nfull), base, [&](long long e) -> Two {
-    vv.x[e] = vv.x[e] + alpha * vv.p[e];
-    const double r = vv.r[e] - alpha * vv.Ap[e];
-    vv.r[e] = r;
-    const double z = vv.inv_diag[e] * r;
-    vv.z[e] = z;
-    return Two{r * r, r * z};
-  }, lds);
-  if (threadIdx.x == 0) {
-    vv.part_a[chunk] = s.a;
-    vv.part_b[chunk] = s.b;
-  }
-}
-
-// D: convergence test ||r|| <= rtol ||rhs||, else beta = rz_new / rz
-__global__ void k_pcg_check(PcgVec vv, long long nchunks, int max_iter, PcgScalars* sc,
-                            long long* counters) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || sc->done) return;
-  const double res = sqrt(seq_sum(vv.part_a, nchunks));
-  sc->res = res;
-  if (res <= sc->tol) {
-    sc->done = 1;
-    sc->converged = 1;
-    sc->iters = sc->it;
-    counters[0] += sc->it;
-    return;
-  }
-  const double rz_new = seq_sum(vv.part_b, nchunks);
-  sc->beta = rz_new / sc->rz;
-  sc->rz = rz_new;
-  if (sc->it >= max_iter) {
-    sc->done = 1;
-    sc->converged = 0;
-    sc->iters = max_iter;
-    counters[0] += max_iter;
-    counters[1] += 1;
-  }
-}
-
-// E: p = z + beta p
-__global__ void k_pcg_p(Dims d, PcgVec vv, const PcgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.n2) return;
-  vv.p[c] = vv.z[c] + sc->beta * vv.p[c];
-}
-
 // Generic pairwise dot (used by tests of the reduction): part[chunk] = sum(a*b)
 __global__ __launch_bounds__(PW_THREADS) void k_pw_dot(const double* __restrict__ a,
                                                        const double* __restrict__ b,
@@ -284,7 +160,10 @@ __global__ __launch_bounds__(PW_THREADS) void k_pw_dot(const double* __restrict_
 }
 
 __global__ void k_seq_sum(const double* __restrict__ part, long long nchunks, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = seq_sum(part, nchunks);
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (long long c = 0; c < nchunks; ++c) s = s + part[c];
+  out[0] = s;
 }
 
 }  // namespace hoc
@@ -337,7 +216,7 @@ __device__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks) {
 // the trailing chunk follows its plan. `elem(e)` may store per-element
 // results; it is called once per index of the part.
 template <int NS, class F>
-__device__ Two part_sum(const PcgPersistArgs& a, int part, F elem, double* lds) {
+__device__ Two part_sum(const PcgArgs& a, int part, F elem, double* lds) {
   const int tid = threadIdx.x;
   const int g = tid >> 3, lane = tid & 7;
   const bool full = part < a.nfull * a.sub;
@@ -359,7 +238,24 @@ __device__ Two part_sum(const PcgPersistArgs& a, int part, F elem, double* lds) 
     const int leaf = leaf0 + li;
     const int off = plan->leaf_off[leaf], len = plan->leaf_len[leaf];
     Two res{0.0, 0.0};
-    if (len >= 8) {
+    if (len == 128) {
+      // 16 elements per lane: evaluate them all first so their loads are in
+      // flight together, then accumulate in NumPy's order
+      Two v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[t] = elem(base + off + 8 * t + lane);
+      Two acc = v[0];
+#pragma unroll
+      for (int t = 1; t < 16; ++t) {
+        acc.a = acc.a + v[t].a;
+        if (NS > 1) acc.b = acc.b + v[t].b;
+      }
+      for (int m = 1; m < 8; m <<= 1) {
+        acc.a = acc.a + shfl_xor_d(acc.a, m);
+        if (NS > 1) acc.b = acc.b + shfl_xor_d(acc.b, m);
+      }
+      res = acc;
+    } else if (len >= 8) {
       const int fl = len - (len % 8);
       Two acc = elem(base + off + lane);
       for (int t = 8; t < fl; t += 8) {
@@ -426,7 +322,7 @@ __device__ Two part_sum(const PcgPersistArgs& a, int part, F elem, double* lds) 
 // In-order float64 sum of a whole vector from the per-part partials: each
 // chunk's top tree levels (perfect, over `sub` subtree sums), then the
 // chunk sums added sequentially. Every workgroup computes it identically.
-__device__ double total_from_parts(const PcgPersistArgs& a, const double* part, double* cs) {
+__device__ double total_from_parts(const PcgArgs& a, const double* part, double* cs) {
   const int tid = threadIdx.x;
   for (int c = tid; c < a.nchunks; c += PCG_WG) {
     double v;
@@ -452,6 +348,48 @@ __device__ double total_from_parts(const PcgPersistArgs& a, const double* part, 
   return s_tot;
 }
 
+
+// Per-element work of the A phase: p_new at the five stencil points (z +
+// beta p_old; p itself on the first iteration), A p_new at e, store p_new
+// and Ap; returns p_new * Ap for the p.Ap sum.
+__device__ __forceinline__ Two pcg_apply_elem(const PcgArgs& a, long long e, bool first,
+                                              double bt, double* pnew, const double* pold) {
+  const Dims& d = a.d;
+  const HelmOp& op = a.op;
+  const int i = (int)(e % d.nx), j = (int)(e / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
+  auto pv = [&](long long q) -> double { return first ? pnew[q] : a.z[q] + bt * pold[q]; };
+  const double pc = pv(e);
+  const double mc = mk(op.mask[e]);
+  const double xc = pc * mc;
+  const double xe = pv(ce) * mk(op.mask[ce]);
+  const double xw = pv(cw) * mk(op.mask[cw]);
+  const double xn = pv(cn) * mk(op.mask[cn]);
+  const double xs = pv(cs) * mk(op.mask[cs]);
+  const double fx = op.Ku[e] * (xe - xc);
+  const double fxw = op.Ku[cw] * (xc - xw);
+  const double fy = op.Kv[e] * (xn - xc);
+  const double fys = op.Kv[cs] * (xc - xs);
+  const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+  const double ap = (xc - op.coef * lap) * mc + xc * (1.0 - mc);
+  if (!first) pnew[e] = pc;
+  a.Ap[e] = ap;
+  return Two{pc * ap, 0.0};
+}
+
+// Per-element work of the update phase: x += alpha p, r -= alpha Ap,
+// z = M^-1 r; returns (r*r, r*z).
+__device__ __forceinline__ Two pcg_update_elem(const PcgArgs& a, long long e, double al,
+                                               const double* pnew) {
+  a.x[e] = a.x[e] + al * pnew[e];
+  const double r = a.r[e] - al * a.Ap[e];
+  a.r[e] = r;
+  const double z = a.inv_diag[e] * r;
+  a.z[e] = z;
+  return Two{r * r, r * z};
+}
+
 __device__ __forceinline__ int logical_block(int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous range of parts so its L2 holds one slab of the vectors.
@@ -460,7 +398,7 @@ __device__ __forceinline__ int logical_block(int nb) {
   return (w % 8) * (nb / 8) + (w / 8);
 }
 
-__global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgPersistArgs a) {
+__global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
   __shared__ double lds[512];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const unsigned nb = gridDim.x;
@@ -508,28 +446,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgPersistArgs a) {
     const double bt = beta;
     {
       const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
-        const int i = (int)(e % d.nx), j = (int)(e / d.nx);
-        const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
-        const long long cn = at2(d, wrap(j, 1, d.ny), i), cs2 = at2(d, wrap(j, -1, d.ny), i);
-        auto pv = [&](long long q) -> double {
-          return first ? pnew[q] : a.z[q] + bt * pold[q];
-        };
-        const double pc = pv(e);
-        const double mc = mk(op.mask[e]);
-        const double xc = pc * mc;
-        const double xe = pv(ce) * mk(op.mask[ce]);
-        const double xw = pv(cw) * mk(op.mask[cw]);
-        const double xn = pv(cn) * mk(op.mask[cn]);
-        const double xs = pv(cs2) * mk(op.mask[cs2]);
-        const double fx = op.Ku[e] * (xe - xc);
-        const double fxw = op.Ku[cw] * (xc - xw);
-        const double fy = op.Kv[e] * (xn - xc);
-        const double fys = op.Kv[cs2] * (xc - xs);
-        const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
-        const double ap = (xc - op.coef * lap) * mc + xc * (1.0 - mc);
-        if (!first) pnew[e] = pc;
-        a.Ap[e] = ap;
-        return Two{pc * ap, 0.0};
+        return pcg_apply_elem(a, e, first, bt, pnew, pold);
       }, lds);
       if (threadIdx.x == 0) a.part_a[part] = s.a;
     }
@@ -539,12 +456,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgPersistArgs a) {
     {
       const double al = alpha;
       const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
-        a.x[e] = a.x[e] + al * pnew[e];
-        const double r = a.r[e] - al * a.Ap[e];
-        a.r[e] = r;
-        const double z = a.inv_diag[e] * r;
-        a.z[e] = z;
-        return Two{r * r, r * z};
+        return pcg_update_elem(a, e, al, pnew);
       }, lds);
       if (threadIdx.x == 0) {
         a.part_b[part] = s.a;
@@ -578,6 +490,124 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgPersistArgs a) {
       a.counters[0] += it;
       if (!conv) a.counters[1] += 1;
     }
+  }
+}
+
+
+// ------------------------------------------- per-phase launches (path 1)
+// The same algorithm and the same reductions as k_pcg_persistent, one
+// launch per phase; the scalars live in PcgScalars between launches.
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_init(PcgArgs a) {
+  __shared__ double lds[512];
+  const int part = blockIdx.x;
+  const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+    const double r = a.rhs[e] * mk(a.op.mask[e]);
+    a.r[e] = r;
+    a.x[e] = 0.0;
+    return Two{r * r, 0.0};
+  }, lds);
+  if (threadIdx.x == 0) a.part_a[part] = s.a;
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_start(PcgArgs a) {
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const double nb = sqrt(total_from_parts(a, a.part_a, cs));
+  if (threadIdx.x == 0) {
+    a.sc->norm_b = nb;
+    a.sc->tol = a.rtol * nb;
+    a.sc->it = 0;
+    a.sc->iters = 0;
+    a.sc->converged = 1;
+    a.sc->done = (nb == 0.0) ? 1 : 0;
+    a.sc->res = 0.0;
+    a.sc->beta = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double lds[512];
+  const int part = blockIdx.x;
+  const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+    const double z = a.inv_diag[e] * a.r[e];
+    a.z[e] = z;
+    a.pb1[e] = z;
+    return Two{a.r[e] * z, 0.0};
+  }, lds);
+  if (threadIdx.x == 0) a.part_b[part] = s.a;
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_rz0(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const double rz = total_from_parts(a, a.part_b, cs);
+  if (threadIdx.x == 0) a.sc->rz = rz;
+}
+
+// A: p_new = z + beta p_old (p_new = p on the first iteration), Ap, p.Ap
+__global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double lds[512];
+  const int part = blockIdx.x;
+  const int it = a.sc->it + 1;
+  const double bt = a.sc->beta;
+  double* pnew = (it & 1) ? a.pb1 : a.pb0;
+  const double* pold = (it & 1) ? a.pb0 : a.pb1;
+  const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
+    return pcg_apply_elem(a, e, it == 1, bt, pnew, pold);
+  }, lds);
+  if (threadIdx.x == 0) a.part_a[part] = s.a;
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_alpha(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const double pap = total_from_parts(a, a.part_a, cs);
+  if (threadIdx.x == 0) {
+    a.sc->alpha = a.sc->rz / pap;
+    a.sc->it = a.sc->it + 1;
+  }
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double lds[512];
+  const int part = blockIdx.x;
+  const double al = a.sc->alpha;
+  const double* pnew = (a.sc->it & 1) ? a.pb1 : a.pb0;
+  const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
+    return pcg_update_elem(a, e, al, pnew);
+  }, lds);
+  if (threadIdx.x == 0) {
+    a.part_b[part] = s.a;
+    a.part_c[part] = s.b;
+  }
+}
+
+__global__ __launch_bounds__(PCG_WG) void k_pcg_check(PcgArgs a) {
+  if (a.sc->done) return;
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const double res = sqrt(total_from_parts(a, a.part_b, cs));
+  const double rz_new = total_from_parts(a, a.part_c, cs);
+  if (threadIdx.x != 0) return;
+  PcgScalars* sc = a.sc;
+  sc->res = res;
+  if (res <= sc->tol) {
+    sc->done = 1;
+    sc->converged = 1;
+    sc->iters = sc->it;
+    a.counters[0] += sc->it;
+    return;
+  }
+  sc->beta = rz_new / sc->rz;
+  sc->rz = rz_new;
+  if (sc->it >= a.max_iter) {
+    sc->done = 1;
+    sc->converged = 0;
+    sc->iters = a.max_iter;
+    a.counters[0] += a.max_iter;
+    a.counters[1] += 1;
   }
 }
 

@@ -31,12 +31,6 @@ struct HelmOp {
   double coef, cx, cy, dx2, dy2;
 };
 
-struct PcgVec {
-  double *x, *r, *z, *p, *Ap;
-  const double* inv_diag;
-  double *part_a, *part_b;  // per-chunk partial sums
-};
-
 struct PcgScalars {
   double rz, alpha, beta, tol, norm_b, res;
   int it, iters, done, converged;
@@ -102,20 +96,6 @@ __global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double
 
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
-__global__ void k_pcg_init(Dims d, HelmOp op, PcgVec vv, const double* rhs,
-                           const PwPlan* plans, long long nfull);
-__global__ void k_pcg_init_z(Dims d, PcgVec vv, const PwPlan* plans, long long nfull,
-                             const PcgScalars* sc);
-__global__ void k_pcg_start(PcgVec vv, long long nchunks, double rtol, PcgScalars* sc);
-__global__ void k_pcg_rz0(PcgVec vv, long long nchunks, PcgScalars* sc);
-__global__ void k_pcg_ap(Dims d, HelmOp op, PcgVec vv, const PwPlan* plans, long long nfull,
-                         const PcgScalars* sc);
-__global__ void k_pcg_alpha(PcgVec vv, long long nchunks, PcgScalars* sc);
-__global__ void k_pcg_update(Dims d, PcgVec vv, const PwPlan* plans, long long nfull,
-                             const PcgScalars* sc);
-__global__ void k_pcg_check(PcgVec vv, long long nchunks, int max_iter, PcgScalars* sc,
-                            long long* counters);
-__global__ void k_pcg_p(Dims d, PcgVec vv, const PcgScalars* sc);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,
                          long long nfull, double* part);
 __global__ void k_seq_sum(const double* part, long long nchunks, double* out);
@@ -129,7 +109,7 @@ namespace hoc {
 // single-wave workgroup each; a trailing partial chunk, if any, is one more
 // workgroup. Partial sums land in part_*[part]; after a grid barrier every
 // workgroup rebuilds each chunk's top levels and the in-order chunk sum.
-struct PcgPersistArgs {
+struct PcgArgs {
   Dims d;
   HelmOp op;
   const double* rhs;
@@ -150,6 +130,14 @@ struct PcgPersistArgs {
 constexpr int PCG_WG = 64;          // one wave per workgroup
 constexpr int PCG_MAX_CHUNKS = 1024;
 
-__global__ void k_pcg_persistent(PcgPersistArgs a);
+__global__ void k_pcg_persistent(PcgArgs a);
+__global__ void k_pcg_init(PcgArgs a);
+__global__ void k_pcg_start(PcgArgs a);
+__global__ void k_pcg_init_z(PcgArgs a);
+__global__ void k_pcg_rz0(PcgArgs a);
+__global__ void k_pcg_ap(PcgArgs a);
+__global__ void k_pcg_alpha(PcgArgs a);
+__global__ void k_pcg_update(PcgArgs a);
+__global__ void k_pcg_check(PcgArgs a);
 
 }  // namespace hoc

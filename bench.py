@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--kernel", default="k_tracers",
                     help="kernel timed with HIP events for the roofline figure")
+    ap.add_argument("--solver-path", choices=("multikernel", "persistent"), default="multikernel",
+                    help="Helmholtz launch strategy (bitwise the same iterates)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -107,6 +109,7 @@ def main() -> int:
     prob = basin(nx=args.nx, nz=args.nz)
     g = prob.domain.grid
     st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+    st.solver_path = args.solver_path
     T = torch.as_tensor(prob.initial_T, device="cuda")
     S = torch.as_tensor(prob.initial_S, device="cuda")
     mask3 = torch.as_tensor(prob.domain.mask3, device="cuda")
@@ -163,7 +166,7 @@ def main() -> int:
                                "dx 10 km, dt 600 s, up3_tvd, TKE, polyTEOS10, theta 0.5 "
                                "PCG-Jacobi rtol 1e-12",
                    "cells": cells, "wet_fraction": round(float(np.mean(prob.domain.mask3)), 4),
-                   "solver": prob.params.solver,
+                   "solver": prob.params.solver, "solver_path": st.solver_path,
                    "solver_iterations_per_step": iters / args.steps,
                    "solver_failures": fails, "state_finite": finite},
         "roofline": roofline,

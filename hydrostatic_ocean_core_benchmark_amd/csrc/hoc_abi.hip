@@ -160,7 +160,7 @@ struct hoc_stepper {
   int* host_flag = nullptr;     // pinned: done, iters
   Reducer red;
   int last_iters = 8;
-  bool force_multikernel = false;  // test hook: use the multi-launch PCG
+  bool force_multikernel = true;   // per-phase launches unless path 0 is asked for
   bool coeffs_built = false;
   // host counters
   long long solver_rebuilds = 0, tridiagonal_solves = 0, eos_cells = 0;
@@ -403,6 +403,7 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   const unsigned np = (unsigned)h->pcg_parts_all;
   if (np == 0) return fail(HOC_ERR_INVALID, "empty grid");
   if (h->pcg_parts > 0 && !h->force_multikernel) {
+    a.sc = h->sc;
     void* args[] = {&a};
     { ProfScope _ps(h, "k_pcg_persistent", s);
       HIP_TRY(hipLaunchCooperativeKernel((const void*)k_pcg_persistent, dim3(h->pcg_parts),
@@ -410,26 +411,26 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
     return pcg_finish(h, s, true, iters_out, conv_out);
   }
   k_pcg_init<<<np, PCG_WG, 0, s>>>(a);
-  k_pcg_start<<<1, PCG_WG, 0, s>>>(a);
   k_pcg_init_z<<<np, PCG_WG, 0, s>>>(a);
-  k_pcg_rz0<<<1, PCG_WG, 0, s>>>(a);
   LAUNCH_CHECK("pcg init");
   const int max_iter = h->p.max_iter;
-  int launched = 0;
-  int batch = std::max(1, std::min(h->last_iters - 1, max_iter));
+  // Iteration it's A launch also finishes iteration it-1, so max_iter
+  // iterations need max_iter + 1 A launches; the device flag stops early.
+  int it = 1;
+  int batch = std::max(1, std::min(h->last_iters + 1, max_iter + 1));
   while (true) {
-    for (int b = 0; b < batch; ++b) {
-      { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a); }
-      k_pcg_alpha<<<1, PCG_WG, 0, s>>>(a);
-      { ProfScope _ps(h, "k_pcg_update", s); k_pcg_update<<<np, PCG_WG, 0, s>>>(a); }
-      k_pcg_check<<<1, PCG_WG, 0, s>>>(a);
+    for (int b = 0; b < batch && it <= max_iter + 1; ++b, ++it) {
+      { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it); }
+      if (it <= max_iter) {
+        ProfScope _ps(h, "k_pcg_update", s);
+        k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it);
+      }
     }
     LAUNCH_CHECK("pcg iteration");
-    launched += batch;
     HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h->host_flag[0]) break;
-    if (launched > max_iter + 8)
+    if (it > max_iter + 1)
       return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
     batch = 4;
   }

@@ -495,9 +495,14 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
 
 
 // ------------------------------------------- per-phase launches (path 1)
-// The same algorithm and the same reductions as k_pcg_persistent, one
-// launch per phase; the scalars live in PcgScalars between launches.
+// The same algorithm and reductions as k_pcg_persistent in two launches per
+// iteration. Every workgroup of a launch recomputes the scalar the previous
+// phase's partial sums define (identically, so all agree); the host passes
+// the iteration number, and rz is double-buffered by iteration parity, so
+// no workgroup reads a scalar that another workgroup of the same launch
+// writes.
 
+// r = rhs * mask, x = 0; partial sums of r*r
 __global__ __launch_bounds__(PCG_WG) void k_pcg_init(PcgArgs a) {
   __shared__ double lds[512];
   const int part = blockIdx.x;
@@ -510,24 +515,20 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_init(PcgArgs a) {
   if (threadIdx.x == 0) a.part_a[part] = s.a;
 }
 
-__global__ __launch_bounds__(PCG_WG) void k_pcg_start(PcgArgs a) {
+// ||rhs|| (zero: done in 0 iterations); z = M^-1 r, p = z; partials of r*z
+__global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
+  __shared__ double lds[512];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const double nb = sqrt(total_from_parts(a, a.part_a, cs));
-  if (threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.sc->norm_b = nb;
     a.sc->tol = a.rtol * nb;
-    a.sc->it = 0;
     a.sc->iters = 0;
     a.sc->converged = 1;
     a.sc->done = (nb == 0.0) ? 1 : 0;
     a.sc->res = 0.0;
-    a.sc->beta = 0.0;
   }
-}
-
-__global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
-  if (a.sc->done) return;
-  __shared__ double lds[512];
+  if (nb == 0.0) return;
   const int part = blockIdx.x;
   const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
     const double z = a.inv_diag[e] * a.r[e];
@@ -538,76 +539,68 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
   if (threadIdx.x == 0) a.part_b[part] = s.a;
 }
 
-__global__ __launch_bounds__(PCG_WG) void k_pcg_rz0(PcgArgs a) {
-  if (a.sc->done) return;
-  __shared__ double cs[PCG_MAX_CHUNKS];
-  const double rz = total_from_parts(a, a.part_b, cs);
-  if (threadIdx.x == 0) a.sc->rz = rz;
-}
-
-// A: p_new = z + beta p_old (p_new = p on the first iteration), Ap, p.Ap
-__global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a) {
+// Iteration `it` (1-based), phase A. For it > 1 it first finishes
+// iteration it-1: the convergence test on ||r||, then beta and the new rz.
+__global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
   if (a.sc->done) return;
   __shared__ double lds[512];
-  const int part = blockIdx.x;
-  const int it = a.sc->it + 1;
-  const double bt = a.sc->beta;
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double beta = 0.0;
+  if (it == 1) {
+    const double rz = total_from_parts(a, a.part_b, cs);
+    if (lead) a.sc->rz2[1] = rz;
+  } else {
+    const double res = sqrt(total_from_parts(a, a.part_b, cs));
+    if (res <= a.sc->tol) {
+      if (lead) {
+        a.sc->res = res;
+        a.sc->done = 1;
+        a.sc->converged = 1;
+        a.sc->iters = it - 1;
+        a.counters[0] += it - 1;
+      }
+      return;
+    }
+    if (it - 1 >= a.max_iter) {
+      if (lead) {
+        a.sc->res = res;
+        a.sc->done = 1;
+        a.sc->converged = 0;
+        a.sc->iters = a.max_iter;
+        a.counters[0] += a.max_iter;
+        a.counters[1] += 1;
+      }
+      return;
+    }
+    const double rz_new = total_from_parts(a, a.part_c, cs);
+    beta = rz_new / a.sc->rz2[(it - 1) & 1];
+    if (lead) a.sc->rz2[it & 1] = rz_new;
+  }
   double* pnew = (it & 1) ? a.pb1 : a.pb0;
   const double* pold = (it & 1) ? a.pb0 : a.pb1;
+  const int part = blockIdx.x;
   const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
-    return pcg_apply_elem(a, e, it == 1, bt, pnew, pold);
+    return pcg_apply_elem(a, e, it == 1, beta, pnew, pold);
   }, lds);
   if (threadIdx.x == 0) a.part_a[part] = s.a;
 }
 
-__global__ __launch_bounds__(PCG_WG) void k_pcg_alpha(PcgArgs a) {
-  if (a.sc->done) return;
-  __shared__ double cs[PCG_MAX_CHUNKS];
-  const double pap = total_from_parts(a, a.part_a, cs);
-  if (threadIdx.x == 0) {
-    a.sc->alpha = a.sc->rz / pap;
-    a.sc->it = a.sc->it + 1;
-  }
-}
-
-__global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a) {
+// Iteration `it`, update phase: alpha = rz / p.Ap, then x, r, z and the
+// partial sums of r*r and r*z.
+__global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a, int it) {
   if (a.sc->done) return;
   __shared__ double lds[512];
+  __shared__ double cs[PCG_MAX_CHUNKS];
+  const double al = a.sc->rz2[it & 1] / total_from_parts(a, a.part_a, cs);
+  const double* pnew = (it & 1) ? a.pb1 : a.pb0;
   const int part = blockIdx.x;
-  const double al = a.sc->alpha;
-  const double* pnew = (a.sc->it & 1) ? a.pb1 : a.pb0;
   const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
     return pcg_update_elem(a, e, al, pnew);
   }, lds);
   if (threadIdx.x == 0) {
     a.part_b[part] = s.a;
     a.part_c[part] = s.b;
-  }
-}
-
-__global__ __launch_bounds__(PCG_WG) void k_pcg_check(PcgArgs a) {
-  if (a.sc->done) return;
-  __shared__ double cs[PCG_MAX_CHUNKS];
-  const double res = sqrt(total_from_parts(a, a.part_b, cs));
-  const double rz_new = total_from_parts(a, a.part_c, cs);
-  if (threadIdx.x != 0) return;
-  PcgScalars* sc = a.sc;
-  sc->res = res;
-  if (res <= sc->tol) {
-    sc->done = 1;
-    sc->converged = 1;
-    sc->iters = sc->it;
-    a.counters[0] += sc->it;
-    return;
-  }
-  sc->beta = rz_new / sc->rz;
-  sc->rz = rz_new;
-  if (sc->it >= a.max_iter) {
-    sc->done = 1;
-    sc->converged = 0;
-    sc->iters = a.max_iter;
-    a.counters[0] += a.max_iter;
-    a.counters[1] += 1;
   }
 }
 

@@ -33,6 +33,7 @@ struct HelmOp {
 
 struct PcgScalars {
   double rz, alpha, beta, tol, norm_b, res;
+  double rz2[2];  // rz by iteration parity (per-phase path)
   int it, iters, done, converged;
 };
 
@@ -132,12 +133,8 @@ constexpr int PCG_MAX_CHUNKS = 1024;
 
 __global__ void k_pcg_persistent(PcgArgs a);
 __global__ void k_pcg_init(PcgArgs a);
-__global__ void k_pcg_start(PcgArgs a);
 __global__ void k_pcg_init_z(PcgArgs a);
-__global__ void k_pcg_rz0(PcgArgs a);
-__global__ void k_pcg_ap(PcgArgs a);
-__global__ void k_pcg_alpha(PcgArgs a);
-__global__ void k_pcg_update(PcgArgs a);
-__global__ void k_pcg_check(PcgArgs a);
+__global__ void k_pcg_ap(PcgArgs a, int it);
+__global__ void k_pcg_update(PcgArgs a, int it);
 
 }  // namespace hoc

@@ -104,9 +104,10 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
 /* Counters accumulated since create (synchronises). */
 int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out);
 
-/* Helmholtz solver launch strategy: 0 = one persistent cooperative launch
- * per solve (default when the device supports it), 1 = one launch per PCG
- * phase. Both give bitwise the same iterates. No reference counterpart. */
+/* Helmholtz solver launch strategy: 1 = two launches per PCG iteration (the
+ * default), 0 = one persistent cooperative launch per solve with grid
+ * barriers (when the device can co-schedule it). Both give bitwise the same
+ * iterates. No reference counterpart. */
 int hoc_stepper_set_solver_path(hoc_stepper* h, int32_t path);
 /* The path the next solve will take (0 or 1; -1 for a null handle). */
 int32_t hoc_stepper_solver_path(hoc_stepper* h);

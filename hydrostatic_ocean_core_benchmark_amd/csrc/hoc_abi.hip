@@ -540,9 +540,11 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     // persistent solver: split each full chunk into `sub` subtrees so that
     // about 256-512 single-wave workgroups share the work
     const long long nfull = h->red.nfull;
-    int sub = 8;
-    while (sub > 1 && nfull * sub > 512) sub >>= 1;
-    const long long parts = nfull * sub + (h->red.nchunks - nfull);
+    // parts of 8192 / PCG_MAX_SUB = 512 elements: one element per thread
+    // in two passes at most, ~8 waves per CU at 512 x 512
+    const int sub = PCG_MAX_SUB;
+    PwPlan tail_plan = make_plan((int)(d.n2 % PW_CHUNK));
+    const long long parts = nfull * sub + (h->red.nchunks > nfull ? tail_plan.nleaves : 0);
     h->pcg_sub = sub;
     h->pcg_parts_all = (int)parts;
     int dev = 0, ncu = 0, coop = 0, per_cu = 0;

@@ -211,57 +211,53 @@ __device__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks) {
   return s_ok != 0;
 }
 
-// This workgroup's part: element range and the pairwise sum(s) over it in
-// NumPy's order. Full-chunk parts are perfect subtrees (leaves of 128);
-// the trailing chunk follows its plan. `elem(e)` may store per-element
-// results; it is called once per index of the part.
+// This workgroup's part and the pairwise sum(s) over it in NumPy's order.
+// Parts: full 8192-element chunks are cut into `sub` aligned subtrees of
+// 8192/sub elements (each a perfect tree of 128-element leaves); the
+// trailing partial chunk contributes one part per leaf of its plan.
+// Each thread evaluates elements independently (`elem(e)` may store
+// per-element results; it runs exactly once per index), the products are
+// staged in LDS, then eight lanes per leaf accumulate them with NumPy's
+// eight interleaved accumulators and the leaves are combined pairwise.
 template <int NS, class F>
-__device__ Two part_sum(const PcgArgs& a, int part, F elem, double* lds) {
+__device__ Two part_sum(const PcgArgs& a, int part, F elem, double* lds /* [2][PCG_STAGE] */,
+                        double* leafs /* [2][64] */) {
   const int tid = threadIdx.x;
-  const int g = tid >> 3, lane = tid & 7;
-  const bool full = part < a.nfull * a.sub;
-  const PwPlan* plan = full ? &a.plans[0] : &a.plans[1];
+  const long long nfp = a.nfull * a.sub;
   long long base;
-  int leaf0, nleaves;
-  if (full) {
+  int nel, nleaves, leaf_len;
+  if (part < nfp) {
     const long long c = part / a.sub;
     const int s = part % a.sub;
-    nleaves = plan->nleaves / a.sub;
-    leaf0 = s * nleaves;
-    base = c * PW_CHUNK;
+    nel = (int)(PW_CHUNK / a.sub);
+    base = c * PW_CHUNK + (long long)s * nel;
+    nleaves = nel / 128;
+    leaf_len = 128;
   } else {
-    base = a.nfull * PW_CHUNK;
-    leaf0 = 0;
-    nleaves = plan->nleaves;
+    const PwPlan* tp = &a.plans[1];
+    const int l = (int)(part - nfp);
+    base = a.nfull * PW_CHUNK + tp->leaf_off[l];
+    nel = tp->leaf_len[l];
+    nleaves = 1;
+    leaf_len = nel;
   }
-  for (int li = g; li < nleaves; li += PCG_WG / 8) {
-    const int leaf = leaf0 + li;
-    const int off = plan->leaf_off[leaf], len = plan->leaf_len[leaf];
+  for (int t = tid; t < nel; t += PCG_WG) {
+    const Two v = elem(base + t);
+    lds[t] = v.a;
+    if (NS > 1) lds[PCG_STAGE + t] = v.b;
+  }
+  __syncthreads();
+  const int g = tid >> 3, lane = tid & 7;
+  if (g < nleaves) {
+    const int lo = g * leaf_len;
+    const int len = leaf_len;
     Two res{0.0, 0.0};
-    if (len == 128) {
-      // 16 elements per lane: evaluate them all first so their loads are in
-      // flight together, then accumulate in NumPy's order
-      Two v[16];
-#pragma unroll
-      for (int t = 0; t < 16; ++t) v[t] = elem(base + off + 8 * t + lane);
-      Two acc = v[0];
-#pragma unroll
-      for (int t = 1; t < 16; ++t) {
-        acc.a = acc.a + v[t].a;
-        if (NS > 1) acc.b = acc.b + v[t].b;
-      }
-      for (int m = 1; m < 8; m <<= 1) {
-        acc.a = acc.a + shfl_xor_d(acc.a, m);
-        if (NS > 1) acc.b = acc.b + shfl_xor_d(acc.b, m);
-      }
-      res = acc;
-    } else if (len >= 8) {
+    if (len >= 8) {
       const int fl = len - (len % 8);
-      Two acc = elem(base + off + lane);
+      Two acc{lds[lo + lane], NS > 1 ? lds[PCG_STAGE + lo + lane] : 0.0};
       for (int t = 8; t < fl; t += 8) {
-        const Two v = elem(base + off + t + lane);
-        acc.a = acc.a + v.a;
-        if (NS > 1) acc.b = acc.b + v.b;
+        acc.a = acc.a + lds[lo + t + lane];
+        if (NS > 1) acc.b = acc.b + lds[PCG_STAGE + lo + t + lane];
       }
       for (int m = 1; m < 8; m <<= 1) {
         acc.a = acc.a + shfl_xor_d(acc.a, m);
@@ -270,75 +266,72 @@ __device__ Two part_sum(const PcgArgs& a, int part, F elem, double* lds) {
       res = acc;
       if (lane == 0) {
         for (int t = fl; t < len; ++t) {
-          const Two v = elem(base + off + t);
-          res.a = res.a + v.a;
-          if (NS > 1) res.b = res.b + v.b;
+          res.a = res.a + lds[lo + t];
+          if (NS > 1) res.b = res.b + lds[PCG_STAGE + lo + t];
         }
       }
     } else if (lane == 0) {
       for (int t = 0; t < len; ++t) {
-        const Two v = elem(base + off + t);
-        res.a = res.a + v.a;
-        if (NS > 1) res.b = res.b + v.b;
+        res.a = res.a + lds[lo + t];
+        if (NS > 1) res.b = res.b + lds[PCG_STAGE + lo + t];
       }
     }
     if (lane == 0) {
-      lds[li] = res.a;
-      if (NS > 1) lds[256 + li] = res.b;
+      leafs[g] = res.a;
+      if (NS > 1) leafs[64 + g] = res.b;
     }
   }
   __syncthreads();
-  if (full) {
-    // perfect pairwise tree over this subtree's leaves (a power of two)
-    for (int w = 1; w < nleaves; w <<= 1) {
-      for (int t = tid; t < nleaves / (2 * w); t += PCG_WG) {
-        lds[2 * w * t] = lds[2 * w * t] + lds[2 * w * t + w];
-        if (NS > 1) lds[256 + 2 * w * t] = lds[256 + 2 * w * t] + lds[256 + 2 * w * t + w];
-      }
-      __syncthreads();
+  // perfect pairwise tree over the part's leaves (1, 2, 4, ... leaves)
+  for (int w = 1; w < nleaves; w <<= 1) {
+    if (tid < nleaves / (2 * w)) {
+      leafs[2 * w * tid] = leafs[2 * w * tid] + leafs[2 * w * tid + w];
+      if (NS > 1) leafs[64 + 2 * w * tid] = leafs[64 + 2 * w * tid] + leafs[64 + 2 * w * tid + w];
     }
-    Two out{lds[0], NS > 1 ? lds[256] : 0.0};
-    __syncthreads();
-    return out;
-  }
-  const int L = plan->nleaves;
-  int start = 0;
-  for (int lv = 0; lv < plan->nlevels; ++lv) {
-    const int end = plan->level_end[lv];
-    for (int n = start + tid; n < end; n += PCG_WG) {
-      const int l = plan->node_l[n], r = plan->node_r[n];
-      lds[L + n] = lds[l] + lds[r];
-      if (NS > 1) lds[256 + L + n] = lds[256 + l] + lds[256 + r];
-    }
-    start = end;
     __syncthreads();
   }
-  const int root = plan->nnodes > 0 ? L + plan->nnodes - 1 : 0;
-  Two out{lds[root], NS > 1 ? lds[256 + root] : 0.0};
+  Two out{leafs[0], NS > 1 ? leafs[64] : 0.0};
   __syncthreads();
   return out;
 }
 
 // In-order float64 sum of a whole vector from the per-part partials: each
-// chunk's top tree levels (perfect, over `sub` subtree sums), then the
-// chunk sums added sequentially. Every workgroup computes it identically.
+// full chunk's top tree levels over its `sub` subtree sums, the trailing
+// chunk's tree from its plan over its leaf sums, then the chunk sums added
+// sequentially. Every workgroup computes it identically.
 __device__ double total_from_parts(const PcgArgs& a, const double* part, double* cs) {
   const int tid = threadIdx.x;
-  for (int c = tid; c < a.nchunks; c += PCG_WG) {
-    double v;
-    if (c < a.nfull) {
-      double t[8];
-      for (int s = 0; s < a.sub; ++s) t[s] = part[c * a.sub + s];
-      for (int w = 1; w < a.sub; w <<= 1)
-        for (int s = 0; s + w < a.sub; s += 2 * w) t[s] = t[s] + t[s + w];
-      v = t[0];
-    } else {
-      v = part[a.nfull * a.sub];
+  __shared__ double s_tot;
+  __shared__ double tail_v[256];
+  const long long nfp = a.nfull * a.sub;
+  for (int c = tid; c < a.nfull; c += PCG_WG) {
+    double t[PCG_MAX_SUB];
+#pragma unroll
+    for (int s = 0; s < PCG_MAX_SUB; ++s)
+      if (s < a.sub) t[s] = part[(long long)c * a.sub + s];
+#pragma unroll
+    for (int w = 1; w < PCG_MAX_SUB; w <<= 1)
+#pragma unroll
+      for (int s = 0; s + w < PCG_MAX_SUB; s += 2 * w)
+        if (s + w < a.sub) t[s] = t[s] + t[s + w];
+    cs[c] = t[0];
+  }
+  if (a.tail) {
+    const PwPlan* tp = &a.plans[1];
+    const int L = tp->nleaves;
+    for (int l = tid; l < L; l += PCG_WG) tail_v[l] = part[nfp + l];
+    __syncthreads();
+    int start = 0;
+    for (int lv = 0; lv < tp->nlevels; ++lv) {
+      const int end = tp->level_end[lv];
+      for (int n = start + tid; n < end; n += PCG_WG)
+        tail_v[L + n] = tail_v[tp->node_l[n]] + tail_v[tp->node_r[n]];
+      start = end;
+      __syncthreads();
     }
-    cs[c] = v;
+    if (tid == 0) cs[a.nfull] = tp->nnodes > 0 ? tail_v[L + tp->nnodes - 1] : tail_v[0];
   }
   __syncthreads();
-  __shared__ double s_tot;
   if (tid == 0) {
     double s = 0.0;
     for (int c = 0; c < a.nchunks; ++c) s = s + cs[c];
@@ -347,7 +340,6 @@ __device__ double total_from_parts(const PcgArgs& a, const double* part, double*
   __syncthreads();
   return s_tot;
 }
-
 
 // Per-element work of the A phase: p_new at the five stencil points (z +
 // beta p_old; p itself on the first iteration), A p_new at e, store p_new
@@ -399,7 +391,8 @@ __device__ __forceinline__ int logical_block(int nb) {
 }
 
 __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
-  __shared__ double lds[512];
+  __shared__ double lds[2 * PCG_STAGE];
+  __shared__ double leafs[128];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const unsigned nb = gridDim.x;
   const int part = logical_block((int)nb);
@@ -416,7 +409,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
       a.r[e] = r;
       a.x[e] = 0.0;
       return Two{r * r, 0.0};
-    }, lds);
+    }, lds, leafs);
     if (threadIdx.x == 0) a.part_a[part] = s.a;
   }
   ok = grid_barrier(a.bar, a.err, nb);
@@ -433,7 +426,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
       a.z[e] = z;
       a.pb1[e] = z;
       return Two{a.r[e] * z, 0.0};
-    }, lds);
+    }, lds, leafs);
     if (threadIdx.x == 0) a.part_b[part] = s.a;
     ok = grid_barrier(a.bar, a.err, nb);
     if (ok) rz = total_from_parts(a, a.part_b, cs);
@@ -447,7 +440,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
     {
       const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
         return pcg_apply_elem(a, e, first, bt, pnew, pold);
-      }, lds);
+      }, lds, leafs);
       if (threadIdx.x == 0) a.part_a[part] = s.a;
     }
     ok = grid_barrier(a.bar, a.err, nb);
@@ -457,7 +450,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
       const double al = alpha;
       const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
         return pcg_update_elem(a, e, al, pnew);
-      }, lds);
+      }, lds, leafs);
       if (threadIdx.x == 0) {
         a.part_b[part] = s.a;
         a.part_c[part] = s.b;
@@ -504,20 +497,22 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_persistent(PcgArgs a) {
 
 // r = rhs * mask, x = 0; partial sums of r*r
 __global__ __launch_bounds__(PCG_WG) void k_pcg_init(PcgArgs a) {
-  __shared__ double lds[512];
+  __shared__ double lds[2 * PCG_STAGE];
+  __shared__ double leafs[128];
   const int part = blockIdx.x;
   const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
     const double r = a.rhs[e] * mk(a.op.mask[e]);
     a.r[e] = r;
     a.x[e] = 0.0;
     return Two{r * r, 0.0};
-  }, lds);
+  }, lds, leafs);
   if (threadIdx.x == 0) a.part_a[part] = s.a;
 }
 
 // ||rhs|| (zero: done in 0 iterations); z = M^-1 r, p = z; partials of r*z
 __global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
-  __shared__ double lds[512];
+  __shared__ double lds[2 * PCG_STAGE];
+  __shared__ double leafs[128];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const double nb = sqrt(total_from_parts(a, a.part_a, cs));
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -535,7 +530,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
     a.z[e] = z;
     a.pb1[e] = z;
     return Two{a.r[e] * z, 0.0};
-  }, lds);
+  }, lds, leafs);
   if (threadIdx.x == 0) a.part_b[part] = s.a;
 }
 
@@ -543,7 +538,8 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
 // iteration it-1: the convergence test on ||r||, then beta and the new rz.
 __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
   if (a.sc->done) return;
-  __shared__ double lds[512];
+  __shared__ double lds[2 * PCG_STAGE];
+  __shared__ double leafs[128];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   double beta = 0.0;
@@ -582,7 +578,7 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
   const int part = blockIdx.x;
   const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
     return pcg_apply_elem(a, e, it == 1, beta, pnew, pold);
-  }, lds);
+  }, lds, leafs);
   if (threadIdx.x == 0) a.part_a[part] = s.a;
 }
 
@@ -590,14 +586,15 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
 // partial sums of r*r and r*z.
 __global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a, int it) {
   if (a.sc->done) return;
-  __shared__ double lds[512];
+  __shared__ double lds[2 * PCG_STAGE];
+  __shared__ double leafs[128];
   __shared__ double cs[PCG_MAX_CHUNKS];
   const double al = a.sc->rz2[it & 1] / total_from_parts(a, a.part_a, cs);
   const double* pnew = (it & 1) ? a.pb1 : a.pb0;
   const int part = blockIdx.x;
   const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
     return pcg_update_elem(a, e, al, pnew);
-  }, lds);
+  }, lds, leafs);
   if (threadIdx.x == 0) {
     a.part_b[part] = s.a;
     a.part_c[part] = s.b;

@@ -105,11 +105,11 @@ __global__ void k_seq_sum(const double* part, long long nchunks, double* out);
 
 namespace hoc {
 
-// Persistent PCG (one cooperative launch per solve). Full 8192-element
-// chunks are cut into `sub` aligned subtrees of the pairwise tree, one
-// single-wave workgroup each; a trailing partial chunk, if any, is one more
-// workgroup. Partial sums land in part_*[part]; after a grid barrier every
-// workgroup rebuilds each chunk's top levels and the in-order chunk sum.
+// PCG arguments. Full 8192-element chunks are cut into `sub` aligned
+// subtrees of the pairwise tree, one workgroup each; the trailing partial
+// chunk, if any, adds one workgroup per leaf of its plan. Partial sums land
+// in part_*[part]; the consumer of a sum rebuilds each chunk's top levels
+// and the in-order chunk sum.
 struct PcgArgs {
   Dims d;
   HelmOp op;
@@ -128,7 +128,9 @@ struct PcgArgs {
   long long* counters;
 };
 
-constexpr int PCG_WG = 64;          // one wave per workgroup
+constexpr int PCG_WG = 256;         // threads per workgroup of a PCG phase
+constexpr int PCG_STAGE = 512;      // elements staged per part (8192 / sub)
+constexpr int PCG_MAX_SUB = 16;     // subtrees per full chunk
 constexpr int PCG_MAX_CHUNKS = 1024;
 
 __global__ void k_pcg_persistent(PcgArgs a);

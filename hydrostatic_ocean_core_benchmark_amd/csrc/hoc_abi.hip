@@ -726,9 +726,9 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
   { ProfScope _ps(h, "k_tracers", s);
-  k_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->T, in->S, in->b, out->u, out->v, h->w,
-                                         h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S,
-                                         out->b); }
+  k_tracers_tiled<<<dim3((d.nx + 31) / 32, (d.ny + 7) / 8), 256, 0, s>>>(
+      d, p, geo, in->T, in->S, in->b, out->u, out->v, h->w, h->Kh, h->sub_b, h->den_b,
+      h->cst_b, out->T, out->S, out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;

@@ -95,6 +95,13 @@ __global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double
                                      const double* v, const double* w, double* au,
                                      double* av);
 
+// hoc_tiled.hip
+__global__ void k_tracers_tiled(Dims d, StepParams p, Geo geo, const double* T, const double* S,
+                                const double* b, const double* u, const double* v,
+                                const double* w, const double* Kh, const double* sub,
+                                const double* den, const double* cst, double* T_out,
+                                double* S_out, double* b_out);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

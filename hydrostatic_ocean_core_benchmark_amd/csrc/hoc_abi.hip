@@ -681,9 +681,9 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
-  k_momentum_explicit<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->b, h->phi, h->w,
-                                                   h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu,
-                                                   h->dzv, h->stu, h->stv); }
+  k_momentum_explicit_tiled<<<dim3((d.nx + 31) / 32, (d.ny + 7) / 8), 256, 0, s>>>(
+      d, p, geo, in->u, in->v, in->b, h->phi, h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv,
+      h->dzu, h->dzv, h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
   if (theta) {
     k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);

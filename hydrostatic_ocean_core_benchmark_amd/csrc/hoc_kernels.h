@@ -102,6 +102,12 @@ __global__ void k_tracers_tiled(Dims d, StepParams p, Geo geo, const double* T, 
                                 const double* den, const double* cst, double* T_out,
                                 double* S_out, double* b_out);
 
+__global__ void k_momentum_explicit_tiled(Dims d, StepParams p, Geo geo, const double* u,
+                                          const double* v, const double* b, const double* phi,
+                                          const double* w, const double* Km, double* pgx,
+                                          double* pgy, double* avu, double* avv, double* dzu,
+                                          double* dzv, double* stu, double* stv);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

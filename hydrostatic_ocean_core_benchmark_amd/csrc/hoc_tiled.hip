@@ -233,3 +233,310 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
 }
 
 }  // namespace hoc
+
+namespace hoc {
+
+struct MomLds {
+  double u[SY][SX], v[SY][SX];          // velocities, 2-cell halo
+  uint8_t mu[SY][SX], mv[SY][SX];       // mask3u, mask3v, 2-cell halo
+  double phi[TY + 1][TX + 1];           // potential, +1 east / north
+  double b[TY + 1][TX + 1];
+  double zc[TY + 1][TX + 1];
+  double w[2][TY + 1][TX + 1];          // w at interfaces k, k+1
+};
+
+// Vertical momentum flux on interface kk of the column: wb times the
+// interface value of the advected velocity (advection_v06.py:95-99,
+// advection.py:75-79); f* hold the velocity at kk-2..kk+1, m* the wet flags
+// of kk-2 and kk+1.
+__device__ __forceinline__ double mom_flux_z(int adv, int kk, int nz, double wb, double far_up,
+                                             double above, double below, double far_dn,
+                                             bool mup, bool mdn) {
+  if (kk <= 0 || kk >= nz) return 0.0;
+  if (adv == ADV_CENTRED2) return wb * 0.5 * (above + below);
+  const bool hu = kk >= 2 && mup;
+  const bool hd = kk <= nz - 2 && mdn;
+  return wb * kappa_face_z(kk >= 2 ? far_up : 0.0, above, below, kk <= nz - 2 ? far_dn : 0.0, wb,
+                           hu, hd, adv == ADV_UP3_TVD);
+}
+
+// Explicit vertical diffusion of a face velocity at level k from its column
+// window f[k-1..k+1], thickness window dz[k-1..k+1] and cell-mask window
+// m[k-1..k+1] (libs/core/vertical_var.py:80-106, mask3 branch).
+__device__ __forceinline__ double explicit_diffusion_w(int k, int nz, const double* f,
+                                                       const double* dz, const bool* m,
+                                                       double nu_k, double nu_k1, double sflux,
+                                                       double drag) {
+  const bool bot_km1 = k >= 1 && m[0] && !m[1];
+  const bool bot_k = m[1] && !((k + 1 < nz) && m[2]);
+  double ft;
+  if (k == 0) {
+    ft = sflux;
+  } else {
+    const double dzi = (k == nz) ? dz[0] : 0.5 * (dz[0] + dz[1]);
+    ft = nu_k * (f[0] - f[1]) * safe_inv(dzi);
+    if (bot_km1) ft = 0.0;
+  }
+  double fb = 0.0;
+  if (k + 1 < nz) {
+    const double dzi = 0.5 * (dz[1] + dz[2]);
+    fb = nu_k1 * (f[1] - f[2]) * safe_inv(dzi);
+    if (bot_k) fb = 0.0;
+  }
+  const double inv = safe_inv(dz[1]);
+  const double out = (ft - fb) * inv;
+  const double dr = drag * f[1] * (bot_k ? 1.0 : 0.0);
+  return (out - dr * inv) * (m[1] ? 1.0 : 0.0);
+}
+
+// The step's explicit momentum terms (libs/core/model3d_v05.py:352-370),
+// tile by tile and level by level; same outputs as k_momentum_explicit.
+__global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
+    Dims d, StepParams p, Geo geo, const double* __restrict__ u, const double* __restrict__ v,
+    const double* __restrict__ b, const double* __restrict__ phi, const double* __restrict__ w,
+    const double* __restrict__ Km, double* __restrict__ pgx, double* __restrict__ pgy,
+    double* __restrict__ avu, double* __restrict__ avv, double* __restrict__ dzu,
+    double* __restrict__ dzv, double* __restrict__ stu, double* __restrict__ stv) {
+  __shared__ MomLds L;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const long long n2 = d.n2;
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool active = i < nx && j < ny;
+  const long long c = active ? (long long)j * nx + i : 0;
+  int adv = p.advection;
+  if (adv == ADV_UPWIND1) adv = ADV_CENTRED2;  // momentum keeps the centred form
+  const bool up3 = adv == ADV_UP3 || adv == ADV_UP3_TVD;
+  const bool lim = adv == ADV_UP3_TVD;
+
+  // own-column windows at k-2..k+2: u, v, mask3u, mask3v; and k-1..k+1 of
+  // dz3u, dz3v, mask3
+  double uw5[5], vw5[5];
+  bool muw[5], mvw[5];
+  double duw[3], dvw[3];
+  bool m3w[3];
+  for (int q = 0; q < 5; ++q) {
+    uw5[q] = vw5[q] = 0.0;
+    muw[q] = mvw[q] = false;
+  }
+  for (int q = 0; q < 3; ++q) {
+    duw[q] = dvw[q] = 0.0;
+    m3w[q] = false;
+  }
+  if (active) {
+    for (int q = 2; q < 5; ++q) {
+      const int kk = q - 2;
+      if (kk < nz) {
+        uw5[q] = u[kk * n2 + c];
+        vw5[q] = v[kk * n2 + c];
+        muw[q] = geo.m3u[kk * n2 + c] != 0;
+        mvw[q] = geo.m3v[kk * n2 + c] != 0;
+      }
+    }
+    for (int q = 1; q < 3; ++q) {
+      const int kk = q - 1;
+      if (kk < nz) {
+        duw[q] = geo.dz3u[kk * n2 + c];
+        dvw[q] = geo.dz3v[kk * n2 + c];
+        m3w[q] = geo.m3[kk * n2 + c] != 0;
+      }
+    }
+  }
+  double km_k = active ? Km[c] : 0.0;
+  double fzu_top = 0.0, fzv_top = 0.0;
+  const int cx = tx + HALO, cy = ty + HALO;
+
+  for (int k = 0; k < nz; ++k) {
+    const long long o = (long long)k * n2;
+    __syncthreads();
+    for (int t = tid; t < SX * SY; t += TILE_THREADS) {
+      const int lx = t % SX, ly = t / SX;
+      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
+      L.u[ly][lx] = u[o + g];
+      L.v[ly][lx] = v[o + g];
+      L.mu[ly][lx] = geo.m3u[o + g];
+      L.mv[ly][lx] = geo.m3v[o + g];
+    }
+    for (int t = tid; t < (TY + 1) * (TX + 1); t += TILE_THREADS) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx, nx);
+      L.phi[ly][lx] = phi[o + g];
+      L.b[ly][lx] = b[o + g];
+      L.zc[ly][lx] = geo.zc[o + g];
+      L.w[0][ly][lx] = w[o + g];
+      L.w[1][ly][lx] = w[o + n2 + g];
+    }
+    __syncthreads();
+    if (active) {
+      const long long t = o + c;
+      const double mu_c = mk(L.mu[cy][cx]), mv_c = mk(L.mv[cy][cx]);
+      // pressure gradient at the face depth (model3d_v05.py:195-230)
+      double gx, gy;
+      if (p.pgf_correction) {
+        const double zc = L.zc[ty][tx];
+        {
+          const double zr = L.zc[ty][tx + 1];
+          const double zf = 0.5 * (zc + zr);
+          const double left = L.phi[ty][tx] + L.b[ty][tx] * (zf - zc);
+          const double right = L.phi[ty][tx + 1] + L.b[ty][tx + 1] * (zf - zr);
+          gx = mu_c * (right - left) / p.dx;
+        }
+        {
+          const double zr = L.zc[ty + 1][tx];
+          const double zf = 0.5 * (zc + zr);
+          const double left = L.phi[ty][tx] + L.b[ty][tx] * (zf - zc);
+          const double right = L.phi[ty + 1][tx] + L.b[ty + 1][tx] * (zf - zr);
+          gy = mv_c * (right - left) / p.dy;
+        }
+      } else {
+        gx = mu_c * (L.phi[ty][tx + 1] - L.phi[ty][tx]) / p.dx * mu_c;
+        gy = mv_c * (L.phi[ty + 1][tx] - L.phi[ty][tx]) / p.dy * mv_c;
+      }
+      pgx[t] = gx;
+      pgy[t] = gy;
+      // Coriolis partners (operators_masked.py:45-61)
+      {
+        const double w0 = mv_c, w1 = mk(L.mv[cy - 1][cx]), w2 = mk(L.mv[cy][cx + 1]),
+                     w3 = mk(L.mv[cy - 1][cx + 1]);
+        const double num = 0.0 + w0 * L.v[cy][cx] + w1 * L.v[cy - 1][cx] + w2 * L.v[cy][cx + 1] +
+                           w3 * L.v[cy - 1][cx + 1];
+        const double den = 0.0 + w0 + w1 + w2 + w3;
+        avv[t] = (den > 0.0 ? num / den : 0.0) * mu_c;
+      }
+      {
+        const double w0 = mu_c, w1 = mk(L.mu[cy][cx - 1]), w2 = mk(L.mu[cy + 1][cx]),
+                     w3 = mk(L.mu[cy + 1][cx - 1]);
+        const double num = 0.0 + w0 * L.u[cy][cx] + w1 * L.u[cy][cx - 1] + w2 * L.u[cy + 1][cx] +
+                           w3 * L.u[cy + 1][cx - 1];
+        const double den = 0.0 + w0 + w1 + w2 + w3;
+        avu[t] = (den > 0.0 ? num / den : 0.0) * mv_c;
+      }
+      // explicit vertical diffusion with surface stress and bottom drag
+      const double km_k1 = Km[o + n2 + c];
+      dzu[t] = explicit_diffusion_w(k, nz, &uw5[1], duw, m3w, km_k, km_k1,
+                                    k == 0 ? geo.tau_u[c] : 0.0, p.bottom_drag);
+      dzv[t] = explicit_diffusion_w(k, nz, &vw5[1], dvw, m3w, km_k, km_k1,
+                                    k == 0 ? geo.tau_v[c] : 0.0, p.bottom_drag);
+      // advection and horizontal viscosity
+      double exu = 0.0, exv = 0.0;
+      if (adv != ADV_NONE) {
+        double ax, ay, bx, by;
+        const double(*U)[SX] = L.u;
+        const double(*V)[SX] = L.v;
+        const uint8_t(*MU)[SX] = L.mu;
+        const uint8_t(*MV)[SX] = L.mv;
+        if (adv == ADV_CENTRED2) {
+          const double ub = 0.5 * (U[cy][cx - 1] + U[cy][cx]);
+          const double ube = 0.5 * (U[cy][cx] + U[cy][cx + 1]);
+          ax = (ube * ube - ub * ub) / p.dx;
+          const double vb = 0.5 * (V[cy][cx] + V[cy][cx + 1]);
+          const double vbs = 0.5 * (V[cy - 1][cx] + V[cy - 1][cx + 1]);
+          ay = (vb * (0.5 * (U[cy][cx] + U[cy + 1][cx])) -
+                vbs * (0.5 * (U[cy - 1][cx] + U[cy][cx]))) / p.dy;
+          const double vv = 0.5 * (V[cy - 1][cx] + V[cy][cx]);
+          const double vvn = 0.5 * (V[cy][cx] + V[cy + 1][cx]);
+          by = (vvn * vvn - vv * vv) / p.dy;
+          const double uu = 0.5 * (U[cy][cx] + U[cy + 1][cx]);
+          const double uuw = 0.5 * (U[cy][cx - 1] + U[cy + 1][cx - 1]);
+          bx = (uu * (0.5 * (V[cy][cx] + V[cy][cx + 1])) -
+                uuw * (0.5 * (V[cy][cx - 1] + V[cy][cx]))) / p.dx;
+        } else {
+          const double ub = 0.5 * (U[cy][cx - 1] + U[cy][cx]);
+          const double ube = 0.5 * (U[cy][cx] + U[cy][cx + 1]);
+          const double uc = kappa_face(U[cy][cx - 2], U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], ub,
+                                       MU[cy][cx - 2] != 0, MU[cy][cx + 1] != 0, lim);
+          const double uce = kappa_face(U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], U[cy][cx + 2],
+                                        ube, MU[cy][cx - 1] != 0, MU[cy][cx + 2] != 0, lim);
+          ax = (ube * uce - ub * uc) / p.dx;
+          const double vb = 0.5 * (V[cy][cx] + V[cy][cx + 1]);
+          const double vbs = 0.5 * (V[cy - 1][cx] + V[cy - 1][cx + 1]);
+          const double uy = kappa_face(U[cy - 1][cx], U[cy][cx], U[cy + 1][cx], U[cy + 2][cx], vb,
+                                       MU[cy - 1][cx] != 0, MU[cy + 2][cx] != 0, lim);
+          const double uys = kappa_face(U[cy - 2][cx], U[cy - 1][cx], U[cy][cx], U[cy + 1][cx],
+                                        vbs, MU[cy - 2][cx] != 0, MU[cy + 1][cx] != 0, lim);
+          ay = (vb * uy - vbs * uys) / p.dy;
+          const double vv = 0.5 * (V[cy - 1][cx] + V[cy][cx]);
+          const double vvn = 0.5 * (V[cy][cx] + V[cy + 1][cx]);
+          const double vc = kappa_face(V[cy - 2][cx], V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], vv,
+                                       MV[cy - 2][cx] != 0, MV[cy + 1][cx] != 0, lim);
+          const double vcn = kappa_face(V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], V[cy + 2][cx],
+                                        vvn, MV[cy - 1][cx] != 0, MV[cy + 2][cx] != 0, lim);
+          by = (vvn * vcn - vv * vc) / p.dy;
+          const double uu = 0.5 * (U[cy][cx] + U[cy + 1][cx]);
+          const double uuw = 0.5 * (U[cy][cx - 1] + U[cy + 1][cx - 1]);
+          const double vx = kappa_face(V[cy][cx - 1], V[cy][cx], V[cy][cx + 1], V[cy][cx + 2], uu,
+                                       MV[cy][cx - 1] != 0, MV[cy][cx + 2] != 0, lim);
+          const double vxw = kappa_face(V[cy][cx - 2], V[cy][cx - 1], V[cy][cx], V[cy][cx + 1],
+                                        uuw, MV[cy][cx - 2] != 0, MV[cy][cx + 1] != 0, lim);
+          bx = (uu * vx - uuw * vxw) / p.dx;
+        }
+        // vertical fluxes: the top one is the bottom one of the level above
+        const double wbu = 0.5 * (L.w[1][ty][tx] + L.w[1][ty][tx + 1]);
+        const double wbv = 0.5 * (L.w[1][ty][tx] + L.w[1][ty + 1][tx]);
+        const double fzu_bot = mom_flux_z(adv, k + 1, nz, wbu, uw5[1], uw5[2], uw5[3], uw5[4],
+                                          muw[1], muw[4]);
+        const double fzv_bot = mom_flux_z(adv, k + 1, nz, wbv, vw5[1], vw5[2], vw5[3], vw5[4],
+                                          mvw[1], mvw[4]);
+        const double az = (fzu_top - fzu_bot) / p.dz_nom;
+        const double bz = (fzv_top - fzv_bot) / p.dz_nom;
+        fzu_top = fzu_bot;
+        fzv_top = fzv_bot;
+        (void)up3;
+        exu = exu - (ax + ay + az) * mu_c;
+        exv = exv - (bx + by + bz) * mv_c;
+      }
+      if (p.has_A_h) {
+        const double(*U)[SX] = L.u;
+        const double(*V)[SX] = L.v;
+        {
+          const double g1 = mk(L.mu[cy][cx]) * (U[cy][cx + 1] - U[cy][cx]) / p.dx;
+          const double g0 = mk(L.mu[cy][cx - 1]) * (U[cy][cx] - U[cy][cx - 1]) / p.dx;
+          const double h1 = mk(L.mv[cy][cx]) * (U[cy + 1][cx] - U[cy][cx]) / p.dy;
+          const double h0 = mk(L.mv[cy - 1][cx]) * (U[cy][cx] - U[cy - 1][cx]) / p.dy;
+          exu = exu + p.A_h * (mu_c * ((g1 - g0) / p.dx + (h1 - h0) / p.dy));
+        }
+        {
+          const double g1 = mk(L.mu[cy][cx]) * (V[cy][cx + 1] - V[cy][cx]) / p.dx;
+          const double g0 = mk(L.mu[cy][cx - 1]) * (V[cy][cx] - V[cy][cx - 1]) / p.dx;
+          const double h1 = mk(L.mv[cy][cx]) * (V[cy + 1][cx] - V[cy][cx]) / p.dy;
+          const double h0 = mk(L.mv[cy - 1][cx]) * (V[cy][cx] - V[cy - 1][cx]) / p.dy;
+          exv = exv + p.A_h * (mv_c * ((g1 - g0) / p.dx + (h1 - h0) / p.dy));
+        }
+      }
+      double s0u = 0.0, s0v = 0.0;
+      if (k == 0) {
+        s0u = p.thv_dt * geo.tau_u[c] * safe_inv(duw[1]);
+        s0v = p.thv_dt * geo.tau_v[c] * safe_inv(dvw[1]);
+      }
+      stu[t] = s0u + p.dt * exu;
+      stv[t] = s0v + p.dt * exv;
+      km_k = km_k1;
+    }
+    // advance the windows
+    for (int q = 0; q < 4; ++q) {
+      uw5[q] = uw5[q + 1];
+      vw5[q] = vw5[q + 1];
+      muw[q] = muw[q + 1];
+      mvw[q] = mvw[q + 1];
+    }
+    for (int q = 0; q < 2; ++q) {
+      duw[q] = duw[q + 1];
+      dvw[q] = dvw[q + 1];
+      m3w[q] = m3w[q + 1];
+    }
+    if (active) {
+      const int k3 = k + 3, k2 = k + 2;
+      uw5[4] = k3 < nz ? u[(long long)k3 * n2 + c] : 0.0;
+      vw5[4] = k3 < nz ? v[(long long)k3 * n2 + c] : 0.0;
+      muw[4] = k3 < nz && geo.m3u[(long long)k3 * n2 + c] != 0;
+      mvw[4] = k3 < nz && geo.m3v[(long long)k3 * n2 + c] != 0;
+      duw[2] = k2 < nz ? geo.dz3u[(long long)k2 * n2 + c] : 0.0;
+      dvw[2] = k2 < nz ? geo.dz3v[(long long)k2 * n2 + c] : 0.0;
+      m3w[2] = k2 < nz && geo.m3[(long long)k2 * n2 + c] != 0;
+    }
+  }
+}
+
+}  // namespace hoc

@@ -61,6 +61,9 @@ def parse():
                     help="kernel timed with HIP events for the roofline figure")
     ap.add_argument("--solver-path", choices=("multikernel", "persistent"), default="multikernel",
                     help="Helmholtz launch strategy (bitwise the same iterates)")
+    ap.add_argument("--solver", default=None,
+                    choices=("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid", "pcg_mg"),
+                    help="Helmholtz solver kind (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -107,6 +110,8 @@ def main() -> int:
         print(json.dumps({"error": "no HIP device"}))
         return 2
     prob = basin(nx=args.nx, nz=args.nz)
+    if args.solver:
+        prob.params.solver = args.solver
     g = prob.domain.grid
     st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
     st.solver_path = args.solver_path

@@ -21,7 +21,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhoc.so"
-SOURCES = ("hoc_abi.hip", "hoc_step_kernels.hip", "hoc_tiled.hip", "hoc_helmholtz.hip")
+SOURCES = ("hoc_abi.hip", "hoc_step_kernels.hip", "hoc_tiled.hip", "hoc_helmholtz.hip", "hoc_mg.hip")
 ARCH = os.environ.get("HOC_OFFLOAD_ARCH", "gfx950")
 
 

@@ -160,6 +160,13 @@ struct hoc_stepper {
   int* host_flag = nullptr;     // pinned: done, iters
   Reducer red;
   int last_iters = 8;
+  // multigrid hierarchy (level 0 = the model grid)
+  static constexpr int MAXLEV = 12;
+  int nlev = 1;
+  MgLevel lev[MAXLEV];
+  double *mg_Ku[MAXLEV] = {}, *mg_Kv[MAXLEV] = {}, *mg_inv[MAXLEV] = {};
+  uint8_t* mg_mask[MAXLEV] = {};
+  double *mg_x[MAXLEV] = {}, *mg_t[MAXLEV] = {}, *mg_b[MAXLEV] = {};
   bool force_multikernel = true;   // per-phase launches unless path 0 is asked for
   bool coeffs_built = false;
   // host counters
@@ -270,8 +277,11 @@ int validate(const hoc_config& c) {
   if (c.advection < 0 || c.advection > 4) return fail(HOC_ERR_INVALID, "unknown advection");
   if (c.eos < 0 || c.eos > 2)
     return fail(HOC_ERR_INVALID, "unknown eos (expected linear|seos|teos10)");
-  if (c.scheme == 0 && c.solver != SOLVER_PCG_JACOBI)
-    return fail(HOC_ERR_UNSUPPORTED, "this build solves the Helmholtz problem with pcg_jacobi");
+  if (c.scheme == 0 && (c.solver < SOLVER_PCG_JACOBI || c.solver > SOLVER_PCG_MG))
+    return fail(HOC_ERR_INVALID, "unknown variable-coefficient solver "
+                                 "(expected pcg_jacobi|pcg_rbgs|rbgs|multigrid|pcg_mg)");
+  if (c.solver == SOLVER_PCG_MG && (c.mg_sweeps < 1 || c.mg_levels < 1 || !(c.mg_omega > 0.0)))
+    return fail(HOC_ERR_INVALID, "pcg_mg needs mg_levels >= 1, mg_sweeps >= 1, mg_omega > 0");
   if (c.max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
   return HOC_OK;
 }
@@ -392,6 +402,92 @@ int pcg_finish(hoc_stepper* h, hipStream_t s, bool persistent, int* iters_out, i
   return HOC_OK;
 }
 
+// Level operators for the face coefficients Ku, Kv (level 0 given, the
+// others agglomerated) and their diagonals.
+int mg_build(hoc_stepper* h, const double* Ku, const double* Kv, const double* inv0,
+             hipStream_t s) {
+  MgLevel& L0 = h->lev[0];
+  L0.Ku = Ku;
+  L0.Kv = Kv;
+  L0.mask = h->m2;
+  L0.inv_diag = inv0;
+  for (int l = 1; l < h->nlev; ++l) {
+    MgLevel& L = h->lev[l];
+    L.Ku = h->mg_Ku[l];
+    L.Kv = h->mg_Kv[l];
+    L.mask = h->mg_mask[l];
+    L.inv_diag = h->mg_inv[l];
+    k_mg_coarsen<<<blocks(L.n), TPB, 0, s>>>(h->lev[l - 1], L, h->mg_Ku[l], h->mg_Kv[l],
+                                             h->mg_mask[l]);
+    k_mg_diag<<<blocks(L.n), TPB, 0, s>>>(L, h->mg_inv[l]);
+  }
+  LAUNCH_CHECK("multigrid levels");
+  return HOC_OK;
+}
+
+// One red-black sweep on x (forward: red then black; reverse: black then
+// red), through the level's spare buffer t; the result is back in x.
+void rbgs_sweep(hoc_stepper* h, int l, const double* b, double* x, bool reverse, hipStream_t s) {
+  const MgLevel& L = h->lev[l];
+  double* t = h->mg_t[l];
+  const int c0 = reverse ? 1 : 0;
+  k_mg_rbgs<<<blocks(L.n), TPB, 0, s>>>(L, c0, 0, b, x, t, h->sc);
+  k_mg_rbgs<<<blocks(L.n), TPB, 0, s>>>(L, 1 - c0, 1, b, t, x, h->sc);
+}
+
+// The reference multigrid V-cycle on x (solvers_var.py:139-151): n_pre
+// forward RBGS sweeps, coarse correction, n_post reverse sweeps.
+void vcycle_rbgs(hoc_stepper* h, int l, double* x, const double* b, hipStream_t s) {
+  for (int k = 0; k < 2; ++k) rbgs_sweep(h, l, b, x, false, s);
+  if (l + 1 < h->nlev) {
+    const MgLevel& C = h->lev[l + 1];
+    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(h->lev[l], C, x, b, h->mg_b[l + 1], h->sc);
+    k_mg_zero<<<blocks(C.n), TPB, 0, s>>>(C.n, h->mg_x[l + 1], h->sc);
+    vcycle_rbgs(h, l + 1, h->mg_x[l + 1], h->mg_b[l + 1], s);
+    k_mg_prolong<<<blocks(h->lev[l].n), TPB, 0, s>>>(h->lev[l], h->mg_x[l + 1], C.nx, x, h->sc);
+  }
+  for (int k = 0; k < 2; ++k) rbgs_sweep(h, l, b, x, true, s);
+}
+
+// pcg_mg preconditioner: one damped-Jacobi V-cycle from a zero guess
+// (the oracle's _mg_precond). Returns the buffer holding the result.
+double* vcycle_jacobi(hoc_stepper* h, int l, const double* b, double* x, double* t, hipStream_t s) {
+  const MgLevel& L = h->lev[l];
+  const double om = h->cfg.mg_omega;
+  k_mg_zero<<<blocks(L.n), TPB, 0, s>>>(L.n, x, h->sc);
+  for (int k = 0; k < h->cfg.mg_sweeps; ++k) {
+    k_mg_jacobi<<<blocks(L.n), TPB, 0, s>>>(L, om, x, b, t, h->sc);
+    std::swap(x, t);
+  }
+  if (l + 1 < h->nlev) {
+    const MgLevel& C = h->lev[l + 1];
+    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(L, C, x, b, h->mg_b[l + 1], h->sc);
+    double* xc = vcycle_jacobi(h, l + 1, h->mg_b[l + 1], h->mg_x[l + 1], h->mg_t[l + 1], s);
+    k_mg_prolong<<<blocks(L.n), TPB, 0, s>>>(L, xc, C.nx, x, h->sc);
+  }
+  for (int k = 0; k < h->cfg.mg_sweeps; ++k) {
+    k_mg_jacobi<<<blocks(L.n), TPB, 0, s>>>(L, om, x, b, t, h->sc);
+    std::swap(x, t);
+  }
+  return x;
+}
+
+// z = M^-1 r for pcg_rbgs (symmetric RBGS sweep pair from zero,
+// solvers_var.py:154-160) and pcg_mg.
+int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
+  if (h->cfg.solver == SOLVER_PCG_RBGS) {
+    k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
+    rbgs_sweep(h, 0, r, z, false, s);
+    rbgs_sweep(h, 0, r, z, true, s);
+  } else {
+    // 2 * mg_sweeps buffer swaps per level: the result is back in z
+    double* out = vcycle_jacobi(h, 0, r, z, h->mg_t[0], s);
+    if (out != z) return fail(HOC_ERR_DEVICE, "pcg_mg: V-cycle result left the z buffer");
+  }
+  LAUNCH_CHECK("preconditioner");
+  return HOC_OK;
+}
+
 // PCG-Jacobi on A = I - coef D[K G .]; x is the output (device). Path 0: one
 // cooperative launch; path 1: one launch per phase, the host polling the
 // device `done` flag between batches of iterations.
@@ -402,6 +498,58 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   PcgArgs a = pcg_args(h, op, inv_diag, rhs, x);
   const unsigned np = (unsigned)h->pcg_parts_all;
   if (np == 0) return fail(HOC_ERR_INVALID, "empty grid");
+  const int kind = h->cfg.solver;
+  if (kind != SOLVER_PCG_JACOBI) {
+    TRY(mg_build(h, Ku, Kv, inv_diag, s));
+    const int max_iter = h->p.max_iter;
+    k_pcg_init<<<np, PCG_WG, 0, s>>>(a);   // r = rhs * mask, x = 0, ||rhs||^2 parts
+    k_mg_start<<<1, PCG_WG, 0, s>>>(a);
+    LAUNCH_CHECK("solver init");
+    if (kind == SOLVER_RBGS || kind == SOLVER_MULTIGRID) {
+      // stationary iteration on x with the masked rhs (a.r); ||rhs - A x||
+      PcgArgs ar = a;
+      ar.rhs = h->r;
+      int it = 1;
+      int batch = std::max(1, std::min(h->last_iters, max_iter));
+      while (true) {
+        for (int b = 0; b < batch && it <= max_iter; ++b, ++it) {
+          if (kind == SOLVER_MULTIGRID) vcycle_rbgs(h, 0, x, h->r, s);
+          else rbgs_sweep(h, 0, h->r, x, false, s);
+          k_mg_res_parts<<<np, PCG_WG, 0, s>>>(ar, h->lev[0], x);
+          k_mg_check<<<1, PCG_WG, 0, s>>>(ar, it);
+        }
+        LAUNCH_CHECK("stationary iteration");
+        HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (h->host_flag[0]) break;
+        if (it > max_iter) return fail(HOC_ERR_DEVICE, "solver did not report completion");
+        batch = std::min(2 * batch, 64);  // slow stationary convergence: fewer polls
+      }
+      return pcg_finish(h, s, false, iters_out, conv_out);
+    }
+    // PCG with the RBGS or multigrid preconditioner
+    TRY(precondition(h, h->r, h->z, s));
+    k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 1);
+    int it = 1;
+    int batch = std::max(1, std::min(h->last_iters + 1, max_iter + 1));
+    while (true) {
+      for (int b = 0; b < batch && it <= max_iter + 1; ++b, ++it) {
+        { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it); }
+        if (it <= max_iter) {
+          { ProfScope _ps(h, "k_pcg_update", s); k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 0); }
+          TRY(precondition(h, h->r, h->z, s));
+          k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 0);
+        }
+      }
+      LAUNCH_CHECK("pcg iteration");
+      HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (h->host_flag[0]) break;
+      if (it > max_iter + 1) return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
+      batch = 4;
+    }
+    return pcg_finish(h, s, false, iters_out, conv_out);
+  }
   if (h->pcg_parts > 0 && !h->force_multikernel) {
     a.sc = h->sc;
     void* args[] = {&a};
@@ -423,7 +571,7 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
       { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it); }
       if (it <= max_iter) {
         ProfScope _ps(h, "k_pcg_update", s);
-        k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it);
+        k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 1);
       }
     }
     LAUNCH_CHECK("pcg iteration");
@@ -573,6 +721,54 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     }
     if (hipHostMalloc((void**)&h->host_flag, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess)
       return cleanup(fail(HOC_ERR_DEVICE, "hipHostMalloc failed"));
+  }
+  {
+    // multigrid levels: the reference's rule for `multigrid` (halve while
+    // both sides are even and stay >= 8), mg_levels for `pcg_mg`
+    int nx = d.nx, ny = d.ny, nl = 1;
+    if (cfg->solver == SOLVER_MULTIGRID) {
+      while (nl < hoc_stepper::MAXLEV && nx % 2 == 0 && ny % 2 == 0 && nx / 2 >= 8 && ny / 2 >= 8) {
+        nx /= 2;
+        ny /= 2;
+        ++nl;
+      }
+    } else if (cfg->solver == SOLVER_PCG_MG) {
+      while (nl < std::min(cfg->mg_levels, hoc_stepper::MAXLEV) && nx % 2 == 0 && ny % 2 == 0 &&
+             nx >= 4 && ny >= 4) {
+        nx /= 2;
+        ny /= 2;
+        ++nl;
+      }
+    }
+    h->nlev = nl;
+    int lx = d.nx, ly = d.ny;
+    double ldx = cfg->dx, ldy = cfg->dy;
+    for (int l = 0; l < nl; ++l) {
+      MgLevel& L = h->lev[l];
+      L.nx = lx;
+      L.ny = ly;
+      L.n = (long long)lx * ly;
+      L.coef = h->p.helm_coef;
+      L.dx2 = std::pow(ldx, 2.0);
+      L.dy2 = std::pow(ldy, 2.0);
+      L.cx = L.coef * (1.0 / L.dx2);
+      L.cy = L.coef * (1.0 / L.dy2);
+      int arc = HOC_OK;
+      if (l > 0) {
+        arc = arc ? arc : h->alloc(&h->mg_Ku[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mg_Kv[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mg_inv[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mg_mask[l], L.n);
+      }
+      arc = arc ? arc : h->alloc(&h->mg_x[l], L.n);
+      arc = arc ? arc : h->alloc(&h->mg_t[l], L.n);
+      arc = arc ? arc : h->alloc(&h->mg_b[l], L.n);
+      if (arc != HOC_OK) return cleanup(arc);
+      lx /= 2;
+      ly /= 2;
+      ldx = ldx * 2.0;
+      ldy = ldy * 2.0;
+    }
   }
   int rc = HOC_OK;
   rc = rc ? rc : upload(h->m2, g->mask, d.n2, "mask");

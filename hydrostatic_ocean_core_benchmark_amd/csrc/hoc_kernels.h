@@ -146,10 +146,35 @@ constexpr int PCG_STAGE = 512;      // elements staged per part (8192 / sub)
 constexpr int PCG_MAX_SUB = 16;     // subtrees per full chunk
 constexpr int PCG_MAX_CHUNKS = 1024;
 
+// One level of the multigrid hierarchy (level 0 is the model grid).
+struct MgLevel {
+  int nx, ny;
+  long long n;
+  const double *Ku, *Kv;
+  const uint8_t* mask;
+  const double* inv_diag;
+  double coef, cx, cy, dx2, dy2;
+};
+
 __global__ void k_pcg_persistent(PcgArgs a);
+__global__ void k_pcg_rz(PcgArgs a, int into_b);
+__global__ void k_mg_coarsen(MgLevel f, MgLevel c, double* cKu, double* cKv, uint8_t* cm);
+__global__ void k_mg_diag(MgLevel L, double* inv_diag);
+__global__ void k_mg_zero(long long n, double* x, const PcgScalars* sc);
+__global__ void k_mg_rbgs(MgLevel L, int colour, int last, const double* b, const double* xin,
+                          double* xout, const PcgScalars* sc);
+__global__ void k_mg_jacobi(MgLevel L, double omega, const double* x, const double* b, double* xo,
+                            const PcgScalars* sc);
+__global__ void k_mg_residual_restrict(MgLevel f, MgLevel c, const double* x, const double* b,
+                                       double* bc, const PcgScalars* sc);
+__global__ void k_mg_prolong(MgLevel f, const double* xc, int cnx, double* x,
+                             const PcgScalars* sc);
+__global__ void k_mg_res_parts(PcgArgs a, MgLevel L, const double* x);
+__global__ void k_mg_check(PcgArgs a, int it);
+__global__ void k_mg_start(PcgArgs a);
 __global__ void k_pcg_init(PcgArgs a);
 __global__ void k_pcg_init_z(PcgArgs a);
 __global__ void k_pcg_ap(PcgArgs a, int it);
-__global__ void k_pcg_update(PcgArgs a, int it);
+__global__ void k_pcg_update(PcgArgs a, int it, int jacobi);
 
 }  // namespace hoc

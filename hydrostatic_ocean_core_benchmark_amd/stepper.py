@@ -61,9 +61,9 @@ class HipStepper:
                 "solver 'fft' needs a flat bottom and periodic boundaries; "
                 "with topography the Helmholtz operator is variable-coefficient "
                 "(spec S10.5). Use pcg_jacobi | pcg_rbgs | rbgs | multigrid.")
-        if params.name == "theta" and params.solver != "pcg_jacobi":
-            raise NotImplementedError(f"solver '{params.solver}' is not implemented on HIP "
-                                      "(pcg_jacobi is)")
+        if params.name == "theta" and params.solver not in SOLVER:
+            raise ValueError(f"unknown variable-coefficient solver '{params.solver}' "
+                             "(expected pcg_jacobi|pcg_rbgs|rbgs|multigrid|pcg_mg)")
         if physics.eos not in EOS:
             raise ValueError(f"unknown eos '{physics.eos}' (expected linear|seos|teos10)")
         if physics.closure == "tke" and physics.mxl not in ("integral", "recursive"):
@@ -91,6 +91,9 @@ class HipStepper:
         cfg.advection = ADVECTION[params.advection]
         cfg.solver = SOLVER.get(params.solver, 0)
         cfg.rtol, cfg.max_iter = params.rtol, params.max_iter
+        cfg.mg_levels = int(getattr(params, "mg_levels", 5))
+        cfg.mg_sweeps = int(getattr(params, "mg_sweeps", 2))
+        cfg.mg_omega = float(getattr(params, "mg_omega", 0.8))
         cfg.g, cfg.f, cfg.rho0 = p.g, p.f, p.rho0
         cfg.nu_b, cfg.bottom_drag, cfg.N2 = p.nu, p.bottom_drag, p.N2
         cfg.tracers_ts = 1 if p.tracers == "TS" else 0
@@ -340,7 +343,7 @@ class HipStepper:
         return px, py
 
     def helmholtz_solve(self, Ku, Kv, rhs):
-        """HelmholtzVar(kind=pcg_jacobi).solve on the given face coefficients."""
+        """HelmholtzVar(kind=params.solver).solve on the given face coefficients."""
         Ku, Kv, rhs = (self._dev(x) for x in (Ku, Kv, rhs))
         x = torch.empty_like(rhs)
         it, cv = C.c_int32(), C.c_int32()

@@ -42,9 +42,11 @@ typedef struct hoc_config {
   double theta, theta_cor, theta_v;
   int32_t n_picard;
   int32_t advection;       /* 0 none, 1 centered2, 2 upwind1, 3 up3, 4 up3_tvd */
-  int32_t solver;          /* 0 pcg_jacobi (others: HOC_ERR_UNSUPPORTED for now) */
+  int32_t solver;          /* 0 pcg_jacobi, 1 pcg_rbgs, 2 rbgs, 3 multigrid, 4 pcg_mg */
   double rtol;
   int32_t max_iter;
+  int32_t mg_levels, mg_sweeps;  /* pcg_mg: levels and damped-Jacobi sweeps per side */
+  double mg_omega;               /* pcg_mg: Jacobi damping */
   double g, f, rho0, nu_b, bottom_drag, N2;
   int32_t tracers_ts;      /* 1: T, S prognostic and b diagnosed; 0: b prognostic */
   double alpha_T, beta_S, T0, S0, cp, A_h, K_h, q_heat, q_salt;

@@ -149,7 +149,10 @@ def test_tracer_advection_matches_oracle(hoc, scheme):
     assert np.array_equal(_np(got), want)
 
 
-def test_helmholtz_pcg_matches_reference(hoc):
+@pytest.mark.parametrize("kind", ["pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid"])
+def test_helmholtz_matches_reference(hoc, kind):
+    """Every reference solver kind on the fixture's 24 x 16 island problem:
+    the reference's solution bit for bit and its iteration count."""
     from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
     from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
     f = load_golden("operators.npz")
@@ -157,12 +160,54 @@ def test_helmholtz_pcg_matches_reference(hoc):
     dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="periodic")
     # coef = g theta^2 dt^2 with theta = 0.5, dt = 600 (the fixture's coef)
     phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0))
-    params = SchemeParams(name="theta", solver="pcg_jacobi", rtol=1e-11, max_iter=5000)
+    params = SchemeParams(name="theta", solver=kind, rtol=1e-11, max_iter=5000)
     st = hoc.HipStepper(dom, phys, params, 600.0)
     assert st.cfg.g * 0.25 * 600.0 ** 2 == float(f["helm_coef"])
     x, iters, conv = st.helmholtz_solve(f["helm_Ku"], f["helm_Kv"], f["helm_rhs"])
-    assert conv and iters == int(f["helm_pcg_jacobi_iters"])
-    assert np.array_equal(_np(x), f["helm_pcg_jacobi_x"])
+    assert conv and iters == int(f[f"helm_{kind}_iters"])
+    assert np.array_equal(_np(x), f[f"helm_{kind}_x"])
+
+
+def _random_helmholtz(shape, bc, seed=7):
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    ny, nx = shape
+    rng = np.random.default_rng(seed)
+    grid = CGrid(nx=nx, ny=ny, Lx=nx * 1.0e4, Ly=ny * 1.0e4, nz=4, depth=4000.0)
+    mask = (rng.random(grid.shape) > 0.1).astype(np.float64)
+    dom = Domain(grid=grid, H=np.full(grid.shape, 4000.0), mask=mask, bc_x=bc,
+                 bc_y=bc)
+    Ku = (1000.0 + 3000.0 * rng.random(grid.shape)) * dom.mask_u
+    Kv = (1000.0 + 3000.0 * rng.random(grid.shape)) * dom.mask_v
+    rhs = rng.standard_normal(grid.shape) * mask
+    return grid, dom, mask, Ku, Kv, rhs
+
+
+@pytest.mark.parametrize("kind,shape,bc", [
+    ("pcg_rbgs", (300, 500), "closed"), ("pcg_rbgs", (25, 33), "periodic"),
+    ("rbgs", (40, 24), "closed"), ("rbgs", (15, 21), "periodic"),
+    ("multigrid", (300, 500), "closed"), ("multigrid", (100, 60), "periodic"),
+    ("pcg_mg", (512, 512), "closed"), ("pcg_mg", (300, 500), "periodic"),
+    ("pcg_mg", (16, 24), "periodic"), ("pcg_mg", (13, 24), "closed")])
+def test_helmholtz_kinds_match_oracle(hoc, kind, shape, bc):
+    """Each solver kind against the oracle, bitwise, iteration counts equal:
+    odd sides (no coarsening, or coarse levels with odd sides and periodic
+    wrap, where red-black neighbours of one colour meet), several chunks with
+    and without a trailing partial chunk."""
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    from oracle.helmholtz import HelmholtzVar
+    grid, dom, mask, Ku, Kv, rhs = _random_helmholtz(shape, bc)
+    ny, nx = shape
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=4000.0))
+    rtol = 1e-8 if kind == "rbgs" else 1e-12
+    params = SchemeParams(name="theta", solver=kind, rtol=rtol, max_iter=5000)
+    st = hoc.HipStepper(dom, phys, params, 600.0)
+    ref = HelmholtzVar(nx, ny, grid.dx, grid.dy, st.cfg.g * 0.25 * 600.0 ** 2, Ku, Kv, mask,
+                       kind=kind, rtol=rtol, max_iter=5000)
+    want, rep = ref.solve(rhs)
+    for call in range(2):  # the second call starts from the last call's batch size
+        x, iters, conv = st.helmholtz_solve(Ku, Kv, rhs)
+        assert conv == rep.converged and iters == rep.iterations, (call, iters, rep.iterations)
+        assert np.array_equal(_np(x), want), (call, rel_l2(_np(x), want))
 
 
 FIELDS = ("u", "v", "b", "eta", "T", "S", "e")
@@ -170,33 +215,48 @@ FIELDS = ("u", "v", "b", "eta", "T", "S", "e")
 
 @pytest.mark.parametrize("name", list(COMPOSED))
 def test_composed_step_matches_reference(hoc, name):
-    """1-3 steps of the reference stepper; the solver is switched to
-    pcg_jacobi where the fixture used another, then the oracle is the check."""
+    """1-3 steps of the reference stepper (each fixture's own solver kind):
+    every field bitwise and the solver iterations step by step."""
     f = load_golden(f"step_{name}.npz")
     dom, phys, params, dt, steps, s0 = build_case(name, f)
     from hydrostatic_ocean_core_benchmark_amd.types import State3D
-    use_ref = params.name == "fb" or params.solver == "pcg_jacobi"
-    if not use_ref:
-        params.solver = "pcg_jacobi"
-        from oracle.stepper import OracleStepper, OState
-        ost = OracleStepper(dom, phys, params, dt)
-        os_ = OState(**s0, t=0.0)
     st = hoc.HipStepper(dom, phys, params, dt)
     s = State3D(**s0, t=0.0)
     for n in range(1, steps + 1):
         s = st.step(s)
-        if use_ref:
-            want = {k: f[f"s{n}_{k}"] for k in FIELDS if f"s{n}_{k}" in f}
-            want_iters = int(f[f"s{n}_iters"])
-        else:
-            os_ = ost.step(os_)
-            want = {k: getattr(os_, k) for k in FIELDS if getattr(os_, k) is not None}
-            want_iters = ost.solver_iterations
+        want = {k: f[f"s{n}_{k}"] for k in FIELDS if f"s{n}_{k}" in f}
         for k, w in want.items():
             got = _np(getattr(s, k))
             assert rel_l2(got, w) < 1e-12, (n, k, rel_l2(got, w))
             assert np.array_equal(got, w), (n, k, "not bitwise", rel_l2(got, w))
-        assert st.solver_iterations == want_iters, n
+        assert st.solver_iterations == int(f[f"s{n}_iters"]), n
+
+
+@pytest.mark.parametrize("name", list(COMPOSED))
+def test_composed_step_pcg_mg_matches_oracle(hoc, name):
+    """The same cases with the pcg_mg solver (no reference counterpart)
+    against the oracle, bitwise."""
+    f = load_golden(f"step_{name}.npz")
+    dom, phys, params, dt, steps, s0 = build_case(name, f)
+    if params.name != "theta":
+        pytest.skip("forward-backward scheme has no Helmholtz solve")
+    params.solver = "pcg_mg"
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    from oracle.stepper import OracleStepper, OState
+    ost = OracleStepper(dom, phys, params, dt)
+    os_ = OState(**s0, t=0.0)
+    st = hoc.HipStepper(dom, phys, params, dt)
+    s = State3D(**s0, t=0.0)
+    for n in range(1, steps + 1):
+        s = st.step(s)
+        os_ = ost.step(os_)
+        for k in FIELDS:
+            w = getattr(os_, k)
+            if w is None:
+                continue
+            got = _np(getattr(s, k))
+            assert np.array_equal(got, w), (n, k, "not bitwise", rel_l2(got, w))
+        assert st.solver_iterations == ost.solver_iterations, n
 
 
 def test_lock_exchange_200_steps_matches_reference(hoc):
@@ -228,18 +288,10 @@ def test_helmholtz_persistent_equals_multikernel_and_oracle(hoc, shape):
     """The persistent (one cooperative launch) and the per-phase PCG give the
     oracle's iterates bit for bit, with and without a trailing partial
     8192-element chunk (300 x 500 = 18 full chunks + 2544)."""
-    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
     from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
     from oracle.helmholtz import HelmholtzVar
     ny, nx = shape
-    rng = np.random.default_rng(7)
-    grid = CGrid(nx=nx, ny=ny, Lx=nx * 1.0e4, Ly=ny * 1.0e4, nz=4, depth=4000.0)
-    mask = (rng.random(grid.shape) > 0.1).astype(np.float64)
-    dom = Domain(grid=grid, H=np.full(grid.shape, 4000.0), mask=mask, bc_x="closed",
-                 bc_y="closed")
-    Ku = (1000.0 + 3000.0 * rng.random(grid.shape)) * dom.mask_u
-    Kv = (1000.0 + 3000.0 * rng.random(grid.shape)) * dom.mask_v
-    rhs = rng.standard_normal(grid.shape) * mask
+    grid, dom, mask, Ku, Kv, rhs = _random_helmholtz(shape, "closed")
     phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=4000.0))
     params = SchemeParams(name="theta", solver="pcg_jacobi", rtol=1e-12, max_iter=5000)
     st = hoc.HipStepper(dom, phys, params, 600.0)

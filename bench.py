@@ -6,9 +6,11 @@ Workload: the closed basin at 512 x 512 x 40 (dx = 10 km, dt = 600 s),
 Gaussian-random-field bathymetry (correlation 200 km, 500-4000 m,
 z-level partial cells), T(z) = 2 + 18 exp(z/800) + N(0, 0.05), S with a weak
 linear gradient, tau_x = 0.1 cos(pi y / Ly), polyTEOS10, up3_tvd, TKE
-(integral mixing length), theta = 0.5 with a PCG-Jacobi barotropic solve at
-rtol 1e-12. Inputs are resident in HBM before the timed region; W untimed
-steps, then exactly K timed steps bracketed by device synchronisation.
+(integral mixing length), theta = 0.5 (2 Picard passes, so 2 barotropic
+solves per step) with the MG-preconditioned PCG (5 levels, 2 + 2 damped-Jacobi
+sweeps at omega 0.8) at rtol 1e-12. Inputs are resident in HBM before the
+timed region; W untimed steps, then exactly K timed steps bracketed by device
+synchronisation.
 A "step" is one full model time step. cells = nx*ny*nz (the reference's
 convention, dry cells included).
 
@@ -16,9 +18,14 @@ The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average launch time (HIP events on the launch stream, inside
                 the timed region), against 8 TB/s;
+  step_model    the whole step against the stage model of the step's
+                compulsory traffic: 336 B per layer cell plus, per barotropic
+                solve, 64 B per column and B_it per column and iteration
+                (B_it = 520 for MG-preconditioned PCG, 120 for PCG-Jacobi),
+                with the measured iteration counts;
   cpu_baseline  the NumPy oracle of the same step on one host core, timed on
                 a bounded sample (the 128 x 128 x 40 basin from the same
-                generator: 1 warm-up step, then 3 timed steps).
+                generator: 1 warm-up step, then 10 timed steps).
 """
 
 from __future__ import annotations
@@ -48,6 +55,10 @@ HBM_PEAK_GBS = 8000.0
 #   k_momentum_explicit  u, v, b, w, K_m read; 8 explicit terms written (104 B)
 ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 128.0,
                       "k_momentum_explicit": 104.0}
+STEP_BYTES_PER_CELL = 336.0
+SOLVE_BYTES_PER_COLUMN = 64.0
+ITER_BYTES_PER_COLUMN = {"pcg_mg": 520.0, "pcg_jacobi": 120.0, "pcg_rbgs": 520.0,
+                         "multigrid": 400.0, "rbgs": 48.0}
 
 
 def parse():
@@ -68,27 +79,28 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(nz: int) -> dict:
-    """The oracle on one core: 128 x 128 x nz basin, 1 warm-up + 3 timed steps."""
+def cpu_baseline(nz: int, solver: str) -> dict:
+    """The oracle on one core: 128 x 128 x nz basin, 1 warm-up + 10 timed steps."""
     import numpy as np
 
     from hydrostatic_ocean_core_benchmark_amd.cases import basin
     from oracle.stepper import OracleStepper, OState
-    p = basin(nx=128, nz=nz)
+    p = basin(nx=128, nz=nz, solver=solver)
     st = OracleStepper(p.domain, p.physics, p.params, p.dt)
     g = p.domain.grid
     b = st.buoyancy_of(p.initial_T, p.initial_S) * p.domain.mask3
     s = OState(u=np.zeros(g.shape3d), v=np.zeros(g.shape3d), b=b, eta=np.zeros(g.shape),
                T=p.initial_T, S=p.initial_S)
     s = st.step(s)
-    n = 3
+    n = 10
     t0 = time.perf_counter()
     for _ in range(n):
         s = st.step(s)
     wall = time.perf_counter() - t0
     cells = g.nx * g.ny * g.nz
     return {"value": cells * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"NumPy oracle, 128x128x{nz} basin from the same generator, 3 steps "
+            "sample": f"NumPy oracle, 128x128x{nz} basin from the same generator ({solver}), "
+                      f"{n} steps "
                       f"after 1 warm-up step ({wall:.1f} s; {platform.processor() or 'cpu'}, "
                       f"numpy {np.__version__}, OMP_NUM_THREADS="
                       f"{os.environ.get('OMP_NUM_THREADS', 'unset')})"}
@@ -162,22 +174,30 @@ def main() -> int:
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None, "alg_bytes_per_launch": alg,
                     "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
+    n_solves = prob.params.n_picard if prob.params.name == "theta" else 0
+    solve_bytes = g.nx * g.ny * (SOLVE_BYTES_PER_COLUMN * n_solves
+                                 + ITER_BYTES_PER_COLUMN[prob.params.solver] * iters / args.steps)
+    step_bytes = STEP_BYTES_PER_CELL * cells + solve_bytes
+    step_gbs = step_bytes / (wall / args.steps) / 1e9
+    step_model = {"bytes_per_step": step_bytes, "achieved": round(step_gbs, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
     rec = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"fp64 plain step, closed GRF basin {g.nx}x{g.ny}x{g.nz}, "
-                               "dx 10 km, dt 600 s, up3_tvd, TKE, polyTEOS10, theta 0.5 "
-                               "PCG-Jacobi rtol 1e-12",
+                               "dx 10 km, dt 600 s, up3_tvd, TKE, polyTEOS10, theta 0.5, "
+                               f"{prob.params.solver} rtol {prob.params.rtol:g}",
                    "cells": cells, "wet_fraction": round(float(np.mean(prob.domain.mask3)), 4),
                    "solver": prob.params.solver, "solver_path": st.solver_path,
                    "solver_iterations_per_step": iters / args.steps,
                    "solver_failures": fails, "state_finite": finite},
         "roofline": roofline,
+        "step_model": step_model,
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(args.nz)
+        rec["cpu_baseline"] = cpu_baseline(args.nz, prob.params.solver)
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -69,7 +69,7 @@ def lock_exchange(nx: int = 128, ny: int = 4, nz: int = 20, n_steps: int = 200,
 
 def basin(nx: int = 512, ny: int | None = None, nz: int = 40, dx: float = 1.0e4,
           dt: float = 600.0, n_steps: int = 100, seed: int = 20261202,
-          sigma_T: float = 0.05, solver: str = "pcg_jacobi", eos: str = "teos10",
+          sigma_T: float = 0.05, solver: str = "pcg_mg", eos: str = "teos10",
           rtol: float = 1e-12, max_iter: int = 5000, kappa_b=1e-5) -> Problem:
     ny = nx if ny is None else ny
     Lx, Ly = nx * dx, ny * dx

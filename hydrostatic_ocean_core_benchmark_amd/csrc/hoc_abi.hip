@@ -479,6 +479,29 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
     k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
     rbgs_sweep(h, 0, r, z, false, s);
     rbgs_sweep(h, 0, r, z, true, s);
+  } else if (h->cfg.mg_sweeps <= MG_FUSED_MAX_SWEEPS) {
+    // fused legs: one launch down and one up per level, the coarsest level
+    // in one workgroup when it fits
+    const int S = h->cfg.mg_sweeps, last = h->nlev - 1;
+    const double om = h->cfg.mg_omega;
+    auto level_b = [&](int l) -> const double* { return l == 0 ? r : h->mg_b[l]; };
+    for (int l = 0; l < last; ++l)
+      launch_mg_down(S, mg_tile_grid(h->lev[l]), s, h->lev[l], h->lev[l + 1], om, level_b(l),
+                     h->mg_x[l], h->mg_b[l + 1], h->sc);
+    double* ec = last == 0 ? z : h->mg_x[last];
+    if (h->lev[last].n <= MG_COARSE_MAX) {
+      k_mg_coarsest<<<1, MG_COARSE_THREADS, 0, s>>>(h->lev[last], om, 2 * S, level_b(last), ec,
+                                                   h->sc);
+    } else {
+      double* out = vcycle_jacobi(h, last, level_b(last), ec, h->mg_t[last], s);
+      if (out != ec) return fail(HOC_ERR_DEVICE, "pcg_mg: coarse result left its buffer");
+    }
+    for (int l = last - 1; l >= 0; --l) {
+      double* out = l == 0 ? z : h->mg_t[l];
+      launch_mg_up(S, mg_tile_grid(h->lev[l]), s, h->lev[l], h->lev[l + 1].nx, om, h->mg_x[l],
+                   ec, level_b(l), out, h->sc);
+      ec = out;
+    }
   } else {
     // 2 * mg_sweeps buffer swaps per level: the result is back in z
     double* out = vcycle_jacobi(h, 0, r, z, h->mg_t[0], s);

@@ -172,6 +172,18 @@ __global__ void k_mg_prolong(MgLevel f, const double* xc, int cnx, double* x,
 __global__ void k_mg_res_parts(PcgArgs a, MgLevel L, const double* x);
 __global__ void k_mg_check(PcgArgs a, int it);
 __global__ void k_mg_start(PcgArgs a);
+constexpr int MG_COARSE_MAX = 4096;      // cells of a level solved in one workgroup
+constexpr int MG_COARSE_THREADS = 1024;
+constexpr int MG_FUSED_MAX_SWEEPS = 3;   // fused V-cycle legs for 1..3 sweeps per side
+__global__ void k_mg_coarsest(MgLevel L, double omega, int nsweeps, const double* b, double* out,
+                              const PcgScalars* sc);
+void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
+                    double omega, const double* b, double* xs_out, double* bc,
+                    const PcgScalars* sc);
+void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
+                  const double* xs_in, const double* ec, const double* b, double* out,
+                  const PcgScalars* sc);
+dim3 mg_tile_grid(const MgLevel& L);
 __global__ void k_pcg_init(PcgArgs a);
 __global__ void k_pcg_init_z(PcgArgs a);
 __global__ void k_pcg_ap(PcgArgs a, int it);

@@ -16,26 +16,50 @@ __device__ __forceinline__ void lvl_ij(const MgLevel& L, long long c, int& i, in
   j = (int)(c / L.nx);
 }
 
-// A x at c with x masked first (solvers_var.py:47-53).
-__device__ __forceinline__ double mg_apply(const MgLevel& L, const double* x, long long c) {
-  int i, j;
-  lvl_ij(L, c, i, j);
-  const long long ce = (long long)j * L.nx + wrap(i, 1, L.nx);
-  const long long cw = (long long)j * L.nx + wrap(i, -1, L.nx);
-  const long long cn = (long long)wrap(j, 1, L.ny) * L.nx + i;
-  const long long cs = (long long)wrap(j, -1, L.ny) * L.nx + i;
+// A x at c from the values at c and its four neighbours, x masked first
+// (solvers_var.py:47-53); ce/cw/cn/cs are the neighbours' indices.
+__device__ __forceinline__ double apply_vals(const MgLevel& L, long long c, long long ce,
+                                             long long cw, long long cn, long long cs,
+                                             double x0, double x_e, double x_w, double x_n,
+                                             double x_s) {
   const double mc = mk(L.mask[c]);
-  const double xc = x[c] * mc;
-  const double xe = x[ce] * mk(L.mask[ce]);
-  const double xw = x[cw] * mk(L.mask[cw]);
-  const double xn = x[cn] * mk(L.mask[cn]);
-  const double xs = x[cs] * mk(L.mask[cs]);
+  const double xc = x0 * mc;
+  const double xe = x_e * mk(L.mask[ce]);
+  const double xw = x_w * mk(L.mask[cw]);
+  const double xn = x_n * mk(L.mask[cn]);
+  const double xs = x_s * mk(L.mask[cs]);
   const double fx = L.Ku[c] * (xe - xc);
   const double fxw = L.Ku[cw] * (xc - xw);
   const double fy = L.Kv[c] * (xn - xc);
   const double fys = L.Kv[cs] * (xc - xs);
   const double lap = (fx - fxw) / L.dx2 + (fy - fys) / L.dy2;
   return (xc - L.coef * lap) * mc + xc * (1.0 - mc);
+}
+
+// Damped Jacobi at c: x + omega (inv_diag (b - A x)) mask.
+__device__ __forceinline__ double jacobi_vals(const MgLevel& L, double omega, long long c,
+                                              long long ce, long long cw, long long cn,
+                                              long long cs, double x0, double x_e, double x_w,
+                                              double x_n, double x_s, double b) {
+  const double ax = apply_vals(L, c, ce, cw, cn, cs, x0, x_e, x_w, x_n, x_s);
+  return x0 + omega * (L.inv_diag[c] * (b - ax)) * mk(L.mask[c]);
+}
+
+struct Nbr {
+  long long c, ce, cw, cn, cs;
+};
+
+__device__ __forceinline__ Nbr nbr_of(const MgLevel& L, int i, int j) {
+  const long long row = (long long)j * L.nx;
+  return Nbr{row + i, row + wrap(i, 1, L.nx), row + wrap(i, -1, L.nx),
+             (long long)wrap(j, 1, L.ny) * L.nx + i, (long long)wrap(j, -1, L.ny) * L.nx + i};
+}
+
+__device__ __forceinline__ double mg_apply(const MgLevel& L, const double* x, long long c) {
+  int i, j;
+  lvl_ij(L, c, i, j);
+  const Nbr n = nbr_of(L, i, j);
+  return apply_vals(L, c, n.ce, n.cw, n.cn, n.cs, x[c], x[n.ce], x[n.cw], x[n.cn], x[n.cs]);
 }
 
 // -(A - diag) x, unmasked x (solvers_var.py:55-60).
@@ -48,6 +72,11 @@ __device__ __forceinline__ double mg_neighbour_sum(const MgLevel& L, const doubl
   const long long cs = (long long)wrap(j, -1, L.ny) * L.nx + i;
   return L.cx * (L.Ku[c] * x[ce] + L.Ku[cw] * x[cw]) +
          L.cy * (L.Kv[c] * x[cn] + L.Kv[cs] * x[cs]);
+}
+
+__device__ __forceinline__ int mg_pmod(int a, int n) {
+  const int r = a % n;
+  return r < 0 ? r + n : r;
 }
 
 __device__ __forceinline__ bool finished(const PcgScalars* sc) { return sc && sc->done; }
@@ -193,6 +222,153 @@ __global__ __launch_bounds__(PCG_WG) void k_mg_start(PcgArgs a) {
     a.sc->done = (nb == 0.0) ? 1 : 0;
     a.sc->res = 0.0;
   }
+}
+
+// ------------------------------------------------- fused pcg_mg V-cycle legs
+// The pcg_mg preconditioner with S damped-Jacobi sweeps per side, as LDS
+// tiles of MG_TW x MG_TH fine cells. Each sweep is evaluated on the tile
+// plus the halo the later steps still need (radius 1 per remaining sweep,
+// neighbours wrapping like the operator), so one launch replaces the
+// zero / S sweeps / residual-restrict launches of the way down and the
+// prolong / S sweeps launches of the way up, with the same arithmetic per
+// cell (apply_vals, jacobi_vals).
+constexpr int MG_TW = 32, MG_TH = 16, MG_THREADS = 256;
+
+// Way down on one level: x_S = S sweeps from zero (written out for the way
+// up) and bc = restrict((b - A x_S) mask).
+template <int S>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevel L, MgLevel C, double omega,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ xs_out,
+                                                        double* __restrict__ bc,
+                                                        const PcgScalars* sc) {
+  if (finished(sc)) return;
+  constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
+  __shared__ double buf[2][RH * RW];
+  const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
+  for (int k = 1; k <= S; ++k) {
+    const int h = S - k + 1;  // halo of x_k
+    const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
+    double* dst = buf[(k - 1) & 1];
+    const double* src = buf[k & 1];
+    for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
+      const int li = S - h + t % w, lj = S - h + t / w;
+      const Nbr n = nbr_of(L, mg_pmod(i0 - S + li, L.nx), mg_pmod(j0 - S + lj, L.ny));
+      const int q = lj * RW + li;
+      dst[q] = (k == 1)
+                   ? jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, 0.0, 0.0, 0.0, 0.0, 0.0,
+                                 b[n.c])
+                   : jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1],
+                                 src[q - 1], src[q + RW], src[q - RW], b[n.c]);
+    }
+    __syncthreads();
+  }
+  const double* xs = buf[(S - 1) & 1];
+  for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
+    const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
+    if (gi < L.nx && gj < L.ny)
+      xs_out[(long long)gj * L.nx + gi] = xs[(S + t / MG_TW) * RW + S + t % MG_TW];
+  }
+  constexpr int CW = MG_TW / 2, CH = MG_TH / 2;
+  for (int t = threadIdx.x; t < CW * CH; t += MG_THREADS) {
+    const int I = i0 / 2 + t % CW, J = j0 / 2 + t / CW;
+    if (I >= C.nx || J >= C.ny) continue;
+    double r[4];
+    for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
+      const int a = qd & 1, bq = qd >> 1;
+      const int gi = 2 * I + a, gj = 2 * J + bq;
+      const Nbr n = nbr_of(L, gi, gj);
+      const int q = (S + gj - j0) * RW + S + gi - i0;
+      const double ax = apply_vals(L, n.c, n.ce, n.cw, n.cn, n.cs, xs[q], xs[q + 1], xs[q - 1],
+                                   xs[q + RW], xs[q - RW]);
+      r[qd] = (b[n.c] - ax) * mk(L.mask[n.c]);
+    }
+    bc[(long long)J * C.nx + I] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
+  }
+}
+
+// Way up on one level: x = x_S + prolong(ec) mask, then S sweeps; out = x.
+template <int S>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevel L, int cnx, double omega,
+                                                      const double* __restrict__ xs_in,
+                                                      const double* __restrict__ ec,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ out,
+                                                      const PcgScalars* sc) {
+  if (finished(sc)) return;
+  constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
+  __shared__ double buf[2][RH * RW];
+  const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
+  for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) {
+    const int gi = mg_pmod(i0 - S + t % RW, L.nx), gj = mg_pmod(j0 - S + t / RW, L.ny);
+    const long long c = (long long)gj * L.nx + gi;
+    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * mk(L.mask[c]);
+  }
+  __syncthreads();
+  for (int k = 1; k <= S; ++k) {
+    const int h = S - k;
+    const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
+    double* dst = buf[k & 1];
+    const double* src = buf[(k - 1) & 1];
+    for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
+      const int li = S - h + t % w, lj = S - h + t / w;
+      const Nbr n = nbr_of(L, mg_pmod(i0 - S + li, L.nx), mg_pmod(j0 - S + lj, L.ny));
+      const int q = lj * RW + li;
+      dst[q] = jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1], src[q - 1],
+                           src[q + RW], src[q - RW], b[n.c]);
+    }
+    __syncthreads();
+  }
+  const double* x = buf[S & 1];
+  for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
+    const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
+    if (gi < L.nx && gj < L.ny) out[(long long)gj * L.nx + gi] = x[(S + t / MG_TW) * RW + S + t % MG_TW];
+  }
+}
+
+// The coarsest level in one workgroup: `nsweeps` sweeps from zero in LDS.
+__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarsest(MgLevel L, double omega,
+                                                                   int nsweeps,
+                                                                   const double* __restrict__ b,
+                                                                   double* __restrict__ out,
+                                                                   const PcgScalars* sc) {
+  if (finished(sc)) return;
+  __shared__ double buf[2][MG_COARSE_MAX];
+  const int n = (int)L.n;
+  for (int k = 1; k <= nsweeps; ++k) {
+    double* dst = buf[k & 1];
+    const double* src = buf[(k - 1) & 1];
+    for (int c = threadIdx.x; c < n; c += MG_COARSE_THREADS) {
+      const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
+      dst[c] = (k == 1) ? jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, 0.0, 0.0, 0.0,
+                                      0.0, 0.0, b[c])
+                        : jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c],
+                                      src[nb.ce], src[nb.cw], src[nb.cn], src[nb.cs], b[c]);
+    }
+    __syncthreads();
+  }
+  const double* x = buf[nsweeps & 1];
+  for (int c = threadIdx.x; c < n; c += MG_COARSE_THREADS) out[c] = x[c];
+}
+
+void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
+                    double omega, const double* b, double* xs_out, double* bc,
+                    const PcgScalars* sc) {
+  if (S == 1) k_mg_down<1><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else if (S == 2) k_mg_down<2><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else k_mg_down<3><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+}
+
+void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
+                  const double* xs_in, const double* ec, const double* b, double* out,
+                  const PcgScalars* sc) {
+  if (S == 1) k_mg_up<1><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
+  else if (S == 2) k_mg_up<2><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
+  else k_mg_up<3><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
+}
+
+dim3 mg_tile_grid(const MgLevel& L) {
+  return dim3((unsigned)((L.nx + MG_TW - 1) / MG_TW), (unsigned)((L.ny + MG_TH - 1) / MG_TH));
 }
 
 }  // namespace hoc

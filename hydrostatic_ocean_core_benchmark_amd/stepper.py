@@ -139,6 +139,17 @@ class HipStepper:
         _lib.check(self.lib.hoc_stepper_build_coefficients(self._h, self._stream()))
         self._bufs: list[dict] = []
 
+    def reset_mixing(self) -> None:
+        """Restore nu / kappa to their backgrounds and rebuild the implicit
+        coefficients, as at construction (a run restarting from its initial
+        state; cfd_exp3d5_cuda.cu:1163-1175)."""
+        p = self.physics
+        self.nu.fill_(float(p.nu))
+        self.kappa.copy_(torch.as_tensor(
+            np.ascontiguousarray(np.broadcast_to(self._host["kappa_b"], self.kappa.shape)),
+            device=self.device))
+        _lib.check(self.lib.hoc_stepper_build_coefficients(self._h, self._stream()))
+
     # ------------------------------------------------------------ plumbing
     def __del__(self):
         h = getattr(self, "_h", None)

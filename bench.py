@@ -76,6 +76,9 @@ def parse():
                     choices=("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid", "pcg_mg"),
                     help="Helmholtz solver kind (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the workload, copy this many MiB device to device once "
+                         "(a kernel of known traffic, to scale HBM byte counters)")
     return ap.parse_args()
 
 
@@ -121,6 +124,13 @@ def main() -> int:
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         return 2
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.arange(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     prob = basin(nx=args.nx, nz=args.nz)
     if args.solver:
         prob.params.solver = args.solver

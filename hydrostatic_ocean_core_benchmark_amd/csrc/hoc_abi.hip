@@ -147,7 +147,7 @@ struct hoc_stepper {
   double *phi, *w, *pgx, *pgy, *avu, *avv, *dzu, *dzv, *stu, *stv;
   double *U0, *V0, *div0, *Ug, *Vg, *rhs;
   double *gh_u[2], *gh_v[2];
-  double *s_n2, *s_sh, *s_l, *s_c;
+  double *s_n2, *s_sh, *s_l, *s_le, *s_c;
   // Helmholtz
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
   unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
@@ -687,6 +687,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   A(h->s_n2, d.ni);
   A(h->s_sh, d.ni);
   A(h->s_l, d.ni);
+  A(h->s_le, d.ni);
   A(h->s_c, d.ni);
   A(h->r, d.n2);
   A(h->z, d.n2);
@@ -726,6 +727,13 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
                                                      PCG_WG, 0) == hipSuccess &&
         coop && parts <= (long long)ncu * per_cu) {
       h->pcg_parts = (int)parts;
+    }
+    if (cfg->closure_tke && !cfg->mxl_recursive) {
+      const int lrc = closure_lengths_prepare(d.nz);
+      if (lrc == -1)
+        return cleanup(fail(HOC_ERR_UNSUPPORTED, "nz too large for the integral mixing-length "
+                                                 "kernel (at most 159 levels)"));
+      if (lrc != 0) return cleanup(fail(HOC_ERR_DEVICE, "hipFuncSetAttribute failed"));
     }
     if (h->red.nchunks > PCG_MAX_CHUNKS)
       return cleanup(fail(HOC_ERR_UNSUPPORTED, "grid too large for the Helmholtz solver "
@@ -862,12 +870,16 @@ int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, co
   hipStream_t s = (hipStream_t)stream;
   const Dims d = h->d;
   if (h->p.mxl_recursive) {
-    k_closure_n2<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, u, v, b, h->s_n2);
-    k_mixing_recursive<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->s_n2, h->s_l);
+    k_closure_shear<<<blocks(d.ni), TPB, 0, s>>>(d, h->geo, u, v, b, h->s_n2, h->s_sh);
+    k_mixing_recursive<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->s_n2, h->s_l,
+                                                     h->s_le);
+  } else {
+    ProfScope _ps(h, "k_closure_lengths", s);
+    launch_closure_lengths(d, h->p, h->geo, e_in, u, v, b, h->s_n2, h->s_sh, h->s_l, h->s_le, s);
   }
   { ProfScope _ps(h, "k_closure", s);
-  k_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, u, v, b, h->Km, h->Kh, e_out,
-                                         h->s_n2, h->s_sh, h->s_l, h->s_c); }
+  k_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->Km, h->Kh, e_out, h->s_n2,
+                                         h->s_sh, h->s_l, h->s_le, h->s_c); }
   LAUNCH_CHECK("k_closure");
   return HOC_OK;
 }

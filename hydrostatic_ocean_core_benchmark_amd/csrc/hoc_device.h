@@ -45,6 +45,20 @@ __device__ __forceinline__ double safe_inv(double x) {
 }
 __device__ __forceinline__ double mk(uint8_t m) { return m ? 1.0 : 0.0; }
 
+// Tile of this workgroup in a 2D grid, remapped so that each of the 8 XCDs
+// (workgroups are dispatched to them round-robin by linear id) owns one
+// contiguous band of tile rows: neighbouring tiles then share their halo
+// rows in the same L2. A bijection for any grid size.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+  const int gx = (int)gridDim.x;
+  const int nb = gx * (int)gridDim.y;
+  const int id = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  const int x = id % 8, q = nb / 8, r = nb % 8;
+  const int L = x * q + (x < r ? x : r) + id / 8;
+  bx = L % gx;
+  by = L / gx;
+}
+
 // Superbee limiter phi(r) = max(0, max(min(2r, 1), min(r, 2))).
 __device__ __forceinline__ double superbee(double r) {
   return nmax(0.0, nmax(nmin(2.0 * r, 1.0), nmin(r, 2.0)));

@@ -41,14 +41,20 @@ struct PcgScalars {
 __global__ void k_geometry(Dims d, StepParams p, const double* dz3, const uint8_t* m3u,
                            const uint8_t* m3v, const double* tau_x, const double* tau_y,
                            double* dzi, double* tau_u, double* tau_v);
-__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* e_in, const double* u,
-                          const double* v, const double* b, double* Km, double* Kh,
-                          double* e_out, double* s_n2, double* s_sh, double* s_l,
-                          double* s_c);
-__global__ void k_closure_n2(Dims d, Geo geo, const double* u, const double* v,
-                             const double* b, double* s_n2);
+__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* e_in, double* Km,
+                          double* Kh, double* e_out, const double* s_n2, const double* s_sh,
+                          const double* s_l, const double* s_le, double* s_c);
+__global__ void k_closure_shear(Dims d, Geo geo, const double* u, const double* v,
+                                const double* b, double* s_n2, double* s_sh);
 __global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* e_in,
-                                   const double* s_n2, double* s_l);
+                                   const double* s_n2, double* s_l, double* s_le);
+void launch_closure_lengths(const Dims& d, const StepParams& p, const Geo& geo,
+                            const double* e_in, const double* u, const double* v,
+                            const double* b, double* s_n2, double* s_sh, double* s_l,
+                            double* s_le, hipStream_t s);
+// Sets the LDS limit of the integral-length kernel for nz levels; < 0 if
+// its staging does not fit one workgroup's LDS.
+int closure_lengths_prepare(int nz);
 __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* Km,
                                const double* Kh, double* sub_m, double* den_m, double* cst_m,
                                double* sub_b, double* den_b, double* cst_b, double* q);

@@ -37,125 +37,246 @@ __global__ void k_geometry(Dims d, StepParams p, const double* __restrict__ dz3,
 }
 
 // ---------------------------------------------------------- TKE closure
-// One lane per column: N^2 and shear on interfaces, Gaspar (integral) or
-// recursive mixing lengths, the implicit interface tridiagonal for e, then
-// K_m and K_h (libs/core/closure.py:141-205). The march of the integral
-// length stops exactly where the reference's full loop stops changing
-// anything (past the column end, or once the energy budget is spent).
-// Scratch (each [nz+1][ny][nx]): s_n2 = N^2, s_sh = shear^2, s_l = l_k,
-// s_c = Thomas c*. e_out receives d* during the forward sweep.
-__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
-                          const double* __restrict__ u, const double* __restrict__ v,
-                          const double* __restrict__ b, double* __restrict__ Km,
-                          double* __restrict__ Kh, double* __restrict__ e_out,
-                          double* __restrict__ s_n2, double* __restrict__ s_sh,
-                          double* __restrict__ s_l, double* __restrict__ s_c) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.n2) return;
-  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
-  const int nz = d.nz;
+// libs/core/closure.py:141-205 in three passes over [nz+1][ny][nx] interface
+// fields:
+//   k_closure_lengths (integral form) or k_closure_shear + k_mixing_recursive
+//     (recursive form): N^2 and shear^2 (s_n2, s_sh) and the mixing lengths
+//     l_k = max(l_min, sqrt(l_up l_dn)) (s_l), l_eps = max(l_min, min(l_up,
+//     l_dn)) (s_le);
+//   k_closure: per column, the implicit interface tridiagonal for e (c* in
+//     s_c, d* in e_out), then back-substitution, floors, K_m and K_h.
+
+// N^2 and shear^2 on interface k of column c (closure.py:43-62,150-156).
+__device__ __forceinline__ void shear_n2(const Dims& d, const Geo& geo, const double* u,
+                                         const double* v, const double* b, long long c, int k,
+                                         double& n2k, double& sh) {
+  n2k = 0.0;
+  sh = 0.0;
+  if (k < 1 || k >= d.nz) return;
   const long long n2 = d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
   const long long cw = at2(d, j, wrap(i, -1, d.nx));
   const long long cs = at2(d, wrap(j, -1, d.ny), i);
-  const uint8_t* m3 = geo.m3;
+  const long long a = (long long)(k - 1) * n2, o = (long long)k * n2;
+  const double uc_prev = 0.5 * (u[a + c] + u[a + cw]);
+  const double vc_prev = 0.5 * (v[a + c] + v[a + cs]);
+  const double uc = 0.5 * (u[o + c] + u[o + cw]);
+  const double vc = 0.5 * (v[o + c] + v[o + cs]);
+  const double wet = mk(geo.m3[a + c]) * mk(geo.m3[o + c]);
+  const double idz = safe_inv(geo.dzi[o + c]);
+  const double du = (uc_prev - uc) * idz;
+  const double dv = (vc_prev - vc) * idz;
+  const double db = (b[a + c] - b[o + c]) * idz;
+  sh = (du * du + dv * dv) * wet;
+  n2k = db * wet;
+}
 
-  // phase A: N^2, shear^2 on interfaces; depth of the wet column
+// Integral (Gaspar) mixing lengths, closure.py:65-103. A workgroup owns
+// MX_COLS consecutive columns: N^2 (clamped at 0) and the interface
+// thicknesses of all their interfaces are staged in LDS once, then
+// MX_TPC lanes per column march the interfaces k, k + MX_TPC, ... from LDS.
+// Each march is the reference's loop with two exact shortcuts: it stops
+// once the budget is spent (l and the budget are then frozen), and it skips
+// the interfaces above the column's first and below its last one with
+// N^2 != 0 or dz != 0 (a march from a dry interface under the bottom starts
+// at the bottom), since a step across them changes neither l nor the budget.
+constexpr int MX_COLS = 64, MX_TPC = 8, MX_THREADS = MX_COLS * MX_TPC;
+
+__global__ __launch_bounds__(MX_THREADS) void k_closure_lengths(
+    Dims d, StepParams p, Geo geo, const double* __restrict__ e_in, const double* __restrict__ u,
+    const double* __restrict__ v, const double* __restrict__ b, double* __restrict__ s_n2,
+    double* __restrict__ s_sh, double* __restrict__ s_l, double* __restrict__ s_le) {
+  extern __shared__ double mx_lds[];
+  __shared__ int kbeg[MX_COLS], kend[MX_COLS];
+  const int nz = d.nz, ni = nz + 1;
+  const long long n2 = d.n2;
+  double* n2p = mx_lds;                 // [ni][MX_COLS] max(N^2, 0)
+  double* dzl = mx_lds + ni * MX_COLS;  // [ni][MX_COLS] interface thickness
+  const long long c0 = (long long)blockIdx.x * MX_COLS;
+  for (int t = threadIdx.x; t < ni * MX_COLS; t += MX_THREADS) {
+    const int kk = t / MX_COLS, col = t % MX_COLS;
+    const long long c = c0 + col;
+    double n2k = 0.0, sh = 0.0, dz = 0.0;
+    if (c < n2) {
+      shear_n2(d, geo, u, v, b, c, kk, n2k, sh);
+      dz = geo.dzi[(long long)kk * n2 + c];
+      s_n2[(long long)kk * n2 + c] = n2k;
+      s_sh[(long long)kk * n2 + c] = sh;
+    }
+    n2p[t] = nmax(n2k, 0.0);
+    dzl[t] = dz;
+  }
+  __syncthreads();
+  if (threadIdx.x < MX_COLS) {
+    const int col = threadIdx.x;
+    int lo = ni, hi = -1;
+    for (int kk = 0; kk < ni; ++kk) {
+      const bool live = n2p[kk * MX_COLS + col] != 0.0 || dzl[kk * MX_COLS + col] != 0.0;
+      if (live) {
+        lo = kk < lo ? kk : lo;
+        hi = kk;
+      }
+    }
+    kbeg[col] = lo;
+    kend[col] = hi + 1;
+  }
+  __syncthreads();
+  const int col = threadIdx.x % MX_COLS, k0 = threadIdx.x / MX_COLS;
+  const long long c = c0 + col;
+  if (c >= n2) return;
+  const int lo = kbeg[col], hi = kend[col];
+  double h = 0.0;
+  for (int k = 0; k < nz; ++k) h = h + geo.dz3[(long long)k * n2 + c];
+  double zi = 0.0;  // depth of interface k: dz3[0] + ... + dz3[k-1], in that order
+  int kz = 0;
+  for (int k = k0; k <= nz; k += MX_TPC) {
+    for (; kz < k; ++kz) zi = zi + geo.dz3[(long long)kz * n2 + c];
+    const long long o = (long long)k * n2 + c;
+    const double wet =
+        (k >= 1 && k < nz) ? mk(geo.m3[o - n2]) * mk(geo.m3[o]) : 0.0;
+    const double budget0 = nmax(e_in[o], p.e_min);
+    // upward: interface k crosses k-1, ..., 0
+    double l = 0.0, bud = budget0;
+    for (int kk = (k - 1 < hi - 1 ? k - 1 : hi - 1); kk >= lo && bud > 0.0; --kk) {
+      const double n2j = n2p[kk * MX_COLS + col];
+      const double dzj = dzl[kk * MX_COLS + col];
+      const double cost = n2j * dzj * (l + 0.5 * dzj);
+      double step;
+      if (cost <= bud) {
+        step = dzj;
+        bud = bud - cost;
+      } else {
+        if (n2j > 0.0) {
+          const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
+          step = nmin(nmax(s, 0.0), dzj);
+        } else {
+          step = dzj;
+        }
+        bud = 0.0;
+      }
+      l = l + step;
+    }
+    const double l_up = nmin(l, zi + p.z_0) * wet;
+    // downward: interface k crosses k+1, ..., nz
+    l = 0.0;
+    bud = budget0;
+    for (int kk = (k + 1 > lo ? k + 1 : lo); kk < hi && bud > 0.0; ++kk) {
+      const double n2j = n2p[kk * MX_COLS + col];
+      const double dzj = dzl[kk * MX_COLS + col];
+      const double cost = n2j * dzj * (l + 0.5 * dzj);
+      double step;
+      if (cost <= bud) {
+        step = dzj;
+        bud = bud - cost;
+      } else {
+        if (n2j > 0.0) {
+          const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
+          step = nmin(nmax(s, 0.0), dzj);
+        } else {
+          step = dzj;
+        }
+        bud = 0.0;
+      }
+      l = l + step;
+    }
+    const double zb = nmax(h - zi, 0.0);
+    const double l_dn = nmin(l, zb + p.z_0) * wet;
+    s_l[o] = nmax(p.l_min, sqrt(l_up * l_dn));
+    s_le[o] = nmax(p.l_min, nmin(l_up, l_dn));
+  }
+}
+
+size_t closure_lengths_lds(int nz) { return (size_t)2 * (nz + 1) * MX_COLS * sizeof(double); }
+
+void launch_closure_lengths(const Dims& d, const StepParams& p, const Geo& geo,
+                            const double* e_in, const double* u, const double* v,
+                            const double* b, double* s_n2, double* s_sh, double* s_l,
+                            double* s_le, hipStream_t s) {
+  const unsigned nb = (unsigned)((d.n2 + MX_COLS - 1) / MX_COLS);
+  k_closure_lengths<<<nb, MX_THREADS, closure_lengths_lds(d.nz), s>>>(d, p, geo, e_in, u, v, b,
+                                                                      s_n2, s_sh, s_l, s_le);
+}
+
+int closure_lengths_prepare(int nz) {
+  const size_t lds = closure_lengths_lds(nz);
+  if (lds > 160 * 1024) return -1;
+  return hipFuncSetAttribute((const void*)k_closure_lengths,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess
+             ? 0
+             : -2;
+}
+
+// N^2 and shear^2 alone, one lane per interface cell (the recursive form
+// needs N^2 before its length recursion).
+__global__ void k_closure_shear(Dims d, Geo geo, const double* __restrict__ u,
+                                const double* __restrict__ v, const double* __restrict__ b,
+                                double* __restrict__ s_n2, double* __restrict__ s_sh) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.ni) return;
+  const int k = (int)(t / d.n2);
+  const long long c = t % d.n2;
+  double n2k, sh;
+  shear_n2(d, geo, u, v, b, c, k, n2k, sh);
+  s_n2[t] = n2k;
+  s_sh[t] = sh;
+}
+
+// Recursive (NEMO nn_mxl=2) mixing length, closure.py:106-129; s_l is the
+// work array of the two sweeps and ends as l_k, s_le as l_eps.
+__global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
+                                   const double* __restrict__ s_n2, double* __restrict__ s_l,
+                                   double* __restrict__ s_le) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int nz = d.nz;
+  const long long n2 = d.n2;
   double h = 0.0;
   for (int k = 0; k < nz; ++k) h = h + geo.dz3[k * n2 + c];
-  double uc_prev = 0.0, vc_prev = 0.0, b_prev = 0.0;
+  double zi = 0.0;
   for (int k = 0; k <= nz; ++k) {
-    double uc = 0.0, vc = 0.0, bk = 0.0;
-    if (k < nz) {
-      uc = 0.5 * (u[k * n2 + c] + u[k * n2 + cw]);
-      vc = 0.5 * (v[k * n2 + c] + v[k * n2 + cs]);
-      bk = b[k * n2 + c];
-    }
-    double n2k = 0.0, sh = 0.0;
-    if (k >= 1 && k < nz) {
-      const double wet = mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]);
-      const double idz = safe_inv(geo.dzi[k * n2 + c]);
-      const double du = (uc_prev - uc) * idz;
-      const double dv = (vc_prev - vc) * idz;
-      const double db = (b_prev - bk) * idz;
-      sh = (du * du + dv * dv) * wet;
-      n2k = db * wet;
-    }
-    s_n2[k * n2 + c] = n2k;
-    s_sh[k * n2 + c] = sh;
-    uc_prev = uc;
-    vc_prev = vc;
-    b_prev = bk;
+    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
+    const double n2c = nmax(s_n2[k * n2 + c], p.n2_min);
+    const double ek = nmax(e_in[k * n2 + c], p.e_min);
+    double l = sqrt(2.0 * ek / n2c);
+    const double zb = nmax(h - zi, 0.0);
+    l = nmin(l, nmin(zi + p.z_0, zb + p.z_0));
+    if (k > 0) l = nmin(l, s_l[(k - 1) * n2 + c] + geo.dzi[k * n2 + c]);
+    s_l[k * n2 + c] = l;
   }
+  for (int k = nz - 1; k >= 0; --k) {
+    const double l = nmin(s_l[k * n2 + c], s_l[(k + 1) * n2 + c] + geo.dzi[(k + 1) * n2 + c]);
+    s_l[k * n2 + c] = l;
+  }
+  for (int k = 0; k <= nz; ++k) {
+    const double wet = (k >= 1 && k < nz)
+                           ? mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]) : 0.0;
+    const double l = s_l[k * n2 + c] * wet;  // l_up = l_dn = l
+    s_l[k * n2 + c] = nmax(p.l_min, sqrt(l * l));
+    s_le[k * n2 + c] = nmax(p.l_min, nmin(l, l));
+  }
+}
 
-  // phase B+C: mixing lengths and the forward sweep, interface by interface
+// The interface tridiagonal for e and the eddy coefficients, one lane per
+// column, from s_n2, s_sh, s_l = l_k and s_le = l_eps.
+__global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
+                          double* __restrict__ Km, double* __restrict__ Kh,
+                          double* __restrict__ e_out, const double* __restrict__ s_n2,
+                          const double* __restrict__ s_sh, const double* __restrict__ s_l,
+                          const double* __restrict__ s_le, double* __restrict__ s_c) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  const uint8_t* m3 = geo.m3;
   const double kb = geo.kappa_b[c];
   const double dt = p.dt;
-  double zi = 0.0;              // depth of interface k below the surface
   double c_prev = 0.0, d_prev = 0.0;
   double rhs0 = 0.0;
   for (int k = 0; k <= nz; ++k) {
-    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
     const double wet = (k >= 1 && k < nz) ? mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]) : 0.0;
     const double ek = e_in[k * n2 + c];
     const double n2k = s_n2[k * n2 + c];
-    double l_up, l_dn;
-    if (!p.mxl_recursive) {
-      const double budget0 = nmax(ek, p.e_min);
-      // upward march: interface k crosses k-1, k-2, ..., 0
-      double l = 0.0, bud = budget0;
-      for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
-        const double n2j = nmax(s_n2[kk * n2 + c], 0.0);
-        const double dzj = geo.dzi[kk * n2 + c];
-        const double cost = n2j * dzj * (l + 0.5 * dzj);
-        double step;
-        if (cost <= bud) {
-          step = dzj;
-          bud = bud - cost;
-        } else {
-          if (n2j > 0.0) {
-            const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
-            step = nmin(nmax(s, 0.0), dzj);
-          } else {
-            step = dzj;
-          }
-          bud = 0.0;
-        }
-        l = l + step;
-      }
-      l_up = nmin(l, zi + p.z_0) * wet;
-      // downward march: interface k crosses k+1, ..., nz
-      l = 0.0;
-      bud = budget0;
-      for (int kk = k + 1; kk <= nz && bud > 0.0; ++kk) {
-        const double n2j = nmax(s_n2[kk * n2 + c], 0.0);
-        const double dzj = geo.dzi[kk * n2 + c];
-        const double cost = n2j * dzj * (l + 0.5 * dzj);
-        double step;
-        if (cost <= bud) {
-          step = dzj;
-          bud = bud - cost;
-        } else {
-          if (n2j > 0.0) {
-            const double s = sqrt(nmax(l * l + 2.0 * bud / n2j, 0.0)) - l;
-            step = nmin(nmax(s, 0.0), dzj);
-          } else {
-            step = dzj;
-          }
-          bud = 0.0;
-        }
-        l = l + step;
-      }
-      const double zb = nmax(h - zi, 0.0);
-      l_dn = nmin(l, zb + p.z_0) * wet;
-    } else {
-      // recursive form: lengths precomputed into s_l by k_mixing_recursive
-      l_up = s_l[k * n2 + c];
-      l_dn = l_up;
-    }
-    const double l_k = nmax(p.l_min, sqrt(l_up * l_dn));
-    const double l_eps = nmax(p.l_min, nmin(l_up, l_dn));
-    s_l[k * n2 + c] = l_k;
-
+    const double l_eps = s_le[k * n2 + c];
     const double kmo = Km[k * n2 + c];
     const double kho = Kh[k * n2 + c];
     const double prod = (kmo - p.nu_b) * s_sh[k * n2 + c] - (kho - kb) * n2k;
@@ -196,7 +317,7 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
     d_prev = dst;
   }
 
-  // phase D: back-substitution, floors, eddy coefficients
+  // back-substitution, floors, eddy coefficients
   double x_next = 0.0;
   for (int k = nz; k >= 0; --k) {
     const double dst = e_out[k * n2 + c];
@@ -211,62 +332,6 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
     e_out[k * n2 + c] = en;
     Km[k * n2 + c] = km;
     Kh[k * n2 + c] = kh;
-  }
-}
-
-// Recursive (NEMO nn_mxl=2) mixing length into s_l (libs/core/closure.py:106-129).
-// Runs after the N^2 pass of k_closure_n2 and before k_closure's solve.
-__global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* __restrict__ e_in,
-                                   const double* __restrict__ s_n2, double* __restrict__ s_l) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.n2) return;
-  const int nz = d.nz;
-  const long long n2 = d.n2;
-  double h = 0.0;
-  for (int k = 0; k < nz; ++k) h = h + geo.dz3[k * n2 + c];
-  double zi = 0.0;
-  for (int k = 0; k <= nz; ++k) {
-    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
-    const double n2c = nmax(s_n2[k * n2 + c], p.n2_min);
-    const double ek = nmax(e_in[k * n2 + c], p.e_min);
-    double l = sqrt(2.0 * ek / n2c);
-    const double zb = nmax(h - zi, 0.0);
-    l = nmin(l, nmin(zi + p.z_0, zb + p.z_0));
-    if (k > 0) l = nmin(l, s_l[(k - 1) * n2 + c] + geo.dzi[k * n2 + c]);
-    s_l[k * n2 + c] = l;
-  }
-  for (int k = nz - 1; k >= 0; --k) {
-    const double l = nmin(s_l[k * n2 + c], s_l[(k + 1) * n2 + c] + geo.dzi[(k + 1) * n2 + c]);
-    s_l[k * n2 + c] = l;
-  }
-  for (int k = 0; k <= nz; ++k) {
-    const double wet = (k >= 1 && k < nz)
-                           ? mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]) : 0.0;
-    s_l[k * n2 + c] = s_l[k * n2 + c] * wet;
-  }
-}
-
-// N^2 / shear pass alone (the recursive form needs N^2 before the solve).
-__global__ void k_closure_n2(Dims d, Geo geo, const double* __restrict__ u,
-                             const double* __restrict__ v, const double* __restrict__ b,
-                             double* __restrict__ s_n2) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.n2) return;
-  const int nz = d.nz;
-  const long long n2 = d.n2;
-  (void)u;
-  (void)v;
-  double b_prev = 0.0;
-  for (int k = 0; k <= nz; ++k) {
-    const double bk = k < nz ? b[k * n2 + c] : 0.0;
-    double n2k = 0.0;
-    if (k >= 1 && k < nz) {
-      const double wet = mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]);
-      const double idz = safe_inv(geo.dzi[k * n2 + c]);
-      n2k = ((b_prev - bk) * idz) * wet;
-    }
-    s_n2[k * n2 + c] = n2k;
-    b_prev = bk;
   }
 }
 

@@ -52,6 +52,94 @@ struct TileLds {
   uint8_t mv[TY + 1][TX];   // mask3v
 };
 
+// Level-k staging of one tile held in registers between its global loads
+// and its LDS stores, so the loads of level k + 1 are in flight while
+// level k is computed (two LDS buffers, one barrier per level).
+constexpr int ST_C = (SX * SY + TILE_THREADS - 1) / TILE_THREADS;        // tracer + mask slots
+constexpr int ST_U = (TY * (TX + 1) + TILE_THREADS - 1) / TILE_THREADS;  // u-face slots
+constexpr int ST_V = ((TY + 1) * TX + TILE_THREADS - 1) / TILE_THREADS;  // v-face slots
+
+struct TileRegs {
+  double c0[ST_C], c1[ST_C];
+  uint8_t m[ST_C];
+  double u[ST_U], du[ST_U];
+  uint8_t mu[ST_U];
+  double v[ST_V], dv[ST_V];
+  uint8_t mv[ST_V];
+};
+
+__device__ __forceinline__ void tile_load(TileRegs& R, int tid, int i0, int j0, int nx, int ny,
+                                          long long o, bool ts, const double* A,
+                                          const double* S, const double* u, const double* v,
+                                          const Geo& geo) {
+#pragma unroll
+  for (int q = 0; q < ST_C; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < SX * SY) {
+      const int lx = t % SX, ly = t / SX;
+      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
+      R.c0[q] = A[o + g];
+      R.c1[q] = ts ? S[o + g] : 0.0;
+      R.m[q] = geo.m3[o + g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_U; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < TY * (TX + 1)) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx - 1, nx);
+      R.u[q] = u[o + g];
+      R.du[q] = geo.dz3u[o + g];
+      R.mu[q] = geo.m3u[o + g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_V; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < (TY + 1) * TX) {
+      const int lx = t % TX, ly = t / TX;
+      const long long g = (long long)pmod(j0 + ly - 1, ny) * nx + pmod(i0 + lx, nx);
+      R.v[q] = v[o + g];
+      R.dv[q] = geo.dz3v[o + g];
+      R.mv[q] = geo.m3v[o + g];
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_store(TileLds& L, const TileRegs& R, int tid) {
+#pragma unroll
+  for (int q = 0; q < ST_C; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < SX * SY) {
+      const int lx = t % SX, ly = t / SX;
+      L.c[0][ly][lx] = R.c0[q];
+      L.c[1][ly][lx] = R.c1[q];
+      L.m[ly][lx] = R.m[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_U; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < TY * (TX + 1)) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      L.u[ly][lx] = R.u[q];
+      L.du[ly][lx] = R.du[q];
+      L.mu[ly][lx] = R.mu[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_V; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < (TY + 1) * TX) {
+      const int lx = t % TX, ly = t / TX;
+      L.v[ly][lx] = R.v[q];
+      L.dv[ly][lx] = R.dv[q];
+      L.mv[ly][lx] = R.mv[q];
+    }
+  }
+}
+
 // Both tracers of the tile's columns: advection + K_h Laplacian + explicit
 // part of the vertical diffusion + surface flux, the forward sweep on tri_b
 // (d* parked in the outputs), then the back-substitution and the EOS
@@ -62,12 +150,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
     const double* __restrict__ w, const double* __restrict__ Kh, const double* __restrict__ sub,
     const double* __restrict__ den, const double* __restrict__ cst, double* __restrict__ T_out,
     double* __restrict__ S_out, double* __restrict__ b_out) {
-  __shared__ TileLds L;
+  __shared__ TileLds LB[2];
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * TX, j0 = by * TY;
   const int i = i0 + tx, j = j0 + ty;
   const bool active = i < nx && j < ny;
   const long long c = active ? (long long)j * nx + i : 0;
@@ -102,132 +192,129 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
   double cf_top[2] = {0.0, 0.0};  // interface value at the top of level k
   double dprev[2] = {0.0, 0.0};
 
+  TileRegs R;
+  tile_load(R, tid, i0, j0, nx, ny, 0, ts, A, S, u, v, geo);
+  tile_store(LB[0], R, tid);
+  __syncthreads();
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
-    __syncthreads();
-    // ---- stage level k
-    for (int t = tid; t < SX * SY; t += TILE_THREADS) {
-      const int lx = t % SX, ly = t / SX;
-      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
-      L.c[0][ly][lx] = A[o + g];
-      if (ts) L.c[1][ly][lx] = S[o + g];
-      L.m[ly][lx] = geo.m3[o + g];
-    }
-    for (int t = tid; t < TY * (TX + 1); t += TILE_THREADS) {
-      const int lx = t % (TX + 1), ly = t / (TX + 1);
-      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx - 1, nx);
-      L.u[ly][lx] = u[o + g];
-      L.du[ly][lx] = geo.dz3u[o + g];
-      L.mu[ly][lx] = geo.m3u[o + g];
-    }
-    for (int t = tid; t < (TY + 1) * TX; t += TILE_THREADS) {
-      const int lx = t % TX, ly = t / TX;
-      const long long g = (long long)pmod(j0 + ly - 1, ny) * nx + pmod(i0 + lx, nx);
-      L.v[ly][lx] = v[o + g];
-      L.dv[ly][lx] = geo.dz3v[o + g];
-      L.mv[ly][lx] = geo.m3v[o + g];
-    }
-    __syncthreads();
-    if (!active) continue;
+    const TileLds& L = LB[k & 1];
+    if (k + 1 < nz) tile_load(R, tid, i0, j0, nx, ny, o + n2, ts, A, S, u, v, geo);
+    if (active) {
+      // ---- own-column values for level k
+      const double w_k1 = w[o + n2 + c];
+      const double kh_k1 = (k + 1 <= nz) ? Kh[o + n2 + c] : 0.0;
+      const double dzi_k1 = geo.dzi[o + n2 + c];
+      const double dz = geo.dz3[o + c];
+      const double inv = safe_inv(dz);
+      const bool m_k = mw[2];
+      const bool bot_km1 = k >= 1 && mw[1] && !mw[2];          // cell k-1 is a bottom cell
+      const bool bot_k = m_k && !((k + 1 < nz) && mw[3]);      // cell k is a bottom cell
+      const int cx = tx + HALO, cy = ty + HALO;
 
-    // ---- own-column values for level k
-    const double w_k1 = w[o + n2 + c];
-    const double kh_k1 = (k + 1 <= nz) ? Kh[o + n2 + c] : 0.0;
-    const double dzi_k1 = geo.dzi[o + n2 + c];
-    const double dz = geo.dz3[o + c];
-    const double inv = safe_inv(dz);
-    const bool m_k = mw[2];
-    const bool bot_km1 = k >= 1 && mw[1] && !mw[2];          // cell k-1 is a bottom cell
-    const bool bot_k = m_k && !((k + 1 < nz) && mw[3]);      // cell k is a bottom cell
-    const int cx = tx + HALO, cy = ty + HALO;
-
-    for (int r = 0; r < ntr; ++r) {
-      const double(*C)[SX] = L.c[r];
-      const double cc = C[cy][cx];
-      double tend = 0.0;
-      if (!ts) tend = p.neg_N2 * (0.5 * (w_k + w_k1));
-      if (adv != ADV_NONE) {
-        const double ue = L.u[ty][tx + 1], uw = L.u[ty][tx];
-        const double vn = L.v[ty + 1][tx], vs = L.v[ty][tx];
-        const double fcx = tracer_face(adv, C[cy][cx - 1], cc, C[cy][cx + 1], C[cy][cx + 2], ue,
-                                       L.m[cy][cx - 1] != 0, L.m[cy][cx + 2] != 0);
-        const double fcxw = tracer_face(adv, C[cy][cx - 2], C[cy][cx - 1], cc, C[cy][cx + 1], uw,
-                                        L.m[cy][cx - 2] != 0, L.m[cy][cx + 1] != 0);
-        const double fcy = tracer_face(adv, C[cy - 1][cx], cc, C[cy + 1][cx], C[cy + 2][cx], vn,
-                                       L.m[cy - 1][cx] != 0, L.m[cy + 2][cx] != 0);
-        const double fcys = tracer_face(adv, C[cy - 2][cx], C[cy - 1][cx], cc, C[cy + 1][cx], vs,
-                                        L.m[cy - 2][cx] != 0, L.m[cy + 1][cx] != 0);
-        const double fx = L.du[ty][tx + 1] * ue * fcx;
-        const double fxw = L.du[ty][tx] * uw * fcxw;
-        const double fy = L.dv[ty + 1][tx] * vn * fcy;
-        const double fys = L.dv[ty][tx] * vs * fcys;
-        const double hdiv = ((fx - fxw) / p.dx + (fy - fys) / p.dy) * inv;
-        const double cft = (k == 0) ? win[r][2] : cf_top[r];
-        const double cfb = tracer_face_z(adv, k + 1, nz, win[r][1], win[r][2], win[r][3],
-                                         win[r][4], mw[1], mw[4], w_k1);
-        cf_top[r] = cfb;
-        const double vdiv = (w_k * cft - w_k1 * cfb) * inv;
-        tend = tend - (hdiv + vdiv) * (m_k ? 1.0 : 0.0);
+      for (int r = 0; r < ntr; ++r) {
+        const double(*C)[SX] = L.c[r];
+        const double cc = C[cy][cx];
+        double tend = 0.0;
+        if (!ts) tend = p.neg_N2 * (0.5 * (w_k + w_k1));
+        if (adv != ADV_NONE) {
+          const double ue = L.u[ty][tx + 1], uw = L.u[ty][tx];
+          const double vn = L.v[ty + 1][tx], vs = L.v[ty][tx];
+          const double fcx = tracer_face(adv, C[cy][cx - 1], cc, C[cy][cx + 1], C[cy][cx + 2], ue,
+                                         L.m[cy][cx - 1] != 0, L.m[cy][cx + 2] != 0);
+          const double fcxw = tracer_face(adv, C[cy][cx - 2], C[cy][cx - 1], cc, C[cy][cx + 1],
+                                          uw, L.m[cy][cx - 2] != 0, L.m[cy][cx + 1] != 0);
+          const double fcy = tracer_face(adv, C[cy - 1][cx], cc, C[cy + 1][cx], C[cy + 2][cx], vn,
+                                         L.m[cy - 1][cx] != 0, L.m[cy + 2][cx] != 0);
+          const double fcys = tracer_face(adv, C[cy - 2][cx], C[cy - 1][cx], cc, C[cy + 1][cx],
+                                          vs, L.m[cy - 2][cx] != 0, L.m[cy + 1][cx] != 0);
+          const double fx = L.du[ty][tx + 1] * ue * fcx;
+          const double fxw = L.du[ty][tx] * uw * fcxw;
+          const double fy = L.dv[ty + 1][tx] * vn * fcy;
+          const double fys = L.dv[ty][tx] * vs * fcys;
+          const double hdiv = ((fx - fxw) / p.dx + (fy - fys) / p.dy) * inv;
+          const double cft = (k == 0) ? win[r][2] : cf_top[r];
+          const double cfb = tracer_face_z(adv, k + 1, nz, win[r][1], win[r][2], win[r][3],
+                                           win[r][4], mw[1], mw[4], w_k1);
+          cf_top[r] = cfb;
+          const double vdiv = (w_k * cft - w_k1 * cfb) * inv;
+          tend = tend - (hdiv + vdiv) * (m_k ? 1.0 : 0.0);
+        }
+        if (p.has_K_h) {
+          const double gx = mk(L.mu[ty][tx + 1]) * (C[cy][cx + 1] - cc) / p.dx;
+          const double gxw = mk(L.mu[ty][tx]) * (cc - C[cy][cx - 1]) / p.dx;
+          const double gy = mk(L.mv[ty + 1][tx]) * (C[cy + 1][cx] - cc) / p.dy;
+          const double gys = mk(L.mv[ty][tx]) * (cc - C[cy - 1][cx]) / p.dy;
+          tend = tend + p.K_h * ((m_k ? 1.0 : 0.0) * ((gx - gxw) / p.dx + (gy - gys) / p.dy));
+        }
+        // explicit vertical diffusion with the surface flux (vertical_var.py:80-106)
+        double ft;
+        if (k == 0) {
+          ft = flux[r];
+        } else {
+          ft = kh_k * (win[r][1] - win[r][2]) * safe_inv(dzi_k);
+          if (bot_km1) ft = 0.0;
+        }
+        double fb = 0.0;
+        if (k + 1 < nz) {
+          fb = kh_k1 * (win[r][2] - win[r][3]) * safe_inv(dzi_k1);
+          if (bot_k) fb = 0.0;
+        }
+        const double out = (ft - fb) * inv;
+        const double dr = 0.0 * win[r][2] * (bot_k ? 1.0 : 0.0);
+        const double dzd = (out - dr * inv) * (m_k ? 1.0 : 0.0);
+        double rhs = cc + p.dt * tend + p.c_dzd * dzd;
+        if (k == 0) rhs = rhs + p.thv_dt * flux[r] * safe_inv(dz);
+        const long long t = o + c;
+        const double dd = (k == 0) ? rhs / den[t] : (rhs - sub[t] * dprev[r]) / den[t];
+        dprev[r] = dd;
+        (r == 0 ? Ao : S_out)[t] = dd;
       }
-      if (p.has_K_h) {
-        const double gx = mk(L.mu[ty][tx + 1]) * (C[cy][cx + 1] - cc) / p.dx;
-        const double gxw = mk(L.mu[ty][tx]) * (cc - C[cy][cx - 1]) / p.dx;
-        const double gy = mk(L.mv[ty + 1][tx]) * (C[cy + 1][cx] - cc) / p.dy;
-        const double gys = mk(L.mv[ty][tx]) * (cc - C[cy - 1][cx]) / p.dy;
-        tend = tend + p.K_h * ((m_k ? 1.0 : 0.0) * ((gx - gxw) / p.dx + (gy - gys) / p.dy));
-      }
-      // explicit vertical diffusion with the surface flux (vertical_var.py:80-106)
-      double ft;
-      if (k == 0) {
-        ft = flux[r];
+      // ---- advance the windows
+      for (int r = 0; r < 2; ++r)
+        for (int q = 0; q < 4; ++q) win[r][q] = win[r][q + 1];
+      for (int q = 0; q < 4; ++q) mw[q] = mw[q + 1];
+      const int kn = k + 3;
+      if (kn < nz) {
+        win[0][4] = A[(long long)kn * n2 + c];
+        if (ts) win[1][4] = S[(long long)kn * n2 + c];
+        mw[4] = geo.m3[(long long)kn * n2 + c] != 0;
       } else {
-        ft = kh_k * (win[r][1] - win[r][2]) * safe_inv(dzi_k);
-        if (bot_km1) ft = 0.0;
+        win[0][4] = 0.0;
+        win[1][4] = 0.0;
+        mw[4] = false;
       }
-      double fb = 0.0;
-      if (k + 1 < nz) {
-        fb = kh_k1 * (win[r][2] - win[r][3]) * safe_inv(dzi_k1);
-        if (bot_k) fb = 0.0;
-      }
-      const double out = (ft - fb) * inv;
-      const double dr = 0.0 * win[r][2] * (bot_k ? 1.0 : 0.0);
-      const double dzd = (out - dr * inv) * (m_k ? 1.0 : 0.0);
-      double rhs = cc + p.dt * tend + p.c_dzd * dzd;
-      if (k == 0) rhs = rhs + p.thv_dt * flux[r] * safe_inv(dz);
-      const long long t = o + c;
-      const double dd = (k == 0) ? rhs / den[t] : (rhs - sub[t] * dprev[r]) / den[t];
-      dprev[r] = dd;
-      (r == 0 ? Ao : S_out)[t] = dd;
+      w_k = w_k1;
+      kh_k = kh_k1;
+      dzi_k = dzi_k1;
     }
-    // ---- advance the windows
-    for (int r = 0; r < 2; ++r)
-      for (int q = 0; q < 4; ++q) win[r][q] = win[r][q + 1];
-    for (int q = 0; q < 4; ++q) mw[q] = mw[q + 1];
-    const int kn = k + 3;
-    if (kn < nz) {
-      win[0][4] = A[(long long)kn * n2 + c];
-      if (ts) win[1][4] = S[(long long)kn * n2 + c];
-      mw[4] = geo.m3[(long long)kn * n2 + c] != 0;
-    } else {
-      win[0][4] = 0.0;
-      win[1][4] = 0.0;
-      mw[4] = false;
-    }
-    w_k = w_k1;
-    kh_k = kh_k1;
-    dzi_k = dzi_k1;
+    if (k + 1 < nz) tile_store(LB[(k + 1) & 1], R, tid);
+    __syncthreads();
   }
   if (!active) return;
+  // back-substitution with the next level's operands loaded one level ahead
   double xa = 0.0, xb = 0.0;
+  long long t = (long long)(nz - 1) * n2 + c;
+  double a_n = Ao[t], s_n = ts ? S_out[t] : 0.0, c_n = cst[t], z_n = ts ? geo.zc[t] : 0.0;
+  uint8_t m_n = geo.m3[t];
   for (int k = nz - 1; k >= 0; --k) {
-    const long long t = (long long)k * n2 + c;
-    const double mm = mk(geo.m3[t]);
-    xa = (k == nz - 1) ? Ao[t] : Ao[t] - cst[t] * xa;
-    Ao[t] = xa * mm;
+    const double a_k = a_n, s_k = s_n, c_k = c_n, z_k = z_n;
+    const double mm = mk(m_n);
+    const long long tk = t;
+    if (k > 0) {
+      t -= n2;
+      a_n = Ao[t];
+      if (ts) s_n = S_out[t];
+      c_n = cst[t];
+      if (ts) z_n = geo.zc[t];
+      m_n = geo.m3[t];
+    }
+    xa = (k == nz - 1) ? a_k : a_k - c_k * xa;
+    Ao[tk] = xa * mm;
     if (ts) {
-      xb = (k == nz - 1) ? S_out[t] : S_out[t] - cst[t] * xb;
-      S_out[t] = xb * mm;
-      b_out[t] = buoyancy(p, xa * mm, xb * mm, geo.zc[t]) * mm;
+      xb = (k == nz - 1) ? s_k : s_k - c_k * xb;
+      S_out[tk] = xb * mm;
+      b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
     }
   }
 }
@@ -302,7 +389,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * TX, j0 = by * TY;
   const int i = i0 + tx, j = j0 + ty;
   const bool active = i < nx && j < ny;
   const long long c = active ? (long long)j * nx + i : 0;

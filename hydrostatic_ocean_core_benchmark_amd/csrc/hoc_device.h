@@ -71,50 +71,47 @@ constexpr double K1P = 1.0 + KAPPA;
 // kappa = 1/3 face value between c0 = c[i] and c1 = c[i+1] for the face
 // velocity vel; cm = c[i-1], c2 = c[i+2]; wm / w2 tell whether the far
 // upstream cell (i-1 for vel > 0, i+2 for vel < 0) is wet, else upwind1.
-// (libs/core/advection_v06.py:21-47)
+// (libs/core/advection_v06.py:21-47). The reference forms both upwind
+// candidates and selects one by the sign of vel; only the selected one is
+// formed here, with its operands chosen first (one division per face):
+// c1 - t is c1 + (-t) in IEEE arithmetic, so the result is the same bits.
 __device__ __forceinline__ double kappa_face(double cm, double c0, double c1, double c2,
                                              double vel, bool wm, bool w2, bool limited) {
-  double fp, fn;
+  const bool up = vel > 0.0;
+  const double base = up ? c0 : c1;
+  const double num = up ? (c0 - cm) : (c2 - c1);
+  const double dc = c1 - c0;
+  double t;
   if (limited) {
-    const double dc = c1 - c0;
-    const bool z = dc == 0.0;
-    const double eps = z ? 1.0 : dc;
-    const double rp = z ? 0.0 : (c0 - cm) / eps;
-    const double rn = z ? 0.0 : (c2 - c1) / eps;
-    fp = c0 + 0.5 * superbee(rp) * dc;
-    fn = c1 - 0.5 * superbee(rn) * dc;
+    const double r = dc == 0.0 ? 0.0 : num / dc;
+    t = 0.5 * superbee(r) * dc;
   } else {
-    fp = c0 + 0.25 * (K1M * (c0 - cm) + K1P * (c1 - c0));
-    fn = c1 - 0.25 * (K1M * (c2 - c1) + K1P * (c1 - c0));
+    t = 0.25 * (K1M * num + K1P * dc);
   }
-  if (!wm) fp = c0;
-  if (!w2) fn = c1;
-  return vel > 0.0 ? fp : fn;
+  const double f = up ? base + t : base - t;
+  return (up ? wm : w2) ? f : base;
 }
 
 // Interior interface value between above = c[k-1] and below = c[k], w > 0
 // upward; far_up = c[k-2] (0 when k < 2), far_dn = c[k+1] (0 when k > nz-2);
 // has_up / has_dn are the wet flags of those far cells.
-// (libs/core/advection_v06.py:50-74)
+// (libs/core/advection_v06.py:50-74); the selected candidate only, as above.
 __device__ __forceinline__ double kappa_face_z(double far_up, double above, double below,
                                                double far_dn, double w, bool has_up,
                                                bool has_dn, bool limited) {
+  const bool up = w > 0.0;
+  const double base = up ? below : above;
+  const double num = up ? (below - far_dn) : (far_up - above);
   const double d = above - below;
-  double fu, fd;
+  double t;
   if (limited) {
-    const bool z = d == 0.0;
-    const double eps = z ? 1.0 : d;
-    const double ru = z ? 0.0 : (below - far_dn) / eps;
-    const double rd = z ? 0.0 : (far_up - above) / eps;
-    fu = below + 0.5 * superbee(ru) * d;
-    fd = above - 0.5 * superbee(rd) * d;
+    const double r = d == 0.0 ? 0.0 : num / d;
+    t = 0.5 * superbee(r) * d;
   } else {
-    fu = below + 0.25 * (K1M * (below - far_dn) + K1P * (above - below));
-    fd = above - 0.25 * (K1M * (far_up - above) + K1P * (above - below));
+    t = 0.25 * (K1M * num + K1P * d);
   }
-  if (!has_dn) fu = below;
-  if (!has_up) fd = above;
-  return w > 0.0 ? fu : fd;
+  const double f = up ? base + t : base - t;
+  return (up ? has_dn : has_up) ? f : base;
 }
 
 // ------------------------------------------------------------------ EOS

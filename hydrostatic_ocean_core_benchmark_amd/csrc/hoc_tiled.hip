@@ -52,6 +52,15 @@ struct TileLds {
   uint8_t mv[TY + 1][TX];   // mask3v
 };
 
+// Face quantities of one level, each computed once: advective flux
+// dz_f u_f c_f and K_h gradient mask_f (c_1 - c_0) / dx on the x-faces
+// (index q: between tile columns q-1 and q) and y-faces (between rows q-1
+// and q), for both tracers.
+struct FaceLds {
+  double fx[2][TY][TX + 1], gx[2][TY][TX + 1];
+  double fy[2][TY + 1][TX], gy[2][TY + 1][TX];
+};
+
 // Level-k staging of one tile held in registers between its global loads
 // and its LDS stores, so the loads of level k + 1 are in flight while
 // level k is computed (two LDS buffers, one barrier per level).
@@ -151,6 +160,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
     const double* __restrict__ den, const double* __restrict__ cst, double* __restrict__ T_out,
     double* __restrict__ S_out, double* __restrict__ b_out) {
   __shared__ TileLds LB[2];
+  __shared__ FaceLds F;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
@@ -196,10 +206,48 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
   tile_load(R, tid, i0, j0, nx, ny, 0, ts, A, S, u, v, geo);
   tile_store(LB[0], R, tid);
   __syncthreads();
+  const bool has_adv = adv != ADV_NONE;
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
     const TileLds& L = LB[k & 1];
     if (k + 1 < nz) tile_load(R, tid, i0, j0, nx, ny, o + n2, ts, A, S, u, v, geo);
+    // ---- faces: the east and north face of every cell, plus the tile's
+    // west column (8) and south row (32) of faces by threads 0..39
+    {
+      auto xface = [&](int fy_, int q) {
+        const int cy = fy_ + HALO, c0x = q - 1 + HALO;
+        const double vel = L.u[fy_][q];
+        const double ddu = L.du[fy_][q];
+        const double mmu = mk(L.mu[fy_][q]);
+        const bool wm = L.m[cy][c0x - 1] != 0, w2 = L.m[cy][c0x + 2] != 0;
+        for (int r = 0; r < ntr; ++r) {
+          const double(*C)[SX] = L.c[r];
+          if (has_adv)
+            F.fx[r][fy_][q] = ddu * vel * tracer_face(adv, C[cy][c0x - 1], C[cy][c0x],
+                                                      C[cy][c0x + 1], C[cy][c0x + 2], vel, wm, w2);
+          F.gx[r][fy_][q] = mmu * (C[cy][c0x + 1] - C[cy][c0x]) / p.dx;
+        }
+      };
+      auto yface = [&](int q, int fx_) {
+        const int cx = fx_ + HALO, c0y = q - 1 + HALO;
+        const double vel = L.v[q][fx_];
+        const double ddv = L.dv[q][fx_];
+        const double mmv = mk(L.mv[q][fx_]);
+        const bool wm = L.m[c0y - 1][cx] != 0, w2 = L.m[c0y + 2][cx] != 0;
+        for (int r = 0; r < ntr; ++r) {
+          const double(*C)[SX] = L.c[r];
+          if (has_adv)
+            F.fy[r][q][fx_] = ddv * vel * tracer_face(adv, C[c0y - 1][cx], C[c0y][cx],
+                                                      C[c0y + 1][cx], C[c0y + 2][cx], vel, wm, w2);
+          F.gy[r][q][fx_] = mmv * (C[c0y + 1][cx] - C[c0y][cx]) / p.dy;
+        }
+      };
+      xface(ty, tx + 1);
+      yface(ty + 1, tx);
+      if (tid < TY) xface(tid, 0);
+      else if (tid < TY + TX) yface(0, tid - TY);
+    }
+    __syncthreads();
     if (active) {
       // ---- own-column values for level k
       const double w_k1 = w[o + n2 + c];
@@ -213,26 +261,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
       const int cx = tx + HALO, cy = ty + HALO;
 
       for (int r = 0; r < ntr; ++r) {
-        const double(*C)[SX] = L.c[r];
-        const double cc = C[cy][cx];
+        const double cc = L.c[r][cy][cx];
         double tend = 0.0;
         if (!ts) tend = p.neg_N2 * (0.5 * (w_k + w_k1));
-        if (adv != ADV_NONE) {
-          const double ue = L.u[ty][tx + 1], uw = L.u[ty][tx];
-          const double vn = L.v[ty + 1][tx], vs = L.v[ty][tx];
-          const double fcx = tracer_face(adv, C[cy][cx - 1], cc, C[cy][cx + 1], C[cy][cx + 2], ue,
-                                         L.m[cy][cx - 1] != 0, L.m[cy][cx + 2] != 0);
-          const double fcxw = tracer_face(adv, C[cy][cx - 2], C[cy][cx - 1], cc, C[cy][cx + 1],
-                                          uw, L.m[cy][cx - 2] != 0, L.m[cy][cx + 1] != 0);
-          const double fcy = tracer_face(adv, C[cy - 1][cx], cc, C[cy + 1][cx], C[cy + 2][cx], vn,
-                                         L.m[cy - 1][cx] != 0, L.m[cy + 2][cx] != 0);
-          const double fcys = tracer_face(adv, C[cy - 2][cx], C[cy - 1][cx], cc, C[cy + 1][cx],
-                                          vs, L.m[cy - 2][cx] != 0, L.m[cy + 1][cx] != 0);
-          const double fx = L.du[ty][tx + 1] * ue * fcx;
-          const double fxw = L.du[ty][tx] * uw * fcxw;
-          const double fy = L.dv[ty + 1][tx] * vn * fcy;
-          const double fys = L.dv[ty][tx] * vs * fcys;
-          const double hdiv = ((fx - fxw) / p.dx + (fy - fys) / p.dy) * inv;
+        if (has_adv) {
+          const double hdiv = ((F.fx[r][ty][tx + 1] - F.fx[r][ty][tx]) / p.dx +
+                               (F.fy[r][ty + 1][tx] - F.fy[r][ty][tx]) / p.dy) * inv;
           const double cft = (k == 0) ? win[r][2] : cf_top[r];
           const double cfb = tracer_face_z(adv, k + 1, nz, win[r][1], win[r][2], win[r][3],
                                            win[r][4], mw[1], mw[4], w_k1);
@@ -241,11 +275,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
           tend = tend - (hdiv + vdiv) * (m_k ? 1.0 : 0.0);
         }
         if (p.has_K_h) {
-          const double gx = mk(L.mu[ty][tx + 1]) * (C[cy][cx + 1] - cc) / p.dx;
-          const double gxw = mk(L.mu[ty][tx]) * (cc - C[cy][cx - 1]) / p.dx;
-          const double gy = mk(L.mv[ty + 1][tx]) * (C[cy + 1][cx] - cc) / p.dy;
-          const double gys = mk(L.mv[ty][tx]) * (cc - C[cy - 1][cx]) / p.dy;
-          tend = tend + p.K_h * ((m_k ? 1.0 : 0.0) * ((gx - gxw) / p.dx + (gy - gys) / p.dy));
+          tend = tend + p.K_h * ((m_k ? 1.0 : 0.0) *
+                                 ((F.gx[r][ty][tx + 1] - F.gx[r][ty][tx]) / p.dx +
+                                  (F.gy[r][ty + 1][tx] - F.gy[r][ty][tx]) / p.dy));
         }
         // explicit vertical diffusion with the surface flux (vertical_var.py:80-106)
         double ft;

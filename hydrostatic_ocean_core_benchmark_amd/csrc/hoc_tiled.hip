@@ -361,8 +361,81 @@ struct MomLds {
   double phi[TY + 1][TX + 1];           // potential, +1 east / north
   double b[TY + 1][TX + 1];
   double zc[TY + 1][TX + 1];
-  double w[2][TY + 1][TX + 1];          // w at interfaces k, k+1
+  double w[TY + 1][TX + 1];             // w at interface k + 1
 };
+
+// Momentum face fluxes of one level, each computed once: advective fluxes
+// (advecting velocity times the face value) and the A_h gradients, on the
+// x-faces (index q: between tile columns q-1 and q) and y-faces (between
+// rows q-1 and q) of the u and v points.
+struct MomFaces {
+  double fux[TY][TX + 1], fvx[TY][TX + 1], gux[TY][TX + 1], gvx[TY][TX + 1];
+  double fuy[TY + 1][TX], fvy[TY + 1][TX], guy[TY + 1][TX], gvy[TY + 1][TX];
+};
+
+constexpr int MS_H = (SX * SY + TILE_THREADS - 1) / TILE_THREADS;              // halo slots
+constexpr int MS_P = ((TY + 1) * (TX + 1) + TILE_THREADS - 1) / TILE_THREADS;  // +1 slots
+
+struct MomRegs {
+  double u[MS_H], v[MS_H];
+  uint8_t mu[MS_H], mv[MS_H];
+  double phi[MS_P], b[MS_P], zc[MS_P], w[MS_P];
+};
+
+__device__ __forceinline__ void mom_load(MomRegs& R, int tid, int i0, int j0, int nx, int ny,
+                                         long long o, long long n2, const double* u,
+                                         const double* v, const double* phi, const double* b,
+                                         const double* w, const Geo& geo) {
+#pragma unroll
+  for (int q = 0; q < MS_H; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < SX * SY) {
+      const int lx = t % SX, ly = t / SX;
+      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
+      R.u[q] = u[o + g];
+      R.v[q] = v[o + g];
+      R.mu[q] = geo.m3u[o + g];
+      R.mv[q] = geo.m3v[o + g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MS_P; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < (TY + 1) * (TX + 1)) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx, nx);
+      R.phi[q] = phi[o + g];
+      R.b[q] = b[o + g];
+      R.zc[q] = geo.zc[o + g];
+      R.w[q] = w[o + n2 + g];
+    }
+  }
+}
+
+__device__ __forceinline__ void mom_store(MomLds& L, const MomRegs& R, int tid) {
+#pragma unroll
+  for (int q = 0; q < MS_H; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < SX * SY) {
+      const int lx = t % SX, ly = t / SX;
+      L.u[ly][lx] = R.u[q];
+      L.v[ly][lx] = R.v[q];
+      L.mu[ly][lx] = R.mu[q];
+      L.mv[ly][lx] = R.mv[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MS_P; ++q) {
+    const int t = tid + q * TILE_THREADS;
+    if (t < (TY + 1) * (TX + 1)) {
+      const int lx = t % (TX + 1), ly = t / (TX + 1);
+      L.phi[ly][lx] = R.phi[q];
+      L.b[ly][lx] = R.b[q];
+      L.zc[ly][lx] = R.zc[q];
+      L.w[ly][lx] = R.w[q];
+    }
+  }
+}
 
 // Vertical momentum flux on interface kk of the column: wb times the
 // interface value of the advected velocity (advection_v06.py:95-99,
@@ -410,13 +483,16 @@ __device__ __forceinline__ double explicit_diffusion_w(int k, int nz, const doub
 
 // The step's explicit momentum terms (libs/core/model3d_v05.py:352-370),
 // tile by tile and level by level; same outputs as k_momentum_explicit.
+// Staging is double-buffered (level k + 1 loads in flight while level k is
+// computed) and every horizontal face flux is formed once.
 __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     Dims d, StepParams p, Geo geo, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ b, const double* __restrict__ phi, const double* __restrict__ w,
     const double* __restrict__ Km, double* __restrict__ pgx, double* __restrict__ pgy,
     double* __restrict__ avu, double* __restrict__ avv, double* __restrict__ dzu,
     double* __restrict__ dzv, double* __restrict__ stu, double* __restrict__ stv) {
-  __shared__ MomLds L;
+  __shared__ MomLds LB[2];
+  __shared__ MomFaces F;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
@@ -429,7 +505,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
   const long long c = active ? (long long)j * nx + i : 0;
   int adv = p.advection;
   if (adv == ADV_UPWIND1) adv = ADV_CENTRED2;  // momentum keeps the centred form
-  const bool up3 = adv == ADV_UP3 || adv == ADV_UP3_TVD;
   const bool lim = adv == ADV_UP3_TVD;
 
   // own-column windows at k-2..k+2: u, v, mask3u, mask3v; and k-1..k+1 of
@@ -469,25 +544,67 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
   double fzu_top = 0.0, fzv_top = 0.0;
   const int cx = tx + HALO, cy = ty + HALO;
 
+  MomRegs R;
+  mom_load(R, tid, i0, j0, nx, ny, 0, n2, u, v, phi, b, w, geo);
+  mom_store(LB[0], R, tid);
+  __syncthreads();
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
-    __syncthreads();
-    for (int t = tid; t < SX * SY; t += TILE_THREADS) {
-      const int lx = t % SX, ly = t / SX;
-      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
-      L.u[ly][lx] = u[o + g];
-      L.v[ly][lx] = v[o + g];
-      L.mu[ly][lx] = geo.m3u[o + g];
-      L.mv[ly][lx] = geo.m3v[o + g];
-    }
-    for (int t = tid; t < (TY + 1) * (TX + 1); t += TILE_THREADS) {
-      const int lx = t % (TX + 1), ly = t / (TX + 1);
-      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx, nx);
-      L.phi[ly][lx] = phi[o + g];
-      L.b[ly][lx] = b[o + g];
-      L.zc[ly][lx] = geo.zc[o + g];
-      L.w[0][ly][lx] = w[o + g];
-      L.w[1][ly][lx] = w[o + n2 + g];
+    const MomLds& L = LB[k & 1];
+    if (k + 1 < nz) mom_load(R, tid, i0, j0, nx, ny, o + n2, n2, u, v, phi, b, w, geo);
+    {
+      const double(*U)[SX] = L.u;
+      const double(*V)[SX] = L.v;
+      const uint8_t(*MU)[SX] = L.mu;
+      const uint8_t(*MV)[SX] = L.mv;
+      // x-face q of row fy_ (between columns q-1 and q)
+      auto xface = [&](int fy_, int q) {
+        const int yy = fy_ + HALO, x0 = q - 1 + HALO;  // x0: the column west of the face
+        if (adv != ADV_NONE) {
+          const double ub = 0.5 * (U[yy][x0] + U[yy][x0 + 1]);
+          const double uu = 0.5 * (U[yy][x0] + U[yy + 1][x0]);
+          if (adv == ADV_CENTRED2) {
+            F.fux[fy_][q] = ub * ub;
+            F.fvx[fy_][q] = uu * (0.5 * (V[yy][x0] + V[yy][x0 + 1]));
+          } else {
+            F.fux[fy_][q] = ub * kappa_face(U[yy][x0 - 1], U[yy][x0], U[yy][x0 + 1],
+                                            U[yy][x0 + 2], ub, MU[yy][x0 - 1] != 0,
+                                            MU[yy][x0 + 2] != 0, lim);
+            F.fvx[fy_][q] = uu * kappa_face(V[yy][x0 - 1], V[yy][x0], V[yy][x0 + 1],
+                                            V[yy][x0 + 2], uu, MV[yy][x0 - 1] != 0,
+                                            MV[yy][x0 + 2] != 0, lim);
+          }
+        }
+        const double m = mk(MU[yy][x0]);
+        F.gux[fy_][q] = m * (U[yy][x0 + 1] - U[yy][x0]) / p.dx;
+        F.gvx[fy_][q] = m * (V[yy][x0 + 1] - V[yy][x0]) / p.dx;
+      };
+      // y-face q of column fx_ (between rows q-1 and q)
+      auto yface = [&](int q, int fx_) {
+        const int xx = fx_ + HALO, y0 = q - 1 + HALO;  // y0: the row south of the face
+        if (adv != ADV_NONE) {
+          const double vb = 0.5 * (V[y0][xx] + V[y0][xx + 1]);
+          const double vv = 0.5 * (V[y0][xx] + V[y0 + 1][xx]);
+          if (adv == ADV_CENTRED2) {
+            F.fuy[q][fx_] = vb * (0.5 * (U[y0][xx] + U[y0 + 1][xx]));
+            F.fvy[q][fx_] = vv * vv;
+          } else {
+            F.fuy[q][fx_] = vb * kappa_face(U[y0 - 1][xx], U[y0][xx], U[y0 + 1][xx],
+                                            U[y0 + 2][xx], vb, MU[y0 - 1][xx] != 0,
+                                            MU[y0 + 2][xx] != 0, lim);
+            F.fvy[q][fx_] = vv * kappa_face(V[y0 - 1][xx], V[y0][xx], V[y0 + 1][xx],
+                                            V[y0 + 2][xx], vv, MV[y0 - 1][xx] != 0,
+                                            MV[y0 + 2][xx] != 0, lim);
+          }
+        }
+        const double m = mk(MV[y0][xx]);
+        F.guy[q][fx_] = m * (U[y0 + 1][xx] - U[y0][xx]) / p.dy;
+        F.gvy[q][fx_] = m * (V[y0 + 1][xx] - V[y0][xx]) / p.dy;
+      };
+      xface(ty, tx + 1);
+      yface(ty + 1, tx);
+      if (tid < TY) xface(tid, 0);
+      else if (tid < TY + TX) yface(0, tid - TY);
     }
     __syncthreads();
     if (active) {
@@ -543,59 +660,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       // advection and horizontal viscosity
       double exu = 0.0, exv = 0.0;
       if (adv != ADV_NONE) {
-        double ax, ay, bx, by;
-        const double(*U)[SX] = L.u;
-        const double(*V)[SX] = L.v;
-        const uint8_t(*MU)[SX] = L.mu;
-        const uint8_t(*MV)[SX] = L.mv;
-        if (adv == ADV_CENTRED2) {
-          const double ub = 0.5 * (U[cy][cx - 1] + U[cy][cx]);
-          const double ube = 0.5 * (U[cy][cx] + U[cy][cx + 1]);
-          ax = (ube * ube - ub * ub) / p.dx;
-          const double vb = 0.5 * (V[cy][cx] + V[cy][cx + 1]);
-          const double vbs = 0.5 * (V[cy - 1][cx] + V[cy - 1][cx + 1]);
-          ay = (vb * (0.5 * (U[cy][cx] + U[cy + 1][cx])) -
-                vbs * (0.5 * (U[cy - 1][cx] + U[cy][cx]))) / p.dy;
-          const double vv = 0.5 * (V[cy - 1][cx] + V[cy][cx]);
-          const double vvn = 0.5 * (V[cy][cx] + V[cy + 1][cx]);
-          by = (vvn * vvn - vv * vv) / p.dy;
-          const double uu = 0.5 * (U[cy][cx] + U[cy + 1][cx]);
-          const double uuw = 0.5 * (U[cy][cx - 1] + U[cy + 1][cx - 1]);
-          bx = (uu * (0.5 * (V[cy][cx] + V[cy][cx + 1])) -
-                uuw * (0.5 * (V[cy][cx - 1] + V[cy][cx]))) / p.dx;
-        } else {
-          const double ub = 0.5 * (U[cy][cx - 1] + U[cy][cx]);
-          const double ube = 0.5 * (U[cy][cx] + U[cy][cx + 1]);
-          const double uc = kappa_face(U[cy][cx - 2], U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], ub,
-                                       MU[cy][cx - 2] != 0, MU[cy][cx + 1] != 0, lim);
-          const double uce = kappa_face(U[cy][cx - 1], U[cy][cx], U[cy][cx + 1], U[cy][cx + 2],
-                                        ube, MU[cy][cx - 1] != 0, MU[cy][cx + 2] != 0, lim);
-          ax = (ube * uce - ub * uc) / p.dx;
-          const double vb = 0.5 * (V[cy][cx] + V[cy][cx + 1]);
-          const double vbs = 0.5 * (V[cy - 1][cx] + V[cy - 1][cx + 1]);
-          const double uy = kappa_face(U[cy - 1][cx], U[cy][cx], U[cy + 1][cx], U[cy + 2][cx], vb,
-                                       MU[cy - 1][cx] != 0, MU[cy + 2][cx] != 0, lim);
-          const double uys = kappa_face(U[cy - 2][cx], U[cy - 1][cx], U[cy][cx], U[cy + 1][cx],
-                                        vbs, MU[cy - 2][cx] != 0, MU[cy + 1][cx] != 0, lim);
-          ay = (vb * uy - vbs * uys) / p.dy;
-          const double vv = 0.5 * (V[cy - 1][cx] + V[cy][cx]);
-          const double vvn = 0.5 * (V[cy][cx] + V[cy + 1][cx]);
-          const double vc = kappa_face(V[cy - 2][cx], V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], vv,
-                                       MV[cy - 2][cx] != 0, MV[cy + 1][cx] != 0, lim);
-          const double vcn = kappa_face(V[cy - 1][cx], V[cy][cx], V[cy + 1][cx], V[cy + 2][cx],
-                                        vvn, MV[cy - 1][cx] != 0, MV[cy + 2][cx] != 0, lim);
-          by = (vvn * vcn - vv * vc) / p.dy;
-          const double uu = 0.5 * (U[cy][cx] + U[cy + 1][cx]);
-          const double uuw = 0.5 * (U[cy][cx - 1] + U[cy + 1][cx - 1]);
-          const double vx = kappa_face(V[cy][cx - 1], V[cy][cx], V[cy][cx + 1], V[cy][cx + 2], uu,
-                                       MV[cy][cx - 1] != 0, MV[cy][cx + 2] != 0, lim);
-          const double vxw = kappa_face(V[cy][cx - 2], V[cy][cx - 1], V[cy][cx], V[cy][cx + 1],
-                                        uuw, MV[cy][cx - 2] != 0, MV[cy][cx + 1] != 0, lim);
-          bx = (uu * vx - uuw * vxw) / p.dx;
-        }
+        const double ax = (F.fux[ty][tx + 1] - F.fux[ty][tx]) / p.dx;
+        const double ay = (F.fuy[ty + 1][tx] - F.fuy[ty][tx]) / p.dy;
+        const double by_ = (F.fvy[ty + 1][tx] - F.fvy[ty][tx]) / p.dy;
+        const double bx_ = (F.fvx[ty][tx + 1] - F.fvx[ty][tx]) / p.dx;
         // vertical fluxes: the top one is the bottom one of the level above
-        const double wbu = 0.5 * (L.w[1][ty][tx] + L.w[1][ty][tx + 1]);
-        const double wbv = 0.5 * (L.w[1][ty][tx] + L.w[1][ty + 1][tx]);
+        const double wbu = 0.5 * (L.w[ty][tx] + L.w[ty][tx + 1]);
+        const double wbv = 0.5 * (L.w[ty][tx] + L.w[ty + 1][tx]);
         const double fzu_bot = mom_flux_z(adv, k + 1, nz, wbu, uw5[1], uw5[2], uw5[3], uw5[4],
                                           muw[1], muw[4]);
         const double fzv_bot = mom_flux_z(adv, k + 1, nz, wbv, vw5[1], vw5[2], vw5[3], vw5[4],
@@ -604,27 +675,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
         const double bz = (fzv_top - fzv_bot) / p.dz_nom;
         fzu_top = fzu_bot;
         fzv_top = fzv_bot;
-        (void)up3;
         exu = exu - (ax + ay + az) * mu_c;
-        exv = exv - (bx + by + bz) * mv_c;
+        exv = exv - (bx_ + by_ + bz) * mv_c;
       }
       if (p.has_A_h) {
-        const double(*U)[SX] = L.u;
-        const double(*V)[SX] = L.v;
-        {
-          const double g1 = mk(L.mu[cy][cx]) * (U[cy][cx + 1] - U[cy][cx]) / p.dx;
-          const double g0 = mk(L.mu[cy][cx - 1]) * (U[cy][cx] - U[cy][cx - 1]) / p.dx;
-          const double h1 = mk(L.mv[cy][cx]) * (U[cy + 1][cx] - U[cy][cx]) / p.dy;
-          const double h0 = mk(L.mv[cy - 1][cx]) * (U[cy][cx] - U[cy - 1][cx]) / p.dy;
-          exu = exu + p.A_h * (mu_c * ((g1 - g0) / p.dx + (h1 - h0) / p.dy));
-        }
-        {
-          const double g1 = mk(L.mu[cy][cx]) * (V[cy][cx + 1] - V[cy][cx]) / p.dx;
-          const double g0 = mk(L.mu[cy][cx - 1]) * (V[cy][cx] - V[cy][cx - 1]) / p.dx;
-          const double h1 = mk(L.mv[cy][cx]) * (V[cy + 1][cx] - V[cy][cx]) / p.dy;
-          const double h0 = mk(L.mv[cy - 1][cx]) * (V[cy][cx] - V[cy - 1][cx]) / p.dy;
-          exv = exv + p.A_h * (mv_c * ((g1 - g0) / p.dx + (h1 - h0) / p.dy));
-        }
+        exu = exu + p.A_h * (mu_c * ((F.gux[ty][tx + 1] - F.gux[ty][tx]) / p.dx +
+                                     (F.guy[ty + 1][tx] - F.guy[ty][tx]) / p.dy));
+        exv = exv + p.A_h * (mv_c * ((F.gvx[ty][tx + 1] - F.gvx[ty][tx]) / p.dx +
+                                     (F.gvy[ty + 1][tx] - F.gvy[ty][tx]) / p.dy));
       }
       double s0u = 0.0, s0v = 0.0;
       if (k == 0) {
@@ -657,6 +715,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       dvw[2] = k2 < nz ? geo.dz3v[(long long)k2 * n2 + c] : 0.0;
       m3w[2] = k2 < nz && geo.m3[(long long)k2 * n2 + c] != 0;
     }
+    if (k + 1 < nz) mom_store(LB[(k + 1) & 1], R, tid);
+    __syncthreads();
   }
 }
 

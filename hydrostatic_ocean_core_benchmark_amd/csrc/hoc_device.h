@@ -114,6 +114,36 @@ __device__ __forceinline__ double kappa_face_z(double far_up, double above, doub
   return (up ? has_dn : has_up) ? f : base;
 }
 
+// Explicit vertical diffusion of a velocity at level k from its column
+// values f0..f2 (levels k-1..k+1), thicknesses dz0..dz2 and cell masks
+// m0..m2 (libs/core/vertical_var.py:80-106, mask3 branch).
+__device__ __forceinline__ double explicit_diffusion_w(int k, int nz, double f0, double f1,
+                                                       double f2, double dz0, double dz1,
+                                                       double dz2, bool m0, bool m1, bool m2,
+                                                       double nu_k, double nu_k1, double sflux,
+                                                       double drag) {
+  const bool bot_km1 = k >= 1 && m0 && !m1;
+  const bool bot_k = m1 && !((k + 1 < nz) && m2);
+  double ft;
+  if (k == 0) {
+    ft = sflux;
+  } else {
+    const double dzi = (k == nz) ? dz0 : 0.5 * (dz0 + dz1);
+    ft = nu_k * (f0 - f1) * safe_inv(dzi);
+    if (bot_km1) ft = 0.0;
+  }
+  double fb = 0.0;
+  if (k + 1 < nz) {
+    const double dzi = 0.5 * (dz1 + dz2);
+    fb = nu_k1 * (f1 - f2) * safe_inv(dzi);
+    if (bot_k) fb = 0.0;
+  }
+  const double inv = safe_inv(dz1);
+  const double out = (ft - fb) * inv;
+  const double dr = drag * f1 * (bot_k ? 1.0 : 0.0);
+  return (out - dr * inv) * (m1 ? 1.0 : 0.0);
+}
+
 // ------------------------------------------------------------------ EOS
 // libs/core/eos.py:68-133; coefficients of Roquet et al. (2015).
 __device__ __forceinline__ double rho_teos10(double CT, double SA, double z) {

@@ -248,6 +248,73 @@ __device__ __forceinline__ Two pcg_update_elem(const PcgArgs& a, long long e, do
   return Two{r * r, r * z};
 }
 
+// Loads of one element of the A phase that do not depend on beta, issued
+// before the scalar reductions so their latency overlaps them.
+struct ApLoads {
+  double z0, z1, z2, z3, z4, p0, p1, p2, p3, p4;  // centre, e, w, n, s
+  double mc, me, mw, mn, ms, ku, kuw, kv, kvs;
+};
+
+__device__ __forceinline__ ApLoads ap_load(const PcgArgs& a, long long e, bool first,
+                                           const double* pnew, const double* pold) {
+  const Dims& d = a.d;
+  const HelmOp& op = a.op;
+  const int i = (int)(e % d.nx), j = (int)(e / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
+  ApLoads L;
+  const double* pp = first ? pnew : pold;
+  L.p0 = pp[e];
+  L.p1 = pp[ce];
+  L.p2 = pp[cw];
+  L.p3 = pp[cn];
+  L.p4 = pp[cs];
+  if (first) {
+    L.z0 = L.z1 = L.z2 = L.z3 = L.z4 = 0.0;
+  } else {
+    L.z0 = a.z[e];
+    L.z1 = a.z[ce];
+    L.z2 = a.z[cw];
+    L.z3 = a.z[cn];
+    L.z4 = a.z[cs];
+  }
+  L.mc = mk(op.mask[e]);
+  L.me = mk(op.mask[ce]);
+  L.mw = mk(op.mask[cw]);
+  L.mn = mk(op.mask[cn]);
+  L.ms = mk(op.mask[cs]);
+  L.ku = op.Ku[e];
+  L.kuw = op.Ku[cw];
+  L.kv = op.Kv[e];
+  L.kvs = op.Kv[cs];
+  return L;
+}
+
+// pcg_apply_elem from preloaded operands (same arithmetic).
+__device__ __forceinline__ double ap_finish(const PcgArgs& a, long long e, bool first, double bt,
+                                            double* pnew, const ApLoads& L) {
+  const HelmOp& op = a.op;
+  const double pc = first ? L.p0 : L.z0 + bt * L.p0;
+  const double pe = first ? L.p1 : L.z1 + bt * L.p1;
+  const double pw = first ? L.p2 : L.z2 + bt * L.p2;
+  const double pn = first ? L.p3 : L.z3 + bt * L.p3;
+  const double ps = first ? L.p4 : L.z4 + bt * L.p4;
+  const double xc = pc * L.mc;
+  const double xe = pe * L.me;
+  const double xw = pw * L.mw;
+  const double xn = pn * L.mn;
+  const double xs = ps * L.ms;
+  const double fx = L.ku * (xe - xc);
+  const double fxw = L.kuw * (xc - xw);
+  const double fy = L.kv * (xn - xc);
+  const double fys = L.kvs * (xc - xs);
+  const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+  const double ap = (xc - op.coef * lap) * L.mc + xc * (1.0 - L.mc);
+  if (!first) pnew[e] = pc;
+  a.Ap[e] = ap;
+  return pc * ap;
+}
+
 __device__ __forceinline__ int logical_block(int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous range of parts so its L2 holds one slab of the vectors.
@@ -406,14 +473,27 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
   if (a.sc->done) return;
   __shared__ double lds[2 * PCG_STAGE];
   __shared__ double leafs[128];
-  __shared__ double cs[PCG_MAX_CHUNKS];
+  __shared__ double cs[2 * PCG_MAX_CHUNKS];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const bool first = it == 1;
+  double* pnew = (it & 1) ? a.pb1 : a.pb0;
+  const double* pold = (it & 1) ? a.pb0 : a.pb1;
+  const int part = blockIdx.x;
+  const PartRange pr = part_range(a, part);
+  // operands of this part's (at most two per thread) elements first
+  const int t0 = threadIdx.x, t1 = threadIdx.x + PCG_WG;
+  const bool h0 = t0 < pr.nel, h1 = t1 < pr.nel;
+  const ApLoads L0 = ap_load(a, pr.base + (h0 ? t0 : 0), first, pnew, pold);
+  const ApLoads L1 = ap_load(a, pr.base + (h1 ? t1 : 0), first, pnew, pold);
   double beta = 0.0;
-  if (it == 1) {
+  if (first) {
     const double rz = total_from_parts(a, a.part_b, cs);
     if (lead) a.sc->rz2[1] = rz;
   } else {
-    const double res = sqrt(total_from_parts(a, a.part_b, cs));
+    const double* parts[2] = {a.part_b, a.part_c};
+    double tot[2];
+    totals_from_parts<2>(a, parts, cs, tot);
+    const double res = sqrt(tot[0]);
     if (res <= a.sc->tol) {
       if (lead) {
         a.sc->res = res;
@@ -435,46 +515,60 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
       }
       return;
     }
-    const double rz_new = total_from_parts(a, a.part_c, cs);
+    const double rz_new = tot[1];
     beta = rz_new / a.sc->rz2[(it - 1) & 1];
     if (lead) a.sc->rz2[it & 1] = rz_new;
   }
-  double* pnew = (it & 1) ? a.pb1 : a.pb0;
-  const double* pold = (it & 1) ? a.pb0 : a.pb1;
-  const int part = blockIdx.x;
-  const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
-    return pcg_apply_elem(a, e, it == 1, beta, pnew, pold);
-  }, lds, leafs);
+  if (h0) lds[t0] = ap_finish(a, pr.base + t0, first, beta, pnew, L0);
+  if (h1) lds[t1] = ap_finish(a, pr.base + t1, first, beta, pnew, L1);
+  const Two s = part_reduce<1>(pr, lds, leafs);
   if (threadIdx.x == 0) a.part_a[part] = s.a;
 }
 
-// Iteration `it`, update phase: alpha = rz / p.Ap, then x, r and the
-// partial sums of r*r; with the Jacobi preconditioner also z = M^-1 r and the
-// partial sums of r*z (other preconditioners compute z in their own
+// Iteration `it`, phase B: alpha = rz / (p.Ap); x += alpha p; r -= alpha Ap;
+// with the Jacobi preconditioner also z = inv_diag r and the partial sums
+// of r*r and r*z, otherwise only r*r (the preconditioner follows in its own
 // launches and then k_pcg_rz).
 __global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a, int it, int jacobi) {
   if (a.sc->done) return;
   __shared__ double lds[2 * PCG_STAGE];
   __shared__ double leafs[128];
   __shared__ double cs[PCG_MAX_CHUNKS];
-  const double al = a.sc->rz2[it & 1] / total_from_parts(a, a.part_a, cs);
   const double* pnew = (it & 1) ? a.pb1 : a.pb0;
   const int part = blockIdx.x;
+  const PartRange pr = part_range(a, part);
+  // element operands first (independent of alpha); element base + t for
+  // t = tid and tid + PCG_WG (a part has at most 2 * PCG_WG elements)
+  const int t0 = threadIdx.x, t1 = threadIdx.x + PCG_WG;
+  const bool h0 = t0 < pr.nel, h1 = t1 < pr.nel;
+  const long long e0 = pr.base + (h0 ? t0 : 0), e1 = pr.base + (h1 ? t1 : 0);
+  const double x0 = a.x[e0], p0 = pnew[e0], r0 = a.r[e0], ap0 = a.Ap[e0];
+  const double x1 = a.x[e1], p1 = pnew[e1], r1 = a.r[e1], ap1 = a.Ap[e1];
+  const double d0 = jacobi ? a.inv_diag[e0] : 0.0, d1 = jacobi ? a.inv_diag[e1] : 0.0;
+  const double al = a.sc->rz2[it & 1] / total_from_parts(a, a.part_a, cs);
+  auto fin = [&](bool h, int t, long long e, double xv, double pv, double rv, double av,
+                 double dv) {
+    if (!h) return;
+    a.x[e] = xv + al * pv;
+    const double r = rv - al * av;
+    a.r[e] = r;
+    lds[t] = r * r;
+    if (jacobi) {
+      const double z = dv * r;
+      a.z[e] = z;
+      lds[PCG_STAGE + t] = r * z;
+    }
+  };
+  fin(h0, t0, e0, x0, p0, r0, ap0, d0);
+  fin(h1, t1, e1, x1, p1, r1, ap1, d1);
   if (jacobi) {
-    const Two s = part_sum<2>(a, part, [&](long long e) -> Two {
-      return pcg_update_elem(a, e, al, pnew);
-    }, lds, leafs);
+    const Two s = part_reduce<2>(pr, lds, leafs);
     if (threadIdx.x == 0) {
       a.part_b[part] = s.a;
       a.part_c[part] = s.b;
     }
   } else {
-    const Two s = part_sum<1>(a, part, [&](long long e) -> Two {
-      a.x[e] = a.x[e] + al * pnew[e];
-      const double r = a.r[e] - al * a.Ap[e];
-      a.r[e] = r;
-      return Two{r * r, 0.0};
-    }, lds, leafs);
+    const Two s = part_reduce<1>(pr, lds, leafs);
     if (threadIdx.x == 0) a.part_b[part] = s.a;
   }
 }

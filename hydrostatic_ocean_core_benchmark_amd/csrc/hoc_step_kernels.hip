@@ -489,8 +489,8 @@ __global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* __restrict
 }
 
 // ------------------------------------------------ explicit momentum terms
-struct Nb {  // periodic neighbour offsets of a column
-  long long c, e, w, ee, ww, n, s, nn, ss, ne, nw, se, sw;
+struct Nb {  // periodic neighbour offsets of a column (a plane has < 2^31 columns)
+  int c, e, w, ee, ww, n, s, nn, ss, ne, nw, se, sw;
 };
 __device__ __forceinline__ Nb neighbours(const Dims& d, int j, int i) {
   Nb q;
@@ -498,19 +498,19 @@ __device__ __forceinline__ Nb neighbours(const Dims& d, int j, int i) {
   const int iee = wrap(i, 2, d.nx), iww = wrap(i, -2, d.nx);
   const int jn = wrap(j, 1, d.ny), js = wrap(j, -1, d.ny);
   const int jnn = wrap(j, 2, d.ny), jss = wrap(j, -2, d.ny);
-  q.c = at2(d, j, i);
-  q.e = at2(d, j, ie);
-  q.w = at2(d, j, iw);
-  q.ee = at2(d, j, iee);
-  q.ww = at2(d, j, iww);
-  q.n = at2(d, jn, i);
-  q.s = at2(d, js, i);
-  q.nn = at2(d, jnn, i);
-  q.ss = at2(d, jss, i);
-  q.ne = at2(d, jn, ie);
-  q.nw = at2(d, jn, iw);
-  q.se = at2(d, js, ie);
-  q.sw = at2(d, js, iw);
+  q.c = (int)at2(d, j, i);
+  q.e = (int)at2(d, j, ie);
+  q.w = (int)at2(d, j, iw);
+  q.ee = (int)at2(d, j, iee);
+  q.ww = (int)at2(d, j, iww);
+  q.n = (int)at2(d, jn, i);
+  q.s = (int)at2(d, js, i);
+  q.nn = (int)at2(d, jnn, i);
+  q.ss = (int)at2(d, jss, i);
+  q.ne = (int)at2(d, jn, ie);
+  q.nw = (int)at2(d, jn, iw);
+  q.se = (int)at2(d, js, ie);
+  q.sw = (int)at2(d, js, iw);
   return q;
 }
 

@@ -452,35 +452,6 @@ __device__ __forceinline__ double mom_flux_z(int adv, int kk, int nz, double wb,
                            hu, hd, adv == ADV_UP3_TVD);
 }
 
-// Explicit vertical diffusion of a face velocity at level k from its column
-// window f[k-1..k+1], thickness window dz[k-1..k+1] and cell-mask window
-// m[k-1..k+1] (libs/core/vertical_var.py:80-106, mask3 branch).
-__device__ __forceinline__ double explicit_diffusion_w(int k, int nz, const double* f,
-                                                       const double* dz, const bool* m,
-                                                       double nu_k, double nu_k1, double sflux,
-                                                       double drag) {
-  const bool bot_km1 = k >= 1 && m[0] && !m[1];
-  const bool bot_k = m[1] && !((k + 1 < nz) && m[2]);
-  double ft;
-  if (k == 0) {
-    ft = sflux;
-  } else {
-    const double dzi = (k == nz) ? dz[0] : 0.5 * (dz[0] + dz[1]);
-    ft = nu_k * (f[0] - f[1]) * safe_inv(dzi);
-    if (bot_km1) ft = 0.0;
-  }
-  double fb = 0.0;
-  if (k + 1 < nz) {
-    const double dzi = 0.5 * (dz[1] + dz[2]);
-    fb = nu_k1 * (f[1] - f[2]) * safe_inv(dzi);
-    if (bot_k) fb = 0.0;
-  }
-  const double inv = safe_inv(dz[1]);
-  const double out = (ft - fb) * inv;
-  const double dr = drag * f[1] * (bot_k ? 1.0 : 0.0);
-  return (out - dr * inv) * (m[1] ? 1.0 : 0.0);
-}
-
 // The step's explicit momentum terms (libs/core/model3d_v05.py:352-370),
 // tile by tile and level by level; same outputs as k_momentum_explicit.
 // Staging is double-buffered (level k + 1 loads in flight while level k is
@@ -653,9 +624,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       }
       // explicit vertical diffusion with surface stress and bottom drag
       const double km_k1 = Km[o + n2 + c];
-      dzu[t] = explicit_diffusion_w(k, nz, &uw5[1], duw, m3w, km_k, km_k1,
+      dzu[t] = explicit_diffusion_w(k, nz, uw5[1], uw5[2], uw5[3], duw[0], duw[1], duw[2],
+                                    m3w[0], m3w[1], m3w[2], km_k, km_k1,
                                     k == 0 ? geo.tau_u[c] : 0.0, p.bottom_drag);
-      dzv[t] = explicit_diffusion_w(k, nz, &vw5[1], dvw, m3w, km_k, km_k1,
+      dzv[t] = explicit_diffusion_w(k, nz, vw5[1], vw5[2], vw5[3], dvw[0], dvw[1], dvw[2],
+                                    m3w[0], m3w[1], m3w[2], km_k, km_k1,
                                     k == 0 ? geo.tau_v[c] : 0.0, p.bottom_drag);
       // advection and horizontal viscosity
       double exu = 0.0, exv = 0.0;

@@ -55,6 +55,12 @@ HBM_PEAK_GBS = 8000.0
 #   k_momentum_explicit  u, v, b, w, K_m read; 8 explicit terms written (104 B)
 ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 128.0,
                       "k_momentum_explicit": 104.0}
+# Measured HBM bytes per launch at 512 x 512 x 40 (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
+# the summary is profiles/basin512x512x40_hbm_traffic_pmc.md). Re-measured
+# whenever the kernel changes.
+MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.902e9, "k_momentum_explicit": 1.584e9,
+                           "k_predictor": 2.190e9, "k_closure": 1.404e9}
 STEP_BYTES_PER_CELL = 336.0
 SOLVE_BYTES_PER_COLUMN = 64.0
 ITER_BYTES_PER_COLUMN = {"pcg_mg": 520.0, "pcg_jacobi": 120.0, "pcg_rbgs": 520.0,
@@ -180,9 +186,11 @@ def main() -> int:
         avg_s = kms / launches / 1e3
         alg = ALG_BYTES_PER_CELL[args.kernel] * cells
         achieved = alg / avg_s / 1e9
+        traffic = (MEASURED_TRAFFIC_512x40.get(args.kernel)
+                   if (g.nx, g.ny, g.nz) == (512, 512, 40) else None)
         roofline = {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": None, "alg_bytes_per_launch": alg,
+                    "traffic": traffic, "alg_bytes_per_launch": alg,
                     "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
     n_solves = prob.params.n_picard if prob.params.name == "theta" else 0
     solve_bytes = g.nx * g.ny * (SOLVE_BYTES_PER_COLUMN * n_solves

@@ -163,6 +163,7 @@ struct hoc_stepper {
   // multigrid hierarchy (level 0 = the model grid)
   static constexpr int MAXLEV = 12;
   int nlev = 1;
+  int mg_cycle_l0 = -1;  // first level of the one-workgroup coarse cycle, -1: none
   MgLevel lev[MAXLEV];
   double *mg_Ku[MAXLEV] = {}, *mg_Kv[MAXLEV] = {}, *mg_inv[MAXLEV] = {};
   uint8_t* mg_mask[MAXLEV] = {};
@@ -480,23 +481,28 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
     rbgs_sweep(h, 0, r, z, false, s);
     rbgs_sweep(h, 0, r, z, true, s);
   } else if (h->cfg.mg_sweeps <= MG_FUSED_MAX_SWEEPS) {
-    // fused legs: one launch down and one up per level, the coarsest level
-    // in one workgroup when it fits
+    // fused legs: one launch down and one up per level above the first level
+    // of at most MG_COARSE_MAX cells; from there to the last level the cycle
+    // runs in one workgroup in LDS
     const int S = h->cfg.mg_sweeps, last = h->nlev - 1;
     const double om = h->cfg.mg_omega;
     auto level_b = [&](int l) -> const double* { return l == 0 ? r : h->mg_b[l]; };
-    for (int l = 0; l < last; ++l)
+    const int l0 = h->mg_cycle_l0 >= 0 ? h->mg_cycle_l0 : last;
+    for (int l = 0; l < l0; ++l)
       launch_mg_down(S, mg_tile_grid(h->lev[l]), s, h->lev[l], h->lev[l + 1], om, level_b(l),
                      h->mg_x[l], h->mg_b[l + 1], h->sc);
-    double* ec = last == 0 ? z : h->mg_x[last];
-    if (h->lev[last].n <= MG_COARSE_MAX) {
-      k_mg_coarsest<<<1, MG_COARSE_THREADS, 0, s>>>(h->lev[last], om, 2 * S, level_b(last), ec,
-                                                   h->sc);
+    double* ec = l0 == 0 ? z : h->mg_x[l0];
+    if (h->mg_cycle_l0 >= 0) {
+      MgCycle C{};
+      C.nl = last - l0 + 1;
+      for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
+      const size_t lds = mg_cycle_layout(C);
+      launch_mg_coarse_cycle(C, lds, om, S, level_b(l0), ec, h->sc, s);
     } else {
       double* out = vcycle_jacobi(h, last, level_b(last), ec, h->mg_t[last], s);
       if (out != ec) return fail(HOC_ERR_DEVICE, "pcg_mg: coarse result left its buffer");
     }
-    for (int l = last - 1; l >= 0; --l) {
+    for (int l = l0 - 1; l >= 0; --l) {
       double* out = l == 0 ? z : h->mg_t[l];
       launch_mg_up(S, mg_tile_grid(h->lev[l]), s, h->lev[l], h->lev[l + 1].nx, om, h->mg_x[l],
                    ec, level_b(l), out, h->sc);
@@ -799,6 +805,20 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
       ly /= 2;
       ldx = ldx * 2.0;
       ldy = ldy * 2.0;
+    }
+  }
+  if (cfg->solver == SOLVER_PCG_MG && cfg->mg_sweeps <= MG_FUSED_MAX_SWEEPS) {
+    int l0 = -1;
+    for (int l = 0; l < h->nlev; ++l)
+      if (h->lev[l].n <= MG_COARSE_MAX) {
+        l0 = l;
+        break;
+      }
+    if (l0 >= 0 && h->nlev - l0 <= MG_CYCLE_LEVELS) {
+      MgCycle C{};
+      C.nl = h->nlev - l0;
+      for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
+      if (mg_cycle_prepare(mg_cycle_layout(C)) == 0) h->mg_cycle_l0 = l0;
     }
   }
   int rc = HOC_OK;

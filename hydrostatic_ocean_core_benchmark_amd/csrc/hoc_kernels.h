@@ -178,11 +178,24 @@ __global__ void k_mg_prolong(MgLevel f, const double* xc, int cnx, double* x,
 __global__ void k_mg_res_parts(PcgArgs a, MgLevel L, const double* x);
 __global__ void k_mg_check(PcgArgs a, int it);
 __global__ void k_mg_start(PcgArgs a);
-constexpr int MG_COARSE_MAX = 4096;      // cells of a level solved in one workgroup
+constexpr int MG_COARSE_MAX = 1024;      // cells of a level solved in one workgroup
 constexpr int MG_COARSE_THREADS = 1024;
 constexpr int MG_FUSED_MAX_SWEEPS = 3;   // fused V-cycle legs for 1..3 sweeps per side
-__global__ void k_mg_coarsest(MgLevel L, double omega, int nsweeps, const double* b, double* out,
-                              const PcgScalars* sc);
+// The coarse end of a V-cycle solved in one workgroup (levels of at most
+// MG_COARSE_MAX cells, all vectors in LDS). Measured at 512 x 512: taking
+// the 64 x 64 level into the workgroup as well (4096 cells) costs 28 us per
+// cycle against 19 us for its two tiled launches plus the 32 x 32 cycle.
+constexpr int MG_CYCLE_LEVELS = 8;
+constexpr size_t MG_CYCLE_LDS = 160 * 1024;
+struct MgCycle {
+  MgLevel lev[MG_CYCLE_LEVELS];
+  int boff[MG_CYCLE_LEVELS], xoff[MG_CYCLE_LEVELS];  // LDS offsets (doubles) of b, x_S
+  int nl;
+};
+size_t mg_cycle_layout(MgCycle& C);  // fills boff/xoff, returns the LDS bytes
+int mg_cycle_prepare(size_t lds);    // < 0 if the cycle does not fit one workgroup's LDS
+void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
+                            double* out, const PcgScalars* sc, hipStream_t s);
 void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
                     double omega, const double* b, double* xs_out, double* bc,
                     const PcgScalars* sc);

@@ -326,29 +326,133 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevel L, int cnx, double
   }
 }
 
-// The coarsest level in one workgroup: `nsweeps` sweeps from zero in LDS.
-__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarsest(MgLevel L, double omega,
-                                                                   int nsweeps,
-                                                                   const double* __restrict__ b,
-                                                                   double* __restrict__ out,
-                                                                   const PcgScalars* sc) {
+// The coarse end of the pcg_mg V-cycle in one workgroup, every vector in
+// LDS: levels C.lev[0..nl) (the first with at most MG_COARSE_MAX cells),
+// S sweeps from zero + residual restriction down, 2S sweeps from zero on the
+// last level, prolongation + S sweeps up; the same arithmetic per cell as
+// the tiled legs (jacobi_vals, apply_vals). b0 is level 0's right-hand side,
+// out receives level 0's result.
+__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
+    MgCycle C, double omega, int S, const double* __restrict__ b0, double* __restrict__ out,
+    const PcgScalars* sc) {
   if (finished(sc)) return;
-  __shared__ double buf[2][MG_COARSE_MAX];
-  const int n = (int)L.n;
-  for (int k = 1; k <= nsweeps; ++k) {
-    double* dst = buf[k & 1];
-    const double* src = buf[(k - 1) & 1];
-    for (int c = threadIdx.x; c < n; c += MG_COARSE_THREADS) {
+  extern __shared__ double cyc[];
+  const int n0 = (int)C.lev[0].n;
+  double* wA = cyc;
+  double* wB = cyc + n0;
+  const int tid = threadIdx.x;
+  // one damped-Jacobi sweep on level L from src (zero if null) into dst
+  auto sweep = [&](const MgLevel& L, const double* b, const double* src, double* dst) {
+    const int n = (int)L.n;
+    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
       const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
-      dst[c] = (k == 1) ? jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, 0.0, 0.0, 0.0,
-                                      0.0, 0.0, b[c])
-                        : jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c],
-                                      src[nb.ce], src[nb.cw], src[nb.cn], src[nb.cs], b[c]);
+      dst[c] = src ? jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c], src[nb.ce],
+                                 src[nb.cw], src[nb.cn], src[nb.cs], b[c])
+                   : jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, 0.0, 0.0, 0.0, 0.0,
+                                 0.0, b[c]);
+    }
+    __syncthreads();
+  };
+  // level 0's right-hand side: read in place when it is the only level,
+  // else staged (the restriction of the levels below overwrites nothing of it)
+  const double* B0 = b0;
+  if (C.nl > 1) {
+    double* st = cyc + C.boff[0];
+    for (int c = tid; c < n0; c += MG_COARSE_THREADS) st[c] = b0[c];
+    __syncthreads();
+    B0 = st;
+  }
+  auto level_b = [&](int l) -> const double* { return l == 0 ? B0 : cyc + C.boff[l]; };
+  // down: x_S into XS[l], restricted residual into B[l + 1]
+  for (int l = 0; l + 1 < C.nl; ++l) {
+    const MgLevel& L = C.lev[l];
+    const double* b = level_b(l);
+    double* xs = cyc + C.xoff[l];
+    const double* src = nullptr;
+    for (int k = 1; k <= S; ++k) {
+      double* dst = (k == S) ? xs : ((k & 1) ? wA : wB);
+      sweep(L, b, src, dst);
+      src = dst;
+    }
+    const MgLevel& Cl = C.lev[l + 1];
+    double* bc = cyc + C.boff[l + 1];
+    const int nc = (int)Cl.n;
+    for (int t = tid; t < nc; t += MG_COARSE_THREADS) {
+      const int I = t % Cl.nx, J = t / Cl.nx;
+      double r[4];
+      for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
+        const int gi = 2 * I + (qd & 1), gj = 2 * J + (qd >> 1);
+        const Nbr nb = nbr_of(L, gi, gj);
+        const double ax = apply_vals(L, nb.c, nb.ce, nb.cw, nb.cn, nb.cs, xs[nb.c], xs[nb.ce],
+                                     xs[nb.cw], xs[nb.cn], xs[nb.cs]);
+        r[qd] = (b[nb.c] - ax) * mk(L.mask[nb.c]);
+      }
+      bc[t] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
     }
     __syncthreads();
   }
-  const double* x = buf[nsweeps & 1];
-  for (int c = threadIdx.x; c < n; c += MG_COARSE_THREADS) out[c] = x[c];
+  // the last level: 2S sweeps from zero
+  double* e = nullptr;
+  {
+    const int l = C.nl - 1;
+    const MgLevel& L = C.lev[l];
+    const double* b = level_b(l);
+    const double* src = nullptr;
+    for (int k = 1; k <= 2 * S; ++k) {
+      double* dst = (k & 1) ? wA : wB;
+      sweep(L, b, src, dst);
+      src = dst;
+      e = dst;
+    }
+  }
+  // up: x = XS[l] + prolong(e) mask, then S sweeps
+  for (int l = C.nl - 2; l >= 0; --l) {
+    const MgLevel& L = C.lev[l];
+    const double* b = level_b(l);
+    const double* xs = cyc + C.xoff[l];
+    const int cnx = C.lev[l + 1].nx;
+    double* xp = (e == wA) ? wB : wA;
+    double* other = (e == wA) ? wA : wB;
+    const int n = (int)L.n;
+    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
+      const int i = c % L.nx, j = c / L.nx;
+      xp[c] = xs[c] + e[(j / 2) * cnx + i / 2] * mk(L.mask[c]);
+    }
+    __syncthreads();
+    const double* src = xp;
+    for (int k = 1; k <= S; ++k) {
+      double* dst = (k & 1) ? other : xp;
+      sweep(L, b, src, dst);
+      src = dst;
+      e = dst;
+    }
+  }
+  for (int c = tid; c < n0; c += MG_COARSE_THREADS) out[c] = e[c];
+}
+
+size_t mg_cycle_layout(MgCycle& C) {
+  const int n0 = (int)C.lev[0].n;
+  int off = 2 * n0;
+  for (int l = 0; l < C.nl; ++l) {
+    C.boff[l] = off;
+    off += (int)C.lev[l].n;
+    C.xoff[l] = off;
+    off += (int)C.lev[l].n;
+  }
+  return (size_t)off * sizeof(double);
+}
+
+int mg_cycle_prepare(size_t lds) {
+  if (lds > MG_CYCLE_LDS) return -1;
+  return hipFuncSetAttribute((const void*)k_mg_coarse_cycle,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess
+             ? 0
+             : -2;
+}
+
+void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
+                            double* out, const PcgScalars* sc, hipStream_t s) {
+  k_mg_coarse_cycle<<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0, out, sc);
 }
 
 void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,

@@ -77,41 +77,60 @@ struct TileRegs {
   uint8_t mv[ST_V];
 };
 
-__device__ __forceinline__ void tile_load(TileRegs& R, int tid, int i0, int j0, int nx, int ny,
-                                          long long o, bool ts, const double* A,
-                                          const double* S, const double* u, const double* v,
-                                          const Geo& geo) {
+// Plane offsets of this thread's staging slots (independent of k; -1: no slot).
+struct TileOffs {
+  int c[ST_C], u[ST_U], v[ST_V];
+};
+
+__device__ __forceinline__ TileOffs tile_offsets(int tid, int i0, int j0, int nx, int ny) {
+  TileOffs F;
 #pragma unroll
   for (int q = 0; q < ST_C; ++q) {
     const int t = tid + q * TILE_THREADS;
-    if (t < SX * SY) {
-      const int lx = t % SX, ly = t / SX;
-      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
-      R.c0[q] = A[o + g];
-      R.c1[q] = ts ? S[o + g] : 0.0;
-      R.m[q] = geo.m3[o + g];
-    }
+    F.c[q] = t < SX * SY ? pmod(j0 + t / SX - HALO, ny) * nx + pmod(i0 + t % SX - HALO, nx) : -1;
   }
 #pragma unroll
   for (int q = 0; q < ST_U; ++q) {
     const int t = tid + q * TILE_THREADS;
-    if (t < TY * (TX + 1)) {
-      const int lx = t % (TX + 1), ly = t / (TX + 1);
-      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx - 1, nx);
-      R.u[q] = u[o + g];
-      R.du[q] = geo.dz3u[o + g];
-      R.mu[q] = geo.m3u[o + g];
-    }
+    F.u[q] = t < TY * (TX + 1) ? pmod(j0 + t / (TX + 1), ny) * nx + pmod(i0 + t % (TX + 1) - 1, nx)
+                               : -1;
   }
 #pragma unroll
   for (int q = 0; q < ST_V; ++q) {
     const int t = tid + q * TILE_THREADS;
-    if (t < (TY + 1) * TX) {
-      const int lx = t % TX, ly = t / TX;
-      const long long g = (long long)pmod(j0 + ly - 1, ny) * nx + pmod(i0 + lx, nx);
-      R.v[q] = v[o + g];
-      R.dv[q] = geo.dz3v[o + g];
-      R.mv[q] = geo.m3v[o + g];
+    F.v[q] = t < (TY + 1) * TX ? pmod(j0 + t / TX - 1, ny) * nx + pmod(i0 + t % TX, nx) : -1;
+  }
+  return F;
+}
+
+__device__ __forceinline__ void tile_load(TileRegs& R, const TileOffs& F, long long o, bool ts,
+                                          const double* A, const double* S, const double* u,
+                                          const double* v, const Geo& geo) {
+#pragma unroll
+  for (int q = 0; q < ST_C; ++q) {
+    if (F.c[q] >= 0) {
+      const long long g = o + F.c[q];
+      R.c0[q] = A[g];
+      R.c1[q] = ts ? S[g] : 0.0;
+      R.m[q] = geo.m3[g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_U; ++q) {
+    if (F.u[q] >= 0) {
+      const long long g = o + F.u[q];
+      R.u[q] = u[g];
+      R.du[q] = geo.dz3u[g];
+      R.mu[q] = geo.m3u[g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ST_V; ++q) {
+    if (F.v[q] >= 0) {
+      const long long g = o + F.v[q];
+      R.v[q] = v[g];
+      R.dv[q] = geo.dz3v[g];
+      R.mv[q] = geo.m3v[g];
     }
   }
 }
@@ -203,14 +222,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
   double dprev[2] = {0.0, 0.0};
 
   TileRegs R;
-  tile_load(R, tid, i0, j0, nx, ny, 0, ts, A, S, u, v, geo);
+  const TileOffs TO = tile_offsets(tid, i0, j0, nx, ny);
+  tile_load(R, TO, 0, ts, A, S, u, v, geo);
   tile_store(LB[0], R, tid);
   __syncthreads();
   const bool has_adv = adv != ADV_NONE;
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
     const TileLds& L = LB[k & 1];
-    if (k + 1 < nz) tile_load(R, tid, i0, j0, nx, ny, o + n2, ts, A, S, u, v, geo);
+    if (k + 1 < nz) tile_load(R, TO, o + n2, ts, A, S, u, v, geo);
     // ---- faces: the east and north face of every cell, plus the tile's
     // west column (8) and south row (32) of faces by threads 0..39
     {
@@ -382,32 +402,48 @@ struct MomRegs {
   double phi[MS_P], b[MS_P], zc[MS_P], w[MS_P];
 };
 
-__device__ __forceinline__ void mom_load(MomRegs& R, int tid, int i0, int j0, int nx, int ny,
-                                         long long o, long long n2, const double* u,
-                                         const double* v, const double* phi, const double* b,
-                                         const double* w, const Geo& geo) {
+struct MomOffs {
+  int h[MS_H], p[MS_P];
+};
+
+__device__ __forceinline__ MomOffs mom_offsets(int tid, int i0, int j0, int nx, int ny) {
+  MomOffs F;
 #pragma unroll
   for (int q = 0; q < MS_H; ++q) {
     const int t = tid + q * TILE_THREADS;
-    if (t < SX * SY) {
-      const int lx = t % SX, ly = t / SX;
-      const long long g = (long long)pmod(j0 + ly - HALO, ny) * nx + pmod(i0 + lx - HALO, nx);
-      R.u[q] = u[o + g];
-      R.v[q] = v[o + g];
-      R.mu[q] = geo.m3u[o + g];
-      R.mv[q] = geo.m3v[o + g];
-    }
+    F.h[q] = t < SX * SY ? pmod(j0 + t / SX - HALO, ny) * nx + pmod(i0 + t % SX - HALO, nx) : -1;
   }
 #pragma unroll
   for (int q = 0; q < MS_P; ++q) {
     const int t = tid + q * TILE_THREADS;
-    if (t < (TY + 1) * (TX + 1)) {
-      const int lx = t % (TX + 1), ly = t / (TX + 1);
-      const long long g = (long long)pmod(j0 + ly, ny) * nx + pmod(i0 + lx, nx);
-      R.phi[q] = phi[o + g];
-      R.b[q] = b[o + g];
-      R.zc[q] = geo.zc[o + g];
-      R.w[q] = w[o + n2 + g];
+    F.p[q] = t < (TY + 1) * (TX + 1)
+                 ? pmod(j0 + t / (TX + 1), ny) * nx + pmod(i0 + t % (TX + 1), nx)
+                 : -1;
+  }
+  return F;
+}
+
+__device__ __forceinline__ void mom_load(MomRegs& R, const MomOffs& F, long long o, long long n2,
+                                         const double* u, const double* v, const double* phi,
+                                         const double* b, const double* w, const Geo& geo) {
+#pragma unroll
+  for (int q = 0; q < MS_H; ++q) {
+    if (F.h[q] >= 0) {
+      const long long g = o + F.h[q];
+      R.u[q] = u[g];
+      R.v[q] = v[g];
+      R.mu[q] = geo.m3u[g];
+      R.mv[q] = geo.m3v[g];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MS_P; ++q) {
+    if (F.p[q] >= 0) {
+      const long long g = o + F.p[q];
+      R.phi[q] = phi[g];
+      R.b[q] = b[g];
+      R.zc[q] = geo.zc[g];
+      R.w[q] = w[g + n2];
     }
   }
 }
@@ -516,13 +552,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
   const int cx = tx + HALO, cy = ty + HALO;
 
   MomRegs R;
-  mom_load(R, tid, i0, j0, nx, ny, 0, n2, u, v, phi, b, w, geo);
+  const MomOffs MO = mom_offsets(tid, i0, j0, nx, ny);
+  mom_load(R, MO, 0, n2, u, v, phi, b, w, geo);
   mom_store(LB[0], R, tid);
   __syncthreads();
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
     const MomLds& L = LB[k & 1];
-    if (k + 1 < nz) mom_load(R, tid, i0, j0, nx, ny, o + n2, n2, u, v, phi, b, w, geo);
+    if (k + 1 < nz) mom_load(R, MO, o + n2, n2, u, v, phi, b, w, geo);
     {
       const double(*U)[SX] = L.u;
       const double(*V)[SX] = L.v;

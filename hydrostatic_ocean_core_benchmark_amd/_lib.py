@@ -25,6 +25,7 @@ class HocConfig(C.Structure):
         ("scheme", i32), ("theta", f64), ("theta_cor", f64), ("theta_v", f64),
         ("n_picard", i32), ("advection", i32), ("solver", i32), ("rtol", f64),
         ("max_iter", i32), ("mg_levels", i32), ("mg_sweeps", i32), ("mg_omega", f64),
+        ("n_split", i32), ("barotropic_mean", i32),
         ("g", f64), ("f", f64), ("rho0", f64), ("nu_b", f64), ("bottom_drag", f64),
         ("N2", f64), ("tracers_ts", i32),
         ("alpha_T", f64), ("beta_S", f64), ("T0", f64), ("S0", f64), ("cp", f64),

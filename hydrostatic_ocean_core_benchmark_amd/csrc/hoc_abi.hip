@@ -146,6 +146,10 @@ struct hoc_stepper {
   // step work arrays
   double *phi, *w, *pgx, *pgy, *avu, *avv, *dzu, *dzv, *stu, *stv;
   double *U0, *V0, *div0, *Ug, *Vg, *rhs;
+  // split_explicit: face depths, barotropic forcing, substep transports and levels
+  double *Hu = nullptr, *Hv = nullptr, *bfx = nullptr, *bfy = nullptr;
+  double *bU[3] = {}, *bV[3] = {}, *be[2] = {}, *acc_u = nullptr, *acc_v = nullptr;
+  long long barotropic_substeps = 0;
   double *gh_u[2], *gh_v[2];
   double *s_n2, *s_sh, *s_l, *s_le, *s_c;
   // Helmholtz
@@ -262,6 +266,9 @@ int derive_params(const hoc_config& c, StepParams& p) {
   p.neg_g = -c.g;
   p.neg_f = -c.f;
   p.neg_N2 = -c.N2;
+  p.n_split = c.n_split;
+  p.bt_mean = c.barotropic_mean != 0;
+  p.ddt = c.n_split > 0 ? c.dt / c.n_split : 0.0;
   return HOC_OK;
 }
 
@@ -270,8 +277,10 @@ int validate(const hoc_config& c) {
     return fail(HOC_ERR_INVALID, "grid must have nx >= 2, ny >= 2, nz >= 1");
   if (!(c.dx > 0.0) || !(c.dy > 0.0) || !(c.dt > 0.0))
     return fail(HOC_ERR_INVALID, "dx, dy and dt must be positive");
-  if (c.scheme != 0 && c.scheme != 1)
-    return fail(HOC_ERR_UNSUPPORTED, "scheme must be theta (0) or fb (1)");
+  if (c.scheme < 0 || c.scheme > 2)
+    return fail(HOC_ERR_INVALID, "scheme must be theta (0), fb (1) or split_explicit (2)");
+  if (c.scheme == 2 && c.n_split < 1)
+    return fail(HOC_ERR_INVALID, "split_explicit needs n_split >= 1");
   if (c.scheme == 0 && !(c.theta > 0.0 && c.theta <= 1.0))
     return fail(HOC_ERR_INVALID, "theta must lie in (0, 1]");
   if (c.n_picard < 1) return fail(HOC_ERR_INVALID, "n_picard must be >= 1");
@@ -682,6 +691,20 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   A(h->stv, d.n3);
   A(h->U0, d.n2);
   A(h->V0, d.n2);
+  if (cfg->scheme == 2) {
+    A(h->Hu, d.n2);
+    A(h->Hv, d.n2);
+    A(h->bfx, d.n2);
+    A(h->bfy, d.n2);
+    for (int q = 0; q < 3; ++q) {
+      A(h->bU[q], d.n2);
+      A(h->bV[q], d.n2);
+    }
+    A(h->be[0], d.n2);
+    A(h->be[1], d.n2);
+    A(h->acc_u, d.n2);
+    A(h->acc_v, d.n2);
+  }
   A(h->div0, d.n2);
   A(h->Ug, d.n2);
   A(h->Vg, d.n2);
@@ -862,6 +885,11 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   geo.tau_v = h->tau_v;
   geo.ustar2 = h->ustar2;
   geo.kappa_b = h->kappa_b;
+  if (cfg->scheme == 2) {
+    k_face_depth<<<blocks(d.n2), TPB>>>(d, geo, h->Hu, h->Hv);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+      return cleanup(fail(HOC_ERR_DEVICE, "face-depth kernel failed"));
+  }
   *out = h;
   return HOC_OK;
 }
@@ -904,6 +932,62 @@ int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, co
   return HOC_OK;
 }
 
+// The barotropic part of split_explicit (model3d_v05.py:312-335,441-462)
+// after the baroclinic Picard passes left gh in (gh_u, gh_v) and their depth
+// sums in (Ug, Vg): forcing, n_split substeps, depth-mean replacement.
+int split_barotropic(hoc_stepper* h, const double* eta, const double* gh_u, const double* gh_v,
+                     hoc_state* out, hipStream_t s) {
+  const Dims d = h->d;
+  const StepParams& p = h->p;
+  const Geo& geo = h->geo;
+  const unsigned nb = blocks(d.n2);
+  k_split_forcing<<<nb, TPB, 0, s>>>(d, p, geo, h->U0, h->V0, h->Ug, h->Vg, h->bfx, h->bfy);
+  if (p.bt_mean) {
+    HIP_TRY(hipMemsetAsync(h->acc_u, 0, d.n2 * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(h->acc_v, 0, d.n2 * sizeof(double), s));
+  }
+  // transports rotate over three buffers: the substep's start and two
+  // Picard iterates; the levels over two, the last substep writing out->eta
+  const double* U = h->U0;
+  const double* V = h->V0;
+  const double* e = eta;
+  int cur = -1;  // buffer index holding U, V (-1: U0, V0)
+  for (int n = 0; n < p.n_split; ++n) {
+    const double* Ui = U;
+    const double* Vi = V;
+    int slot = -1;
+    for (int m = 0; m < p.n_picard; ++m) {
+      int q = 0;
+      while (q == cur || q == slot) ++q;
+      k_bt_picard<<<nb, TPB, 0, s>>>(d, p, geo, U, V, e, Ui, Vi, h->bfx, h->bfy, h->Ku, h->Kv,
+                                     h->bU[q], h->bV[q]);
+      Ui = h->bU[q];
+      Vi = h->bV[q];
+      slot = q;
+    }
+    U = Ui;
+    V = Vi;
+    cur = slot;
+    double* e_new = (n == p.n_split - 1) ? out->eta : h->be[n & 1];
+    k_bt_eta<<<nb, TPB, 0, s>>>(d, p, geo, e, U, V, e_new, p.bt_mean ? h->acc_u : nullptr,
+                                h->acc_v);
+    e = e_new;
+  }
+  LAUNCH_CHECK("barotropic substeps");
+  h->barotropic_substeps += p.n_split;
+  if (p.bt_mean) {
+    int q = 0;
+    while (q == cur) ++q;
+    k_bt_mean<<<nb, TPB, 0, s>>>(d, p, h->acc_u, h->acc_v, h->bU[q], h->bV[q]);
+    U = h->bU[q];
+    V = h->bV[q];
+  }
+  k_split_final<<<blocks(d.n3), TPB, 0, s>>>(d, geo, gh_u, gh_v, h->Ug, h->Vg, U, V, h->Hu,
+                                             h->Hv, out->u, out->v);
+  LAUNCH_CHECK("split depth-mean replacement");
+  return HOC_OK;
+}
+
 int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream) {
   if (!h || !in || !out) return fail(HOC_ERR_INVALID, "null argument");
   if (!in->u || !in->v || !in->b || !in->eta || !out->u || !out->v || !out->b || !out->eta)
@@ -936,10 +1020,9 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       d, p, geo, in->u, in->v, in->b, h->phi, h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv,
       h->dzu, h->dzv, h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
-  if (theta) {
-    k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
-    k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, h->div0);
-  }
+  const bool split = h->cfg.scheme == 2;
+  if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
+  if (theta) k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, h->div0);
   const double* u_it = in->u;
   const double* v_it = in->v;
   for (int it = 0; it < h->cfg.n_picard; ++it) {
@@ -967,7 +1050,9 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       v_it = gv;
     }
   }
-  if (!theta) {
+  if (split) {
+    TRY(split_barotropic(h, in->eta, u_it, v_it, out, s));
+  } else if (!theta) {
     HIP_TRY(hipMemcpyAsync(out->u, u_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(out->v, v_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, out->u, out->v, h->U0, h->V0);
@@ -995,7 +1080,7 @@ int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out) {
   out->solver_failures = dc[1];
   out->solver_rebuilds = h->solver_rebuilds;
   out->tridiagonal_solves = h->tridiagonal_solves;
-  out->barotropic_substeps = 0;
+  out->barotropic_substeps = h->barotropic_substeps;
   out->eos_cells = h->eos_cells;
   return HOC_OK;
 }

@@ -76,6 +76,21 @@ __global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* eta,
                           double* rhs);
 __global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* eta, const double* U,
                          const double* V, double* eta_new);
+__global__ void k_split_forcing(Dims d, StepParams p, Geo geo, const double* U0, const double* V0,
+                                const double* Um, const double* Vm, double* fx, double* fy);
+__global__ void k_bt_picard(Dims d, StepParams p, Geo geo, const double* U, const double* V,
+                            const double* e, const double* Ui, const double* Vi,
+                            const double* fx, const double* fy, const double* Ku,
+                            const double* Kv, double* Uo, double* Vo);
+__global__ void k_bt_eta(Dims d, StepParams p, Geo geo, const double* e, const double* U,
+                         const double* V, double* e_new, double* acc_u, double* acc_v);
+__global__ void k_bt_mean(Dims d, StepParams p, const double* acc_u, const double* acc_v,
+                          double* U, double* V);
+__global__ void k_split_final(Dims d, Geo geo, const double* gh_u, const double* gh_v,
+                              const double* Um, const double* Vm, const double* U,
+                              const double* V, const double* Hu, const double* Hv, double* u,
+                              double* v);
+__global__ void k_face_depth(Dims d, Geo geo, double* Hu, double* Hv);
 __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* u, const double* v,
                             const double* eta, const double* u_it, const double* v_it,
                             const double* pgx, const double* pgy, const double* avu,

@@ -5,7 +5,7 @@
 
 namespace hoc {
 
-enum Scheme { SCHEME_THETA = 0, SCHEME_FB = 1 };
+enum Scheme { SCHEME_THETA = 0, SCHEME_FB = 1, SCHEME_SPLIT = 2 };
 enum Advection { ADV_NONE = 0, ADV_CENTRED2 = 1, ADV_UPWIND1 = 2, ADV_UP3 = 3, ADV_UP3_TVD = 4 };
 enum Eos { EOS_LINEAR = 0, EOS_SEOS = 1, EOS_TEOS10 = 2 };
 enum Solver { SOLVER_PCG_JACOBI = 0, SOLVER_PCG_RBGS = 1, SOLVER_RBGS = 2,
@@ -23,6 +23,7 @@ struct StepParams {
   int scheme, advection, tracers_ts, eos;
   int pgf_remove, pgf_correction, closure_tke, mxl_recursive;
   int n_picard, solver, max_iter, has_A_h, has_K_h;
+  int n_split, bt_mean;  // split_explicit: barotropic substeps, averaged transports
   double dx, dy, dz_nom, dt;
   double theta, theta_cor, theta_v;
   double g, f, rho0, nu_b, bottom_drag, N2;
@@ -42,6 +43,7 @@ struct StepParams {
   double helm_cx, helm_cy, dx2, dy2;
   double flux_T, flux_S;
   double neg_alpha_T, neg_g, neg_f, neg_N2;
+  double ddt;           // dt / n_split (split_explicit)
 };
 
 // Static geometry and forcing on the device (set once per stepper).

@@ -833,6 +833,102 @@ __global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* __restrict
   eta_new[c] = eta[c] - p.dt * div2(d, p, geo, U, V, c);
 }
 
+// ---------------------------------------------- split-explicit mode
+// Barotropic forcing of the split scheme (model3d_v05.py:445-452): the
+// Coriolis the barotropic step adds back, formed with the same 2D transport
+// averages, removed from the depth-integrated baroclinic tendency.
+__global__ void k_split_forcing(Dims d, StepParams p, Geo geo, const double* __restrict__ U0,
+                                const double* __restrict__ V0, const double* __restrict__ Um,
+                                const double* __restrict__ Vm, double* __restrict__ fx,
+                                double* __restrict__ fy) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const double cor_x = p.f * (p.one_m_tc * avg_v_to_u(V0, geo.mv2, geo.mu2, 0, q) +
+                              p.theta_cor * avg_v_to_u(Vm, geo.mv2, geo.mu2, 0, q));
+  const double cor_y = p.neg_f * (p.one_m_tc * avg_u_to_v(U0, geo.mu2, geo.mv2, 0, q) +
+                                  p.theta_cor * avg_u_to_v(Um, geo.mu2, geo.mv2, 0, q));
+  fx[c] = (Um[c] - U0[c]) / p.dt - cor_x;
+  fy[c] = (Vm[c] - V0[c]) / p.dt - cor_y;
+}
+
+// One Picard pass of a barotropic substep (model3d_v05.py:321-327): from the
+// substep's start U, V, e and the current iterate Ui, Vi.
+__global__ void k_bt_picard(Dims d, StepParams p, Geo geo, const double* __restrict__ U,
+                            const double* __restrict__ V, const double* __restrict__ e,
+                            const double* __restrict__ Ui, const double* __restrict__ Vi,
+                            const double* __restrict__ fx, const double* __restrict__ fy,
+                            const double* __restrict__ Ku, const double* __restrict__ Kv,
+                            double* __restrict__ Uo, double* __restrict__ Vo) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const double mu = mk(geo.mu2[c]), mv = mk(geo.mv2[c]);
+  const double gxe = mu * (e[q.e] - e[c]) / p.dx;
+  const double gye = mv * (e[q.n] - e[c]) / p.dy;
+  const double v_cor = p.one_m_tc * avg_v_to_u(V, geo.mv2, geo.mu2, 0, q) +
+                       p.theta_cor * avg_v_to_u(Vi, geo.mv2, geo.mu2, 0, q);
+  const double u_cor = p.one_m_tc * avg_u_to_v(U, geo.mu2, geo.mv2, 0, q) +
+                       p.theta_cor * avg_u_to_v(Ui, geo.mu2, geo.mv2, 0, q);
+  Uo[c] = (U[c] + p.ddt * (p.f * v_cor - p.g * Ku[c] * gxe + fx[c])) * mu;
+  Vo[c] = (V[c] + p.ddt * (p.neg_f * u_cor - p.g * Kv[c] * gye + fy[c])) * mv;
+}
+
+// e_new = e - ddt div(U, V); the transports summed for the averaged mode
+// (model3d_v05.py:329-331).
+__global__ void k_bt_eta(Dims d, StepParams p, Geo geo, const double* __restrict__ e,
+                         const double* __restrict__ U, const double* __restrict__ V,
+                         double* __restrict__ e_new, double* __restrict__ acc_u,
+                         double* __restrict__ acc_v) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  e_new[c] = e[c] - p.ddt * div2(d, p, geo, U, V, c);
+  if (acc_u) {
+    acc_u[c] = acc_u[c] + U[c];
+    acc_v[c] = acc_v[c] + V[c];
+  }
+}
+
+__global__ void k_bt_mean(Dims d, StepParams p, const double* __restrict__ acc_u,
+                          const double* __restrict__ acc_v, double* __restrict__ U,
+                          double* __restrict__ V) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  U[c] = acc_u[c] / (double)p.n_split;
+  V[c] = acc_v[c] / (double)p.n_split;
+}
+
+// Depth-mean replacement (model3d_v05.py:454-462): u = (gh_u - Um/Hu + U/Hu)
+// on the wet u faces, with Hu = sum_k dz3u.
+__global__ void k_split_final(Dims d, Geo geo, const double* __restrict__ gh_u,
+                              const double* __restrict__ gh_v, const double* __restrict__ Um,
+                              const double* __restrict__ Vm, const double* __restrict__ U,
+                              const double* __restrict__ V, const double* __restrict__ Hu,
+                              const double* __restrict__ Hv, double* __restrict__ u,
+                              double* __restrict__ v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const double ihu = safe_inv(Hu[c]), ihv = safe_inv(Hv[c]);
+  u[t] = (gh_u[t] - Um[c] * ihu + U[c] * ihu) * mk(geo.m3u[t]);
+  v[t] = (gh_v[t] - Vm[c] * ihv + V[c] * ihv) * mk(geo.m3v[t]);
+}
+
+// Hu = sum_k dz3u, Hv = sum_k dz3v, summed from the surface.
+__global__ void k_face_depth(Dims d, Geo geo, double* __restrict__ Hu, double* __restrict__ Hv) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    su = (k == 0) ? geo.dz3u[c] : su + geo.dz3u[k * d.n2 + c];
+    sv = (k == 0) ? geo.dz3v[c] : sv + geo.dz3v[k * d.n2 + c];
+  }
+  Hu[c] = su;
+  Hv[c] = sv;
+}
+
 // ------------------------------------------- predictor and column solves
 // One lane per column, one Picard pass (model3d_v05.py:387-396):
 //   v_cor = (1-tc) avv + tc avg_v_to_u(v_it),  u_cor likewise
@@ -857,6 +953,7 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restr
   const Nb q = neighbours(d, j, i);
   const long long n2 = d.n2;
   const int nz = d.nz;
+  const bool split = p.scheme == SCHEME_SPLIT;
   const double gx2 = mk(geo.mu2[c]) * (eta[q.e] - eta[c]) / p.dx;
   const double gy2 = mk(geo.mv2[c]) * (eta[q.n] - eta[c]) / p.dy;
   double du_prev = 0.0, dv_prev = 0.0;
@@ -867,8 +964,13 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restr
     const double ucor = p.one_m_tc * avu[t] + p.theta_cor * avg_u_to_v(u_it, geo.m3u, geo.m3v, o, q);
     const double gxe = gx2 * mk(geo.m3u[t]);
     const double gye = gy2 * mk(geo.m3v[t]);
-    const double gu = u[t] + p.dt * (p.f * vcor + pgx[t]) - p.c_gx * gxe + p.c_dzd * dzu[t];
-    const double gv = v[t] + p.dt * (p.neg_f * ucor + pgy[t]) - p.c_gx * gye + p.c_dzd * dzv[t];
+    // split_explicit leaves the surface gradient to the barotropic substeps
+    // (model3d_v05.py:431-432)
+    const double gu = split ? u[t] + p.dt * (p.f * vcor + pgx[t]) + p.c_dzd * dzu[t]
+                            : u[t] + p.dt * (p.f * vcor + pgx[t]) - p.c_gx * gxe + p.c_dzd * dzu[t];
+    const double gv = split ? v[t] + p.dt * (p.neg_f * ucor + pgy[t]) + p.c_dzd * dzv[t]
+                            : v[t] + p.dt * (p.neg_f * ucor + pgy[t]) - p.c_gx * gye +
+                                  p.c_dzd * dzv[t];
     const double ru = gu + stu[t];
     const double rv = gv + stv[t];
     double du, dv;

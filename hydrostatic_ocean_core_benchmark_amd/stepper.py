@@ -54,8 +54,6 @@ class HipStepper:
             raise NotImplementedError("HipStepper runs the z-level coordinate")
         if "open" in (domain.bc_x, domain.bc_y):
             raise NotImplementedError("Flather open boundaries are not implemented")
-        if params.name == "split_explicit":
-            raise NotImplementedError("split_explicit is not implemented")
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:
             raise ValueError(
                 "solver 'fft' needs a flat bottom and periodic boundaries; "
@@ -76,6 +74,19 @@ class HipStepper:
         self.physics, self.params = physics, params
         self.dt = float(dt)
         self.n_split = 0
+        if params.name == "split_explicit":
+            # libs/core/model3d_v05.py:111-123: the deepest column sets the
+            # fastest surface gravity wave
+            h_max = float(np.max(domain.H[domain.mask > 0]))
+            c = np.sqrt(physics.g * h_max)
+            gr = domain.grid
+            dt_baro = 1.0 / (c * np.sqrt(1.0 / gr.dx ** 2 + 1.0 / gr.dy ** 2))
+            self.n_split = (params.n_split if params.n_split > 0
+                            else max(1, int(np.ceil(1.2 * self.dt / dt_baro))))
+            if self.dt / self.n_split > dt_baro:
+                raise ValueError(
+                    f"n_split={self.n_split} leaves a barotropic substep of "
+                    f"{self.dt / self.n_split:.1f}s above the limit {dt_baro:.1f}s")
         self._eos_host = 0
         g = self.grid
         nz, ny, nx = g.nz, g.ny, g.nx
@@ -85,7 +96,9 @@ class HipStepper:
         cfg = _lib.HocConfig()
         cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
         cfg.dx, cfg.dy, cfg.dz_nominal, cfg.dt = g.dx, g.dy, g.dz, self.dt
-        cfg.scheme = 0 if params.name == "theta" else 1
+        cfg.scheme = {"theta": 0, "fb": 1, "split_explicit": 2}[params.name]
+        cfg.n_split = self.n_split
+        cfg.barotropic_mean = 1 if getattr(params, "barotropic_average", "none") == "mean" else 0
         cfg.theta, cfg.theta_cor, cfg.theta_v = params.theta, params.theta_cor, params.theta_v
         cfg.n_picard = params.n_picard
         cfg.advection = ADVECTION[params.advection]
@@ -280,7 +293,7 @@ class HipStepper:
 
     @property
     def barotropic_substeps(self) -> int:
-        return 0
+        return int(self._counters().barotropic_substeps)
 
     @property
     def eos_cells(self) -> int:

@@ -38,7 +38,7 @@ typedef struct hoc_config {
   double dx, dy;
   double dz_nominal;       /* grid.dz = physics.H / nz, used by momentum vertical advection */
   double dt;
-  int32_t scheme;          /* 0 theta, 1 fb */
+  int32_t scheme;          /* 0 theta, 1 fb, 2 split_explicit */
   double theta, theta_cor, theta_v;
   int32_t n_picard;
   int32_t advection;       /* 0 none, 1 centered2, 2 upwind1, 3 up3, 4 up3_tvd */
@@ -47,6 +47,8 @@ typedef struct hoc_config {
   int32_t max_iter;
   int32_t mg_levels, mg_sweeps;  /* pcg_mg: levels and damped-Jacobi sweeps per side */
   double mg_omega;               /* pcg_mg: Jacobi damping */
+  int32_t n_split;               /* split_explicit: barotropic substeps per step (>= 1) */
+  int32_t barotropic_mean;       /* split_explicit: 1 = mean of the substep transports */
   double g, f, rho0, nu_b, bottom_drag, N2;
   int32_t tracers_ts;      /* 1: T, S prognostic and b diagnosed; 0: b prognostic */
   double alpha_T, beta_S, T0, S0, cp, A_h, K_h, q_heat, q_salt;

@@ -1,10 +1,10 @@
 """The oracle's time step (libs/core/model3d_v05.py:78-545, z-level only).
 
 `OracleStepper(domain, physics, params, dt).step(state)` returns a new
-state exactly as `Stepper3DV05.step` does, for the theta and fb schemes,
-closure none|tke, tracers buoyancy|TS, every advection scheme and every EOS.
-Not restated (no z-level hot-path caller): split_explicit, Flather open
-boundaries, moving vertical coordinates.
+state exactly as `Stepper3DV05.step` does, for the theta, fb and
+split_explicit schemes, closure none|tke, tracers buoyancy|TS, every
+advection scheme and every EOS. Not restated: Flather open boundaries and
+moving vertical coordinates.
 
 Inputs are duck-typed: `domain` needs grid (nx, ny, nz, dx, dy, dz), mask,
 mask_u, mask_v, mask3, mask3u, mask3v, dz3_ref, face_thickness(),
@@ -47,8 +47,6 @@ class OracleStepper:
             raise ValueError("the oracle restates the z-level coordinate only")
         if "open" in (domain.bc_x, domain.bc_y):
             raise ValueError("the oracle does not restate Flather open boundaries")
-        if params.name == "split_explicit":
-            raise ValueError("the oracle does not restate split_explicit")
         self.domain, self.grid = domain, domain.grid
         self.physics, self.params = physics, params
         self.dt = float(dt)
@@ -70,6 +68,18 @@ class OracleStepper:
         self.z_centre = d.depth_at_centres()
         self.solver = None
         self._build_coefficients()
+        if params.name == "split_explicit":
+            # libs/core/model3d_v05.py:111-123: the deepest column sets the
+            # barotropic CFL
+            h_max = float(np.max(d.H[d.mask > 0]))
+            c = np.sqrt(physics.g * h_max)
+            dt_baro = 1.0 / (c * np.sqrt(1.0 / self.grid.dx ** 2 + 1.0 / self.grid.dy ** 2))
+            self.n_split = (params.n_split if params.n_split > 0
+                            else max(1, int(np.ceil(1.2 * self.dt / dt_baro))))
+            if self.dt / self.n_split > dt_baro:
+                raise ValueError(
+                    f"n_split={self.n_split} leaves a barotropic substep of "
+                    f"{self.dt / self.n_split:.1f}s above the limit {dt_baro:.1f}s")
 
     # ---------------------------------------------------------- coefficients
     def _build_coefficients(self):
@@ -202,6 +212,9 @@ class OracleStepper:
         stress_v[0] = thv * dt * tau_v * col.safe_inv(self.dz3v[0])
         stress_u = stress_u + dt * ex_u
         stress_v = stress_v + dt * ex_v
+        if pr.name == "split_explicit":
+            return self._step_split(s, pgx, pgy, av_prev_u, av_prev_v, dzu_diff, dzv_diff,
+                                    stress_u, stress_v)
 
         gx_eta = hz.gradx(s.eta, g.dx, d.mask_u)[None] * d.mask3u
         gy_eta = hz.grady(s.eta, g.dy, d.mask_v)[None] * d.mask3v
@@ -236,6 +249,70 @@ class OracleStepper:
                                                 np.sum(self.dz3v * v_it, axis=0))
         w = self.w_diag(u_it, v_it)
         return self._advance_tracers(s, u_it, v_it, w, eta_new)
+
+    def _barotropic(self, U0, V0, eta, fx, fy):
+        """n_split forward-backward substeps of the depth-integrated flow
+        with Picard-iterated Coriolis (libs/core/model3d_v05.py:312-335)."""
+        d, p, g = self.domain, self.physics, self.grid
+        ddt = self.dt / self.n_split
+        U, V, e = U0, V0, eta
+        acc_u, acc_v = np.zeros_like(U0), np.zeros_like(V0)
+        tc, npic = self.params.theta_cor, self.params.n_picard
+        for _ in range(self.n_split):
+            av_v = hz.avg_v_to_u(V, d.mask_v, d.mask_u)
+            av_u = hz.avg_u_to_v(U, d.mask_u, d.mask_v)
+            gxe = hz.gradx(e, g.dx, d.mask_u)
+            gye = hz.grady(e, g.dy, d.mask_v)
+            Ui, Vi = U, V
+            for _ in range(npic):
+                v_cor = (1.0 - tc) * av_v + tc * hz.avg_v_to_u(Vi, d.mask_v, d.mask_u)
+                u_cor = (1.0 - tc) * av_u + tc * hz.avg_u_to_v(Ui, d.mask_u, d.mask_v)
+                Ui = (U + ddt * (p.f * v_cor - p.g * self.Ku * gxe + fx)) * d.mask_u
+                Vi = (V + ddt * (-p.f * u_cor - p.g * self.Kv * gye + fy)) * d.mask_v
+            U, V = Ui, Vi
+            e = e - ddt * self._div_bt(U, V)
+            acc_u += U
+            acc_v += V
+        self.barotropic_substeps += self.n_split
+        if self.params.barotropic_average == "mean":
+            return acc_u / self.n_split, acc_v / self.n_split, e
+        return U, V, e
+
+    def _step_split(self, s, pgx, pgy, av_prev_u, av_prev_v, dzu_diff, dzv_diff, stress_u,
+                    stress_v):
+        """Baroclinic predictor, barotropic substeps, depth-mean replacement
+        (libs/core/model3d_v05.py:417-463)."""
+        d, p = self.domain, self.physics
+        dt, tc, thv = self.dt, self.params.theta_cor, self.params.theta_v
+        mask3 = d.mask3
+        U0 = np.sum(self.dz3u * s.u, axis=0)
+        V0 = np.sum(self.dz3v * s.v, axis=0)
+        u_it, v_it = s.u, s.v
+        gh_u = gh_v = None
+        for _ in range(self.params.n_picard):
+            v_cor = (1.0 - tc) * av_prev_v + tc * hz.avg_v_to_u(v_it, d.mask3v, d.mask3u)
+            u_cor = (1.0 - tc) * av_prev_u + tc * hz.avg_u_to_v(u_it, d.mask3u, d.mask3v)
+            gu = s.u + dt * (p.f * v_cor + pgx) + (1.0 - thv) * dt * dzu_diff
+            gv = s.v + dt * (-p.f * u_cor + pgy) + (1.0 - thv) * dt * dzv_diff
+            gh_u = self._solve_column(self.tri_m, gu + stress_u) * mask3
+            gh_v = self._solve_column(self.tri_m, gv + stress_v) * mask3
+            u_it, v_it = gh_u, gh_v
+        Um, Vm = np.sum(self.dz3u * gh_u, axis=0), np.sum(self.dz3v * gh_v, axis=0)
+        cor_x = p.f * ((1.0 - tc) * hz.avg_v_to_u(V0, d.mask_v, d.mask_u)
+                       + tc * hz.avg_v_to_u(Vm, d.mask_v, d.mask_u))
+        cor_y = -p.f * ((1.0 - tc) * hz.avg_u_to_v(U0, d.mask_u, d.mask_v)
+                        + tc * hz.avg_u_to_v(Um, d.mask_u, d.mask_v))
+        fx = (Um - U0) / dt - cor_x
+        fy = (Vm - V0) / dt - cor_y
+        U, V, eta_new = self._barotropic(U0, V0, s.eta, fx, fy)
+        Hu = np.sum(self.dz3u, axis=0)
+        Hv = np.sum(self.dz3v, axis=0)
+        inv_hu = col.safe_inv(Hu)
+        inv_hv = col.safe_inv(Hv)
+        u_new = (gh_u - np.sum(self.dz3u * gh_u, axis=0) * inv_hu + U * inv_hu) * d.mask3u
+        v_new = (gh_v - np.sum(self.dz3v * gh_v, axis=0) * inv_hv + V * inv_hv) * d.mask3v
+        w = self.w_diag(u_new, v_new)
+        return self._advance_tracers(s, u_new, v_new, w, eta_new)
 
     # ---------------------------------------------------------------- tracers
     def tracer_advection(self, c, u, v, w):

@@ -98,6 +98,7 @@ def composed_fixtures() -> dict:
             rec[f"s{n}_nu"] = stepper.nu
             rec[f"s{n}_kappa"] = stepper.kappa
             rec[f"s{n}_iters"] = np.array(stepper.solver_iterations)
+            rec[f"s{n}_substeps"] = np.array(stepper.barotropic_substeps)
         rec["counters"] = np.array([stepper.solver_iterations, stepper.solver_failures,
                                     stepper.solver_rebuilds, stepper.tridiagonal_solves,
                                     stepper.eos_cells])
@@ -207,13 +208,18 @@ def operator_fixtures() -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", type=Path, default=Path(__file__).resolve().parent)
+    ap.add_argument("--composed-only", action="store_true",
+                    help="rewrite only the composed-step fixtures")
     args = ap.parse_args()
     args.out.mkdir(parents=True, exist_ok=True)
-    np.savez_compressed(args.out / "operators.npz", **operator_fixtures())
-    print("operators.npz written")
+    if not args.composed_only:
+        np.savez_compressed(args.out / "operators.npz", **operator_fixtures())
+        print("operators.npz written")
     for name, rec in composed_fixtures().items():
         np.savez_compressed(args.out / f"step_{name}.npz", **rec)
         print(f"step_{name}.npz written (iterations {int(rec['counters'][0])})")
+    if args.composed_only:
+        return 0
     rec = lock_exchange_fixture()
     np.savez_compressed(args.out / "lock_exchange_128x4x20.npz", **rec)
     print("lock_exchange_128x4x20.npz written, iterations", rec["iters_per_step"].sum())

@@ -29,6 +29,22 @@ COMPOSED = {
         dict(eos="linear", pgf="keep", closure="tke", mxl="integral"),
         dict(name="theta", advection="upwind1", solver="pcg_rbgs", rtol=1e-12, theta=0.6),
         600.0, 2),
+    "split_tke_up3_seamount": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
+             bc_x="closed", bc_y="periodic"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-5, kappa=1e-6, rho0=1025.0,
+             tau_x=0.1, tau_y=0.0, tracers="TS", A_h=10.0, K_h=10.0),
+        dict(eos="teos10", pgf="keep", closure="tke", mxl="integral"),
+        dict(name="split_explicit", advection="up3", solver="pcg_jacobi"), 600.0, 3),
+    "split_mean_centred_seos_forcing": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
+             bc_x="closed", bc_y="closed"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-3, kappa=1e-3, rho0=1025.0,
+             tau_x=0.1, tau_y=-0.05, tracers="TS", A_h=0.0, K_h=0.0, q_heat=100.0,
+             bottom_drag=1e-3),
+        dict(eos="seos", pgf="remove", closure="none"),
+        dict(name="split_explicit", advection="centered2", solver="pcg_jacobi", n_split=12,
+             barotropic_average="mean", theta_cor=0.6, n_picard=3), 600.0, 2),
     "fb_centred_buoyancy": (
         dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
              bc_x="periodic", bc_y="periodic"),

@@ -256,6 +256,8 @@ def test_composed_step_matches_reference(hoc, name):
             assert rel_l2(got, w) < 1e-12, (n, k, rel_l2(got, w))
             assert np.array_equal(got, w), (n, k, "not bitwise", rel_l2(got, w))
         assert st.solver_iterations == int(f[f"s{n}_iters"]), n
+        if f"s{n}_substeps" in f:
+            assert st.barotropic_substeps == int(f[f"s{n}_substeps"]), n
 
 
 @pytest.mark.parametrize("name", list(COMPOSED))

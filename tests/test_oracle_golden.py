@@ -62,6 +62,8 @@ def test_oracle_composed_step_bitwise(name):
                 assert np.array_equal(getattr(s, k), f[key]), (n, k)
         assert np.array_equal(st.nu, f[f"s{n}_nu"]) and np.array_equal(st.kappa, f[f"s{n}_kappa"])
         assert st.solver_iterations == int(f[f"s{n}_iters"])
+        if f"s{n}_substeps" in f:
+            assert st.barotropic_substeps == int(f[f"s{n}_substeps"])
     c = f["counters"]
     assert [st.solver_iterations, st.solver_failures, st.solver_rebuilds,
             st.tridiagonal_solves, st.eos_cells] == [int(x) for x in c]

@@ -25,7 +25,8 @@ class HocConfig(C.Structure):
         ("scheme", i32), ("theta", f64), ("theta_cor", f64), ("theta_v", f64),
         ("n_picard", i32), ("advection", i32), ("solver", i32), ("rtol", f64),
         ("max_iter", i32), ("mg_levels", i32), ("mg_sweeps", i32), ("mg_omega", f64),
-        ("n_split", i32), ("barotropic_mean", i32),
+        ("n_split", i32), ("barotropic_mean", i32), ("open_x", i32), ("open_y", i32),
+        ("eta_ext", f64), ("u_ext", f64),
         ("g", f64), ("f", f64), ("rho0", f64), ("nu_b", f64), ("bottom_drag", f64),
         ("N2", f64), ("tracers_ts", i32),
         ("alpha_T", f64), ("beta_S", f64), ("T0", f64), ("S0", f64), ("cp", f64),
@@ -43,6 +44,7 @@ class HocGeometry(C.Structure):
         ("mask3", P_u8), ("mask3u", P_u8), ("mask3v", P_u8),
         ("dz3", P_f64), ("dz3u", P_f64), ("dz3v", P_f64), ("z_centre", P_f64),
         ("tau_x", P_f64), ("tau_y", P_f64), ("ustar2", P_f64), ("kappa_b", P_f64),
+        ("H", P_f64),
     ]
 
 

@@ -138,7 +138,7 @@ struct hoc_stepper {
           *m3v = nullptr;
   double *dz3 = nullptr, *dz3u = nullptr, *dz3v = nullptr, *zc = nullptr, *dzi = nullptr;
   double *tau_x = nullptr, *tau_y = nullptr, *tau_u = nullptr, *tau_v = nullptr;
-  double *ustar2 = nullptr, *kappa_b = nullptr;
+  double *ustar2 = nullptr, *kappa_b = nullptr, *Hd = nullptr;
   // mixing (caller-owned)
   double *Km = nullptr, *Kh = nullptr;
   // coefficients
@@ -269,6 +269,10 @@ int derive_params(const hoc_config& c, StepParams& p) {
   p.n_split = c.n_split;
   p.bt_mean = c.barotropic_mean != 0;
   p.ddt = c.n_split > 0 ? c.dt / c.n_split : 0.0;
+  p.open_x = c.open_x != 0;
+  p.open_y = c.open_y != 0;
+  p.eta_ext = c.eta_ext;
+  p.u_ext = c.u_ext;
   return HOC_OK;
 }
 
@@ -859,6 +863,11 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   rc = rc ? rc : upload(h->tau_y, g->tau_y, d.n2, "tau_y");
   rc = rc ? rc : upload(h->ustar2, g->ustar2, d.n2, "ustar2");
   rc = rc ? rc : upload(h->kappa_b, g->kappa_b, d.n2, "kappa_b");
+  if (rc == HOC_OK && (cfg->open_x || cfg->open_y)) {
+    if (!g->H) return cleanup(fail(HOC_ERR_INVALID, "open boundaries need geometry H"));
+    rc = h->alloc(&h->Hd, d.n2);
+    rc = rc ? rc : upload(h->Hd, g->H, d.n2, "H");
+  }
   if (rc != HOC_OK) return cleanup(rc);
   if (hipMemset(h->bar, 0, 2 * sizeof(unsigned)) != hipSuccess ||
       hipMemset(h->bar_err, 0, sizeof(int)) != hipSuccess)
@@ -885,6 +894,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   geo.tau_v = h->tau_v;
   geo.ustar2 = h->ustar2;
   geo.kappa_b = h->kappa_b;
+  geo.H = h->Hd;
   if (cfg->scheme == 2) {
     k_face_depth<<<blocks(d.n2), TPB>>>(d, geo, h->Hu, h->Hv);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
@@ -1022,7 +1032,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   LAUNCH_CHECK("explicit momentum");
   const bool split = h->cfg.scheme == 2;
   if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
-  if (theta) k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, h->div0);
+  if (theta) k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, in->eta, h->div0);
   const double* u_it = in->u;
   const double* v_it = in->v;
   for (int it = 0; it < h->cfg.n_picard; ++it) {

@@ -70,7 +70,7 @@ __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double*
 __global__ void k_depth_sum(Dims d, Geo geo, const double* u, const double* v, double* U,
                             double* V);
 __global__ void k_div2(Dims d, StepParams p, Geo geo, const double* U, const double* V,
-                       double* out);
+                       const double* eta, double* out);
 __global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* eta,
                           const double* div_old, const double* Ug, const double* Vg,
                           double* rhs);

@@ -24,6 +24,7 @@ struct StepParams {
   int pgf_remove, pgf_correction, closure_tke, mxl_recursive;
   int n_picard, solver, max_iter, has_A_h, has_K_h;
   int n_split, bt_mean;  // split_explicit: barotropic substeps, averaged transports
+  int open_x, open_y;    // Flather open edges
   double dx, dy, dz_nom, dt;
   double theta, theta_cor, theta_v;
   double g, f, rho0, nu_b, bottom_drag, N2;
@@ -44,6 +45,7 @@ struct StepParams {
   double flux_T, flux_S;
   double neg_alpha_T, neg_g, neg_f, neg_N2;
   double ddt;           // dt / n_split (split_explicit)
+  double eta_ext, u_ext; // Flather external state
 };
 
 // Static geometry and forcing on the device (set once per stepper).
@@ -54,6 +56,7 @@ struct Geo {
   const double *dzi;                    // centre spacing on interfaces [nz+1]
   const double *tau_u, *tau_v;          // (tau/rho0) * mask3[u|v][0]   [ny,nx]
   const double *ustar2, *kappa_b;       // surface u*^2, background kappa [ny,nx]
+  const double *H;                      // depth [ny,nx] (open edges only, else null)
 };
 
 }  // namespace hoc

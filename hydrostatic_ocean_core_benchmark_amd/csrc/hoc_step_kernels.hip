@@ -807,11 +807,50 @@ __device__ __forceinline__ double div2(const Dims& d, const StepParams& p, const
   return mk(geo.m2[c]) * ((U[c] - U[cw]) / p.dx + (V[c] - V[cs]) / p.dy);
 }
 
+// Barotropic divergence with the Flather transports on open edges
+// (model3d_v05.py:262-309): on an open edge column the wrapped neighbour
+// flux of the masked divergence is swapped for the radiating transport
+// u_ext -/+ sqrt(g max(H, 1)) (eta - eta_ext), x edges first, then the
+// result is masked again.
+__device__ __forceinline__ double div_bt(const Dims& d, const StepParams& p, const Geo& geo,
+                                         const double* U, const double* V, const double* eta,
+                                         long long c) {
+  double dv = div2(d, p, geo, U, V, c);
+  if (!p.open_x && !p.open_y) return dv;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  if (p.open_x) {
+    const long long cw = at2(d, j, wrap(i, -1, d.nx));
+    const double cg = sqrt(p.g * nmax(geo.H[c], 1.0));
+    if (i == 0) {
+      const double bw = p.u_ext - cg * (eta[c] - p.eta_ext);
+      dv = dv + ((U[c] - bw) / p.dx - ((U[c] - U[cw]) / p.dx));
+    }
+    if (i == d.nx - 1) {
+      const double be = p.u_ext + cg * (eta[c] - p.eta_ext);
+      dv = dv + ((be - U[cw]) / p.dx - ((U[c] - U[cw]) / p.dx));
+    }
+  }
+  if (p.open_y) {
+    const long long cs = at2(d, wrap(j, -1, d.ny), i);
+    const double cg = sqrt(p.g * nmax(geo.H[c], 1.0));
+    if (j == 0) {
+      const double bs = p.u_ext - cg * (eta[c] - p.eta_ext);
+      dv = dv + ((V[c] - bs) / p.dy - ((V[c] - V[cs]) / p.dy));
+    }
+    if (j == d.ny - 1) {
+      const double bn = p.u_ext + cg * (eta[c] - p.eta_ext);
+      dv = dv + ((bn - V[cs]) / p.dy - ((V[c] - V[cs]) / p.dy));
+    }
+  }
+  return dv * mk(geo.m2[c]);
+}
+
 __global__ void k_div2(Dims d, StepParams p, Geo geo, const double* __restrict__ U,
-                       const double* __restrict__ V, double* __restrict__ out) {
+                       const double* __restrict__ V, const double* __restrict__ eta,
+                       double* __restrict__ out) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  out[c] = div2(d, p, geo, U, V, c);
+  out[c] = div_bt(d, p, geo, U, V, eta, c);
 }
 
 // rhs = eta - dt ((1 - th) div_old + th div(U_g, V_g))  (model3d_v05.py:398-400)
@@ -820,7 +859,7 @@ __global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* __restric
                           const double* __restrict__ Vg, double* __restrict__ rhs) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  const double dg = div2(d, p, geo, Ug, Vg, c);
+  const double dg = div_bt(d, p, geo, Ug, Vg, eta, c);
   rhs[c] = eta[c] - p.dt * (p.one_m_th * div_old[c] + p.th * dg);
 }
 
@@ -830,7 +869,7 @@ __global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* __restrict
                          double* __restrict__ eta_new) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  eta_new[c] = eta[c] - p.dt * div2(d, p, geo, U, V, c);
+  eta_new[c] = eta[c] - p.dt * div_bt(d, p, geo, U, V, eta, c);
 }
 
 // ---------------------------------------------- split-explicit mode
@@ -884,7 +923,7 @@ __global__ void k_bt_eta(Dims d, StepParams p, Geo geo, const double* __restrict
                          double* __restrict__ acc_v) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  e_new[c] = e[c] - p.ddt * div2(d, p, geo, U, V, c);
+  e_new[c] = e[c] - p.ddt * div_bt(d, p, geo, U, V, e, c);
   if (acc_u) {
     acc_u[c] = acc_u[c] + U[c];
     acc_v[c] = acc_v[c] + V[c];

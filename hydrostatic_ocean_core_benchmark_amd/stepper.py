@@ -52,8 +52,6 @@ class HipStepper:
         params.validate()
         if domain.vcoord != "zlevel":
             raise NotImplementedError("HipStepper runs the z-level coordinate")
-        if "open" in (domain.bc_x, domain.bc_y):
-            raise NotImplementedError("Flather open boundaries are not implemented")
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:
             raise ValueError(
                 "solver 'fft' needs a flat bottom and periodic boundaries; "
@@ -99,6 +97,10 @@ class HipStepper:
         cfg.scheme = {"theta": 0, "fb": 1, "split_explicit": 2}[params.name]
         cfg.n_split = self.n_split
         cfg.barotropic_mean = 1 if getattr(params, "barotropic_average", "none") == "mean" else 0
+        cfg.open_x = 1 if domain.bc_x == "open" else 0
+        cfg.open_y = 1 if domain.bc_y == "open" else 0
+        cfg.eta_ext = float(getattr(physics, "eta_ext", 0.0))
+        cfg.u_ext = float(getattr(physics, "u_ext", 0.0))
         cfg.theta, cfg.theta_cor, cfg.theta_v = params.theta, params.theta_cor, params.theta_v
         cfg.n_picard = params.n_picard
         cfg.advection = ADVECTION[params.advection]
@@ -131,7 +133,7 @@ class HipStepper:
             tau_x=_field2d(p.tau_x, shape2), tau_y=_field2d(p.tau_y, shape2),
             ustar2=_field2d(np.hypot(p.tau_x, p.tau_y) / p.rho0 if ustar2 is None else ustar2,
                             shape2),
-            kappa_b=_field2d(p.kappa, shape2))
+            kappa_b=_field2d(p.kappa, shape2), H=_f64(d.H))
         geo = _lib.HocGeometry()
         for name, arr in self._host.items():
             ctype = _lib.P_u8 if arr.dtype == np.uint8 else _lib.P_f64

@@ -49,6 +49,8 @@ typedef struct hoc_config {
   double mg_omega;               /* pcg_mg: Jacobi damping */
   int32_t n_split;               /* split_explicit: barotropic substeps per step (>= 1) */
   int32_t barotropic_mean;       /* split_explicit: 1 = mean of the substep transports */
+  int32_t open_x, open_y;        /* Flather open boundaries on the x / y domain edges */
+  double eta_ext, u_ext;         /* Flather external level and transport */
   double g, f, rho0, nu_b, bottom_drag, N2;
   int32_t tracers_ts;      /* 1: T, S prognostic and b diagnosed; 0: b prognostic */
   double alpha_T, beta_S, T0, S0, cp, A_h, K_h, q_heat, q_salt;
@@ -69,6 +71,7 @@ typedef struct hoc_geometry {
   const double *tau_x, *tau_y;                 /* [ny][nx] wind stress, N m-2 */
   const double *ustar2;                        /* [ny][nx] hypot(tau_x, tau_y) / rho0 */
   const double *kappa_b;                       /* [ny][nx] background diffusivity */
+  const double *H;                             /* [ny][nx] depth; read when an edge is open */
 } hoc_geometry;
 
 /* Prognostic state (device). T and S may be NULL when tracers_ts == 0;

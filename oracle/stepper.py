@@ -3,8 +3,8 @@
 `OracleStepper(domain, physics, params, dt).step(state)` returns a new
 state exactly as `Stepper3DV05.step` does, for the theta, fb and
 split_explicit schemes, closure none|tke, tracers buoyancy|TS, every
-advection scheme and every EOS. Not restated: Flather open boundaries and
-moving vertical coordinates.
+advection scheme and every EOS, periodic, closed or Flather open boundaries.
+Not restated: moving vertical coordinates.
 
 Inputs are duck-typed: `domain` needs grid (nx, ny, nz, dx, dy, dz), mask,
 mask_u, mask_v, mask3, mask3u, mask3v, dz3_ref, face_thickness(),
@@ -45,8 +45,6 @@ class OracleStepper:
         params.validate()
         if domain.vcoord != "zlevel":
             raise ValueError("the oracle restates the z-level coordinate only")
-        if "open" in (domain.bc_x, domain.bc_y):
-            raise ValueError("the oracle does not restate Flather open boundaries")
         self.domain, self.grid = domain, domain.grid
         self.physics, self.params = physics, params
         self.dt = float(dt)
@@ -177,9 +175,39 @@ class OracleStepper:
             ex_v += self.physics.A_h * hz.laplacian(v, g.dx, g.dy, d.mask3u, d.mask3v, d.mask3v)
         return ex_u, ex_v
 
-    def _div_bt(self, U, V):
-        g = self.grid
-        return hz.div(U, V, g.dx, g.dy, self.domain.mask)
+    def _flather(self, eta):
+        """Flather radiation transports on the open edges
+        (libs/core/model3d_v05.py:262-288)."""
+        d, p = self.domain, self.physics
+        bnd = {}
+        if d.bc_x == "open":
+            Hw = np.maximum(d.H[:, 0], 1.0)
+            He = np.maximum(d.H[:, -1], 1.0)
+            bnd["west"] = p.u_ext - np.sqrt(p.g * Hw) * (eta[:, 0] - p.eta_ext)
+            bnd["east"] = p.u_ext + np.sqrt(p.g * He) * (eta[:, -1] - p.eta_ext)
+        if d.bc_y == "open":
+            Hs = np.maximum(d.H[0, :], 1.0)
+            Hn = np.maximum(d.H[-1, :], 1.0)
+            bnd["south"] = p.u_ext - np.sqrt(p.g * Hs) * (eta[0, :] - p.eta_ext)
+            bnd["north"] = p.u_ext + np.sqrt(p.g * Hn) * (eta[-1, :] - p.eta_ext)
+        return bnd
+
+    def _div_bt(self, U, V, eta):
+        """Barotropic divergence with the open-edge fluxes
+        (libs/core/model3d_v05.py:290-309)."""
+        d, g = self.domain, self.grid
+        div = hz.div(U, V, g.dx, g.dy, d.mask)
+        if d.bc_x != "open" and d.bc_y != "open":
+            return div
+        b = self._flather(eta)
+        div = div.copy()
+        if "west" in b:
+            div[:, 0] += (U[:, 0] - b["west"]) / g.dx - ((U[:, 0] - U[:, -1]) / g.dx)
+            div[:, -1] += (b["east"] - U[:, -2]) / g.dx - ((U[:, -1] - U[:, -2]) / g.dx)
+        if "south" in b:
+            div[0, :] += (V[0, :] - b["south"]) / g.dy - ((V[0, :] - V[-1, :]) / g.dy)
+            div[-1, :] += (b["north"] - V[-2, :]) / g.dy - ((V[-1, :] - V[-2, :]) / g.dy)
+        return div * d.mask
 
     def _solve_column(self, coeffs, rhs):
         self.tridiagonal_solves += 1
@@ -220,7 +248,7 @@ class OracleStepper:
         gy_eta = hz.grady(s.eta, g.dy, d.mask_v)[None] * d.mask3v
         U_old = np.sum(self.dz3u * s.u, axis=0)
         V_old = np.sum(self.dz3v * s.v, axis=0)
-        div_old = self._div_bt(U_old, V_old)
+        div_old = self._div_bt(U_old, V_old, s.eta)
 
         u_it, v_it, eta_new = s.u, s.v, s.eta
         for _ in range(pr.n_picard):
@@ -234,7 +262,7 @@ class OracleStepper:
             gh_v = self._solve_column(self.tri_m, gv + stress_v) * mask3
             if pr.name == "theta":
                 div_g = self._div_bt(np.sum(self.dz3u * gh_u, axis=0),
-                                     np.sum(self.dz3v * gh_v, axis=0))
+                                     np.sum(self.dz3v * gh_v, axis=0), s.eta)
                 rhs = s.eta - dt * ((1.0 - th) * div_old + th * div_g)
                 eta_new, rep = self.solver.solve(rhs)
                 self.solver_iterations += rep.iterations
@@ -246,7 +274,7 @@ class OracleStepper:
                 u_it, v_it = gh_u, gh_v
         if pr.name == "fb":
             eta_new = s.eta - dt * self._div_bt(np.sum(self.dz3u * u_it, axis=0),
-                                                np.sum(self.dz3v * v_it, axis=0))
+                                                np.sum(self.dz3v * v_it, axis=0), s.eta)
         w = self.w_diag(u_it, v_it)
         return self._advance_tracers(s, u_it, v_it, w, eta_new)
 
@@ -270,7 +298,7 @@ class OracleStepper:
                 Ui = (U + ddt * (p.f * v_cor - p.g * self.Ku * gxe + fx)) * d.mask_u
                 Vi = (V + ddt * (-p.f * u_cor - p.g * self.Kv * gye + fy)) * d.mask_v
             U, V = Ui, Vi
-            e = e - ddt * self._div_bt(U, V)
+            e = e - ddt * self._div_bt(U, V, e)
             acc_u += U
             acc_v += V
         self.barotropic_substeps += self.n_split

@@ -45,6 +45,28 @@ COMPOSED = {
         dict(eos="seos", pgf="remove", closure="none"),
         dict(name="split_explicit", advection="centered2", solver="pcg_jacobi", n_split=12,
              barotropic_average="mean", theta_cor=0.6, n_picard=3), 600.0, 2),
+    "theta_open_x_flather": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
+             bc_x="open", bc_y="closed"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-4, kappa=1e-5, rho0=1025.0,
+             tau_x=0.05, tau_y=0.02, tracers="TS", A_h=50.0, K_h=50.0),
+        dict(eos="linear", pgf="keep", closure="tke", mxl="integral", eta_ext=0.01, u_ext=0.02),
+        dict(name="theta", advection="up3_tvd", solver="pcg_jacobi", rtol=1e-12), 600.0, 3),
+    "fb_open_y_buoyancy": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
+             bc_x="closed", bc_y="open"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=1e-5, nu=1e-3, kappa=1e-3, rho0=1025.0,
+             tau_x=0.02, tau_y=0.0, tracers="buoyancy", A_h=10.0, K_h=10.0),
+        dict(eos="linear", pgf="remove", closure="none", eta_ext=-0.02, u_ext=0.01),
+        dict(name="fb", advection="centered2", solver="pcg_jacobi"), 20.0, 3),
+    "split_open_both_seos": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
+             bc_x="open", bc_y="open"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-4, kappa=1e-5, rho0=1025.0,
+             tau_x=0.05, tau_y=0.0, tracers="TS", A_h=0.0, K_h=10.0),
+        dict(eos="seos", pgf="keep", closure="tke", mxl="recursive", eta_ext=0.0, u_ext=0.0),
+        dict(name="split_explicit", advection="up3", solver="pcg_jacobi",
+             barotropic_average="mean"), 600.0, 2),
     "fb_centred_buoyancy": (
         dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
              bc_x="periodic", bc_y="periodic"),

@@ -26,7 +26,7 @@ class HocConfig(C.Structure):
         ("n_picard", i32), ("advection", i32), ("solver", i32), ("rtol", f64),
         ("max_iter", i32), ("mg_levels", i32), ("mg_sweeps", i32), ("mg_omega", f64),
         ("n_split", i32), ("barotropic_mean", i32), ("open_x", i32), ("open_y", i32),
-        ("eta_ext", f64), ("u_ext", f64),
+        ("vcoord", i32), ("face_rule", i32), ("eta_ext", f64), ("u_ext", f64),
         ("g", f64), ("f", f64), ("rho0", f64), ("nu_b", f64), ("bottom_drag", f64),
         ("N2", f64), ("tracers_ts", i32),
         ("alpha_T", f64), ("beta_S", f64), ("T0", f64), ("S0", f64), ("cp", f64),
@@ -66,6 +66,7 @@ PROTOTYPES = {
     "hoc_stepper_build_coefficients": ([VP, VP], i32),
     "hoc_stepper_step": ([VP, C.POINTER(HocState), C.POINTER(HocState), VP], i32),
     "hoc_stepper_counters": ([VP, C.POINTER(HocCounters)], i32),
+    "hoc_stepper_z_centre": ([VP, VP, VP], i32),
     "hoc_stepper_tke_step": ([VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_momentum_advection": ([VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_tracer_advection": ([VP, VP, VP, VP, VP, VP, VP], i32),

@@ -139,6 +139,7 @@ struct hoc_stepper {
   double *dz3 = nullptr, *dz3u = nullptr, *dz3v = nullptr, *zc = nullptr, *dzi = nullptr;
   double *tau_x = nullptr, *tau_y = nullptr, *tau_u = nullptr, *tau_v = nullptr;
   double *ustar2 = nullptr, *kappa_b = nullptr, *Hd = nullptr;
+  double* dz3_ref = nullptr;  // resting thickness (moving coordinates)
   // mixing (caller-owned)
   double *Km = nullptr, *Kh = nullptr;
   // coefficients
@@ -269,6 +270,8 @@ int derive_params(const hoc_config& c, StepParams& p) {
   p.n_split = c.n_split;
   p.bt_mean = c.barotropic_mean != 0;
   p.ddt = c.n_split > 0 ? c.dt / c.n_split : 0.0;
+  p.vcoord = c.vcoord;
+  p.face_rule = c.face_rule;
   p.open_x = c.open_x != 0;
   p.open_y = c.open_y != 0;
   p.eta_ext = c.eta_ext;
@@ -285,6 +288,10 @@ int validate(const hoc_config& c) {
     return fail(HOC_ERR_INVALID, "scheme must be theta (0), fb (1) or split_explicit (2)");
   if (c.scheme == 2 && c.n_split < 1)
     return fail(HOC_ERR_INVALID, "split_explicit needs n_split >= 1");
+  if (c.vcoord < 0 || c.vcoord > 2)
+    return fail(HOC_ERR_INVALID, "vcoord must be zlevel (0), zstar (1) or sigma (2)");
+  if (c.face_rule < 0 || c.face_rule > 2)
+    return fail(HOC_ERR_INVALID, "face_rule must be min (0), mean (1) or harmonic (2)");
   if (c.scheme == 0 && !(c.theta > 0.0 && c.theta <= 1.0))
     return fail(HOC_ERR_INVALID, "theta must lie in (0, 1]");
   if (c.n_picard < 1) return fail(HOC_ERR_INVALID, "n_picard must be >= 1");
@@ -863,10 +870,16 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   rc = rc ? rc : upload(h->tau_y, g->tau_y, d.n2, "tau_y");
   rc = rc ? rc : upload(h->ustar2, g->ustar2, d.n2, "ustar2");
   rc = rc ? rc : upload(h->kappa_b, g->kappa_b, d.n2, "kappa_b");
-  if (rc == HOC_OK && (cfg->open_x || cfg->open_y)) {
-    if (!g->H) return cleanup(fail(HOC_ERR_INVALID, "open boundaries need geometry H"));
+  if (rc == HOC_OK && (cfg->open_x || cfg->open_y || cfg->vcoord != 0)) {
+    if (!g->H)
+      return cleanup(fail(HOC_ERR_INVALID,
+                          "open boundaries and moving coordinates need geometry H"));
     rc = h->alloc(&h->Hd, d.n2);
     rc = rc ? rc : upload(h->Hd, g->H, d.n2, "H");
+  }
+  if (rc == HOC_OK && cfg->vcoord != 0) {
+    rc = h->alloc(&h->dz3_ref, d.n3);
+    rc = rc ? rc : upload(h->dz3_ref, g->dz3, d.n3, "dz3");
   }
   if (rc != HOC_OK) return cleanup(rc);
   if (hipMemset(h->bar, 0, 2 * sizeof(unsigned)) != hipSuccess ||
@@ -1014,6 +1027,19 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   const Geo& geo = h->geo;
   const bool theta = h->cfg.scheme == 0;
 
+  if (h->cfg.vcoord != 0) {
+    // z* / sigma: thickness, faces, centre depths and coefficients follow
+    // the free surface (model3d_v05.py:346-347,128-139); the rebuilt solver
+    // is a new one, not counted as a rebuild
+    k_moving_dz<<<blocks(d.n2), TPB, 0, s>>>(d, h->m2, h->Hd, in->eta, h->dz3_ref, h->dz3,
+                                             h->zc, h->dzi);
+    k_moving_faces<<<blocks(d.n3), TPB, 0, s>>>(d, p, h->m3u, h->m3v, h->dz3, h->dz3u,
+                                                h->dz3v);
+    if (h->cfg.scheme == 2) k_face_depth<<<blocks(d.n2), TPB, 0, s>>>(d, geo, h->Hu, h->Hv);
+    LAUNCH_CHECK("moving geometry");
+    h->coeffs_built = false;
+    TRY(build_coefficients(h, s));
+  }
   if (tke) {
     TRY(hoc_stepper_tke_step(h, in->e, in->u, in->v, in->b, out->e, stream));
     h->tridiagonal_solves += 1;
@@ -1078,6 +1104,13 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;
+  return HOC_OK;
+}
+
+int hoc_stepper_z_centre(hoc_stepper* h, double* out, void* stream) {
+  if (!h || !out) return fail(HOC_ERR_INVALID, "null argument");
+  HIP_TRY(hipMemcpyAsync(out, h->zc, h->d.n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream));
   return HOC_OK;
 }
 

@@ -76,6 +76,10 @@ __global__ void k_rhs_eta(Dims d, StepParams p, Geo geo, const double* eta,
                           double* rhs);
 __global__ void k_eta_fb(Dims d, StepParams p, Geo geo, const double* eta, const double* U,
                          const double* V, double* eta_new);
+__global__ void k_moving_dz(Dims d, const uint8_t* m2, const double* H, const double* eta,
+                            const double* dz3_ref, double* dz3, double* zc, double* dzi);
+__global__ void k_moving_faces(Dims d, StepParams p, const uint8_t* m3u, const uint8_t* m3v,
+                               const double* dz3, double* dz3u, double* dz3v);
 __global__ void k_split_forcing(Dims d, StepParams p, Geo geo, const double* U0, const double* V0,
                                 const double* Um, const double* Vm, double* fx, double* fy);
 __global__ void k_bt_picard(Dims d, StepParams p, Geo geo, const double* U, const double* V,

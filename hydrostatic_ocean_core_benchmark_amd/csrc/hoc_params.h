@@ -25,6 +25,7 @@ struct StepParams {
   int n_picard, solver, max_iter, has_A_h, has_K_h;
   int n_split, bt_mean;  // split_explicit: barotropic substeps, averaged transports
   int open_x, open_y;    // Flather open edges
+  int vcoord, face_rule; // vertical coordinate (0 z-level, 1 z*, 2 sigma), face rule
   double dx, dy, dz_nom, dt;
   double theta, theta_cor, theta_v;
   double g, f, rho0, nu_b, bottom_drag, N2;

@@ -36,6 +36,52 @@ __global__ void k_geometry(Dims d, StepParams p, const double* __restrict__ dz3,
   tau_v[c] = (tau_y[c] / p.rho0) * mk(m3v[c]);
 }
 
+// ------------------------------------------ moving vertical coordinates
+// z* and sigma (libs/core/domain.py layer_thickness / face_thickness /
+// depth_at_centres, rebuilt from eta every step, model3d_v05.py:128-139):
+// dz3 = dz3_ref * (1 + eta / H) in wet columns, centre depths from the
+// surface, interface spacing; then the face thickness by the face rule.
+__global__ void k_moving_dz(Dims d, const uint8_t* __restrict__ m2, const double* __restrict__ H,
+                            const double* __restrict__ eta, const double* __restrict__ dz3_ref,
+                            double* __restrict__ dz3, double* __restrict__ zc,
+                            double* __restrict__ dzi) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const double scale = m2[c] ? 1.0 + eta[c] / (H[c] > 0.0 ? H[c] : 1.0) : 1.0;
+  double above = 0.0, prev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const double dz = dz3_ref[k * n2 + c] * scale;
+    dz3[k * n2 + c] = dz;
+    if (k == 1) above = prev;
+    else if (k > 1) above = above + prev;
+    zc[k * n2 + c] = above + 0.5 * dz;
+    dzi[k * n2 + c] = (k == 0) ? dz : 0.5 * (prev + dz);
+    prev = dz;
+  }
+  dzi[nz * n2 + c] = prev;
+}
+
+__device__ __forceinline__ double face_value(int rule, double a, double b) {
+  if (rule == 0) return nmin(a, b);
+  if (rule == 1) return 0.5 * (a + b);
+  const double s = a + b;
+  return s > 0.0 ? 2.0 * a * b / s : 0.0;
+}
+
+__global__ void k_moving_faces(Dims d, StepParams p, const uint8_t* __restrict__ m3u,
+                               const uint8_t* __restrict__ m3v, const double* __restrict__ dz3,
+                               double* __restrict__ dz3u, double* __restrict__ dz3v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2, o = t - c;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cn = at2(d, wrap(j, 1, d.ny), i);
+  dz3u[t] = mk(m3u[t]) * face_value(p.face_rule, dz3[t], dz3[o + ce]);
+  dz3v[t] = mk(m3v[t]) * face_value(p.face_rule, dz3[t], dz3[o + cn]);
+}
+
 // ---------------------------------------------------------- TKE closure
 // libs/core/closure.py:141-205 in three passes over [nz+1][ny][nx] interface
 // fields:

@@ -50,8 +50,6 @@ class HipStepper:
         """`ustar2` optionally replaces the surface friction-velocity field
         u*^2 [ny, nx] (default hypot(tau_x, tau_y) / rho0, as the reference)."""
         params.validate()
-        if domain.vcoord != "zlevel":
-            raise NotImplementedError("HipStepper runs the z-level coordinate")
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:
             raise ValueError(
                 "solver 'fft' needs a flat bottom and periodic boundaries; "
@@ -97,6 +95,8 @@ class HipStepper:
         cfg.scheme = {"theta": 0, "fb": 1, "split_explicit": 2}[params.name]
         cfg.n_split = self.n_split
         cfg.barotropic_mean = 1 if getattr(params, "barotropic_average", "none") == "mean" else 0
+        cfg.vcoord = {"zlevel": 0, "zstar": 1, "sigma": 2}[domain.vcoord]
+        cfg.face_rule = {"min": 0, "mean": 1, "harmonic": 2}[domain.face_rule]
         cfg.open_x = 1 if domain.bc_x == "open" else 0
         cfg.open_y = 1 if domain.bc_y == "open" else 0
         cfg.eta_ext = float(getattr(physics, "eta_ext", 0.0))
@@ -255,6 +255,9 @@ class HipStepper:
         p = self.physics
         Td, Sd = self._dev(T), self._dev(S)
         b = torch.empty_like(Td)
+        if self.moving_coordinate:   # centre depths follow the last step's eta
+            _lib.check(self.lib.hoc_stepper_z_centre(self._h, _lib.ptr(self.z_centre),
+                                                     self._stream()))
         _lib.check(self.lib.hoc_eos_buoyancy(EOS[p.eos], p.g, p.rho0, p.alpha_T, p.beta_S,
                                              p.T0, p.S0, _lib.ptr(Td), _lib.ptr(Sd),
                                              _lib.ptr(self.z_centre), _lib.ptr(b), Td.numel(),
@@ -307,7 +310,7 @@ class HipStepper:
 
     @property
     def moving_coordinate(self) -> bool:
-        return False
+        return self.domain.vcoord != "zlevel"
 
     def counters(self) -> dict:
         c = self._counters()

@@ -50,6 +50,8 @@ typedef struct hoc_config {
   int32_t n_split;               /* split_explicit: barotropic substeps per step (>= 1) */
   int32_t barotropic_mean;       /* split_explicit: 1 = mean of the substep transports */
   int32_t open_x, open_y;        /* Flather open boundaries on the x / y domain edges */
+  int32_t vcoord;                /* 0 z-level, 1 z* (zstar), 2 sigma: moving unless 0 */
+  int32_t face_rule;             /* face thickness: 0 min, 1 mean, 2 harmonic */
   double eta_ext, u_ext;         /* Flather external level and transport */
   double g, f, rho0, nu_b, bottom_drag, N2;
   int32_t tracers_ts;      /* 1: T, S prognostic and b diagnosed; 0: b prognostic */
@@ -71,7 +73,8 @@ typedef struct hoc_geometry {
   const double *tau_x, *tau_y;                 /* [ny][nx] wind stress, N m-2 */
   const double *ustar2;                        /* [ny][nx] hypot(tau_x, tau_y) / rho0 */
   const double *kappa_b;                       /* [ny][nx] background diffusivity */
-  const double *H;                             /* [ny][nx] depth; read when an edge is open */
+  const double *H;                             /* [ny][nx] depth; read for open edges and
+                                                  moving vertical coordinates */
 } hoc_geometry;
 
 /* Prognostic state (device). T and S may be NULL when tracers_ts == 0;
@@ -107,6 +110,11 @@ int hoc_stepper_build_coefficients(hoc_stepper* h, void* stream);
  * (distinct buffers). Synchronises on `stream` once per Helmholtz solve to
  * poll convergence. */
 int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream);
+
+/* The current depth of the cell centres [nz][ny][nx] (device, written on
+ * `stream`): fixed for z-level, rebuilt from eta every step for z* and sigma
+ * (the reference's self.z_centre, model3d_v05.py:128-139). */
+int hoc_stepper_z_centre(hoc_stepper* h, double* out, void* stream);
 
 /* Counters accumulated since create (synchronises). */
 int hoc_stepper_counters(hoc_stepper* h, hoc_counters* out);

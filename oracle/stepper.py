@@ -3,8 +3,8 @@
 `OracleStepper(domain, physics, params, dt).step(state)` returns a new
 state exactly as `Stepper3DV05.step` does, for the theta, fb and
 split_explicit schemes, closure none|tke, tracers buoyancy|TS, every
-advection scheme and every EOS, periodic, closed or Flather open boundaries.
-Not restated: moving vertical coordinates.
+advection scheme and every EOS, periodic, closed or Flather open
+boundaries, on z-level or moving (z*, sigma) vertical coordinates.
 
 Inputs are duck-typed: `domain` needs grid (nx, ny, nz, dx, dy, dz), mask,
 mask_u, mask_v, mask3, mask3u, mask3v, dz3_ref, face_thickness(),
@@ -43,8 +43,6 @@ class OState:
 class OracleStepper:
     def __init__(self, domain, physics, params, dt):
         params.validate()
-        if domain.vcoord != "zlevel":
-            raise ValueError("the oracle restates the z-level coordinate only")
         self.domain, self.grid = domain, domain.grid
         self.physics, self.params = physics, params
         self.dt = float(dt)
@@ -60,16 +58,11 @@ class OracleStepper:
         self.nu = np.full(shape3i, physics.nu)
         self.kappa = np.broadcast_to(np.asarray(physics.kappa, dtype=np.float64),
                                      shape3i).copy()
-        d = domain
-        self.dz3 = d.dz3_ref
-        self.dz3u, self.dz3v = d.face_thickness(self.dz3)
-        self.z_centre = d.depth_at_centres()
-        self.solver = None
-        self._build_coefficients()
+        self._build_geometry(np.zeros(self.grid.shape))
         if params.name == "split_explicit":
             # libs/core/model3d_v05.py:111-123: the deepest column sets the
             # barotropic CFL
-            h_max = float(np.max(d.H[d.mask > 0]))
+            h_max = float(np.max(domain.H[domain.mask > 0]))
             c = np.sqrt(physics.g * h_max)
             dt_baro = 1.0 / (c * np.sqrt(1.0 / self.grid.dx ** 2 + 1.0 / self.grid.dy ** 2))
             self.n_split = (params.n_split if params.n_split > 0
@@ -78,6 +71,21 @@ class OracleStepper:
                 raise ValueError(
                     f"n_split={self.n_split} leaves a barotropic substep of "
                     f"{self.dt / self.n_split:.1f}s above the limit {dt_baro:.1f}s")
+
+    @property
+    def moving_coordinate(self):
+        return self.domain.vcoord != "zlevel"
+
+    def _build_geometry(self, eta):
+        """Layer and face thickness, centre depths and the implicit/Helmholtz
+        coefficients for the free surface eta (libs/core/model3d_v05.py:128-139);
+        once for z-level, every step for z* and sigma."""
+        d = self.domain
+        self.dz3 = d.layer_thickness(eta)
+        self.dz3u, self.dz3v = d.face_thickness(self.dz3)
+        self.z_centre = d.depth_at_centres(eta)
+        self.solver = None
+        self._build_coefficients()
 
     # ---------------------------------------------------------- coefficients
     def _build_coefficients(self):
@@ -221,6 +229,8 @@ class OracleStepper:
         dt, th, tc, thv = self.dt, pr.theta, pr.theta_cor, pr.theta_v
         if pr.name != "theta":
             th = 0.0
+        if self.moving_coordinate:
+            self._build_geometry(s.eta)
         e_new = self._closure_step(s) if p.closure == "tke" else s.e
         self._e_new = e_new
         mask3 = d.mask3

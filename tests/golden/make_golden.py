@@ -38,11 +38,11 @@ from libs.core.schemes import SchemeParams  # noqa: E402
 from libs.core.solvers_var import HelmholtzVar  # noqa: E402
 
 
-def _domain(nx, ny, nz, Lx, Ly, H0, topo, bc_x, bc_y, island=False, seed=7):
+def _domain(nx, ny, nz, Lx, Ly, H0, topo, bc_x, bc_y, island=False, seed=7, vcoord="zlevel"):
     grid = CGrid(nx=nx, ny=ny, Lx=Lx, Ly=Ly, nz=nz, depth=H0)
     H, mask = build_bathymetry(grid, topo, H0, r_target=0.25, seed=seed, island=island,
                                h_rel=0.6)
-    return Domain(grid=grid, H=H, mask=mask, vcoord="zlevel", bc_x=bc_x, bc_y=bc_y)
+    return Domain(grid=grid, H=H, mask=mask, vcoord=vcoord, bc_x=bc_x, bc_y=bc_y)
 
 
 def _geometry(d: Domain, prefix: str) -> dict:

@@ -67,6 +67,27 @@ COMPOSED = {
         dict(eos="seos", pgf="keep", closure="tke", mxl="recursive", eta_ext=0.0, u_ext=0.0),
         dict(name="split_explicit", advection="up3", solver="pcg_jacobi",
              barotropic_average="mean"), 600.0, 2),
+    "theta_zstar_tke_tvd": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
+             bc_x="closed", bc_y="periodic", vcoord="zstar"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-4, kappa=1e-5, rho0=1025.0,
+             tau_x=0.05, tau_y=0.02, tracers="TS", A_h=50.0, K_h=50.0),
+        dict(eos="teos10", pgf="keep", closure="tke", mxl="integral"),
+        dict(name="theta", advection="up3_tvd", solver="pcg_jacobi", rtol=1e-12), 600.0, 3),
+    "fb_sigma_buoyancy": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
+             bc_x="periodic", bc_y="periodic", vcoord="sigma"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=1e-5, nu=1e-3, kappa=1e-3, rho0=1025.0,
+             tau_x=0.02, tau_y=0.0, tracers="buoyancy", A_h=10.0, K_h=10.0),
+        dict(eos="linear", pgf="remove", closure="none"),
+        dict(name="fb", advection="centered2", solver="pcg_jacobi"), 20.0, 3),
+    "split_zstar_open_x": (
+        dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
+             bc_x="open", bc_y="closed", vcoord="zstar"),
+        dict(g=9.80616, f=1e-4, H=1000.0, N2=0.0, nu=1e-4, kappa=1e-5, rho0=1025.0,
+             tau_x=0.1, tau_y=0.0, tracers="TS", A_h=10.0, K_h=10.0),
+        dict(eos="seos", pgf="keep", closure="tke", mxl="recursive", eta_ext=0.01),
+        dict(name="split_explicit", advection="up3_tvd", solver="pcg_jacobi"), 600.0, 2),
     "fb_centred_buoyancy": (
         dict(nx=24, ny=16, nz=8, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="seamount",
              bc_x="periodic", bc_y="periodic"),
@@ -85,7 +106,7 @@ def build_case(name, golden):
     dk, pk, vk, sk, dt, steps = COMPOSED[name]
     grid = CGrid(nx=dk["nx"], ny=dk["ny"], Lx=dk["Lx"], Ly=dk["Ly"], nz=dk["nz"], depth=dk["H0"])
     dom = Domain(grid=grid, H=golden["geo_H"], mask=golden["geo_mask"], bc_x=dk["bc_x"],
-                 bc_y=dk["bc_y"])
+                 bc_y=dk["bc_y"], vcoord=dk.get("vcoord", "zlevel"))
     phys = PhysicsV05(base=Physics3D(**pk), **vk)
     params = SchemeParams(**sk)
     s0 = {k: (golden[f"in_{k}"] if f"in_{k}" in golden else None)

@@ -30,7 +30,8 @@ def test_domain_geometry_matches_reference(name):
     H, mask = build_bathymetry(grid, dk["topo"], dk["H0"], r_target=0.25, seed=7,
                                island=dk.get("island", False), h_rel=0.6)
     assert np.array_equal(H, f["geo_H"]) and np.array_equal(mask, f["geo_mask"])
-    d = Domain(grid=grid, H=H, mask=mask, bc_x=dk["bc_x"], bc_y=dk["bc_y"])
+    d = Domain(grid=grid, H=H, mask=mask, bc_x=dk["bc_x"], bc_y=dk["bc_y"],
+               vcoord=dk.get("vcoord", "zlevel"))
     dzu, dzv = d.face_thickness(d.dz3_ref)
     got = dict(dz3=d.dz3_ref, mask3=d.mask3, mask3u=d.mask3u, mask3v=d.mask3v,
                mask_u=d.mask_u, mask_v=d.mask_v, dz3u=dzu, dz3v=dzv, zc=d.depth_at_centres())

@@ -74,6 +74,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--dx", type=float, default=1.0e4)
+    ap.add_argument("--dt", type=float, default=600.0)
     ap.add_argument("--kernel", default="k_tracers",
                     help="kernel timed with HIP events for the roofline figure")
     ap.add_argument("--solver-path", choices=("multikernel", "persistent"), default="multikernel",
@@ -81,7 +83,12 @@ def parse():
     ap.add_argument("--solver", default=None,
                     choices=("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid", "pcg_mg"),
                     help="Helmholtz solver kind (default: the workload's)")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=50,
+                    help="steps of the plain-vs-mixed comparison on the 1024 x 1024 x 60 "
+                         "basin (0: skip it)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the workload, copy this many MiB device to device once "
                          "(a kernel of known traffic, to scale HBM byte counters)")
@@ -115,6 +122,66 @@ def cpu_baseline(nz: int, solver: str) -> dict:
                       f"{os.environ.get('OMP_NUM_THREADS', 'unset')})"}
 
 
+def _initial_fields(prob, st, torch):
+    g = prob.domain.grid
+    T = torch.as_tensor(prob.initial_T, device=st.device)
+    S = torch.as_tensor(prob.initial_S, device=st.device)
+    b = st.buoyancy_of(T, S) * torch.as_tensor(prob.domain.mask3, device=st.device)
+    zero3 = torch.zeros(g.shape3d, dtype=torch.float64, device=st.device)
+    return {"u": zero3.clone(), "v": zero3.clone(), "b": b, "T": T, "S": S,
+            "eta": torch.zeros(g.shape, dtype=torch.float64, device=st.device),
+            "e": torch.full((g.nz + 1,) + g.shape, float(prob.physics.e_min),
+                            dtype=torch.float64, device=st.device)}
+
+
+def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60) -> dict:
+    """The plain fp64 step and the mixed step on the 1024 x 1024 x 60 basin
+    (dx 5 km, dt 300 s, same generator), each timed over `steps` steps after
+    one untimed warm-up step from the same initial state; per-field relative
+    L2 of the mixed final state against the fp64 one (the fp64 GPU step is
+    the oracle's bit for bit on the parity cases)."""
+    import torch
+
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    prob = basin(nx=nx, nz=nz, dx=5.0e3, dt=300.0)
+    g = prob.domain.grid
+    cells = g.nx * g.ny * g.nz
+    final, rows = {}, {}
+    for prec in ("fp64", "mixed"):
+        st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=prec)
+        bufs = [st._new_state_buffers(), st._new_state_buffers()]
+        st._step_into(_initial_fields(prob, st, torch), bufs[1])  # warm-up
+        st.reset_mixing()
+        cur = _initial_fields(prob, st, torch)
+        torch.cuda.synchronize()
+        it0 = st.solver_iterations
+        t0 = time.perf_counter()
+        for n in range(steps):
+            st._step_into(cur, bufs[n % 2])
+            cur = bufs[n % 2]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        final[prec] = {k: cur[k].clone() for k in ("eta", "T", "S", "u", "v", "b")}
+        rows[prec] = {"value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                      "solver_iterations": st.solver_iterations - it0,
+                      "solver_failures": st.solver_failures}
+        del st, bufs, cur
+        torch.cuda.empty_cache()
+    err = {k: float(torch.linalg.vector_norm(final["mixed"][k] - final["fp64"][k])
+                    / torch.linalg.vector_norm(final["fp64"][k])) for k in final["fp64"]}
+    return {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 5 km, dt 300 s, "
+                        f"{steps} steps, polyTEOS10, up3_tvd, TKE, theta 0.5, pcg_mg "
+                        "(5 levels, 2 + 2 damped-Jacobi sweeps, omega 0.8) rtol 1e-12",
+            "unit": "cell-steps/s", "fp64": rows["fp64"], "mixed": rows["mixed"],
+            "speedup": rows["mixed"]["value"] / rows["fp64"]["value"],
+            "rel_l2_vs_fp64": err,
+            "fp32_parts": "pcg_mg V-cycle preconditioner, advective fluxes (tracers and "
+                          "momentum), EOS (anomaly form)",
+            "fp64_parts": "state updates, column (Thomas) solves, closure, outer PCG "
+                          "residual / dot products / update"}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -137,21 +204,13 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    prob = basin(nx=args.nx, nz=args.nz)
+    prob = basin(nx=args.nx, nz=args.nz, dx=args.dx, dt=args.dt)
     if args.solver:
         prob.params.solver = args.solver
     g = prob.domain.grid
-    st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+    st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=args.precision)
     st.solver_path = args.solver_path
-    T = torch.as_tensor(prob.initial_T, device="cuda")
-    S = torch.as_tensor(prob.initial_S, device="cuda")
-    mask3 = torch.as_tensor(prob.domain.mask3, device="cuda")
-    b = st.buoyancy_of(T, S) * mask3
-    zero3 = torch.zeros(g.shape3d, dtype=torch.float64, device="cuda")
-    cur = {"u": zero3.clone(), "v": zero3.clone(), "b": b, "T": T, "S": S,
-           "eta": torch.zeros(g.shape, dtype=torch.float64, device="cuda"),
-           "e": torch.full((g.nz + 1,) + g.shape, float(prob.physics.e_min),
-                           dtype=torch.float64, device="cuda")}
+    cur = _initial_fields(prob, st, torch)
     bufs = [st._new_state_buffers(), st._new_state_buffers()]
     n = 0
 
@@ -202,10 +261,13 @@ def main() -> int:
     rec = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": f"fp64 plain step, closed GRF basin {g.nx}x{g.ny}x{g.nz}, "
-                               "dx 10 km, dt 600 s, up3_tvd, TKE, polyTEOS10, theta 0.5, "
+        "config": {"workload": ("fp64 plain" if args.precision == "fp64" else "mixed")
+                               + f" step, closed GRF basin {g.nx}x{g.ny}x{g.nz}, "
+                               f"dx {args.dx / 1e3:g} km, dt {args.dt:g} s, up3_tvd, TKE, "
+                               "polyTEOS10, theta 0.5, "
                                f"{prob.params.solver} rtol {prob.params.rtol:g}",
                    "cells": cells, "wet_fraction": round(float(np.mean(prob.domain.mask3)), 4),
                    "solver": prob.params.solver, "solver_path": st.solver_path,
@@ -214,6 +276,10 @@ def main() -> int:
         "roofline": roofline,
         "step_model": step_model,
     }
+    if args.mixed_steps > 0:
+        del st, bufs, cur
+        torch.cuda.empty_cache()
+        rec["mixed"] = mixed_comparison(args.mixed_steps)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args.nz, prob.params.solver)
     print(json.dumps(rec), flush=True)

@@ -173,6 +173,10 @@ struct hoc_stepper {
   double *mg_Ku[MAXLEV] = {}, *mg_Kv[MAXLEV] = {}, *mg_inv[MAXLEV] = {};
   uint8_t* mg_mask[MAXLEV] = {};
   double *mg_x[MAXLEV] = {}, *mg_t[MAXLEV] = {}, *mg_b[MAXLEV] = {};
+  // the fp32 V-cycle of the mixed-precision preconditioner (same hierarchy)
+  MgLevelF levf[MAXLEV];
+  float *mgf_Ku[MAXLEV] = {}, *mgf_Kv[MAXLEV] = {}, *mgf_inv[MAXLEV] = {};
+  float *mgf_x[MAXLEV] = {}, *mgf_t[MAXLEV] = {}, *mgf_b[MAXLEV] = {};
   bool force_multikernel = true;   // per-phase launches unless path 0 is asked for
   bool coeffs_built = false;
   // host counters
@@ -279,6 +283,59 @@ int derive_params(const hoc_config& c, StepParams& p) {
   return HOC_OK;
 }
 
+// polyTEOS10-bsq coefficients E_ijk of ss^i tt^j zh^k (libs/core/eos.py:68-100;
+// Roquet et al. 2015), for the host-side re-expansion of the mixed EOS.
+struct TeosTerm {
+  int i, j, k;
+  double e;
+};
+constexpr TeosTerm TEOS_TERMS[] = {
+    {0, 0, 0, 8.0189615746e+02}, {1, 0, 0, 8.6672408165e+02}, {2, 0, 0, -1.7864682637e+03},
+    {3, 0, 0, 2.0375295546e+03}, {4, 0, 0, -1.2849161071e+03}, {5, 0, 0, 4.3227585684e+02},
+    {6, 0, 0, -6.0579916612e+01}, {0, 1, 0, 2.6010145068e+01}, {1, 1, 0, -6.5281885265e+01},
+    {2, 1, 0, 8.1770425108e+01}, {3, 1, 0, -5.6888046321e+01}, {4, 1, 0, 1.7681814114e+01},
+    {5, 1, 0, -1.9193502195e+00}, {0, 2, 0, -3.7074170417e+01}, {1, 2, 0, 6.1548258127e+01},
+    {2, 2, 0, -6.0362551501e+01}, {3, 2, 0, 2.9130021253e+01}, {4, 2, 0, -5.4723692739e+00},
+    {0, 3, 0, 2.1661789529e+01}, {1, 3, 0, -3.3449108469e+01}, {2, 3, 0, 1.9717078466e+01},
+    {3, 3, 0, -3.1742946532e+00}, {0, 4, 0, -8.3627885467e+00}, {1, 4, 0, 1.1311538584e+01},
+    {2, 4, 0, -5.3563304045e+00}, {0, 5, 0, 5.4048723791e-01}, {1, 5, 0, 4.5111434961e-01},
+    {0, 6, 0, -1.9098268277e-01}, {0, 0, 1, 1.9681925209e+01}, {1, 0, 1, -4.2549998214e+01},
+    {2, 0, 1, 5.0774768218e+01}, {3, 0, 1, -3.0938076334e+01}, {4, 0, 1, 6.6051753097e+00},
+    {0, 1, 1, -1.3336301113e+01}, {1, 1, 1, -4.4870114575e+00}, {2, 1, 1, 5.0042598061e+00},
+    {3, 1, 1, -6.5399043664e-01}, {0, 2, 1, 6.7080479603e+00}, {1, 2, 1, 3.5063081279e+00},
+    {2, 2, 1, -1.8795372996e+00}, {0, 3, 1, -2.4649669534e+00}, {1, 3, 1, -5.5077101279e-01},
+    {0, 4, 1, 5.5927935970e-01}, {0, 0, 2, 2.0660924175e+00}, {1, 0, 2, -4.9527603989e+00},
+    {2, 0, 2, 2.5019633803e+00}, {0, 1, 2, 2.0564311499e+00}, {1, 1, 2, -2.1311365518e-01},
+    {0, 2, 2, -1.2419983026e+00}, {0, 0, 3, -2.3342758797e-02}, {1, 0, 3, -1.8507636718e-02},
+    {0, 1, 3, 3.7969820455e-01}};
+
+// The mixed EOS's anomaly polynomial: each z-power k of polyTEOS10 as a
+// polynomial in a = tt - tt_r, c = ss - ss_r (binomial re-expansion about
+// tt_r = T0 / 40, ss_r = sqrt((S0 + 32) 0.875 / 35.16504)); G[k][p][q] holds
+// the coefficient of a^p c^q, the constant terms go to ref[k] (fp64).
+void teos_anomaly_table(double T0, double S0, double ref[4], float G[4][7][7],
+                        double* ss_ref) {
+  const double tt_r = T0 * (1.0 / 40.0);
+  const double ss_r = std::sqrt((S0 + 32.0) * (0.875 / 35.16504));
+  *ss_ref = ss_r;
+  double acc[4][7][7] = {};
+  auto binom = [](int n, int r) {
+    double b = 1.0;
+    for (int t = 1; t <= r; ++t) b = b * (n - r + t) / t;
+    return b;
+  };
+  for (const TeosTerm& t : TEOS_TERMS)
+    for (int pp = 0; pp <= t.j; ++pp)
+      for (int q = 0; q <= t.i; ++q)
+        acc[t.k][pp][q] += t.e * binom(t.j, pp) * std::pow(tt_r, t.j - pp) * binom(t.i, q) *
+                           std::pow(ss_r, t.i - q);
+  for (int k = 0; k < 4; ++k) {
+    ref[k] = acc[k][0][0];
+    for (int pp = 0; pp < 7; ++pp)
+      for (int q = 0; q < 7; ++q) G[k][pp][q] = (pp == 0 && q == 0) ? 0.0f : (float)acc[k][pp][q];
+  }
+}
+
 int validate(const hoc_config& c) {
   if (c.nx < 2 || c.ny < 2 || c.nz < 1)
     return fail(HOC_ERR_INVALID, "grid must have nx >= 2, ny >= 2, nz >= 1");
@@ -304,6 +361,12 @@ int validate(const hoc_config& c) {
   if (c.solver == SOLVER_PCG_MG && (c.mg_sweeps < 1 || c.mg_levels < 1 || !(c.mg_omega > 0.0)))
     return fail(HOC_ERR_INVALID, "pcg_mg needs mg_levels >= 1, mg_sweeps >= 1, mg_omega > 0");
   if (c.max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
+  if (c.precision < 0 || c.precision > 1)
+    return fail(HOC_ERR_INVALID, "precision must be fp64 (0) or mixed (1)");
+  if (c.precision == 1 && c.scheme == 0 && c.solver != SOLVER_PCG_MG)
+    return fail(HOC_ERR_INVALID, "mixed precision runs the theta scheme with pcg_mg only");
+  if (c.precision == 1 && c.mg_sweeps > MG_FUSED_MAX_SWEEPS)
+    return fail(HOC_ERR_INVALID, "mixed precision needs mg_sweeps <= 3");
   return HOC_OK;
 }
 
@@ -442,6 +505,18 @@ int mg_build(hoc_stepper* h, const double* Ku, const double* Kv, const double* i
                                              h->mg_mask[l]);
     k_mg_diag<<<blocks(L.n), TPB, 0, s>>>(L, h->mg_inv[l]);
   }
+  if (h->cfg.precision == 1) {
+    for (int l = 0; l < h->nlev; ++l) {
+      const MgLevel& L = h->lev[l];
+      MgLevelF& F = h->levf[l];
+      F.Ku = h->mgf_Ku[l];
+      F.Kv = h->mgf_Kv[l];
+      F.inv_diag = h->mgf_inv[l];
+      F.mask = L.mask;
+      k_mg_to_f32<<<blocks(L.n), TPB, 0, s>>>(L.n, L.Ku, L.Kv, L.inv_diag, h->mgf_Ku[l],
+                                              h->mgf_Kv[l], h->mgf_inv[l]);
+    }
+  }
   LAUNCH_CHECK("multigrid levels");
   return HOC_OK;
 }
@@ -500,6 +575,33 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
     k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
     rbgs_sweep(h, 0, r, z, false, s);
     rbgs_sweep(h, 0, r, z, true, s);
+  } else if (h->cfg.precision == 1) {
+    // the fp32 V-cycle: as the fused fp64 path below, every level's vectors
+    // and coefficients in fp32; level 0 reads the fp64 residual r and writes
+    // the fp64 z (the outer PCG stays fp64)
+    const int S = h->cfg.mg_sweeps, l0 = h->mg_cycle_l0;
+    const float om = (float)h->cfg.mg_omega;
+    if (l0 < 0) return fail(HOC_ERR_DEVICE, "mixed pcg_mg: no one-workgroup coarse cycle");
+    for (int l = 0; l < l0; ++l)
+      launch_mg_down_f32(S, mg_tile_grid(h->levf[l]), s, h->levf[l], h->levf[l + 1], om,
+                         l == 0 ? r : nullptr, l == 0 ? nullptr : h->mgf_b[l], h->mgf_x[l],
+                         h->mgf_b[l + 1], h->sc);
+    {
+      MgCycleF C{};
+      C.nl = h->nlev - l0;
+      for (int l = 0; l < C.nl; ++l) C.lev[l] = h->levf[l0 + l];
+      const size_t lds = mg_cycle_layout(C);
+      launch_mg_coarse_cycle_f32(C, lds, om, S, l0 == 0 ? r : nullptr,
+                                 l0 == 0 ? nullptr : h->mgf_b[l0], l0 == 0 ? z : nullptr,
+                                 l0 == 0 ? nullptr : h->mgf_x[l0], h->sc, s);
+    }
+    const float* ec = h->mgf_x[l0];
+    for (int l = l0 - 1; l >= 0; --l) {
+      launch_mg_up_f32(S, mg_tile_grid(h->levf[l]), s, h->levf[l], h->levf[l + 1].nx, om,
+                       h->mgf_x[l], ec, l == 0 ? r : nullptr, l == 0 ? nullptr : h->mgf_b[l],
+                       l == 0 ? z : nullptr, l == 0 ? nullptr : h->mgf_t[l], h->sc);
+      ec = h->mgf_t[l];
+    }
   } else if (h->cfg.mg_sweeps <= MG_FUSED_MAX_SWEEPS) {
     // fused legs: one launch down and one up per level above the first level
     // of at most MG_COARSE_MAX cells; from there to the last level the cycle
@@ -663,6 +765,14 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     int _rc = h->alloc(&(ptr), (n));     \
     if (_rc != HOC_OK) return cleanup(_rc); \
   } while (0)
+  if (cfg->precision == 1) {
+    float G[4][7][7];
+    teos_anomaly_table(cfg->T0, cfg->S0, h->p.eos_ref, G, &h->p.ss_ref);
+    float* dg = nullptr;
+    A(dg, 4 * 7 * 7);
+    if (upload(dg, &G[0][0][0], 4 * 7 * 7, "eos table") != HOC_OK) return cleanup(HOC_ERR_DEVICE);
+    h->p.eos_g = dg;
+  }
   A(h->m2, d.n2);
   A(h->mu2, d.n2);
   A(h->mv2, d.n2);
@@ -834,6 +944,23 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
       arc = arc ? arc : h->alloc(&h->mg_x[l], L.n);
       arc = arc ? arc : h->alloc(&h->mg_t[l], L.n);
       arc = arc ? arc : h->alloc(&h->mg_b[l], L.n);
+      if (cfg->precision == 1) {
+        MgLevelF& F = h->levf[l];
+        F.nx = lx;
+        F.ny = ly;
+        F.n = L.n;
+        F.coef = (float)L.coef;
+        F.dx2 = (float)L.dx2;
+        F.dy2 = (float)L.dy2;
+        F.cx = (float)L.cx;
+        F.cy = (float)L.cy;
+        arc = arc ? arc : h->alloc(&h->mgf_Ku[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mgf_Kv[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mgf_inv[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mgf_x[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mgf_t[l], L.n);
+        arc = arc ? arc : h->alloc(&h->mgf_b[l], L.n);
+      }
       if (arc != HOC_OK) return cleanup(arc);
       lx /= 2;
       ly /= 2;
@@ -848,12 +975,22 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
         l0 = l;
         break;
       }
+    if (l0 < 0 && h->lev[h->nlev - 1].n <= MG_COARSE_LAST_MAX) l0 = h->nlev - 1;
     if (l0 >= 0 && h->nlev - l0 <= MG_CYCLE_LEVELS) {
       MgCycle C{};
       C.nl = h->nlev - l0;
       for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
       if (mg_cycle_prepare(mg_cycle_layout(C)) == 0) h->mg_cycle_l0 = l0;
+      if (cfg->precision == 1) {
+        MgCycleF CF{};
+        CF.nl = C.nl;
+        for (int l = 0; l < CF.nl; ++l) CF.lev[l] = h->levf[l0 + l];
+        if (mg_cycle_prepare_f32(mg_cycle_layout(CF)) != 0) h->mg_cycle_l0 = -1;
+      }
     }
+    if (cfg->precision == 1 && h->mg_cycle_l0 < 0)
+      return cleanup(fail(HOC_ERR_INVALID, "mixed precision: the coarse end of the multigrid "
+                                           "hierarchy does not fit one workgroup"));
   }
   int rc = HOC_OK;
   rc = rc ? rc : upload(h->m2, g->mask, d.n2, "mask");
@@ -1052,9 +1189,9 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
-  k_momentum_explicit_tiled<<<dim3((d.nx + 31) / 32, (d.ny + 7) / 8), 256, 0, s>>>(
-      d, p, geo, in->u, in->v, in->b, h->phi, h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv,
-      h->dzu, h->dzv, h->stu, h->stv); }
+  launch_momentum_explicit_tiled(h->cfg.precision == 1, s, d, p, geo, in->u, in->v, in->b, h->phi,
+                                 h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
+                                 h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
   const bool split = h->cfg.scheme == 2;
   if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
@@ -1098,9 +1235,8 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
   { ProfScope _ps(h, "k_tracers", s);
-  k_tracers_tiled<<<dim3((d.nx + 31) / 32, (d.ny + 7) / 8), 256, 0, s>>>(
-      d, p, geo, in->T, in->S, in->b, out->u, out->v, h->w, h->Kh, h->sub_b, h->den_b,
-      h->cst_b, out->T, out->S, out->b); }
+  launch_tracers_tiled(h->cfg.precision == 1, s, d, p, geo, in->T, in->S, in->b, out->u, out->v,
+                       h->w, h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S, out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;

@@ -39,6 +39,12 @@ __device__ __forceinline__ double nmax(double a, double b) {
 __device__ __forceinline__ double nmin(double a, double b) {
   return (a <= b || a != a) ? a : b;
 }
+__device__ __forceinline__ float nmax(float a, float b) {
+  return (a >= b || a != a) ? a : b;
+}
+__device__ __forceinline__ float nmin(float a, float b) {
+  return (a <= b || a != a) ? a : b;
+}
 // 1/x where x > 0, else 0 (the reference's `_safe`).
 __device__ __forceinline__ double safe_inv(double x) {
   return x > 0.0 ? 1.0 / x : 0.0;
@@ -60,8 +66,11 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by) {
 }
 
 // Superbee limiter phi(r) = max(0, max(min(2r, 1), min(r, 2))).
-__device__ __forceinline__ double superbee(double r) {
-  return nmax(0.0, nmax(nmin(2.0 * r, 1.0), nmin(r, 2.0)));
+// F is the arithmetic type (double: the reference's; float: the mixed
+// step's advective fluxes).
+template <class F>
+__device__ __forceinline__ F superbee(F r) {
+  return nmax(F(0), nmax(nmin(F(2) * r, F(1)), nmin(r, F(2))));
 }
 
 constexpr double KAPPA = 1.0 / 3.0;
@@ -75,20 +84,21 @@ constexpr double K1P = 1.0 + KAPPA;
 // candidates and selects one by the sign of vel; only the selected one is
 // formed here, with its operands chosen first (one division per face):
 // c1 - t is c1 + (-t) in IEEE arithmetic, so the result is the same bits.
-__device__ __forceinline__ double kappa_face(double cm, double c0, double c1, double c2,
-                                             double vel, bool wm, bool w2, bool limited) {
-  const bool up = vel > 0.0;
-  const double base = up ? c0 : c1;
-  const double num = up ? (c0 - cm) : (c2 - c1);
-  const double dc = c1 - c0;
-  double t;
+template <class F>
+__device__ __forceinline__ F kappa_face(F cm, F c0, F c1, F c2, F vel, bool wm, bool w2,
+                                        bool limited) {
+  const bool up = vel > F(0);
+  const F base = up ? c0 : c1;
+  const F num = up ? (c0 - cm) : (c2 - c1);
+  const F dc = c1 - c0;
+  F t;
   if (limited) {
-    const double r = dc == 0.0 ? 0.0 : num / dc;
-    t = 0.5 * superbee(r) * dc;
+    const F r = dc == F(0) ? F(0) : num / dc;
+    t = F(0.5) * superbee(r) * dc;
   } else {
-    t = 0.25 * (K1M * num + K1P * dc);
+    t = F(0.25) * (F(K1M) * num + F(K1P) * dc);
   }
-  const double f = up ? base + t : base - t;
+  const F f = up ? base + t : base - t;
   return (up ? wm : w2) ? f : base;
 }
 
@@ -96,21 +106,21 @@ __device__ __forceinline__ double kappa_face(double cm, double c0, double c1, do
 // upward; far_up = c[k-2] (0 when k < 2), far_dn = c[k+1] (0 when k > nz-2);
 // has_up / has_dn are the wet flags of those far cells.
 // (libs/core/advection_v06.py:50-74); the selected candidate only, as above.
-__device__ __forceinline__ double kappa_face_z(double far_up, double above, double below,
-                                               double far_dn, double w, bool has_up,
-                                               bool has_dn, bool limited) {
-  const bool up = w > 0.0;
-  const double base = up ? below : above;
-  const double num = up ? (below - far_dn) : (far_up - above);
-  const double d = above - below;
-  double t;
+template <class F>
+__device__ __forceinline__ F kappa_face_z(F far_up, F above, F below, F far_dn, F w,
+                                          bool has_up, bool has_dn, bool limited) {
+  const bool up = w > F(0);
+  const F base = up ? below : above;
+  const F num = up ? (below - far_dn) : (far_up - above);
+  const F d = above - below;
+  F t;
   if (limited) {
-    const double r = d == 0.0 ? 0.0 : num / d;
-    t = 0.5 * superbee(r) * d;
+    const F r = d == F(0) ? F(0) : num / d;
+    t = F(0.5) * superbee(r) * d;
   } else {
-    t = 0.25 * (K1M * num + K1P * d);
+    t = F(0.25) * (F(K1M) * num + F(K1P) * d);
   }
-  const double f = up ? base + t : base - t;
+  const F f = up ? base + t : base - t;
   return (up ? has_dn : has_up) ? f : base;
 }
 
@@ -146,49 +156,50 @@ __device__ __forceinline__ double explicit_diffusion_w(int k, int nz, double f0,
 
 // ------------------------------------------------------------------ EOS
 // libs/core/eos.py:68-133; coefficients of Roquet et al. (2015).
-__device__ __forceinline__ double rho_teos10(double CT, double SA, double z) {
-  const double R00 = 4.6494977072e+01, R01 = -5.2099962525e+00, R02 = 2.2601900708e-01,
-               R03 = 6.4326772569e-02, R04 = 1.5616995503e-02, R05 = -1.7243708991e-03;
-  const double E000 = 8.0189615746e+02, E100 = 8.6672408165e+02, E200 = -1.7864682637e+03,
-               E300 = 2.0375295546e+03, E400 = -1.2849161071e+03, E500 = 4.3227585684e+02,
-               E600 = -6.0579916612e+01;
-  const double E010 = 2.6010145068e+01, E110 = -6.5281885265e+01, E210 = 8.1770425108e+01,
-               E310 = -5.6888046321e+01, E410 = 1.7681814114e+01, E510 = -1.9193502195e+00;
-  const double E020 = -3.7074170417e+01, E120 = 6.1548258127e+01, E220 = -6.0362551501e+01,
-               E320 = 2.9130021253e+01, E420 = -5.4723692739e+00;
-  const double E030 = 2.1661789529e+01, E130 = -3.3449108469e+01, E230 = 1.9717078466e+01,
-               E330 = -3.1742946532e+00;
-  const double E040 = -8.3627885467e+00, E140 = 1.1311538584e+01, E240 = -5.3563304045e+00;
-  const double E050 = 5.4048723791e-01, E150 = 4.5111434961e-01;
-  const double E060 = -1.9098268277e-01;
-  const double E001 = 1.9681925209e+01, E101 = -4.2549998214e+01, E201 = 5.0774768218e+01,
-               E301 = -3.0938076334e+01, E401 = 6.6051753097e+00;
-  const double E011 = -1.3336301113e+01, E111 = -4.4870114575e+00, E211 = 5.0042598061e+00,
-               E311 = -6.5399043664e-01;
-  const double E021 = 6.7080479603e+00, E121 = 3.5063081279e+00, E221 = -1.8795372996e+00;
-  const double E031 = -2.4649669534e+00, E131 = -5.5077101279e-01;
-  const double E041 = 5.5927935970e-01;
-  const double E002 = 2.0660924175e+00, E102 = -4.9527603989e+00, E202 = 2.5019633803e+00;
-  const double E012 = 2.0564311499e+00, E112 = -2.1311365518e-01;
-  const double E022 = -1.2419983026e+00;
-  const double E003 = -2.3342758797e-02, E103 = -1.8507636718e-02;
-  const double E013 = 3.7969820455e-01;
-  const double RDELTA_S = 32.0;
-  const double R1_S0 = 0.875 / 35.16504;
-  const double R1_T0 = 1.0 / 40.0;
-  const double R1_Z0 = 1.0e-4;
+template <class F>
+__device__ __forceinline__ F rho_teos10(F CT, F SA, F z) {
+  const F R00 = F(4.6494977072e+01), R01 = F(-5.2099962525e+00), R02 = F(2.2601900708e-01),
+               R03 = F(6.4326772569e-02), R04 = F(1.5616995503e-02), R05 = F(-1.7243708991e-03);
+  const F E000 = F(8.0189615746e+02), E100 = F(8.6672408165e+02), E200 = F(-1.7864682637e+03),
+               E300 = F(2.0375295546e+03), E400 = F(-1.2849161071e+03), E500 = F(4.3227585684e+02),
+               E600 = F(-6.0579916612e+01);
+  const F E010 = F(2.6010145068e+01), E110 = F(-6.5281885265e+01), E210 = F(8.1770425108e+01),
+               E310 = F(-5.6888046321e+01), E410 = F(1.7681814114e+01), E510 = F(-1.9193502195e+00);
+  const F E020 = F(-3.7074170417e+01), E120 = F(6.1548258127e+01), E220 = F(-6.0362551501e+01),
+               E320 = F(2.9130021253e+01), E420 = F(-5.4723692739e+00);
+  const F E030 = F(2.1661789529e+01), E130 = F(-3.3449108469e+01), E230 = F(1.9717078466e+01),
+               E330 = F(-3.1742946532e+00);
+  const F E040 = F(-8.3627885467e+00), E140 = F(1.1311538584e+01), E240 = F(-5.3563304045e+00);
+  const F E050 = F(5.4048723791e-01), E150 = F(4.5111434961e-01);
+  const F E060 = F(-1.9098268277e-01);
+  const F E001 = F(1.9681925209e+01), E101 = F(-4.2549998214e+01), E201 = F(5.0774768218e+01),
+               E301 = F(-3.0938076334e+01), E401 = F(6.6051753097e+00);
+  const F E011 = F(-1.3336301113e+01), E111 = F(-4.4870114575e+00), E211 = F(5.0042598061e+00),
+               E311 = F(-6.5399043664e-01);
+  const F E021 = F(6.7080479603e+00), E121 = F(3.5063081279e+00), E221 = F(-1.8795372996e+00);
+  const F E031 = F(-2.4649669534e+00), E131 = F(-5.5077101279e-01);
+  const F E041 = F(5.5927935970e-01);
+  const F E002 = F(2.0660924175e+00), E102 = F(-4.9527603989e+00), E202 = F(2.5019633803e+00);
+  const F E012 = F(2.0564311499e+00), E112 = F(-2.1311365518e-01);
+  const F E022 = F(-1.2419983026e+00);
+  const F E003 = F(-2.3342758797e-02), E103 = F(-1.8507636718e-02);
+  const F E013 = F(3.7969820455e-01);
+  const F RDELTA_S = F(32.0);
+  const F R1_S0 = F(0.875) / 35.16504;
+  const F R1_T0 = F(1.0) / 40.0;
+  const F R1_Z0 = F(1.0e-4);
 
-  const double zh = z * R1_Z0;
-  const double ss = sqrt((SA + RDELTA_S) * R1_S0);
-  const double tt = CT * R1_T0;
-  const double r0 = (((((R05 * zh + R04) * zh + R03) * zh + R02) * zh + R01) * zh + R00) * zh;
-  const double rz3 = E013 * tt + E103 * ss + E003;
-  const double rz2 = ((E022 * tt + E112 * ss + E012) * tt + (E202 * ss + E102) * ss + E002);
-  const double rz1 =
+  const F zh = z * R1_Z0;
+  const F ss = sqrt((SA + RDELTA_S) * R1_S0);
+  const F tt = CT * R1_T0;
+  const F r0 = (((((R05 * zh + R04) * zh + R03) * zh + R02) * zh + R01) * zh + R00) * zh;
+  const F rz3 = E013 * tt + E103 * ss + E003;
+  const F rz2 = ((E022 * tt + E112 * ss + E012) * tt + (E202 * ss + E102) * ss + E002);
+  const F rz1 =
       ((((E041 * tt + E131 * ss + E031) * tt + (E221 * ss + E121) * ss + E021) * tt +
         ((E311 * ss + E211) * ss + E111) * ss + E011) * tt +
        (((E401 * ss + E301) * ss + E201) * ss + E101) * ss + E001);
-  const double rz0 =
+  const F rz0 =
       ((((((E060 * tt + E150 * ss + E050) * tt + (E240 * ss + E140) * ss + E040) * tt +
           ((E330 * ss + E230) * ss + E130) * ss + E030) * tt +
          (((E420 * ss + E320) * ss + E220) * ss + E120) * ss + E020) * tt +
@@ -197,19 +208,74 @@ __device__ __forceinline__ double rho_teos10(double CT, double SA, double z) {
   return ((rz3 * zh + rz2) * zh + rz1) * zh + rz0 + r0;
 }
 
-// b = -g rho'/rho0 for the configured EOS (libs/core/eos.py:103-133).
+// b = -g rho'/rho0 for the configured EOS (libs/core/eos.py:103-133), in
+// the arithmetic type F (double: the reference's; float: the mixed step).
+template <class F>
+__device__ __forceinline__ F buoyancy_t(const StepParams& p, F T, F S, F z) {
+  F rp;
+  if (p.eos == EOS_LINEAR) {
+    rp = F(p.rho0) * (F(p.neg_alpha_T) * (T - F(p.T0)) + F(p.beta_S) * (S - F(p.S0)));
+  } else if (p.eos == EOS_SEOS) {
+    const F a0 = F(1.6550e-1), b0 = F(7.6554e-1), lam1 = F(5.9520e-2), lam2 = F(7.4914e-4),
+            mu1 = F(1.4970e-4), mu2 = F(1.1090e-5), nu_ts = F(2.4341e-3);
+    const F Ta = T - F(10), Sa = S - F(35);
+    rp = (-a0 * (F(1) + F(0.5) * lam1 * Ta + mu1 * z) * Ta +
+          b0 * (F(1) - F(0.5) * lam2 * Sa - mu2 * z) * Sa - nu_ts * Ta * Sa);
+  } else {
+    rp = rho_teos10<F>(T, S, z) - F(p.rho0);
+  }
+  return F(p.neg_g) * rp / F(p.rho0);
+}
 __device__ __forceinline__ double buoyancy(const StepParams& p, double T, double S, double z) {
+  return buoyancy_t<double>(p, T, S, z);
+}
+
+// Buoyancy of the mixed-precision step. The T/S-dependent part of the EOS is
+// evaluated in fp32 on anomalies formed in fp64, so that the large constant
+// part of the density never passes through fp32 (rho ~ 1027 against
+// rho' ~ 1 kg m-3): linear and SEOS forms on T - T0, S - S0 (T - 10,
+// S - 35), polyTEOS10 as its polynomial re-expanded about (T0, S0) --
+// rho = rho_ref(z) [fp64] + sum_k zh^k d_k(a, c) [fp32], a = (T - T0) / 40,
+// c = ss - ss_ref = (S - S0) 0.875 / 35.16504 / (ss + ss_ref).
+__device__ __forceinline__ float teos_anomaly_poly(const float* G, int k, int deg, float a,
+                                                   float c) {
+  float acc = 0.0f;
+  for (int pp = deg; pp >= 0; --pp) {
+    float inner = 0.0f;
+    for (int q = deg - pp; q >= 0; --q) inner = inner * c + G[(k * 7 + pp) * 7 + q];
+    acc = acc * a + inner;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double buoyancy_mixed(const StepParams& p, double T, double S,
+                                                 double z) {
   double rp;
   if (p.eos == EOS_LINEAR) {
-    rp = p.rho0 * (p.neg_alpha_T * (T - p.T0) + p.beta_S * (S - p.S0));
+    const float dT = (float)(T - p.T0), dS = (float)(S - p.S0);
+    rp = (double)((float)p.rho0 * ((float)p.neg_alpha_T * dT + (float)p.beta_S * dS));
   } else if (p.eos == EOS_SEOS) {
-    const double a0 = 1.6550e-1, b0 = 7.6554e-1, lam1 = 5.9520e-2, lam2 = 7.4914e-4,
-                 mu1 = 1.4970e-4, mu2 = 1.1090e-5, nu_ts = 2.4341e-3;
-    const double Ta = T - 10.0, Sa = S - 35.0;
-    rp = (-a0 * (1.0 + 0.5 * lam1 * Ta + mu1 * z) * Ta +
-          b0 * (1.0 - 0.5 * lam2 * Sa - mu2 * z) * Sa - nu_ts * Ta * Sa);
+    const float a0 = 1.6550e-1f, b0 = 7.6554e-1f, lam1 = 5.9520e-2f, lam2 = 7.4914e-4f,
+                mu1 = 1.4970e-4f, mu2 = 1.1090e-5f, nu_ts = 2.4341e-3f;
+    const float Ta = (float)(T - 10.0), Sa = (float)(S - 35.0), zf = (float)z;
+    rp = (double)(-a0 * (1.0f + 0.5f * lam1 * Ta + mu1 * zf) * Ta +
+                  b0 * (1.0f - 0.5f * lam2 * Sa - mu2 * zf) * Sa - nu_ts * Ta * Sa);
   } else {
-    rp = rho_teos10(T, S, z) - p.rho0;
+    const double R00 = 4.6494977072e+01, R01 = -5.2099962525e+00, R02 = 2.2601900708e-01,
+                 R03 = 6.4326772569e-02, R04 = 1.5616995503e-02, R05 = -1.7243708991e-03;
+    const double zh = z * 1.0e-4;
+    const double r0 = (((((R05 * zh + R04) * zh + R03) * zh + R02) * zh + R01) * zh + R00) * zh;
+    const double ref = ((p.eos_ref[3] * zh + p.eos_ref[2]) * zh + p.eos_ref[1]) * zh +
+                       p.eos_ref[0] + r0;
+    const float a = (float)(T - p.T0) * (1.0f / 40.0f);
+    const float ss = sqrtf((float)((S + 32.0) * (0.875 / 35.16504)));
+    const float c = (float)((S - p.S0) * (0.875 / 35.16504)) / (ss + (float)p.ss_ref);
+    const float zf = (float)zh;
+    const float* G = p.eos_g;
+    const float d = ((teos_anomaly_poly(G, 3, 1, a, c) * zf + teos_anomaly_poly(G, 2, 2, a, c)) *
+                         zf + teos_anomaly_poly(G, 1, 4, a, c)) * zf +
+                    teos_anomaly_poly(G, 0, 6, a, c);
+    rp = (ref - p.rho0) + (double)d;
   }
   return p.neg_g * rp / p.rho0;
 }

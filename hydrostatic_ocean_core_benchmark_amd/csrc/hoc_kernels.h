@@ -120,18 +120,18 @@ __global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double
                                      const double* v, const double* w, double* au,
                                      double* av);
 
-// hoc_tiled.hip
-__global__ void k_tracers_tiled(Dims d, StepParams p, Geo geo, const double* T, const double* S,
-                                const double* b, const double* u, const double* v,
-                                const double* w, const double* Kh, const double* sub,
-                                const double* den, const double* cst, double* T_out,
-                                double* S_out, double* b_out);
-
-__global__ void k_momentum_explicit_tiled(Dims d, StepParams p, Geo geo, const double* u,
-                                          const double* v, const double* b, const double* phi,
-                                          const double* w, const double* Km, double* pgx,
-                                          double* pgy, double* avu, double* avv, double* dzu,
-                                          double* dzv, double* stu, double* stv);
+// hoc_tiled.hip: the LDS-tiled tracer and explicit-momentum passes; mixed
+// selects the fp32 advective fluxes (and EOS), else the fp64 plain step
+void launch_tracers_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
+                          const double* T, const double* S, const double* b, const double* u,
+                          const double* v, const double* w, const double* Kh, const double* sub,
+                          const double* den, const double* cst, double* T_out, double* S_out,
+                          double* b_out);
+void launch_momentum_explicit_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
+                                    const double* u, const double* v, const double* b,
+                                    const double* phi, const double* w, const double* Km,
+                                    double* pgx, double* pgy, double* avu, double* avv,
+                                    double* dzu, double* dzv, double* stu, double* stv);
 
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
@@ -172,14 +172,20 @@ constexpr int PCG_MAX_SUB = 16;     // subtrees per full chunk
 constexpr int PCG_MAX_CHUNKS = 1024;
 
 // One level of the multigrid hierarchy (level 0 is the model grid).
-struct MgLevel {
+// One multigrid level: face coefficients, diagonal and constants in the
+// arithmetic type R (double for the solvers, float for the mixed-precision
+// V-cycle preconditioner); the mask is shared.
+template <class R>
+struct MgLevelT {
   int nx, ny;
   long long n;
-  const double *Ku, *Kv;
+  const R *Ku, *Kv;
   const uint8_t* mask;
-  const double* inv_diag;
-  double coef, cx, cy, dx2, dy2;
+  const R* inv_diag;
+  R coef, cx, cy, dx2, dy2;
 };
+using MgLevel = MgLevelT<double>;
+using MgLevelF = MgLevelT<float>;
 
 __global__ void k_pcg_persistent(PcgArgs a);
 __global__ void k_pcg_rz(PcgArgs a, int into_b);
@@ -198,21 +204,30 @@ __global__ void k_mg_res_parts(PcgArgs a, MgLevel L, const double* x);
 __global__ void k_mg_check(PcgArgs a, int it);
 __global__ void k_mg_start(PcgArgs a);
 constexpr int MG_COARSE_MAX = 1024;      // cells of a level solved in one workgroup
+constexpr int MG_COARSE_LAST_MAX = 4096; // ... also when it is the hierarchy's last level
 constexpr int MG_COARSE_THREADS = 1024;
 constexpr int MG_FUSED_MAX_SWEEPS = 3;   // fused V-cycle legs for 1..3 sweeps per side
 // The coarse end of a V-cycle solved in one workgroup (levels of at most
 // MG_COARSE_MAX cells, all vectors in LDS). Measured at 512 x 512: taking
 // the 64 x 64 level into the workgroup as well (4096 cells) costs 28 us per
 // cycle against 19 us for its two tiled launches plus the 32 x 32 cycle.
+// A last level of up to MG_COARSE_LAST_MAX cells (the 64 x 64 end of a
+// 5-level hierarchy on 1024 x 1024) is one workgroup's 2S sweeps instead of
+// 2S + 1 launches.
 constexpr int MG_CYCLE_LEVELS = 8;
 constexpr size_t MG_CYCLE_LDS = 160 * 1024;
-struct MgCycle {
-  MgLevel lev[MG_CYCLE_LEVELS];
-  int boff[MG_CYCLE_LEVELS], xoff[MG_CYCLE_LEVELS];  // LDS offsets (doubles) of b, x_S
+template <class R>
+struct MgCycleT {
+  MgLevelT<R> lev[MG_CYCLE_LEVELS];
+  int boff[MG_CYCLE_LEVELS], xoff[MG_CYCLE_LEVELS];  // LDS offsets (elements) of b, x_S
   int nl;
 };
-size_t mg_cycle_layout(MgCycle& C);  // fills boff/xoff, returns the LDS bytes
-int mg_cycle_prepare(size_t lds);    // < 0 if the cycle does not fit one workgroup's LDS
+using MgCycle = MgCycleT<double>;
+using MgCycleF = MgCycleT<float>;
+template <class R>
+size_t mg_cycle_layout(MgCycleT<R>& C);  // fills boff/xoff, returns the LDS bytes
+int mg_cycle_prepare(size_t lds);        // < 0 if the fp64 cycle does not fit one workgroup
+int mg_cycle_prepare_f32(size_t lds);    // the same for the fp32 cycles
 void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
                             double* out, const PcgScalars* sc, hipStream_t s);
 void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
@@ -221,7 +236,24 @@ void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgL
 void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
                   const double* xs_in, const double* ec, const double* b, double* out,
                   const PcgScalars* sc);
-dim3 mg_tile_grid(const MgLevel& L);
+// fp32 V-cycle of the mixed-precision preconditioner: level 0 reads the fp64
+// residual and writes the fp64 result, every other vector is fp32
+void launch_mg_coarse_cycle_f32(const MgCycleF& C, size_t lds, float omega, int S,
+                                const double* b0_d, const float* b0_f, double* out_d,
+                                float* out_f, const PcgScalars* sc, hipStream_t s);
+void launch_mg_down_f32(int S, dim3 grid, hipStream_t s, const MgLevelF& L, const MgLevelF& C,
+                        float omega, const double* b_d, const float* b_f, float* xs_out,
+                        float* bc, const PcgScalars* sc);
+void launch_mg_up_f32(int S, dim3 grid, hipStream_t s, const MgLevelF& L, int cnx, float omega,
+                      const float* xs_in, const float* ec, const double* b_d, const float* b_f,
+                      double* out_d, float* out_f, const PcgScalars* sc);
+__global__ void k_mg_to_f32(long long n, const double* Ku, const double* Kv, const double* inv,
+                            float* fKu, float* fKv, float* finv);
+constexpr int MG_TW = 32, MG_TH = 16, MG_THREADS = 256;  // fused-leg tile and workgroup
+template <class R>
+dim3 mg_tile_grid(const MgLevelT<R>& L) {
+  return dim3((unsigned)((L.nx + MG_TW - 1) / MG_TW), (unsigned)((L.ny + MG_TH - 1) / MG_TH));
+}
 __global__ void k_pcg_init(PcgArgs a);
 __global__ void k_pcg_init_z(PcgArgs a);
 __global__ void k_pcg_ap(PcgArgs a, int it);

@@ -11,45 +11,53 @@
 
 namespace hoc {
 
-__device__ __forceinline__ void lvl_ij(const MgLevel& L, long long c, int& i, int& j) {
+template <class R>
+__device__ __forceinline__ void lvl_ij(const MgLevelT<R>& L, long long c, int& i, int& j) {
   i = (int)(c % L.nx);
   j = (int)(c / L.nx);
 }
 
+template <class R>
+__device__ __forceinline__ R mkr(uint8_t m) {
+  return m ? R(1) : R(0);
+}
+
 // A x at c from the values at c and its four neighbours, x masked first
-// (solvers_var.py:47-53); ce/cw/cn/cs are the neighbours' indices.
-__device__ __forceinline__ double apply_vals(const MgLevel& L, long long c, long long ce,
-                                             long long cw, long long cn, long long cs,
-                                             double x0, double x_e, double x_w, double x_n,
-                                             double x_s) {
-  const double mc = mk(L.mask[c]);
-  const double xc = x0 * mc;
-  const double xe = x_e * mk(L.mask[ce]);
-  const double xw = x_w * mk(L.mask[cw]);
-  const double xn = x_n * mk(L.mask[cn]);
-  const double xs = x_s * mk(L.mask[cs]);
-  const double fx = L.Ku[c] * (xe - xc);
-  const double fxw = L.Ku[cw] * (xc - xw);
-  const double fy = L.Kv[c] * (xn - xc);
-  const double fys = L.Kv[cs] * (xc - xs);
-  const double lap = (fx - fxw) / L.dx2 + (fy - fys) / L.dy2;
-  return (xc - L.coef * lap) * mc + xc * (1.0 - mc);
+// (solvers_var.py:47-53); ce/cw/cn/cs are the neighbours' indices. In the
+// level's type R throughout (for R = double the reference's arithmetic).
+template <class R>
+__device__ __forceinline__ R apply_vals(const MgLevelT<R>& L, long long c, long long ce,
+                                        long long cw, long long cn, long long cs, R x0, R x_e,
+                                        R x_w, R x_n, R x_s) {
+  const R mc = mkr<R>(L.mask[c]);
+  const R xc = x0 * mc;
+  const R xe = x_e * mkr<R>(L.mask[ce]);
+  const R xw = x_w * mkr<R>(L.mask[cw]);
+  const R xn = x_n * mkr<R>(L.mask[cn]);
+  const R xs = x_s * mkr<R>(L.mask[cs]);
+  const R fx = L.Ku[c] * (xe - xc);
+  const R fxw = L.Ku[cw] * (xc - xw);
+  const R fy = L.Kv[c] * (xn - xc);
+  const R fys = L.Kv[cs] * (xc - xs);
+  const R lap = (fx - fxw) / L.dx2 + (fy - fys) / L.dy2;
+  return (xc - L.coef * lap) * mc + xc * (R(1) - mc);
 }
 
 // Damped Jacobi at c: x + omega (inv_diag (b - A x)) mask.
-__device__ __forceinline__ double jacobi_vals(const MgLevel& L, double omega, long long c,
-                                              long long ce, long long cw, long long cn,
-                                              long long cs, double x0, double x_e, double x_w,
-                                              double x_n, double x_s, double b) {
-  const double ax = apply_vals(L, c, ce, cw, cn, cs, x0, x_e, x_w, x_n, x_s);
-  return x0 + omega * (L.inv_diag[c] * (b - ax)) * mk(L.mask[c]);
+template <class R>
+__device__ __forceinline__ R jacobi_vals(const MgLevelT<R>& L, R omega, long long c, long long ce,
+                                         long long cw, long long cn, long long cs, R x0, R x_e,
+                                         R x_w, R x_n, R x_s, R b) {
+  const R ax = apply_vals<R>(L, c, ce, cw, cn, cs, x0, x_e, x_w, x_n, x_s);
+  return x0 + omega * (L.inv_diag[c] * (b - ax)) * mkr<R>(L.mask[c]);
 }
 
 struct Nbr {
   long long c, ce, cw, cn, cs;
 };
 
-__device__ __forceinline__ Nbr nbr_of(const MgLevel& L, int i, int j) {
+template <class R>
+__device__ __forceinline__ Nbr nbr_of(const MgLevelT<R>& L, int i, int j) {
   const long long row = (long long)j * L.nx;
   return Nbr{row + i, row + wrap(i, 1, L.nx), row + wrap(i, -1, L.nx),
              (long long)wrap(j, 1, L.ny) * L.nx + i, (long long)wrap(j, -1, L.ny) * L.nx + i};
@@ -231,39 +239,42 @@ __global__ __launch_bounds__(PCG_WG) void k_mg_start(PcgArgs a) {
 // neighbours wrapping like the operator), so one launch replaces the
 // zero / S sweeps / residual-restrict launches of the way down and the
 // prolong / S sweeps launches of the way up, with the same arithmetic per
-// cell (apply_vals, jacobi_vals).
-constexpr int MG_TW = 32, MG_TH = 16, MG_THREADS = 256;
+// cell (apply_vals, jacobi_vals). R is the arithmetic and storage type of
+// the level's vectors; RB the type of the right-hand side read (double on
+// level 0 of the fp32 cycle, where it is the fp64 PCG residual), RO the
+// type of the way-up result (double on level 0 of the fp32 cycle).
 
 // Way down on one level: x_S = S sweeps from zero (written out for the way
 // up) and bc = restrict((b - A x_S) mask).
-template <int S>
-__global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevel L, MgLevel C, double omega,
-                                                        const double* __restrict__ b,
-                                                        double* __restrict__ xs_out,
-                                                        double* __restrict__ bc,
+template <int S, class R, class RB>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevelT<R> L, MgLevelT<R> C, R omega,
+                                                        const RB* __restrict__ b,
+                                                        R* __restrict__ xs_out,
+                                                        R* __restrict__ bc,
                                                         const PcgScalars* sc) {
   if (finished(sc)) return;
   constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
-  __shared__ double buf[2][RH * RW];
+  __shared__ R buf[2][RH * RW];
   const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
   for (int k = 1; k <= S; ++k) {
     const int h = S - k + 1;  // halo of x_k
     const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
-    double* dst = buf[(k - 1) & 1];
-    const double* src = buf[k & 1];
+    R* dst = buf[(k - 1) & 1];
+    const R* src = buf[k & 1];
     for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
       const int li = S - h + t % w, lj = S - h + t / w;
       const Nbr n = nbr_of(L, mg_pmod(i0 - S + li, L.nx), mg_pmod(j0 - S + lj, L.ny));
       const int q = lj * RW + li;
+      const R bb = (R)b[n.c];
       dst[q] = (k == 1)
-                   ? jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, 0.0, 0.0, 0.0, 0.0, 0.0,
-                                 b[n.c])
-                   : jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1],
-                                 src[q - 1], src[q + RW], src[q - RW], b[n.c]);
+                   ? jacobi_vals<R>(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, R(0), R(0), R(0), R(0),
+                                    R(0), bb)
+                   : jacobi_vals<R>(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1],
+                                    src[q - 1], src[q + RW], src[q - RW], bb);
     }
     __syncthreads();
   }
-  const double* xs = buf[(S - 1) & 1];
+  const R* xs = buf[(S - 1) & 1];
   for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
     const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
     if (gi < L.nx && gj < L.ny)
@@ -273,133 +284,134 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevel L, MgLevel C, do
   for (int t = threadIdx.x; t < CW * CH; t += MG_THREADS) {
     const int I = i0 / 2 + t % CW, J = j0 / 2 + t / CW;
     if (I >= C.nx || J >= C.ny) continue;
-    double r[4];
+    R r[4];
     for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
       const int a = qd & 1, bq = qd >> 1;
       const int gi = 2 * I + a, gj = 2 * J + bq;
       const Nbr n = nbr_of(L, gi, gj);
       const int q = (S + gj - j0) * RW + S + gi - i0;
-      const double ax = apply_vals(L, n.c, n.ce, n.cw, n.cn, n.cs, xs[q], xs[q + 1], xs[q - 1],
-                                   xs[q + RW], xs[q - RW]);
-      r[qd] = (b[n.c] - ax) * mk(L.mask[n.c]);
+      const R ax = apply_vals<R>(L, n.c, n.ce, n.cw, n.cn, n.cs, xs[q], xs[q + 1], xs[q - 1],
+                                 xs[q + RW], xs[q - RW]);
+      r[qd] = ((R)b[n.c] - ax) * mkr<R>(L.mask[n.c]);
     }
-    bc[(long long)J * C.nx + I] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
+    bc[(long long)J * C.nx + I] = R(0.25) * (r[0] + r[1] + r[2] + r[3]);
   }
 }
 
 // Way up on one level: x = x_S + prolong(ec) mask, then S sweeps; out = x.
-template <int S>
-__global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevel L, int cnx, double omega,
-                                                      const double* __restrict__ xs_in,
-                                                      const double* __restrict__ ec,
-                                                      const double* __restrict__ b,
-                                                      double* __restrict__ out,
+template <int S, class R, class RB, class RO>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R omega,
+                                                      const R* __restrict__ xs_in,
+                                                      const R* __restrict__ ec,
+                                                      const RB* __restrict__ b,
+                                                      RO* __restrict__ out,
                                                       const PcgScalars* sc) {
   if (finished(sc)) return;
   constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
-  __shared__ double buf[2][RH * RW];
+  __shared__ R buf[2][RH * RW];
   const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
   for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) {
     const int gi = mg_pmod(i0 - S + t % RW, L.nx), gj = mg_pmod(j0 - S + t / RW, L.ny);
     const long long c = (long long)gj * L.nx + gi;
-    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * mk(L.mask[c]);
+    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * mkr<R>(L.mask[c]);
   }
   __syncthreads();
   for (int k = 1; k <= S; ++k) {
     const int h = S - k;
     const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
-    double* dst = buf[k & 1];
-    const double* src = buf[(k - 1) & 1];
+    R* dst = buf[k & 1];
+    const R* src = buf[(k - 1) & 1];
     for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
       const int li = S - h + t % w, lj = S - h + t / w;
       const Nbr n = nbr_of(L, mg_pmod(i0 - S + li, L.nx), mg_pmod(j0 - S + lj, L.ny));
       const int q = lj * RW + li;
-      dst[q] = jacobi_vals(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1], src[q - 1],
-                           src[q + RW], src[q - RW], b[n.c]);
+      dst[q] = jacobi_vals<R>(L, omega, n.c, n.ce, n.cw, n.cn, n.cs, src[q], src[q + 1],
+                              src[q - 1], src[q + RW], src[q - RW], (R)b[n.c]);
     }
     __syncthreads();
   }
-  const double* x = buf[S & 1];
+  const R* x = buf[S & 1];
   for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
     const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
-    if (gi < L.nx && gj < L.ny) out[(long long)gj * L.nx + gi] = x[(S + t / MG_TW) * RW + S + t % MG_TW];
+    if (gi < L.nx && gj < L.ny)
+      out[(long long)gj * L.nx + gi] = (RO)x[(S + t / MG_TW) * RW + S + t % MG_TW];
   }
 }
 
 // The coarse end of the pcg_mg V-cycle in one workgroup, every vector in
-// LDS: levels C.lev[0..nl) (the first with at most MG_COARSE_MAX cells),
-// S sweeps from zero + residual restriction down, 2S sweeps from zero on the
-// last level, prolongation + S sweeps up; the same arithmetic per cell as
-// the tiled legs (jacobi_vals, apply_vals). b0 is level 0's right-hand side,
-// out receives level 0's result.
+// LDS: levels C.lev[0..nl) (the first with at most MG_COARSE_MAX cells, or
+// the last level alone), S sweeps from zero + residual restriction down, 2S
+// sweeps from zero on the last level, prolongation + S sweeps up; the same
+// arithmetic per cell as the tiled legs (jacobi_vals, apply_vals). b0 is
+// level 0's right-hand side (staged in LDS), out receives level 0's result.
+template <class R, class RB, class RO>
 __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
-    MgCycle C, double omega, int S, const double* __restrict__ b0, double* __restrict__ out,
+    MgCycleT<R> C, R omega, int S, const RB* __restrict__ b0, RO* __restrict__ out,
     const PcgScalars* sc) {
   if (finished(sc)) return;
-  extern __shared__ double cyc[];
+  extern __shared__ __align__(16) unsigned char cyc_raw[];
+  R* cyc = reinterpret_cast<R*>(cyc_raw);
   const int n0 = (int)C.lev[0].n;
-  double* wA = cyc;
-  double* wB = cyc + n0;
+  R* wA = cyc;
+  R* wB = cyc + n0;
   const int tid = threadIdx.x;
   // one damped-Jacobi sweep on level L from src (zero if null) into dst
-  auto sweep = [&](const MgLevel& L, const double* b, const double* src, double* dst) {
+  auto sweep = [&](const MgLevelT<R>& L, const R* b, const R* src, R* dst) {
     const int n = (int)L.n;
     for (int c = tid; c < n; c += MG_COARSE_THREADS) {
       const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
-      dst[c] = src ? jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c], src[nb.ce],
-                                 src[nb.cw], src[nb.cn], src[nb.cs], b[c])
-                   : jacobi_vals(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, 0.0, 0.0, 0.0, 0.0,
-                                 0.0, b[c]);
+      dst[c] = src ? jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c], src[nb.ce],
+                                    src[nb.cw], src[nb.cn], src[nb.cs], b[c])
+                   : jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, R(0), R(0), R(0),
+                                    R(0), R(0), b[c]);
     }
     __syncthreads();
   };
-  // level 0's right-hand side: read in place when it is the only level,
-  // else staged (the restriction of the levels below overwrites nothing of it)
-  const double* B0 = b0;
-  if (C.nl > 1) {
-    double* st = cyc + C.boff[0];
-    for (int c = tid; c < n0; c += MG_COARSE_THREADS) st[c] = b0[c];
+  // level 0's right-hand side, staged (the restriction of the levels below
+  // overwrites nothing of it)
+  {
+    R* st = cyc + C.boff[0];
+    for (int c = tid; c < n0; c += MG_COARSE_THREADS) st[c] = (R)b0[c];
     __syncthreads();
-    B0 = st;
   }
-  auto level_b = [&](int l) -> const double* { return l == 0 ? B0 : cyc + C.boff[l]; };
+  auto level_b = [&](int l) -> const R* { return cyc + C.boff[l]; };
   // down: x_S into XS[l], restricted residual into B[l + 1]
   for (int l = 0; l + 1 < C.nl; ++l) {
-    const MgLevel& L = C.lev[l];
-    const double* b = level_b(l);
-    double* xs = cyc + C.xoff[l];
-    const double* src = nullptr;
+    const MgLevelT<R>& L = C.lev[l];
+    const R* b = level_b(l);
+    R* xs = cyc + C.xoff[l];
+    const R* src = nullptr;
     for (int k = 1; k <= S; ++k) {
-      double* dst = (k == S) ? xs : ((k & 1) ? wA : wB);
+      R* dst = (k == S) ? xs : ((k & 1) ? wA : wB);
       sweep(L, b, src, dst);
       src = dst;
     }
-    const MgLevel& Cl = C.lev[l + 1];
-    double* bc = cyc + C.boff[l + 1];
+    const MgLevelT<R>& Cl = C.lev[l + 1];
+    R* bc = cyc + C.boff[l + 1];
     const int nc = (int)Cl.n;
     for (int t = tid; t < nc; t += MG_COARSE_THREADS) {
       const int I = t % Cl.nx, J = t / Cl.nx;
-      double r[4];
+      R r[4];
       for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
         const int gi = 2 * I + (qd & 1), gj = 2 * J + (qd >> 1);
         const Nbr nb = nbr_of(L, gi, gj);
-        const double ax = apply_vals(L, nb.c, nb.ce, nb.cw, nb.cn, nb.cs, xs[nb.c], xs[nb.ce],
-                                     xs[nb.cw], xs[nb.cn], xs[nb.cs]);
-        r[qd] = (b[nb.c] - ax) * mk(L.mask[nb.c]);
+        const R ax = apply_vals<R>(L, nb.c, nb.ce, nb.cw, nb.cn, nb.cs, xs[nb.c], xs[nb.ce],
+                                   xs[nb.cw], xs[nb.cn], xs[nb.cs]);
+        r[qd] = (b[nb.c] - ax) * mkr<R>(L.mask[nb.c]);
       }
-      bc[t] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
+      bc[t] = R(0.25) * (r[0] + r[1] + r[2] + r[3]);
     }
     __syncthreads();
   }
   // the last level: 2S sweeps from zero
-  double* e = nullptr;
+  R* e = nullptr;
   {
     const int l = C.nl - 1;
-    const MgLevel& L = C.lev[l];
-    const double* b = level_b(l);
-    const double* src = nullptr;
+    const MgLevelT<R>& L = C.lev[l];
+    const R* b = level_b(l);
+    const R* src = nullptr;
     for (int k = 1; k <= 2 * S; ++k) {
-      double* dst = (k & 1) ? wA : wB;
+      R* dst = (k & 1) ? wA : wB;
       sweep(L, b, src, dst);
       src = dst;
       e = dst;
@@ -407,30 +419,31 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   }
   // up: x = XS[l] + prolong(e) mask, then S sweeps
   for (int l = C.nl - 2; l >= 0; --l) {
-    const MgLevel& L = C.lev[l];
-    const double* b = level_b(l);
-    const double* xs = cyc + C.xoff[l];
+    const MgLevelT<R>& L = C.lev[l];
+    const R* b = level_b(l);
+    const R* xs = cyc + C.xoff[l];
     const int cnx = C.lev[l + 1].nx;
-    double* xp = (e == wA) ? wB : wA;
-    double* other = (e == wA) ? wA : wB;
+    R* xp = (e == wA) ? wB : wA;
+    R* other = (e == wA) ? wA : wB;
     const int n = (int)L.n;
     for (int c = tid; c < n; c += MG_COARSE_THREADS) {
       const int i = c % L.nx, j = c / L.nx;
-      xp[c] = xs[c] + e[(j / 2) * cnx + i / 2] * mk(L.mask[c]);
+      xp[c] = xs[c] + e[(j / 2) * cnx + i / 2] * mkr<R>(L.mask[c]);
     }
     __syncthreads();
-    const double* src = xp;
+    const R* src = xp;
     for (int k = 1; k <= S; ++k) {
-      double* dst = (k & 1) ? other : xp;
+      R* dst = (k & 1) ? other : xp;
       sweep(L, b, src, dst);
       src = dst;
       e = dst;
     }
   }
-  for (int c = tid; c < n0; c += MG_COARSE_THREADS) out[c] = e[c];
+  for (int c = tid; c < n0; c += MG_COARSE_THREADS) out[c] = (RO)e[c];
 }
 
-size_t mg_cycle_layout(MgCycle& C) {
+template <class R>
+size_t mg_cycle_layout(MgCycleT<R>& C) {
   const int n0 = (int)C.lev[0].n;
   int off = 2 * n0;
   for (int l = 0; l < C.nl; ++l) {
@@ -439,40 +452,122 @@ size_t mg_cycle_layout(MgCycle& C) {
     C.xoff[l] = off;
     off += (int)C.lev[l].n;
   }
-  return (size_t)off * sizeof(double);
+  return (size_t)off * sizeof(R);
 }
+template size_t mg_cycle_layout<double>(MgCycle&);
+template size_t mg_cycle_layout<float>(MgCycleF&);
 
-int mg_cycle_prepare(size_t lds) {
+namespace {
+int set_cycle_lds(const void* fn, size_t lds) {
   if (lds > MG_CYCLE_LDS) return -1;
-  return hipFuncSetAttribute((const void*)k_mg_coarse_cycle,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                 hipSuccess
              ? 0
              : -2;
+}
+}  // namespace
+
+int mg_cycle_prepare(size_t lds) {
+  return set_cycle_lds((const void*)k_mg_coarse_cycle<double, double, double>, lds);
+}
+
+int mg_cycle_prepare_f32(size_t lds) {
+  int rc = set_cycle_lds((const void*)k_mg_coarse_cycle<float, double, double>, lds);
+  return rc ? rc : set_cycle_lds((const void*)k_mg_coarse_cycle<float, float, float>, lds);
 }
 
 void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
                             double* out, const PcgScalars* sc, hipStream_t s) {
-  k_mg_coarse_cycle<<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0, out, sc);
+  k_mg_coarse_cycle<double, double, double>
+      <<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0, out, sc);
+}
+
+void launch_mg_coarse_cycle_f32(const MgCycleF& C, size_t lds, float omega, int S,
+                                const double* b0_d, const float* b0_f, double* out_d,
+                                float* out_f, const PcgScalars* sc, hipStream_t s) {
+  if (b0_d)
+    k_mg_coarse_cycle<float, double, double>
+        <<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0_d, out_d, sc);
+  else
+    k_mg_coarse_cycle<float, float, float>
+        <<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0_f, out_f, sc);
 }
 
 void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
                     double omega, const double* b, double* xs_out, double* bc,
                     const PcgScalars* sc) {
-  if (S == 1) k_mg_down<1><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
-  else if (S == 2) k_mg_down<2><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
-  else k_mg_down<3><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  if (S == 1)
+    k_mg_down<1, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else if (S == 2)
+    k_mg_down<2, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else
+    k_mg_down<3, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
 }
 
 void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
                   const double* xs_in, const double* ec, const double* b, double* out,
                   const PcgScalars* sc) {
-  if (S == 1) k_mg_up<1><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
-  else if (S == 2) k_mg_up<2><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
-  else k_mg_up<3><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b, out, sc);
+  if (S == 1)
+    k_mg_up<1, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
+                                                                   out, sc);
+  else if (S == 2)
+    k_mg_up<2, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
+                                                                   out, sc);
+  else
+    k_mg_up<3, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
+                                                                   out, sc);
 }
 
-dim3 mg_tile_grid(const MgLevel& L) {
-  return dim3((unsigned)((L.nx + MG_TW - 1) / MG_TW), (unsigned)((L.ny + MG_TH - 1) / MG_TH));
+namespace {
+template <int S>
+void down_f32(dim3 grid, hipStream_t s, const MgLevelF& L, const MgLevelF& C, float omega,
+              const double* b_d, const float* b_f, float* xs_out, float* bc,
+              const PcgScalars* sc) {
+  if (b_d)
+    k_mg_down<S, float, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b_d, xs_out, bc, sc);
+  else
+    k_mg_down<S, float, float><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b_f, xs_out, bc, sc);
+}
+template <int S>
+void up_f32(dim3 grid, hipStream_t s, const MgLevelF& L, int cnx, float omega, const float* xs_in,
+            const float* ec, const double* b_d, const float* b_f, double* out_d, float* out_f,
+            const PcgScalars* sc) {
+  if (b_d)
+    k_mg_up<S, float, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b_d,
+                                                                  out_d, sc);
+  else
+    k_mg_up<S, float, float, float><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b_f,
+                                                                out_f, sc);
+}
+}  // namespace
+
+void launch_mg_down_f32(int S, dim3 grid, hipStream_t s, const MgLevelF& L, const MgLevelF& C,
+                        float omega, const double* b_d, const float* b_f, float* xs_out,
+                        float* bc, const PcgScalars* sc) {
+  if (S == 1) down_f32<1>(grid, s, L, C, omega, b_d, b_f, xs_out, bc, sc);
+  else if (S == 2) down_f32<2>(grid, s, L, C, omega, b_d, b_f, xs_out, bc, sc);
+  else down_f32<3>(grid, s, L, C, omega, b_d, b_f, xs_out, bc, sc);
+}
+
+void launch_mg_up_f32(int S, dim3 grid, hipStream_t s, const MgLevelF& L, int cnx, float omega,
+                      const float* xs_in, const float* ec, const double* b_d, const float* b_f,
+                      double* out_d, float* out_f, const PcgScalars* sc) {
+  if (S == 1) up_f32<1>(grid, s, L, cnx, omega, xs_in, ec, b_d, b_f, out_d, out_f, sc);
+  else if (S == 2) up_f32<2>(grid, s, L, cnx, omega, xs_in, ec, b_d, b_f, out_d, out_f, sc);
+  else up_f32<3>(grid, s, L, cnx, omega, xs_in, ec, b_d, b_f, out_d, out_f, sc);
+}
+
+// fp32 copies of one level's face coefficients and inverse diagonal
+// (round to nearest).
+__global__ void k_mg_to_f32(long long n, const double* __restrict__ Ku,
+                            const double* __restrict__ Kv, const double* __restrict__ inv,
+                            float* __restrict__ fKu, float* __restrict__ fKv,
+                            float* __restrict__ finv) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  fKu[c] = (float)Ku[c];
+  fKv[c] = (float)Kv[c];
+  finv[c] = (float)inv[c];
 }
 
 }  // namespace hoc

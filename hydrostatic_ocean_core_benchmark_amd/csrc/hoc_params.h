@@ -47,6 +47,12 @@ struct StepParams {
   double neg_alpha_T, neg_g, neg_f, neg_N2;
   double ddt;           // dt / n_split (split_explicit)
   double eta_ext, u_ext; // Flather external state
+  // mixed-precision EOS (anomaly form about (T0, S0)): eos_ref[k] are the
+  // polyTEOS10 z-power-k coefficients at the reference point (fp64), eos_g
+  // the device table [4][7][7] of the T/S anomaly polynomial (fp32)
+  double eos_ref[4];
+  double ss_ref;          // sqrt((S0 + 32) * 0.875 / 35.16504)
+  const float* eos_g;
 };
 
 // Static geometry and forcing on the device (set once per stepper).

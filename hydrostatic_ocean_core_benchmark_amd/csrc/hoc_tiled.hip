@@ -19,35 +19,39 @@ __device__ __forceinline__ int pmod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
-// Tracer face value for the configured scheme (model3d_v05.py:486-494).
-__device__ __forceinline__ double tracer_face(int adv, double cm, double c0, double c1, double c2,
-                                              double vel, bool wm, bool w2) {
+// Tracer face value for the configured scheme (model3d_v05.py:486-494), in
+// the flux type R (double: the reference's arithmetic; float: the mixed step).
+template <class R>
+__device__ __forceinline__ R tracer_face(int adv, R cm, R c0, R c1, R c2, R vel, bool wm,
+                                         bool w2) {
   if (adv == ADV_UP3 || adv == ADV_UP3_TVD)
-    return kappa_face(cm, c0, c1, c2, vel, wm, w2, adv == ADV_UP3_TVD);
-  if (adv == ADV_CENTRED2) return 0.5 * (c0 + c1);
-  return vel > 0.0 ? c0 : c1;
+    return kappa_face<R>(cm, c0, c1, c2, vel, wm, w2, adv == ADV_UP3_TVD);
+  if (adv == ADV_CENTRED2) return R(0.5) * (c0 + c1);
+  return vel > R(0) ? c0 : c1;
 }
 
 // Vertical interface value at interface kk of a column whose window holds
 // c[kk-2], c[kk-1], c[kk], c[kk+1] and the wet flags of c[kk-2], c[kk+1].
-__device__ __forceinline__ double tracer_face_z(int adv, int kk, int nz, double far_up,
-                                                double above, double below, double far_dn,
-                                                bool mup, bool mdn, double w) {
-  if (kk == nz) return 0.0;
-  if (!(adv == ADV_UP3 || adv == ADV_UP3_TVD)) return 0.5 * (above + below);
+template <class R>
+__device__ __forceinline__ R tracer_face_z(int adv, int kk, int nz, R far_up, R above, R below,
+                                           R far_dn, bool mup, bool mdn, R w) {
+  if (kk == nz) return R(0);
+  if (!(adv == ADV_UP3 || adv == ADV_UP3_TVD)) return R(0.5) * (above + below);
   const bool hu = kk >= 2 && mup;
   const bool hd = kk <= nz - 2 && mdn;
-  return kappa_face_z(kk >= 2 ? far_up : 0.0, above, below, kk <= nz - 2 ? far_dn : 0.0, w, hu,
-                      hd, adv == ADV_UP3_TVD);
+  return kappa_face_z<R>(kk >= 2 ? far_up : R(0), above, below, kk <= nz - 2 ? far_dn : R(0), w,
+                         hu, hd, adv == ADV_UP3_TVD);
 }
 
+// Staged tile of one level in the flux type R.
+template <class R>
 struct TileLds {
-  double c[2][SY][SX];      // the tracers (T, S) or b, with halo
+  R c[2][SY][SX];           // the tracers (T, S) or b, with halo
   uint8_t m[SY][SX];        // mask3 with halo
-  double u[TY][TX + 1];     // u at i-1 .. i
-  double v[TY + 1][TX];     // v at j-1 .. j
-  double du[TY][TX + 1];    // dz3u
-  double dv[TY + 1][TX];    // dz3v
+  R u[TY][TX + 1];          // u at i-1 .. i
+  R v[TY + 1][TX];          // v at j-1 .. j
+  R du[TY][TX + 1];         // dz3u
+  R dv[TY + 1][TX];         // dz3v
   uint8_t mu[TY][TX + 1];   // mask3u
   uint8_t mv[TY + 1][TX];   // mask3v
 };
@@ -56,9 +60,10 @@ struct TileLds {
 // dz_f u_f c_f and K_h gradient mask_f (c_1 - c_0) / dx on the x-faces
 // (index q: between tile columns q-1 and q) and y-faces (between rows q-1
 // and q), for both tracers.
+template <class R>
 struct FaceLds {
-  double fx[2][TY][TX + 1], gx[2][TY][TX + 1];
-  double fy[2][TY + 1][TX], gy[2][TY + 1][TX];
+  R fx[2][TY][TX + 1], gx[2][TY][TX + 1];
+  R fy[2][TY + 1][TX], gy[2][TY + 1][TX];
 };
 
 // Level-k staging of one tile held in registers between its global loads
@@ -68,12 +73,13 @@ constexpr int ST_C = (SX * SY + TILE_THREADS - 1) / TILE_THREADS;        // trac
 constexpr int ST_U = (TY * (TX + 1) + TILE_THREADS - 1) / TILE_THREADS;  // u-face slots
 constexpr int ST_V = ((TY + 1) * TX + TILE_THREADS - 1) / TILE_THREADS;  // v-face slots
 
+template <class R>
 struct TileRegs {
-  double c0[ST_C], c1[ST_C];
+  R c0[ST_C], c1[ST_C];
   uint8_t m[ST_C];
-  double u[ST_U], du[ST_U];
+  R u[ST_U], du[ST_U];
   uint8_t mu[ST_U];
-  double v[ST_V], dv[ST_V];
+  R v[ST_V], dv[ST_V];
   uint8_t mv[ST_V];
 };
 
@@ -103,15 +109,16 @@ __device__ __forceinline__ TileOffs tile_offsets(int tid, int i0, int j0, int nx
   return F;
 }
 
-__device__ __forceinline__ void tile_load(TileRegs& R, const TileOffs& F, long long o, bool ts,
+template <class RT>
+__device__ __forceinline__ void tile_load(TileRegs<RT>& R, const TileOffs& F, long long o, bool ts,
                                           const double* A, const double* S, const double* u,
                                           const double* v, const Geo& geo) {
 #pragma unroll
   for (int q = 0; q < ST_C; ++q) {
     if (F.c[q] >= 0) {
       const long long g = o + F.c[q];
-      R.c0[q] = A[g];
-      R.c1[q] = ts ? S[g] : 0.0;
+      R.c0[q] = (RT)A[g];
+      R.c1[q] = ts ? (RT)S[g] : RT(0);
       R.m[q] = geo.m3[g];
     }
   }
@@ -119,8 +126,8 @@ __device__ __forceinline__ void tile_load(TileRegs& R, const TileOffs& F, long l
   for (int q = 0; q < ST_U; ++q) {
     if (F.u[q] >= 0) {
       const long long g = o + F.u[q];
-      R.u[q] = u[g];
-      R.du[q] = geo.dz3u[g];
+      R.u[q] = (RT)u[g];
+      R.du[q] = (RT)geo.dz3u[g];
       R.mu[q] = geo.m3u[g];
     }
   }
@@ -128,14 +135,15 @@ __device__ __forceinline__ void tile_load(TileRegs& R, const TileOffs& F, long l
   for (int q = 0; q < ST_V; ++q) {
     if (F.v[q] >= 0) {
       const long long g = o + F.v[q];
-      R.v[q] = v[g];
-      R.dv[q] = geo.dz3v[g];
+      R.v[q] = (RT)v[g];
+      R.dv[q] = (RT)geo.dz3v[g];
       R.mv[q] = geo.m3v[g];
     }
   }
 }
 
-__device__ __forceinline__ void tile_store(TileLds& L, const TileRegs& R, int tid) {
+template <class RT>
+__device__ __forceinline__ void tile_store(TileLds<RT>& L, const TileRegs<RT>& R, int tid) {
 #pragma unroll
   for (int q = 0; q < ST_C; ++q) {
     const int t = tid + q * TILE_THREADS;
@@ -171,15 +179,23 @@ __device__ __forceinline__ void tile_store(TileLds& L, const TileRegs& R, int ti
 // Both tracers of the tile's columns: advection + K_h Laplacian + explicit
 // part of the vertical diffusion + surface flux, the forward sweep on tri_b
 // (d* parked in the outputs), then the back-substitution and the EOS
-// (libs/core/model3d_v05.py:466-533).
-__global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
+// (libs/core/model3d_v05.py:466-533). R is the type of the horizontal and
+// vertical advective fluxes, the K_h gradients and the EOS: double for the
+// plain step (bitwise the reference), float for the mixed step; the column
+// update, the vertical diffusion and the tridiagonal solve stay fp64.
+// The fp32-flux instantiation runs at 4 waves per SIMD (128 VGPRs): measured
+// at 1024 x 1024 x 60, 3.65 ms per step against 4.19 ms at the compiler's
+// default of 3 waves.
+template <class R>
+__global__ __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tiled(
     Dims d, StepParams p, Geo geo, const double* __restrict__ T, const double* __restrict__ S,
     const double* __restrict__ b, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ w, const double* __restrict__ Kh, const double* __restrict__ sub,
     const double* __restrict__ den, const double* __restrict__ cst, double* __restrict__ T_out,
     double* __restrict__ S_out, double* __restrict__ b_out) {
-  __shared__ TileLds LB[2];
-  __shared__ FaceLds F;
+  __shared__ TileLds<R> LB[2];
+  __shared__ FaceLds<R> F;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
@@ -218,48 +234,50 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
   double w_k = active ? w[c] : 0.0;
   double kh_k = active ? Kh[c] : 0.0;
   double dzi_k = active ? geo.dzi[c] : 0.0;
-  double cf_top[2] = {0.0, 0.0};  // interface value at the top of level k
+  R cf_top[2] = {R(0), R(0)};  // interface value at the top of level k
   double dprev[2] = {0.0, 0.0};
 
-  TileRegs R;
+  TileRegs<R> RG;
   const TileOffs TO = tile_offsets(tid, i0, j0, nx, ny);
-  tile_load(R, TO, 0, ts, A, S, u, v, geo);
-  tile_store(LB[0], R, tid);
+  tile_load(RG, TO, 0, ts, A, S, u, v, geo);
+  tile_store(LB[0], RG, tid);
   __syncthreads();
   const bool has_adv = adv != ADV_NONE;
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
-    const TileLds& L = LB[k & 1];
-    if (k + 1 < nz) tile_load(R, TO, o + n2, ts, A, S, u, v, geo);
+    const TileLds<R>& L = LB[k & 1];
+    if (k + 1 < nz) tile_load(RG, TO, o + n2, ts, A, S, u, v, geo);
     // ---- faces: the east and north face of every cell, plus the tile's
     // west column (8) and south row (32) of faces by threads 0..39
     {
       auto xface = [&](int fy_, int q) {
         const int cy = fy_ + HALO, c0x = q - 1 + HALO;
-        const double vel = L.u[fy_][q];
-        const double ddu = L.du[fy_][q];
-        const double mmu = mk(L.mu[fy_][q]);
+        const R vel = L.u[fy_][q];
+        const R ddu = L.du[fy_][q];
+        const R mmu = R(mk(L.mu[fy_][q]));
         const bool wm = L.m[cy][c0x - 1] != 0, w2 = L.m[cy][c0x + 2] != 0;
         for (int r = 0; r < ntr; ++r) {
-          const double(*C)[SX] = L.c[r];
+          const R(*C)[SX] = L.c[r];
           if (has_adv)
-            F.fx[r][fy_][q] = ddu * vel * tracer_face(adv, C[cy][c0x - 1], C[cy][c0x],
-                                                      C[cy][c0x + 1], C[cy][c0x + 2], vel, wm, w2);
-          F.gx[r][fy_][q] = mmu * (C[cy][c0x + 1] - C[cy][c0x]) / p.dx;
+            F.fx[r][fy_][q] = ddu * vel * tracer_face<R>(adv, C[cy][c0x - 1], C[cy][c0x],
+                                                         C[cy][c0x + 1], C[cy][c0x + 2], vel, wm,
+                                                         w2);
+          F.gx[r][fy_][q] = mmu * (C[cy][c0x + 1] - C[cy][c0x]) / R(p.dx);
         }
       };
       auto yface = [&](int q, int fx_) {
         const int cx = fx_ + HALO, c0y = q - 1 + HALO;
-        const double vel = L.v[q][fx_];
-        const double ddv = L.dv[q][fx_];
-        const double mmv = mk(L.mv[q][fx_]);
+        const R vel = L.v[q][fx_];
+        const R ddv = L.dv[q][fx_];
+        const R mmv = R(mk(L.mv[q][fx_]));
         const bool wm = L.m[c0y - 1][cx] != 0, w2 = L.m[c0y + 2][cx] != 0;
         for (int r = 0; r < ntr; ++r) {
-          const double(*C)[SX] = L.c[r];
+          const R(*C)[SX] = L.c[r];
           if (has_adv)
-            F.fy[r][q][fx_] = ddv * vel * tracer_face(adv, C[c0y - 1][cx], C[c0y][cx],
-                                                      C[c0y + 1][cx], C[c0y + 2][cx], vel, wm, w2);
-          F.gy[r][q][fx_] = mmv * (C[c0y + 1][cx] - C[c0y][cx]) / p.dy;
+            F.fy[r][q][fx_] = ddv * vel * tracer_face<R>(adv, C[c0y - 1][cx], C[c0y][cx],
+                                                         C[c0y + 1][cx], C[c0y + 2][cx], vel, wm,
+                                                         w2);
+          F.gy[r][q][fx_] = mmv * (C[c0y + 1][cx] - C[c0y][cx]) / R(p.dy);
         }
       };
       xface(ty, tx + 1);
@@ -281,23 +299,23 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
       const int cx = tx + HALO, cy = ty + HALO;
 
       for (int r = 0; r < ntr; ++r) {
-        const double cc = L.c[r][cy][cx];
+        const double cc = win[r][2];  // the tile's own cell, in fp64
         double tend = 0.0;
         if (!ts) tend = p.neg_N2 * (0.5 * (w_k + w_k1));
         if (has_adv) {
-          const double hdiv = ((F.fx[r][ty][tx + 1] - F.fx[r][ty][tx]) / p.dx +
-                               (F.fy[r][ty + 1][tx] - F.fy[r][ty][tx]) / p.dy) * inv;
-          const double cft = (k == 0) ? win[r][2] : cf_top[r];
-          const double cfb = tracer_face_z(adv, k + 1, nz, win[r][1], win[r][2], win[r][3],
-                                           win[r][4], mw[1], mw[4], w_k1);
+          const R hdiv = ((F.fx[r][ty][tx + 1] - F.fx[r][ty][tx]) / R(p.dx) +
+                          (F.fy[r][ty + 1][tx] - F.fy[r][ty][tx]) / R(p.dy)) * R(inv);
+          const R cft = (k == 0) ? R(win[r][2]) : cf_top[r];
+          const R cfb = tracer_face_z<R>(adv, k + 1, nz, R(win[r][1]), R(win[r][2]),
+                                         R(win[r][3]), R(win[r][4]), mw[1], mw[4], R(w_k1));
           cf_top[r] = cfb;
-          const double vdiv = (w_k * cft - w_k1 * cfb) * inv;
-          tend = tend - (hdiv + vdiv) * (m_k ? 1.0 : 0.0);
+          const R vdiv = (R(w_k) * cft - R(w_k1) * cfb) * R(inv);
+          tend = tend - (double)((hdiv + vdiv) * (m_k ? R(1) : R(0)));
         }
         if (p.has_K_h) {
-          tend = tend + p.K_h * ((m_k ? 1.0 : 0.0) *
-                                 ((F.gx[r][ty][tx + 1] - F.gx[r][ty][tx]) / p.dx +
-                                  (F.gy[r][ty + 1][tx] - F.gy[r][ty][tx]) / p.dy));
+          tend = tend + p.K_h * (double)((m_k ? R(1) : R(0)) *
+                                         ((F.gx[r][ty][tx + 1] - F.gx[r][ty][tx]) / R(p.dx) +
+                                          (F.gy[r][ty + 1][tx] - F.gy[r][ty][tx]) / R(p.dy)));
         }
         // explicit vertical diffusion with the surface flux (vertical_var.py:80-106)
         double ft;
@@ -340,7 +358,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
       kh_k = kh_k1;
       dzi_k = dzi_k1;
     }
-    if (k + 1 < nz) tile_store(LB[(k + 1) & 1], R, tid);
+    if (k + 1 < nz) tile_store(LB[(k + 1) & 1], RG, tid);
     __syncthreads();
   }
   if (!active) return;
@@ -366,7 +384,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
     if (ts) {
       xb = (k == nz - 1) ? s_k : s_k - c_k * xb;
       S_out[tk] = xb * mm;
-      b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
+      if constexpr (sizeof(R) == sizeof(double))
+        b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
+      else
+        b_out[tk] = buoyancy_mixed(p, xa * mm, xb * mm, z_k) * mm;
     }
   }
 }
@@ -375,31 +396,37 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tracers_tiled(
 
 namespace hoc {
 
+// Staged tile of one level: the advected velocities and w in the flux type
+// R, the pressure-gradient inputs in fp64.
+template <class R>
 struct MomLds {
-  double u[SY][SX], v[SY][SX];          // velocities, 2-cell halo
+  R u[SY][SX], v[SY][SX];               // velocities, 2-cell halo
   uint8_t mu[SY][SX], mv[SY][SX];       // mask3u, mask3v, 2-cell halo
   double phi[TY + 1][TX + 1];           // potential, +1 east / north
   double b[TY + 1][TX + 1];
   double zc[TY + 1][TX + 1];
-  double w[TY + 1][TX + 1];             // w at interface k + 1
+  R w[TY + 1][TX + 1];                  // w at interface k + 1
 };
 
 // Momentum face fluxes of one level, each computed once: advective fluxes
 // (advecting velocity times the face value) and the A_h gradients, on the
 // x-faces (index q: between tile columns q-1 and q) and y-faces (between
 // rows q-1 and q) of the u and v points.
+template <class R>
 struct MomFaces {
-  double fux[TY][TX + 1], fvx[TY][TX + 1], gux[TY][TX + 1], gvx[TY][TX + 1];
-  double fuy[TY + 1][TX], fvy[TY + 1][TX], guy[TY + 1][TX], gvy[TY + 1][TX];
+  R fux[TY][TX + 1], fvx[TY][TX + 1], gux[TY][TX + 1], gvx[TY][TX + 1];
+  R fuy[TY + 1][TX], fvy[TY + 1][TX], guy[TY + 1][TX], gvy[TY + 1][TX];
 };
 
 constexpr int MS_H = (SX * SY + TILE_THREADS - 1) / TILE_THREADS;              // halo slots
 constexpr int MS_P = ((TY + 1) * (TX + 1) + TILE_THREADS - 1) / TILE_THREADS;  // +1 slots
 
+template <class R>
 struct MomRegs {
-  double u[MS_H], v[MS_H];
+  R u[MS_H], v[MS_H];
   uint8_t mu[MS_H], mv[MS_H];
-  double phi[MS_P], b[MS_P], zc[MS_P], w[MS_P];
+  double phi[MS_P], b[MS_P], zc[MS_P];
+  R w[MS_P];
 };
 
 struct MomOffs {
@@ -423,15 +450,16 @@ __device__ __forceinline__ MomOffs mom_offsets(int tid, int i0, int j0, int nx, 
   return F;
 }
 
-__device__ __forceinline__ void mom_load(MomRegs& R, const MomOffs& F, long long o, long long n2,
+template <class RT>
+__device__ __forceinline__ void mom_load(MomRegs<RT>& R, const MomOffs& F, long long o, long long n2,
                                          const double* u, const double* v, const double* phi,
                                          const double* b, const double* w, const Geo& geo) {
 #pragma unroll
   for (int q = 0; q < MS_H; ++q) {
     if (F.h[q] >= 0) {
       const long long g = o + F.h[q];
-      R.u[q] = u[g];
-      R.v[q] = v[g];
+      R.u[q] = (RT)u[g];
+      R.v[q] = (RT)v[g];
       R.mu[q] = geo.m3u[g];
       R.mv[q] = geo.m3v[g];
     }
@@ -443,12 +471,13 @@ __device__ __forceinline__ void mom_load(MomRegs& R, const MomOffs& F, long long
       R.phi[q] = phi[g];
       R.b[q] = b[g];
       R.zc[q] = geo.zc[g];
-      R.w[q] = w[g + n2];
+      R.w[q] = (RT)w[g + n2];
     }
   }
 }
 
-__device__ __forceinline__ void mom_store(MomLds& L, const MomRegs& R, int tid) {
+template <class RT>
+__device__ __forceinline__ void mom_store(MomLds<RT>& L, const MomRegs<RT>& R, int tid) {
 #pragma unroll
   for (int q = 0; q < MS_H; ++q) {
     const int t = tid + q * TILE_THREADS;
@@ -477,29 +506,33 @@ __device__ __forceinline__ void mom_store(MomLds& L, const MomRegs& R, int tid) 
 // interface value of the advected velocity (advection_v06.py:95-99,
 // advection.py:75-79); f* hold the velocity at kk-2..kk+1, m* the wet flags
 // of kk-2 and kk+1.
-__device__ __forceinline__ double mom_flux_z(int adv, int kk, int nz, double wb, double far_up,
-                                             double above, double below, double far_dn,
-                                             bool mup, bool mdn) {
-  if (kk <= 0 || kk >= nz) return 0.0;
-  if (adv == ADV_CENTRED2) return wb * 0.5 * (above + below);
+template <class R>
+__device__ __forceinline__ R mom_flux_z(int adv, int kk, int nz, R wb, R far_up, R above, R below,
+                                        R far_dn, bool mup, bool mdn) {
+  if (kk <= 0 || kk >= nz) return R(0);
+  if (adv == ADV_CENTRED2) return wb * R(0.5) * (above + below);
   const bool hu = kk >= 2 && mup;
   const bool hd = kk <= nz - 2 && mdn;
-  return wb * kappa_face_z(kk >= 2 ? far_up : 0.0, above, below, kk <= nz - 2 ? far_dn : 0.0, wb,
-                           hu, hd, adv == ADV_UP3_TVD);
+  return wb * kappa_face_z<R>(kk >= 2 ? far_up : R(0), above, below, kk <= nz - 2 ? far_dn : R(0),
+                              wb, hu, hd, adv == ADV_UP3_TVD);
 }
 
 // The step's explicit momentum terms (libs/core/model3d_v05.py:352-370),
 // tile by tile and level by level; same outputs as k_momentum_explicit.
 // Staging is double-buffered (level k + 1 loads in flight while level k is
-// computed) and every horizontal face flux is formed once.
+// computed) and every horizontal face flux is formed once. R is the type of
+// the advective fluxes and the A_h gradients (double: bitwise the
+// reference; float: the mixed step); the pressure gradient, the Coriolis
+// partners, the vertical diffusion and the tendencies stay fp64.
+template <class R>
 __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     Dims d, StepParams p, Geo geo, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ b, const double* __restrict__ phi, const double* __restrict__ w,
     const double* __restrict__ Km, double* __restrict__ pgx, double* __restrict__ pgy,
     double* __restrict__ avu, double* __restrict__ avv, double* __restrict__ dzu,
     double* __restrict__ dzv, double* __restrict__ stu, double* __restrict__ stv) {
-  __shared__ MomLds LB[2];
-  __shared__ MomFaces F;
+  __shared__ MomLds<R> LB[2];
+  __shared__ MomFaces<R> F;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
   const int tid = threadIdx.x;
@@ -548,66 +581,66 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     }
   }
   double km_k = active ? Km[c] : 0.0;
-  double fzu_top = 0.0, fzv_top = 0.0;
+  R fzu_top = R(0), fzv_top = R(0);
   const int cx = tx + HALO, cy = ty + HALO;
 
-  MomRegs R;
+  MomRegs<R> RG;
   const MomOffs MO = mom_offsets(tid, i0, j0, nx, ny);
-  mom_load(R, MO, 0, n2, u, v, phi, b, w, geo);
-  mom_store(LB[0], R, tid);
+  mom_load(RG, MO, 0, n2, u, v, phi, b, w, geo);
+  mom_store(LB[0], RG, tid);
   __syncthreads();
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
-    const MomLds& L = LB[k & 1];
-    if (k + 1 < nz) mom_load(R, MO, o + n2, n2, u, v, phi, b, w, geo);
+    const MomLds<R>& L = LB[k & 1];
+    if (k + 1 < nz) mom_load(RG, MO, o + n2, n2, u, v, phi, b, w, geo);
     {
-      const double(*U)[SX] = L.u;
-      const double(*V)[SX] = L.v;
+      const R(*U)[SX] = L.u;
+      const R(*V)[SX] = L.v;
       const uint8_t(*MU)[SX] = L.mu;
       const uint8_t(*MV)[SX] = L.mv;
       // x-face q of row fy_ (between columns q-1 and q)
       auto xface = [&](int fy_, int q) {
         const int yy = fy_ + HALO, x0 = q - 1 + HALO;  // x0: the column west of the face
         if (adv != ADV_NONE) {
-          const double ub = 0.5 * (U[yy][x0] + U[yy][x0 + 1]);
-          const double uu = 0.5 * (U[yy][x0] + U[yy + 1][x0]);
+          const R ub = R(0.5) * (U[yy][x0] + U[yy][x0 + 1]);
+          const R uu = R(0.5) * (U[yy][x0] + U[yy + 1][x0]);
           if (adv == ADV_CENTRED2) {
             F.fux[fy_][q] = ub * ub;
-            F.fvx[fy_][q] = uu * (0.5 * (V[yy][x0] + V[yy][x0 + 1]));
+            F.fvx[fy_][q] = uu * (R(0.5) * (V[yy][x0] + V[yy][x0 + 1]));
           } else {
-            F.fux[fy_][q] = ub * kappa_face(U[yy][x0 - 1], U[yy][x0], U[yy][x0 + 1],
-                                            U[yy][x0 + 2], ub, MU[yy][x0 - 1] != 0,
-                                            MU[yy][x0 + 2] != 0, lim);
-            F.fvx[fy_][q] = uu * kappa_face(V[yy][x0 - 1], V[yy][x0], V[yy][x0 + 1],
-                                            V[yy][x0 + 2], uu, MV[yy][x0 - 1] != 0,
-                                            MV[yy][x0 + 2] != 0, lim);
+            F.fux[fy_][q] = ub * kappa_face<R>(U[yy][x0 - 1], U[yy][x0], U[yy][x0 + 1],
+                                               U[yy][x0 + 2], ub, MU[yy][x0 - 1] != 0,
+                                               MU[yy][x0 + 2] != 0, lim);
+            F.fvx[fy_][q] = uu * kappa_face<R>(V[yy][x0 - 1], V[yy][x0], V[yy][x0 + 1],
+                                               V[yy][x0 + 2], uu, MV[yy][x0 - 1] != 0,
+                                               MV[yy][x0 + 2] != 0, lim);
           }
         }
-        const double m = mk(MU[yy][x0]);
-        F.gux[fy_][q] = m * (U[yy][x0 + 1] - U[yy][x0]) / p.dx;
-        F.gvx[fy_][q] = m * (V[yy][x0 + 1] - V[yy][x0]) / p.dx;
+        const R m = R(mk(MU[yy][x0]));
+        F.gux[fy_][q] = m * (U[yy][x0 + 1] - U[yy][x0]) / R(p.dx);
+        F.gvx[fy_][q] = m * (V[yy][x0 + 1] - V[yy][x0]) / R(p.dx);
       };
       // y-face q of column fx_ (between rows q-1 and q)
       auto yface = [&](int q, int fx_) {
         const int xx = fx_ + HALO, y0 = q - 1 + HALO;  // y0: the row south of the face
         if (adv != ADV_NONE) {
-          const double vb = 0.5 * (V[y0][xx] + V[y0][xx + 1]);
-          const double vv = 0.5 * (V[y0][xx] + V[y0 + 1][xx]);
+          const R vb = R(0.5) * (V[y0][xx] + V[y0][xx + 1]);
+          const R vv = R(0.5) * (V[y0][xx] + V[y0 + 1][xx]);
           if (adv == ADV_CENTRED2) {
-            F.fuy[q][fx_] = vb * (0.5 * (U[y0][xx] + U[y0 + 1][xx]));
+            F.fuy[q][fx_] = vb * (R(0.5) * (U[y0][xx] + U[y0 + 1][xx]));
             F.fvy[q][fx_] = vv * vv;
           } else {
-            F.fuy[q][fx_] = vb * kappa_face(U[y0 - 1][xx], U[y0][xx], U[y0 + 1][xx],
-                                            U[y0 + 2][xx], vb, MU[y0 - 1][xx] != 0,
-                                            MU[y0 + 2][xx] != 0, lim);
-            F.fvy[q][fx_] = vv * kappa_face(V[y0 - 1][xx], V[y0][xx], V[y0 + 1][xx],
-                                            V[y0 + 2][xx], vv, MV[y0 - 1][xx] != 0,
-                                            MV[y0 + 2][xx] != 0, lim);
+            F.fuy[q][fx_] = vb * kappa_face<R>(U[y0 - 1][xx], U[y0][xx], U[y0 + 1][xx],
+                                               U[y0 + 2][xx], vb, MU[y0 - 1][xx] != 0,
+                                               MU[y0 + 2][xx] != 0, lim);
+            F.fvy[q][fx_] = vv * kappa_face<R>(V[y0 - 1][xx], V[y0][xx], V[y0 + 1][xx],
+                                               V[y0 + 2][xx], vv, MV[y0 - 1][xx] != 0,
+                                               MV[y0 + 2][xx] != 0, lim);
           }
         }
-        const double m = mk(MV[y0][xx]);
-        F.guy[q][fx_] = m * (U[y0 + 1][xx] - U[y0][xx]) / p.dy;
-        F.gvy[q][fx_] = m * (V[y0 + 1][xx] - V[y0][xx]) / p.dy;
+        const R m = R(mk(MV[y0][xx]));
+        F.guy[q][fx_] = m * (U[y0 + 1][xx] - U[y0][xx]) / R(p.dy);
+        F.gvy[q][fx_] = m * (V[y0 + 1][xx] - V[y0][xx]) / R(p.dy);
       };
       xface(ty, tx + 1);
       yface(ty + 1, tx);
@@ -642,20 +675,31 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       }
       pgx[t] = gx;
       pgy[t] = gy;
-      // Coriolis partners (operators_masked.py:45-61)
+      // Coriolis partners (operators_masked.py:45-61), fp64: from the tile
+      // in the plain step, from the fp64 fields (neighbours in cache) when
+      // the tile holds fp32 velocities
+      double vc0, vs, ve, vse, uc0, uw, un, unw;
+      if constexpr (sizeof(R) == sizeof(double)) {
+        vc0 = L.v[cy][cx], vs = L.v[cy - 1][cx], ve = L.v[cy][cx + 1], vse = L.v[cy - 1][cx + 1];
+        uc0 = L.u[cy][cx], uw = L.u[cy][cx - 1], un = L.u[cy + 1][cx], unw = L.u[cy + 1][cx - 1];
+      } else {
+        const long long rj = (long long)j * nx, rs = (long long)pmod(j - 1, ny) * nx,
+                        rn = (long long)pmod(j + 1, ny) * nx;
+        const int ie = pmod(i + 1, nx), iw = pmod(i - 1, nx);
+        vc0 = vw5[2], vs = v[o + rs + i], ve = v[o + rj + ie], vse = v[o + rs + ie];
+        uc0 = uw5[2], uw = u[o + rj + iw], un = u[o + rn + i], unw = u[o + rn + iw];
+      }
       {
         const double w0 = mv_c, w1 = mk(L.mv[cy - 1][cx]), w2 = mk(L.mv[cy][cx + 1]),
                      w3 = mk(L.mv[cy - 1][cx + 1]);
-        const double num = 0.0 + w0 * L.v[cy][cx] + w1 * L.v[cy - 1][cx] + w2 * L.v[cy][cx + 1] +
-                           w3 * L.v[cy - 1][cx + 1];
+        const double num = 0.0 + w0 * vc0 + w1 * vs + w2 * ve + w3 * vse;
         const double den = 0.0 + w0 + w1 + w2 + w3;
         avv[t] = (den > 0.0 ? num / den : 0.0) * mu_c;
       }
       {
         const double w0 = mu_c, w1 = mk(L.mu[cy][cx - 1]), w2 = mk(L.mu[cy + 1][cx]),
                      w3 = mk(L.mu[cy + 1][cx - 1]);
-        const double num = 0.0 + w0 * L.u[cy][cx] + w1 * L.u[cy][cx - 1] + w2 * L.u[cy + 1][cx] +
-                           w3 * L.u[cy + 1][cx - 1];
+        const double num = 0.0 + w0 * uc0 + w1 * uw + w2 * un + w3 * unw;
         const double den = 0.0 + w0 + w1 + w2 + w3;
         avu[t] = (den > 0.0 ? num / den : 0.0) * mv_c;
       }
@@ -670,29 +714,29 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       // advection and horizontal viscosity
       double exu = 0.0, exv = 0.0;
       if (adv != ADV_NONE) {
-        const double ax = (F.fux[ty][tx + 1] - F.fux[ty][tx]) / p.dx;
-        const double ay = (F.fuy[ty + 1][tx] - F.fuy[ty][tx]) / p.dy;
-        const double by_ = (F.fvy[ty + 1][tx] - F.fvy[ty][tx]) / p.dy;
-        const double bx_ = (F.fvx[ty][tx + 1] - F.fvx[ty][tx]) / p.dx;
+        const R ax = (F.fux[ty][tx + 1] - F.fux[ty][tx]) / R(p.dx);
+        const R ay = (F.fuy[ty + 1][tx] - F.fuy[ty][tx]) / R(p.dy);
+        const R by_ = (F.fvy[ty + 1][tx] - F.fvy[ty][tx]) / R(p.dy);
+        const R bx_ = (F.fvx[ty][tx + 1] - F.fvx[ty][tx]) / R(p.dx);
         // vertical fluxes: the top one is the bottom one of the level above
-        const double wbu = 0.5 * (L.w[ty][tx] + L.w[ty][tx + 1]);
-        const double wbv = 0.5 * (L.w[ty][tx] + L.w[ty + 1][tx]);
-        const double fzu_bot = mom_flux_z(adv, k + 1, nz, wbu, uw5[1], uw5[2], uw5[3], uw5[4],
-                                          muw[1], muw[4]);
-        const double fzv_bot = mom_flux_z(adv, k + 1, nz, wbv, vw5[1], vw5[2], vw5[3], vw5[4],
-                                          mvw[1], mvw[4]);
-        const double az = (fzu_top - fzu_bot) / p.dz_nom;
-        const double bz = (fzv_top - fzv_bot) / p.dz_nom;
+        const R wbu = R(0.5) * (L.w[ty][tx] + L.w[ty][tx + 1]);
+        const R wbv = R(0.5) * (L.w[ty][tx] + L.w[ty + 1][tx]);
+        const R fzu_bot = mom_flux_z<R>(adv, k + 1, nz, wbu, R(uw5[1]), R(uw5[2]), R(uw5[3]),
+                                        R(uw5[4]), muw[1], muw[4]);
+        const R fzv_bot = mom_flux_z<R>(adv, k + 1, nz, wbv, R(vw5[1]), R(vw5[2]), R(vw5[3]),
+                                        R(vw5[4]), mvw[1], mvw[4]);
+        const R az = (fzu_top - fzu_bot) / R(p.dz_nom);
+        const R bz = (fzv_top - fzv_bot) / R(p.dz_nom);
         fzu_top = fzu_bot;
         fzv_top = fzv_bot;
-        exu = exu - (ax + ay + az) * mu_c;
-        exv = exv - (bx_ + by_ + bz) * mv_c;
+        exu = exu - (double)((ax + ay + az) * R(mu_c));
+        exv = exv - (double)((bx_ + by_ + bz) * R(mv_c));
       }
       if (p.has_A_h) {
-        exu = exu + p.A_h * (mu_c * ((F.gux[ty][tx + 1] - F.gux[ty][tx]) / p.dx +
-                                     (F.guy[ty + 1][tx] - F.guy[ty][tx]) / p.dy));
-        exv = exv + p.A_h * (mv_c * ((F.gvx[ty][tx + 1] - F.gvx[ty][tx]) / p.dx +
-                                     (F.gvy[ty + 1][tx] - F.gvy[ty][tx]) / p.dy));
+        exu = exu + p.A_h * (double)(R(mu_c) * ((F.gux[ty][tx + 1] - F.gux[ty][tx]) / R(p.dx) +
+                                                (F.guy[ty + 1][tx] - F.guy[ty][tx]) / R(p.dy)));
+        exv = exv + p.A_h * (double)(R(mv_c) * ((F.gvx[ty][tx + 1] - F.gvx[ty][tx]) / R(p.dx) +
+                                                (F.gvy[ty + 1][tx] - F.gvy[ty][tx]) / R(p.dy)));
       }
       double s0u = 0.0, s0v = 0.0;
       if (k == 0) {
@@ -725,9 +769,37 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       dvw[2] = k2 < nz ? geo.dz3v[(long long)k2 * n2 + c] : 0.0;
       m3w[2] = k2 < nz && geo.m3[(long long)k2 * n2 + c] != 0;
     }
-    if (k + 1 < nz) mom_store(LB[(k + 1) & 1], R, tid);
+    if (k + 1 < nz) mom_store(LB[(k + 1) & 1], RG, tid);
     __syncthreads();
   }
+}
+
+void launch_tracers_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
+                          const double* T, const double* S, const double* b, const double* u,
+                          const double* v, const double* w, const double* Kh, const double* sub,
+                          const double* den, const double* cst, double* T_out, double* S_out,
+                          double* b_out) {
+  const dim3 grid((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY);
+  if (mixed)
+    k_tracers_tiled<float><<<grid, TILE_THREADS, 0, s>>>(d, p, geo, T, S, b, u, v, w, Kh, sub,
+                                                         den, cst, T_out, S_out, b_out);
+  else
+    k_tracers_tiled<double><<<grid, TILE_THREADS, 0, s>>>(d, p, geo, T, S, b, u, v, w, Kh, sub,
+                                                          den, cst, T_out, S_out, b_out);
+}
+
+void launch_momentum_explicit_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
+                                    const double* u, const double* v, const double* b,
+                                    const double* phi, const double* w, const double* Km,
+                                    double* pgx, double* pgy, double* avu, double* avv,
+                                    double* dzu, double* dzv, double* stu, double* stv) {
+  const dim3 grid((d.nx + TX - 1) / TX, (d.ny + TY - 1) / TY);
+  if (mixed)
+    k_momentum_explicit_tiled<float><<<grid, TILE_THREADS, 0, s>>>(
+        d, p, geo, u, v, b, phi, w, Km, pgx, pgy, avu, avv, dzu, dzv, stu, stv);
+  else
+    k_momentum_explicit_tiled<double><<<grid, TILE_THREADS, 0, s>>>(
+        d, p, geo, u, v, b, phi, w, Km, pgx, pgy, avu, avv, dzu, dzv, stu, stv);
 }
 
 }  // namespace hoc

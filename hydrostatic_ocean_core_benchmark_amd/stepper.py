@@ -42,14 +42,27 @@ def _field2d(value, shape) -> np.ndarray:
     return _f64(np.broadcast_to(np.asarray(value, dtype=np.float64), shape))
 
 
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
 class HipStepper:
     """One-step integrator over a Domain, on the GPU."""
 
     def __init__(self, domain, physics, params, dt, device: str | torch.device = "cuda:0", *,
-                 ustar2=None):
+                 ustar2=None, precision: str = "fp64"):
         """`ustar2` optionally replaces the surface friction-velocity field
-        u*^2 [ny, nx] (default hypot(tau_x, tau_y) / rho0, as the reference)."""
+        u*^2 [ny, nx] (default hypot(tau_x, tau_y) / rho0, as the reference).
+        `precision`: "fp64" is the plain step (the reference's arithmetic);
+        "mixed" runs the pcg_mg V-cycle preconditioner, the advective fluxes
+        and the equation of state in fp32 and keeps the state updates, the
+        column solves and the outer PCG (residual, dot products, update) in
+        fp64 -- theta scheme with pcg_mg only."""
         params.validate()
+        if precision not in PRECISION:
+            raise ValueError(f"unknown precision {precision!r} (expected fp64|mixed)")
+        if precision == "mixed" and params.name == "theta" and params.solver != "pcg_mg":
+            raise ValueError("mixed precision runs the theta scheme with pcg_mg only")
+        self.precision = precision
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:
             raise ValueError(
                 "solver 'fft' needs a flat bottom and periodic boundaries; "
@@ -109,6 +122,7 @@ class HipStepper:
         cfg.mg_levels = int(getattr(params, "mg_levels", 5))
         cfg.mg_sweeps = int(getattr(params, "mg_sweeps", 2))
         cfg.mg_omega = float(getattr(params, "mg_omega", 0.8))
+        cfg.precision = PRECISION[precision]
         cfg.g, cfg.f, cfg.rho0 = p.g, p.f, p.rho0
         cfg.nu_b, cfg.bottom_drag, cfg.N2 = p.nu, p.bottom_drag, p.N2
         cfg.tracers_ts = 1 if p.tracers == "TS" else 0

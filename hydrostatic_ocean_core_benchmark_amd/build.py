@@ -21,7 +21,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhoc.so"
-SOURCES = ("hoc_abi.hip", "hoc_step_kernels.hip", "hoc_tiled.hip", "hoc_helmholtz.hip", "hoc_mg.hip")
+SOURCES = ("hoc_abi.hip", "hoc_step_kernels.hip", "hoc_tiled.hip", "hoc_helmholtz.hip", "hoc_mg.hip",
+           "hoc_adjoint.hip")
 ARCH = os.environ.get("HOC_OFFLOAD_ARCH", "gfx950")
 
 

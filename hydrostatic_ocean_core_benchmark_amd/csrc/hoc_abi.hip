@@ -1378,6 +1378,46 @@ int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
   return HOC_OK;
 }
 
+int hoc_thomas_adjoint(const double* sub, const double* diag, const double* sup, const double* x,
+                       const double* g, double* lambda, double* d_sub, double* d_diag,
+                       double* d_sup, double* scratch, int64_t n, int64_t ncol, void* stream) {
+  if (!sub || !diag || !sup || !x || !g || !lambda || !d_sub || !d_diag || !d_sup || !scratch)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (n < 1 || ncol < 1) return fail(HOC_ERR_INVALID, "n and ncol must be positive");
+  k_thomas_adjoint<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(
+      ncol, (int)n, sub, diag, sup, x, g, lambda, scratch, d_sub, d_diag, d_sup);
+  LAUNCH_CHECK("k_thomas_adjoint");
+  return HOC_OK;
+}
+
+int hoc_stepper_tracer_advection_adjoint(hoc_stepper* h, const double* c, const double* u,
+                                         const double* v, const double* w, const double* g_adv,
+                                         double* g_c, double* g_u, double* g_v, double* g_w,
+                                         void* stream) {
+  if (!h || !c || !u || !v || !w || !g_adv || !g_c || !g_u || !g_v || !g_w)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (h->p.advection != ADV_UP3 && h->p.advection != ADV_UP3_TVD)
+    return fail(HOC_ERR_UNSUPPORTED, "the advection adjoint covers up3 and up3_tvd");
+  if (h->d.nx < 4 || h->d.ny < 4)
+    return fail(HOC_ERR_UNSUPPORTED, "the advection adjoint needs nx, ny >= 4");
+  const Dims d = h->d;
+  k_tracer_advection_adjoint<<<blocks(d.n3), TPB, 0, (hipStream_t)stream>>>(
+      d, h->p, h->geo, c, u, v, w, g_adv, g_c, g_u, g_v, g_w);
+  LAUNCH_CHECK("k_tracer_advection_adjoint");
+  return HOC_OK;
+}
+
+int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream) {
+  if (!h || !a || !b) return fail(HOC_ERR_INVALID, "null argument");
+  if (n < 0) return fail(HOC_ERR_INVALID, "n must be >= 0");
+  for (int32_t k = 0; k < n; ++k) {
+    const int rc = (k % 2 == 0) ? hoc_stepper_step(h, a, b, stream)
+                                : hoc_stepper_step(h, b, a, stream);
+    if (rc != HOC_OK) return rc;
+  }
+  return HOC_OK;
+}
+
 int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double beta_S,
                      double T0, double S0, const double* T, const double* S, const double* z,
                      double* b, int64_t n, void* stream) {

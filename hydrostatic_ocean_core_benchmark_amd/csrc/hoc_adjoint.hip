@@ -1,0 +1,222 @@
+// Hand-written adjoints (reverse-mode derivatives) of two operators of the
+// step, for the differentiable step and its checkpointed gradient:
+//   k_thomas_adjoint            the batched tridiagonal solve x = A^-1 r:
+//                               lambda = A^-T g (a Thomas sweep on the
+//                               transposed bands), r_bar = lambda and the
+//                               band gradients -lambda x^T restricted to the
+//                               three bands;
+//   k_tracer_advection_adjoint  the flux-form kappa = 1/3 (optionally
+//                               Superbee-limited) tracer advection of
+//                               libs/core/model3d_v05.py:482-512, gathered per
+//                               cell: gradients with respect to the tracer,
+//                               the face velocities u, v and the interface w.
+// The upwind selections (sign of the face velocity, wet far cell) and the
+// limiter branch are held at their forward values; at a tie the branch the
+// forward takes (NumPy's maximum/minimum keep the first operand) is the one
+// differentiated. The face velocity enters only through the selection, so
+// its derivative is the flux's linear factor.
+#include "hoc_device.h"
+#include "hoc_kernels.h"
+
+namespace hoc {
+
+__global__ void k_thomas_adjoint(long long ncol, int n, const double* __restrict__ sub,
+                                 const double* __restrict__ diag, const double* __restrict__ sup,
+                                 const double* __restrict__ x, const double* __restrict__ g,
+                                 double* __restrict__ lam, double* __restrict__ cs,
+                                 double* __restrict__ d_sub, double* __restrict__ d_diag,
+                                 double* __restrict__ d_sup) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  // A^T: sub_T[k] = sup[k-1], sup_T[k] = sub[k+1], same diagonal
+  double cp = 0.0, dp = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const long long t = k * ncol + c;
+    const double supT = (k + 1 < n) ? sub[t + ncol] : 0.0;
+    double cc, dd;
+    if (k == 0) {
+      cc = supT / diag[t];
+      dd = g[t] / diag[t];
+    } else {
+      const double subT = sup[t - ncol];
+      const double den = diag[t] - subT * cp;
+      cc = supT / den;
+      dd = (g[t] - subT * dp) / den;
+    }
+    cs[t] = cc;
+    lam[t] = dd;
+    cp = cc;
+    dp = dd;
+  }
+  double ln = lam[(long long)(n - 1) * ncol + c];
+  for (int k = n - 2; k >= 0; --k) {
+    const long long t = k * ncol + c;
+    ln = lam[t] - cs[t] * ln;
+    lam[t] = ln;
+  }
+  // band gradients: dL/dA_kj = -lambda_k x_j
+  double x_prev = 0.0, x_k = x[c];
+  for (int k = 0; k < n; ++k) {
+    const long long t = k * ncol + c;
+    const double x_next = (k + 1 < n) ? x[t + ncol] : 0.0;
+    const double l = lam[t];
+    d_sub[t] = k > 0 ? -l * x_prev : 0.0;
+    d_diag[t] = -l * x_k;
+    d_sup[t] = (k + 1 < n) ? -l * x_next : 0.0;
+    x_prev = x_k;
+    x_k = x_next;
+  }
+}
+
+// Superbee phi(r) and dphi/dr on the branch the forward selects
+// (superbee in hoc_device.h).
+__device__ __forceinline__ void superbee_d(double r, double& phi, double& dphi) {
+  const bool a_lin = 2.0 * r <= 1.0;           // nmin(2r, 1) keeps 2r
+  const double a = a_lin ? 2.0 * r : 1.0;
+  const bool b_lin = r <= 2.0;                 // nmin(r, 2) keeps r
+  const double bb = b_lin ? r : 2.0;
+  const bool take_a = a >= bb;                 // nmax(a, bb) keeps a
+  const double m = take_a ? a : bb;
+  const double dm = take_a ? (a_lin ? 2.0 : 0.0) : (b_lin ? 1.0 : 0.0);
+  if (0.0 >= m) {                              // nmax(0, m) keeps 0
+    phi = 0.0;
+    dphi = 0.0;
+  } else {
+    phi = m;
+    dphi = dm;
+  }
+}
+
+// Partial derivatives of kappa_face(cm, c0, c1, c2, vel, wm, w2, limited)
+// with respect to (cm, c0, c1, c2), and its value.
+__device__ __forceinline__ double kappa_face_d(double cm, double c0, double c1, double c2,
+                                               double vel, bool wm, bool w2, bool limited,
+                                               double dp[4]) {
+  dp[0] = dp[1] = dp[2] = dp[3] = 0.0;
+  const bool up = vel > 0.0;
+  if (!(up ? wm : w2)) {
+    if (up) dp[1] = 1.0;
+    else dp[2] = 1.0;
+    return up ? c0 : c1;
+  }
+  if (!limited) {
+    const double a = 0.25 * K1M, b = 0.25 * K1P;
+    if (up) {
+      dp[0] = -a;
+      dp[1] = 1.0 + a - b;
+      dp[2] = b;
+    } else {
+      dp[3] = -a;
+      dp[2] = 1.0 + a - b;
+      dp[1] = b;
+    }
+    return kappa_face(cm, c0, c1, c2, vel, wm, w2, false);
+  }
+  const double dc = c1 - c0;
+  const double num = up ? (c0 - cm) : (c2 - c1);
+  double r = 0.0, phi = 0.0, dphi = 0.0;
+  if (dc != 0.0) r = num / dc;
+  superbee_d(r, phi, dphi);
+  // t = phi(r) dc / 2, r = num / dc:  dt = (dphi dnum + (phi - dphi r) ddc) / 2
+  const double a = 0.5 * dphi, b = 0.5 * (phi - dphi * r);
+  if (up) {  // f = c0 + t
+    dp[0] = -a;
+    dp[1] = 1.0 + a - b;
+    dp[2] = b;
+  } else {   // f = c1 - t
+    dp[3] = -a;
+    dp[2] = 1.0 + a - b;
+    dp[1] = b;
+  }
+  return kappa_face(cm, c0, c1, c2, vel, wm, w2, true);
+}
+
+__device__ __forceinline__ int pwrap(int a, int n) {
+  const int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// Gradients of L = sum(G * adv(c, u, v, w)) for one cell (k, j, i): gc at
+// the cell, gu / gv at its east / north face, gw at its top interface (and
+// the bottom interface nz, which carries no flux, for k = nz - 1).
+__global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
+                                           const double* __restrict__ cc,
+                                           const double* __restrict__ u,
+                                           const double* __restrict__ v,
+                                           const double* __restrict__ w,
+                                           const double* __restrict__ G,
+                                           double* __restrict__ gc, double* __restrict__ gu,
+                                           double* __restrict__ gv, double* __restrict__ gw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long n2 = d.n2;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int k = (int)(t / n2);
+  const long long c = t % n2;
+  const int j = (int)(c / nx), i = (int)(c % nx);
+  const long long o = (long long)k * n2;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  auto col = [&](int jj, int ii) -> long long { return (long long)pwrap(jj, ny) * nx + pwrap(ii, nx); };
+  // Ghat = G mask / dz at a cell of this level
+  auto ghat = [&](long long cell, long long off) -> double {
+    const long long q = off + cell;
+    return G[q] * mk(geo.m3[q]) * safe_inv(geo.dz3[q]);
+  };
+  const double* C = cc + o;
+  const uint8_t* M = geo.m3 + o;
+  double acc = 0.0;
+  // ---- x-faces: face f (east face of column f) reads columns f-1..f+2
+  for (int s = -2; s <= 1; ++s) {  // f = i + s; this cell plays role 1 - s
+    const long long f = col(j, i + s);
+    const long long fw = col(j, i + s - 1), fe = col(j, i + s + 1), fee = col(j, i + s + 2);
+    const double vel = u[o + f];
+    double dp[4];
+    const double face = kappa_face_d(C[fw], C[f], C[fe], C[fee], vel, M[fw] != 0, M[fee] != 0,
+                                     lim, dp);
+    const double fbar = (ghat(f, o) - ghat(fe, o)) / p.dx;
+    acc += fbar * geo.dz3u[o + f] * vel * dp[1 - s];
+    if (s == 0) gu[t] = fbar * geo.dz3u[o + f] * face;
+  }
+  // ---- y-faces
+  for (int s = -2; s <= 1; ++s) {
+    const long long f = col(j + s, i);
+    const long long fs = col(j + s - 1, i), fn = col(j + s + 1, i), fnn = col(j + s + 2, i);
+    const double vel = v[o + f];
+    double dp[4];
+    const double face = kappa_face_d(C[fs], C[f], C[fn], C[fnn], vel, M[fs] != 0, M[fnn] != 0,
+                                     lim, dp);
+    const double fbar = (ghat(f, o) - ghat(fn, o)) / p.dy;
+    acc += fbar * geo.dz3v[o + f] * vel * dp[1 - s];
+    if (s == 0) gv[t] = fbar * geo.dz3v[o + f] * face;
+  }
+  // ---- vertical: interface kk (1..nz-1) reads layers kk-2..kk+1 as
+  // kappa_face(cm = far_dn, c0 = below, c1 = above, c2 = far_up)
+  auto zbar = [&](int kk) -> double {  // adjoint of the flux w cf at interface kk
+    const double below = ghat(c, (long long)kk * n2);
+    const double above = kk >= 1 ? ghat(c, (long long)(kk - 1) * n2) : 0.0;
+    return below - above;
+  };
+  for (int kk = k - 1; kk <= k + 2; ++kk) {
+    if (kk < 1 || kk > nz - 1) continue;
+    const long long a = (long long)kk * n2 + c;
+    const double far_up = kk >= 2 ? cc[a - 2 * n2] : 0.0;
+    const double far_dn = kk <= nz - 2 ? cc[a + n2] : 0.0;
+    const bool hu = kk >= 2 && geo.m3[a - 2 * n2] != 0;
+    const bool hd = kk <= nz - 2 && geo.m3[a + n2] != 0;
+    double dp[4];
+    const double face = kappa_face_d(far_dn, cc[a], cc[a - n2], far_up, w[a], hd, hu, lim, dp);
+    const double wb = zbar(kk) * w[a];
+    // this layer is far_dn (kk = k - 1), below (kk = k), above (k + 1), far_up (k + 2)
+    acc += wb * dp[kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3];
+    if (kk == k) gw[t] = zbar(kk) * face;
+  }
+  if (k == 0) {  // the surface interface carries w[0] c[0]
+    const double gz0 = ghat(c, 0);
+    acc += gz0 * w[c];
+    gw[c] = gz0 * cc[c];
+  }
+  if (k == nz - 1) gw[(long long)nz * n2 + c] = 0.0;
+  gc[t] = acc;
+}
+
+}  // namespace hoc

@@ -133,6 +133,16 @@ void launch_momentum_explicit_tiled(bool mixed, hipStream_t s, Dims d, StepParam
                                     double* pgx, double* pgy, double* avu, double* avv,
                                     double* dzu, double* dzv, double* stu, double* stv);
 
+// hoc_adjoint.hip
+__global__ void k_thomas_adjoint(long long ncol, int n, const double* sub, const double* diag,
+                                 const double* sup, const double* x, const double* g,
+                                 double* lam, double* cs, double* d_sub, double* d_diag,
+                                 double* d_sup);
+__global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo, const double* cc,
+                                           const double* u, const double* v, const double* w,
+                                           const double* G, double* gc, double* gu, double* gv,
+                                           double* gw);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

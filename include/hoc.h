@@ -178,6 +178,32 @@ int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
                        const double* rhs, double* x, double* scratch, int64_t n, int64_t ncol,
                        void* stream);
 
+/* Adjoint of hoc_thomas_batched for x = A^-1 rhs (x from the forward solve):
+ * lambda = A^-T g, rhs_bar = lambda, and the band gradients
+ * d_sub[k] = -lambda[k] x[k-1], d_diag[k] = -lambda[k] x[k],
+ * d_sup[k] = -lambda[k] x[k+1] (zero in the unused sub[0], sup[n-1]).
+ * No reference counterpart (the reference has no adjoint); it is the
+ * transposed solve of vertical.thomas_batched (libs/core/vertical.py:17-44).
+ * All arrays [n][ncol] on the device; scratch n*ncol doubles. */
+int hoc_thomas_adjoint(const double* sub, const double* diag, const double* sup, const double* x,
+                       const double* g, double* lambda, double* d_sub, double* d_diag,
+                       double* d_sup, double* scratch, int64_t n, int64_t ncol, void* stream);
+
+/* Adjoint of hoc_stepper_tracer_advection (model3d_v05.py:482-512; no
+ * reference counterpart): for L = sum(g_adv * adv(c, u, v, w)) writes dL/dc,
+ * dL/du, dL/dv ([nz][ny][nx]) and dL/dw ([nz+1][ny][nx]), the upwind and
+ * limiter branches held at their forward values. Needs advection up3 or
+ * up3_tvd. */
+int hoc_stepper_tracer_advection_adjoint(hoc_stepper* h, const double* c, const double* u,
+                                         const double* v, const double* w, const double* g_adv,
+                                         double* g_c, double* g_u, double* g_v, double* g_w,
+                                         void* stream);
+
+/* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
+ * steps from *a, ping-ponging between the caller's state sets a and b; the
+ * final state is in a when n is even, in b when n is odd. */
+int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream);
+
 /* eos.buoyancy (libs/core/eos.py:130-133) for n points; eos 0/1/2 = linear,
  * seos, teos10; g, rho0, alpha_T, beta_S, T0, S0 as in Physics3D. */
 int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double beta_S,

@@ -23,6 +23,12 @@ The JSON line also carries
                 solve, 64 B per column and B_it per column and iteration
                 (B_it = 520 for MG-preconditioned PCG, 120 for PCG-Jacobi),
                 with the measured iteration counts;
+  mixed         the plain fp64 step and the mixed-precision step on the
+                1024 x 1024 x 60 basin (dx 5 km, dt 300 s, 50 steps): cell-steps/s
+                of each, their ratio, and the per-field relative L2 of the mixed
+                final state against the fp64 one;
+  adjoint       the adjoint:forward wall-time ratio of dJ/dT0 and dJ/dkappa_b
+                on the 256 x 256 x 30 basin (60 steps, checkpoints every 10);
   cpu_baseline  the NumPy oracle of the same step on one host core, timed on
                 a bounded sample (the 128 x 128 x 40 basin from the same
                 generator: 1 warm-up step, then 10 timed steps).
@@ -86,6 +92,9 @@ def parse():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=60,
+                    help="steps of the adjoint:forward measurement on the 256 x 256 x 30 "
+                         "basin (0: skip it)")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the plain-vs-mixed comparison on the 1024 x 1024 x 60 "
                          "basin (0: skip it)")
@@ -180,6 +189,38 @@ def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60) -> dict:
                           "momentum), EOS (anomaly form)",
             "fp64_parts": "state updates, column (Thomas) solves, closure, outer PCG "
                           "residual / dot products / update"}
+
+
+def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30) -> dict:
+    """dJ/dT0 and dJ/dkappa_b after `steps` steps of the 256 x 256 x 30 basin
+    (same generator, dx 10 km, dt 600 s; kappa_b = 1e-5 (1 + 0.2 xi), xi
+    seeded N(0, 1) per column), checkpoints every `every` steps: wall time of
+    the gradient (forward sweep with checkpoints + reverse sweep) over the
+    wall time of the plain forward of the same steps, after one warm-up."""
+    import numpy as np
+    import torch
+
+    from hydrostatic_ocean_core_benchmark_amd.adjoint import Adjoint
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    prob = basin(nx=nx, nz=nz)
+    g = prob.domain.grid
+    xi = np.random.default_rng(20261203).standard_normal(g.shape)
+    kb = torch.as_tensor(1e-5 * (1.0 + 0.2 * xi), device="cuda")
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    adj = Adjoint(prob.domain, prob.physics, prob.params, prob.dt)
+    adj.gradient(T0, S0, kb, n_steps=2, every=every)
+    torch.cuda.synchronize()
+    r = adj.gradient(T0, S0, kb, n_steps=steps, every=every)
+    return {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 10 km, dt 600 s, "
+                        f"{steps} steps, checkpoints every {every}, J = depth-weighted mean "
+                        "SSH^2 + 0.5 volume-mean face-weighted speed^2; gradients w.r.t. T0 "
+                        "and kappa_b(x, y)",
+            "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
+            "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
+            "hand_written": "Thomas (transposed sweep), Helmholtz (PCG on the adjoint rhs), "
+                            "tracer advection (reverse kernel); the rest by autograd through "
+                            "the torch restatement"}
 
 
 def main() -> int:
@@ -280,6 +321,8 @@ def main() -> int:
         del st, bufs, cur
         torch.cuda.empty_cache()
         rec["mixed"] = mixed_comparison(args.mixed_steps)
+    if args.adjoint_steps > 0:
+        rec["adjoint"] = adjoint_ratio(args.adjoint_steps)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args.nz, prob.params.solver)
     print(json.dumps(rec), flush=True)

@@ -460,7 +460,7 @@ class DiffStep:
         d, g = domain, domain.grid
         self.grid = g
         dev = dict(dtype=F64, device=self.device)
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), **dev)
+        t = lambda a: torch.as_tensor(np.array(a, dtype=np.float64, order="C"), **dev)
         dz3 = d.layer_thickness(np.zeros(g.shape))
         dzu, dzv = d.face_thickness(dz3)
         self.dz3, self.dz3u, self.dz3v = t(dz3), t(dzu), t(dzv)

@@ -79,6 +79,9 @@ PROTOTYPES = {
     "hoc_thomas_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_stepper_tracer_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
+    "hoc_gaspar_lengths": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, i64, i64, VP], i32),
+    "hoc_gaspar_lengths_adjoint": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, VP, VP, i64, i64,
+                                    VP], i32),
     "hoc_pairwise_dot": ([VP, VP, i64, VP, VP, VP], i32),
     "hoc_stepper_set_solver_path": ([VP, i32], i32),
     "hoc_stepper_solver_path": ([VP], i32),

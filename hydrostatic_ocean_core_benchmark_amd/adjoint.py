@@ -5,6 +5,9 @@ Functions with hand-written adjoints, and the checkpointed gradient of a
 scalar of the state after n steps.
 
 Hand-written adjoints (kernels="hip"):
+  mixing lengths         forward: the Gaspar budget march on the device
+                         (hoc_gaspar_lengths); adjoint: the reverse march per
+                         interface (hoc_gaspar_lengths_adjoint);
   Thomas solves          forward: the HIP batched Thomas solve; adjoint: the
                          transposed solve on the device (hoc_thomas_adjoint);
   Helmholtz solve        forward: the stepper's PCG (hoc_stepper_helmholtz_solve);
@@ -13,11 +16,11 @@ Hand-written adjoints (kernels="hip"):
                          coefficient gradients of -lambda^T (dA/dK) x;
   tracer advection       forward: the HIP operator; adjoint: the gathered
                          reverse kernel (hoc_stepper_tracer_advection_adjoint).
-Everything else of the step (closure, EOS, momentum advection, Coriolis,
+Everything else of the step (the rest of the closure, EOS, momentum advection, Coriolis,
 pressure gradients, the Picard loop) is differentiated by autograd through
-its torch operations. kernels="torch" replaces the three Functions by torch
-operations (a sequential Thomas sweep, a dense solve of the Helmholtz
-operator, the advection formula), so autograd differentiates the whole step:
+its torch operations. kernels="torch" replaces the four Functions by torch
+operations (the march, a sequential Thomas sweep, a dense solve of the
+Helmholtz operator, the advection formula), so autograd differentiates the whole step:
 the reference gradient the hand-written adjoints are checked against, on
 grids small enough for the dense solve.
 
@@ -48,11 +51,15 @@ KAPPA = 1.0 / 3.0
 
 # ------------------------------------------------------------ elementwise
 def nmax(a, b):
-    return torch.where((a >= b) | (a != a), a, b)
+    """NumPy's maximum (NaN propagates, a tie keeps a); a or b may be a scalar."""
+    keep = (a >= b) | (a != a) if isinstance(a, torch.Tensor) else (b <= a)
+    return torch.where(keep, a, b)
 
 
 def nmin(a, b):
-    return torch.where((a <= b) | (a != a), a, b)
+    """NumPy's minimum (NaN propagates, a tie keeps a); a or b may be a scalar."""
+    keep = (a <= b) | (a != a) if isinstance(a, torch.Tensor) else (b >= a)
+    return torch.where(keep, a, b)
 
 
 def ssqrt(x):
@@ -134,8 +141,7 @@ def laplacian(a, dx, dy, mask_u, mask_v, mask):
 
 # ---------------------------------------------------- kappa = 1/3 faces
 def superbee(r):
-    return nmax(torch.zeros_like(r), nmax(nmin(2.0 * r, torch.ones_like(r)),
-                                          nmin(r, torch.full_like(r, 2.0))))
+    return nmax(0.0, nmax(nmin(2.0 * r, 1.0), nmin(r, 2.0)))
 
 
 def kappa_face(c, vel, dim, wet, limited):
@@ -326,6 +332,37 @@ class _ThomasHip(torch.autograd.Function):
                                               _ptr(lam), _ptr(d_sub), _ptr(d_diag), _ptr(d_sup),
                                               _ptr(scratch), n, ncol, ctx.stream()))
         return d_sub, d_diag, d_sup, lam, None, None
+
+
+class _GasparHip(torch.autograd.Function):
+    """l_up, l_dn of the integral mixing length (closure.py:65-103): HIP
+    forward march, hand-written reverse march (hoc_gaspar_lengths_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, e, n2, dzi, zs, zb, wet, e_min, z_0, lib, stream):
+        ins = [t.contiguous() for t in (e, n2, dzi, zs, zb, wet)]
+        nz1 = ins[0].shape[0]
+        ncol = ins[0].numel() // nz1
+        l_up, l_dn = torch.empty_like(ins[0]), torch.empty_like(ins[0])
+        _lib.check(lib.hoc_gaspar_lengths(*(_ptr(t) for t in ins), e_min, z_0, _ptr(l_up),
+                                          _ptr(l_dn), nz1, ncol, stream()))
+        ctx.save_for_backward(*ins)
+        ctx.args = (e_min, z_0, lib, stream)
+        return l_up, l_dn
+
+    @staticmethod
+    def backward(ctx, g_up, g_dn):
+        ins = ctx.saved_tensors
+        e_min, z_0, lib, stream = ctx.args
+        e = ins[0]
+        g_up = torch.zeros_like(e) if g_up is None else g_up.contiguous()
+        g_dn = torch.zeros_like(e) if g_dn is None else g_dn.contiguous()
+        g_e, g_n2 = torch.empty_like(e), torch.empty_like(e)
+        nz1 = e.shape[0]
+        _lib.check(lib.hoc_gaspar_lengths_adjoint(*(_ptr(t) for t in ins), e_min, z_0,
+                                                  _ptr(g_up), _ptr(g_dn), _ptr(g_e), _ptr(g_n2),
+                                                  nz1, e.numel() // nz1, stream()))
+        return g_e, g_n2, None, None, None, None, None, None, None, None
 
 
 class _HelmholtzHip(torch.autograd.Function):
@@ -592,8 +629,8 @@ class DiffStep:
         cost = n2_j * dz_j * (l + 0.5 * dz_j)
         full = cost <= budget
         a = torch.where(n2_j > 0.0, n2_j, 1.0)
-        s_part = ssqrt(nmax(l * l + 2.0 * budget / a, torch.zeros_like(l))) - l
-        clipped = nmin(nmax(s_part, torch.zeros_like(s_part)), dz_j.expand_as(s_part))
+        s_part = ssqrt(nmax(l * l + 2.0 * budget / a, 0.0)) - l
+        clipped = nmin(nmax(s_part, 0.0), dz_j.expand_as(s_part))
         step = torch.where(full, dz_j.expand_as(l), torch.where(n2_j > 0.0, clipped, dz_j))
         return l + torch.where(budget > 0.0, step, 0.0), torch.where(full, budget - cost, 0.0)
 
@@ -603,7 +640,7 @@ class DiffStep:
         dz3, mask3 = self.dz3, self.mask3
         nz = dz3.shape[0]
         zi = torch.cat([torch.zeros_like(dz3[:1]), torch.cumsum(dz3, dim=0)], dim=0)
-        z_b = nmax(zi[-1][None] - zi, torch.zeros_like(zi))
+        z_b = nmax(zi[-1][None] - zi, 0.0)
         z_s = zi
         wet = torch.cat([torch.zeros_like(zi[:1]), mask3[:-1] * mask3[1:],
                          torch.zeros_like(zi[:1])], dim=0)
@@ -618,9 +655,9 @@ class DiffStep:
         shear2 = (du ** 2 + dv ** 2) * wet
         n2 = db * wet
         nz1 = nz + 1
-        e_min = torch.full_like(e, p.e_min)
+        e_min = float(p.e_min)
         if p.mxl == "recursive":
-            n2c = nmax(n2, torch.full_like(n2, p.n2_min))
+            n2c = nmax(n2, float(p.n2_min))
             l = ssqrt(2.0 * nmax(e, e_min) / n2c)
             l = nmin(l, nmin(z_s + p.z_0, z_b + p.z_0))
             rows = [l[0]]
@@ -632,8 +669,12 @@ class DiffStep:
                 down[k] = nmin(rows[k], down[k + 1] + dzi[k + 1])
             l = torch.stack(down, dim=0) * wet
             l_up = l_dn = l
+        elif self.kernels == "hip" and nz1 <= 128:
+            st = self.owner
+            l_up, l_dn = _GasparHip.apply(e, n2, dzi, z_s, z_b, wet, float(p.e_min),
+                                          float(p.z_0), st.lib, st._stream)
         else:
-            n2p = nmax(n2, torch.zeros_like(n2))
+            n2p = nmax(n2, 0.0)
             e_k = nmax(e, e_min)
             l_up = torch.zeros_like(e)
             l_dn = torch.zeros_like(e)
@@ -647,7 +688,7 @@ class DiffStep:
                 b_dn = torch.cat([bd, b_dn[nz1 - j:]], dim=0)
             l_up = nmin(l_up, z_s + p.z_0) * wet
             l_dn = nmin(l_dn, z_b + p.z_0) * wet
-        l_min = torch.full_like(e, p.l_min)
+        l_min = float(p.l_min)
         l_k = nmax(l_min, ssqrt(l_up * l_dn))
         l_eps = nmax(l_min, nmin(l_up, l_dn))
         nu_b = p.nu

@@ -1407,6 +1407,33 @@ int hoc_stepper_tracer_advection_adjoint(hoc_stepper* h, const double* c, const 
   return HOC_OK;
 }
 
+int hoc_gaspar_lengths(const double* e, const double* n2, const double* dzi, const double* zs,
+                       const double* zb, const double* wet, double e_min, double z_0,
+                       double* l_up, double* l_dn, int64_t nz1, int64_t ncol, void* stream) {
+  if (!e || !n2 || !dzi || !zs || !zb || !wet || !l_up || !l_dn)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (nz1 < 1 || ncol < 1) return fail(HOC_ERR_INVALID, "nz1 and ncol must be positive");
+  k_gaspar_lengths<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(ncol, (int)nz1, e_min, z_0, e,
+                                                                  n2, dzi, zs, zb, wet, l_up, l_dn);
+  LAUNCH_CHECK("k_gaspar_lengths");
+  return HOC_OK;
+}
+
+int hoc_gaspar_lengths_adjoint(const double* e, const double* n2, const double* dzi,
+                               const double* zs, const double* zb, const double* wet,
+                               double e_min, double z_0, const double* g_up, const double* g_dn,
+                               double* g_e, double* g_n2, int64_t nz1, int64_t ncol,
+                               void* stream) {
+  if (!e || !n2 || !dzi || !zs || !zb || !wet || !g_up || !g_dn || !g_e || !g_n2)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (nz1 < 1 || ncol < 1) return fail(HOC_ERR_INVALID, "nz1 and ncol must be positive");
+  if (nz1 > 128) return fail(HOC_ERR_UNSUPPORTED, "the mixing-length adjoint needs nz + 1 <= 128");
+  k_gaspar_lengths_adjoint<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(
+      ncol, (int)nz1, e_min, z_0, e, n2, dzi, zs, zb, wet, g_up, g_dn, g_e, g_n2);
+  LAUNCH_CHECK("k_gaspar_lengths_adjoint");
+  return HOC_OK;
+}
+
 int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream) {
   if (!h || !a || !b) return fail(HOC_ERR_INVALID, "null argument");
   if (n < 0) return fail(HOC_ERR_INVALID, "n must be >= 0");

@@ -220,3 +220,151 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
 }
 
 }  // namespace hoc
+
+namespace hoc {
+
+// ------------------------------------------------ Gaspar mixing lengths
+// The budget march of libs/core/closure.py:65-103 for one interface and one
+// direction, every crossing evaluated (no early exit), in the arithmetic of
+// the march: cost = N2 dz (l + dz/2); a full crossing adds dz and spends
+// cost; otherwise the parcel stops inside the layer (l grows by
+// clip(sqrt(l^2 + 2 budget / N2) - l, 0, dz)) and the budget is spent.
+struct MarchIn {
+  double l, b;
+};
+
+__device__ __forceinline__ MarchIn march_step(double l, double b, double n2j, double dzj) {
+  const double cost = n2j * dzj * (l + 0.5 * dzj);
+  const bool full = cost <= b;
+  const double a = n2j > 0.0 ? n2j : 1.0;
+  const double A = nmax(l * l + 2.0 * b / a, 0.0);
+  const double sq = A > 0.0 ? sqrt(A) : 0.0;
+  const double s = sq - l;
+  const double clipped = nmin(nmax(s, 0.0), dzj);
+  const double step = full ? dzj : (n2j > 0.0 ? clipped : dzj);
+  return MarchIn{l + (b > 0.0 ? step : 0.0), full ? b - cost : 0.0};
+}
+
+// Reverse of march_step: given the adjoints of (l', b') return those of
+// (l, b) and add the one of n2j. Branches held at their forward values;
+// at a tie of a maximum/minimum the first operand's branch (NumPy's choice)
+// is differentiated.
+__device__ __forceinline__ void march_step_adjoint(double l, double b, double n2j, double dzj,
+                                                   double lb_out, double bb_out, double& lb,
+                                                   double& bb, double& n2b) {
+  const double cost = n2j * dzj * (l + 0.5 * dzj);
+  const bool full = cost <= b;
+  lb = lb_out;
+  bb = 0.0;
+  if (full) {  // b' = b - cost
+    bb += bb_out;
+    lb += -bb_out * n2j * dzj;
+    n2b += -bb_out * dzj * (l + 0.5 * dzj);
+  }
+  if (b > 0.0 && !full && n2j > 0.0) {  // l' = l + clip(s, 0, dz)
+    const double a = n2j;
+    const double A0 = l * l + 2.0 * b / a;
+    const bool a_pass = A0 >= 0.0 || A0 != A0;  // nmax(A0, 0) keeps A0
+    const double A = a_pass ? A0 : 0.0;
+    const double sq = A > 0.0 ? sqrt(A) : 0.0;
+    const double s = sq - l;
+    const bool s_pass = s >= 0.0 || s != s;     // nmax(s, 0) keeps s
+    const double m = s_pass ? s : 0.0;
+    const bool m_pass = m <= dzj || m != m;     // nmin(m, dz) keeps m
+    if (s_pass && m_pass) {
+      const double dA = (a_pass && A > 0.0) ? lb_out / (2.0 * sq) : 0.0;  // adjoint of A0
+      lb += dA * 2.0 * l - lb_out;
+      bb += dA * 2.0 / a;
+      n2b += -dA * 2.0 * b / (a * a);
+    }
+  }
+}
+
+// l_up, l_dn on every interface of a column (layout [nz1][ncol]): the
+// march above from e_k = max(e, e_min), then min(l, z + z_0) * wet.
+__global__ void k_gaspar_lengths(long long ncol, int nz1, double e_min, double z_0,
+                                 const double* __restrict__ e, const double* __restrict__ n2,
+                                 const double* __restrict__ dzi, const double* __restrict__ zs,
+                                 const double* __restrict__ zb, const double* __restrict__ wet,
+                                 double* __restrict__ l_up, double* __restrict__ l_dn) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  for (int k = 0; k < nz1; ++k) {
+    const long long t = (long long)k * ncol + c;
+    const double ek = nmax(e[t], e_min);
+    MarchIn u{0.0, ek}, dn{0.0, ek};
+    for (int j = 1; j <= k; ++j) {
+      const long long q = (long long)(k - j) * ncol + c;
+      u = march_step(u.l, u.b, nmax(n2[q], 0.0), dzi[q]);
+    }
+    for (int j = 1; k + j < nz1; ++j) {
+      const long long q = (long long)(k + j) * ncol + c;
+      dn = march_step(dn.l, dn.b, nmax(n2[q], 0.0), dzi[q]);
+    }
+    l_up[t] = nmin(u.l, zs[t] + z_0) * wet[t];
+    l_dn[t] = nmin(dn.l, zb[t] + z_0) * wet[t];
+  }
+}
+
+constexpr int GASPAR_MAX_NZ1 = 128;
+
+// Adjoint of k_gaspar_lengths: g_e and g_n2 (overwritten) from the adjoints
+// of l_up and l_dn. Each march is re-run forward with its states kept in a
+// per-thread array, then reversed.
+__global__ void k_gaspar_lengths_adjoint(long long ncol, int nz1, double e_min, double z_0,
+                                         const double* __restrict__ e,
+                                         const double* __restrict__ n2,
+                                         const double* __restrict__ dzi,
+                                         const double* __restrict__ zs,
+                                         const double* __restrict__ zb,
+                                         const double* __restrict__ wet,
+                                         const double* __restrict__ g_up,
+                                         const double* __restrict__ g_dn,
+                                         double* __restrict__ g_e, double* __restrict__ g_n2) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  double n2b[GASPAR_MAX_NZ1];  // adjoint of max(n2, 0) per interface
+  double ls[GASPAR_MAX_NZ1], bs[GASPAR_MAX_NZ1];
+  for (int k = 0; k < nz1; ++k) n2b[k] = 0.0;
+  for (int k = 0; k < nz1; ++k) {
+    const long long t = (long long)k * ncol + c;
+    const double ek = nmax(e[t], e_min);
+    double bbar0 = 0.0;
+    for (int dir = 0; dir < 2; ++dir) {
+      const int nstep = dir == 0 ? k : nz1 - 1 - k;
+      // forward, keeping the state before every crossing
+      double l = 0.0, b = ek;
+      for (int j = 1; j <= nstep; ++j) {
+        const int kk = dir == 0 ? k - j : k + j;
+        const long long q = (long long)kk * ncol + c;
+        ls[j] = l;
+        bs[j] = b;
+        const MarchIn o = march_step(l, b, nmax(n2[q], 0.0), dzi[q]);
+        l = o.l;
+        b = o.b;
+      }
+      // l_out = min(l, z + z_0) * wet
+      const double cap = (dir == 0 ? zs[t] : zb[t]) + z_0;
+      const double gout = (dir == 0 ? g_up[t] : g_dn[t]) * wet[t];
+      double lb = (l <= cap || l != l) ? gout : 0.0;
+      double bb = 0.0;
+      for (int j = nstep; j >= 1; --j) {
+        const int kk = dir == 0 ? k - j : k + j;
+        const long long q = (long long)kk * ncol + c;
+        double lb_in, bb_in, nb = 0.0;
+        march_step_adjoint(ls[j], bs[j], nmax(n2[q], 0.0), dzi[q], lb, bb, lb_in, bb_in, nb);
+        lb = lb_in;
+        bb = bb_in;
+        n2b[kk] += nb;
+      }
+      bbar0 += bb;  // the march starts from budget e_k (l starts at the constant 0)
+    }
+    g_e[t] = (e[t] >= e_min || e[t] != e[t]) ? bbar0 : 0.0;
+  }
+  for (int k = 0; k < nz1; ++k) {
+    const long long t = (long long)k * ncol + c;
+    g_n2[t] = (n2[t] >= 0.0 || n2[t] != n2[t]) ? n2b[k] : 0.0;
+  }
+}
+
+}  // namespace hoc

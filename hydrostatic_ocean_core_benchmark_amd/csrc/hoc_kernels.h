@@ -143,6 +143,16 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo, const 
                                            const double* G, double* gc, double* gu, double* gv,
                                            double* gw);
 
+__global__ void k_gaspar_lengths(long long ncol, int nz1, double e_min, double z_0,
+                                 const double* e, const double* n2, const double* dzi,
+                                 const double* zs, const double* zb, const double* wet,
+                                 double* l_up, double* l_dn);
+__global__ void k_gaspar_lengths_adjoint(long long ncol, int nz1, double e_min, double z_0,
+                                         const double* e, const double* n2, const double* dzi,
+                                         const double* zs, const double* zb, const double* wet,
+                                         const double* g_up, const double* g_dn, double* g_e,
+                                         double* g_n2);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

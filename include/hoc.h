@@ -199,6 +199,24 @@ int hoc_stepper_tracer_advection_adjoint(hoc_stepper* h, const double* c, const 
                                          double* g_c, double* g_u, double* g_v, double* g_w,
                                          void* stream);
 
+/* Gaspar integral mixing lengths of closure.gaspar_lengths
+ * (libs/core/closure.py:65-103) on every interface, arrays [nz1][ncol] on
+ * the device: from e, the interface N^2 (n2), the interface spacing dzi, the
+ * depth below the surface zs and above the bottom zb and the interior-wet
+ * flag wet (closure.py:43-62), write l_up and l_dn. */
+int hoc_gaspar_lengths(const double* e, const double* n2, const double* dzi, const double* zs,
+                       const double* zb, const double* wet, double e_min, double z_0,
+                       double* l_up, double* l_dn, int64_t nz1, int64_t ncol, void* stream);
+
+/* Adjoint of hoc_gaspar_lengths (no reference counterpart): g_e and g_n2 from
+ * the adjoints g_up, g_dn of l_up, l_dn, the march's branches held at their
+ * forward values. nz1 <= 128. */
+int hoc_gaspar_lengths_adjoint(const double* e, const double* n2, const double* dzi,
+                               const double* zs, const double* zb, const double* wet,
+                               double e_min, double z_0, const double* g_up, const double* g_dn,
+                               double* g_e, double* g_n2, int64_t nz1, int64_t ncol,
+                               void* stream);
+
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */

@@ -69,6 +69,59 @@ def test_thomas_adjoint_against_autograd(adj):
     assert float(got[0][0].abs().max()) == 0.0 and float(got[2][-1].abs().max()) == 0.0
 
 
+def test_mixing_length_adjoint_against_autograd(adj):
+    """hoc_gaspar_lengths and its adjoint against the torch march and
+    autograd through it, on the closure inputs of a state two steps into
+    the 32 x 32 x 40 basin (e perturbed so budgets run out at varied
+    depths)."""
+    prob = _basin(32)
+    kb = _kappa(prob)
+    hip = adj.DiffStep(prob.domain, prob.physics, prob.params, prob.dt, kernels="hip")
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    with torch.no_grad():
+        s = hip.initial_state(T0, S0, kb)
+        for _ in range(2):
+            s = hip.step(s, kb)
+    dz3 = hip.dz3
+    nz = dz3.shape[0]
+    zi = torch.cat([torch.zeros_like(dz3[:1]), torch.cumsum(dz3, 0)], 0)
+    zb = adj.nmax(zi[-1][None] - zi, torch.zeros_like(zi))
+    wet = torch.cat([torch.zeros_like(zi[:1]), hip.mask3[:-1] * hip.mask3[1:],
+                     torch.zeros_like(zi[:1])], 0)
+    dzi = hip.interface_thickness(dz3)
+    db = torch.cat([torch.zeros_like(zi[:1]), (s.b[:-1] - s.b[1:]) * adj.safe_inv(dzi[1:nz]),
+                    torch.zeros_like(zi[:1])], 0)
+    n2 = db * wet
+    e = s.e * torch.exp(torch.randn_like(s.e))
+    p = prob.physics
+    from hydrostatic_ocean_core_benchmark_amd import _lib
+    lib = _lib.load()
+    stream = lambda: torch.cuda.current_stream().cuda_stream
+    e1, n1 = e.clone().requires_grad_(), n2.clone().requires_grad_()
+    lu, ld = adj._GasparHip.apply(e1, n1, dzi, zi, zb, wet, float(p.e_min), float(p.z_0), lib,
+                                  stream)
+    wu, wd = torch.randn_like(e), torch.randn_like(e)
+    got = torch.autograd.grad((lu * wu).sum() + (ld * wd).sum(), [e1, n1])
+    # the torch march of DiffStep.tke_step
+    e2, n2r = e.clone().requires_grad_(), n2.clone().requires_grad_()
+    n2p = adj.nmax(n2r, torch.zeros_like(n2r))
+    ek = adj.nmax(e2, torch.full_like(e2, float(p.e_min)))
+    nz1 = nz + 1
+    l_up, l_dn, b_up, b_dn = torch.zeros_like(e), torch.zeros_like(e), ek, ek
+    for j in range(1, nz1):
+        lu2, bu2 = adj.DiffStep._march(l_up[j:], b_up[j:], n2p[:nz1 - j], dzi[:nz1 - j])
+        l_up, b_up = torch.cat([l_up[:j], lu2], 0), torch.cat([b_up[:j], bu2], 0)
+        ld2, bd2 = adj.DiffStep._march(l_dn[:nz1 - j], b_dn[:nz1 - j], n2p[j:], dzi[j:])
+        l_dn, b_dn = torch.cat([ld2, l_dn[nz1 - j:]], 0), torch.cat([bd2, b_dn[nz1 - j:]], 0)
+    l_up = adj.nmin(l_up, zi + p.z_0) * wet
+    l_dn = adj.nmin(l_dn, zb + p.z_0) * wet
+    assert _rel(lu.detach(), l_up.detach()) < 1e-14 and _rel(ld.detach(), l_dn.detach()) < 1e-14
+    want = torch.autograd.grad((l_up * wu).sum() + (l_dn * wd).sum(), [e2, n2r])
+    assert _rel(got[0], want[0]) < 1e-11, _rel(got[0], want[0])
+    assert _rel(got[1], want[1]) < 1e-11, _rel(got[1], want[1])
+
+
 @pytest.mark.parametrize("advection", ["up3_tvd", "up3"])
 def test_tracer_advection_adjoint_against_autograd(adj, advection):
     """The reverse advection kernel against autograd through the advection

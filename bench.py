@@ -27,6 +27,9 @@ The JSON line also carries
                 1024 x 1024 x 60 basin (dx 5 km, dt 300 s, 50 steps): cell-steps/s
                 of each, their ratio, and the per-field relative L2 of the mixed
                 final state against the fp64 one;
+  ensemble      aggregate cell-steps/s of the 64-member ensemble of uneven
+                grids (16 each at 128^2 .. 384^2, 30 levels) replayed from one
+                HIP graph per two steps, beside the members run one by one;
   adjoint       the adjoint:forward wall-time ratio of dJ/dT0 and dJ/dkappa_b
                 on the 256 x 256 x 30 basin (60 steps, checkpoints every 10);
   cpu_baseline  the NumPy oracle of the same step on one host core, timed on
@@ -89,9 +92,14 @@ def parse():
     ap.add_argument("--solver", default=None,
                     choices=("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid", "pcg_mg"),
                     help="Helmholtz solver kind (default: the workload's)")
+    ap.add_argument("--mg-tile", type=int, default=None,
+                    help="fused V-cycle leg tile shape 0..3 (32x16, 64x8, 16x16, 64x16); "
+                         "default: the library's")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="steps of the 64-member ensemble throughput run (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=60,
                     help="steps of the adjoint:forward measurement on the 256 x 256 x 30 "
                          "basin (0: skip it)")
@@ -223,6 +231,33 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30)
                             "the torch restatement"}
 
 
+def ensemble_throughput(steps: int = 100) -> dict:
+    """The 64-member ensemble (16 each at 128^2, 192^2, 256^2, 384^2, 30
+    levels, per-member seeds, T noise N(0, 0.01)): leg tile shapes by the
+    autotuner, solve budgets calibrated, one HIP graph per two steps;
+    aggregate cell-steps/s over `steps` steps, beside the same steps run
+    member after member with polled solves."""
+    from hydrostatic_ocean_core_benchmark_amd.ensemble import Ensemble, benchmark_members
+    t0 = time.perf_counter()
+    e = Ensemble(benchmark_members())
+    setup = time.perf_counter() - t0
+    tiles = e.autotune()
+    budgets = e.calibrate()
+    e.capture()
+    e.run(2)
+    r = e.run(steps)
+    eager = e.run_eager(steps)
+    return {"workload": "64 members: 16 each at 128x128, 192x192, 256x256, 384x384, 30 levels, "
+                        "GRF basins from per-member seeds, T noise N(0, 0.01 K), fp64, "
+                        f"{steps} steps, one HIP graph replay per 2 steps",
+            "unit": "cell-steps/s", "value": r.cell_steps_per_s, "cells": r.cells,
+            "wall_s": round(r.wall_s, 3), "solver_failures": r.solver_failures,
+            "eager_sequential_value": r.cells * steps / eager,
+            "graph_chains": len(e.chains), "tile_choice": {k: v["tile"] for k, v in tiles.items()},
+            "solve_budget_range": [min(budgets.values()), max(budgets.values())],
+            "setup_s": round(setup, 1)}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -251,6 +286,9 @@ def main() -> int:
     g = prob.domain.grid
     st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=args.precision)
     st.solver_path = args.solver_path
+    if args.mg_tile is not None:
+        from hydrostatic_ocean_core_benchmark_amd import _lib
+        _lib.check(st.lib.hoc_stepper_set_mg_tile(st._h, args.mg_tile))
     cur = _initial_fields(prob, st, torch)
     bufs = [st._new_state_buffers(), st._new_state_buffers()]
     n = 0
@@ -321,6 +359,8 @@ def main() -> int:
         del st, bufs, cur
         torch.cuda.empty_cache()
         rec["mixed"] = mixed_comparison(args.mixed_steps)
+    if args.ensemble_steps > 0:
+        rec["ensemble"] = ensemble_throughput(args.ensemble_steps)
     if args.adjoint_steps > 0:
         rec["adjoint"] = adjoint_ratio(args.adjoint_steps)
     if not args.no_cpu_baseline:

@@ -79,6 +79,8 @@ PROTOTYPES = {
     "hoc_thomas_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_stepper_tracer_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
+    "hoc_stepper_set_sync_free": ([VP, i32], i32),
+    "hoc_stepper_set_mg_tile": ([VP, i32], i32),
     "hoc_gaspar_lengths": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, i64, i64, VP], i32),
     "hoc_gaspar_lengths_adjoint": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, VP, VP, i64, i64,
                                     VP], i32),

@@ -588,4 +588,13 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_rz(PcgArgs a, int into_b) {
   if (threadIdx.x == 0) (into_b ? a.part_b : a.part_c)[part] = s.a;
 }
 
+// After a sync-free solve of n_it launched iterations: a solve whose done
+// flag is still clear did not converge within the budget; it is counted as
+// a failure with n_it iterations.
+__global__ void k_pcg_budget_check(PcgArgs a, int n_it) {
+  if (threadIdx.x != 0 || a.sc->done) return;
+  a.counters[0] += n_it;
+  a.counters[1] += 1;
+}
+
 }  // namespace hoc

@@ -250,10 +250,11 @@ int mg_cycle_prepare(size_t lds);        // < 0 if the fp64 cycle does not fit o
 int mg_cycle_prepare_f32(size_t lds);    // the same for the fp32 cycles
 void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
                             double* out, const PcgScalars* sc, hipStream_t s);
-void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
+constexpr int MG_TILE_SHAPES = 4;  // fused-leg tile shapes (32x16, 64x8, 16x16, 64x16)
+void launch_mg_down(int S, int shape, hipStream_t s, const MgLevel& L, const MgLevel& C,
                     double omega, const double* b, double* xs_out, double* bc,
                     const PcgScalars* sc);
-void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
+void launch_mg_up(int S, int shape, hipStream_t s, const MgLevel& L, int cnx, double omega,
                   const double* xs_in, const double* ec, const double* b, double* out,
                   const PcgScalars* sc);
 // fp32 V-cycle of the mixed-precision preconditioner: level 0 reads the fp64
@@ -278,5 +279,6 @@ __global__ void k_pcg_init(PcgArgs a);
 __global__ void k_pcg_init_z(PcgArgs a);
 __global__ void k_pcg_ap(PcgArgs a, int it);
 __global__ void k_pcg_update(PcgArgs a, int it, int jacobi);
+__global__ void k_pcg_budget_check(PcgArgs a, int n_it);
 
 }  // namespace hoc

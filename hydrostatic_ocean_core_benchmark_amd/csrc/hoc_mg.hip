@@ -246,19 +246,19 @@ __global__ __launch_bounds__(PCG_WG) void k_mg_start(PcgArgs a) {
 
 // Way down on one level: x_S = S sweeps from zero (written out for the way
 // up) and bc = restrict((b - A x_S) mask).
-template <int S, class R, class RB>
+template <int S, class R, class RB, int TW = MG_TW, int TH = MG_TH>
 __global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevelT<R> L, MgLevelT<R> C, R omega,
                                                         const RB* __restrict__ b,
                                                         R* __restrict__ xs_out,
                                                         R* __restrict__ bc,
                                                         const PcgScalars* sc) {
   if (finished(sc)) return;
-  constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
+  constexpr int RW = TW + 2 * S, RH = TH + 2 * S;
   __shared__ R buf[2][RH * RW];
-  const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
+  const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
   for (int k = 1; k <= S; ++k) {
     const int h = S - k + 1;  // halo of x_k
-    const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
+    const int w = TW + 2 * h, hh = TH + 2 * h;
     R* dst = buf[(k - 1) & 1];
     const R* src = buf[k & 1];
     for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
@@ -275,12 +275,12 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevelT<R> L, MgLevelT<
     __syncthreads();
   }
   const R* xs = buf[(S - 1) & 1];
-  for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
-    const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
+  for (int t = threadIdx.x; t < TW * TH; t += MG_THREADS) {
+    const int gi = i0 + t % TW, gj = j0 + t / TW;
     if (gi < L.nx && gj < L.ny)
-      xs_out[(long long)gj * L.nx + gi] = xs[(S + t / MG_TW) * RW + S + t % MG_TW];
+      xs_out[(long long)gj * L.nx + gi] = xs[(S + t / TW) * RW + S + t % TW];
   }
-  constexpr int CW = MG_TW / 2, CH = MG_TH / 2;
+  constexpr int CW = TW / 2, CH = TH / 2;
   for (int t = threadIdx.x; t < CW * CH; t += MG_THREADS) {
     const int I = i0 / 2 + t % CW, J = j0 / 2 + t / CW;
     if (I >= C.nx || J >= C.ny) continue;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_down(MgLevelT<R> L, MgLevelT<
 }
 
 // Way up on one level: x = x_S + prolong(ec) mask, then S sweeps; out = x.
-template <int S, class R, class RB, class RO>
+template <int S, class R, class RB, class RO, int TW = MG_TW, int TH = MG_TH>
 __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R omega,
                                                       const R* __restrict__ xs_in,
                                                       const R* __restrict__ ec,
@@ -307,9 +307,9 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R 
                                                       RO* __restrict__ out,
                                                       const PcgScalars* sc) {
   if (finished(sc)) return;
-  constexpr int RW = MG_TW + 2 * S, RH = MG_TH + 2 * S;
+  constexpr int RW = TW + 2 * S, RH = TH + 2 * S;
   __shared__ R buf[2][RH * RW];
-  const int i0 = blockIdx.x * MG_TW, j0 = blockIdx.y * MG_TH;
+  const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
   for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) {
     const int gi = mg_pmod(i0 - S + t % RW, L.nx), gj = mg_pmod(j0 - S + t / RW, L.ny);
     const long long c = (long long)gj * L.nx + gi;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R 
   __syncthreads();
   for (int k = 1; k <= S; ++k) {
     const int h = S - k;
-    const int w = MG_TW + 2 * h, hh = MG_TH + 2 * h;
+    const int w = TW + 2 * h, hh = TH + 2 * h;
     R* dst = buf[k & 1];
     const R* src = buf[(k - 1) & 1];
     for (int t = threadIdx.x; t < w * hh; t += MG_THREADS) {
@@ -331,10 +331,10 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R 
     __syncthreads();
   }
   const R* x = buf[S & 1];
-  for (int t = threadIdx.x; t < MG_TW * MG_TH; t += MG_THREADS) {
-    const int gi = i0 + t % MG_TW, gj = j0 + t / MG_TW;
+  for (int t = threadIdx.x; t < TW * TH; t += MG_THREADS) {
+    const int gi = i0 + t % TW, gj = j0 + t / TW;
     if (gi < L.nx && gj < L.ny)
-      out[(long long)gj * L.nx + gi] = (RO)x[(S + t / MG_TW) * RW + S + t % MG_TW];
+      out[(long long)gj * L.nx + gi] = (RO)x[(S + t / TW) * RW + S + t % TW];
   }
 }
 
@@ -493,28 +493,59 @@ void launch_mg_coarse_cycle_f32(const MgCycleF& C, size_t lds, float omega, int 
         <<<1, MG_COARSE_THREADS, lds, s>>>(C, omega, S, b0_f, out_f, sc);
 }
 
-void launch_mg_down(int S, dim3 grid, hipStream_t s, const MgLevel& L, const MgLevel& C,
-                    double omega, const double* b, double* xs_out, double* bc,
-                    const PcgScalars* sc) {
-  if (S == 1)
-    k_mg_down<1, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
-  else if (S == 2)
-    k_mg_down<2, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
-  else
-    k_mg_down<3, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+// Fused-leg tile shapes (cells per workgroup tile; every shape gives the
+// same values, they differ only in halo recomputation and occupancy):
+// 0: 32 x 16, 1: 64 x 8, 2: 16 x 16, 3: 64 x 16. Shapes 1-3 are built for
+// 2 sweeps per side (the default), other sweep counts use shape 0.
+dim3 mg_tile_grid_shape(const MgLevel& L, int shape) {
+  static const int tw[MG_TILE_SHAPES] = {32, 64, 16, 64}, th[MG_TILE_SHAPES] = {16, 8, 16, 16};
+  const int q = (shape >= 0 && shape < MG_TILE_SHAPES) ? shape : 0;
+  return dim3((unsigned)((L.nx + tw[q] - 1) / tw[q]), (unsigned)((L.ny + th[q] - 1) / th[q]));
 }
 
-void launch_mg_up(int S, dim3 grid, hipStream_t s, const MgLevel& L, int cnx, double omega,
+void launch_mg_down(int S, int shape, hipStream_t s, const MgLevel& L, const MgLevel& C,
+                    double omega, const double* b, double* xs_out, double* bc,
+                    const PcgScalars* sc) {
+  if (S != 2) shape = 0;
+  const dim3 grid = mg_tile_grid_shape(L, shape);
+  if (S == 1)
+    k_mg_down<1, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else if (S == 3)
+    k_mg_down<3, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else if (shape == 1)
+    k_mg_down<2, double, double, 64, 8><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+  else if (shape == 2)
+    k_mg_down<2, double, double, 16, 16><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc,
+                                                                    sc);
+  else if (shape == 3)
+    k_mg_down<2, double, double, 64, 16><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc,
+                                                                    sc);
+  else
+    k_mg_down<2, double, double><<<grid, MG_THREADS, 0, s>>>(L, C, omega, b, xs_out, bc, sc);
+}
+
+void launch_mg_up(int S, int shape, hipStream_t s, const MgLevel& L, int cnx, double omega,
                   const double* xs_in, const double* ec, const double* b, double* out,
                   const PcgScalars* sc) {
+  if (S != 2) shape = 0;
+  const dim3 grid = mg_tile_grid_shape(L, shape);
   if (S == 1)
     k_mg_up<1, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
                                                                    out, sc);
-  else if (S == 2)
-    k_mg_up<2, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
-                                                                   out, sc);
-  else
+  else if (S == 3)
     k_mg_up<3, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
+                                                                   out, sc);
+  else if (shape == 1)
+    k_mg_up<2, double, double, double, 64, 8><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec,
+                                                                          b, out, sc);
+  else if (shape == 2)
+    k_mg_up<2, double, double, double, 16, 16><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in,
+                                                                           ec, b, out, sc);
+  else if (shape == 3)
+    k_mg_up<2, double, double, double, 64, 16><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in,
+                                                                           ec, b, out, sc);
+  else
+    k_mg_up<2, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
                                                                    out, sc);
 }
 

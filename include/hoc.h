@@ -217,6 +217,19 @@ int hoc_gaspar_lengths_adjoint(const double* e, const double* n2, const double* 
                                double* g_e, double* g_n2, int64_t nz1, int64_t ncol,
                                void* stream);
 
+/* Sync-free stepping (no reference counterpart; for HIP-graph capture of the
+ * step, e.g. an ensemble's): budget > 0 makes every Helmholtz solve launch
+ * exactly `budget` PCG iterations with no host read-back; phases after
+ * convergence return at once, and a solve not converged within the budget
+ * counts as a solver failure (read with hoc_stepper_counters). 0 restores
+ * the polled solve. Theta scheme with pcg_mg or pcg_rbgs. */
+int hoc_stepper_set_sync_free(hoc_stepper* h, int32_t budget);
+
+/* Tile shape of the fused multigrid V-cycle legs of pcg_mg (no reference
+ * counterpart; a launch choice for the autotuner, the results are the same
+ * bits for every shape): 0 32x16 (default), 1 64x8, 2 16x16, 3 64x16 cells. */
+int hoc_stepper_set_mg_tile(hoc_stepper* h, int32_t shape);
+
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */

@@ -1,0 +1,109 @@
+"""Ensembles of uneven members replayed from one HIP graph: every replay
+bitwise equal to the members stepped one by one with polled solves (with
+eager work and synchronisations between the replays), the multigrid leg tile
+shapes bitwise interchangeable, and a too-small solve budget reported as a
+failure rather than passing silently."""
+
+from __future__ import annotations
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FIELDS = ("u", "v", "b", "eta", "T", "S", "e")
+
+
+@pytest.fixture(scope="module")
+def ens(gpu_available):
+    from hydrostatic_ocean_core_benchmark_amd import ensemble
+    return ensemble
+
+
+def _members(ens):
+    return ens.benchmark_members(sizes=(32, 48, 64), per_size=2, nz=30, base_seed=77)
+
+
+def _eager(prob, n_steps):
+    """States after each step of one member, polled solves."""
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    from hydrostatic_ocean_core_benchmark_amd.ensemble import _initial_fields
+    st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+    cur = _initial_fields(prob, st)
+    out = []
+    for _ in range(n_steps):
+        nxt = st._new_state_buffers()
+        st._step_into(cur, nxt)
+        cur = nxt
+        out.append({k: cur[k].clone() for k in FIELDS})
+    return out
+
+
+def test_graph_replays_equal_polled_steps(ens):
+    probs = _members(ens)
+    want = [_eager(p, 4) for p in probs]
+    e = ens.Ensemble(probs)
+    e.autotune(steps=1, repeats=1)
+    e.calibrate(steps=2)
+    e.capture()
+    e.restart()
+    for pair in range(2):
+        e.replay(1)
+        torch.cuda.synchronize()
+        scratch = torch.randn(1 << 20, device="cuda", dtype=torch.float64)  # eager work between
+        assert float(scratch.abs().sum()) > 0.0
+        torch.cuda.synchronize()
+        for i in range(len(probs)):
+            got = e.state(i)
+            for k in FIELDS:
+                assert torch.equal(got[k], want[i][2 * pair + 1][k]), (pair, i, k)
+    assert e.failures() == 0
+    r1 = e.run(4)
+    s1 = [{k: e.state(i)[k].clone() for k in FIELDS} for i in range(len(probs))]
+    r2 = e.run(4)
+    assert r1.solver_failures == 0 and r2.solver_failures == 0
+    for i in range(len(probs)):
+        for k in FIELDS:
+            assert torch.equal(e.state(i)[k], s1[i][k]), (i, k)
+            assert torch.equal(s1[i][k], want[i][3][k]), (i, k)
+
+
+def test_mg_tile_shapes_give_the_same_bits(ens):
+    """Every fused V-cycle leg tile shape: the same state after 2 steps of a
+    96 x 96 x 30 member (not a multiple of 64 wide: partial tiles)."""
+    from hydrostatic_ocean_core_benchmark_amd import _lib
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    from hydrostatic_ocean_core_benchmark_amd.ensemble import _initial_fields
+    prob = basin(nx=96, nz=30, seed=5)
+    results = []
+    for shape in range(len(ens.TILE_SHAPES)):
+        st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+        _lib.check(st.lib.hoc_stepper_set_mg_tile(st._h, shape))
+        cur = _initial_fields(prob, st)
+        for _ in range(2):
+            nxt = st._new_state_buffers()
+            st._step_into(cur, nxt)
+            cur = nxt
+        results.append((cur, st.solver_iterations))
+    ref, it0 = results[0]
+    for cur, it in results[1:]:
+        assert it == it0
+        for k in FIELDS:
+            assert torch.equal(cur[k], ref[k]), k
+
+
+def test_exhausted_budget_is_reported(ens):
+    """A solve budget below the iterations a solve needs leaves the failure
+    counter raised (the state is then not the polled one)."""
+    probs = ens.benchmark_members(sizes=(32,), per_size=1, nz=30, base_seed=3)
+    e = ens.Ensemble(probs)
+    e.calibrate(steps=1)
+    m = e.members[0]
+    m.budget = 2
+    e.capture()
+    rep = e.run(2)
+    assert rep.solver_failures > 0
+    with pytest.raises(ValueError):
+        e.run(3)

@@ -81,6 +81,9 @@ PROTOTYPES = {
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
     "hoc_stepper_set_sync_free": ([VP, i32], i32),
     "hoc_stepper_set_mg_tile": ([VP, i32], i32),
+    "hoc_stepper_momentum_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_eos_buoyancy_adjoint": ([i32, f64, f64, f64, f64, f64, f64, VP, VP, VP, VP, VP, VP, i64,
+                                  VP], i32),
     "hoc_gaspar_lengths": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, i64, i64, VP], i32),
     "hoc_gaspar_lengths_adjoint": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, VP, VP, i64, i64,
                                     VP], i32),

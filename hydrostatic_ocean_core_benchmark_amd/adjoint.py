@@ -5,9 +5,13 @@ Functions with hand-written adjoints, and the checkpointed gradient of a
 scalar of the state after n steps.
 
 Hand-written adjoints (kernels="hip"):
+  equation of state      forward: the HIP EOS; adjoint: dual numbers through
+                         the same expression (hoc_eos_buoyancy_adjoint);
   mixing lengths         forward: the Gaspar budget march on the device
                          (hoc_gaspar_lengths); adjoint: the reverse march per
                          interface (hoc_gaspar_lengths_adjoint);
+  momentum advection     forward: the HIP operator; adjoint: the gathered
+                         reverse kernel (hoc_stepper_momentum_advection_adjoint);
   Thomas solves          forward: the HIP batched Thomas solve; adjoint: the
                          transposed solve on the device (hoc_thomas_adjoint);
   Helmholtz solve        forward: the stepper's PCG (hoc_stepper_helmholtz_solve);
@@ -16,10 +20,10 @@ Hand-written adjoints (kernels="hip"):
                          coefficient gradients of -lambda^T (dA/dK) x;
   tracer advection       forward: the HIP operator; adjoint: the gathered
                          reverse kernel (hoc_stepper_tracer_advection_adjoint).
-Everything else of the step (the rest of the closure, EOS, momentum advection, Coriolis,
+Everything else of the step (the rest of the closure, Coriolis,
 pressure gradients, the Picard loop) is differentiated by autograd through
-its torch operations. kernels="torch" replaces the four Functions by torch
-operations (the march, a sequential Thomas sweep, a dense solve of the
+its torch operations. kernels="torch" replaces the six Functions by torch
+operations (the momentum advection formula, the EOS polynomial, the march, a sequential Thomas sweep, a dense solve of the
 Helmholtz operator, the advection formula), so autograd differentiates the whole step:
 the reference gradient the hand-written adjoints are checked against, on
 grids small enough for the dense solve.
@@ -365,6 +369,65 @@ class _GasparHip(torch.autograd.Function):
         return g_e, g_n2, None, None, None, None, None, None, None, None
 
 
+class _MomentumAdvectionHip(torch.autograd.Function):
+    """(au, av) of the kappa = 1/3 momentum advection: the HIP operator
+    forward, the gathered reverse kernel
+    (hoc_stepper_momentum_advection_adjoint) backward."""
+
+    @staticmethod
+    def forward(ctx, u, v, w, owner):
+        u, v, w = u.contiguous(), v.contiguous(), w.contiguous()
+        au, av = torch.empty_like(u), torch.empty_like(v)
+        _lib.check(owner.lib.hoc_stepper_momentum_advection(owner._h, _ptr(u), _ptr(v), _ptr(w),
+                                                            _ptr(au), _ptr(av),
+                                                            owner._stream()))
+        ctx.save_for_backward(u, v, w)
+        ctx.owner = owner
+        return au, av
+
+    @staticmethod
+    def backward(ctx, g_au, g_av):
+        u, v, w = ctx.saved_tensors
+        owner = ctx.owner
+        g_au = torch.zeros_like(u) if g_au is None else g_au.contiguous()
+        g_av = torch.zeros_like(v) if g_av is None else g_av.contiguous()
+        gu, gv, gw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+        _lib.check(owner.lib.hoc_stepper_momentum_advection_adjoint(
+            owner._h, _ptr(u), _ptr(v), _ptr(w), _ptr(g_au), _ptr(g_av), _ptr(gu), _ptr(gv),
+            _ptr(gw), owner._stream()))
+        return gu, gv, gw, None
+
+
+_EOS_KIND = {"linear": 0, "seos": 1, "teos10": 2}
+
+
+class _EosHip(torch.autograd.Function):
+    """b(T, S) at fixed z: the HIP EOS forward, its derivative by dual numbers
+    through the same expression (hoc_eos_buoyancy_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, T, S, z, phys, lib, stream):
+        T, S, z = T.contiguous(), S.contiguous(), z.contiguous()
+        b = torch.empty_like(T)
+        args = (_EOS_KIND[phys.eos], phys.g, phys.rho0, phys.alpha_T, phys.beta_S, phys.T0,
+                phys.S0)
+        _lib.check(lib.hoc_eos_buoyancy(*args, _ptr(T), _ptr(S), _ptr(z), _ptr(b), T.numel(),
+                                        stream()))
+        ctx.save_for_backward(T, S, z)
+        ctx.args = (args, lib, stream)
+        return b
+
+    @staticmethod
+    def backward(ctx, g):
+        T, S, z = ctx.saved_tensors
+        args, lib, stream = ctx.args
+        g = g.contiguous()
+        gT, gS = torch.empty_like(T), torch.empty_like(S)
+        _lib.check(lib.hoc_eos_buoyancy_adjoint(*args, _ptr(T), _ptr(S), _ptr(z), _ptr(g),
+                                                _ptr(gT), _ptr(gS), T.numel(), stream()))
+        return gT, gS, None, None, None, None
+
+
 class _HelmholtzHip(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Ku, Kv, rhs, op):
@@ -619,6 +682,9 @@ class DiffStep:
 
     def buoyancy(self, T, S):
         p = self.physics
+        if self.kernels == "hip":
+            st = self.owner
+            return _EosHip.apply(T, S, self.zc, p, st.lib, st._stream)
         return buoyancy(p.eos, T, S, self.zc, g=p.g, rho0=p.rho0, alpha_T=p.alpha_T,
                         beta_S=p.beta_S, T0=p.T0, S0=p.S0)
 
@@ -741,8 +807,11 @@ class DiffStep:
         dzu_diff = self.apply_diffusion(s.u, Km, self.dz3u, self.tau_u, p.bottom_drag)
         dzv_diff = self.apply_diffusion(s.v, Km, self.dz3v, self.tau_v, p.bottom_drag)
         w_old = self.w_diag(s.u, s.v)
-        au, av = momentum_advection_up3(s.u, s.v, w_old, g.dx, g.dy, g.dz, m3u, m3v,
-                                        pr.advection == "up3_tvd")
+        if self.kernels == "hip":
+            au, av = _MomentumAdvectionHip.apply(s.u, s.v, w_old, self.owner)
+        else:
+            au, av = momentum_advection_up3(s.u, s.v, w_old, g.dx, g.dy, g.dz, m3u, m3v,
+                                            pr.advection == "up3_tvd")
         ex_u = -(au * m3u)
         ex_v = -(av * m3v)
         if p.A_h > 0.0:

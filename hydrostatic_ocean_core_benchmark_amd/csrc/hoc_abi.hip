@@ -1480,6 +1480,23 @@ int hoc_gaspar_lengths_adjoint(const double* e, const double* n2, const double* 
   return HOC_OK;
 }
 
+int hoc_stepper_momentum_advection_adjoint(hoc_stepper* h, const double* u, const double* v,
+                                           const double* w, const double* g_au,
+                                           const double* g_av, double* g_u, double* g_v,
+                                           double* g_w, void* stream) {
+  if (!h || !u || !v || !w || !g_au || !g_av || !g_u || !g_v || !g_w)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (h->p.advection != ADV_UP3 && h->p.advection != ADV_UP3_TVD)
+    return fail(HOC_ERR_UNSUPPORTED, "the advection adjoint covers up3 and up3_tvd");
+  if (h->d.nx < 4 || h->d.ny < 4)
+    return fail(HOC_ERR_UNSUPPORTED, "the advection adjoint needs nx, ny >= 4");
+  const Dims d = h->d;
+  k_momentum_advection_adjoint<<<blocks(d.n3), TPB, 0, (hipStream_t)stream>>>(
+      d, h->p, h->geo, u, v, w, g_au, g_av, g_u, g_v, g_w);
+  LAUNCH_CHECK("k_momentum_advection_adjoint");
+  return HOC_OK;
+}
+
 int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream) {
   if (!h || !a || !b) return fail(HOC_ERR_INVALID, "null argument");
   if (n < 0) return fail(HOC_ERR_INVALID, "n must be >= 0");
@@ -1510,6 +1527,29 @@ int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double 
   p.neg_g = -g;
   k_eos<<<blocks(n), TPB, 0, (hipStream_t)stream>>>(n, p, T, S, z, b);
   LAUNCH_CHECK("k_eos");
+  return HOC_OK;
+}
+
+int hoc_eos_buoyancy_adjoint(int32_t eos, double g, double rho0, double alpha_T, double beta_S,
+                             double T0, double S0, const double* T, const double* S,
+                             const double* z, const double* g_b, double* g_T, double* g_S,
+                             int64_t n, void* stream) {
+  if (!T || !S || !z || !g_b || !g_T || !g_S) return fail(HOC_ERR_INVALID, "null argument");
+  if (eos < 0 || eos > 2) return fail(HOC_ERR_INVALID, "unknown eos (expected linear|seos|teos10)");
+  if (n <= 0) return HOC_OK;
+  StepParams p;
+  std::memset(&p, 0, sizeof(p));
+  p.eos = eos;
+  p.g = g;
+  p.rho0 = rho0;
+  p.alpha_T = alpha_T;
+  p.beta_S = beta_S;
+  p.T0 = T0;
+  p.S0 = S0;
+  p.neg_alpha_T = -alpha_T;
+  p.neg_g = -g;
+  k_eos_adjoint<<<blocks(n), TPB, 0, (hipStream_t)stream>>>(n, p, T, S, z, g_b, g_T, g_S);
+  LAUNCH_CHECK("k_eos_adjoint");
   return HOC_OK;
 }
 

@@ -368,3 +368,182 @@ __global__ void k_gaspar_lengths_adjoint(long long ncol, int nz1, double e_min, 
 }
 
 }  // namespace hoc
+
+namespace hoc {
+
+// ------------------------------------------------------------ EOS adjoint
+// Forward-mode dual number (value, derivative) for the templated EOS
+// (buoyancy_t, rho_teos10 in hoc_device.h): one evaluation per seeded input
+// gives the exact derivative of the evaluated expression.
+struct Dual {
+  double v, d;
+  __device__ Dual() : v(0.0), d(0.0) {}
+  __device__ Dual(double x) : v(x), d(0.0) {}
+  __device__ Dual(double x, double dx) : v(x), d(dx) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) {
+  return Dual(a.v * b.v, a.d * b.v + a.v * b.d);
+}
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ Dual sqrt(Dual a) {
+  const double r = ::sqrt(a.v);
+  return Dual(r, r > 0.0 ? a.d / (2.0 * r) : 0.0);
+}
+
+// dL/dT, dL/dS of L = sum(g_b * b(T, S, z)).
+__global__ void k_eos_adjoint(long long n, StepParams p, const double* __restrict__ T,
+                              const double* __restrict__ S, const double* __restrict__ z,
+                              const double* __restrict__ g_b, double* __restrict__ g_T,
+                              double* __restrict__ g_S) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const Dual zz(z[t]);
+  const Dual dT = buoyancy_t<Dual>(p, Dual(T[t], 1.0), Dual(S[t]), zz);
+  const Dual dS = buoyancy_t<Dual>(p, Dual(T[t]), Dual(S[t], 1.0), zz);
+  g_T[t] = g_b[t] * dT.d;
+  g_S[t] = g_b[t] * dS.d;
+}
+
+}  // namespace hoc
+
+namespace hoc {
+
+// ------------------------------------------------ momentum advection adjoint
+// Gradients of L = sum(Gu * au + Gv * av) for the kappa = 1/3 momentum
+// advection of libs/core/advection_v06.py:77-108 (the operator of
+// hoc_stepper_momentum_advection, before masking), gathered at one point
+// (k, j, i): gu, gv at its u and v points and gw at interface k. For the
+// u equation the x fluxes live at cell centres c (Fx = ub uc, ub the mean of
+// u at c-1 and c, uc the face value from u[c-2..c+1]), the y fluxes at
+// corners (Fy = vb uy, vb the mean of v at i and i+1, uy from u[j-1..j+2])
+// and the vertical fluxes at interfaces (Fz = wb uz, wb the mean of w at i
+// and i+1); the v equation is the transpose of that layout. Every flux
+// enters its two neighbours' divergences with opposite signs, which gives
+// the flux adjoints (Fbar) the gather reads.
+__global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
+                                             const double* __restrict__ u,
+                                             const double* __restrict__ v,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ Gu,
+                                             const double* __restrict__ Gv,
+                                             double* __restrict__ gu, double* __restrict__ gv,
+                                             double* __restrict__ gw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long n2 = d.n2;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int k = (int)(t / n2);
+  const long long c = t % n2;
+  const int j = (int)(c / nx), i = (int)(c % nx);
+  const long long o = (long long)k * n2;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  auto at = [&](int jj, int ii) -> long long {
+    return (long long)pwrap(jj, ny) * nx + pwrap(ii, nx);
+  };
+  auto U = [&](int jj, int ii) { return u[o + at(jj, ii)]; };
+  auto V = [&](int jj, int ii) { return v[o + at(jj, ii)]; };
+  auto MU = [&](int jj, int ii) { return geo.m3u[o + at(jj, ii)] != 0; };
+  auto MV = [&](int jj, int ii) { return geo.m3v[o + at(jj, ii)] != 0; };
+  auto GU = [&](int jj, int ii) { return Gu[o + at(jj, ii)]; };
+  auto GV = [&](int jj, int ii) { return Gv[o + at(jj, ii)]; };
+  double gU = 0.0, gV = 0.0, dp[4];
+  // ---------------- u equation
+  for (int s = -1; s <= 2; ++s) {  // x flux at centre cc = i + s
+    const int cc = i + s;
+    const double ub = 0.5 * (U(j, cc - 1) + U(j, cc));
+    const double uc = kappa_face_d(U(j, cc - 2), U(j, cc - 1), U(j, cc), U(j, cc + 1), ub,
+                                   MU(j, cc - 2), MU(j, cc + 1), lim, dp);
+    const double fbar = (GU(j, cc - 1) - GU(j, cc)) / p.dx;
+    gU += fbar * ub * dp[2 - s];
+    if (s == 0 || s == 1) gU += fbar * 0.5 * uc;
+  }
+  for (int s = -2; s <= 1; ++s) {  // y flux at corner jc = j + s
+    const int jc = j + s;
+    const double vb = 0.5 * (V(jc, i) + V(jc, i + 1));
+    kappa_face_d(U(jc - 1, i), U(jc, i), U(jc + 1, i), U(jc + 2, i), vb, MU(jc - 1, i),
+                 MU(jc + 2, i), lim, dp);
+    const double fbar = (GU(jc, i) - GU(jc + 1, i)) / p.dy;
+    gU += fbar * vb * dp[1 - s];
+  }
+  for (int s = -1; s <= 0; ++s) {  // v enters the corners (j, i - 1) and (j, i)
+    const int ii = i + s;
+    const double vb = 0.5 * (V(j, ii) + V(j, ii + 1));
+    const double uy = kappa_face_d(U(j - 1, ii), U(j, ii), U(j + 1, ii), U(j + 2, ii), vb,
+                                   MU(j - 1, ii), MU(j + 2, ii), lim, dp);
+    gV += (GU(j, ii) - GU(j + 1, ii)) / p.dy * uy * 0.5;
+  }
+  // ---------------- v equation
+  for (int s = -1; s <= 2; ++s) {  // y flux at centre cc = j + s
+    const int cc = j + s;
+    const double vb = 0.5 * (V(cc - 1, i) + V(cc, i));
+    const double vc = kappa_face_d(V(cc - 2, i), V(cc - 1, i), V(cc, i), V(cc + 1, i), vb,
+                                   MV(cc - 2, i), MV(cc + 1, i), lim, dp);
+    const double fbar = (GV(cc - 1, i) - GV(cc, i)) / p.dy;
+    gV += fbar * vb * dp[2 - s];
+    if (s == 0 || s == 1) gV += fbar * 0.5 * vc;
+  }
+  for (int s = -2; s <= 1; ++s) {  // x flux at corner ic = i + s
+    const int ic = i + s;
+    const double ub = 0.5 * (U(j, ic) + U(j + 1, ic));
+    kappa_face_d(V(j, ic - 1), V(j, ic), V(j, ic + 1), V(j, ic + 2), ub, MV(j, ic - 1),
+                 MV(j, ic + 2), lim, dp);
+    const double fbar = (GV(j, ic) - GV(j, ic + 1)) / p.dx;
+    gV += fbar * ub * dp[1 - s];
+  }
+  for (int s = -1; s <= 0; ++s) {  // u enters the corners (j - 1, i) and (j, i)
+    const int jj = j + s;
+    const double ub = 0.5 * (U(jj, i) + U(jj + 1, i));
+    const double vx = kappa_face_d(V(jj, i - 1), V(jj, i), V(jj, i + 1), V(jj, i + 2), ub,
+                                   MV(jj, i - 1), MV(jj, i + 2), lim, dp);
+    gU += (GV(jj, i) - GV(jj, i + 1)) / p.dx * vx * 0.5;
+  }
+  // ---------------- vertical fluxes (interfaces 1..nz-1 carry them)
+  // face value and partials at interface kk of column col for field f
+  auto zface = [&](const double* f, const uint8_t* m3, int kk, long long col, double wb,
+                   double* pd) -> double {
+    const long long a = (long long)kk * n2 + col;
+    const double far_up = kk >= 2 ? f[a - 2 * n2] : 0.0;
+    const double far_dn = kk <= nz - 2 ? f[a + n2] : 0.0;
+    const bool hu = kk >= 2 && m3[a - 2 * n2] != 0;
+    const bool hd = kk <= nz - 2 && m3[a + n2] != 0;
+    return kappa_face_d(far_dn, f[a], f[a - n2], far_up, wb, hd, hu, lim, pd);
+  };
+  const long long ce = at(j, i + 1), cn = at(j + 1, i);
+  for (int kk = k - 1; kk <= k + 2; ++kk) {
+    if (kk < 1 || kk > nz - 1) continue;
+    const long long a = (long long)kk * n2;
+    const int role = kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3;
+    const double wbu = 0.5 * (w[a + c] + w[a + ce]);
+    zface(u, geo.m3u, kk, c, wbu, dp);
+    gU += (Gu[a + c] - Gu[a - n2 + c]) / p.dz_nom * wbu * dp[role];
+    const double wbv = 0.5 * (w[a + c] + w[a + cn]);
+    zface(v, geo.m3v, kk, c, wbv, dp);
+    gV += (Gv[a + c] - Gv[a - n2 + c]) / p.dz_nom * wbv * dp[role];
+  }
+  double gW = 0.0;
+  if (k >= 1) {  // interface k: w enters wb of the u columns i-1, i and v rows j-1, j
+    const long long a = (long long)k * n2;
+    for (int s = -1; s <= 0; ++s) {
+      const long long cu = at(j, i + s), cue = at(j, i + s + 1);
+      const double wbu = 0.5 * (w[a + cu] + w[a + cue]);
+      const double fu = zface(u, geo.m3u, k, cu, wbu, dp);
+      gW += (Gu[a + cu] - Gu[a - n2 + cu]) / p.dz_nom * fu * 0.5;
+      const long long cv = at(j + s, i), cvn = at(j + s + 1, i);
+      const double wbv = 0.5 * (w[a + cv] + w[a + cvn]);
+      const double fv = zface(v, geo.m3v, k, cv, wbv, dp);
+      gW += (Gv[a + cv] - Gv[a - n2 + cv]) / p.dz_nom * fv * 0.5;
+    }
+  }
+  gu[t] = gU;
+  gv[t] = gV;
+  gw[t] = gW;
+  if (k == nz - 1) gw[(long long)nz * n2 + c] = 0.0;
+}
+
+}  // namespace hoc

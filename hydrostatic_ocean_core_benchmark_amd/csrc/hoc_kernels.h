@@ -153,6 +153,13 @@ __global__ void k_gaspar_lengths_adjoint(long long ncol, int nz1, double e_min, 
                                          const double* g_up, const double* g_dn, double* g_e,
                                          double* g_n2);
 
+__global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo, const double* u,
+                                             const double* v, const double* w, const double* Gu,
+                                             const double* Gv, double* gu, double* gv,
+                                             double* gw);
+__global__ void k_eos_adjoint(long long n, StepParams p, const double* T, const double* S,
+                              const double* z, const double* g_b, double* g_T, double* g_S);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

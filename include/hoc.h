@@ -230,6 +230,23 @@ int hoc_stepper_set_sync_free(hoc_stepper* h, int32_t budget);
  * bits for every shape): 0 32x16 (default), 1 64x8, 2 16x16, 3 64x16 cells. */
 int hoc_stepper_set_mg_tile(hoc_stepper* h, int32_t shape);
 
+/* Adjoint of hoc_eos_buoyancy (no reference counterpart): g_T = g_b db/dT,
+ * g_S = g_b db/dS at n points, the derivatives of the evaluated EOS
+ * expression (forward-mode dual numbers through the same formula). */
+int hoc_eos_buoyancy_adjoint(int32_t eos, double g, double rho0, double alpha_T, double beta_S,
+                             double T0, double S0, const double* T, const double* S,
+                             const double* z, const double* g_b, double* g_T, double* g_S,
+                             int64_t n, void* stream);
+
+/* Adjoint of hoc_stepper_momentum_advection (advection_v06.py:77-108; no
+ * reference counterpart): from the adjoints g_au, g_av of the two advection
+ * terms, dL/du, dL/dv ([nz][ny][nx]) and dL/dw ([nz+1][ny][nx]), the upwind
+ * and limiter branches held at their forward values. up3 / up3_tvd. */
+int hoc_stepper_momentum_advection_adjoint(hoc_stepper* h, const double* u, const double* v,
+                                           const double* w, const double* g_au,
+                                           const double* g_av, double* g_u, double* g_v,
+                                           double* g_w, void* stream);
+
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */

@@ -69,6 +69,34 @@ def test_thomas_adjoint_against_autograd(adj):
     assert float(got[0][0].abs().max()) == 0.0 and float(got[2][-1].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("eos", ["linear", "seos", "teos10"])
+def test_eos_adjoint_against_autograd(adj, eos):
+    """hoc_eos_buoyancy_adjoint against autograd through the torch EOS on
+    random (T, S, z) of ocean range (1e-12)."""
+    from hydrostatic_ocean_core_benchmark_amd import _lib
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=4000.0, rho0=1025.0, alpha_T=2e-4,
+                                     beta_S=7.4e-4, T0=10.0, S0=35.0), eos=eos)
+    gen = torch.Generator().manual_seed(4)
+    n = 50_000
+    T = (2.0 + 25.0 * torch.rand(n, generator=gen, dtype=torch.float64)).cuda()
+    S = (33.0 + 4.0 * torch.rand(n, generator=gen, dtype=torch.float64)).cuda()
+    z = (4000.0 * torch.rand(n, generator=gen, dtype=torch.float64)).cuda()
+    w = torch.randn(n, generator=gen, dtype=torch.float64).cuda()
+    lib = _lib.load()
+    stream = lambda: torch.cuda.current_stream().cuda_stream
+    T1, S1 = T.clone().requires_grad_(), S.clone().requires_grad_()
+    b = adj._EosHip.apply(T1, S1, z, phys, lib, stream)
+    got = torch.autograd.grad((b * w).sum(), [T1, S1])
+    T2, S2 = T.clone().requires_grad_(), S.clone().requires_grad_()
+    b2 = adj.buoyancy(eos, T2, S2, z, g=phys.g, rho0=phys.rho0, alpha_T=phys.alpha_T,
+                      beta_S=phys.beta_S, T0=phys.T0, S0=phys.S0)
+    assert _rel(b.detach(), b2.detach()) < 1e-13
+    want = torch.autograd.grad((b2 * w).sum(), [T2, S2])
+    assert _rel(got[0], want[0]) < 1e-11, _rel(got[0], want[0])
+    assert _rel(got[1], want[1]) < 1e-11, _rel(got[1], want[1])
+
+
 def test_mixing_length_adjoint_against_autograd(adj):
     """hoc_gaspar_lengths and its adjoint against the torch march and
     autograd through it, on the closure inputs of a state two steps into
@@ -146,6 +174,34 @@ def test_tracer_advection_adjoint_against_autograd(adj, advection):
     assert _rel(a.detach(), a2.detach()) < 1e-13
     want = torch.autograd.grad((a2 * G).sum(), ins2)
     for name, x, y in zip("cuvw", got, want):
+        assert _rel(x, y) < 1e-11, (name, _rel(x, y))
+
+
+@pytest.mark.parametrize("advection", ["up3_tvd", "up3"])
+def test_momentum_advection_adjoint_against_autograd(adj, advection):
+    """The reverse momentum-advection kernel against autograd through the
+    advection formula, on a state two steps into the 32 x 32 x 40 basin."""
+    prob = _basin(32, advection=advection)
+    kb = _kappa(prob)
+    hip = adj.DiffStep(prob.domain, prob.physics, prob.params, prob.dt, kernels="hip")
+    g = prob.domain.grid
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    with torch.no_grad():
+        s = hip.initial_state(T0, S0, kb)
+        for _ in range(2):
+            s = hip.step(s, kb)
+        w = hip.w_diag(s.u, s.v)
+    ins = [t.clone().requires_grad_() for t in (s.u, s.v, w)]
+    Gu, Gv = torch.randn_like(s.u), torch.randn_like(s.v)
+    au, av = adj._MomentumAdvectionHip.apply(*ins, hip.owner)
+    got = torch.autograd.grad((au * Gu).sum() + (av * Gv).sum(), ins)
+    ins2 = [t.clone().requires_grad_() for t in (s.u, s.v, w)]
+    au2, av2 = adj.momentum_advection_up3(*ins2, g.dx, g.dy, g.dz, hip.mask3u, hip.mask3v,
+                                          advection == "up3_tvd")
+    assert _rel(au.detach(), au2.detach()) < 1e-13 and _rel(av.detach(), av2.detach()) < 1e-13
+    want = torch.autograd.grad((au2 * Gu).sum() + (av2 * Gv).sum(), ins2)
+    for name, x, y in zip("uvw", got, want):
         assert _rel(x, y) < 1e-11, (name, _rel(x, y))
 
 

@@ -92,6 +92,8 @@ def parse():
     ap.add_argument("--solver", default=None,
                     choices=("pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid", "pcg_mg"),
                     help="Helmholtz solver kind (default: the workload's)")
+    ap.add_argument("--mg-levels", type=int, default=None, help="pcg_mg levels (default: the workload's)")
+    ap.add_argument("--mg-sweeps", type=int, default=None, help="pcg_mg sweeps per side")
     ap.add_argument("--mg-tile", type=int, default=None,
                     help="fused V-cycle leg tile shape 0..3 (32x16, 64x8, 16x16, 64x16); "
                          "default: the library's")
@@ -283,6 +285,10 @@ def main() -> int:
     prob = basin(nx=args.nx, nz=args.nz, dx=args.dx, dt=args.dt)
     if args.solver:
         prob.params.solver = args.solver
+    if args.mg_levels is not None:
+        prob.params.mg_levels = args.mg_levels
+    if args.mg_sweeps is not None:
+        prob.params.mg_sweeps = args.mg_sweeps
     g = prob.domain.grid
     st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=args.precision)
     st.solver_path = args.solver_path

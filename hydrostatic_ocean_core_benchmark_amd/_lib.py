@@ -39,6 +39,16 @@ class HocConfig(C.Structure):
     ]
 
 
+class HocSw2dConfig(C.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32),
+        ("dx", f64), ("dy", f64), ("g", f64), ("f", f64), ("H", f64), ("dt", f64),
+        ("scheme", i32),
+        ("theta", f64), ("theta_cor", f64), ("rtol", f64),
+        ("max_iter", i32),
+    ]
+
+
 class HocGeometry(C.Structure):
     _fields_ = [
         ("mask", P_u8), ("mask_u", P_u8), ("mask_v", P_u8),
@@ -81,6 +91,13 @@ PROTOTYPES = {
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
     "hoc_stepper_set_sync_free": ([VP, i32], i32),
     "hoc_stepper_set_mg_tile": ([VP, i32], i32),
+    "hoc_sw2d_create": ([VP, VP], i32),
+    "hoc_sw2d_destroy": ([VP], i32),
+    "hoc_sw2d_momentum": ([VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_sw2d_rhs": ([VP, VP, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_sw2d_backsub": ([VP, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_sw2d_fb_eta": ([VP, VP, VP, VP, VP, VP], i32),
+    "hoc_sw2d_pcg": ([VP, VP, VP, C.POINTER(i32), C.POINTER(i32), VP], i32),
     "hoc_stepper_momentum_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_eos_buoyancy_adjoint": ([i32, f64, f64, f64, f64, f64, f64, VP, VP, VP, VP, VP, VP, i64,
                                   VP], i32),

@@ -4,7 +4,15 @@ subcommands with `--case/--nx/--cfl/--steps`).
 
     python main.py [--config config] [--set key=value ...] run    [--case C] [--nx N] [--cfl X] [--steps K]
     python main.py ... verify --against PREFIX_state3d5.bin [--steps K]
+    python main.py ... verify --case igw [--study space|time]
     python main.py ... bench  [--sweep 128,256,512] [--steps K] [--repeat R]
+
+With --case igw | igw_broadband | geo_balance the subcommands drive the 2D
+shallow-water model instead, as the reference's own run|verify|bench do
+(main.py:45-275): run integrates one case and reports the error against its
+exact solution, mass and energy drift; verify --study space|time is the
+convergence study (observed order between refinements); bench is the
+error-vs-wallclock frontier over the CFL sweep.
 
 run     integrate one case (lock_exchange or basin) and write the final state
         (the bundle's `_state3d5.bin` layout) and a metrics JSON into
@@ -73,6 +81,72 @@ def build_problem(config, case: str | None, nx: int | None, cfl: float | None,
     return prob
 
 
+CASES_2D = ("igw", "igw_broadband", "geo_balance")
+
+
+def simulate2d(config, nx: int, cfl: float | None, case_name: str,
+               n_steps_override: int | None = None) -> dict:
+    """One 2D run (main.py:45-98): dt from the CFL of the forward-backward
+    limit (or pinned by time.dt), n_steps to land on the case's t_final,
+    metrics against the exact solution."""
+    from . import cases2d as C2
+    from .domain import CGrid
+    from .model2d import SWStepper, State, explicit_dt_max, physics_from_config
+    from .types import SchemeParams
+    grid = CGrid(nx=nx, ny=nx, Lx=float(config.get("grid.Lx")), Ly=float(config.get("grid.Ly")))
+    physics = physics_from_config(config)
+    params = SchemeParams(name=str(config.get("scheme.name")),
+                          theta=float(config.get("scheme.theta")),
+                          theta_cor=float(config.get("scheme.theta_cor")),
+                          n_picard=int(config.get("scheme.n_picard")),
+                          solver=str(config.get("solver.kind")),
+                          rtol=float(config.get("solver.rtol")),
+                          max_iter=int(config.get("solver.max_iter")))
+    case, spec = C2.build_case(case_name, grid, physics, config)
+    dt_max = explicit_dt_max(grid, physics)
+    pinned = float(config.get("time.dt", 0.0))
+    dt_req = pinned if pinned > 0.0 else (cfl if cfl is not None
+                                          else float(config.get("time.cfl_factor"))) * dt_max
+    t_final = spec.t_final
+    if n_steps_override is not None:
+        n_steps, dt = int(n_steps_override), dt_req
+        t_final = dt * n_steps
+    else:
+        n_steps = max(1, int(round(t_final / dt_req)))
+        dt = t_final / n_steps
+    import torch
+    st = SWStepper(grid, physics, params, dt)
+    s0 = case.initial()
+    m0, e0 = C2.total_mass(s0, grid), C2.total_energy(s0, grid, physics)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fin = st.integrate(s0, n_steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    final = State(u=fin.u.cpu().numpy(), v=fin.v.cpu().numpy(), eta=fin.eta.cpu().numpy(),
+                  t=fin.t)
+    exact = case.exact(t_final)
+    metrics = C2.summarize(final, exact, grid, physics, m0, e0)
+    if case_name == "igw":
+        metrics.update(C2.wave_errors(final, exact, grid, case.k, case.l))
+    metrics.update({"nx": grid.nx, "ny": grid.ny, "dx": grid.dx, "dt": dt,
+                    "dt_max_explicit": dt_max, "cfl": dt / dt_max, "n_steps": n_steps,
+                    "t_final": t_final, "scheme": params.name, "theta": params.theta,
+                    "n_picard": params.n_picard, "solver": params.solver,
+                    "solver_iterations": st.solver_iterations,
+                    "solver_failures": st.solver_failures, "wall_s": wall})
+    return {"state": final, "exact": exact, "metrics": metrics, "params": params}
+
+
+def _order(errors, resolution):
+    """Observed order between successive refinements (main.py:101-118)."""
+    out = [float("nan")]
+    for e0, e1, r0, r1 in zip(errors[:-1], errors[1:], resolution[:-1], resolution[1:]):
+        ok = e0 > 0.0 and e1 > 0.0 and r0 > 0.0 and r1 > 0.0 and r1 != r0
+        out.append(float(np.log(e0 / e1) / np.log(r1 / r0)) if ok else float("nan"))
+    return out
+
+
 def _run_dir(config, tag: str) -> Path:
     stamp = datetime.now().strftime("%Y%m%d-%H%M%S")
     path = Path(str(config.get("paths.output_dir", "output"))) / f"{tag}_{stamp}"
@@ -115,7 +189,85 @@ def _host_state(final) -> dict:
     return out
 
 
+def cmd_run2d(config, args, case_name) -> int:
+    res = simulate2d(config, args.nx or int(config.get("grid.nx")), args.cfl, case_name,
+                     n_steps_override=args.steps)
+    m = res["metrics"]
+    out = _run_dir(config, f"run_{case_name}_{m['scheme']}")
+    np.savez(out / "state2d.npz", u=res["state"].u, v=res["state"].v, eta=res["state"].eta)
+    (out / "metrics.json").write_text(json.dumps(m, indent=2) + "\n", encoding="utf-8")
+    print(f"case={case_name} nx={m['nx']} scheme={m['scheme']} theta={m['theta']} "
+          f"dt={m['dt']:.2f}s steps={m['n_steps']} solver={m['solver']} "
+          f"iters={m['solver_iterations']}")
+    print(f"L2(eta)={m['l2_rel_eta']:.4e}  mass drift={m['mass_drift']:+.3e}  "
+          f"energy drift={m['energy_drift']:+.3e}  -> {out}")
+    return 0
+
+
+def cmd_verify2d(config, args, case_name) -> int:
+    if args.study == "space":
+        sizes = list(config.get("grid.sweep.nx_list"))
+        rows = [simulate2d(config, int(n), args.cfl, case_name)["metrics"] for n in sizes]
+        errors = [r["l2_rel_eta"] for r in rows]
+        resolution = [float(r["nx"]) for r in rows]
+        label, axis = "nx", [float(r["nx"]) for r in rows]
+    else:
+        nx = args.nx or int(config.get("grid.nx"))
+        cfls = sorted(config.get("time.sweep.cfl_list"), reverse=True)
+        ref = simulate2d(config, nx, min(cfls) / 8.0, case_name)["state"]
+        rows, errors = [], []
+        for cfl in cfls:
+            res = simulate2d(config, nx, cfl, case_name)
+            err = float(np.sqrt(np.sum((res["state"].eta - ref.eta) ** 2))
+                        / np.sqrt(np.sum(ref.eta ** 2)))
+            res["metrics"]["l2_rel_eta_vs_reference"] = err
+            rows.append(res["metrics"])
+            errors.append(err)
+        resolution = [1.0 / r["dt"] for r in rows]
+        label, axis = "1/cfl", [1.0 / r["cfl"] for r in rows]
+    orders = _order(errors, resolution)
+    print(f"--- convergence study: {args.study}, case={case_name}, scheme={rows[0]['scheme']}, "
+          f"theta={rows[0]['theta']}, n_picard={rows[0]['n_picard']} ---")
+    print(f"{label:>8} {'dx [km]':>10} {'dt [s]':>10} {'cfl':>7} {'L2(eta)':>12} {'order':>7} "
+          f"{'amp_ratio':>10} {'phase[rad]':>11}")
+    for row, err, order, ax in zip(rows, errors, orders, axis):
+        row["observed_order"] = order
+        print(f"{ax:>8.3g} {row['dx'] / 1000:>10.2f} {row['dt']:>10.2f} {row['cfl']:>7.3f} "
+              f"{err:>12.4e} {order:>7.2f} {row.get('amp_ratio', float('nan')):>10.6f} "
+              f"{row.get('phase_err_rad', float('nan')):>11.3e}")
+    out = _run_dir(config, f"verify_{args.study}_{case_name}")
+    (out / "metrics.json").write_text(json.dumps({"study": args.study, "case": case_name,
+                                                  "rows": rows}, indent=2) + "\n",
+                                      encoding="utf-8")
+    return 0
+
+
+def cmd_bench2d(config, args, case_name) -> int:
+    nx = args.nx or int(config.get("grid.nx"))
+    rows = []
+    print(f"{'cfl':>7} {'dt [s]':>10} {'steps':>7} {'L2(eta)':>12} {'wall [s]':>10} {'us/step':>9}")
+    for cfl in config.get("time.sweep.cfl_list"):
+        m = simulate2d(config, nx, float(cfl), case_name, n_steps_override=args.steps)["metrics"]
+        m["us_per_step"] = 1e6 * m["wall_s"] / m["n_steps"]
+        rows.append(m)
+        print(f"{cfl:>7.3f} {m['dt']:>10.2f} {m['n_steps']:>7d} {m['l2_rel_eta']:>12.4e} "
+              f"{m['wall_s']:>10.4f} {m['us_per_step']:>9.1f}")
+    out = _run_dir(config, f"bench_{case_name}")
+    (out / "metrics.json").write_text(json.dumps({"mode": "bench", "case": case_name, "nx": nx,
+                                                  "rows": rows}, indent=2) + "\n",
+                                      encoding="utf-8")
+    return 0
+
+
+def _case_2d(config, args):
+    name = args.case or str(config.get("case.name", "lock_exchange"))
+    return name if name in CASES_2D else None
+
+
 def cmd_run(config, args) -> int:
+    c2 = _case_2d(config, args)
+    if c2:
+        return cmd_run2d(config, args, c2)
     from .bundle import write_state_file
     prob = build_problem(config, args.case, args.nx, args.cfl, args.steps)
     st, final, wall, iters = _integrate(prob)
@@ -138,6 +290,11 @@ def cmd_run(config, args) -> int:
 
 
 def cmd_verify(config, args) -> int:
+    c2 = _case_2d(config, args)
+    if c2:
+        return cmd_verify2d(config, args, c2)
+    if not args.against:
+        raise SystemExit("verify of a 3D case needs --against PREFIX_state3d5.bin")
     from .bundle import read_state_file
     prob = build_problem(config, args.case, args.nx, args.cfl, args.steps)
     _, final, _, _ = _integrate(prob)
@@ -158,6 +315,9 @@ def cmd_verify(config, args) -> int:
 
 
 def cmd_bench(config, args) -> int:
+    c2 = _case_2d(config, args)
+    if c2:
+        return cmd_bench2d(config, args, c2)
     sizes = [int(n) for n in (args.sweep or "128,256,512").split(",")]
     print(f"{'nx':>6}{'nz':>5}{'cells':>11}{'steps':>7}{'wall [s]':>11}{'ms/step':>10}"
           f"{'ns/cell/step':>14}{'its/step':>10}")
@@ -190,12 +350,16 @@ def build_parser() -> argparse.ArgumentParser:
     sub = ap.add_subparsers(dest="command", required=True)
     p_run = sub.add_parser("run", help="single simulation")
     p_ver = sub.add_parser("verify", help="backend gate against another backend's state file")
-    p_ver.add_argument("--against", required=True, help="final state file (bundle layout)")
+    p_ver.add_argument("--against", default=None,
+                       help="3D cases: final state file (bundle layout) of another backend")
+    p_ver.add_argument("--study", choices=("space", "time"), default="space",
+                       help="2D cases: convergence in space (nx sweep) or time (cfl sweep)")
     p_ben = sub.add_parser("bench", help="step cost over grid sizes")
     p_ben.add_argument("--sweep", default=None, help="comma-separated nx list")
     p_ben.add_argument("--repeat", type=int, default=3)
     for sp in (p_run, p_ver, p_ben):
-        sp.add_argument("--case", default=None, help="lock_exchange | basin")
+        sp.add_argument("--case", default=None,
+                        help="lock_exchange | basin (3D) | igw | igw_broadband | geo_balance (2D)")
         sp.add_argument("--nx", type=int, default=None)
         sp.add_argument("--cfl", type=float, default=None,
                         help="dt as a fraction of the surface-gravity limit")

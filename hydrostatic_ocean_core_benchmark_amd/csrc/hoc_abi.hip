@@ -763,7 +763,191 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
 
 }  // namespace
 
+// ------------------------------------------------------- the 2D model
+struct hoc_sw2d {
+  hoc_sw2d_config cfg{};
+  Sw2d S{};
+  Reducer red;
+  double *r = nullptr, *z = nullptr, *p = nullptr, *ap = nullptr, *part = nullptr;
+  double* dot = nullptr;
+  double* host = nullptr;  // pinned read-back of one dot product
+  long long iterations = 0, failures = 0;
+  std::vector<void*> allocs;
+  int alloc(double** ptr, long long count) {
+    void* v = nullptr;
+    HIP_TRY(hipMalloc(&v, std::max<long long>(count, 1) * (long long)sizeof(double)));
+    allocs.push_back(v);
+    *ptr = (double*)v;
+    return HOC_OK;
+  }
+  ~hoc_sw2d() {
+    for (void* v : allocs) (void)hipFree(v);
+    if (host) (void)hipHostFree(host);
+    red.release();
+  }
+};
+
+namespace {
+
+// sum(a * b) in NumPy's pairwise order, read back to the host.
+int sw2d_dot(hoc_sw2d* h, const double* a, const double* b, hipStream_t s, double* out) {
+  if (h->red.nchunks > 0)
+    k_pw_dot<<<(unsigned)h->red.nchunks, 512, 0, s>>>(a, b, h->red.plans, h->red.nfull, h->part);
+  k_seq_sum<<<1, 1, 0, s>>>(h->part, h->red.nchunks, h->dot);
+  LAUNCH_CHECK("sw2d dot");
+  HIP_TRY(hipMemcpyAsync(h->host, h->dot, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *out = h->host[0];
+  return HOC_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int hoc_sw2d_create(const hoc_sw2d_config* cfg, hoc_sw2d** out) {
+  if (!cfg || !out) return fail(HOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 2 || cfg->ny < 2) return fail(HOC_ERR_INVALID, "grid must have nx, ny >= 2");
+  if (!(cfg->dx > 0.0) || !(cfg->dy > 0.0) || !(cfg->dt > 0.0))
+    return fail(HOC_ERR_INVALID, "dx, dy and dt must be positive");
+  if (cfg->scheme != 0 && cfg->scheme != 1)
+    return fail(HOC_ERR_INVALID, "scheme must be theta (0) or fb (1)");
+  if (cfg->scheme == 0 && !(cfg->theta > 0.0 && cfg->theta <= 1.0))
+    return fail(HOC_ERR_INVALID, "theta must lie in (0, 1]");
+  if (cfg->max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
+  hoc_sw2d* h = new hoc_sw2d();
+  h->cfg = *cfg;
+  Sw2d& S = h->S;
+  S.nx = cfg->nx;
+  S.ny = cfg->ny;
+  S.n = (long long)cfg->nx * cfg->ny;
+  S.dx = cfg->dx;
+  S.dy = cfg->dy;
+  S.g = cfg->g;
+  S.f = cfg->f;
+  S.neg_f = -cfg->f;
+  S.H = cfg->H;
+  S.dt = cfg->dt;
+  // schemes.py evaluates these products left to right on Python floats
+  const double th = cfg->theta, tc = cfg->theta_cor;
+  S.th = th;
+  S.tc = tc;
+  S.one_m_tc = 1.0 - tc;
+  S.one_m_th = 1.0 - th;
+  S.dt_f = cfg->dt * cfg->f;
+  S.g_dt_1mth = cfg->g * cfg->dt * (1.0 - th);
+  S.dt_H = cfg->dt * cfg->H;
+  S.g_dt_th = cfg->g * cfg->dt * th;
+  S.coef = cfg->g * cfg->H * std::pow(th, 2.0) * std::pow(cfg->dt, 2.0);
+  S.inv_diag = 1.0 / (1.0 + S.coef * (2.0 / std::pow(cfg->dx, 2.0) + 2.0 / std::pow(cfg->dy, 2.0)));
+  int rc = h->red.init(S.n);
+  rc = rc ? rc : h->alloc(&h->r, S.n);
+  rc = rc ? rc : h->alloc(&h->z, S.n);
+  rc = rc ? rc : h->alloc(&h->p, S.n);
+  rc = rc ? rc : h->alloc(&h->ap, S.n);
+  rc = rc ? rc : h->alloc(&h->part, std::max<long long>(h->red.nchunks, 1));
+  rc = rc ? rc : h->alloc(&h->dot, 1);
+  if (rc == HOC_OK && hipHostMalloc((void**)&h->host, sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = fail(HOC_ERR_DEVICE, "hipHostMalloc failed");
+  if (rc != HOC_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return HOC_OK;
+}
+
+int hoc_sw2d_destroy(hoc_sw2d* h) {
+  delete h;
+  return HOC_OK;
+}
+
+int hoc_sw2d_momentum(hoc_sw2d* h, const double* u0, const double* v0, const double* eta0,
+                      const double* u_it, const double* v_it, double* out_u, double* out_v,
+                      void* stream) {
+  if (!h || !u0 || !v0 || !eta0 || !u_it || !v_it || !out_u || !out_v)
+    return fail(HOC_ERR_INVALID, "null argument");
+  k_sw_momentum<<<blocks(h->S.n), TPB, 0, (hipStream_t)stream>>>(
+      h->S, h->cfg.scheme == 0 ? 1 : 0, u0, v0, eta0, u_it, v_it, out_u, out_v);
+  LAUNCH_CHECK("k_sw_momentum");
+  return HOC_OK;
+}
+
+int hoc_sw2d_rhs(hoc_sw2d* h, const double* eta0, const double* u0, const double* v0,
+                 const double* gu, const double* gv, double* rhs, void* stream) {
+  if (!h || !eta0 || !u0 || !v0 || !gu || !gv || !rhs) return fail(HOC_ERR_INVALID, "null argument");
+  k_sw_rhs<<<blocks(h->S.n), TPB, 0, (hipStream_t)stream>>>(h->S, eta0, u0, v0, gu, gv, rhs);
+  LAUNCH_CHECK("k_sw_rhs");
+  return HOC_OK;
+}
+
+int hoc_sw2d_backsub(hoc_sw2d* h, const double* gu, const double* gv, const double* eta,
+                     double* out_u, double* out_v, void* stream) {
+  if (!h || !gu || !gv || !eta || !out_u || !out_v) return fail(HOC_ERR_INVALID, "null argument");
+  k_sw_backsub<<<blocks(h->S.n), TPB, 0, (hipStream_t)stream>>>(h->S, gu, gv, eta, out_u, out_v);
+  LAUNCH_CHECK("k_sw_backsub");
+  return HOC_OK;
+}
+
+int hoc_sw2d_fb_eta(hoc_sw2d* h, const double* eta0, const double* u, const double* v,
+                    double* out, void* stream) {
+  if (!h || !eta0 || !u || !v || !out) return fail(HOC_ERR_INVALID, "null argument");
+  k_sw_fb_eta<<<blocks(h->S.n), TPB, 0, (hipStream_t)stream>>>(h->S, eta0, u, v, out);
+  LAUNCH_CHECK("k_sw_fb_eta");
+  return HOC_OK;
+}
+
+int hoc_sw2d_pcg(hoc_sw2d* h, const double* rhs, double* x, int32_t* iterations,
+                 int32_t* converged, void* stream) {
+  if (!h || !rhs || !x) return fail(HOC_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const Sw2d& S = h->S;
+  const unsigned nb = blocks(S.n);
+  const size_t bytes = (size_t)S.n * sizeof(double);
+  // x = 0 and r = rhs - A x = rhs (A 0 = 0 exactly)
+  HIP_TRY(hipMemsetAsync(x, 0, bytes, s));
+  HIP_TRY(hipMemcpyAsync(h->r, rhs, bytes, hipMemcpyDeviceToDevice, s));
+  double nb2 = 0.0;
+  TRY(sw2d_dot(h, rhs, rhs, s, &nb2));
+  const double norm_b = std::sqrt(nb2);
+  int it_out = 0, conv = 1;
+  if (norm_b != 0.0) {
+    k_sw_z<<<nb, TPB, 0, s>>>(S, h->r, h->z);
+    HIP_TRY(hipMemcpyAsync(h->p, h->z, bytes, hipMemcpyDeviceToDevice, s));
+    double rz = 0.0;
+    TRY(sw2d_dot(h, h->r, h->z, s, &rz));
+    conv = 0;
+    it_out = h->cfg.max_iter;
+    for (int it = 1; it <= h->cfg.max_iter; ++it) {
+      k_sw_apply<<<nb, TPB, 0, s>>>(S, h->p, h->ap);
+      double pap = 0.0;
+      TRY(sw2d_dot(h, h->p, h->ap, s, &pap));
+      const double alpha = rz / pap;
+      k_sw_xr<<<nb, TPB, 0, s>>>(S, alpha, h->p, h->ap, x, h->r);
+      double rr = 0.0;
+      TRY(sw2d_dot(h, h->r, h->r, s, &rr));
+      const double residual = std::sqrt(rr) / norm_b;
+      if (residual < h->cfg.rtol) {
+        it_out = it;
+        conv = 1;
+        break;
+      }
+      k_sw_z<<<nb, TPB, 0, s>>>(S, h->r, h->z);
+      double rz_new = 0.0;
+      TRY(sw2d_dot(h, h->r, h->z, s, &rz_new));
+      k_sw_p<<<nb, TPB, 0, s>>>(S, rz_new / rz, h->z, h->p);
+      rz = rz_new;
+    }
+    LAUNCH_CHECK("sw2d pcg");
+  }
+  h->iterations += it_out;
+  if (!conv) h->failures += 1;
+  if (iterations) *iterations = it_out;
+  if (converged) *converged = conv;
+  return HOC_OK;
+}
+
 
 const char* hoc_last_error(void) { return g_error.c_str(); }
 

@@ -160,6 +160,35 @@ __global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo, cons
 __global__ void k_eos_adjoint(long long n, StepParams p, const double* T, const double* S,
                               const double* z, const double* g_b, double* g_T, double* g_S);
 
+// hoc_sw2d.hip: the 2D shallow-water model (libs/core/schemes.py:97-177).
+// Scalars formed on the host in Python's left-to-right order.
+struct Sw2d {
+  int nx, ny;
+  long long n;
+  double dx, dy, g, f, neg_f, H, dt, th, tc;
+  double one_m_tc, one_m_th;
+  double dt_f;       // dt * f
+  double g_dt_1mth;  // g * dt * (1 - th)
+  double dt_H;       // dt * H
+  double g_dt_th;    // g * dt * th
+  double coef;       // g * H * th**2 * dt**2
+  double inv_diag;   // 1 / (1 + coef (2 / dx**2 + 2 / dy**2))
+};
+__global__ void k_sw_momentum(Sw2d S, int theta, const double* u0, const double* v0,
+                              const double* eta0, const double* u_it, const double* v_it,
+                              double* out_u, double* out_v);
+__global__ void k_sw_rhs(Sw2d S, const double* eta0, const double* u0, const double* v0,
+                         const double* gu, const double* gv, double* rhs);
+__global__ void k_sw_backsub(Sw2d S, const double* gu, const double* gv, const double* eta,
+                             double* out_u, double* out_v);
+__global__ void k_sw_fb_eta(Sw2d S, const double* eta0, const double* u, const double* v,
+                            double* out);
+__global__ void k_sw_apply(Sw2d S, const double* p, double* ap);
+__global__ void k_sw_xr(Sw2d S, double alpha, const double* p, const double* ap, double* x,
+                        double* r);
+__global__ void k_sw_z(Sw2d S, const double* r, double* z);
+__global__ void k_sw_p(Sw2d S, double beta, const double* z, double* p);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

@@ -247,6 +247,43 @@ int hoc_stepper_momentum_advection_adjoint(hoc_stepper* h, const double* u, cons
                                            const double* g_av, double* g_u, double* g_v,
                                            double* g_w, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The 2D rotating shallow-water model of the reference's original driver
+ * (libs/core/schemes.py:97-177; State u, v, eta [ny][nx] on a doubly
+ * periodic C-grid, Physics g, f, H). The Python SWStepper drives the Picard
+ * loop with these operators, as Stepper._step_fb/_step_theta do. */
+typedef struct hoc_sw2d_config {
+  int32_t nx, ny;
+  double dx, dy, g, f, H, dt;
+  int32_t scheme;              /* 0 theta, 1 fb */
+  double theta, theta_cor, rtol;
+  int32_t max_iter;
+} hoc_sw2d_config;
+
+typedef struct hoc_sw2d hoc_sw2d;
+
+int hoc_sw2d_create(const hoc_sw2d_config* cfg, hoc_sw2d** out);
+int hoc_sw2d_destroy(hoc_sw2d* h);
+/* One Picard update (schemes.py:127-135, 149-152): fb writes u', v'; theta
+ * writes the predictors gu, gv. u0, v0, eta0: the step's start; u_it, v_it:
+ * the current iterate. */
+int hoc_sw2d_momentum(hoc_sw2d* h, const double* u0, const double* v0, const double* eta0,
+                      const double* u_it, const double* v_it, double* out_u, double* out_v,
+                      void* stream);
+/* rhs = eta0 - dt H ((1 - theta) D[u0, v0] + theta D[gu, gv]) (schemes.py:154-155). */
+int hoc_sw2d_rhs(hoc_sw2d* h, const double* eta0, const double* u0, const double* v0,
+                 const double* gu, const double* gv, double* rhs, void* stream);
+/* u = gu - g dt theta Gx eta, v likewise (schemes.py:161-162). */
+int hoc_sw2d_backsub(hoc_sw2d* h, const double* gu, const double* gv, const double* eta,
+                     double* out_u, double* out_v, void* stream);
+/* eta' = eta0 - dt H D[u, v] (schemes.py:137). */
+int hoc_sw2d_fb_eta(hoc_sw2d* h, const double* eta0, const double* u, const double* v,
+                    double* out, void* stream);
+/* HelmholtzPCG.solve with the Jacobi preconditioner (solvers.py:55-117):
+ * x = A^-1 rhs, A = I - g H theta^2 dt^2 D[G .]; synchronises. */
+int hoc_sw2d_pcg(hoc_sw2d* h, const double* rhs, double* x, int32_t* iterations,
+                 int32_t* converged, void* stream);
+
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */

@@ -10,7 +10,7 @@ cells, the masked Coriolis averages at coasts, the Helmholtz solvers'
 trajectories on arbitrary coefficients, and `Domain` geometry. Only inputs
 and outputs are stored; the GPU box never sees the reference.
 
-Usage:  python tests/golden/make_golden.py [--out tests/golden]
+Usage:  python tests/golden/make_golden.py [--out tests/golden] [--model2d-only]
 """
 
 from __future__ import annotations
@@ -205,13 +205,50 @@ def operator_fixtures() -> dict:
     return rec
 
 
+def model2d_fixtures() -> dict:
+    """The reference's 2D model (libs/core/schemes.py Stepper): 4 steps of the
+    broadband inertia-gravity wave (8 x 8 modes) at 32 x 24, from its own
+    case initial state, for fb, theta/pcg_jacobi and theta/fft; per step
+    the state and the solver iteration count."""
+    from libs.core.cases import InertiaGravityWaveBroadband
+    from libs.core.schemes import Physics, SchemeParams as SP2, Stepper, explicit_dt_max
+    grid = CGrid(nx=32, ny=24, Lx=1.0e7, Ly=0.75e7)
+    phys = Physics(g=9.80616, f=1.0e-4, H=1000.0)
+    case = InertiaGravityWaveBroadband(grid, phys, 8, 1.0, 1.0)
+    s0 = case.initial()
+    rec = {"u0": s0.u, "v0": s0.v, "eta0": s0.eta,
+           "grid": np.array([grid.nx, grid.ny, grid.Lx, grid.Ly]),
+           "physics": np.array([phys.g, phys.f, phys.H])}
+    dt_max = explicit_dt_max(grid, phys)
+    for tag, name, solver, cfl in (("fb", "fb", "fft", 0.5), ("pcgj", "theta", "pcg_jacobi", 4.0),
+                                   ("fft", "theta", "fft", 4.0)):
+        params = SP2(name=name, theta=0.5, theta_cor=0.5, n_picard=2, solver=solver,
+                     rtol=1e-12, max_iter=2000)
+        dt = cfl * dt_max
+        st = Stepper(grid, phys, params, dt)
+        s = s0
+        rec[f"{tag}_dt"] = np.array(dt)
+        for n in range(1, 5):
+            before = st.solver_iterations
+            s = st.step(s)
+            rec[f"{tag}_s{n}_u"], rec[f"{tag}_s{n}_v"], rec[f"{tag}_s{n}_eta"] = s.u, s.v, s.eta
+            rec[f"{tag}_s{n}_iters"] = np.array(st.solver_iterations - before)
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", type=Path, default=Path(__file__).resolve().parent)
     ap.add_argument("--composed-only", action="store_true",
                     help="rewrite only the composed-step fixtures")
+    ap.add_argument("--model2d-only", action="store_true",
+                    help="rewrite only the 2D-model fixture")
     args = ap.parse_args()
     args.out.mkdir(parents=True, exist_ok=True)
+    if args.model2d_only:
+        np.savez_compressed(args.out / "model2d_igw_broadband_32x24.npz", **model2d_fixtures())
+        print("model2d_igw_broadband_32x24.npz written")
+        return 0
     if not args.composed_only:
         np.savez_compressed(args.out / "operators.npz", **operator_fixtures())
         print("operators.npz written")
@@ -223,6 +260,8 @@ def main() -> int:
     rec = lock_exchange_fixture()
     np.savez_compressed(args.out / "lock_exchange_128x4x20.npz", **rec)
     print("lock_exchange_128x4x20.npz written, iterations", rec["iters_per_step"].sum())
+    np.savez_compressed(args.out / "model2d_igw_broadband_32x24.npz", **model2d_fixtures())
+    print("model2d_igw_broadband_32x24.npz written")
     return 0
 
 

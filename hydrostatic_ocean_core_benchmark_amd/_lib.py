@@ -44,7 +44,16 @@ class HocSw2dConfig(C.Structure):
         ("nx", i32), ("ny", i32),
         ("dx", f64), ("dy", f64), ("g", f64), ("f", f64), ("H", f64), ("dt", f64),
         ("scheme", i32),
-        ("theta", f64), ("theta_cor", f64), ("rtol", f64),
+        ("theta", f64), ("theta_cor", f64),
+    ]
+
+
+class HocHelm2dConfig(C.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32),
+        ("dx", f64), ("dy", f64), ("coef", f64),
+        ("kind", i32),
+        ("rtol", f64),
         ("max_iter", i32),
     ]
 
@@ -97,7 +106,10 @@ PROTOTYPES = {
     "hoc_sw2d_rhs": ([VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_sw2d_backsub": ([VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_sw2d_fb_eta": ([VP, VP, VP, VP, VP, VP], i32),
-    "hoc_sw2d_pcg": ([VP, VP, VP, C.POINTER(i32), C.POINTER(i32), VP], i32),
+    "hoc_helm2d_create": ([VP, VP], i32),
+    "hoc_helm2d_destroy": ([VP], i32),
+    "hoc_helm2d_solve": ([VP, VP, VP, C.POINTER(i32), C.POINTER(f64), C.POINTER(i32), VP], i32),
+    "hoc_helm2d_levels": ([VP, C.POINTER(i32)], i32),
     "hoc_stepper_momentum_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_eos_buoyancy_adjoint": ([i32, f64, f64, f64, f64, f64, f64, VP, VP, VP, VP, VP, VP, i64,
                                   VP], i32),

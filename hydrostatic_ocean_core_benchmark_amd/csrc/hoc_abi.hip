@@ -767,11 +767,18 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
 struct hoc_sw2d {
   hoc_sw2d_config cfg{};
   Sw2d S{};
+};
+
+// ------------------------------------- the constant-coefficient Helmholtz
+struct hoc_helm2d {
+  hoc_helm2d_config cfg{};
+  std::vector<H2Level> lv;
+  std::vector<double*> lx, lb;   // per level from 1: correction and restricted residual
   Reducer red;
-  double *r = nullptr, *z = nullptr, *p = nullptr, *ap = nullptr, *part = nullptr;
-  double* dot = nullptr;
-  double* host = nullptr;  // pinned read-back of one dot product
-  long long iterations = 0, failures = 0;
+  double inv_diag = 0.0;
+  double *r = nullptr, *z = nullptr, *p = nullptr, *ap = nullptr, *tmp = nullptr;
+  double *part = nullptr, *dot = nullptr;
+  double* host = nullptr;        // pinned read-back of one dot product
   std::vector<void*> allocs;
   int alloc(double** ptr, long long count) {
     void* v = nullptr;
@@ -780,7 +787,7 @@ struct hoc_sw2d {
     *ptr = (double*)v;
     return HOC_OK;
   }
-  ~hoc_sw2d() {
+  ~hoc_helm2d() {
     for (void* v : allocs) (void)hipFree(v);
     if (host) (void)hipHostFree(host);
     red.release();
@@ -789,16 +796,93 @@ struct hoc_sw2d {
 
 namespace {
 
-// sum(a * b) in NumPy's pairwise order, read back to the host.
-int sw2d_dot(hoc_sw2d* h, const double* a, const double* b, hipStream_t s, double* out) {
+H2Level h2_level(int nx, int ny, double dx, double dy, double coef) {
+  H2Level L{};
+  L.nx = nx;
+  L.ny = ny;
+  L.n = (long long)nx * ny;
+  L.dx = dx;
+  L.dy = dy;
+  L.dx2 = std::pow(dx, 2.0);
+  L.dy2 = std::pow(dy, 2.0);
+  L.cx = coef / L.dx2;
+  L.cy = coef / L.dy2;
+  L.diag = 1.0 + 2.0 * L.cx + 2.0 * L.cy;
+  L.coef = coef;
+  return L;
+}
+
+// sum(a * b) over the finest level in NumPy's pairwise order, read back.
+int h2_dot(hoc_helm2d* h, const double* a, const double* b, hipStream_t s, double* out) {
   if (h->red.nchunks > 0)
     k_pw_dot<<<(unsigned)h->red.nchunks, 512, 0, s>>>(a, b, h->red.plans, h->red.nfull, h->part);
   k_seq_sum<<<1, 1, 0, s>>>(h->part, h->red.nchunks, h->dot);
-  LAUNCH_CHECK("sw2d dot");
+  LAUNCH_CHECK("helm2d dot");
   HIP_TRY(hipMemcpyAsync(h->host, h->dot, sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   *out = h->host[0];
   return HOC_OK;
+}
+
+// n_sweeps red-black sweeps of level l on x (bit k of pattern: sweep k runs
+// black first). Small levels run whole in one workgroup; larger even levels
+// update each colour in place; larger odd levels go through tmp.
+int h2_sweeps(hoc_helm2d* h, int l, double* x, const double* b, int n_sweeps, unsigned pattern,
+              hipStream_t s) {
+  const H2Level& L = h->lv[l];
+  if (L.n <= H2_COARSE_MAX) {
+    for (int s0 = 0; s0 < n_sweeps; s0 += 32) {
+      const int m = std::min(32, n_sweeps - s0);
+      k_h2_gs_block<<<1, H2_COARSE_THREADS, 0, s>>>(L, m, s0 < 32 ? pattern >> s0 : 0u, x, b);
+    }
+    LAUNCH_CHECK("k_h2_gs_block");
+    return HOC_OK;
+  }
+  const bool even = (L.nx % 2 == 0) && (L.ny % 2 == 0);
+  for (int k = 0; k < n_sweeps; ++k) {
+    const int first = (k < 32 && ((pattern >> k) & 1u)) ? 1 : 0;
+    for (int c = 0; c < 2; ++c) {
+      const int colour = c == 0 ? first : 1 - first;
+      if (even) {
+        k_h2_gs_colour<<<blocks(L.n / 2), TPB, 0, s>>>(L, colour, x, b);
+      } else {
+        k_h2_gs_colour_copy<<<blocks(L.n), TPB, 0, s>>>(L, colour, x, b, h->tmp);
+        k_h2_copy<<<blocks(L.n), TPB, 0, s>>>(L.n, h->tmp, x);
+      }
+    }
+  }
+  LAUNCH_CHECK("k_h2_gs_colour");
+  return HOC_OK;
+}
+
+// HelmholtzMultigrid._vcycle (solvers.py:226-245): 2 pre-sweeps forward then
+// backward, restriction, the coarser cycle, prolongation, 2 post-sweeps
+// backward then forward; 40 forward sweeps on the coarsest level.
+int h2_vcycle(hoc_helm2d* h, int l, double* x, const double* b, hipStream_t s) {
+  const int last = (int)h->lv.size() - 1;
+  if (l == last) return h2_sweeps(h, l, x, b, 40, 0u, s);
+  TRY(h2_sweeps(h, l, x, b, 2, 0x2u, s));
+  const H2Level& F = h->lv[l];
+  const H2Level& Cl = h->lv[l + 1];
+  k_h2_restrict<<<blocks(Cl.n), TPB, 0, s>>>(F, Cl, x, b, h->lb[l + 1], h->lx[l + 1]);
+  LAUNCH_CHECK("k_h2_restrict");
+  TRY(h2_vcycle(h, l + 1, h->lx[l + 1], h->lb[l + 1], s));
+  k_h2_prolong<<<blocks(F.n), TPB, 0, s>>>(F, Cl, x, h->lx[l + 1]);
+  LAUNCH_CHECK("k_h2_prolong");
+  return h2_sweeps(h, l, x, b, 2, 0x1u, s);
+}
+
+// z = M^-1 r (HelmholtzPCG._precondition, solvers.py:75-86)
+int h2_precondition(hoc_helm2d* h, const double* r, double* z, hipStream_t s) {
+  const H2Level& L = h->lv[0];
+  if (h->cfg.kind == 0) {
+    k_h2_jacobi<<<blocks(L.n), TPB, 0, s>>>(L.n, h->inv_diag, r, z);
+    LAUNCH_CHECK("k_h2_jacobi");
+    return HOC_OK;
+  }
+  k_h2_fill<<<blocks(L.n), TPB, 0, s>>>(L.n, 0.0, z);
+  LAUNCH_CHECK("k_h2_fill");
+  return h2_sweeps(h, 0, z, r, 2, 0x2u, s);
 }
 
 }  // namespace
@@ -815,7 +899,6 @@ int hoc_sw2d_create(const hoc_sw2d_config* cfg, hoc_sw2d** out) {
     return fail(HOC_ERR_INVALID, "scheme must be theta (0) or fb (1)");
   if (cfg->scheme == 0 && !(cfg->theta > 0.0 && cfg->theta <= 1.0))
     return fail(HOC_ERR_INVALID, "theta must lie in (0, 1]");
-  if (cfg->max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
   hoc_sw2d* h = new hoc_sw2d();
   h->cfg = *cfg;
   Sw2d& S = h->S;
@@ -839,21 +922,6 @@ int hoc_sw2d_create(const hoc_sw2d_config* cfg, hoc_sw2d** out) {
   S.g_dt_1mth = cfg->g * cfg->dt * (1.0 - th);
   S.dt_H = cfg->dt * cfg->H;
   S.g_dt_th = cfg->g * cfg->dt * th;
-  S.coef = cfg->g * cfg->H * std::pow(th, 2.0) * std::pow(cfg->dt, 2.0);
-  S.inv_diag = 1.0 / (1.0 + S.coef * (2.0 / std::pow(cfg->dx, 2.0) + 2.0 / std::pow(cfg->dy, 2.0)));
-  int rc = h->red.init(S.n);
-  rc = rc ? rc : h->alloc(&h->r, S.n);
-  rc = rc ? rc : h->alloc(&h->z, S.n);
-  rc = rc ? rc : h->alloc(&h->p, S.n);
-  rc = rc ? rc : h->alloc(&h->ap, S.n);
-  rc = rc ? rc : h->alloc(&h->part, std::max<long long>(h->red.nchunks, 1));
-  rc = rc ? rc : h->alloc(&h->dot, 1);
-  if (rc == HOC_OK && hipHostMalloc((void**)&h->host, sizeof(double), hipHostMallocDefault) != hipSuccess)
-    rc = fail(HOC_ERR_DEVICE, "hipHostMalloc failed");
-  if (rc != HOC_OK) {
-    delete h;
-    return rc;
-  }
   *out = h;
   return HOC_OK;
 }
@@ -898,56 +966,146 @@ int hoc_sw2d_fb_eta(hoc_sw2d* h, const double* eta0, const double* u, const doub
   return HOC_OK;
 }
 
-int hoc_sw2d_pcg(hoc_sw2d* h, const double* rhs, double* x, int32_t* iterations,
-                 int32_t* converged, void* stream) {
+int hoc_helm2d_create(const hoc_helm2d_config* cfg, hoc_helm2d** out) {
+  if (!cfg || !out) return fail(HOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 2 || cfg->ny < 2) return fail(HOC_ERR_INVALID, "grid must have nx, ny >= 2");
+  if (!(cfg->dx > 0.0) || !(cfg->dy > 0.0))
+    return fail(HOC_ERR_INVALID, "dx and dy must be positive");
+  if (!(cfg->coef >= 0.0)) return fail(HOC_ERR_INVALID, "coef must be >= 0");
+  if (cfg->kind < 0 || cfg->kind > 3)
+    return fail(HOC_ERR_INVALID, "kind must be 0 pcg_jacobi, 1 pcg_rbgs, 2 rbgs or 3 multigrid");
+  if (cfg->kind > 0 && (cfg->nx % 2 || cfg->ny % 2))
+    return fail(HOC_ERR_INVALID, "the red-black solvers need even nx and ny for a consistent "
+                                 "colouring under periodic wrap");
+  if (cfg->max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
+  hoc_helm2d* h = new hoc_helm2d();
+  h->cfg = *cfg;
+  int nx = cfg->nx, ny = cfg->ny;
+  double dx = cfg->dx, dy = cfg->dy;
+  // HelmholtzMultigrid.__init__ (solvers.py:213-220): halve while both sizes
+  // are even and the smaller is above 4
+  while (true) {
+    h->lv.push_back(h2_level(nx, ny, dx, dy, cfg->coef));
+    if (cfg->kind != 3 || nx % 2 || ny % 2 || std::min(nx, ny) <= 4) break;
+    nx /= 2;
+    ny /= 2;
+    dx = dx * 2;
+    dy = dy * 2;
+  }
+  h->inv_diag = 1.0 / (1.0 + cfg->coef * (2.0 / h->lv[0].dx2 + 2.0 / h->lv[0].dy2));
+  const long long n = h->lv[0].n;
+  int rc = h->red.init(n);
+  rc = rc ? rc : h->alloc(&h->r, n);
+  rc = rc ? rc : h->alloc(&h->z, n);
+  rc = rc ? rc : h->alloc(&h->p, n);
+  rc = rc ? rc : h->alloc(&h->ap, n);
+  rc = rc ? rc : h->alloc(&h->tmp, n);
+  rc = rc ? rc : h->alloc(&h->part, std::max<long long>(h->red.nchunks, 1));
+  rc = rc ? rc : h->alloc(&h->dot, 1);
+  h->lx.assign(h->lv.size(), nullptr);
+  h->lb.assign(h->lv.size(), nullptr);
+  for (size_t l = 1; l < h->lv.size() && rc == HOC_OK; ++l) {
+    rc = h->alloc(&h->lx[l], h->lv[l].n);
+    rc = rc ? rc : h->alloc(&h->lb[l], h->lv[l].n);
+  }
+  if (rc == HOC_OK && hipHostMalloc((void**)&h->host, sizeof(double), hipHostMallocDefault) != hipSuccess)
+    rc = fail(HOC_ERR_DEVICE, "hipHostMalloc failed");
+  if (rc != HOC_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return HOC_OK;
+}
+
+int hoc_helm2d_destroy(hoc_helm2d* h) {
+  delete h;
+  return HOC_OK;
+}
+
+int hoc_helm2d_levels(hoc_helm2d* h, int32_t* n_levels) {
+  if (!h || !n_levels) return fail(HOC_ERR_INVALID, "null argument");
+  *n_levels = (int32_t)h->lv.size();
+  return HOC_OK;
+}
+
+int hoc_helm2d_solve(hoc_helm2d* h, const double* rhs, double* x, int32_t* iterations,
+                     double* residual_out, int32_t* converged, void* stream) {
   if (!h || !rhs || !x) return fail(HOC_ERR_INVALID, "null argument");
   hipStream_t s = (hipStream_t)stream;
-  const Sw2d& S = h->S;
-  const unsigned nb = blocks(S.n);
-  const size_t bytes = (size_t)S.n * sizeof(double);
-  // x = 0 and r = rhs - A x = rhs (A 0 = 0 exactly)
-  HIP_TRY(hipMemsetAsync(x, 0, bytes, s));
-  HIP_TRY(hipMemcpyAsync(h->r, rhs, bytes, hipMemcpyDeviceToDevice, s));
+  const H2Level& L = h->lv[0];
+  const long long n = L.n;
+  const int kind = h->cfg.kind, max_iter = h->cfg.max_iter;
+  const double rtol = h->cfg.rtol;
+  k_h2_fill<<<blocks(n), TPB, 0, s>>>(n, 0.0, x);
+  LAUNCH_CHECK("k_h2_fill");
   double nb2 = 0.0;
-  TRY(sw2d_dot(h, rhs, rhs, s, &nb2));
-  const double norm_b = std::sqrt(nb2);
-  int it_out = 0, conv = 1;
-  if (norm_b != 0.0) {
-    k_sw_z<<<nb, TPB, 0, s>>>(S, h->r, h->z);
-    HIP_TRY(hipMemcpyAsync(h->p, h->z, bytes, hipMemcpyDeviceToDevice, s));
-    double rz = 0.0;
-    TRY(sw2d_dot(h, h->r, h->z, s, &rz));
-    conv = 0;
-    it_out = h->cfg.max_iter;
-    for (int it = 1; it <= h->cfg.max_iter; ++it) {
-      k_sw_apply<<<nb, TPB, 0, s>>>(S, h->p, h->ap);
-      double pap = 0.0;
-      TRY(sw2d_dot(h, h->p, h->ap, s, &pap));
-      const double alpha = rz / pap;
-      k_sw_xr<<<nb, TPB, 0, s>>>(S, alpha, h->p, h->ap, x, h->r);
+  TRY(h2_dot(h, rhs, rhs, s, &nb2));
+  double norm_b = std::sqrt(nb2);
+  int it_out = max_iter, conv = 0;
+  double residual = 0.0;
+  if (kind <= 1) {
+    // HelmholtzPCG.solve (solvers.py:91-117); r = rhs - A 0 = rhs exactly
+    if (norm_b == 0.0) {
+      it_out = 0;
+      conv = 1;
+    } else {
+      k_h2_copy<<<blocks(n), TPB, 0, s>>>(n, rhs, h->r);
+      TRY(h2_precondition(h, h->r, h->z, s));
+      k_h2_copy<<<blocks(n), TPB, 0, s>>>(n, h->z, h->p);
+      double rz = 0.0;
+      TRY(h2_dot(h, h->r, h->z, s, &rz));
+      for (int it = 1; it <= max_iter; ++it) {
+        k_h2_apply_dg<<<blocks(n), TPB, 0, s>>>(L, h->p, h->ap);
+        double pap = 0.0;
+        TRY(h2_dot(h, h->p, h->ap, s, &pap));
+        const double alpha = rz / pap;
+        k_h2_xr<<<blocks(n), TPB, 0, s>>>(n, alpha, h->p, h->ap, x, h->r);
+        double rr = 0.0;
+        TRY(h2_dot(h, h->r, h->r, s, &rr));
+        residual = std::sqrt(rr) / norm_b;
+        if (residual < rtol) {
+          it_out = it;
+          conv = 1;
+          break;
+        }
+        TRY(h2_precondition(h, h->r, h->z, s));
+        double rz_new = 0.0;
+        TRY(h2_dot(h, h->r, h->z, s, &rz_new));
+        k_h2_p<<<blocks(n), TPB, 0, s>>>(n, rz_new / rz, h->z, h->p);
+        rz = rz_new;
+      }
+      LAUNCH_CHECK("helm2d pcg");
+    }
+  } else {
+    // HelmholtzRBGS.solve / HelmholtzMultigrid.solve (solvers.py:165-184,
+    // 247-258): norm_b = |rhs| or 1
+    if (norm_b == 0.0) norm_b = 1.0;
+    for (int it = 1; it <= max_iter; ++it) {
+      if (kind == 2) {
+        TRY(h2_sweeps(h, 0, x, rhs, 1, 0u, s));
+        if (!(it % 10 == 0 || it == max_iter)) continue;
+      } else {
+        TRY(h2_vcycle(h, 0, x, rhs, s));
+      }
+      k_h2_resid<<<blocks(n), TPB, 0, s>>>(L, x, rhs, h->r);
+      LAUNCH_CHECK("k_h2_resid");
       double rr = 0.0;
-      TRY(sw2d_dot(h, h->r, h->r, s, &rr));
-      const double residual = std::sqrt(rr) / norm_b;
-      if (residual < h->cfg.rtol) {
+      TRY(h2_dot(h, h->r, h->r, s, &rr));
+      residual = std::sqrt(rr) / norm_b;
+      if (residual < rtol) {
         it_out = it;
         conv = 1;
         break;
       }
-      k_sw_z<<<nb, TPB, 0, s>>>(S, h->r, h->z);
-      double rz_new = 0.0;
-      TRY(sw2d_dot(h, h->r, h->z, s, &rz_new));
-      k_sw_p<<<nb, TPB, 0, s>>>(S, rz_new / rz, h->z, h->p);
-      rz = rz_new;
     }
-    LAUNCH_CHECK("sw2d pcg");
   }
-  h->iterations += it_out;
-  if (!conv) h->failures += 1;
   if (iterations) *iterations = it_out;
+  if (residual_out) *residual_out = residual;
   if (converged) *converged = conv;
   return HOC_OK;
 }
-
 
 const char* hoc_last_error(void) { return g_error.c_str(); }
 

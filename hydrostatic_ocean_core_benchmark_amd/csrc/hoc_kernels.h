@@ -171,8 +171,6 @@ struct Sw2d {
   double g_dt_1mth;  // g * dt * (1 - th)
   double dt_H;       // dt * H
   double g_dt_th;    // g * dt * th
-  double coef;       // g * H * th**2 * dt**2
-  double inv_diag;   // 1 / (1 + coef (2 / dx**2 + 2 / dy**2))
 };
 __global__ void k_sw_momentum(Sw2d S, int theta, const double* u0, const double* v0,
                               const double* eta0, const double* u_it, const double* v_it,
@@ -183,11 +181,36 @@ __global__ void k_sw_backsub(Sw2d S, const double* gu, const double* gv, const d
                              double* out_u, double* out_v);
 __global__ void k_sw_fb_eta(Sw2d S, const double* eta0, const double* u, const double* v,
                             double* out);
-__global__ void k_sw_apply(Sw2d S, const double* p, double* ap);
-__global__ void k_sw_xr(Sw2d S, double alpha, const double* p, const double* ap, double* x,
+
+// hoc_helm2d.hip: one level of the constant-coefficient periodic Helmholtz
+struct H2Level {
+  int nx, ny;
+  long long n;
+  double dx, dy;
+  double dx2, dy2;     // dx**2, dy**2
+  double cx, cy;       // coef / dx**2, coef / dy**2
+  double diag;         // 1 + 2 cx + 2 cy
+  double coef;
+};
+constexpr int H2_COARSE_THREADS = 1024;
+constexpr int H2_COARSE_PER = 16;
+constexpr long long H2_COARSE_MAX = (long long)H2_COARSE_THREADS * H2_COARSE_PER;
+__global__ void k_h2_apply_dg(H2Level L, const double* p, double* ap);
+__global__ void k_h2_resid(H2Level L, const double* x, const double* b, double* r);
+__global__ void k_h2_gs_colour(H2Level L, int colour, double* x, const double* b);
+__global__ void k_h2_gs_colour_copy(H2Level L, int colour, const double* in, const double* b,
+                                    double* out);
+__global__ void k_h2_gs_block(H2Level L, int n_sweeps, unsigned pattern, double* x,
+                              const double* b);
+__global__ void k_h2_restrict(H2Level F, H2Level Cl, const double* x, const double* b,
+                              double* bc, double* xc);
+__global__ void k_h2_prolong(H2Level F, H2Level Cl, double* x, const double* ec);
+__global__ void k_h2_xr(long long n, double alpha, const double* p, const double* ap, double* x,
                         double* r);
-__global__ void k_sw_z(Sw2d S, const double* r, double* z);
-__global__ void k_sw_p(Sw2d S, double beta, const double* z, double* p);
+__global__ void k_h2_jacobi(long long n, double inv_diag, const double* r, double* z);
+__global__ void k_h2_p(long long n, double beta, const double* z, double* p);
+__global__ void k_h2_fill(long long n, double value, double* x);
+__global__ void k_h2_copy(long long n, const double* in, double* out);
 
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);

@@ -1,9 +1,8 @@
 // The 2D rotating shallow-water step of the reference's original (v0.1)
 // model, libs/core/schemes.py:97-177, on a doubly periodic C-grid with
 // constant depth H: forward-backward and theta (Casulli) schemes with a
-// Picard loop on the Coriolis term, and the constant-coefficient Helmholtz
-// operator A = I - coef D[G .] of libs/core/solvers.py:55-117 for the
-// PCG-Jacobi solve. One thread per cell; every expression keeps the
+// Picard loop on the Coriolis term (the Helmholtz solves are in
+// hoc_helm2d.hip). One thread per cell; every expression keeps the
 // reference's operand order (host constants are formed in Python's
 // left-to-right order), so results are bitwise the NumPy model's.
 #include "hoc_device.h"
@@ -100,43 +99,6 @@ __global__ void k_sw_fb_eta(Sw2d S, const double* __restrict__ eta0, const doubl
   int i, j;
   sw_ij(S, t, i, j);
   out[t] = eta0[t] - S.dt_H * sw_div(S, u, v, j, i);
-}
-
-// A p = p - coef L[p], L = D o G   (solvers.py:83-84, operators.py:49-53)
-__global__ void k_sw_apply(Sw2d S, const double* __restrict__ p, double* __restrict__ ap) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.n) return;
-  int i, j;
-  sw_ij(S, t, i, j);
-  const int iw = wrap(i, -1, S.nx), js = wrap(j, -1, S.ny);
-  const double gx = sw_gradx(S, p, j, i), gxw = sw_gradx(S, p, j, iw);
-  const double gy = sw_grady(S, p, j, i), gys = sw_grady(S, p, js, i);
-  const double lap = (gx - gxw) / S.dx + (gy - gys) / S.dy;
-  ap[t] = p[t] - S.coef * lap;
-}
-
-// x = x + alpha p, r = r - alpha ap
-__global__ void k_sw_xr(Sw2d S, double alpha, const double* __restrict__ p,
-                        const double* __restrict__ ap, double* __restrict__ x,
-                        double* __restrict__ r) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.n) return;
-  x[t] = x[t] + alpha * p[t];
-  r[t] = r[t] - alpha * ap[t];
-}
-
-// z = inv_diag r
-__global__ void k_sw_z(Sw2d S, const double* __restrict__ r, double* __restrict__ z) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.n) return;
-  z[t] = S.inv_diag * r[t];
-}
-
-// p = z + beta p
-__global__ void k_sw_p(Sw2d S, double beta, const double* __restrict__ z, double* __restrict__ p) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.n) return;
-  p[t] = z[t] + beta * p[t];
 }
 
 }  // namespace hoc

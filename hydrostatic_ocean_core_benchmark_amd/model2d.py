@@ -6,15 +6,17 @@
 `State(u, v, eta, t)` on the doubly periodic C-grid (u at east faces, v at
 north faces, eta at centres; [ny, nx]). Both schemes run their Picard loop
 here with the HIP operators of the C-ABI (hoc_sw2d_*): forward-backward, and
-the theta scheme with a Helmholtz solve per Picard pass by
+the theta scheme with a Helmholtz solve per Picard pass (`Helmholtz2D`,
+build_solver of libs/core/solvers.py:261-281) by
   fft         the spectral solve of HelmholtzFFT (solvers.py:33-52) with the
               device FFT (rocFFT through torch.fft): the same symbol, agreement
               with NumPy's FFT to round-off (not bitwise);
-  pcg_jacobi  HelmholtzPCG (solvers.py:55-117) on the device, the dot products
-              in NumPy's pairwise order, so iterates and iteration counts are
-              bitwise the reference's.
-The other solver kinds of the reference (pcg_rbgs, rbgs, multigrid) are
-refused with NotImplementedError.
+  pcg_jacobi, pcg_rbgs, rbgs, multigrid
+              HelmholtzPCG (Jacobi or symmetric red-black Gauss-Seidel
+              preconditioner), HelmholtzRBGS and HelmholtzMultigrid on the
+              device (hoc_helm2d_*), the dot products in NumPy's pairwise
+              order, so iterates and iteration counts are bitwise the
+              reference's.
 """
 
 from __future__ import annotations
@@ -68,7 +70,83 @@ def physics_from_config(config) -> Physics:
                    H=float(config.get("physics.H")))
 
 
-SOLVERS = ("fft", "pcg_jacobi")
+SOLVERS = ("fft", "pcg_jacobi", "pcg_rbgs", "rbgs", "multigrid")
+_KIND = {"pcg_jacobi": 0, "pcg_rbgs": 1, "rbgs": 2, "multigrid": 3}
+
+
+@dataclass
+class SolveReport:
+    """Per-solve diagnostics (libs/core/solvers.py:24-30)."""
+    iterations: int = 0
+    residual: float = 0.0
+    converged: bool = True
+
+
+class Helmholtz2D:
+    """The free-surface solve (I - coef L) x = rhs on a doubly periodic grid,
+    as build_solver(kind, grid, coef, rtol, max_iter) builds it
+    (libs/core/solvers.py:261-281); `solve(rhs)` returns (x, SolveReport)
+    with x a device tensor. Invalid setups raise the reference's ValueError."""
+
+    def __init__(self, kind: str, grid, coef: float, rtol: float, max_iter: int,
+                 device="cuda:0"):
+        if kind not in SOLVERS:
+            raise ValueError(f"unknown solver kind '{kind}' "
+                             "(expected fft|pcg_jacobi|pcg_rbgs|rbgs|multigrid)")
+        if kind == "fft" and (getattr(grid, "bc_x", "periodic") != "periodic"
+                              or getattr(grid, "bc_y", "periodic") != "periodic"):
+            raise ValueError("the fft solver requires doubly periodic boundaries")
+        if kind in ("rbgs", "multigrid", "pcg_rbgs") and (grid.nx % 2 or grid.ny % 2):
+            raise ValueError(f"{kind} needs even nx and ny for a consistent red-black "
+                             f"colouring under periodic wrap; got {grid.nx}x{grid.ny}")
+        self.kind = kind
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        if self.device.type != "cuda" or not torch.cuda.is_available():
+            raise RuntimeError("Helmholtz2D needs a HIP device (torch.cuda)")
+        self.shape = (grid.ny, grid.nx)
+        self._h = None
+        self._symbol = None
+        if kind == "fft":
+            # HelmholtzFFT's symbol 1 - coef * lambda (solvers.py:38-47)
+            theta_x = 2.0 * np.pi * np.arange(grid.nx // 2 + 1) / grid.nx
+            theta_y = 2.0 * np.pi * np.arange(grid.ny) / grid.ny
+            lam_x = 2.0 * (np.cos(theta_x) - 1.0) / grid.dx ** 2
+            lam_y = 2.0 * (np.cos(theta_y) - 1.0) / grid.dy ** 2
+            lam = lam_y[:, None] + lam_x[None, :]
+            self._symbol = torch.as_tensor(1.0 - coef * lam, device=self.device)
+            self.n_levels = 1
+            return
+        cfg = _lib.HocHelm2dConfig()
+        cfg.nx, cfg.ny, cfg.dx, cfg.dy = grid.nx, grid.ny, grid.dx, grid.dy
+        cfg.coef, cfg.kind, cfg.rtol, cfg.max_iter = coef, _KIND[kind], rtol, max_iter
+        h = C.c_void_p()
+        _lib.check(self.lib.hoc_helm2d_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        nl = C.c_int32(0)
+        _lib.check(self.lib.hoc_helm2d_levels(h, C.byref(nl)))
+        self.n_levels = nl.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self.lib.hoc_helm2d_destroy(h)
+            self._h = None
+
+    def solve(self, rhs: torch.Tensor):
+        rhs = rhs.to(device=self.device, dtype=torch.float64).contiguous()
+        if tuple(rhs.shape) != self.shape:
+            raise ValueError(f"rhs shape {tuple(rhs.shape)} does not match the grid {self.shape}")
+        if self._symbol is not None:
+            spec = torch.fft.rfft2(rhs)
+            return (torch.fft.irfft2(spec / self._symbol, s=self.shape).contiguous(),
+                    SolveReport())
+        x = torch.empty_like(rhs)
+        it, res, cv = C.c_int32(0), C.c_double(0.0), C.c_int32(0)
+        st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check(self.lib.hoc_helm2d_solve(self._h, _lib.ptr(rhs), _lib.ptr(x), C.byref(it),
+                                             C.byref(res), C.byref(cv), st))
+        return x, SolveReport(it.value, res.value, bool(cv.value))
 
 
 class SWStepper:
@@ -81,13 +159,6 @@ class SWStepper:
         if getattr(grid, "bc_x", "periodic") != "periodic" or \
                 getattr(grid, "bc_y", "periodic") != "periodic":
             raise NotImplementedError("the 2D model is doubly periodic")
-        if params.name == "theta" and params.solver not in SOLVERS:
-            if params.solver in ("pcg_rbgs", "rbgs", "multigrid"):
-                raise NotImplementedError(
-                    f"the 2D model runs the fft and pcg_jacobi solvers; '{params.solver}' "
-                    "is not built for it")
-            raise ValueError(f"unknown solver kind '{params.solver}' "
-                             "(expected fft|pcg_jacobi|pcg_rbgs|rbgs|multigrid)")
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda" or not torch.cuda.is_available():
@@ -102,20 +173,15 @@ class SWStepper:
         cfg.g, cfg.f, cfg.H, cfg.dt = physics.g, physics.f, physics.H, self.dt
         cfg.scheme = 0 if params.name == "theta" else 1
         cfg.theta, cfg.theta_cor = params.theta, params.theta_cor
-        cfg.rtol, cfg.max_iter = params.rtol, params.max_iter
         h = C.c_void_p()
         _lib.check(self.lib.hoc_sw2d_create(C.byref(cfg), C.byref(h)))
         self._h = h
-        self._symbol = None
-        if params.name == "theta" and params.solver == "fft":
-            # HelmholtzFFT's symbol 1 - coef * lambda (solvers.py:38-47)
+        self.solver = None
+        if params.name == "theta":
+            # schemes.py:110-113
             coef = physics.g * physics.H * params.theta ** 2 * self.dt ** 2
-            theta_x = 2.0 * np.pi * np.arange(grid.nx // 2 + 1) / grid.nx
-            theta_y = 2.0 * np.pi * np.arange(grid.ny) / grid.ny
-            lam_x = 2.0 * (np.cos(theta_x) - 1.0) / grid.dx ** 2
-            lam_y = 2.0 * (np.cos(theta_y) - 1.0) / grid.dy ** 2
-            lam = lam_y[:, None] + lam_x[None, :]
-            self._symbol = torch.as_tensor(1.0 - coef * lam, device=self.device)
+            self.solver = Helmholtz2D(params.solver, grid, coef, params.rtol, params.max_iter,
+                                      device=self.device)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -132,15 +198,10 @@ class SWStepper:
         return torch.as_tensor(np.array(a, dtype=np.float64, order="C"), device=self.device)
 
     def _solve(self, rhs: torch.Tensor) -> torch.Tensor:
-        if self._symbol is not None:
-            spec = torch.fft.rfft2(rhs)
-            return torch.fft.irfft2(spec / self._symbol, s=tuple(rhs.shape)).contiguous()
-        x = torch.empty_like(rhs)
-        it, cv = C.c_int32(0), C.c_int32(0)
-        _lib.check(self.lib.hoc_sw2d_pcg(self._h, _lib.ptr(rhs), _lib.ptr(x), C.byref(it),
-                                         C.byref(cv), self._stream()))
-        self.solver_iterations += it.value
-        self.solver_failures += 0 if cv.value else 1
+        x, rep = self.solver.solve(rhs)
+        self.solver_iterations += rep.iterations
+        if not rep.converged:
+            self.solver_failures += 1
         return x
 
     def step(self, s: State) -> State:

@@ -256,8 +256,7 @@ typedef struct hoc_sw2d_config {
   int32_t nx, ny;
   double dx, dy, g, f, H, dt;
   int32_t scheme;              /* 0 theta, 1 fb */
-  double theta, theta_cor, rtol;
-  int32_t max_iter;
+  double theta, theta_cor;
 } hoc_sw2d_config;
 
 typedef struct hoc_sw2d hoc_sw2d;
@@ -279,10 +278,35 @@ int hoc_sw2d_backsub(hoc_sw2d* h, const double* gu, const double* gv, const doub
 /* eta' = eta0 - dt H D[u, v] (schemes.py:137). */
 int hoc_sw2d_fb_eta(hoc_sw2d* h, const double* eta0, const double* u, const double* v,
                     double* out, void* stream);
-/* HelmholtzPCG.solve with the Jacobi preconditioner (solvers.py:55-117):
- * x = A^-1 rhs, A = I - g H theta^2 dt^2 D[G .]; synchronises. */
-int hoc_sw2d_pcg(hoc_sw2d* h, const double* rhs, double* x, int32_t* iterations,
-                 int32_t* converged, void* stream);
+
+/* ------------------------------------------------------------------------
+ * The constant-coefficient free-surface Helmholtz solve of the original
+ * models, (I - coef L) x = rhs on a doubly periodic nx x ny grid
+ * (libs/core/solvers.py:55-281, built by build_solver :261-281): kind
+ * 0 pcg_jacobi, 1 pcg_rbgs (HelmholtzPCG, precond "jacobi"/"rbgs"),
+ * 2 rbgs (HelmholtzRBGS), 3 multigrid (HelmholtzMultigrid). The spectral
+ * kind is the device FFT on the Python side. coef is g H theta^2 dt^2 as the
+ * caller's stepper forms it. Kinds 1-3 need even nx and ny (the reference's
+ * ValueError). */
+typedef struct hoc_helm2d_config {
+  int32_t nx, ny;
+  double dx, dy, coef;
+  int32_t kind;
+  double rtol;
+  int32_t max_iter;
+} hoc_helm2d_config;
+
+typedef struct hoc_helm2d hoc_helm2d;
+
+int hoc_helm2d_create(const hoc_helm2d_config* cfg, hoc_helm2d** out);
+int hoc_helm2d_destroy(hoc_helm2d* h);
+/* solve(rhs) -> (x, SolveReport): writes x, the iteration (or sweep) count,
+ * the final relative residual and whether it met rtol; synchronises (the
+ * reference's convergence tests are host decisions). */
+int hoc_helm2d_solve(hoc_helm2d* h, const double* rhs, double* x, int32_t* iterations,
+                     double* residual, int32_t* converged, void* stream);
+/* Number of multigrid levels (1 for the other kinds). */
+int hoc_helm2d_levels(hoc_helm2d* h, int32_t* n_levels);
 
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the

@@ -208,8 +208,10 @@ def operator_fixtures() -> dict:
 def model2d_fixtures() -> dict:
     """The reference's 2D model (libs/core/schemes.py Stepper): 4 steps of the
     broadband inertia-gravity wave (8 x 8 modes) at 32 x 24, from its own
-    case initial state, for fb, theta/pcg_jacobi and theta/fft; per step
-    the state and the solver iteration count."""
+    case initial state, for fb and theta with each of the five solver kinds
+    (fft, pcg_jacobi, pcg_rbgs, rbgs, multigrid); per step the state and the
+    solver iteration count. The 32 x 24 grid coarsens to a 4 x 3 level, so
+    the multigrid fixture covers a coarsest level of odd size."""
     from libs.core.cases import InertiaGravityWaveBroadband
     from libs.core.schemes import Physics, SchemeParams as SP2, Stepper, explicit_dt_max
     grid = CGrid(nx=32, ny=24, Lx=1.0e7, Ly=0.75e7)
@@ -221,7 +223,10 @@ def model2d_fixtures() -> dict:
            "physics": np.array([phys.g, phys.f, phys.H])}
     dt_max = explicit_dt_max(grid, phys)
     for tag, name, solver, cfl in (("fb", "fb", "fft", 0.5), ("pcgj", "theta", "pcg_jacobi", 4.0),
-                                   ("fft", "theta", "fft", 4.0)):
+                                   ("fft", "theta", "fft", 4.0),
+                                   ("pcgrb", "theta", "pcg_rbgs", 4.0),
+                                   ("rbgs", "theta", "rbgs", 1.5),
+                                   ("mg", "theta", "multigrid", 4.0)):
         params = SP2(name=name, theta=0.5, theta_cor=0.5, n_picard=2, solver=solver,
                      rtol=1e-12, max_iter=2000)
         dt = cfl * dt_max

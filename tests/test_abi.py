@@ -47,7 +47,8 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     from hydrostatic_ocean_core_benchmark_amd import _lib
     structs = {"hoc_config": _lib.HocConfig, "hoc_geometry": _lib.HocGeometry,
                "hoc_state": _lib.HocState, "hoc_counters": _lib.HocCounters,
-               "hoc_sw2d_config": _lib.HocSw2dConfig}
+               "hoc_sw2d_config": _lib.HocSw2dConfig,
+               "hoc_helm2d_config": _lib.HocHelm2dConfig}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
              'int main(void) {']
     for cname, py in structs.items():

@@ -1,8 +1,9 @@
 """The 2D shallow-water model (the reference's original driver,
 libs/core/schemes.py:97-177): the oracle against the reference's own
-4-step fixtures (bitwise), the two Helmholtz solves against each other, and
-the device SWStepper against the fixtures (bitwise for fb and PCG-Jacobi,
-FFT to round-off) through the C-ABI."""
+4-step fixtures (bitwise) for fb and theta with each of the five Helmholtz
+solves, the solves against each other, and the device SWStepper against the
+fixtures (bitwise except the spectral solve, which agrees to round-off)
+through the C-ABI."""
 
 from __future__ import annotations
 
@@ -13,7 +14,8 @@ from conftest import load_golden
 
 torch = pytest.importorskip("torch")
 
-CASES = (("fb", "fb", "fft"), ("pcgj", "theta", "pcg_jacobi"), ("fft", "theta", "fft"))
+CASES = (("fb", "fb", "fft"), ("pcgj", "theta", "pcg_jacobi"), ("fft", "theta", "fft"),
+         ("pcgrb", "theta", "pcg_rbgs"), ("rbgs", "theta", "rbgs"), ("mg", "theta", "multigrid"))
 
 
 def _setup(f):

@@ -48,6 +48,22 @@ class HocSw2dConfig(C.Structure):
     ]
 
 
+class HocM3Config(C.Structure):
+    _fields_ = [
+        ("nx", i32), ("ny", i32), ("nz", i32),
+        ("dx", f64), ("dy", f64), ("dz", f64), ("dt", f64),
+        ("g", f64), ("f", f64), ("H", f64), ("N2", f64), ("nu", f64), ("kappa", f64),
+        ("rho0", f64), ("tau_x", f64), ("tau_y", f64), ("bottom_drag", f64),
+        ("tracers", i32),
+        ("alpha_T", f64), ("beta_S", f64), ("T0", f64), ("S0", f64), ("cp", f64),
+        ("A_h", f64), ("K_h", f64), ("q_heat", f64), ("q_salt", f64),
+        ("scheme", i32),
+        ("theta", f64), ("theta_cor", f64), ("theta_v", f64),
+        ("advection", i32),
+        ("n_split", i32),
+    ]
+
+
 class HocHelm2dConfig(C.Structure):
     _fields_ = [
         ("nx", i32), ("ny", i32),
@@ -106,6 +122,21 @@ PROTOTYPES = {
     "hoc_sw2d_rhs": ([VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_sw2d_backsub": ([VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_sw2d_fb_eta": ([VP, VP, VP, VP, VP, VP], i32),
+    "hoc_m3_create": ([VP, VP], i32),
+    "hoc_m3_destroy": ([VP], i32),
+    "hoc_m3_q": ([VP, VP], i32),
+    "hoc_m3_prelude": ([VP] * 11, i32),
+    "hoc_m3_predict": ([VP] * 16, i32),
+    "hoc_m3_rhs": ([VP] * 8, i32),
+    "hoc_m3_backsub": ([VP] * 7, i32),
+    "hoc_m3_eta_div": ([VP, f64, VP, VP, VP, VP, VP], i32),
+    "hoc_m3_forcing": ([VP] * 10, i32),
+    "hoc_m3_bt_momentum": ([VP] * 11, i32),
+    "hoc_m3_bt_acc": ([VP] * 6, i32),
+    "hoc_m3_split_final": ([VP, i32, VP, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_m3_w": ([VP] * 5, i32),
+    "hoc_m3_tracer": ([VP, i32, VP, VP, VP, VP, VP, VP], i32),
+    "hoc_m3_eos": ([VP] * 5, i32),
     "hoc_helm2d_create": ([VP, VP], i32),
     "hoc_helm2d_destroy": ([VP], i32),
     "hoc_helm2d_solve": ([VP, VP, VP, C.POINTER(i32), C.POINTER(f64), C.POINTER(i32), VP], i32),

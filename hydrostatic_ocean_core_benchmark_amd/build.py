@@ -22,7 +22,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libhoc.so"
 SOURCES = ("hoc_abi.hip", "hoc_step_kernels.hip", "hoc_tiled.hip", "hoc_helmholtz.hip", "hoc_mg.hip",
-           "hoc_adjoint.hip", "hoc_sw2d.hip", "hoc_helm2d.hip")
+           "hoc_adjoint.hip", "hoc_sw2d.hip", "hoc_helm2d.hip",
+           "hoc_model3d.hip")
 ARCH = os.environ.get("HOC_OFFLOAD_ARCH", "gfx950")
 
 

@@ -794,6 +794,17 @@ struct hoc_helm2d {
   }
 };
 
+// ------------------------------------------------------- the v0.2 3D model
+struct hoc_m3 {
+  hoc_m3_config cfg{};
+  M3 M{};
+  std::vector<double> q;        // host copy of the effective-depth profile
+  double* coeffs = nullptr;     // device: sub_m, den_m, cst_m, sub_b, den_b, cst_b, q
+  ~hoc_m3() {
+    if (coeffs) (void)hipFree(coeffs);
+  }
+};
+
 namespace {
 
 H2Level h2_level(int nx, int ny, double dx, double dy, double coef) {
@@ -1104,6 +1115,247 @@ int hoc_helm2d_solve(hoc_helm2d* h, const double* rhs, double* x, int32_t* itera
   if (iterations) *iterations = it_out;
   if (residual_out) *residual_out = residual;
   if (converged) *converged = conv;
+  return HOC_OK;
+}
+
+int hoc_m3_create(const hoc_m3_config* cfg, hoc_m3** out) {
+  if (!cfg || !out) return fail(HOC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const hoc_m3_config& c = *cfg;
+  if (c.nx < 2 || c.ny < 2 || c.nz < 1) return fail(HOC_ERR_INVALID, "grid must have nx, ny >= 2 and nz >= 1");
+  if (!(c.dx > 0.0) || !(c.dy > 0.0) || !(c.dz > 0.0) || !(c.dt > 0.0) || !(c.H > 0.0))
+    return fail(HOC_ERR_INVALID, "dx, dy, dz, dt and H must be positive");
+  if (c.scheme < 0 || c.scheme > 2) return fail(HOC_ERR_INVALID, "scheme must be theta (0), fb (1) or split_explicit (2)");
+  if (c.advection < 0 || c.advection > 2)
+    return fail(HOC_ERR_INVALID, "advection must be none (0), centered2 (1) or upwind1 (2)");
+  if (c.tracers != 0 && c.tracers != 1) return fail(HOC_ERR_INVALID, "tracers must be buoyancy (0) or TS (1)");
+  if (c.scheme == 2 && c.n_split < 1) return fail(HOC_ERR_INVALID, "split_explicit needs n_split >= 1");
+  hoc_m3* h = new hoc_m3();
+  h->cfg = c;
+  M3& M = h->M;
+  M.nx = c.nx;
+  M.ny = c.ny;
+  M.nz = c.nz;
+  M.n2 = (long long)c.nx * c.ny;
+  M.n3 = M.n2 * c.nz;
+  M.dx = c.dx;
+  M.dy = c.dy;
+  M.dz = c.dz;
+  M.dx2 = std::pow(c.dx, 2.0);
+  M.dy2 = std::pow(c.dy, 2.0);
+  M.neg_dz = -c.dz;
+  M.dt = c.dt;
+  M.f = c.f;
+  M.neg_f = -c.f;
+  M.g = c.g;
+  M.H = c.H;
+  // model3d.py evaluates these scalar products left to right on Python floats
+  const double th = c.scheme == 0 ? c.theta : 0.0, tc = c.theta_cor, thv = c.theta_v;
+  M.th = th;
+  M.one_m_th = 1.0 - th;
+  M.tc = tc;
+  M.one_m_tc = 1.0 - tc;
+  M.c_eta = c.g * c.dt * (1.0 - th);
+  M.c_diff = (1.0 - thv) * c.dt;
+  M.c_bs = c.g * c.dt * th;
+  M.nu = c.nu;
+  M.kappa = c.kappa;
+  M.bd = c.bottom_drag;
+  M.tau_u = c.tau_x / c.rho0;
+  M.tau_v = c.tau_y / c.rho0;
+  M.s0u = thv * c.dt * M.tau_u / c.dz;
+  M.s0v = thv * c.dt * M.tau_v / c.dz;
+  M.A_h = c.A_h;
+  M.K_h = c.K_h;
+  M.adv = c.advection;
+  M.neg_N2 = -c.N2;
+  M.flux_T = c.q_heat / (c.rho0 * c.cp);
+  M.flux_S = c.q_salt;
+  M.s0T = thv * c.dt * M.flux_T / c.dz;
+  M.s0S = thv * c.dt * M.flux_S / c.dz;
+  M.s0b = thv * c.dt * 0.0 / c.dz;
+  M.alpha_T = c.alpha_T;
+  M.beta_S = c.beta_S;
+  M.T0 = c.T0;
+  M.S0 = c.S0;
+  M.ddt = c.scheme == 2 ? c.dt / c.n_split : 0.0;
+  M.gh = c.g * c.H;
+  // diffusion_coeffs + the column-independent part of thomas_batched
+  // (vertical.py:17-72): sub, the pivots (den[0] = diag[0]) and c*.
+  const int nz = c.nz;
+  auto factor = [&](double visc, double drag, double* sub, double* den, double* cst) {
+    const double fac = thv * c.dt / std::pow(c.dz, 2.0);
+    std::vector<double> sup(nz), diag(nz);
+    for (int k = 0; k < nz; ++k) {
+      const double a_top = fac * visc, a_bot = fac * visc;
+      sub[k] = -a_top;
+      sup[k] = -a_bot;
+      diag[k] = 1.0 + a_top + a_bot;
+    }
+    diag[0] = 1.0 + fac * visc;
+    sub[0] = 0.0;
+    diag[nz - 1] = 1.0 + fac * visc + thv * c.dt * drag / c.dz;
+    sup[nz - 1] = 0.0;
+    cst[0] = sup[0] / diag[0];
+    den[0] = diag[0];
+    for (int k = 1; k < nz; ++k) {
+      den[k] = diag[k] - sub[k] * cst[k - 1];
+      cst[k] = sup[k] / den[k];
+    }
+  };
+  std::vector<double> hc(7 * (size_t)nz);
+  double *sub_m = &hc[0], *den_m = &hc[nz], *cst_m = &hc[2 * nz];
+  double *sub_b = &hc[3 * nz], *den_b = &hc[4 * nz], *cst_b = &hc[5 * nz], *qv = &hc[6 * nz];
+  factor(c.nu, c.bottom_drag, sub_m, den_m, cst_m);
+  factor(c.kappa, 0.0, sub_b, den_b, cst_b);
+  // q = A^-1 1 with the momentum factors (model3d.py:125-126)
+  std::vector<double> d(nz);
+  for (int k = 0; k < nz; ++k) d[k] = k == 0 ? 1.0 / den_m[0] : (1.0 - sub_m[k] * d[k - 1]) / den_m[k];
+  qv[nz - 1] = d[nz - 1];
+  for (int k = nz - 2; k >= 0; --k) qv[k] = d[k] - cst_m[k] * qv[k + 1];
+  h->q.assign(qv, qv + nz);
+  if (hipMalloc((void**)&h->coeffs, hc.size() * sizeof(double)) != hipSuccess ||
+      hipMemcpy(h->coeffs, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    delete h;
+    return fail(HOC_ERR_DEVICE, "device allocation of the column factors failed");
+  }
+  M.sub_m = h->coeffs;
+  M.den_m = h->coeffs + nz;
+  M.cst_m = h->coeffs + 2 * nz;
+  M.sub_b = h->coeffs + 3 * nz;
+  M.den_b = h->coeffs + 4 * nz;
+  M.cst_b = h->coeffs + 5 * nz;
+  M.q = h->coeffs + 6 * nz;
+  *out = h;
+  return HOC_OK;
+}
+
+int hoc_m3_destroy(hoc_m3* h) {
+  delete h;
+  return HOC_OK;
+}
+
+int hoc_m3_q(hoc_m3* h, double* q) {
+  if (!h || !q) return fail(HOC_ERR_INVALID, "null argument");
+  std::copy(h->q.begin(), h->q.end(), q);
+  return HOC_OK;
+}
+
+int hoc_m3_prelude(hoc_m3* h, const double* u, const double* v, const double* b, double* phi,
+                   double* U0, double* V0, double* w, double* st_u, double* st_v, void* stream) {
+  if (!h || !u || !v || !b || !phi || !U0 || !V0 || !w || !st_u || !st_v)
+    return fail(HOC_ERR_INVALID, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const M3& M = h->M;
+  k_m3_phi<<<blocks(M.n2), TPB, 0, s>>>(M, b, phi);
+  k_m3_depth<<<blocks(M.n2), TPB, 0, s>>>(M, u, v, U0, V0);
+  k_m3_w<<<blocks(M.n2), TPB, 0, s>>>(M, u, v, w);
+  k_m3_stress<<<blocks(M.n3), TPB, 0, s>>>(M, u, v, w, st_u, st_v);
+  LAUNCH_CHECK("m3 prelude");
+  return HOC_OK;
+}
+
+int hoc_m3_predict(hoc_m3* h, const double* u0, const double* v0, const double* phi,
+                   const double* eta, const double* u_it, const double* v_it, const double* st_u,
+                   const double* st_v, double* gh_u, double* gh_v, double* GU, double* GV,
+                   double* cor_x, double* cor_y, void* stream) {
+  if (!h || !u0 || !v0 || !phi || !eta || !u_it || !v_it || !st_u || !st_v || !gh_u || !gh_v ||
+      !GU || !GV)
+    return fail(HOC_ERR_INVALID, "null argument");
+  const int split = h->cfg.scheme == 2 ? 1 : 0;
+  if (split && (!cor_x || !cor_y)) return fail(HOC_ERR_INVALID, "split_explicit needs cor_x and cor_y");
+  k_m3_predict<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(
+      h->M, split, u0, v0, phi, eta, u_it, v_it, st_u, st_v, gh_u, gh_v, GU, GV, cor_x, cor_y);
+  LAUNCH_CHECK("k_m3_predict");
+  return HOC_OK;
+}
+
+int hoc_m3_rhs(hoc_m3* h, const double* eta, const double* U0, const double* V0,
+               const double* GU, const double* GV, double* rhs, void* stream) {
+  if (!h || !eta || !U0 || !V0 || !GU || !GV || !rhs) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_rhs<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, eta, U0, V0, GU, GV, rhs);
+  LAUNCH_CHECK("k_m3_rhs");
+  return HOC_OK;
+}
+
+int hoc_m3_backsub(hoc_m3* h, const double* gh_u, const double* gh_v, const double* eta,
+                   double* u, double* v, void* stream) {
+  if (!h || !gh_u || !gh_v || !eta || !u || !v) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_backsub<<<blocks(h->M.n3), TPB, 0, (hipStream_t)stream>>>(h->M, gh_u, gh_v, eta, u, v);
+  LAUNCH_CHECK("k_m3_backsub");
+  return HOC_OK;
+}
+
+int hoc_m3_eta_div(hoc_m3* h, double dt_sub, const double* eta, const double* U,
+                   const double* V, double* out, void* stream) {
+  if (!h || !eta || !U || !V || !out) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_eta_div<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, dt_sub, eta, U, V, out);
+  LAUNCH_CHECK("k_m3_eta_div");
+  return HOC_OK;
+}
+
+int hoc_m3_forcing(hoc_m3* h, const double* GU, const double* GV, const double* U0,
+                   const double* V0, const double* cor_x, const double* cor_y, double* fx,
+                   double* fy, void* stream) {
+  if (!h || !GU || !GV || !U0 || !V0 || !cor_x || !cor_y || !fx || !fy)
+    return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_forcing<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, GU, GV, U0, V0, cor_x,
+                                                                 cor_y, fx, fy);
+  LAUNCH_CHECK("k_m3_forcing");
+  return HOC_OK;
+}
+
+int hoc_m3_bt_momentum(hoc_m3* h, const double* u_i, const double* v_i, const double* e,
+                       const double* ui, const double* vi, const double* fx, const double* fy,
+                       double* ui_out, double* vi_out, void* stream) {
+  if (!h || !u_i || !v_i || !e || !ui || !vi || !fx || !fy || !ui_out || !vi_out)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (ui_out == ui || vi_out == vi || ui_out == u_i || vi_out == v_i)
+    return fail(HOC_ERR_INVALID, "barotropic momentum outputs must not alias its inputs");
+  k_m3_bt_momentum<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, u_i, v_i, e, ui, vi,
+                                                                     fx, fy, ui_out, vi_out);
+  LAUNCH_CHECK("k_m3_bt_momentum");
+  return HOC_OK;
+}
+
+int hoc_m3_bt_acc(hoc_m3* h, const double* u_i, const double* v_i, double* acc_u, double* acc_v,
+                  void* stream) {
+  if (!h || !u_i || !v_i || !acc_u || !acc_v) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_bt_acc<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, u_i, v_i, acc_u, acc_v);
+  LAUNCH_CHECK("k_m3_bt_acc");
+  return HOC_OK;
+}
+
+int hoc_m3_split_final(hoc_m3* h, int32_t mean_n, const double* gh_u, const double* gh_v,
+                       const double* Ub, const double* Vb, double* u, double* v, void* stream) {
+  if (!h || !gh_u || !gh_v || !Ub || !Vb || !u || !v) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_split_final<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, mean_n, gh_u, gh_v, Ub,
+                                                                     Vb, u, v);
+  LAUNCH_CHECK("k_m3_split_final");
+  return HOC_OK;
+}
+
+int hoc_m3_w(hoc_m3* h, const double* u, const double* v, double* w, void* stream) {
+  if (!h || !u || !v || !w) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_w<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, u, v, w);
+  LAUNCH_CHECK("k_m3_w");
+  return HOC_OK;
+}
+
+int hoc_m3_tracer(hoc_m3* h, int32_t which, const double* c, const double* u, const double* v,
+                  const double* w, double* out, void* stream) {
+  if (!h || !c || !u || !v || !w || !out) return fail(HOC_ERR_INVALID, "null argument");
+  if (which < 0 || which > 2) return fail(HOC_ERR_INVALID, "which must be 0 (b), 1 (T) or 2 (S)");
+  if (out == c) return fail(HOC_ERR_INVALID, "tracer output must not alias its input");
+  k_m3_tracer<<<blocks(h->M.n2), TPB, 0, (hipStream_t)stream>>>(h->M, which, c, u, v, w, out);
+  LAUNCH_CHECK("k_m3_tracer");
+  return HOC_OK;
+}
+
+int hoc_m3_eos(hoc_m3* h, const double* T, const double* S, double* b, void* stream) {
+  if (!h || !T || !S || !b) return fail(HOC_ERR_INVALID, "null argument");
+  k_m3_eos<<<blocks(h->M.n3), TPB, 0, (hipStream_t)stream>>>(h->M, T, S, b);
+  LAUNCH_CHECK("k_m3_eos");
   return HOC_OK;
 }
 

@@ -212,6 +212,58 @@ __global__ void k_h2_p(long long n, double beta, const double* z, double* p);
 __global__ void k_h2_fill(long long n, double value, double* x);
 __global__ void k_h2_copy(long long n, const double* in, double* out);
 
+// hoc_model3d.hip: the v0.2 3D model (flat bottom, uniform layers, periodic)
+struct M3 {
+  int nx, ny, nz;
+  long long n2, n3;
+  double dx, dy, dz, dx2, dy2, neg_dz;
+  double dt, f, neg_f, g, H;
+  double one_m_tc, tc, one_m_th, th;
+  double c_eta;          // g * dt * (1 - th), th = 0 for fb
+  double c_diff;         // (1 - theta_v) * dt
+  double c_bs;           // g * dt * th
+  double nu, kappa, bd;  // viscosity, diffusivity, bottom drag
+  double tau_u, tau_v;   // tau_x / rho0, tau_y / rho0
+  double s0u, s0v;       // theta_v * dt * tau / dz
+  double A_h, K_h;
+  int adv;               // 0 none, 1 centered2, 2 upwind1
+  double neg_N2;
+  double flux_T, flux_S; // q_heat / (rho0 * cp), q_salt
+  double s0T, s0S, s0b;  // theta_v * dt * flux / dz
+  double alpha_T, beta_S, T0, S0;
+  double ddt, gh;        // dt / n_split, g * H
+  const double *sub_m, *den_m, *cst_m, *sub_b, *den_b, *cst_b, *q;
+};
+__global__ void k_m3_phi(M3 M, const double* b, double* phi);
+__global__ void k_m3_depth(M3 M, const double* u, const double* v, double* U, double* V);
+__global__ void k_m3_w(M3 M, const double* u, const double* v, double* w);
+__global__ void k_m3_stress(M3 M, const double* u, const double* v, const double* w, double* st_u,
+                            double* st_v);
+__global__ void k_m3_predict(M3 M, int split, const double* u0, const double* v0,
+                             const double* phi, const double* eta, const double* u_it,
+                             const double* v_it, const double* st_u, const double* st_v,
+                             double* gh_u, double* gh_v, double* GU, double* GV, double* cor_x,
+                             double* cor_y);
+__global__ void k_m3_rhs(M3 M, const double* eta, const double* U0, const double* V0,
+                         const double* GU, const double* GV, double* rhs);
+__global__ void k_m3_backsub(M3 M, const double* gh_u, const double* gh_v, const double* eta,
+                             double* u, double* v);
+__global__ void k_m3_eta_div(M3 M, double dt, const double* eta, const double* U, const double* V,
+                             double* out);
+__global__ void k_m3_forcing(M3 M, const double* GU, const double* GV, const double* U0,
+                             const double* V0, const double* cor_x, const double* cor_y,
+                             double* fx, double* fy);
+__global__ void k_m3_bt_momentum(M3 M, const double* u_i, const double* v_i, const double* e,
+                                 const double* ui, const double* vi, const double* fx,
+                                 const double* fy, double* ui_out, double* vi_out);
+__global__ void k_m3_bt_acc(M3 M, const double* u_i, const double* v_i, double* acc_u,
+                            double* acc_v);
+__global__ void k_m3_split_final(M3 M, int mean_n, const double* gh_u, const double* gh_v,
+                                 const double* Ub, const double* Vb, double* u, double* v);
+__global__ void k_m3_tracer(M3 M, int which, const double* c, const double* u, const double* v,
+                            const double* w, double* out);
+__global__ void k_m3_eos(M3 M, const double* T, const double* S, double* b);
+
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,

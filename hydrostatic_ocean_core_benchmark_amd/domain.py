@@ -70,6 +70,17 @@ class CGrid:
         y = (np.arange(self.ny, dtype=np.float64) + oy) * self.dy
         return np.meshgrid(x, y, indexing="xy")
 
+    def z_centre(self) -> np.ndarray:
+        """Height above the flat bottom of the layer centres (libs/core/grid.py:63-66)."""
+        k = np.arange(self.nz, dtype=np.float64)
+        return (self.nz - k - 0.5) * self.dz
+
+    def z_interface(self) -> np.ndarray:
+        """Height above the flat bottom of the interfaces, k = 0 the surface
+        (libs/core/grid.py:68-71)."""
+        k = np.arange(self.nz + 1, dtype=np.float64)
+        return (self.nz - k) * self.dz
+
 
 # ----------------------------------------------------------------- bathymetry
 def _powerlaw_field(nx: int, ny: int, slope: float, seed: int, kmax: int = 0) -> np.ndarray:

@@ -70,6 +70,10 @@ class Physics3D:
     q_heat: float = 0.0
     q_salt: float = 0.0
 
+    def buoyancy(self, T, S):
+        """Linear equation of state (libs/core/model3d.py:84-88)."""
+        return self.g * (self.alpha_T * (T - self.T0) - self.beta_S * (S - self.S0))
+
     @property
     def c(self) -> float:
         return float(np.sqrt(self.g * self.H))

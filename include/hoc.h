@@ -308,6 +308,80 @@ int hoc_helm2d_solve(hoc_helm2d* h, const double* rhs, double* x, int32_t* itera
 /* Number of multigrid levels (1 for the other kinds). */
 int hoc_helm2d_levels(hoc_helm2d* h, int32_t* n_levels);
 
+/* ------------------------------------------------------------------------
+ * The v0.2 3D model (libs/core/model3d.py:93-391, Stepper3D): flat bottom,
+ * nz uniform layers, doubly periodic; fields [nz][ny][nx], w [nz+1][ny][nx],
+ * 2D fields [ny][nx]. The Python Stepper3D drives the step with these
+ * operators (the Picard loop, the Helmholtz solve through hoc_helm2d or the
+ * device FFT, the barotropic substeps), as Stepper3D.step/_step_split do.
+ * Every pointer is a device buffer; every call is asynchronous on stream. */
+typedef struct hoc_m3_config {
+  int32_t nx, ny, nz;
+  double dx, dy, dz, dt;
+  double g, f, H, N2, nu, kappa, rho0, tau_x, tau_y, bottom_drag;
+  int32_t tracers;             /* 0 buoyancy, 1 T and S */
+  double alpha_T, beta_S, T0, S0, cp, A_h, K_h, q_heat, q_salt;
+  int32_t scheme;              /* 0 theta, 1 fb, 2 split_explicit */
+  double theta, theta_cor, theta_v;
+  int32_t advection;           /* 0 none, 1 centered2, 2 upwind1 */
+  int32_t n_split;             /* barotropic substeps (split_explicit) */
+} hoc_m3_config;
+
+typedef struct hoc_m3 hoc_m3;
+
+/* Stepper3D.__init__ (model3d.py:96-151): the tridiagonal factors of
+ * A = I - theta_v dt Dz for momentum (viscosity, bottom drag) and tracers
+ * (diffusivity), and the effective-depth profile q = A^-1 1. */
+int hoc_m3_create(const hoc_m3_config* cfg, hoc_m3** out);
+int hoc_m3_destroy(hoc_m3* h);
+/* q[nz] on the host (every column has the same profile). */
+int hoc_m3_q(hoc_m3* h, double* q);
+/* Once per step (model3d.py:158-161, 166-180, 329-343): phi' (the depth-mean
+ * free buoyancy potential), U0 = dz sum u, V0 = dz sum v, w from D[u, v] and
+ * the explicit stress st_u, st_v (surface stress, advection, A_h). */
+int hoc_m3_prelude(hoc_m3* h, const double* u, const double* v, const double* b, double* phi,
+                   double* U0, double* V0, double* w, double* st_u, double* st_v, void* stream);
+/* One Picard pass of the predictor and its vertical solve
+ * (model3d.py:352-364; split form :276-288): gh_u, gh_v, GU = dz sum gh_u,
+ * GV, and for split_explicit cor_x = dz sum f v_cor, cor_y = -dz sum f u_cor
+ * (cor_x, cor_y may be null otherwise). */
+int hoc_m3_predict(hoc_m3* h, const double* u0, const double* v0, const double* phi,
+                   const double* eta, const double* u_it, const double* v_it, const double* st_u,
+                   const double* st_v, double* gh_u, double* gh_v, double* GU, double* GV,
+                   double* cor_x, double* cor_y, void* stream);
+/* rhs = eta - dt ((1 - theta) D[U0, V0] + theta D[GU, GV]) (model3d.py:367-369). */
+int hoc_m3_rhs(hoc_m3* h, const double* eta, const double* U0, const double* V0,
+               const double* GU, const double* GV, double* rhs, void* stream);
+/* u = gh_u - g dt theta q Gx eta, v likewise (model3d.py:374-375). */
+int hoc_m3_backsub(hoc_m3* h, const double* gh_u, const double* gh_v, const double* eta,
+                   double* u, double* v, void* stream);
+/* out = eta - dt_sub D[U, V] (fb surface, model3d.py:379-381; barotropic
+ * substep, :235). */
+int hoc_m3_eta_div(hoc_m3* h, double dt_sub, const double* eta, const double* U,
+                   const double* V, double* out, void* stream);
+/* Barotropic forcing fx = (GU - U0) / dt - cor_x, fy likewise (model3d.py:292-293). */
+int hoc_m3_forcing(hoc_m3* h, const double* GU, const double* GV, const double* U0,
+                   const double* V0, const double* cor_x, const double* cor_y, double* fx,
+                   double* fy, void* stream);
+/* One Picard pass of a barotropic substep (model3d.py:224-232). */
+int hoc_m3_bt_momentum(hoc_m3* h, const double* u_i, const double* v_i, const double* e,
+                       const double* ui, const double* vi, const double* fx, const double* fy,
+                       double* ui_out, double* vi_out, void* stream);
+/* acc_u += u_i, acc_v += v_i (model3d.py:236-237). */
+int hoc_m3_bt_acc(hoc_m3* h, const double* u_i, const double* v_i, double* acc_u, double* acc_v,
+                  void* stream);
+/* u = gh_u - <gh_u>_z + Ub / (mean_n > 0 ? mean_n : 1) / H (model3d.py:241-246, 297-300). */
+int hoc_m3_split_final(hoc_m3* h, int32_t mean_n, const double* gh_u, const double* gh_v,
+                       const double* Ub, const double* Vb, double* u, double* v, void* stream);
+/* w at interfaces from D[u, v] (model3d.py:384). */
+int hoc_m3_w(hoc_m3* h, const double* u, const double* v, double* w, void* stream);
+/* One tracer step (model3d.py:182-198): which 0 buoyancy (source -N2 w),
+ * 1 temperature, 2 salinity (surface fluxes q_heat / (rho0 cp), q_salt). */
+int hoc_m3_tracer(hoc_m3* h, int32_t which, const double* c, const double* u, const double* v,
+                  const double* w, double* out, void* stream);
+/* b = g (alpha_T (T - T0) - beta_S (S - S0)) (model3d.py:84-88). */
+int hoc_m3_eos(hoc_m3* h, const double* T, const double* S, double* b, void* stream);
+
 /* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */

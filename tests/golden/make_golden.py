@@ -241,6 +241,70 @@ def model2d_fixtures() -> dict:
     return rec
 
 
+MODEL3D_V02_CONFIGS = {
+    # name: (scheme overrides, physics overrides, tracers)
+    "theta_pcgj_c2": (dict(name="theta", solver="pcg_jacobi", advection="centered2"),
+                      dict(A_h=5.0e3, K_h=2.0e3, tau_x=0.1, tau_y=-0.05, bottom_drag=1.0e-3),
+                      "buoyancy"),
+    "theta_mg_up1_ts": (dict(name="theta", solver="multigrid", advection="upwind1", theta=0.6),
+                        dict(q_heat=50.0, q_salt=1.0e-6, K_h=1.0e3), "TS"),
+    "fb_up1": (dict(name="fb", advection="upwind1", n_picard=3), dict(tau_x=0.05), "buoyancy"),
+    "split_mean_c2_ts": (dict(name="split_explicit", advection="centered2",
+                              barotropic_average="mean"),
+                         dict(A_h=2.0e3, q_heat=-20.0, bottom_drag=5.0e-4), "TS"),
+    "split_none": (dict(name="split_explicit", advection="none", n_split=7), {}, "buoyancy"),
+    "theta_pcgrb_none": (dict(name="theta", solver="pcg_rbgs", advection="none", theta_v=1.0),
+                         dict(nu=0.0, kappa=0.0), "buoyancy"),
+}
+
+
+def model3d_v02_fixtures() -> dict:
+    """The reference's v0.2 3D model (libs/core/model3d.py Stepper3D): 3
+    steps at 16 x 12 x 5 from a seeded smooth state, per configuration of
+    MODEL3D_V02_CONFIGS (each scheme, the advection kinds, buoyancy and T/S
+    tracers, stress, drag and horizontal mixing); per step the state and the
+    solver iteration / barotropic substep counters."""
+    from libs.core.model3d import Physics3D, State3D, Stepper3D
+    from libs.core.schemes import SchemeParams as SP
+    nx, ny, nz, Lx, Ly, H = 16, 12, 5, 1.6e6, 1.2e6, 1000.0
+    grid = CGrid(nx=nx, ny=ny, Lx=Lx, Ly=Ly, nz=nz, depth=H)
+    rng = np.random.default_rng(2024)
+    x, y = grid.coords("eta")
+    zc = grid.z_centre()[:, None, None]
+    def smooth(scale):
+        a, b2, c = rng.uniform(0.5, 1.5, 3)
+        return scale * (np.cos(2 * np.pi * (x / Lx * a + y / Ly)) [None] * np.cos(zc / H * c * 3.0)
+                        + 0.3 * np.sin(2 * np.pi * (2 * x / Lx - y / Ly * b2))[None])
+    u0, v0 = smooth(0.2), smooth(0.15)
+    b0 = 1.0e-4 * (zc - H) + smooth(1.0e-3)
+    T0 = 10.0 + smooth(2.0)
+    S0 = 35.0 + smooth(0.3)
+    eta0 = 0.2 * np.cos(2 * np.pi * x / Lx) * np.sin(2 * np.pi * y / Ly)
+    rec = {"u0": u0, "v0": v0, "b0": b0, "T0": T0, "S0": S0, "eta0": eta0,
+           "grid": np.array([nx, ny, nz, Lx, Ly, H])}
+    base_phys = dict(g=9.80616, f=1.0e-4, H=H, N2=1.0e-4, nu=1.0e-2, kappa=5.0e-3)
+    dt = 600.0
+    for name, (sch, ph, tracers) in MODEL3D_V02_CONFIGS.items():
+        phys = Physics3D(**{**base_phys, **ph, "tracers": tracers})
+        params = SP(**{"theta": 0.5, "theta_cor": 0.5, "n_picard": 2, "theta_v": 0.5,
+                       "rtol": 1e-12, "max_iter": 2000, "solver": "fft", **sch})
+        st = Stepper3D(grid, phys, params, dt)
+        if tracers == "TS":
+            s = State3D(u=u0, v=v0, b=phys.buoyancy(T0, S0), eta=eta0, T=T0, S=S0)
+        else:
+            s = State3D(u=u0, v=v0, b=b0, eta=eta0)
+        for n in range(1, 4):
+            s = st.step(s)
+            for k in ("u", "v", "b", "eta", "T", "S"):
+                a = getattr(s, k)
+                if a is not None:
+                    rec[f"{name}_s{n}_{k}"] = a
+            rec[f"{name}_s{n}_counters"] = np.array([st.solver_iterations,
+                                                     st.barotropic_substeps, st.n_split])
+    rec["dt"] = np.array(dt)
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", type=Path, default=Path(__file__).resolve().parent)
@@ -248,8 +312,14 @@ def main() -> int:
                     help="rewrite only the composed-step fixtures")
     ap.add_argument("--model2d-only", action="store_true",
                     help="rewrite only the 2D-model fixture")
+    ap.add_argument("--model3d-v02-only", action="store_true",
+                    help="rewrite only the v0.2 3D-model fixture")
     args = ap.parse_args()
     args.out.mkdir(parents=True, exist_ok=True)
+    if args.model3d_v02_only:
+        np.savez_compressed(args.out / "model3d_v02_16x12x5.npz", **model3d_v02_fixtures())
+        print("model3d_v02_16x12x5.npz written")
+        return 0
     if args.model2d_only:
         np.savez_compressed(args.out / "model2d_igw_broadband_32x24.npz", **model2d_fixtures())
         print("model2d_igw_broadband_32x24.npz written")
@@ -267,6 +337,8 @@ def main() -> int:
     print("lock_exchange_128x4x20.npz written, iterations", rec["iters_per_step"].sum())
     np.savez_compressed(args.out / "model2d_igw_broadband_32x24.npz", **model2d_fixtures())
     print("model2d_igw_broadband_32x24.npz written")
+    np.savez_compressed(args.out / "model3d_v02_16x12x5.npz", **model3d_v02_fixtures())
+    print("model3d_v02_16x12x5.npz written")
     return 0
 
 

@@ -48,7 +48,8 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     structs = {"hoc_config": _lib.HocConfig, "hoc_geometry": _lib.HocGeometry,
                "hoc_state": _lib.HocState, "hoc_counters": _lib.HocCounters,
                "hoc_sw2d_config": _lib.HocSw2dConfig,
-               "hoc_helm2d_config": _lib.HocHelm2dConfig}
+               "hoc_helm2d_config": _lib.HocHelm2dConfig,
+               "hoc_m3_config": _lib.HocM3Config}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
              'int main(void) {']
     for cname, py in structs.items():

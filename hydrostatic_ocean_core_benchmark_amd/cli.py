@@ -6,13 +6,18 @@ subcommands with `--case/--nx/--cfl/--steps`).
     python main.py ... verify --against PREFIX_state3d5.bin [--steps K]
     python main.py ... verify --case igw [--study space|time]
     python main.py ... bench  [--sweep 128,256,512] [--steps K] [--repeat R]
+    python main.py ... verify3d [--case C] [--sweep 16,32,64] [--nz N] [--scale-nz] [--dim nx|nz]
+    python main.py ... bench3d  [--case C] [--sweep 64,100,128] [--nz N] [--repeat R]
 
 With --case igw | igw_broadband | geo_balance the subcommands drive the 2D
 shallow-water model instead, as the reference's own run|verify|bench do
 (main.py:45-275): run integrates one case and reports the error against its
 exact solution, mass and energy drift; verify --study space|time is the
 convergence study (observed order between refinements); bench is the
-error-vs-wallclock frontier over the CFL sweep.
+error-vs-wallclock frontier over the CFL sweep. verify3d and bench3d drive
+the v0.2 3D model (main.py:199-264, libs/core/driver3d.py) over its five
+cases: the convergence table against the exact solutions and the per-cell
+cost across grid sizes.
 
 run     integrate one case (lock_exchange or basin) and write the final state
         (the bundle's `_state3d5.bin` layout) and a metrics JSON into
@@ -259,6 +264,156 @@ def cmd_bench2d(config, args, case_name) -> int:
     return 0
 
 
+def _rel_l2(a, b) -> float:
+    den = float(np.sqrt(np.sum(b * b)))
+    num = float(np.sqrt(np.sum((a - b) ** 2)))
+    return num / den if den > 0.0 else num
+
+
+def simulate3d(config, nx: int, nz: int, cfl: float | None, case_name: str,
+               n_steps_override: int | None = None, n_repeat: int = 1) -> dict:
+    """One v0.2 3D run (libs/core/driver3d.py:53-126): dt from the CFL of
+    the forward-backward limit (or pinned by time.dt), n_steps to land on the
+    case's t_final (or pinned), the device Stepper3D integrated n_repeat
+    times (median wall time), metrics against the exact solution."""
+    import torch
+    from . import cases3d_v02 as C3
+    from .domain import CGrid
+    from .model2d import explicit_dt_max
+    from .model3d_v02 import Stepper3D
+    from .types import SchemeParams, State3D
+    grid = CGrid(nx=nx, ny=nx, Lx=float(config.get("grid.Lx")), Ly=float(config.get("grid.Ly")),
+                 bc_x=str(config.get("grid.bc_x")), bc_y=str(config.get("grid.bc_y")),
+                 nz=nz, depth=float(config.get("physics.H")))
+    physics = C3.physics3d_from_config(config)
+    params = SchemeParams(name=str(config.get("scheme.name")),
+                          theta=float(config.get("scheme.theta")),
+                          theta_cor=float(config.get("scheme.theta_cor")),
+                          theta_v=float(config.get("scheme.theta_v", 0.5)),
+                          advection=str(config.get("scheme.advection", "none")),
+                          n_split=int(config.get("scheme.n_split", 0)),
+                          barotropic_average=str(config.get("scheme.barotropic_average", "none")),
+                          n_picard=int(config.get("scheme.n_picard")),
+                          solver=str(config.get("solver.kind")),
+                          rtol=float(config.get("solver.rtol")),
+                          max_iter=int(config.get("solver.max_iter")))
+    params.validate()
+    case, t_final = C3.build_case3d(case_name, grid, physics, config)
+    dt_max = explicit_dt_max(grid, physics)
+    pinned = float(config.get("time.dt", 0.0))
+    factor = cfl if cfl is not None else float(config.get("time.cfl_factor"))
+    dt_req = pinned if pinned > 0.0 else factor * dt_max
+    if n_steps_override is not None:
+        n_steps, dt = int(n_steps_override), dt_req
+        t_final = dt * n_steps
+    else:
+        n_steps = max(1, int(round(t_final / dt_req)))
+        dt = t_final / n_steps
+    st = Stepper3D(grid, physics, params, dt)
+    initial = case.initial()
+    walls, fin = [], None
+    for _ in range(max(1, n_repeat)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fin = st.integrate(initial, n_steps)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    final = State3D(u=fin.u.cpu().numpy(), v=fin.v.cpu().numpy(), b=fin.b.cpu().numpy(),
+                    eta=fin.eta.cpu().numpy(), t=fin.t,
+                    T=None if fin.T is None else fin.T.cpu().numpy(),
+                    S=None if fin.S is None else fin.S.cpu().numpy())
+    exact = case.exact(t_final)
+    ordered = sorted(walls)
+    median = ordered[len(ordered) // 2] if len(ordered) % 2 else \
+        0.5 * (ordered[len(ordered) // 2 - 1] + ordered[len(ordered) // 2])
+    cells = grid.nx * grid.ny * grid.nz
+    solves = max(1, n_steps * params.n_picard)
+    lim = C3.time_step_limits(grid, physics, u_scale=float(np.max(np.abs(initial.u))) or 1.0)
+    dt_allowed = lim.for_scheme(params.name, params.advection, physics.A_h)
+    mem = C3.memory_model(grid, params.name)
+    per_step = median / n_steps
+    sdpd = dt_allowed / per_step if per_step > 0 else float("nan")
+    metrics = {
+        "case": case_name, "nx": grid.nx, "ny": grid.ny, "nz": grid.nz, "cells": cells,
+        "dx": grid.dx, "dz": grid.dz, "dt": dt, "dt_max_explicit": dt_max, "cfl": dt / dt_max,
+        "n_steps": n_steps, "t_final": t_final, "scheme": params.name, "theta": params.theta,
+        "theta_v": params.theta_v, "n_picard": params.n_picard, "solver": params.solver,
+        "h_eff": st.h_eff, "n_split": st.n_split,
+        "barotropic_substeps": st.barotropic_substeps,
+        "solver_iterations": st.solver_iterations,
+        "pcg_per_solve": st.solver_iterations / solves,
+        "tridiagonal_solves": st.tridiagonal_solves,
+        "l2_rel_u": _rel_l2(final.u, exact.u),
+        "l2_rel_T": (_rel_l2(final.T - physics.T0, exact.T - physics.T0)
+                     if final.T is not None else float("nan")),
+        "l2_rel_b": _rel_l2(final.b, exact.b) if np.any(exact.b) else float("nan"),
+        "l2_rel_eta": _rel_l2(final.eta, exact.eta) if np.any(exact.eta) else float("nan"),
+        "max_abs_eta": float(np.max(np.abs(final.eta))),
+        "wall_s": median, "us_per_step": 1e6 * median / n_steps,
+        "ns_per_cell_step": 1e9 * median / (n_steps * cells),
+        "dt_limit_scheme": dt_allowed, "dt_limits": lim.to_dict(),
+        "sim_seconds_per_wall_second": sdpd, "sim_days_per_wall_day": sdpd,
+        "bytes": mem["bytes"], "bytes_per_cell": mem["bytes_per_cell"], "gib": mem["gib"],
+        "max_nx_32gib_nz30": mem["max_nx_32gib_nz30"],
+    }
+    return {"grid": grid, "physics": physics, "params": params, "case": case,
+            "state": final, "exact": exact, "metrics": metrics}
+
+
+def cmd_verify3d(config, args) -> int:
+    """Convergence study of the v0.2 3D model (main.py:199-237)."""
+    case_name = args.case or "barotropic3d"
+    sizes = [int(n) for n in (args.sweep or "16,32,64").split(",")]
+    if args.dim == "nz":
+        pairs = [(args.nx or int(config.get("grid.nx")), n) for n in sizes]
+    elif args.scale_nz:
+        pairs = [(n, max(1, n // 2)) for n in sizes]
+    else:
+        pairs = [(n, args.nz) for n in sizes]
+    rows, errors = [], []
+    for n, (nx_i, nz_i) in zip(sizes, pairs):
+        m = simulate3d(config, nx_i, nz_i, args.cfl, case_name,
+                       n_steps_override=(args.steps or 4 * n))["metrics"]
+        field = {"baroclinic_igw": "l2_rel_b", "tracer_advect": "l2_rel_T"}.get(case_name,
+                                                                                  "l2_rel_u")
+        rows.append(m)
+        errors.append(m[field])
+    orders = _order(errors, [float(n) for n in sizes])
+    print(f"--- 3D convergence: case={case_name}, scheme={rows[0]['scheme']}, "
+          f"theta={rows[0]['theta']}, theta_v={rows[0]['theta_v']} ---")
+    print(f"{'nx':>6}{'nz':>5}{'cells':>10}{'dt [s]':>10}{'L2':>13}{'order':>7}{'pcg/solve':>11}")
+    for m, e, o in zip(rows, errors, orders):
+        m["observed_order"] = o
+        print(f"{m['nx']:>6}{m['nz']:>5}{m['cells']:>10}{m['dt']:>10.2f}"
+              f"{e:>13.4e}{o:>7.2f}{m['pcg_per_solve']:>11.1f}")
+    out = _run_dir(config, f"verify3d_{case_name}")
+    (out / "metrics.json").write_text(json.dumps({"study": "3d", "case": case_name,
+                                                  "rows": rows}, indent=2) + "\n",
+                                      encoding="utf-8")
+    return 0
+
+
+def cmd_bench3d(config, args) -> int:
+    """Per-cell cost of the v0.2 3D model across grid sizes (main.py:240-264)."""
+    case_name = args.case or "baroclinic_igw"
+    sizes = [int(n) for n in (args.sweep or "64,100,128").split(",")]
+    rows = []
+    print(f"{'nx':>6}{'nz':>5}{'cells':>10}{'steps':>7}{'wall [s]':>11}"
+          f"{'ms/step':>10}{'ns/cell/step':>14}{'pcg/solve':>11}")
+    for n in sizes:
+        m = simulate3d(config, n, args.nz, args.cfl, case_name,
+                       n_steps_override=(args.steps or 20), n_repeat=args.repeat)["metrics"]
+        rows.append(m)
+        print(f"{m['nx']:>6}{m['nz']:>5}{m['cells']:>10}{m['n_steps']:>7}"
+              f"{m['wall_s']:>11.4f}{m['us_per_step'] / 1000:>10.2f}"
+              f"{m['ns_per_cell_step']:>14.2f}{m['pcg_per_solve']:>11.1f}")
+    out = _run_dir(config, f"bench3d_{case_name}")
+    (out / "metrics.json").write_text(json.dumps({"mode": "bench3d", "case": case_name,
+                                                  "rows": rows}, indent=2) + "\n",
+                                      encoding="utf-8")
+    return 0
+
+
 def _case_2d(config, args):
     name = args.case or str(config.get("case.name", "lock_exchange"))
     return name if name in CASES_2D else None
@@ -357,9 +512,22 @@ def build_parser() -> argparse.ArgumentParser:
     p_ben = sub.add_parser("bench", help="step cost over grid sizes")
     p_ben.add_argument("--sweep", default=None, help="comma-separated nx list")
     p_ben.add_argument("--repeat", type=int, default=3)
-    for sp in (p_run, p_ver, p_ben):
+    p_v3d = sub.add_parser("verify3d", help="3D convergence study (v0.2 model)")
+    p_v3d.add_argument("--sweep", default=None, help="comma-separated nx list")
+    p_v3d.add_argument("--scale-nz", action="store_true",
+                       help="scale nz with nx (nz = nx/2) for a full 3D refinement")
+    p_v3d.add_argument("--dim", choices=("nx", "nz"), default="nx",
+                       help="which dimension the sweep refines (default nx)")
+    p_b3d = sub.add_parser("bench3d", help="3D cost across grid sizes (v0.2 model)")
+    p_b3d.add_argument("--sweep", default=None, help="comma-separated nx list")
+    p_b3d.add_argument("--repeat", type=int, default=1)
+    for sp in (p_v3d, p_b3d):
+        sp.add_argument("--nz", type=int, default=30)
+    for sp in (p_run, p_ver, p_ben, p_v3d, p_b3d):
         sp.add_argument("--case", default=None,
-                        help="lock_exchange | basin (3D) | igw | igw_broadband | geo_balance (2D)")
+                        help="lock_exchange | basin (3D) | igw | igw_broadband | geo_balance (2D)"
+                             " | barotropic3d | vdiffusion | baroclinic_igw | tracer_advect |"
+                             " ekman (verify3d, bench3d)")
         sp.add_argument("--nx", type=int, default=None)
         sp.add_argument("--cfl", type=float, default=None,
                         help="dt as a fraction of the surface-gravity limit")
@@ -373,7 +541,8 @@ def main(argv: list[str] | None = None) -> int:
         config = apply_overrides(load_config(args.config), args.set)
     except ValueError as exc:
         raise SystemExit(str(exc)) from exc
-    return {"run": cmd_run, "verify": cmd_verify, "bench": cmd_bench}[args.command](config, args)
+    return {"run": cmd_run, "verify": cmd_verify, "bench": cmd_bench, "verify3d": cmd_verify3d,
+            "bench3d": cmd_bench3d}[args.command](config, args)
 
 
 if __name__ == "__main__":

@@ -305,6 +305,50 @@ def model3d_v02_fixtures() -> dict:
     return rec
 
 
+def case_fixtures() -> dict:
+    """Initial and exact states of the reference's verification cases
+    (libs/core/cases.py, libs/core/cases3d.py) on small grids: the 2D
+    inertia-gravity wave, its broadband form and the geostrophic balance at
+    12 x 10, the five v0.2 3D cases at 8 x 6 x 4, each at t = 0 and at a
+    later time."""
+    from libs.core.cases import (GeostrophicBalance, InertiaGravityWave,
+                                 InertiaGravityWaveBroadband)
+    from libs.core.cases3d import (BaroclinicIGW, Barotropic3D, EkmanSpiral, TracerAdvection,
+                                   VerticalDiffusion)
+    from libs.core.model3d import Physics3D
+    from libs.core.schemes import Physics
+    rec = {}
+    g2 = CGrid(nx=12, ny=10, Lx=1.2e6, Ly=1.0e6)
+    p2 = Physics(g=9.80616, f=1.0e-4, H=1000.0)
+    for name, case in (("igw", InertiaGravityWave(g2, p2, 2, 1, 1.0)),
+                       ("igw_broadband", InertiaGravityWaveBroadband(g2, p2, 3, 1.0, 1.5)),
+                       ("geo_balance", GeostrophicBalance(g2, p2, 2, 1, 0.5))):
+        for tag, t in (("t0", 0.0), ("t1", 0.37 * case.period)):
+            s = case.initial() if t == 0.0 else case.exact(t)
+            for k in ("u", "v", "eta"):
+                rec[f"{name}_{tag}_{k}"] = getattr(s, k)
+        rec[f"{name}_period"] = np.array(case.period)
+    g3 = CGrid(nx=8, ny=6, Lx=8.0e5, Ly=6.0e5, nz=4, depth=500.0)
+    base = dict(g=9.80616, f=1.0e-4, H=500.0, N2=1.0e-4, nu=1.0e-2, kappa=1.0e-3)
+    cases = (("barotropic3d", lambda: Barotropic3D(g3, Physics3D(**base), 2, 1, 1.0)),
+             ("vdiffusion", lambda: VerticalDiffusion(g3, Physics3D(**{**base, "f": 0.0}), 1, 1.0)),
+             ("baroclinic_igw", lambda: BaroclinicIGW(g3, Physics3D(**base), 2, 1, 2, 1.0)),
+             ("tracer_advect", lambda: TracerAdvection(
+                 g3, Physics3D(**{**base, "f": 0.0, "tracers": "TS"}), 2, 1, 1.0, 0.5, 0.25)),
+             ("ekman", lambda: EkmanSpiral(g3, Physics3D(**{**base, "tau_x": 0.1, "tau_y": 0.05}),
+                                           2.0)))
+    for name, make in cases:
+        case = make()
+        for tag, t in (("t0", 0.0), ("t1", 0.37 * min(case.period, 1.0e6))):
+            s = case.initial() if t == 0.0 else case.exact(t)
+            for k in ("u", "v", "b", "eta", "T", "S"):
+                a = getattr(s, k)
+                if a is not None:
+                    rec[f"{name}_{tag}_{k}"] = a
+        rec[f"{name}_period"] = np.array(case.period)
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", type=Path, default=Path(__file__).resolve().parent)
@@ -314,8 +358,14 @@ def main() -> int:
                     help="rewrite only the 2D-model fixture")
     ap.add_argument("--model3d-v02-only", action="store_true",
                     help="rewrite only the v0.2 3D-model fixture")
+    ap.add_argument("--cases-only", action="store_true",
+                    help="rewrite only the verification-case fixture")
     args = ap.parse_args()
     args.out.mkdir(parents=True, exist_ok=True)
+    if args.cases_only:
+        np.savez_compressed(args.out / "cases_2d_3d.npz", **case_fixtures())
+        print("cases_2d_3d.npz written")
+        return 0
     if args.model3d_v02_only:
         np.savez_compressed(args.out / "model3d_v02_16x12x5.npz", **model3d_v02_fixtures())
         print("model3d_v02_16x12x5.npz written")
@@ -339,6 +389,8 @@ def main() -> int:
     print("model2d_igw_broadband_32x24.npz written")
     np.savez_compressed(args.out / "model3d_v02_16x12x5.npz", **model3d_v02_fixtures())
     print("model3d_v02_16x12x5.npz written")
+    np.savez_compressed(args.out / "cases_2d_3d.npz", **case_fixtures())
+    print("cases_2d_3d.npz written")
     return 0
 
 

@@ -29,9 +29,8 @@ verify  the reference's backend gate (tools/compare_backends3d5.py:28-29,
 bench   per-cell cost of the step over grid sizes (ns/cell/step, median of
         `--repeat` timed runs after one warm-up run).
 
-The reference's own `run|verify|bench` drive its 2D barotropic model, which is
-not on this backend's path; the contract (flags, config tree, overrides,
-output directory) is what is kept.
+Without a 2D case, run|verify|bench drive the v0.5 step with the same
+contract (flags, config tree, overrides, output directory).
 """
 
 from __future__ import annotations

@@ -9,19 +9,24 @@ Only tests/, `__graft_entry__.smoke()` and bench.py's CPU-baseline leg may
 import this package, and only as the checker or the timed CPU baseline. The
 product (`hydrostatic_ocean_core_benchmark_amd`) never imports it.
 
-Pinning (how each part is checked against something other than itself):
-  column.thomas            dense solve on tiny systems; discrete vertical
-                           eigenmode decay (closed form)
+Pinning (how each part is checked against something other than itself;
+tests/test_oracle_pins.py unless named):
+  column.thomas            dense solve on random columns; discrete vertical
+                           eigenmode (closed form)
   column.* coefficients    matrix identity (I - theta dt Dz) vs the explicit
                            operator; reference fixtures
   horizontal.kappa_face    closed-form faces, Superbee definition, the
                            reference's recorded operator errors (v6_1)
-  eos                      closed forms; TEOS-10 check value
-  closure                  reference fixtures; Kato-Phillips golden rows
-  helmholtz                SPD/symmetry, SciPy direct solve, the reference's
-                           recorded iteration counts
+  eos                      closed forms; TEOS-10 check value (test_gpu_parity)
+  closure                  reference fixtures; the recorded mixing-length
+                           checks; recorded Kato-Phillips hourly depths (v6_3)
+  helmholtz                SPD/symmetry and a dense direct solve; the
+                           recorded R2 gate (314 / 157 / 10 iterations,
+                           4.16147e-10); reference fixtures
   stepper                  reference fixtures (1..3 steps, and 200 steps of
-                           the 128x4x20 lock exchange)
+                           the 128x4x20 lock exchange; test_oracle_golden)
+  model2d, model3d_v02     reference fixtures of the original 2D and v0.2
+                           models (test_model2d, test_model3d_v02)
 The `pcg_mg` solver and the array-valued wind stress / background
 diffusivity have no reference counterpart: parity unpinned beyond their
 reduction to the reference when uniform (pcg_mg is pinned only by its

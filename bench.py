@@ -32,6 +32,13 @@ The JSON line also carries
                 HIP graph per two steps, beside the members run one by one;
   adjoint       the adjoint:forward wall-time ratio of dJ/dT0 and dJ/dkappa_b
                 on the 256 x 256 x 30 basin (60 steps, checkpoints every 10);
+  ladder        the reference's own published workload, lock_exchange_v06
+                (TKE + up3_tvd + TEOS-10 over rough topography, tier-2 case of
+                tools/tier2_sweep.sh:85) at the grids of its hardware ladder,
+                timed as the reference times it (50 steps from the initial
+                state, median of repeats) beside the H100 CUDA ms/step the
+                reference publishes for the same grid and scheme
+                (docs/32_hardware_ladder.md); context, not vs_baseline;
   cpu_baseline  the NumPy oracle of the same step on one host core, timed on
                 a bounded sample (the 128 x 128 x 40 basin from the same
                 generator: 1 warm-up step, then 10 timed steps).
@@ -100,6 +107,9 @@ def parse():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ladder", default=",".join(LADDER_DEFAULT),
+                    help="reference hardware-ladder workloads to time: comma list of "
+                         f"{', '.join(LADDER)}, 'all' or 'none'")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="steps of the 64-member ensemble throughput run (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=60,
@@ -260,6 +270,70 @@ def ensemble_throughput(steps: int = 100) -> dict:
             "setup_s": round(setup, 1)}
 
 
+# name: (nx, scheme, solver, cfl, published H100 CUDA ms/step, source line)
+LADDER = {
+    "fb_400": (400, "fb", "multigrid", 0.5, 7.59, "docs/32_hardware_ladder.md:137"),
+    "fb_1000": (1000, "fb", "multigrid", 0.5, 48.91, "docs/32_hardware_ladder.md:140"),
+    "fb_2000": (2000, "fb", "multigrid", 0.5, 185.50, "docs/32_hardware_ladder.md:143"),
+    "theta_mg_2000": (2000, "theta", "multigrid", 2.0, 214.16, "docs/32_hardware_ladder.md:109"),
+    "theta_pcgj_2000": (2000, "theta", "pcg_jacobi", 2.0, 206.77,
+                        "docs/32_hardware_ladder.md:117"),
+}
+LADDER_DEFAULT = tuple(LADDER)
+
+
+def reference_ladder(names, steps: int = 50, repeats: int = 3) -> dict:
+    """The reference's hardware-ladder workloads on this GPU: per workload, a
+    5-step warm-up, then `repeats` runs of `steps` steps from the initial
+    state, median ms/step, beside the published H100 CUDA figure."""
+    import numpy as np
+    import torch
+    from hydrostatic_ocean_core_benchmark_amd.cases import ladder_lock_exchange
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    out = {"case": "lock_exchange_v06 (tools/tier2_sweep.sh:85), 30 levels, fp64",
+           "protocol": f"5-step warm-up, median of {repeats} runs of {steps} steps from the "
+                       "initial state", "rows": []}
+    for name in names:
+        nx, scheme, solver, cfl, h100, src = LADDER[name]
+        t_setup = time.perf_counter()
+        prob = ladder_lock_exchange(nx, 30, scheme, solver, cfl, steps)
+        st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+        init = _initial_fields(prob, st, torch)
+        bufs = [st._new_state_buffers(), st._new_state_buffers()]
+
+        def run(n):
+            cur = init
+            for k in range(n):
+                st._step_into(cur, bufs[k % 2])
+                cur = bufs[k % 2]
+            return cur
+
+        run(5)
+        torch.cuda.synchronize()
+        setup_s = time.perf_counter() - t_setup
+        walls, it0 = [], st.solver_iterations
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fin = run(steps)
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(walls)) / steps
+        cells = nx * nx * 30
+        out["rows"].append({
+            "workload": name, "grid": f"{nx}x{nx}x30", "scheme": prob.meta["scheme"],
+            "cfl": cfl, "dt": prob.dt, "ms_per_step": round(ms, 3),
+            "cell_steps_per_s": cells / (ms / 1e3),
+            "solver_iterations_per_step": (st.solver_iterations - it0) / (repeats * steps),
+            "solver_failures": st.solver_failures,
+            "state_finite": bool(torch.isfinite(fin["T"]).all()),
+            "h100_cuda_ms_per_step": h100, "h100_source": src,
+            "speedup_vs_h100_cuda": round(h100 / ms, 2), "setup_s": round(setup_s, 1)})
+        del st, bufs, init, fin
+        torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -369,6 +443,9 @@ def main() -> int:
         rec["ensemble"] = ensemble_throughput(args.ensemble_steps)
     if args.adjoint_steps > 0:
         rec["adjoint"] = adjoint_ratio(args.adjoint_steps)
+    if args.ladder != "none":
+        names = list(LADDER) if args.ladder == "all" else args.ladder.split(",")
+        rec["ladder"] = reference_ladder(names)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(args.nz, prob.params.solver)
     print(json.dumps(rec), flush=True)

@@ -9,6 +9,15 @@ lock_exchange(): the 128x4x20 lock exchange -- flat 20 m, Lx = 64 km,
     the reference's lock_exchange_v06 values (tools/tier2_sweep.sh:85):
     f0 = 0, nu = 1e-5, kappa = 1e-6, tau_x = 0.05, A_h = K_h = 10, pgf = keep.
 
+ladder_lock_exchange(): the reference's published hardware-ladder workload,
+    lock_exchange_v06 (tools/tier2_sweep.sh:85 over config/*.toml): nx x nx x
+    nz over `rough` topography (r 0.1, seed 20260911), Lx = 64 km, Ly = 6.4 km,
+    H = 20 m, periodic, z-level; TEOS-10, up3_tvd, TKE (integral), A_h = K_h =
+    10, nu = 1e-5, kappa = 1e-6, tau_x = 0.05, f0 = 0, pgf keep; the v0.5
+    LockExchange initial state with delta_T = 2 (libs/core/cases3d_v05.py:
+    245-258); dt = cfl x the barotropic limit of the deepest column
+    (tools/toml2nml.py:143-170).
+
 basin(): a closed basin of nx x ny x nz cells at spacing dx with z-level
     partial-cell bathymetry from a seeded Gaussian random field (correlation
     length 200 km) clipped to 500-4000 m, >= 3 wet layers; initial
@@ -65,6 +74,31 @@ def lock_exchange(nx: int = 128, ny: int = 4, nz: int = 20, n_steps: int = 200,
     S = np.full(grid.shape3d, 35.0) * dom.mask3
     return Problem("lock_exchange", dom, phys, params, 1.0, n_steps, T, S,
                    {"grid": f"{nx}x{ny}x{nz}", "Lx": grid.Lx, "Ly": grid.Ly})
+
+
+def ladder_lock_exchange(nx: int, nz: int = 30, scheme: str = "fb", solver: str = "multigrid",
+                         cfl: float = 0.5, n_steps: int = 50) -> Problem:
+    grid = CGrid(nx=nx, ny=nx, Lx=6.4e4, Ly=6.4e3, nz=nz, depth=20.0)
+    H, mask = build_bathymetry(grid, "rough", 20.0, h_rel=0.9, length=0.0, slope=0.5,
+                               spectrum_slope=1.5, spectrum_kmax=0, r_target=0.1, seed=20260911,
+                               island=False)
+    dom = Domain(grid=grid, H=H, mask=mask, vcoord="zlevel", bc_x="periodic", bc_y="periodic",
+                 min_partial=0.1, face_rule="min")
+    base = Physics3D(g=9.80616, f=0.0, H=20.0, N2=0.0, nu=1e-5, kappa=1e-6, rho0=1025.0,
+                     tau_x=0.05, tau_y=0.0, bottom_drag=0.0, tracers="TS", alpha_T=2e-4,
+                     beta_S=7.4e-4, T0=10.0, S0=35.0, cp=3990.0, A_h=10.0, K_h=10.0)
+    phys = PhysicsV05(base=base, eos="teos10", pgf="keep", closure="tke", mxl="integral")
+    params = SchemeParams(name=scheme, theta=0.5, theta_cor=0.5, n_picard=2, theta_v=0.5,
+                          advection="up3_tvd", solver=solver, rtol=1e-12, max_iter=2000)
+    c = np.sqrt(base.g * float(np.max(H[mask > 0])))
+    dt = cfl * (1.0 / (c * np.sqrt(1.0 / grid.dx ** 2 + 1.0 / grid.dy ** 2)))
+    x, _ = grid.coords("eta")
+    warm = (x > 0.5 * grid.Lx).astype(np.float64)
+    T = (base.T0 + 2.0 * warm)[None] * np.ones(grid.shape3d) * dom.mask3
+    S = np.full(grid.shape3d, base.S0) * dom.mask3
+    label = "fb" if scheme == "fb" else f"theta0.5 {solver}"
+    return Problem("lock_exchange_v06", dom, phys, params, dt, n_steps, T, S,
+                   {"grid": f"{nx}x{nx}x{nz}", "scheme": label, "cfl": cfl})
 
 
 def basin(nx: int = 512, ny: int | None = None, nz: int = 40, dx: float = 1.0e4,

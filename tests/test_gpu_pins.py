@@ -10,9 +10,9 @@ import pytest
 
 from test_oracle_pins import V6_3_H, _r2_case, kato_phillips_rows
 
-pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("kind,iters", [("pcg_jacobi", 314), ("pcg_rbgs", 157),
                                         ("multigrid", 10)])
 def test_helmholtz_r2_gate_on_device(gpu_available, kind, iters):
@@ -40,8 +40,93 @@ def test_helmholtz_r2_gate_on_device(gpu_available, kind, iters):
     assert np.array_equal(x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x), want)
 
 
+@pytest.mark.gpu
 def test_kato_phillips_30_hours_on_device(gpu_available):
     """30 hours (1800 steps) of the wind-mixed column on the device: the
     hourly mixed-layer depths equal the recorded ones."""
     from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
     assert kato_phillips_rows(HipStepper, 30) == [float(h) for h in V6_3_H]
+
+
+TIER2_LOCK_EXCHANGE_V06 = [  # tools/tier2_sweep.sh:85 (case lock_exchange_v06)
+    "bathymetry.kind=rough", "physics3d_v04.tracers=TS", "scheme.advection=up3_tvd",
+    "physics3d_v06.closure=tke", "physics3d_v05.eos=teos10", "physics3d_v04.A_h=10.0",
+    "physics3d_v04.K_h=10.0", "physics.f0=0.0", "physics3d.N2=0.0", "physics3d.nu=1e-5",
+    "physics3d.kappa=1e-6", "physics3d.tau_x=0.05", "physics3d_v05.pgf=keep", "grid.Lx=6.4e4",
+    "grid.Ly=6.4e3", "physics.H=20.0"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx,kind,total,dt_rec", [
+    (400, "multigrid", 3012, "4.031701e+00"), (400, "pcg_jacobi", 5076, "4.031701e+00"),
+    (1000, "multigrid", 2856, "1.569937e+00"), (1000, "pcg_jacobi", 4956, "1.569937e+00")])
+def test_tier2_lock_exchange_iteration_totals(gpu_available, nx, kind, total, dt_rec):
+    """The recorded tier-2 rows (expr/E13_tier2_ladder/data/
+    tier2_ktcloud_20260911-190530_main1_2445045.csv:12,14 and
+    tier2_ktcloud_20260911-191740_main1_2447311.csv:12,14): lock_exchange_v06
+    at 400 x 400 x 30 and 1000 x 1000 x 30 over rough topography, theta 0.5,
+    CFL 4, 50 steps -- 3012 / 2856 multigrid and 5076 / 4956 PCG-Jacobi
+    iterations in total. The case is built as tools/toml2nml.py:143-185 builds
+    it (the v0.5 LockExchange of libs/core/cases3d_v05.py:183-259, config
+    delta_T 2)."""
+    import torch
+    from pathlib import Path
+    from hydrostatic_ocean_core_benchmark_amd.cli import barotropic_dt_max
+    from hydrostatic_ocean_core_benchmark_amd.config import apply_overrides, load_config
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain, bathymetry_from_config
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    from hydrostatic_ocean_core_benchmark_amd.types import (SchemeParams, State3D,
+                                                            physics_v05_from_config)
+    root = Path(__file__).resolve().parents[1]
+    cfg = apply_overrides(load_config(str(root / "config")), TIER2_LOCK_EXCHANGE_V06)
+    grid = CGrid(nx=nx, ny=nx, Lx=float(cfg.get("grid.Lx")), Ly=float(cfg.get("grid.Ly")),
+                 nz=30, depth=float(cfg.get("physics.H")))
+    H, mask = bathymetry_from_config(grid, cfg)
+    dom = Domain(grid=grid, H=H, mask=mask, vcoord="zlevel", bc_x="periodic", bc_y="periodic",
+                 min_partial=0.1, face_rule="min")
+    phys = physics_v05_from_config(cfg)
+    params = SchemeParams(name="theta", theta=0.5, theta_cor=0.5, n_picard=2, theta_v=0.5,
+                          advection="up3_tvd", solver=kind, rtol=1e-12, max_iter=2000)
+    dt = 4.0 * barotropic_dt_max(dom, phys.g)
+    assert f"{dt:.6e}" == dt_rec
+    st = HipStepper(dom, phys, params, dt)
+    x, _ = grid.coords("eta")
+    warm = (x > 0.5 * grid.Lx).astype(np.float64)
+    T = (phys.T0 + 2.0 * warm)[None, :, :] * np.ones(grid.shape3d)
+    S = np.full(grid.shape3d, phys.S0)
+    b = st.buoyancy_of(T, S).cpu().numpy() * dom.mask3
+    zero = np.zeros(grid.shape3d)
+    s = State3D(u=zero, v=zero, b=b, eta=np.zeros(grid.shape), t=0.0, T=T * dom.mask3,
+                S=S * dom.mask3)
+    import time
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = st.integrate(s, 50)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"tier-2 lock_exchange_v06 {nx}x{nx}x30 {kind}: {st.solver_iterations} iterations "
+          f"(recorded {total}), failures {st.solver_failures}, 50 steps in {wall:.3f} s "
+          f"(first call, includes set-up)", flush=True)
+    assert st.solver_failures == 0
+    assert st.solver_iterations == total
+
+
+def test_ladder_builder_matches_config_path():
+    """cases.ladder_lock_exchange builds the same domain, physics and dt as
+    the config tree with the tier-2 overrides (the path whose iteration
+    totals match the recorded ones above)."""
+    from pathlib import Path
+    from hydrostatic_ocean_core_benchmark_amd import cases as C
+    from hydrostatic_ocean_core_benchmark_amd.cli import barotropic_dt_max
+    from hydrostatic_ocean_core_benchmark_amd.config import apply_overrides, load_config
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, bathymetry_from_config
+    from hydrostatic_ocean_core_benchmark_amd.types import physics_v05_from_config
+    root = Path(__file__).resolve().parents[1]
+    cfg = apply_overrides(load_config(str(root / "config")), TIER2_LOCK_EXCHANGE_V06)
+    prob = C.ladder_lock_exchange(64, nz=30, scheme="theta", solver="multigrid", cfl=4.0)
+    grid = CGrid(nx=64, ny=64, Lx=float(cfg.get("grid.Lx")), Ly=float(cfg.get("grid.Ly")),
+                 nz=30, depth=float(cfg.get("physics.H")))
+    H, mask = bathymetry_from_config(grid, cfg)
+    assert np.array_equal(prob.domain.H, H) and np.array_equal(prob.domain.mask, mask)
+    assert prob.physics == physics_v05_from_config(cfg)
+    assert prob.dt == 4.0 * barotropic_dt_max(prob.domain, prob.physics.g)

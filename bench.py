@@ -239,8 +239,9 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30)
             "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
             "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
             "hand_written": "Thomas (transposed sweep), Helmholtz (PCG on the adjoint rhs), "
-                            "tracer advection (reverse kernel); the rest by autograd through "
-                            "the torch restatement"}
+                            "tracer and momentum advection, Gaspar mixing lengths and the EOS "
+                            "(reverse kernels); the rest by autograd through the torch "
+                            "restatement"}
 
 
 def ensemble_throughput(steps: int = 100) -> dict:

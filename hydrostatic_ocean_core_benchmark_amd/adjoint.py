@@ -1032,6 +1032,6 @@ class Adjoint:
         g_kb += gk0 if gk0 is not None else 0.0
         torch.cuda.synchronize(self.device)
         gradient_s = time.perf_counter() - t0
-        return GradientResult(J=float(J), dJ_dT0=gT.detach(), dJ_dkappa=g_kb.detach(),
+        return GradientResult(J=float(J.detach()), dJ_dT0=gT.detach(), dJ_dkappa=g_kb.detach(),
                               forward_s=forward_s, gradient_s=gradient_s, n_steps=n_steps,
                               every=every)

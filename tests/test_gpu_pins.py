@@ -1,7 +1,10 @@
 """The device path against numbers the reference's own verification runs
 recorded: the R2 Helmholtz gate iteration counts (docs/25_kernel_matrix.md:
-37-43) and 30 hours of Kato-Phillips mixed-layer deepening under the TKE
-closure (v06_verification_20260923.json, key v6_3)."""
+37-43), 30 hours of Kato-Phillips mixed-layer deepening under the TKE
+closure (v06_verification_20260923.json, key v6_3) and the tier-2
+lock-exchange solver-iteration totals at 400^2 and 1000^2 (expr/E13_tier2_ladder);
+plus a CPU check that the benchmark's ladder workload builder agrees with
+the config path."""
 
 from __future__ import annotations
 

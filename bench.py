@@ -15,9 +15,11 @@ A "step" is one full model time step. cells = nx*ny*nz (the reference's
 convention, dry cells included).
 
 The JSON line also carries
-  roofline      the dominant kernel's algorithmic bytes per launch over its
+  roofline      the dominant kernel's (k_predictor, the largest share of the
+                step at 512 x 512 x 40) algorithmic bytes per launch over its
                 average launch time (HIP events on the launch stream, inside
-                the timed region), against 8 TB/s;
+                the timed region), against 8 TB/s; roofline_tracers the same
+                for the tracer kernel (second by share) over 10 further steps;
   step_model    the whole step against the stage model of the step's
                 compulsory traffic: 336 B per layer cell plus, per barotropic
                 solve, 64 B per column and B_it per column and iteration
@@ -92,8 +94,12 @@ def parse():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--dx", type=float, default=1.0e4)
     ap.add_argument("--dt", type=float, default=600.0)
-    ap.add_argument("--kernel", default="k_tracers",
-                    help="kernel timed with HIP events for the roofline figure")
+    ap.add_argument("--kernel", default="k_predictor",
+                    help="kernel timed with HIP events for the roofline figure (the largest "
+                         "share of step time at 512 x 512 x 40)")
+    ap.add_argument("--kernel2", default="k_tracers",
+                    help="second kernel timed over 10 extra steps for roofline_<name> "
+                         "('none' to skip)")
     ap.add_argument("--solver-path", choices=("multikernel", "persistent"), default="multikernel",
                     help="Helmholtz launch strategy (bitwise the same iterates)")
     ap.add_argument("--solver", default=None,
@@ -400,17 +406,28 @@ def main() -> int:
 
     cells = g.nx * g.ny * g.nz
     value = cells * args.steps / wall
-    roofline = None
-    if launches > 0 and args.kernel in ALG_BYTES_PER_CELL:
+    def roofline_of(kernel, launches, kms):
+        if launches <= 0 or kernel not in ALG_BYTES_PER_CELL:
+            return None
         avg_s = kms / launches / 1e3
-        alg = ALG_BYTES_PER_CELL[args.kernel] * cells
+        alg = ALG_BYTES_PER_CELL[kernel] * cells
         achieved = alg / avg_s / 1e9
-        traffic = (MEASURED_TRAFFIC_512x40.get(args.kernel)
+        traffic = (MEASURED_TRAFFIC_512x40.get(kernel)
                    if (g.nx, g.ny, g.nz) == (512, 512, 40) else None)
-        roofline = {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "alg_bytes_per_launch": alg,
-                    "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "alg_bytes_per_launch": alg,
+                "avg_launch_ms": round(avg_s * 1e3, 4), "launches": launches}
+
+    roofline = roofline_of(args.kernel, launches, kms)
+    roofline2 = None
+    if args.kernel2 != "none":
+        st.profile_kernel(args.kernel2)
+        for _ in range(10):
+            one_step()
+        torch.cuda.synchronize()
+        roofline2 = roofline_of(args.kernel2, *st.profile_read())
+        st.profile_kernel(None)
     n_solves = prob.params.n_picard if prob.params.name == "theta" else 0
     solve_bytes = g.nx * g.ny * (SOLVE_BYTES_PER_COLUMN * n_solves
                                  + ITER_BYTES_PER_COLUMN[prob.params.solver] * iters / args.steps)
@@ -436,6 +453,8 @@ def main() -> int:
         "roofline": roofline,
         "step_model": step_model,
     }
+    if roofline2 is not None:
+        rec[f"roofline_{args.kernel2.removeprefix('k_')}"] = roofline2
     if args.mixed_steps > 0:
         del st, bufs, cur
         torch.cuda.empty_cache()

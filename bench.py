@@ -15,11 +15,13 @@ A "step" is one full model time step. cells = nx*ny*nz (the reference's
 convention, dry cells included).
 
 The JSON line also carries
-  roofline      the dominant kernel's (k_predictor, the largest share of the
-                step at 512 x 512 x 40) algorithmic bytes per launch over its
-                average launch time (HIP events on the launch stream, inside
-                the timed region), against 8 TB/s; roofline_tracers the same
-                for the tracer kernel (second by share) over 10 further steps;
+  roofline      the tracer kernel's algorithmic bytes per launch (stage E of
+                the step's byte model, 72 B per layer cell) over its average
+                launch time (HIP events on the launch stream, inside the timed
+                region), against 8 TB/s; roofline_predictor the same for the
+                predictor (tied with it for the largest share of the step) with
+                the kernel's own compulsory bytes, 131 B per layer cell, over
+                10 further steps (no stage of the model is that kernel alone);
   step_model    the whole step against the stage model of the step's
                 compulsory traffic: 336 B per layer cell plus, per barotropic
                 solve, 64 B per column and B_it per column and iteration
@@ -68,10 +70,14 @@ HBM_PEAK_GBS = 8000.0
 # ignored). Stage figures follow the stage model of the step:
 #   k_tracers      T, S, u, v, w, K_h, dz3 read; T, S, b written     (72 B)
 #   k_closure      e, K_m, K_h, u, v, b, dz3 read; e, K_m, K_h written (80 B)
-#   k_predictor    u, v, (u, v)_iter, 4 explicit terms x 2, K_m, dz3u, dz3v
-#                  read; gh_u, gh_v written                        (128 B)
+#   k_predictor    u, v, the Picard iterate (u, v)_it, pgx, pgy, the Coriolis
+#                  partners av_prev (2), explicit diffusion (2) and stress (2)
+#                  terms, the Thomas factors sub, den, c* read, masks m3, m3u,
+#                  m3v (3 B); gh_u, gh_v written: 139 B, or 123 B in the first
+#                  Picard pass (the iterate is the state itself); 131 B per
+#                  launch on average over the two passes
 #   k_momentum_explicit  u, v, b, w, K_m read; 8 explicit terms written (104 B)
-ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 128.0,
+ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 131.0,
                       "k_momentum_explicit": 104.0}
 # Measured HBM bytes per launch at 512 x 512 x 40 (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
@@ -94,10 +100,10 @@ def parse():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--dx", type=float, default=1.0e4)
     ap.add_argument("--dt", type=float, default=600.0)
-    ap.add_argument("--kernel", default="k_predictor",
-                    help="kernel timed with HIP events for the roofline figure (the largest "
-                         "share of step time at 512 x 512 x 40)")
-    ap.add_argument("--kernel2", default="k_tracers",
+    ap.add_argument("--kernel", default="k_tracers",
+                    help="kernel timed with HIP events for the roofline figure (the tracer "
+                         "kernel: one whole stage of the step's byte model)")
+    ap.add_argument("--kernel2", default="k_predictor",
                     help="second kernel timed over 10 extra steps for roofline_<name> "
                          "('none' to skip)")
     ap.add_argument("--solver-path", choices=("multikernel", "persistent"), default="multikernel",

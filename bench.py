@@ -119,6 +119,8 @@ def parse():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ladder-only", action="store_true",
+                    help="time only the ladder workloads and print {'ladder': ...} (profiling)")
     ap.add_argument("--ladder", default=",".join(LADDER_DEFAULT),
                     help="reference hardware-ladder workloads to time: comma list of "
                          f"{', '.join(LADDER)}, 'all' or 'none'")
@@ -362,6 +364,10 @@ def main() -> int:
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         return 2
+    if args.ladder_only:
+        names = list(LADDER) if args.ladder in ("all", "none") else args.ladder.split(",")
+        print(json.dumps({"ladder": reference_ladder(names)}), flush=True)
+        return 0
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 8
         src = torch.arange(n, dtype=torch.float64, device="cuda")

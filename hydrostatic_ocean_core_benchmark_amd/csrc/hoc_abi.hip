@@ -536,10 +536,22 @@ void rbgs_sweep(hoc_stepper* h, int l, const double* b, double* x, bool reverse,
   k_mg_rbgs<<<blocks(L.n), TPB, 0, s>>>(L, 1 - c0, 1, b, t, x, h->sc);
 }
 
+// Two sweeps in a row on x (directions r1 then r2), one launch each through
+// the level's spare buffer t and back; the result is back in x and equals
+// two calls of rbgs_sweep bit for bit.
+void rbgs_sweep_pair(hoc_stepper* h, int l, const double* b, double* x, bool r1, bool r2,
+                     hipStream_t s) {
+  const MgLevel& L = h->lev[l];
+  double* t = h->mg_t[l];
+  const dim3 grid((unsigned)((L.nx + RB_TW - 1) / RB_TW), (unsigned)((L.ny + RB_TH - 1) / RB_TH));
+  k_mg_rbgs2<<<grid, RB_TW * RB_TH, 0, s>>>(L, r1 ? 1 : 0, b, x, t, h->sc);
+  k_mg_rbgs2<<<grid, RB_TW * RB_TH, 0, s>>>(L, r2 ? 1 : 0, b, t, x, h->sc);
+}
+
 // The reference multigrid V-cycle on x (solvers_var.py:139-151): n_pre
 // forward RBGS sweeps, coarse correction, n_post reverse sweeps.
 void vcycle_rbgs(hoc_stepper* h, int l, double* x, const double* b, hipStream_t s) {
-  for (int k = 0; k < 2; ++k) rbgs_sweep(h, l, b, x, false, s);
+  rbgs_sweep_pair(h, l, b, x, false, false, s);
   if (l + 1 < h->nlev) {
     const MgLevel& C = h->lev[l + 1];
     k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(h->lev[l], C, x, b, h->mg_b[l + 1], h->sc);
@@ -547,7 +559,7 @@ void vcycle_rbgs(hoc_stepper* h, int l, double* x, const double* b, hipStream_t 
     vcycle_rbgs(h, l + 1, h->mg_x[l + 1], h->mg_b[l + 1], s);
     k_mg_prolong<<<blocks(h->lev[l].n), TPB, 0, s>>>(h->lev[l], h->mg_x[l + 1], C.nx, x, h->sc);
   }
-  for (int k = 0; k < 2; ++k) rbgs_sweep(h, l, b, x, true, s);
+  rbgs_sweep_pair(h, l, b, x, true, true, s);
 }
 
 // pcg_mg preconditioner: one damped-Jacobi V-cycle from a zero guess
@@ -578,8 +590,7 @@ double* vcycle_jacobi(hoc_stepper* h, int l, const double* b, double* x, double*
 int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
   if (h->cfg.solver == SOLVER_PCG_RBGS) {
     k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
-    rbgs_sweep(h, 0, r, z, false, s);
-    rbgs_sweep(h, 0, r, z, true, s);
+    rbgs_sweep_pair(h, 0, r, z, false, true, s);
   } else if (h->cfg.precision == 1) {
     // the fp32 V-cycle: as the fused fp64 path below, every level's vectors
     // and coefficients in fp32; level 0 reads the fp64 residual r and writes

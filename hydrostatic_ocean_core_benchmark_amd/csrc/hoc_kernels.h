@@ -323,6 +323,9 @@ __global__ void k_pcg_rz(PcgArgs a, int into_b);
 __global__ void k_mg_coarsen(MgLevel f, MgLevel c, double* cKu, double* cKv, uint8_t* cm);
 __global__ void k_mg_diag(MgLevel L, double* inv_diag);
 __global__ void k_mg_zero(long long n, double* x, const PcgScalars* sc);
+constexpr int RB_TW = 32, RB_TH = 8;
+__global__ void k_mg_rbgs2(MgLevel L, int c0, const double* b, const double* xin, double* xout,
+                           const PcgScalars* sc);
 __global__ void k_mg_rbgs(MgLevel L, int colour, int last, const double* b, const double* xin,
                           double* xout, const PcgScalars* sc);
 __global__ void k_mg_jacobi(MgLevel L, double omega, const double* x, const double* b, double* xo,

@@ -144,6 +144,51 @@ __global__ void k_mg_rbgs(MgLevel L, int colour, int last, const double* __restr
   xout[c] = last ? v * mk(L.mask[c]) : v;
 }
 
+// One whole red-black sweep (first colour c0, then the other) in one launch,
+// xout = sweep(xin) masked, exactly as the two launches of k_mg_rbgs compute
+// it: a RB_TW x RB_TH tile computes the first colour on the tile and a
+// one-cell halo from the old field into LDS (the other colour keeps its old
+// value there), then the second colour on the tile from the LDS values.
+// Every first-colour update reads the old field only and every second-colour
+// update reads first-colour values and old second-colour values, so odd
+// periodic sides (where same-coloured cells meet) keep the whole-array
+// semantics. Halo positions wrap; on sides shorter than the tile they alias
+// cells of the tile and hold the same values.
+__global__ __launch_bounds__(RB_TW * RB_TH) void k_mg_rbgs2(MgLevel L, int c0,
+                                                            const double* __restrict__ b,
+                                                            const double* __restrict__ xin,
+                                                            double* __restrict__ xout,
+                                                            const PcgScalars* sc) {
+  if (finished(sc)) return;
+  constexpr int SW = RB_TW + 2, SH = RB_TH + 2;
+  __shared__ double v1[SH][SW];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB_TW, j0 = blockIdx.y * RB_TH;
+  for (int q = tid; q < SW * SH; q += RB_TW * RB_TH) {
+    const int ly = q / SW, lx = q % SW;
+    const int i = mg_pmod(i0 + lx - 1, L.nx), j = mg_pmod(j0 + ly - 1, L.ny);
+    const long long g = (long long)j * L.nx + i;
+    v1[ly][lx] = (((i + j) & 1) == c0) ? L.inv_diag[g] * (b[g] + mg_neighbour_sum(L, xin, g))
+                                       : xin[g];
+  }
+  __syncthreads();
+  const int tx = tid % RB_TW, ty = tid / RB_TW;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i >= L.nx || j >= L.ny) return;
+  const long long c = (long long)j * L.nx + i;
+  double v;
+  if (((i + j) & 1) != c0) {
+    const long long cw = (long long)j * L.nx + wrap(i, -1, L.nx);
+    const long long cs = (long long)wrap(j, -1, L.ny) * L.nx + i;
+    const double nb = L.cx * (L.Ku[c] * v1[ty + 1][tx + 2] + L.Ku[cw] * v1[ty + 1][tx]) +
+                      L.cy * (L.Kv[c] * v1[ty + 2][tx + 1] + L.Kv[cs] * v1[ty][tx + 1]);
+    v = L.inv_diag[c] * (b[c] + nb);
+  } else {
+    v = v1[ty + 1][tx + 1];
+  }
+  xout[c] = v * mk(L.mask[c]);
+}
+
 // Damped Jacobi: xo = x + omega (inv_diag (b - A x)) mask.
 __global__ void k_mg_jacobi(MgLevel L, double omega, const double* __restrict__ x,
                             const double* __restrict__ b, double* __restrict__ xo,

@@ -554,8 +554,8 @@ void vcycle_rbgs(hoc_stepper* h, int l, double* x, const double* b, hipStream_t 
   rbgs_sweep_pair(h, l, b, x, false, false, s);
   if (l + 1 < h->nlev) {
     const MgLevel& C = h->lev[l + 1];
-    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(h->lev[l], C, x, b, h->mg_b[l + 1], h->sc);
-    k_mg_zero<<<blocks(C.n), TPB, 0, s>>>(C.n, h->mg_x[l + 1], h->sc);
+    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(h->lev[l], C, x, b, h->mg_b[l + 1],
+                                                       h->mg_x[l + 1], h->sc);
     vcycle_rbgs(h, l + 1, h->mg_x[l + 1], h->mg_b[l + 1], s);
     k_mg_prolong<<<blocks(h->lev[l].n), TPB, 0, s>>>(h->lev[l], h->mg_x[l + 1], C.nx, x, h->sc);
   }
@@ -574,7 +574,8 @@ double* vcycle_jacobi(hoc_stepper* h, int l, const double* b, double* x, double*
   }
   if (l + 1 < h->nlev) {
     const MgLevel& C = h->lev[l + 1];
-    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(L, C, x, b, h->mg_b[l + 1], h->sc);
+    k_mg_residual_restrict<<<blocks(C.n), TPB, 0, s>>>(L, C, x, b, h->mg_b[l + 1], nullptr,
+                                                       h->sc);
     double* xc = vcycle_jacobi(h, l + 1, h->mg_b[l + 1], h->mg_x[l + 1], h->mg_t[l + 1], s);
     k_mg_prolong<<<blocks(L.n), TPB, 0, s>>>(L, xc, C.nx, x, h->sc);
   }

@@ -331,7 +331,7 @@ __global__ void k_mg_rbgs(MgLevel L, int colour, int last, const double* b, cons
 __global__ void k_mg_jacobi(MgLevel L, double omega, const double* x, const double* b, double* xo,
                             const PcgScalars* sc);
 __global__ void k_mg_residual_restrict(MgLevel f, MgLevel c, const double* x, const double* b,
-                                       double* bc, const PcgScalars* sc);
+                                       double* bc, double* xc_zero, const PcgScalars* sc);
 __global__ void k_mg_prolong(MgLevel f, const double* xc, int cnx, double* x,
                              const PcgScalars* sc);
 __global__ void k_mg_res_parts(PcgArgs a, MgLevel L, const double* x);

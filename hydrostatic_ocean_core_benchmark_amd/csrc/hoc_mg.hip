@@ -203,7 +203,7 @@ __global__ void k_mg_jacobi(MgLevel L, double omega, const double* __restrict__ 
 // [0::2,0::2] + [0::2,1::2] + [1::2,0::2] + [1::2,1::2].
 __global__ void k_mg_residual_restrict(MgLevel f, MgLevel c, const double* __restrict__ x,
                                        const double* __restrict__ b, double* __restrict__ bc,
-                                       const PcgScalars* sc) {
+                                       double* __restrict__ xc_zero, const PcgScalars* sc) {
   if (finished(sc)) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
@@ -213,6 +213,7 @@ __global__ void k_mg_residual_restrict(MgLevel f, MgLevel c, const double* __res
   double r[4];
   for (int q = 0; q < 4; ++q) r[q] = (b[ch[q]] - mg_apply(f, x, ch[q])) * mk(f.mask[ch[q]]);
   bc[t] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
+  if (xc_zero) xc_zero[t] = 0.0;  // the coarse correction's zero start, when asked
 }
 
 // x += prolong(xc) * mask (piecewise constant).

@@ -366,11 +366,12 @@ int validate(const hoc_config& c) {
   if (c.solver == SOLVER_PCG_MG && (c.mg_sweeps < 1 || c.mg_levels < 1 || !(c.mg_omega > 0.0)))
     return fail(HOC_ERR_INVALID, "pcg_mg needs mg_levels >= 1, mg_sweeps >= 1, mg_omega > 0");
   if (c.max_iter < 1) return fail(HOC_ERR_INVALID, "max_iter must be >= 1");
-  if (c.precision < 0 || c.precision > 1)
-    return fail(HOC_ERR_INVALID, "precision must be fp64 (0) or mixed (1)");
-  if (c.precision == 1 && c.scheme == 0 && c.solver != SOLVER_PCG_MG)
+  if (c.precision < 0 || c.precision > 2)
+    return fail(HOC_ERR_INVALID, "precision must be fp64 (0), mixed (1) or mixed with fp32 "
+                                 "momentum fluxes (2)");
+  if (c.precision >= 1 && c.scheme == 0 && c.solver != SOLVER_PCG_MG)
     return fail(HOC_ERR_INVALID, "mixed precision runs the theta scheme with pcg_mg only");
-  if (c.precision == 1 && c.mg_sweeps > MG_FUSED_MAX_SWEEPS)
+  if (c.precision >= 1 && c.mg_sweeps > MG_FUSED_MAX_SWEEPS)
     return fail(HOC_ERR_INVALID, "mixed precision needs mg_sweeps <= 3");
   return HOC_OK;
 }
@@ -510,7 +511,7 @@ int mg_build(hoc_stepper* h, const double* Ku, const double* Kv, const double* i
                                              h->mg_mask[l]);
     k_mg_diag<<<blocks(L.n), TPB, 0, s>>>(L, h->mg_inv[l]);
   }
-  if (h->cfg.precision == 1) {
+  if (h->cfg.precision >= 1) {
     for (int l = 0; l < h->nlev; ++l) {
       const MgLevel& L = h->lev[l];
       MgLevelF& F = h->levf[l];
@@ -592,7 +593,7 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
   if (h->cfg.solver == SOLVER_PCG_RBGS) {
     k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
     rbgs_sweep_pair(h, 0, r, z, false, true, s);
-  } else if (h->cfg.precision == 1) {
+  } else if (h->cfg.precision >= 1) {
     // the fp32 V-cycle: as the fused fp64 path below, every level's vectors
     // and coefficients in fp32; level 0 reads the fp64 residual r and writes
     // the fp64 z (the outer PCG stays fp64)
@@ -1397,7 +1398,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     int _rc = h->alloc(&(ptr), (n));     \
     if (_rc != HOC_OK) return cleanup(_rc); \
   } while (0)
-  if (cfg->precision == 1) {
+  if (cfg->precision >= 1) {
     float G[4][7][7];
     teos_anomaly_table(cfg->T0, cfg->S0, h->p.eos_ref, G, &h->p.ss_ref);
     float* dg = nullptr;
@@ -1576,7 +1577,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
       arc = arc ? arc : h->alloc(&h->mg_x[l], L.n);
       arc = arc ? arc : h->alloc(&h->mg_t[l], L.n);
       arc = arc ? arc : h->alloc(&h->mg_b[l], L.n);
-      if (cfg->precision == 1) {
+      if (cfg->precision >= 1) {
         MgLevelF& F = h->levf[l];
         F.nx = lx;
         F.ny = ly;
@@ -1613,14 +1614,14 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
       C.nl = h->nlev - l0;
       for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
       if (mg_cycle_prepare(mg_cycle_layout(C)) == 0) h->mg_cycle_l0 = l0;
-      if (cfg->precision == 1) {
+      if (cfg->precision >= 1) {
         MgCycleF CF{};
         CF.nl = C.nl;
         for (int l = 0; l < CF.nl; ++l) CF.lev[l] = h->levf[l0 + l];
         if (mg_cycle_prepare_f32(mg_cycle_layout(CF)) != 0) h->mg_cycle_l0 = -1;
       }
     }
-    if (cfg->precision == 1 && h->mg_cycle_l0 < 0)
+    if (cfg->precision >= 1 && h->mg_cycle_l0 < 0)
       return cleanup(fail(HOC_ERR_INVALID, "mixed precision: the coarse end of the multigrid "
                                            "hierarchy does not fit one workgroup"));
   }
@@ -1821,7 +1822,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
-  launch_momentum_explicit_tiled(h->cfg.precision == 1, s, d, p, geo, in->u, in->v, in->b, h->phi,
+  launch_momentum_explicit_tiled(h->cfg.precision == 2, s, d, p, geo, in->u, in->v, in->b, h->phi,
                                  h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
                                  h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
@@ -1867,7 +1868,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
   { ProfScope _ps(h, "k_tracers", s);
-  launch_tracers_tiled(h->cfg.precision == 1, s, d, p, geo, in->T, in->S, in->b, out->u, out->v,
+  launch_tracers_tiled(h->cfg.precision >= 1, s, d, p, geo, in->T, in->S, in->b, out->u, out->v,
                        h->w, h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S, out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;

@@ -42,7 +42,7 @@ def _field2d(value, shape) -> np.ndarray:
     return _f64(np.broadcast_to(np.asarray(value, dtype=np.float64), shape))
 
 
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_fp32_momentum": 2}
 
 
 class HipStepper:
@@ -53,14 +53,18 @@ class HipStepper:
         """`ustar2` optionally replaces the surface friction-velocity field
         u*^2 [ny, nx] (default hypot(tau_x, tau_y) / rho0, as the reference).
         `precision`: "fp64" is the plain step (the reference's arithmetic);
-        "mixed" runs the pcg_mg V-cycle preconditioner, the advective fluxes
-        and the equation of state in fp32 and keeps the state updates, the
-        column solves and the outer PCG (residual, dot products, update) in
-        fp64 -- theta scheme with pcg_mg only."""
+        "mixed" runs the pcg_mg V-cycle preconditioner, the tracer advective
+        fluxes and the equation of state in fp32 and keeps the momentum
+        fluxes, the state updates, the column solves and the outer PCG
+        (residual, dot products, update) in fp64; "mixed_fp32_momentum" also
+        runs the momentum advective fluxes in fp32 (on MI355X that kernel's
+        fp32 form is slower than its fp64 form, and the step's error is the
+        same; kept for comparison) -- theta scheme with pcg_mg only."""
         params.validate()
         if precision not in PRECISION:
-            raise ValueError(f"unknown precision {precision!r} (expected fp64|mixed)")
-        if precision == "mixed" and params.name == "theta" and params.solver != "pcg_mg":
+            raise ValueError(f"unknown precision {precision!r} "
+                             "(expected fp64|mixed|mixed_fp32_momentum)")
+        if precision != "fp64" and params.name == "theta" and params.solver != "pcg_mg":
             raise ValueError("mixed precision runs the theta scheme with pcg_mg only")
         self.precision = precision
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:

@@ -61,8 +61,9 @@ typedef struct hoc_config {
   int32_t closure_tke, mxl_recursive;
   double c_k, c_eps, pr_t, kappa_vk, z_0, e_min, n2_min, l_min, e_bb;
   int32_t precision;       /* 0 fp64 (the plain step); 1 mixed: the pcg_mg V-cycle
-                              preconditioner, the advective fluxes and the EOS in fp32,
-                              state updates and the outer PCG in fp64 */
+                              preconditioner, the tracer advective fluxes and the EOS in
+                              fp32, momentum fluxes, state updates and the outer PCG in
+                              fp64; 2 as 1 with the momentum advective fluxes in fp32 */
 } hoc_config;
 
 /* Static z-level geometry and surface forcing (host arrays). Built like

@@ -99,5 +99,6 @@ def test_stepper_precision_arguments_are_checked_before_any_device_use():
     with pytest.raises(ValueError, match="unknown precision"):
         HipStepper(dom, phys, params, dt, precision="fp16")
     assert params.solver != "pcg_mg"
-    with pytest.raises(ValueError, match="pcg_mg only"):
-        HipStepper(dom, phys, params, dt, precision="mixed")
+    for prec in ("mixed", "mixed_fp32_momentum"):
+        with pytest.raises(ValueError, match="pcg_mg only"):
+            HipStepper(dom, phys, params, dt, precision=prec)

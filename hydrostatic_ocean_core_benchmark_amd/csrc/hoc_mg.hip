@@ -146,9 +146,10 @@ __global__ void k_mg_rbgs(MgLevel L, int colour, int last, const double* __restr
 
 // One whole red-black sweep (first colour c0, then the other) in one launch,
 // xout = sweep(xin) masked, exactly as the two launches of k_mg_rbgs compute
-// it: a RB_TW x RB_TH tile computes the first colour on the tile and a
-// one-cell halo from the old field into LDS (the other colour keeps its old
-// value there), then the second colour on the tile from the LDS values.
+// it: a RB_TW x RB_TH tile stages the old field with a two-cell halo in LDS,
+// computes the first colour on the tile and a one-cell halo from it (the
+// other colour keeps its old value there), then the second colour on the
+// tile from those values.
 // Every first-colour update reads the old field only and every second-colour
 // update reads first-colour values and old second-colour values, so odd
 // periodic sides (where same-coloured cells meet) keep the whole-array
@@ -160,16 +161,30 @@ __global__ __launch_bounds__(RB_TW * RB_TH) void k_mg_rbgs2(MgLevel L, int c0,
                                                             double* __restrict__ xout,
                                                             const PcgScalars* sc) {
   if (finished(sc)) return;
-  constexpr int SW = RB_TW + 2, SH = RB_TH + 2;
+  constexpr int SW = RB_TW + 2, SH = RB_TH + 2;   // first-colour region: tile + 1
+  constexpr int XW = RB_TW + 4, XH = RB_TH + 4;   // old field: tile + 2
+  __shared__ double xs[XH][XW];
   __shared__ double v1[SH][SW];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * RB_TW, j0 = blockIdx.y * RB_TH;
+  for (int q = tid; q < XW * XH; q += RB_TW * RB_TH) {
+    const int ly = q / XW, lx = q % XW;
+    xs[ly][lx] = xin[(long long)mg_pmod(j0 + ly - 2, L.ny) * L.nx + mg_pmod(i0 + lx - 2, L.nx)];
+  }
+  __syncthreads();
   for (int q = tid; q < SW * SH; q += RB_TW * RB_TH) {
     const int ly = q / SW, lx = q % SW;
     const int i = mg_pmod(i0 + lx - 1, L.nx), j = mg_pmod(j0 + ly - 1, L.ny);
     const long long g = (long long)j * L.nx + i;
-    v1[ly][lx] = (((i + j) & 1) == c0) ? L.inv_diag[g] * (b[g] + mg_neighbour_sum(L, xin, g))
-                                       : xin[g];
+    if (((i + j) & 1) == c0) {
+      const long long cw = (long long)j * L.nx + wrap(i, -1, L.nx);
+      const long long cs = (long long)wrap(j, -1, L.ny) * L.nx + i;
+      const double nb = L.cx * (L.Ku[g] * xs[ly + 1][lx + 2] + L.Ku[cw] * xs[ly + 1][lx]) +
+                        L.cy * (L.Kv[g] * xs[ly + 2][lx + 1] + L.Kv[cs] * xs[ly][lx + 1]);
+      v1[ly][lx] = L.inv_diag[g] * (b[g] + nb);
+    } else {
+      v1[ly][lx] = xs[ly + 1][lx + 1];
+    }
   }
   __syncthreads();
   const int tx = tid % RB_TW, ty = tid / RB_TW;

@@ -472,6 +472,9 @@ PcgArgs pcg_args(hoc_stepper* h, const HelmOp& op, const double* inv_diag, const
   a.err = h->bar_err;
   a.sc = h->sc;
   a.counters = h->dcount;
+  // beyond 256 chunks (2M cells) the phases' scalars are formed once per
+  // grid (k_pcg_totals) instead of in every workgroup
+  a.central = a.nchunks > 256 ? 1 : 0;
   return a;
 }
 
@@ -706,8 +709,10 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
       // as a failure
       const int n_it = std::min(h->sync_free_budget, max_iter + 1);
       for (int it = 1; it <= n_it; ++it) {
+        if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, it == 1 ? 1 : 2);
         k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it);
         if (it <= max_iter) {
+          if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, 0);
           k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 0);
           TRY(precondition(h, h->r, h->z, s));
           k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 0);
@@ -723,8 +728,10 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
     int batch = std::max(1, std::min(h->last_iters + 1, max_iter + 1));
     while (true) {
       for (int b = 0; b < batch && it <= max_iter + 1; ++b, ++it) {
+        if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, it == 1 ? 1 : 2);
         { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it); }
         if (it <= max_iter) {
+          if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, 0);
           { ProfScope _ps(h, "k_pcg_update", s); k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 0); }
           TRY(precondition(h, h->r, h->z, s));
           k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 0);
@@ -757,8 +764,10 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   int batch = std::max(1, std::min(h->last_iters + 1, max_iter + 1));
   while (true) {
     for (int b = 0; b < batch && it <= max_iter + 1; ++b, ++it) {
+      if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, it == 1 ? 1 : 2);
       { ProfScope _ps(h, "k_pcg_ap", s); k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it); }
       if (it <= max_iter) {
+        if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, 0);
         ProfScope _ps(h, "k_pcg_update", s);
         k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 1);
       }

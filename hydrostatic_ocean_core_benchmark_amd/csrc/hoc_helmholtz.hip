@@ -467,6 +467,33 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_init_z(PcgArgs a) {
   if (threadIdx.x == 0) a.part_b[part] = s.a;
 }
 
+// The scalars the next phase needs from the last phase's partial sums,
+// formed once for the grid by one workgroup (grids with many chunks, where
+// every workgroup forming them itself would repeat the in-order chunk sum
+// thousands of times); the same reductions as in the phases, so the
+// iterates do not change. which 0: tot[0] = sum part_a (p.Ap); 1: tot[1] =
+// sum part_b (the first rz); 2: tot[1], tot[2] = sums of part_b (r.r) and
+// part_c (r.z).
+__global__ __launch_bounds__(PCG_WG) void k_pcg_totals(PcgArgs a, int which) {
+  if (a.sc->done) return;
+  __shared__ double cs[2 * PCG_MAX_CHUNKS];
+  if (which == 0) {
+    const double t = total_from_parts(a, a.part_a, cs);
+    if (threadIdx.x == 0) a.sc->tot[0] = t;
+  } else if (which == 1) {
+    const double t = total_from_parts(a, a.part_b, cs);
+    if (threadIdx.x == 0) a.sc->tot[1] = t;
+  } else {
+    const double* parts[2] = {a.part_b, a.part_c};
+    double tot[2];
+    totals_from_parts<2>(a, parts, cs, tot);
+    if (threadIdx.x == 0) {
+      a.sc->tot[1] = tot[0];
+      a.sc->tot[2] = tot[1];
+    }
+  }
+}
+
 // Iteration `it` (1-based), phase A. For it > 1 it first finishes
 // iteration it-1: the convergence test on ||r||, then beta and the new rz.
 __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
@@ -487,12 +514,17 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_ap(PcgArgs a, int it) {
   const ApLoads L1 = ap_load(a, pr.base + (h1 ? t1 : 0), first, pnew, pold);
   double beta = 0.0;
   if (first) {
-    const double rz = total_from_parts(a, a.part_b, cs);
+    const double rz = a.central ? a.sc->tot[1] : total_from_parts(a, a.part_b, cs);
     if (lead) a.sc->rz2[1] = rz;
   } else {
     const double* parts[2] = {a.part_b, a.part_c};
     double tot[2];
-    totals_from_parts<2>(a, parts, cs, tot);
+    if (a.central) {
+      tot[0] = a.sc->tot[1];
+      tot[1] = a.sc->tot[2];
+    } else {
+      totals_from_parts<2>(a, parts, cs, tot);
+    }
     const double res = sqrt(tot[0]);
     if (res <= a.sc->tol) {
       if (lead) {
@@ -545,7 +577,8 @@ __global__ __launch_bounds__(PCG_WG) void k_pcg_update(PcgArgs a, int it, int ja
   const double x0 = a.x[e0], p0 = pnew[e0], r0 = a.r[e0], ap0 = a.Ap[e0];
   const double x1 = a.x[e1], p1 = pnew[e1], r1 = a.r[e1], ap1 = a.Ap[e1];
   const double d0 = jacobi ? a.inv_diag[e0] : 0.0, d1 = jacobi ? a.inv_diag[e1] : 0.0;
-  const double al = a.sc->rz2[it & 1] / total_from_parts(a, a.part_a, cs);
+  const double al =
+      a.sc->rz2[it & 1] / (a.central ? a.sc->tot[0] : total_from_parts(a, a.part_a, cs));
   auto fin = [&](bool h, int t, long long e, double xv, double pv, double rv, double av,
                  double dv) {
     if (!h) return;

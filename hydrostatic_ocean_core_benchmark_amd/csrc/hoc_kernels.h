@@ -35,6 +35,7 @@ struct PcgScalars {
   double rz, alpha, beta, tol, norm_b, res;
   double rz2[2];  // rz by iteration parity (per-phase path)
   int it, iters, done, converged;
+  double tot[3];  // sums of part_a, part_b, part_c when computed once (k_pcg_totals)
 };
 
 // hoc_step_kernels.hip
@@ -295,6 +296,7 @@ struct PcgArgs {
   int* err;
   PcgScalars* sc;
   long long* counters;
+  int central;          // 1: k_pcg_totals forms the phase scalars once per grid
 };
 
 constexpr int PCG_WG = 256;         // threads per workgroup of a PCG phase
@@ -319,6 +321,7 @@ using MgLevel = MgLevelT<double>;
 using MgLevelF = MgLevelT<float>;
 
 __global__ void k_pcg_persistent(PcgArgs a);
+__global__ void k_pcg_totals(PcgArgs a, int which);
 __global__ void k_pcg_rz(PcgArgs a, int into_b);
 __global__ void k_mg_coarsen(MgLevel f, MgLevel c, double* cKu, double* cKv, uint8_t* cm);
 __global__ void k_mg_diag(MgLevel L, double* inv_diag);

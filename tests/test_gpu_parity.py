@@ -335,3 +335,26 @@ def test_helmholtz_persistent_equals_multikernel_and_oracle(hoc, shape):
     for path, (used, x, iters, conv) in results.items():
         assert conv and iters == rep.iterations, (path, used, iters, rep.iterations)
         assert np.array_equal(x, want), (path, used, rel_l2(x, want))
+
+
+@pytest.mark.parametrize("kind", ["pcg_jacobi", "pcg_mg"])
+def test_helmholtz_large_grid_matches_oracle(hoc, kind):
+    """A 1450 x 1450 grid (256 full reduction chunks and a partial one,
+    just past the size where the PCG phase scalars are formed once per grid
+    by k_pcg_totals instead of in every workgroup): the solution and the
+    iteration count stay the oracle's bit for bit."""
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    from oracle.helmholtz import HelmholtzVar
+    shape = (1450, 1450)
+    grid, dom, mask, Ku, Kv, rhs = _random_helmholtz(shape, "closed", seed=13)
+    ny, nx = shape
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=4000.0))
+    params = SchemeParams(name="theta", solver=kind, rtol=1e-10, max_iter=3000)
+    st = hoc.HipStepper(dom, phys, params, 600.0)
+    ref = HelmholtzVar(nx, ny, grid.dx, grid.dy, st.cfg.g * 0.25 * 600.0 ** 2, Ku, Kv, mask,
+                       kind=kind, rtol=1e-10, max_iter=3000)
+    want, rep = ref.solve(rhs)
+    x, iters, conv = st.helmholtz_solve(Ku, Kv, rhs)
+    print(f"large {kind}: {iters} iterations (oracle {rep.iterations})", flush=True)
+    assert conv == rep.converged and iters == rep.iterations
+    assert np.array_equal(_np(x), want)

@@ -134,3 +134,20 @@ def test_bundle_run_matches_reference(tmp_path, gpu_available):
     assert list(on_disk) == keys and on_disk == m
     assert m["n_repeat"] == 2 and not m["diverged"] and m["cells"] == g.nx * g.ny * g.nz
     assert m["solver_iterations"] == int(np.sum(f["iters_per_step"][:10]))
+
+
+def test_launcher_runs_the_bundle_entry_point(tmp_path):
+    """cfd_exp3d5_hip (the backend binary for the reference's gate scripts)
+    reaches the bundle runner: a missing namelist is the runner's FATAL, exit 1,
+    before any device use."""
+    import os
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parents[1] / "hydrostatic_ocean_core_benchmark_amd" / "cfd_exp3d5_hip"
+    assert os.access(exe, os.X_OK)
+    proc = subprocess.run([str(exe), str(tmp_path / "missing.nml")], capture_output=True,
+                          text=True, timeout=120)
+    assert proc.returncode == 1
+    assert "FATAL" in proc.stderr
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 1 and "usage" in proc.stderr

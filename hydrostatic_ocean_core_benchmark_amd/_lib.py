@@ -148,6 +148,7 @@ PROTOTYPES = {
     "hoc_gaspar_lengths_adjoint": ([VP, VP, VP, VP, VP, VP, f64, f64, VP, VP, VP, VP, i64, i64,
                                     VP], i32),
     "hoc_pairwise_dot": ([VP, VP, i64, VP, VP, VP], i32),
+    "hoc_helm_divgrad": ([i32, i32, f64, f64, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_set_solver_path": ([VP, i32], i32),
     "hoc_stepper_solver_path": ([VP], i32),
     "hoc_stepper_profile_kernel": ([VP, C.c_char_p], i32),

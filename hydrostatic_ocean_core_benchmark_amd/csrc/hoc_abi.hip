@@ -2169,6 +2169,25 @@ int hoc_eos_buoyancy_adjoint(int32_t eos, double g, double rho0, double alpha_T,
   return HOC_OK;
 }
 
+int hoc_helm_divgrad(int32_t nx, int32_t ny, double dx, double dy, const double* Ku,
+                     const double* Kv, const uint8_t* mask, const double* x, double* out,
+                     void* stream) {
+  if (!Ku || !Kv || !mask || !x || !out) return fail(HOC_ERR_INVALID, "null argument");
+  if (nx < 1 || ny < 1) return fail(HOC_ERR_INVALID, "nx and ny must be positive");
+  if (!(dx > 0.0) || !(dy > 0.0)) return fail(HOC_ERR_INVALID, "dx and dy must be positive");
+  Dims d{};
+  d.nx = nx;
+  d.ny = ny;
+  d.nz = 1;
+  d.n2 = (long long)nx * ny;
+  d.n3 = d.n2;
+  d.ni = d.n2;
+  const HelmOp op{Ku, Kv, mask, 0.0, 0.0, 0.0, dx * dx, dy * dy};
+  k_helm_divgrad<<<blocks(d.n2), TPB, 0, (hipStream_t)stream>>>(d, op, x, out);
+  LAUNCH_CHECK("k_helm_divgrad");
+  return HOC_OK;
+}
+
 int hoc_pairwise_dot(const double* a, const double* b, int64_t n, double* out, double* scratch,
                      void* stream) {
   if (!a || !b || !out || !scratch) return fail(HOC_ERR_INVALID, "null argument");

@@ -110,9 +110,11 @@ __device__ __forceinline__ long long chunk_len(long long chunk, long long n) {
   return (n - b) < PW_CHUNK ? (n - b) : PW_CHUNK;
 }
 
-// Masked Helmholtz operator A x at flat index c (libs/core/solvers_var.py:47-53).
-__device__ __forceinline__ double apply_A(const Dims& d, const HelmOp& op, const double* x,
-                                          long long c) {
+// D[K G x] at flat index c over the masked field: the five-point
+// variable-coefficient Laplacian with face coefficients Ku (east) and Kv
+// (north). Returns x[c] * mask[c] through xc_out.
+__device__ __forceinline__ double divgrad_at(const Dims& d, const HelmOp& op, const double* x,
+                                             long long c, double* xc_out) {
   const int i = (int)(c % d.nx), j = (int)(c / d.nx);
   const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
   const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
@@ -126,8 +128,28 @@ __device__ __forceinline__ double apply_A(const Dims& d, const HelmOp& op, const
   const double fxw = op.Ku[cw] * (xc - xw);
   const double fy = op.Kv[c] * (xn - xc);
   const double fys = op.Kv[cs] * (xc - xs);
-  const double lap = (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+  *xc_out = xc;
+  return (fx - fxw) / op.dx2 + (fy - fys) / op.dy2;
+}
+
+// Masked Helmholtz operator A x at flat index c (libs/core/solvers_var.py:47-53).
+__device__ __forceinline__ double apply_A(const Dims& d, const HelmOp& op, const double* x,
+                                          long long c) {
+  double xc;
+  const double lap = divgrad_at(d, op, x, c, &xc);
+  const double mc = mk(op.mask[c]);
   return (xc - op.coef * lap) * mc + xc * (1.0 - mc);
+}
+
+// out = mask * D[K G x] (the operator whose truncation order the reference
+// measures, tools/verify_v05.py:113-176).
+__global__ void k_helm_divgrad(Dims d, HelmOp op, const double* __restrict__ x,
+                               double* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  double xc;
+  const double lap = divgrad_at(d, op, x, c, &xc);
+  out[c] = lap * mk(op.mask[c]);
 }
 
 // diag = 1 + cx (Ku + Ku_w) + cy (Kv + Kv_s), 1 on dry cells; inv = 1/diag.

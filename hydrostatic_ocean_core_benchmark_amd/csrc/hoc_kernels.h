@@ -267,6 +267,7 @@ __global__ void k_m3_eos(M3 M, const double* T, const double* S, double* b);
 
 // hoc_helmholtz.hip
 __global__ void k_helm_diag(Dims d, HelmOp op, double* inv_diag);
+__global__ void k_helm_divgrad(Dims d, HelmOp op, const double* x, double* out);
 __global__ void k_pw_dot(const double* a, const double* b, const PwPlan* plans,
                          long long nfull, double* part);
 __global__ void k_seq_sum(const double* part, long long nchunks, double* out);

@@ -276,12 +276,49 @@ class Domain:
                 and self.bc_y == "periodic" and bool(np.all(self.mask == 1.0))
                 and float(np.ptp(self.H)) == 0.0)
 
+    @property
+    def roughness(self) -> Roughness:
+        """libs/core/domain.py:157-170: r_std and H_max/H_min over wet cells;
+        rx0 = max |H_a - H_b| / (H_a + H_b) over wet faces only, so the jump
+        across a closed boundary's wrap-around is not counted."""
+        h = self.H[self.mask > 0]
+        rx = 0.0
+        for axis, face_mask in ((-1, self.mask_u), (-2, self.mask_v)):
+            sel = face_mask > 0
+            if np.any(sel):
+                a, b = self.H, _shift(self.H, -1, axis)
+                rx = max(rx, float(np.max(np.abs(a - b)[sel] / (a + b)[sel])))
+        return Roughness(r_std=float(np.std(h) / np.mean(h)), rx0=rx,
+                         h_ratio=float(np.max(h) / np.min(h)))
+
+    def depth_error(self) -> float:
+        """Largest |sum_k dz3_ref - H| over wet columns (libs/core/domain.py:
+        172-178): zero for sigma, the partial-cell truncation for z-level."""
+        wet = self.mask > 0
+        if not np.any(wet):
+            return 0.0
+        return float(np.max(np.abs(self.dz3_ref.sum(axis=0) - self.H)[wet]))
+
     def summary(self) -> dict:
+        """libs/core/domain.py:187-196"""
         wet = self.mask > 0
         return {"vcoord": self.vcoord, "bc_x": self.bc_x, "bc_y": self.bc_y,
                 "face_rule": self.face_rule, "H_min": float(np.min(self.H[wet])),
                 "H_max": float(np.max(self.H[wet])),
-                "wet_fraction": float(np.mean(self.mask3))}
+                "wet_fraction": float(np.mean(self.mask3)),
+                "depth_error_max": self.depth_error(), **self.roughness.as_dict()}
+
+
+@dataclass(frozen=True)
+class Roughness:
+    """Roughness measures of a bathymetry (libs/core/bathymetry.py:20-29)."""
+
+    r_std: float       # std(H) / mean(H)
+    rx0: float         # max |dH| / (H_a + H_b) over wet faces
+    h_ratio: float     # H_max / H_min
+
+    def as_dict(self) -> dict[str, float]:
+        return {"r_std": self.r_std, "rx0": self.rx0, "h_ratio": self.h_ratio}
 
 
 def build_domain(grid: CGrid, H: np.ndarray, mask: np.ndarray, config=None, **over) -> Domain:

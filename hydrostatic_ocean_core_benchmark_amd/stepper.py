@@ -7,10 +7,11 @@ through libhoc.so. Arrays come back as torch tensors on the device (call
 diffusivity between steps as `nu` and `kappa` ([nz+1, ny, nx] device
 tensors), as the reference does (model3d_v05.py:97-98,179).
 
-Supported: z-level coordinate, periodic and closed boundaries, the theta
-(PCG-Jacobi) and fb schemes, closure none|tke (integral|recursive), every
+Supported: zlevel|zstar|sigma coordinates, periodic, closed and open
+(Flather) boundaries, the theta (pcg_jacobi|pcg_rbgs|rbgs|multigrid|pcg_mg),
+fb and split_explicit schemes, closure none|tke (integral|recursive), every
 advection scheme, linear|seos|teos10, tracers TS|buoyancy. Other settings
-raise NotImplementedError, never run a fallback.
+raise an error, never run a fallback.
 """
 
 from __future__ import annotations

@@ -394,6 +394,17 @@ int hoc_eos_buoyancy(int32_t eos, double g, double rho0, double alpha_T, double 
                      double T0, double S0, const double* T, const double* S, const double* z,
                      double* b, int64_t n, void* stream);
 
+/* out = mask * D[K G x] on a doubly periodic ny x nx grid: the masked
+ * five-point variable-coefficient operator of the barotropic Helmholtz
+ * solve, with face coefficients Ku (east faces) and Kv (north faces). It is
+ * operators_masked.div_m(Ku * gradx_u_m(x), Kv * grady_v_m(x)) of
+ * libs/core/operators_masked.py:18-36 with unit face masks, the operator whose
+ * truncation order tools/verify_v05.py:113-176 measures; x is masked by
+ * `mask` before differencing, as in the Helmholtz solve. */
+int hoc_helm_divgrad(int32_t nx, int32_t ny, double dx, double dy, const double* Ku,
+                     const double* Kv, const uint8_t* mask, const double* x, double* out,
+                     void* stream);
+
 /* numpy.sum(a * b) for float64, bit for bit (sequential over 8192-element
  * chunks, pairwise inside each). out is one device double; scratch holds
  * ceil(n / 8192) doubles. */

@@ -1,0 +1,153 @@
+"""Host-side parts of the v0.5 verification cases: domain roughness and
+summary against the values the reference recorded, the cases' diagnostics
+on their own initial states (each is zero or one by construction), the
+config sections, and the lock-exchange hypsometry against brute force."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from hydrostatic_ocean_core_benchmark_amd import cases3d_v05 as C5
+from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain, build_bathymetry
+from hydrostatic_ocean_core_benchmark_amd.types import physics_v05_from_config
+from hydrostatic_ocean_core_benchmark_amd.verify_v05 import suite_config
+
+
+@pytest.fixture(scope="module")
+def cfg():
+    return suite_config()
+
+
+@pytest.mark.parametrize("topo,rx0", [("flat", 0.0), ("seamount", 0.25047610613656307),
+                                      ("rough", 0.06983750304386985)])
+@pytest.mark.parametrize("vcoord", ["zlevel", "zstar", "sigma"])
+def test_roughness_matches_recorded_rx0(cfg, topo, rx0, vcoord):
+    """rx0 as the reference's V5-3 run recorded it for the 48 x 48 x 20
+    domains (v05_verification_20260923.json)."""
+    d = C5.build_domain3d(cfg, 48, 20, vcoord=vcoord, bathymetry=topo)
+    assert d.roughness.rx0 == rx0
+    s = d.summary()
+    for key in ("vcoord", "bc_x", "bc_y", "face_rule", "H_min", "H_max", "wet_fraction",
+                "depth_error_max", "r_std", "rx0", "h_ratio"):
+        assert key in s
+    if vcoord == "sigma":
+        assert s["depth_error_max"] < 1e-12
+
+
+def test_roughness_brute_force():
+    """rx0 = max |H_a - H_b| / (H_a + H_b) over wet faces, by loops; a closed
+    boundary drops the wrap-around face."""
+    grid = CGrid(nx=7, ny=5, Lx=7e3, Ly=5e3, nz=4, depth=100.0)
+    rng = np.random.default_rng(3)
+    H = 50.0 + 50.0 * rng.random(grid.shape)
+    mask = np.ones(grid.shape)
+    mask[2, 3] = 0.0
+    for bc in ("periodic", "closed"):
+        d = Domain(grid=grid, H=H, mask=mask, bc_x=bc, bc_y=bc)
+        best = 0.0
+        for j in range(grid.ny):
+            for i in range(grid.nx):
+                for jj, ii, wrap in ((j, (i + 1) % grid.nx, i == grid.nx - 1),
+                                     ((j + 1) % grid.ny, i, j == grid.ny - 1)):
+                    if mask[j, i] == 0 or mask[jj, ii] == 0 or (wrap and bc == "closed"):
+                        continue
+                    best = max(best, abs(H[j, i] - H[jj, ii]) / (H[j, i] + H[jj, ii]))
+        assert d.roughness.rx0 == pytest.approx(best, rel=1e-15)
+        h = H[mask > 0]
+        assert d.roughness.h_ratio == pytest.approx(h.max() / h.min(), rel=1e-15)
+
+
+@pytest.mark.parametrize("name", ["seamount_rest", "zstar_constancy", "flather_radiate",
+                                  "basin_seiche", "baroclinic_igw", "lock_exchange"])
+def test_cases_build_from_config(cfg, name):
+    c = cfg.with_overrides({"physics3d.N2": 1e-4})
+    d = C5.build_domain3d(c, 16, 6, bathymetry="seamount", bc="closed")
+    case = C5.build_case3d_v05(name, d, physics_v05_from_config(c), c)
+    assert case.t_final > 0.0
+
+
+def test_unknown_case_raises(cfg):
+    d = C5.build_domain3d(cfg, 8, 4)
+    with pytest.raises(ValueError, match="unknown v0.5 case"):
+        C5.build_case3d_v05("nope", d, physics_v05_from_config(cfg), cfg)
+
+
+def test_diagnostics_vanish_on_initial_states(cfg):
+    phys = physics_v05_from_config(cfg)
+    d = C5.build_domain3d(cfg, 16, 6, vcoord="zstar", bathymetry="seamount")
+    case = C5.build_case3d_v05("seamount_rest", d, phys, cfg)
+    s0 = case.initial()
+    assert np.all(s0.b <= 0.0) and s0.b.shape == d.grid.shape3d
+    r = case.evaluate(s0)
+    assert r.metric == 0.0 and r.extra["rx0"] == d.roughness.rx0
+
+    case = C5.build_case3d_v05("zstar_constancy", d, phys, cfg)
+    r = case.evaluate(case.initial())
+    assert r.metric == 0.0 and r.extra["tracer_integral_drift"] == 0.0
+    assert r.extra["tracer_integral_drift_live"] == 0.0 and r.extra["T_mean"] == phys.T0
+
+    d_open = C5.build_domain3d(cfg, 16, 6, bathymetry="flat", bc="open")
+    case = C5.build_case3d_v05("flather_radiate", d_open, phys, cfg)
+    assert case.evaluate(case.initial()).metric == 1.0
+
+    d_closed = C5.build_domain3d(cfg, 16, 6, bathymetry="seamount", bc="closed")
+    case = C5.build_case3d_v05("basin_seiche", d_closed, phys, cfg)
+    s0 = case.initial()
+    r = case.evaluate(s0)
+    assert r.metric == 0.0 and r.extra["eta_mean_drift"] == 0.0
+    assert abs(float(np.sum(s0.eta))) < 1e-12
+
+
+def test_zstar_live_thickness_holds_surface_volume(cfg):
+    """z-level keeps dz fixed and puts eta into the top cell; sigma stretches
+    every layer by 1 + eta / H. Either way a column holds its resting
+    thickness plus eta (sigma's resting thickness is exactly H)."""
+    phys = physics_v05_from_config(cfg)
+    for vcoord in ("zlevel", "sigma"):
+        d = C5.build_domain3d(cfg, 12, 5, vcoord=vcoord, bathymetry="slope", slope=0.5)
+        case = C5.ZStarConstancy(d, phys, 1.0, 0.25)
+        eta = 0.3 * np.ones(d.grid.shape)
+        live = case._live_thickness(eta)
+        col = live.sum(axis=0)
+        assert np.allclose(col, d.dz3_ref.sum(axis=0) + eta, rtol=0, atol=1e-11)
+
+
+def test_hypsometry_against_brute_force():
+    """Volume below height z from the sorted prefix sums equals the direct
+    sum over cells of clip(z - z_bottom, 0, dz) * area, at every height."""
+    grid = CGrid(nx=6, ny=4, Lx=6e3, Ly=4e3, nz=5, depth=100.0)
+    H, mask = build_bathymetry(grid, "seamount", 100.0, h_rel=0.6)
+    d = Domain(grid=grid, H=H, mask=mask, vcoord="zlevel")
+    case = C5.LockExchange.__new__(C5.LockExchange)
+    case.domain, case._hyps = d, None
+    vol_below, heights = case._hypsometry()
+    dz = d.dz3_ref
+    edges = np.concatenate([np.zeros((1,) + dz.shape[1:]), np.cumsum(dz, axis=0)])
+    deepest = edges[-1].max()
+    wet = d.mask3 > 0
+    zb, zt = (deepest - edges[1:])[wet], (deepest - edges[:-1])[wet]
+    area = grid.dx * grid.dy
+    for z, v in zip(heights, vol_below):
+        direct = area * sum(min(max(z - b, 0.0), t - b) for b, t in zip(zb, zt))
+        assert v == pytest.approx(direct, rel=1e-12, abs=1e-6)
+    assert vol_below[-1] == pytest.approx(area * float(np.sum(dz[wet])), rel=1e-13)
+
+
+def test_rpe_of_a_stable_column_in_a_box():
+    """A flat box with buoyancy increasing upward is already sorted. Each
+    cell is assigned the height its top reaches once stacked: in layer L
+    (from the bottom, 10 m thick, 6 cells of one buoyancy) the k-th cell
+    reaches 10 L + 10 k / 6, so RPE = vol sum_L b_L sum_k (10 L + 10 k / 6)."""
+    grid = CGrid(nx=3, ny=2, Lx=3e3, Ly=2e3, nz=4, depth=40.0)
+    d = Domain(grid=grid, H=np.full(grid.shape, 40.0), mask=np.ones(grid.shape))
+    case = C5.LockExchange.__new__(C5.LockExchange)
+    case.domain, case._hyps = d, None
+    b_bottom_up = np.array([0.01, 0.02, 0.03, 0.04])
+    b = np.broadcast_to(b_bottom_up[::-1][:, None, None], grid.shape3d).copy()
+    vol = grid.dx * grid.dy * 10.0
+    want = vol * sum(b_bottom_up[L] * sum(10.0 * L + 10.0 * k / 6.0 for k in range(1, 7))
+                     for L in range(4))
+    assert case._rpe(b) == pytest.approx(want, rel=1e-12)
+    # an unstable arrangement has the same RPE: the sort removes the order
+    assert case._rpe(b[::-1].copy()) == pytest.approx(want, rel=1e-12)

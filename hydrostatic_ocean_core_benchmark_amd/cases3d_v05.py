@@ -195,10 +195,14 @@ class LockExchange:
             self._hyps = (area * (ramp(z_bot) - ramp(z_top)), heights)
         return self._hyps
 
-    def _rpe(self, b: np.ndarray) -> float:
+    def _rpe(self, b: np.ndarray, thickness: np.ndarray | None = None) -> float:
+        """Sort the buoyancy densest-first, stack the cells' volumes (from
+        `thickness`, default the resting thickness) into the basin's
+        hypsometric curve and integrate b z dV."""
         d = self.domain
         wet = d.mask3 > 0
-        vol = (d.grid.dx * d.grid.dy * d.dz3_ref)[wet]
+        dz = d.dz3_ref if thickness is None else thickness
+        vol = (d.grid.dx * d.grid.dy * dz)[wet]
         bw = b[wet]
         order = np.argsort(bw)
         v = vol[order]

@@ -43,28 +43,44 @@ __device__ __forceinline__ R tracer_face_z(int adv, int kk, int nz, R far_up, R 
                          hu, hd, adv == ADV_UP3_TVD);
 }
 
-// Staged tile of one level in the flux type R.
+// Staged tile of one level in the flux type R: what the face pass and the
+// cell pass both read (double-buffered, so level k + 1 lands while level k
+// is still read) ...
 template <class R>
 struct TileLds {
   R c[2][SY][SX];           // the tracers (T, S) or b, with halo
   uint8_t m[SY][SX];        // mask3 with halo
-  R u[TY][TX + 1];          // u at i-1 .. i
-  R v[TY + 1][TX];          // v at j-1 .. j
-  R du[TY][TX + 1];         // dz3u
-  R dv[TY + 1][TX];         // dz3v
   uint8_t mu[TY][TX + 1];   // mask3u
   uint8_t mv[TY + 1][TX];   // mask3v
 };
 
-// Face quantities of one level, each computed once: advective flux
-// dz_f u_f c_f and K_h gradient mask_f (c_1 - c_0) / dx on the x-faces
-// (index q: between tile columns q-1 and q) and y-faces (between rows q-1
-// and q), for both tracers.
+// ... and what only the face pass reads (one buffer: the barrier after the
+// face pass ends its reads before level k + 1 is stored).
+template <class R>
+struct VelLds {
+  R u[TY][TX + 1];          // u at i-1 .. i
+  R v[TY + 1][TX];          // v at j-1 .. j
+  R du[TY][TX + 1];         // dz3u
+  R dv[TY + 1][TX];         // dz3v
+};
+
+// Advective face fluxes dz_f u_f c_f of one level, each computed once, on
+// the x-faces (index q: between tile columns q-1 and q) and y-faces
+// (between rows q-1 and q), for both tracers. The K_h face gradients are
+// formed by the cell that uses them, from the staged tile (each interior
+// gradient twice, from the same operands): with the single velocity buffer
+// this holds a workgroup to 33 KB of LDS, so four fit on a CU.
 template <class R>
 struct FaceLds {
-  R fx[2][TY][TX + 1], gx[2][TY][TX + 1];
-  R fy[2][TY + 1][TX], gy[2][TY + 1][TX];
+  R fx[2][TY][TX + 1];
+  R fy[2][TY + 1][TX];
 };
+
+// K_h gradient mask_f (c_1 - c_0) / d across one face of the staged tile.
+template <class R>
+__device__ __forceinline__ R face_grad(uint8_t mf, R c0, R c1, double d) {
+  return R(mk(mf)) * (c1 - c0) / R(d);
+}
 
 // Level-k staging of one tile held in registers between its global loads
 // and its LDS stores, so the loads of level k + 1 are in flight while
@@ -143,7 +159,8 @@ __device__ __forceinline__ void tile_load(TileRegs<RT>& R, const TileOffs& F, lo
 }
 
 template <class RT>
-__device__ __forceinline__ void tile_store(TileLds<RT>& L, const TileRegs<RT>& R, int tid) {
+__device__ __forceinline__ void tile_store(TileLds<RT>& L, VelLds<RT>& V, const TileRegs<RT>& R,
+                                           int tid) {
 #pragma unroll
   for (int q = 0; q < ST_C; ++q) {
     const int t = tid + q * TILE_THREADS;
@@ -159,8 +176,8 @@ __device__ __forceinline__ void tile_store(TileLds<RT>& L, const TileRegs<RT>& R
     const int t = tid + q * TILE_THREADS;
     if (t < TY * (TX + 1)) {
       const int lx = t % (TX + 1), ly = t / (TX + 1);
-      L.u[ly][lx] = R.u[q];
-      L.du[ly][lx] = R.du[q];
+      V.u[ly][lx] = R.u[q];
+      V.du[ly][lx] = R.du[q];
       L.mu[ly][lx] = R.mu[q];
     }
   }
@@ -169,8 +186,8 @@ __device__ __forceinline__ void tile_store(TileLds<RT>& L, const TileRegs<RT>& R
     const int t = tid + q * TILE_THREADS;
     if (t < (TY + 1) * TX) {
       const int lx = t % TX, ly = t / TX;
-      L.v[ly][lx] = R.v[q];
-      L.dv[ly][lx] = R.dv[q];
+      V.v[ly][lx] = R.v[q];
+      V.dv[ly][lx] = R.dv[q];
       L.mv[ly][lx] = R.mv[q];
     }
   }
@@ -195,6 +212,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
     const double* __restrict__ den, const double* __restrict__ cst, double* __restrict__ T_out,
     double* __restrict__ S_out, double* __restrict__ b_out) {
   __shared__ TileLds<R> LB[2];
+  __shared__ VelLds<R> VL;
   __shared__ FaceLds<R> F;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long n2 = d.n2;
@@ -240,7 +258,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
   TileRegs<R> RG;
   const TileOffs TO = tile_offsets(tid, i0, j0, nx, ny);
   tile_load(RG, TO, 0, ts, A, S, u, v, geo);
-  tile_store(LB[0], RG, tid);
+  tile_store(LB[0], VL, RG, tid);
   __syncthreads();
   const bool has_adv = adv != ADV_NONE;
   for (int k = 0; k < nz; ++k) {
@@ -252,38 +270,34 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
     {
       auto xface = [&](int fy_, int q) {
         const int cy = fy_ + HALO, c0x = q - 1 + HALO;
-        const R vel = L.u[fy_][q];
-        const R ddu = L.du[fy_][q];
-        const R mmu = R(mk(L.mu[fy_][q]));
+        const R vel = VL.u[fy_][q];
+        const R ddu = VL.du[fy_][q];
         const bool wm = L.m[cy][c0x - 1] != 0, w2 = L.m[cy][c0x + 2] != 0;
         for (int r = 0; r < ntr; ++r) {
           const R(*C)[SX] = L.c[r];
-          if (has_adv)
-            F.fx[r][fy_][q] = ddu * vel * tracer_face<R>(adv, C[cy][c0x - 1], C[cy][c0x],
-                                                         C[cy][c0x + 1], C[cy][c0x + 2], vel, wm,
-                                                         w2);
-          F.gx[r][fy_][q] = mmu * (C[cy][c0x + 1] - C[cy][c0x]) / R(p.dx);
+          F.fx[r][fy_][q] = ddu * vel * tracer_face<R>(adv, C[cy][c0x - 1], C[cy][c0x],
+                                                       C[cy][c0x + 1], C[cy][c0x + 2], vel, wm,
+                                                       w2);
         }
       };
       auto yface = [&](int q, int fx_) {
         const int cx = fx_ + HALO, c0y = q - 1 + HALO;
-        const R vel = L.v[q][fx_];
-        const R ddv = L.dv[q][fx_];
-        const R mmv = R(mk(L.mv[q][fx_]));
+        const R vel = VL.v[q][fx_];
+        const R ddv = VL.dv[q][fx_];
         const bool wm = L.m[c0y - 1][cx] != 0, w2 = L.m[c0y + 2][cx] != 0;
         for (int r = 0; r < ntr; ++r) {
           const R(*C)[SX] = L.c[r];
-          if (has_adv)
-            F.fy[r][q][fx_] = ddv * vel * tracer_face<R>(adv, C[c0y - 1][cx], C[c0y][cx],
-                                                         C[c0y + 1][cx], C[c0y + 2][cx], vel, wm,
-                                                         w2);
-          F.gy[r][q][fx_] = mmv * (C[c0y + 1][cx] - C[c0y][cx]) / R(p.dy);
+          F.fy[r][q][fx_] = ddv * vel * tracer_face<R>(adv, C[c0y - 1][cx], C[c0y][cx],
+                                                       C[c0y + 1][cx], C[c0y + 2][cx], vel, wm,
+                                                       w2);
         }
       };
-      xface(ty, tx + 1);
-      yface(ty + 1, tx);
-      if (tid < TY) xface(tid, 0);
-      else if (tid < TY + TX) yface(0, tid - TY);
+      if (has_adv) {
+        xface(ty, tx + 1);
+        yface(ty + 1, tx);
+        if (tid < TY) xface(tid, 0);
+        else if (tid < TY + TX) yface(0, tid - TY);
+      }
     }
     __syncthreads();
     if (active) {
@@ -313,9 +327,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
           tend = tend - (double)((hdiv + vdiv) * (m_k ? R(1) : R(0)));
         }
         if (p.has_K_h) {
+          const R(*C)[SX] = L.c[r];
+          const R ge = face_grad<R>(L.mu[ty][tx + 1], C[cy][cx], C[cy][cx + 1], p.dx);
+          const R gw = face_grad<R>(L.mu[ty][tx], C[cy][cx - 1], C[cy][cx], p.dx);
+          const R gn = face_grad<R>(L.mv[ty + 1][tx], C[cy][cx], C[cy + 1][cx], p.dy);
+          const R gs = face_grad<R>(L.mv[ty][tx], C[cy - 1][cx], C[cy][cx], p.dy);
           tend = tend + p.K_h * (double)((m_k ? R(1) : R(0)) *
-                                         ((F.gx[r][ty][tx + 1] - F.gx[r][ty][tx]) / R(p.dx) +
-                                          (F.gy[r][ty + 1][tx] - F.gy[r][ty][tx]) / R(p.dy)));
+                                         ((ge - gw) / R(p.dx) + (gn - gs) / R(p.dy)));
         }
         // explicit vertical diffusion with the surface flux (vertical_var.py:80-106)
         double ft;
@@ -358,7 +376,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
       kh_k = kh_k1;
       dzi_k = dzi_k1;
     }
-    if (k + 1 < nz) tile_store(LB[(k + 1) & 1], RG, tid);
+    if (k + 1 < nz) tile_store(LB[(k + 1) & 1], VL, RG, tid);
     __syncthreads();
   }
   if (!active) return;

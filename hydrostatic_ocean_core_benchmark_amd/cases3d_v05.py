@@ -169,6 +169,7 @@ class LockExchange:
         self.t_final = float(n_hours) * 3600.0
         self._hyps = None
         self._rpe0 = None
+        self.device = "cuda:0"      # where the initial buoyancy is evaluated
 
     def _hypsometry(self) -> tuple[np.ndarray, np.ndarray]:
         """(volume below height z, z) over the distinct cell-edge heights,
@@ -219,7 +220,7 @@ class LockExchange:
         warm = (x > 0.5 * d.grid.Lx).astype(np.float64)
         T = (p.T0 + self.delta_T * warm)[None, :, :] * np.ones(d.grid.shape3d)
         S = np.full(d.grid.shape3d, p.S0)
-        dev = dict(dtype=torch.float64, device="cuda")
+        dev = dict(dtype=torch.float64, device=self.device)
         b = eos_buoyancy(p.eos, torch.as_tensor(T, **dev), torch.as_tensor(S, **dev),
                          torch.as_tensor(d.depth_at_centres(), **dev), g=p.g, rho0=p.rho0,
                          alpha_T=p.alpha_T, beta_S=p.beta_S, T0=p.T0, S0=p.S0)
@@ -406,6 +407,7 @@ def simulate3d_v05(config, case_name: str, domain: Domain, physics: PhysicsV05, 
     dt = cfl * barotropic_dt_max(domain, physics.g)
     stepper = HipStepper(domain, physics, params, dt, device=device, precision=precision)
     case = build_case3d_v05(case_name, domain, physics, config)
+    case.device = device
     n_steps = (int(n_steps_override) if n_steps_override is not None
                else max(1, int(round(case.t_final / dt))))
     s0 = case.initial()

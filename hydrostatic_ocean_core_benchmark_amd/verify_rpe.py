@@ -47,6 +47,7 @@ def run(advection: str, scheme: str, nx: int, nz: int, steps: int,
     pinned = float(cfg.get("time.dt", 0.0))
     dt = pinned if pinned > 0.0 else cfl * barotropic_dt_max(domain, physics.g)
     case = build_case3d_v05("lock_exchange", domain, physics, cfg)
+    case.device = device
     s0 = case.initial()
     stepper = HipStepper(domain, physics, params, dt, device=device)
     s = to_numpy(stepper.integrate(s0, steps))

@@ -265,6 +265,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
     const long long o = (long long)k * n2;
     const TileLds<R>& L = LB[k & 1];
     if (k + 1 < nz) tile_load(RG, TO, o + n2, ts, A, S, u, v, geo);
+    // the own column's operands of level k, issued before the face pass so
+    // their latency hides behind it
+    double w_k1 = 0.0, kh_k1 = 0.0, dzi_k1 = 0.0, dz = 0.0, sub_k = 0.0, den_k = 1.0;
+    if (active) {
+      w_k1 = w[o + n2 + c];
+      kh_k1 = Kh[o + n2 + c];
+      dzi_k1 = geo.dzi[o + n2 + c];
+      dz = geo.dz3[o + c];
+      sub_k = sub[o + c];
+      den_k = den[o + c];
+    }
     // ---- faces: the east and north face of every cell, plus the tile's
     // west column (8) and south row (32) of faces by threads 0..39
     {
@@ -302,10 +313,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
     __syncthreads();
     if (active) {
       // ---- own-column values for level k
-      const double w_k1 = w[o + n2 + c];
-      const double kh_k1 = (k + 1 <= nz) ? Kh[o + n2 + c] : 0.0;
-      const double dzi_k1 = geo.dzi[o + n2 + c];
-      const double dz = geo.dz3[o + c];
       const double inv = safe_inv(dz);
       const bool m_k = mw[2];
       const bool bot_km1 = k >= 1 && mw[1] && !mw[2];          // cell k-1 is a bottom cell
@@ -354,7 +361,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
         double rhs = cc + p.dt * tend + p.c_dzd * dzd;
         if (k == 0) rhs = rhs + p.thv_dt * flux[r] * safe_inv(dz);
         const long long t = o + c;
-        const double dd = (k == 0) ? rhs / den[t] : (rhs - sub[t] * dprev[r]) / den[t];
+        const double dd = (k == 0) ? rhs / den_k : (rhs - sub_k * dprev[r]) / den_k;
         dprev[r] = dd;
         (r == 0 ? Ao : S_out)[t] = dd;
       }
@@ -611,6 +618,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     const long long o = (long long)k * n2;
     const MomLds<R>& L = LB[k & 1];
     if (k + 1 < nz) mom_load(RG, MO, o + n2, n2, u, v, phi, b, w, geo);
+    // the own column's viscosity below level k, issued before the face pass
+    const double km_k1 = active ? Km[o + n2 + c] : 0.0;
     {
       const R(*U)[SX] = L.u;
       const R(*V)[SX] = L.v;
@@ -722,7 +731,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
         avu[t] = (den > 0.0 ? num / den : 0.0) * mv_c;
       }
       // explicit vertical diffusion with surface stress and bottom drag
-      const double km_k1 = Km[o + n2 + c];
       dzu[t] = explicit_diffusion_w(k, nz, uw5[1], uw5[2], uw5[3], duw[0], duw[1], duw[2],
                                     m3w[0], m3w[1], m3w[2], km_k, km_k1,
                                     k == 0 ? geo.tau_u[c] : 0.0, p.bottom_drag);

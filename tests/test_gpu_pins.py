@@ -2,7 +2,8 @@
 recorded: the R2 Helmholtz gate iteration counts (docs/25_kernel_matrix.md:
 37-43), 30 hours of Kato-Phillips mixed-layer deepening under the TKE
 closure (v06_verification_20260923.json, key v6_3) and the tier-2
-lock-exchange solver-iteration totals at 400^2 and 1000^2 (expr/E13_tier2_ladder);
+lock-exchange solver-iteration totals at 400^2 and 1000^2 (expr/E13_tier2_ladder),
+the v6_1 tracer-advection operator errors (v06_verification_20260923.json);
 plus a CPU check that the benchmark's ladder workload builder agrees with
 the config path."""
 
@@ -133,3 +134,34 @@ def test_ladder_builder_matches_config_path():
     assert np.array_equal(prob.domain.H, H) and np.array_equal(prob.domain.mask, mask)
     assert prob.physics == physics_v05_from_config(cfg)
     assert prob.dt == 4.0 * barotropic_dt_max(prob.domain, prob.physics.g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["centered2", "upwind1", "up3", "up3_tvd"])
+def test_tracer_advection_operator_errors_v6_1_on_device(gpu_available, scheme):
+    """The recorded v6_1 operator errors (tools/verify_v06.py:77-105) from
+    the device's tracer-advection kernel: DIV(u C) of 10 + sin(kx) carried by
+    u = 1 against k cos(kx) on flat periodic n x n x 4 grids, n = 32 .. 256,
+    to the same 1e-10 relative as the oracle's pin."""
+    from test_oracle_pins import V6_1
+
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.stepper import HipStepper
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    errs = []
+    for n in (32, 64, 128, 256):
+        print(f"v6_1 {scheme} n={n}", flush=True)
+        grid = CGrid(nx=n, ny=n, Lx=1.0e5, Ly=float(f"{1.0e5 * 4 / n:.6e}"), nz=4, depth=100.0)
+        dom = Domain(grid=grid, H=np.full(grid.shape, 100.0), mask=np.ones(grid.shape))
+        phys = PhysicsV05(base=Physics3D(g=9.80616, f=0.0, H=100.0, tracers="TS", alpha_T=0.0,
+                                         beta_S=0.0), pgf="keep")
+        st = HipStepper(dom, phys, SchemeParams(name="fb", advection=scheme), 1.0)
+        x, _ = grid.coords("eta")
+        k = 2.0 * np.pi / grid.Lx
+        c = np.broadcast_to(10.0 + np.sin(k * x), grid.shape3d).copy()
+        u = np.full(grid.shape3d, 1.0)
+        w = np.zeros((grid.nz + 1,) + grid.shape)
+        div = st.tracer_advection(c, u, np.zeros_like(u), w).cpu().numpy()
+        exact = np.broadcast_to(k * np.cos(k * x), grid.shape3d)
+        errs.append(float(np.sqrt(np.mean((div - exact) ** 2)) / np.sqrt(np.mean(exact ** 2))))
+    assert np.allclose(errs, V6_1[scheme], rtol=1e-10, atol=0), (scheme, errs)

@@ -20,7 +20,7 @@ The JSON line also carries
                 launch time (HIP events on the launch stream, inside the timed
                 region), against 8 TB/s; roofline_predictor the same for the
                 predictor (tied with it for the largest share of the step) with
-                the kernel's own compulsory bytes, 131 B per layer cell, over
+                the kernel's own compulsory bytes, 115 B per layer cell, over
                 10 further steps (no stage of the model is that kernel alone);
   step_model    the whole step against the stage model of the step's
                 compulsory traffic: 336 B per layer cell plus, per barotropic
@@ -70,15 +70,16 @@ HBM_PEAK_GBS = 8000.0
 # ignored). Stage figures follow the stage model of the step:
 #   k_tracers      T, S, u, v, w, K_h, dz3 read; T, S, b written     (72 B)
 #   k_closure      e, K_m, K_h, u, v, b, dz3 read; e, K_m, K_h written (80 B)
-#   k_predictor    u, v, the Picard iterate (u, v)_it, pgx, pgy, the Coriolis
-#                  partners av_prev (2), explicit diffusion (2) and stress (2)
-#                  terms, the Thomas factors sub, den, c* read, masks m3, m3u,
-#                  m3v (3 B); gh_u, gh_v written: 139 B, or 123 B in the first
-#                  Picard pass (the iterate is the state itself); 131 B per
-#                  launch on average over the two passes
-#   k_momentum_explicit  u, v, b, w, K_m read; 8 explicit terms written (104 B)
-ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 131.0,
-                      "k_momentum_explicit": 104.0}
+#   k_predictor    u, v, the Picard iterate (u, v)_it, pgx, pgy, explicit
+#                  diffusion (2) and stress (2) terms, the Thomas factors sub,
+#                  den, c* read, masks m3, m3u, m3v (3 B); gh_u, gh_v written:
+#                  123 B, or 107 B in the first Picard pass (the iterate is the
+#                  state itself); 115 B per launch on average over the two
+#                  passes (the Coriolis partners of the state are formed from
+#                  u and v, whose neighbours are cached)
+#   k_momentum_explicit  u, v, b, w, K_m read; 6 explicit terms written (88 B)
+ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 115.0,
+                      "k_momentum_explicit": 88.0}
 # Measured HBM bytes per launch at 512 x 512 x 40 (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
 # the summary is profiles/basin512x512x40_hbm_traffic_pmc.md). Re-measured

@@ -1832,7 +1832,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
   launch_momentum_explicit_tiled(h->cfg.precision == 2, s, d, p, geo, in->u, in->v, in->b, h->phi,
-                                 h->w, h->Km, h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
+                                 h->w, h->Km, h->pgx, h->pgy, nullptr, nullptr, h->dzu, h->dzv,
                                  h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
   const bool split = h->cfg.scheme == 2;
@@ -1845,9 +1845,8 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     double* gv = h->gh_v[it & 1];
     { ProfScope _ps(h, "k_predictor", s);
   k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
-                                             h->pgx, h->pgy, h->avu, h->avv, h->dzu, h->dzv,
-                                             h->stu, h->stv, h->sub_m, h->den_m, h->cst_m, gu, gv,
-                                             h->Ug, h->Vg); }
+                                             h->pgx, h->pgy, h->dzu, h->dzv, h->stu, h->stv,
+                                             h->sub_m, h->den_m, h->cst_m, gu, gv, h->Ug, h->Vg); }
     LAUNCH_CHECK("k_predictor");
     h->tridiagonal_solves += 2;
     if (theta) {

@@ -98,9 +98,9 @@ __global__ void k_split_final(Dims d, Geo geo, const double* gh_u, const double*
 __global__ void k_face_depth(Dims d, Geo geo, double* Hu, double* Hv);
 __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* u, const double* v,
                             const double* eta, const double* u_it, const double* v_it,
-                            const double* pgx, const double* pgy, const double* avu,
-                            const double* avv, const double* dzu, const double* dzv,
-                            const double* stu, const double* stv, const double* sub,
+                            const double* pgx, const double* pgy, const double* dzu,
+                            const double* dzv, const double* stu, const double* stv,
+                            const double* sub,
                             const double* den, const double* cst, double* gh_u, double* gh_v,
                             double* Ug, double* Vg);
 __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* gh_u,

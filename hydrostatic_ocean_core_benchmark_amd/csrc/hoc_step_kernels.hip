@@ -789,8 +789,10 @@ __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double*
   pgx[t] = gx;
   pgy[t] = gy;
 
-  avv[t] = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
-  avu[t] = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
+  if (avv) {  // the Coriolis partners, for callers that ask for them
+    avv[t] = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
+    avu[t] = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
+  }
 
   dzu[t] = explicit_diffusion(d, geo.m3, u, Km, geo.dz3u, k == 0 ? geo.tau_u[c] : 0.0,
                               p.bottom_drag, c, k);
@@ -1025,7 +1027,6 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restr
                             const double* __restrict__ v, const double* __restrict__ eta,
                             const double* __restrict__ u_it, const double* __restrict__ v_it,
                             const double* __restrict__ pgx, const double* __restrict__ pgy,
-                            const double* __restrict__ avu, const double* __restrict__ avv,
                             const double* __restrict__ dzu, const double* __restrict__ dzv,
                             const double* __restrict__ stu, const double* __restrict__ stv,
                             const double* __restrict__ sub, const double* __restrict__ den,
@@ -1045,8 +1046,15 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restr
   for (int k = 0; k < nz; ++k) {
     const long long o = k * n2;
     const long long t = o + c;
-    const double vcor = p.one_m_tc * avv[t] + p.theta_cor * avg_v_to_u(v_it, geo.m3v, geo.m3u, o, q);
-    const double ucor = p.one_m_tc * avu[t] + p.theta_cor * avg_u_to_v(u_it, geo.m3u, geo.m3v, o, q);
+    // Coriolis partners of the old state, formed here from u and v (their
+    // neighbours are in cache) instead of stored and re-read; in the first
+    // Picard pass the iterate is the old state itself
+    const double av_v = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
+    const double av_u = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
+    const double av_v_it = v_it == v ? av_v : avg_v_to_u(v_it, geo.m3v, geo.m3u, o, q);
+    const double av_u_it = u_it == u ? av_u : avg_u_to_v(u_it, geo.m3u, geo.m3v, o, q);
+    const double vcor = p.one_m_tc * av_v + p.theta_cor * av_v_it;
+    const double ucor = p.one_m_tc * av_u + p.theta_cor * av_u_it;
     const double gxe = gx2 * mk(geo.m3u[t]);
     const double gye = gy2 * mk(geo.m3v[t]);
     // split_explicit leaves the surface gradient to the barotropic substeps

@@ -716,14 +716,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
         vc0 = vw5[2], vs = v[o + rs + i], ve = v[o + rj + ie], vse = v[o + rs + ie];
         uc0 = uw5[2], uw = u[o + rj + iw], un = u[o + rn + i], unw = u[o + rn + iw];
       }
-      {
+      if (avv) {  // the Coriolis partners, for callers that ask for them
         const double w0 = mv_c, w1 = mk(L.mv[cy - 1][cx]), w2 = mk(L.mv[cy][cx + 1]),
                      w3 = mk(L.mv[cy - 1][cx + 1]);
         const double num = 0.0 + w0 * vc0 + w1 * vs + w2 * ve + w3 * vse;
         const double den = 0.0 + w0 + w1 + w2 + w3;
         avv[t] = (den > 0.0 ? num / den : 0.0) * mu_c;
       }
-      {
+      if (avu) {
         const double w0 = mu_c, w1 = mk(L.mu[cy][cx - 1]), w2 = mk(L.mu[cy + 1][cx]),
                      w3 = mk(L.mu[cy + 1][cx - 1]);
         const double num = 0.0 + w0 * uc0 + w1 * uw + w2 * un + w3 * unw;

@@ -1799,6 +1799,10 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     return fail(HOC_ERR_INVALID, "tracers_ts needs T and S");
   const bool tke = h->cfg.closure_tke != 0;
   if (tke && (!in->e || !out->e)) return fail(HOC_ERR_INVALID, "closure_tke needs e");
+  // the step reads the old state after it has written parts of the new one
+  if (in->u == out->u || in->v == out->v || in->b == out->b || in->eta == out->eta ||
+      (ts && (in->T == out->T || in->S == out->S)))
+    return fail(HOC_ERR_INVALID, "the output state must not alias the input state");
   if (!h->coeffs_built) TRY(build_coefficients(h, (hipStream_t)stream));
   hipStream_t s = (hipStream_t)stream;
   const Dims d = h->d;

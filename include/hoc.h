@@ -111,8 +111,9 @@ int hoc_stepper_bind_mixing(hoc_stepper* h, double* K_m, double* K_h);
 int hoc_stepper_build_coefficients(hoc_stepper* h, void* stream);
 
 /* Stepper3DV05.step (model3d_v05.py:337-415): reads `in`, writes `out`
- * (distinct buffers). Synchronises on `stream` once per Helmholtz solve to
- * poll convergence. */
+ * (distinct buffers: an out field that aliases its in field is refused with
+ * HOC_ERR_INVALID before anything runs). Synchronises on `stream` once per
+ * Helmholtz solve to poll convergence. */
 int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream);
 
 /* The current depth of the cell centres [nz][ny][nx] (device, written on

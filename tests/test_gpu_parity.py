@@ -358,3 +358,19 @@ def test_helmholtz_large_grid_matches_oracle(hoc, kind):
     print(f"large {kind}: {iters} iterations (oracle {rep.iterations})", flush=True)
     assert conv == rep.converged and iters == rep.iterations
     assert np.array_equal(_np(x), want)
+
+
+def test_step_rejects_an_output_that_aliases_the_input(hoc):
+    """The step reads the old state after writing parts of the new one, so
+    stepping a state in place is refused before any kernel runs."""
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    grid = CGrid(nx=16, ny=8, Lx=1.6e5, Ly=8.0e4, nz=4, depth=100.0)
+    dom = Domain(grid=grid, H=np.full(grid.shape, 100.0), mask=np.ones(grid.shape))
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=100.0))
+    st = hoc.HipStepper(dom, phys, SchemeParams(name="theta", solver="pcg_jacobi"), 60.0)
+    bufs = st._new_state_buffers()
+    for t in bufs.values():
+        t.zero_()
+    with pytest.raises(ValueError, match="must not alias"):
+        st._step_into(bufs, bufs)

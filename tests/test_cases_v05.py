@@ -151,3 +151,11 @@ def test_rpe_of_a_stable_column_in_a_box():
     assert case._rpe(b) == pytest.approx(want, rel=1e-12)
     # an unstable arrangement has the same RPE: the sort removes the order
     assert case._rpe(b[::-1].copy()) == pytest.approx(want, rel=1e-12)
+
+
+def test_suite_refuses_unknown_checks(capsys):
+    """An unknown --only id fails before any check runs (an empty run must
+    not pass), as tools/verify_v05.py:302-311."""
+    from hydrostatic_ocean_core_benchmark_amd.verify_v05 import main
+    assert main(["--only", "v5_9"]) == 1
+    assert "unknown check" in capsys.readouterr().out

@@ -2,11 +2,12 @@
 
 A NumPy fp64 restatement of the reference step (libs/core/model3d_v05.py
 `Stepper3DV05.step` and the operators it calls), written to reproduce the
-reference's floating-point operation order so that it is a bit-faithful
-checker. Every function cites the reference file:line it follows.
+reference's floating-point operation order so that results can be checked
+against it bit for bit. Every function cites the reference file:line it
+follows.
 
 Only tests/, `__graft_entry__.smoke()` and bench.py's CPU-baseline leg may
-import this package, and only as the checker or the timed CPU baseline. The
+import this package, and only to check results or as the timed CPU baseline. The
 product (`hydrostatic_ocean_core_benchmark_amd`) never imports it.
 
 Pinning (how each part is checked against something other than itself;

@@ -8,7 +8,7 @@ round-off: the composed step (1-3 steps on tiny grids, and 200 steps of the
 `momentum_advection_up3` next to walls, the pressure gradient with partial
 cells, the masked Coriolis averages at coasts, the Helmholtz solvers'
 trajectories on arbitrary coefficients, and `Domain` geometry. Only inputs
-and outputs are stored; the GPU box never sees the reference.
+and outputs are stored; the reference itself is never needed to run the tests.
 
 Usage:  python tests/golden/make_golden.py [--out tests/golden] [--model2d-only]
 """

@@ -382,49 +382,36 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
 }
 
 // ------------------------------------------------ vertical coefficients
-// Rows of A = I - theta_v dt Dz for viscosity nu on the layer grid with the
-// surface, bottom-drag and dry-cell rows of libs/core/vertical_var.py:35-77,
-// stored factored as (sub, den, c*) of the Thomas forward sweep
-// (libs/core/vertical.py:33-38), which depends on the matrix alone. Each
-// later solve with this matrix then runs d*_k = (r_k - sub_k d*_{k-1}) / den_k
-// and x_k = d*_k - c*_k x_{k+1}, bitwise the unfactored sweep.
-__device__ __forceinline__ void factor_column(const Dims& d, const StepParams& p, const Geo& geo,
-                                              long long c, const double* __restrict__ nu,
-                                              double drag_term, double* __restrict__ sub_o,
-                                              double* __restrict__ den_o,
-                                              double* __restrict__ cst_o) {
-  const int nz = d.nz;
-  const long long n2 = d.n2;
-  double c_prev = 0.0;
-  for (int k = 0; k < nz; ++k) {
-    const double inv_dz = safe_inv(geo.dz3[k * n2 + c]);
-    const double fac = p.thv_dt * inv_dz;
-    const double a_top = fac * nu[k * n2 + c] * safe_inv(geo.dzi[k * n2 + c]);
-    const double a_bot = fac * nu[(k + 1) * n2 + c] * safe_inv(geo.dzi[(k + 1) * n2 + c]);
-    double sub = -a_top, sup = -a_bot;
-    double diag = 1.0 + a_top + a_bot;
-    if (k == 0) {
-      diag = 1.0 + a_bot;
-      sub = 0.0;
-    }
-    const bool wet = geo.m3[k * n2 + c] != 0;
-    const bool below = (k + 1 < nz) && geo.m3[(k + 1) * n2 + c] != 0;
-    if (wet && !below) {
-      diag = 1.0 + a_top + drag_term * inv_dz;
-      sup = 0.0;
-    }
-    if (!wet) {
-      diag = 1.0;
-      sub = 0.0;
-      sup = 0.0;
-    }
-    const double den = (k == 0) ? diag : diag - sub * c_prev;
-    const double cst = sup / den;
-    sub_o[k * n2 + c] = sub;
-    den_o[k * n2 + c] = den;
-    cst_o[k * n2 + c] = cst;
-    c_prev = cst;
+// Row k of one implicit vertical-diffusion matrix (I - theta_v dt D[nu]) of
+// a column, factored for the Thomas sweep: sub, den = diag - sub c*_{k-1}
+// and c* = sup / den (libs/core/vertical_var.py:35-77). inv_dz, idzi_t and
+// idzi_b are 1/dz3 and 1/dzi above and below the cell.
+__device__ __forceinline__ void factor_row(const StepParams& p, int k, double inv_dz,
+                                           double nu_t, double nu_b, double idzi_t,
+                                           double idzi_b, bool wet, bool below,
+                                           double drag_term, double c_prev, double& sub_o,
+                                           double& den_o, double& cst_o) {
+  const double fac = p.thv_dt * inv_dz;
+  const double a_top = fac * nu_t * idzi_t;
+  const double a_bot = fac * nu_b * idzi_b;
+  double sub = -a_top, sup = -a_bot;
+  double diag = 1.0 + a_top + a_bot;
+  if (k == 0) {
+    diag = 1.0 + a_bot;
+    sub = 0.0;
   }
+  if (wet && !below) {
+    diag = 1.0 + a_top + drag_term * inv_dz;
+    sup = 0.0;
+  }
+  if (!wet) {
+    diag = 1.0;
+    sub = 0.0;
+    sup = 0.0;
+  }
+  den_o = (k == 0) ? diag : diag - sub * c_prev;
+  cst_o = sup / den_o;
+  sub_o = sub;
 }
 
 // tri_m (viscosity Km, bottom drag) and tri_b (diffusivity Kh, no drag),
@@ -436,16 +423,41 @@ __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* __re
                                double* __restrict__ cst_b, double* __restrict__ q) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  factor_column(d, p, geo, c, Km, p.thv_dt_drag, sub_m, den_m, cst_m);
-  factor_column(d, p, geo, c, Kh, p.thv_dt * 0.0, sub_b, den_b, cst_b);
   const int nz = d.nz;
   const long long n2 = d.n2;
-  double d_prev = 0.0;
+  // both factorizations and the forward sweep of q in one pass over the
+  // column, so the geometry is read once and tri_m is not read back
+  const double drag_b = p.thv_dt * 0.0;
+  double cm_prev = 0.0, cb_prev = 0.0, d_prev = 0.0;
+  double idzi_t = safe_inv(geo.dzi[c]);
+  double km_t = Km[c], kh_t = Kh[c];
+  bool wet = geo.m3[c] != 0;
   for (int k = 0; k < nz; ++k) {
     const long long o = k * n2 + c;
-    const double dst = (k == 0) ? 1.0 / den_m[o] : (1.0 - sub_m[o] * d_prev) / den_m[o];
+    const double inv_dz = safe_inv(geo.dz3[o]);
+    const double idzi_b = safe_inv(geo.dzi[o + n2]);
+    const double km_b = Km[o + n2], kh_b = Kh[o + n2];
+    const bool below = (k + 1 < nz) && geo.m3[o + n2] != 0;
+    double sm, dm, cm, sb, db, cb;
+    factor_row(p, k, inv_dz, km_t, km_b, idzi_t, idzi_b, wet, below, p.thv_dt_drag, cm_prev,
+               sm, dm, cm);
+    factor_row(p, k, inv_dz, kh_t, kh_b, idzi_t, idzi_b, wet, below, drag_b, cb_prev, sb, db,
+               cb);
+    sub_m[o] = sm;
+    den_m[o] = dm;
+    cst_m[o] = cm;
+    sub_b[o] = sb;
+    den_b[o] = db;
+    cst_b[o] = cb;
+    const double dst = (k == 0) ? 1.0 / dm : (1.0 - sm * d_prev) / dm;
     q[o] = dst;
     d_prev = dst;
+    cm_prev = cm;
+    cb_prev = cb;
+    idzi_t = idzi_b;
+    km_t = km_b;
+    kh_t = kh_b;
+    wet = below;
   }
   double x_next = q[(nz - 1) * n2 + c];
   for (int k = nz - 2; k >= 0; --k) {

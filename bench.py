@@ -84,8 +84,8 @@ ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 115.0
 # WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
 # the summary is profiles/basin512x512x40_hbm_traffic_pmc.md). Re-measured
 # whenever the kernel changes.
-MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.900e9, "k_momentum_explicit": 1.386e9,
-                           "k_predictor": 2.092e9, "k_closure": 1.405e9}
+MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.899e9, "k_momentum_explicit": 1.397e9,
+                           "k_predictor": 2.088e9, "k_closure": 1.402e9}
 STEP_BYTES_PER_CELL = 336.0
 SOLVE_BYTES_PER_COLUMN = 64.0
 ITER_BYTES_PER_COLUMN = {"pcg_mg": 520.0, "pcg_jacobi": 120.0, "pcg_rbgs": 520.0,

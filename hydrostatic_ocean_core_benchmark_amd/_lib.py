@@ -6,7 +6,9 @@ error: there is no CPU fallback for the step.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libhoc.so"
@@ -193,3 +195,25 @@ def ptr(t) -> C.c_void_p | None:
     if t is None:
         return None
     return C.c_void_p(t.data_ptr())
+
+
+# Objects that own a library handle. Their close() runs from an atexit hook,
+# so every handle is destroyed while the HIP runtime is still whole (Python's
+# atexit hooks run before the C library's exit handlers tear the runtime
+# down); a handle left to the garbage collector at interpreter shutdown would
+# otherwise free device memory from inside that teardown.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def register_handle(owner) -> None:
+    """Track `owner` (an object with close()) for the exit hook."""
+    _LIVE.add(owner)
+
+
+@atexit.register
+def _close_live_handles() -> None:
+    for owner in list(_LIVE):
+        try:
+            owner.close()
+        except Exception:   # noqa: BLE001 - best effort at exit
+            pass

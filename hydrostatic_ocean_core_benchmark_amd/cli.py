@@ -30,7 +30,20 @@ bench   per-cell cost of the step over grid sizes (ns/cell/step, median of
         `--repeat` timed runs after one warm-up run).
 
 Without a 2D case, run|verify|bench drive the v0.5 step with the same
-contract (flags, config tree, overrides, output directory).
+contract (flags, config tree, overrides, output directory). The lock_exchange
+and basin cases are fixed workloads (their physics and scheme are those of
+the benchmark); of the config tree the v0.5 route reads `[run] backend`,
+`[run] precision`, `basin.nz`, `basin.solver` and `paths.output_dir`.
+
+[run] keys (config/settings.toml; the reference declares the same keys):
+  backend    "hip" -- the only backend of this package; any other value
+             (the reference's ref | fortran_cpu | openacc | cuda | py_gpu)
+             exits with an error instead of running something else;
+  precision  "float64" (the plain step, the reference's arithmetic) |
+             "mixed" | "mixed_fp32_momentum" (theta + pcg_mg only; see
+             HipStepper); the reference's whole-core "float32" is not
+             offered and exits with an error. The 2D and v0.2 models run
+             float64 only.
 """
 
 from __future__ import annotations
@@ -68,6 +81,24 @@ def field_error(a: np.ndarray, b: np.ndarray, state_scale: float) -> float:
     return num / state_scale if state_scale > 0.0 else num
 
 
+PRECISIONS = {"float64": "fp64", "fp64": "fp64", "mixed": "mixed",
+              "mixed_fp32_momentum": "mixed_fp32_momentum"}
+
+
+def run_settings(config) -> tuple[str, str]:
+    """([run] backend, the HipStepper precision for [run] precision); an
+    unsupported value exits with an error naming it."""
+    backend = str(config.get("run.backend", "hip"))
+    if backend != "hip":
+        raise SystemExit(f"run.backend = {backend!r} is not this package's backend: it runs "
+                         "'hip' (the MI355X kernels); set run.backend = \"hip\"")
+    precision = str(config.get("run.precision", "float64"))
+    if precision not in PRECISIONS:
+        raise SystemExit(f"run.precision = {precision!r} is not offered "
+                         "(float64 | mixed | mixed_fp32_momentum)")
+    return backend, PRECISIONS[precision]
+
+
 def build_problem(config, case: str | None, nx: int | None, cfl: float | None,
                   steps: int | None) -> C.Problem:
     name = case or str(config.get("case.name", "lock_exchange"))
@@ -82,6 +113,7 @@ def build_problem(config, case: str | None, nx: int | None, cfl: float | None,
         prob.dt = cfl * barotropic_dt_max(prob.domain, prob.physics.g)
     if steps is not None:
         prob.n_steps = int(steps)
+    prob.meta["precision"] = run_settings(config)[1]
     return prob
 
 
@@ -164,7 +196,11 @@ def _integrate(prob: C.Problem, n_warmup: int = 0, n_repeat: int = 1):
     from .stepper import HipStepper
     from .types import State3D
     g = prob.domain.grid
-    st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt)
+    try:
+        st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt,
+                        precision=prob.meta.get("precision", "fp64"))
+    except ValueError as exc:
+        raise SystemExit(f"{prob.name}: {exc}") from exc
     b = st.buoyancy_of(prob.initial_T, prob.initial_S) * torch.as_tensor(
         prob.domain.mask3, device=st.device)
     s0 = State3D(u=np.zeros(g.shape3d), v=np.zeros(g.shape3d), b=b, eta=np.zeros(g.shape),
@@ -433,12 +469,14 @@ def cmd_run(config, args) -> int:
     metrics = {"case": prob.name, "nx": g.nx, "ny": g.ny, "nz": g.nz,
                "cells": g.nx * g.ny * g.nz, "dt": prob.dt, "n_steps": prob.n_steps,
                "scheme": prob.params.name, "solver": prob.params.solver,
+               "backend": "hip", "precision": st.precision,
                "solver_iterations": iters, "solver_failures": st.solver_failures,
                "wall_s": wall, "eta_max": float(np.max(np.abs(host["eta"]))),
                "finite": bool(all(np.isfinite(a).all() for a in host.values()))}
     (out / "metrics.json").write_text(json.dumps(metrics, indent=2) + "\n", encoding="utf-8")
     print(f"case={prob.name} {g.nx}x{g.ny}x{g.nz} scheme={prob.params.name} "
-          f"solver={prob.params.solver} dt={prob.dt:.3f}s steps={prob.n_steps} "
+          f"solver={prob.params.solver} precision={st.precision} dt={prob.dt:.3f}s "
+          f"steps={prob.n_steps} "
           f"wall={wall:.3f}s iters={iters} -> {out}")
     return 0 if metrics["finite"] else 1
 
@@ -540,6 +578,11 @@ def main(argv: list[str] | None = None) -> int:
         config = apply_overrides(load_config(args.config), args.set)
     except ValueError as exc:
         raise SystemExit(str(exc)) from exc
+    _, precision = run_settings(config)
+    two_d = args.command in ("run", "verify", "bench") and _case_2d(config, args)
+    if precision != "fp64" and (two_d or args.command in ("verify3d", "bench3d")):
+        raise SystemExit(f"run.precision = {config.get('run.precision')!r}: the 2D and v0.2 "
+                         "models run float64 only")
     return {"run": cmd_run, "verify": cmd_verify, "bench": cmd_bench, "verify3d": cmd_verify3d,
             "bench3d": cmd_bench3d}[args.command](config, args)
 

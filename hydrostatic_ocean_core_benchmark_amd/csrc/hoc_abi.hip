@@ -211,6 +211,13 @@ struct hoc_stepper {
 
 namespace {
 
+// A handle's device buffers are freed only after the device has finished the
+// work queued on them, and only while the runtime is healthy: at process exit
+// the HIP runtime may already be torn down (and after a device error every
+// call fails), and then the handle is left to the process's end rather than
+// calling into the runtime again.
+bool release_is_safe() { return hipDeviceSynchronize() == hipSuccess; }
+
 int derive_params(const hoc_config& c, StepParams& p) {
   std::memset(&p, 0, sizeof(p));
   p.scheme = c.scheme;
@@ -960,7 +967,7 @@ int hoc_sw2d_create(const hoc_sw2d_config* cfg, hoc_sw2d** out) {
 }
 
 int hoc_sw2d_destroy(hoc_sw2d* h) {
-  delete h;
+  if (h && release_is_safe()) delete h;
   return HOC_OK;
 }
 
@@ -1053,7 +1060,7 @@ int hoc_helm2d_create(const hoc_helm2d_config* cfg, hoc_helm2d** out) {
 }
 
 int hoc_helm2d_destroy(hoc_helm2d* h) {
-  delete h;
+  if (h && release_is_safe()) delete h;
   return HOC_OK;
 }
 
@@ -1253,7 +1260,7 @@ int hoc_m3_create(const hoc_m3_config* cfg, hoc_m3** out) {
 }
 
 int hoc_m3_destroy(hoc_m3* h) {
-  delete h;
+  if (h && release_is_safe()) delete h;
   return HOC_OK;
 }
 
@@ -1697,7 +1704,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
 }
 
 int hoc_stepper_destroy(hoc_stepper* h) {
-  delete h;
+  if (h && release_is_safe()) delete h;
   return HOC_OK;
 }
 
@@ -1799,10 +1806,23 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     return fail(HOC_ERR_INVALID, "tracers_ts needs T and S");
   const bool tke = h->cfg.closure_tke != 0;
   if (tke && (!in->e || !out->e)) return fail(HOC_ERR_INVALID, "closure_tke needs e");
-  // the step reads the old state after it has written parts of the new one
-  if (in->u == out->u || in->v == out->v || in->b == out->b || in->eta == out->eta ||
-      (ts && (in->T == out->T || in->S == out->S)))
-    return fail(HOC_ERR_INVALID, "the output state must not alias the input state");
+  // the step reads the old state after it has written parts of the new one:
+  // no output array may be any input array, and the outputs are distinct
+  {
+    const double* ins[7] = {in->u, in->v, in->b, in->eta, ts ? in->T : nullptr,
+                            ts ? in->S : nullptr, tke ? in->e : nullptr};
+    const double* outs[7] = {out->u, out->v, out->b, out->eta, ts ? out->T : nullptr,
+                             ts ? out->S : nullptr, tke ? out->e : nullptr};
+    for (int a = 0; a < 7; ++a) {
+      if (!outs[a]) continue;
+      for (int q = 0; q < 7; ++q)
+        if (outs[a] == ins[q])
+          return fail(HOC_ERR_INVALID, "the output state must not alias the input state");
+      for (int q = a + 1; q < 7; ++q)
+        if (outs[a] == outs[q])
+          return fail(HOC_ERR_INVALID, "the output state's arrays must be distinct");
+    }
+  }
   if (!h->coeffs_built) TRY(build_coefficients(h, (hipStream_t)stream));
   hipStream_t s = (hipStream_t)stream;
   const Dims d = h->d;

@@ -801,9 +801,9 @@ __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double*
   pgx[t] = gx;
   pgy[t] = gy;
 
-  if (avv) {  // the Coriolis partners, for callers that ask for them
-    avv[t] = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
-    avu[t] = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
+  {  // the Coriolis partners, for callers that ask for them
+    if (avv) avv[t] = avg_v_to_u(v, geo.m3v, geo.m3u, o, q);
+    if (avu) avu[t] = avg_u_to_v(u, geo.m3u, geo.m3v, o, q);
   }
 
   dzu[t] = explicit_diffusion(d, geo.m3, u, Km, geo.dz3u, k == 0 ? geo.tau_u[c] : 0.0,
@@ -1030,7 +1030,8 @@ __global__ void k_face_depth(Dims d, Geo geo, double* __restrict__ Hu, double* _
 
 // ------------------------------------------- predictor and column solves
 // One lane per column, one Picard pass (model3d_v05.py:387-396):
-//   v_cor = (1-tc) avv + tc avg_v_to_u(v_it),  u_cor likewise
+//   v_cor = (1-tc) avg_v_to_u(v) + tc avg_v_to_u(v_it),  u_cor likewise
+// (both partners formed here, from the state and from the iterate)
 //   gu = u + dt (f v_cor + pgx) - g dt (1-th) Gx(eta) + (1-thv) dt dzu
 //   gh_u = A_m^-1 (gu + stu) * mask3, and the same for v,
 // then the depth sums Ug = sum dz3u gh_u, Vg = sum dz3v gh_v for the

@@ -22,6 +22,7 @@ build_solver of libs/core/solvers.py:261-281) by
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass, replace
 
 import numpy as np
@@ -123,15 +124,21 @@ class Helmholtz2D:
         h = C.c_void_p()
         _lib.check(self.lib.hoc_helm2d_create(C.byref(cfg), C.byref(h)))
         self._h = h
+        _lib.register_handle(self)
         nl = C.c_int32(0)
         _lib.check(self.lib.hoc_helm2d_levels(h, C.byref(nl)))
         self.n_levels = nl.value
 
-    def __del__(self):
+    def close(self) -> None:
+        """Destroy the library handle. Idempotent; also runs at interpreter exit."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            self.lib.hoc_helm2d_destroy(h)
             self._h = None
+            self.lib.hoc_helm2d_destroy(h)
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
     def solve(self, rhs: torch.Tensor):
         rhs = rhs.to(device=self.device, dtype=torch.float64).contiguous()
@@ -176,6 +183,7 @@ class SWStepper:
         h = C.c_void_p()
         _lib.check(self.lib.hoc_sw2d_create(C.byref(cfg), C.byref(h)))
         self._h = h
+        _lib.register_handle(self)
         self.solver = None
         if params.name == "theta":
             # schemes.py:110-113
@@ -183,11 +191,16 @@ class SWStepper:
             self.solver = Helmholtz2D(params.solver, grid, coef, params.rtol, params.max_iter,
                                       device=self.device)
 
-    def __del__(self):
+    def close(self) -> None:
+        """Destroy the library handle. Idempotent; also runs at interpreter exit."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            self.lib.hoc_sw2d_destroy(h)
             self._h = None
+            self.lib.hoc_sw2d_destroy(h)
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)

@@ -19,6 +19,7 @@ the states are bitwise the reference's.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 import torch
@@ -86,6 +87,7 @@ class Stepper3D:
         h = C.c_void_p()
         _lib.check(self.lib.hoc_m3_create(C.byref(cfg), C.byref(h)))
         self._h = h
+        _lib.register_handle(self)
         self.tridiagonal_solves += 1
 
         # q = A^-1 1 and the effective depth (model3d.py:125-134): the column
@@ -112,11 +114,16 @@ class Stepper3D:
         """The effective-depth profile [nz, ny, nx] (every column alike)."""
         return np.broadcast_to(self._q1[:, None, None], self.grid.shape3d)
 
-    def __del__(self):
+    def close(self) -> None:
+        """Destroy the library handle. Idempotent; also runs at interpreter exit."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            self.lib.hoc_m3_destroy(h)
             self._h = None
+            self.lib.hoc_m3_destroy(h)
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
     # ---------------------------------------------------------------- helpers
     def _stream(self):

@@ -17,6 +17,7 @@ raise an error, never run a fallback.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import replace
 
 import numpy as np
@@ -160,6 +161,7 @@ class HipStepper:
         handle = C.c_void_p()
         _lib.check(self.lib.hoc_stepper_create(C.byref(cfg), C.byref(geo), C.byref(handle)))
         self._h = handle
+        _lib.register_handle(self)
         self.cfg = cfg
 
         opts = dict(dtype=torch.float64, device=self.device)
@@ -185,15 +187,26 @@ class HipStepper:
         _lib.check(self.lib.hoc_stepper_build_coefficients(self._h, self._stream()))
 
     # ------------------------------------------------------------ plumbing
-    def __del__(self):
+    def close(self) -> None:
+        """Destroy the library handle (its workspaces and multigrid levels)
+        after the device has finished queued work. Idempotent; the stepper
+        is unusable afterwards. Also runs at interpreter exit."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            try:
-                torch.cuda.synchronize(self.device)
-            except Exception:   # noqa: BLE001 - interpreter shutdown
-                pass
-            self.lib.hoc_stepper_destroy(h)
             self._h = None
+            self.lib.hoc_stepper_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        # at interpreter shutdown the exit hook has already closed live
+        # handles; never call into the runtime from the finaliser then
+        if not sys.is_finalizing():
+            self.close()
 
     def _stream(self) -> C.c_void_p:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)

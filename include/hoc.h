@@ -98,6 +98,9 @@ typedef struct hoc_stepper hoc_stepper;
 /* Stepper3DV05.__init__ (libs/core/model3d_v05.py:81-125): validates, uploads
  * the geometry and allocates the workspaces. */
 int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* geo, hoc_stepper** out);
+/* Waits for the device, then frees the handle. If the runtime reports an
+ * error (torn down at process exit, or a device fault) nothing is freed and
+ * no further runtime call is made. Every *_destroy below behaves the same. */
 int hoc_stepper_destroy(hoc_stepper* h);
 
 /* The stepper's vertical viscosity / diffusivity on interfaces (the

@@ -146,6 +146,115 @@ def lock_exchange_fixture() -> dict:
     return rec
 
 
+def array_forcing_fixture() -> dict:
+    """The step with array-valued tau_x(y), tau_y(x) and kappa_b(x, y)
+    (golden_cases.ARRAY_FORCING) -- the reference takes them by NumPy
+    broadcasting -- plus tke_step with an array kappa_b, checked here to equal
+    the reference's scalar-kappa_b runs assembled column by column (the
+    closure is columnwise), so broadcasting means per-column values."""
+    from golden_cases import ARRAY_FORCING, array_forcing_fields
+    dk, pk, vk, sk, dt, steps = ARRAY_FORCING
+    d = _domain(**dk)
+    tau_x, tau_y, kappa_b = array_forcing_fields(d.grid)
+    phys = PhysicsV05(base=Physics3D(**pk, tau_x=tau_x, tau_y=tau_y, kappa=kappa_b), **vk)
+    params = SchemeParams(**sk)
+    rng = np.random.default_rng(zlib.crc32(b"array_forcing"))
+    st0 = _random_state(d, rng, "TS", e=True)
+    stepper = Stepper3DV05(d, phys, params, dt)
+    assert np.array_equal(stepper.kappa[0], kappa_b)
+    st0["b"] = stepper.buoyancy_of(st0["T"], st0["S"]) * d.mask3
+    s = State3D(u=st0["u"], v=st0["v"], b=st0["b"], eta=st0["eta"], t=0.0,
+                T=st0["T"], S=st0["S"], e=st0["e"])
+    rec = _geometry(d, "geo_")
+    rec.update(tau_x=tau_x, tau_y=tau_y, kappa_b=kappa_b)
+    for k, v in st0.items():
+        rec[f"in_{k}"] = v
+    for n in range(1, steps + 1):
+        s = stepper.step(s)
+        for k in ("u", "v", "b", "eta", "T", "S", "e"):
+            rec[f"s{n}_{k}"] = getattr(s, k)
+        rec[f"s{n}_nu"] = stepper.nu
+        rec[f"s{n}_kappa"] = stepper.kappa
+        rec[f"s{n}_iters"] = np.array(stepper.solver_iterations)
+    # tke_step with kappa_b on three values in a pattern of columns
+    g = d.grid
+    nz1 = g.nz + 1
+    depth = d.depth_at_centres()
+    e = 1e-6 + 3e-4 * rng.random((nz1,) + g.shape)
+    u = 0.2 * rng.standard_normal(g.shape3d) * d.mask3u
+    v = 0.2 * rng.standard_normal(g.shape3d) * d.mask3v
+    b = (2e-3 * np.exp(-depth / 400.0) + 2e-5 * rng.standard_normal(g.shape3d)) * d.mask3
+    Km = 1e-4 + 1e-2 * rng.random((nz1,) + g.shape)
+    Kh = 1e-5 + 1e-2 * rng.random((nz1,) + g.shape)
+    ustar2 = 1e-4 * (1.0 + rng.random(g.shape))
+    values = np.array([0.8e-5, 1.0e-5, 1.2e-5])
+    pick = rng.integers(0, 3, g.shape)
+    kb = values[pick]
+    rec.update(tke_e=e, tke_u=u, tke_v=v, tke_b=b, tke_Km=Km, tke_Kh=Kh, tke_ustar2=ustar2,
+               tke_kappa_b=kb)
+    for mxl in ("integral", "recursive"):
+        p = TKEParams(mxl=mxl)
+        whole = tke_step(e, u, v, b, d.dz3_ref, d.mask3, ustar2, 300.0, p, 1e-4, kb, Km, Kh)
+        assembled = [np.zeros_like(a) for a in whole]
+        for q, val in enumerate(values):
+            part = tke_step(e, u, v, b, d.dz3_ref, d.mask3, ustar2, 300.0, p, 1e-4,
+                            float(val), Km, Kh)
+            for a, w in zip(assembled, part):
+                a[:, pick == q] = w[:, pick == q]
+        for a, w in zip(assembled, whole):
+            assert np.array_equal(a, w)
+        rec[f"tke_{mxl}_e"], rec[f"tke_{mxl}_Km"], rec[f"tke_{mxl}_Kh"] = whole
+    return rec
+
+
+# The lock_exchange_v06 settings of the reference's ladder sweep
+# (tools/tier2_sweep.sh:85), theta with PCG-Jacobi.
+LOCK_V06 = {"bathymetry.kind": "rough", "physics3d_v04.tracers": "TS",
+            "scheme.advection": "up3_tvd", "physics3d_v06.closure": "tke",
+            "physics3d_v05.eos": "teos10", "physics3d_v04.A_h": 10.0,
+            "physics3d_v04.K_h": 10.0, "physics.f0": 0.0, "physics3d.N2": 0.0,
+            "physics3d.nu": 1e-5, "physics3d.kappa": 1e-6, "physics3d.tau_x": 0.05,
+            "physics3d_v05.pgf": "keep", "grid.Lx": 6.4e4, "grid.Ly": 6.4e3,
+            "physics.H": 20.0, "scheme.name": "theta", "solver.kind": "pcg_jacobi"}
+
+
+def reference_bundle_fixture(out: Path) -> None:
+    """A v0.5 bundle written by the reference's own writer
+    (tools/toml2nml.py:143-273, write_v05_bundle) from its own config tree
+    with the lock_exchange_v06 settings at 16x16x8, CFL 2, 4 steps, into
+    out/ref_bundle/ (prefix ref_lock, paths relative to that folder), and
+    the reference stepper's state after those steps on the writer's own
+    problem objects (expected.npz)."""
+    import os
+    import shutil
+    import tempfile
+    sys.path.insert(0, str(REF / "tools"))
+    from toml2nml import write_v05_bundle
+    from libs.utils.config import load_config
+    cfg = load_config(REF / "config").with_overrides(LOCK_V06)
+    dest = out / "ref_bundle"
+    dest.mkdir(parents=True, exist_ok=True)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            res = write_v05_bundle("ref_lock", cfg, nx=16, nz=8, case_name="lock_exchange",
+                                   cfl=2.0, n_steps_override=4, n_repeat=1, n_warmup=0)
+        finally:
+            os.chdir(cwd)
+        for name in ("ref_lock.nml", "ref_lock_domain.bin", "ref_lock_init.bin"):
+            shutil.copy(Path(tmp) / name, dest / name)
+    st = Stepper3DV05(res["domain"], res["physics"], res["params"], res["dt"])
+    s = res["state0"]
+    iters = []
+    for _ in range(res["n_steps"]):
+        before = st.solver_iterations
+        s = st.step(s)
+        iters.append(st.solver_iterations - before)
+    np.savez_compressed(dest / "expected.npz", eta=s.eta, u=s.u, v=s.v, b=s.b, T=s.T, S=s.S,
+                        iters_per_step=np.array(iters), dt=np.array(res["dt"]))
+
+
 def operator_fixtures() -> dict:
     rng = np.random.default_rng(20261202)
     d = _domain(nx=24, ny=16, nz=10, Lx=2.4e5, Ly=1.6e5, H0=1000.0, topo="rough",
@@ -360,8 +469,20 @@ def main() -> int:
                     help="rewrite only the v0.2 3D-model fixture")
     ap.add_argument("--cases-only", action="store_true",
                     help="rewrite only the verification-case fixture")
+    ap.add_argument("--ref-bundle-only", action="store_true",
+                    help="rewrite only the bundle written by the reference's writer")
+    ap.add_argument("--array-forcing-only", action="store_true",
+                    help="rewrite only the array-forcing fixture")
     args = ap.parse_args()
     args.out.mkdir(parents=True, exist_ok=True)
+    if args.ref_bundle_only:
+        reference_bundle_fixture(args.out)
+        print("ref_bundle/ written")
+        return 0
+    if args.array_forcing_only:
+        np.savez_compressed(args.out / "step_theta_array_forcing.npz", **array_forcing_fixture())
+        print("step_theta_array_forcing.npz written")
+        return 0
     if args.cases_only:
         np.savez_compressed(args.out / "cases_2d_3d.npz", **case_fixtures())
         print("cases_2d_3d.npz written")
@@ -391,6 +512,10 @@ def main() -> int:
     print("model3d_v02_16x12x5.npz written")
     np.savez_compressed(args.out / "cases_2d_3d.npz", **case_fixtures())
     print("cases_2d_3d.npz written")
+    np.savez_compressed(args.out / "step_theta_array_forcing.npz", **array_forcing_fixture())
+    print("step_theta_array_forcing.npz written")
+    reference_bundle_fixture(args.out)
+    print("ref_bundle/ written")
     return 0
 
 

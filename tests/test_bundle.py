@@ -151,3 +151,84 @@ def test_launcher_runs_the_bundle_entry_point(tmp_path):
     assert "FATAL" in proc.stderr
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 1 and "usage" in proc.stderr
+
+
+REF_BUNDLE = ROOT / "tests" / "golden" / "ref_bundle"
+
+
+def _ref_bundle_copy(tmp_path, monkeypatch):
+    """The bundle the reference's writer produced (its paths are relative to
+    the folder it was written in), copied into a scratch working folder."""
+    import shutil
+    for p in REF_BUNDLE.iterdir():
+        shutil.copy(p, tmp_path / p.name)
+    monkeypatch.chdir(tmp_path)
+    return tmp_path / "ref_lock.nml"
+
+
+def test_reference_written_bundle_parses_and_rewrites_identically(tmp_path, monkeypatch):
+    """tools/toml2nml.py's own output (lock_exchange_v06 at 16x16x8): parsed
+    here, and written back by this package's writer, the domain and initial
+    state files are byte-identical to the reference's and every namelist key
+    the reference writes holds the same value."""
+    nml = _ref_bundle_copy(tmp_path, monkeypatch)
+    b = B.parse_bundle(nml)
+    exp = np.load(REF_BUNDLE / "expected.npz")
+    g = b.domain.grid
+    assert (g.nx, g.ny, g.nz, b.n_steps) == (16, 16, 8, 4) and b.dt == float(exp["dt"])
+    assert b.params.name == "theta" and b.params.solver == "pcg_jacobi"
+    assert b.physics.closure == "tke" and b.physics.eos == "teos10" and b.ts
+    init = B.read_state_file(b.init_file, g.nx, g.ny, g.nz, True)
+    s0 = State3D(u=init["u"], v=init["v"], b=init["b"], eta=init["eta"], T=init["T"],
+                 S=init["S"])
+    out = B.write_bundle(tmp_path / "mine", b.domain, b.physics, b.params, s0, b.dt,
+                         b.n_steps, case_name=b.case_name, out_prefix="ref_lock")
+    for suffix in ("_domain.bin", "_init.bin"):
+        assert (tmp_path / f"mine{suffix}").read_bytes() == \
+            (tmp_path / f"ref_lock{suffix}").read_bytes(), suffix
+    ref_kv, my_kv = B.read_namelist(nml), B.read_namelist(out)
+    skip = {"domain_file", "init_file", "n_repeat", "n_warmup", "omp_min_points",
+            "tridiag_kernel", "pcg_sync", "pcg_check_every"}
+    for key, raw in ref_kv.items():
+        if key in skip:
+            continue
+        assert key in my_kv, key
+        try:
+            same = B.fortran_double(raw) == B.fortran_double(my_kv[key])
+        except ValueError:
+            same = B.unquote(raw) == B.unquote(my_kv[key])
+        assert same, (key, raw, my_kv[key])
+
+
+@pytest.mark.gpu
+def test_reference_written_bundle_runs_to_the_reference_state(tmp_path, monkeypatch,
+                                                              gpu_available):
+    """Run the bundle the reference's writer produced: the final state is
+    the reference stepper's after its 4 steps on the same problem, bit for
+    bit, with the same solver iterations."""
+    nml = _ref_bundle_copy(tmp_path, monkeypatch)
+    m = B.run_bundle(nml, log=lambda *a: None)
+    exp = np.load(REF_BUNDLE / "expected.npz")
+    got = B.read_state_file(tmp_path / "ref_lock_state3d5.bin", 16, 16, 8, True)
+    for k in ("eta", "u", "v", "b", "T", "S"):
+        assert np.array_equal(got[k], exp[k]), k
+    assert m["solver_iterations"] == int(np.sum(exp["iters_per_step"]))
+    assert m["solver_failures"] == 0 and not m["diverged"]
+
+
+def test_oracle_runs_the_reference_bundle_to_the_reference_state(tmp_path, monkeypatch):
+    """The problem parsed from the reference-written bundle, stepped by the
+    oracle, gives the reference's state (pins the parser's reading of every
+    namelist key that reaches the step)."""
+    from oracle.stepper import OracleStepper, OState
+    nml = _ref_bundle_copy(tmp_path, monkeypatch)
+    b = B.parse_bundle(nml)
+    g = b.domain.grid
+    init = B.read_state_file(b.init_file, g.nx, g.ny, g.nz, True)
+    ost = OracleStepper(b.domain, b.physics, b.params, b.dt)
+    s = ost.integrate(OState(u=init["u"], v=init["v"], b=init["b"], eta=init["eta"],
+                             T=init["T"], S=init["S"]), b.n_steps)
+    exp = np.load(REF_BUNDLE / "expected.npz")
+    for k in ("eta", "u", "v", "b", "T", "S"):
+        assert np.array_equal(getattr(s, k), exp[k]), k
+    assert ost.solver_iterations == int(np.sum(exp["iters_per_step"]))

@@ -1,0 +1,94 @@
+"""The plain fp64 step on the benchmark basin (GRF partial-cell bathymetry
+over 500-4000 m, polyTEOS10, up3_tvd, TKE with the integral mixing length,
+theta 0.5 with two Picard passes, array wind stress tau_x(y)) against the
+oracle, under the reference's R2 gate (RULES.md:23-31: relative L2 < 1e-12
+per field per step from the same input state, solver iteration counts
+equal). The kernels keep the oracle's operation order, so the bar here is
+bit equality, with the 1e-12 gate reported beside it.
+
+Sizes: 64x64x40 and 128x128x40 (the oracle takes about half a second per
+step at the larger one), each with pcg_mg (the benchmark's solver) and the
+reference's pcg_jacobi; one case with a spatially varying background
+diffusivity kappa_b(x, y) = 1e-5 (1 + 0.2 xi)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FIELDS = ("u", "v", "b", "eta", "T", "S", "e")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def hoc(gpu_available):
+    from hydrostatic_ocean_core_benchmark_amd import stepper
+    return stepper
+
+
+def _kappa_field(nx, seed=5):
+    rng = np.random.default_rng(seed)
+    return 1.0e-5 * (1.0 + 0.2 * rng.uniform(-1.0, 1.0, (nx, nx)))
+
+
+def _gate(hoc, prob, n_steps):
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    from oracle.stepper import OracleStepper, OState
+    dom = prob.domain
+    g = dom.grid
+    ost = OracleStepper(dom, prob.physics, prob.params, prob.dt)
+    st = hoc.HipStepper(dom, prob.physics, prob.params, prob.dt)
+    b0 = ost.buoyancy_of(prob.initial_T, prob.initial_S) * dom.mask3
+    assert np.array_equal(_np(st.buoyancy_of(prob.initial_T, prob.initial_S)) * dom.mask3, b0)
+    zero = np.zeros(g.shape3d)
+    s0 = dict(u=zero, v=zero, b=b0, eta=np.zeros(g.shape), T=prob.initial_T, S=prob.initial_S)
+    os_ = OState(**s0, t=0.0)
+    for n in range(1, n_steps + 1):
+        # each step starts from the oracle's state, so every step is gated on
+        # its own (the R2 one-step gate), then the GPU state carries on too
+        it0, oit0 = st.solver_iterations, ost.solver_iterations
+        nu0, ka0 = ost.nu.copy(), ost.kappa.copy()
+        got = st.step(State3D(u=os_.u, v=os_.v, b=os_.b, eta=os_.eta, t=os_.t, T=os_.T,
+                              S=os_.S, e=os_.e))
+        os_ = ost.step(os_)
+        assert st.solver_iterations - it0 == ost.solver_iterations - oit0, n
+        assert st.solver_failures == 0 and ost.solver_failures == 0
+        worst = 0.0
+        for k in FIELDS:
+            w = getattr(os_, k)
+            gk = _np(getattr(got, k))
+            err = rel_l2(gk, w)
+            worst = max(worst, err)
+            assert err < 1e-12, (n, k, err)
+            assert np.array_equal(gk, w), (n, k, "not bitwise", err)
+        assert np.array_equal(_np(st.nu), ost.nu) and np.array_equal(_np(st.kappa), ost.kappa)
+        assert not np.array_equal(nu0, ost.nu) and not np.array_equal(ka0, ost.kappa)
+        print(f"{prob.meta['grid']} {prob.params.solver} step {n}: max rel L2 {worst:.1e}, "
+              f"{ost.solver_iterations - oit0} iterations", flush=True)
+
+
+@pytest.mark.parametrize("nx", [64, 128])
+@pytest.mark.parametrize("solver", ["pcg_mg", "pcg_jacobi"])
+def test_basin_step_gate(hoc, nx, solver):
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    prob = basin(nx=nx, nz=40, solver=solver)
+    print(f"basin {nx}x{nx}x40 {solver}", flush=True)
+    _gate(hoc, prob, 3)
+
+
+def test_basin_step_gate_array_kappa_b(hoc):
+    """kappa_b(x, y) enters the initial K_h, the TKE production and
+    K_h = (K_m - nu_b)/Pr + kappa_b in every column."""
+    from hydrostatic_ocean_core_benchmark_amd.cases import basin
+    prob = basin(nx=64, nz=40, kappa_b=_kappa_field(64))
+    print("basin 64x64x40 pcg_mg, array kappa_b", flush=True)
+    _gate(hoc, prob, 3)

@@ -107,3 +107,35 @@ def test_exhausted_budget_is_reported(ens):
     assert rep.solver_failures > 0
     with pytest.raises(ValueError):
         e.run(3)
+
+
+def test_graph_members_match_the_oracle(ens):
+    """A small uneven ensemble (2 members each at 32x32x30 and 48x48x30, GRF
+    basins from per-member seeds, pcg_mg) replayed from the graph: after one
+    replay (2 steps) every member equals the oracle stepped from the same
+    initial state, bit for bit, with the same solver iteration total."""
+    import numpy as np
+    from oracle.stepper import OracleStepper, OState
+    probs = ens.benchmark_members(sizes=(32, 48), per_size=2, nz=30, base_seed=101)
+    e = ens.Ensemble(probs)
+    e.calibrate(steps=2)
+    e.capture()
+    e.restart()
+    it0 = [m.stepper.solver_iterations for m in e.members]
+    e.replay(1)
+    torch.cuda.synchronize()
+    assert e.failures() == 0
+    for i, prob in enumerate(probs):
+        dom, g = prob.domain, prob.domain.grid
+        ost = OracleStepper(dom, prob.physics, prob.params, prob.dt)
+        b0 = ost.buoyancy_of(prob.initial_T, prob.initial_S) * dom.mask3
+        assert np.array_equal(e.members[i].initial["b"].cpu().numpy(), b0), i
+        zero = np.zeros(g.shape3d)
+        s = ost.integrate(OState(u=zero, v=zero, b=b0, eta=np.zeros(g.shape),
+                                 T=prob.initial_T, S=prob.initial_S), 2)
+        got = e.state(i)
+        for k in FIELDS:
+            assert np.array_equal(got[k].cpu().numpy(), getattr(s, k)), (i, k)
+        # the sync-free solves run a fixed budget of launches, but the
+        # iterations counted are the converged ones
+        assert e.members[i].stepper.solver_iterations - it0[i] == ost.solver_iterations, i

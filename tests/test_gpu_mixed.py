@@ -18,11 +18,12 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 # Per-field relative L2 budgets of the mixed steps after the steps of each
-# test, measured against fp64 (the 1024 x 1024 x 60 basin after 50 steps
-# gives eta 1.6e-6, u and v 1.1e-5, T 3.6e-8, S 4.6e-9, b 1.5e-8 for both
-# forms: the momentum fluxes' precision moves none of these in the first
-# three digits).
-BUDGET = {"eta": 1e-5, "u": 1e-4, "v": 1e-4, "T": 1e-6, "S": 1e-7, "b": 1e-6}
+# test, measured against fp64. SSH (eta) and T carry the stated target of
+# 1e-6 relative; u, v, S and b have no stated target and keep budgets from
+# measurement (the 1024 x 1024 x 60 basin after 50 steps gives eta 1.6e-6,
+# u and v 1.1e-5, T 3.6e-8, S 4.6e-9, b 1.5e-8 for both forms: the momentum
+# fluxes' precision moves none of these in the first three digits).
+BUDGET = {"eta": 1e-6, "u": 1e-4, "v": 1e-4, "T": 1e-6, "S": 1e-7, "b": 1e-6}
 MIXED = ("mixed", "mixed_fp32_momentum")
 
 

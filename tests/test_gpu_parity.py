@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from conftest import load_golden, rel_l2
-from golden_cases import COMPOSED, build_case, lock_exchange_problem
+from golden_cases import (COMPOSED, build_array_forcing_case, build_case,
+                          lock_exchange_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +99,46 @@ def test_tke_step_matches_reference(hoc, mxl):
     assert np.array_equal(_np(e), f[f"tke_{mxl}_e"])
     assert np.array_equal(_np(st.nu), f[f"tke_{mxl}_Km"])
     assert np.array_equal(_np(st.kappa), f[f"tke_{mxl}_Kh"])
+
+
+@pytest.mark.parametrize("mxl", ["integral", "recursive"])
+def test_tke_step_array_kappa_b_matches_reference(hoc, mxl):
+    """The closure with kappa_b on three values over the columns, against
+    the reference's own output."""
+    from hydrostatic_ocean_core_benchmark_amd.domain import CGrid, Domain
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    f = load_golden("step_theta_array_forcing.npz")
+    grid = CGrid(nx=24, ny=16, Lx=2.4e5, Ly=1.6e5, nz=8, depth=1000.0)
+    dom = Domain(grid=grid, H=f["geo_H"], mask=f["geo_mask"], bc_x="closed", bc_y="closed")
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=1000.0, nu=1e-4,
+                                     kappa=f["tke_kappa_b"]), closure="tke", mxl=mxl)
+    st = hoc.HipStepper(dom, phys, SchemeParams(name="fb"), 300.0, ustar2=f["tke_ustar2"])
+    st.nu.copy_(torch.as_tensor(f["tke_Km"], device="cuda"))
+    st.kappa.copy_(torch.as_tensor(f["tke_Kh"], device="cuda"))
+    e = st.tke_step(f["tke_e"], f["tke_u"], f["tke_v"], f["tke_b"])
+    assert np.array_equal(_np(e), f[f"tke_{mxl}_e"])
+    assert np.array_equal(_np(st.nu), f[f"tke_{mxl}_Km"])
+    assert np.array_equal(_np(st.kappa), f[f"tke_{mxl}_Kh"])
+
+
+def test_array_forcing_step_matches_reference(hoc):
+    """Array tau_x(y), tau_y(x) and kappa_b(x, y) (the basin's forcing)
+    through 3 steps: the reference's own states, K_m and K_h bit for bit,
+    iteration counts step by step."""
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    f = load_golden("step_theta_array_forcing.npz")
+    dom, phys, params, dt, steps, s0 = build_array_forcing_case(f)
+    st = hoc.HipStepper(dom, phys, params, dt)
+    assert np.array_equal(_np(st.kappa), np.broadcast_to(f["kappa_b"], st.kappa.shape))
+    s = State3D(**s0, t=0.0)
+    for n in range(1, steps + 1):
+        s = st.step(s)
+        for k in FIELDS:
+            got = _np(getattr(s, k))
+            assert np.array_equal(got, f[f"s{n}_{k}"]), (n, k, rel_l2(got, f[f"s{n}_{k}"]))
+        assert np.array_equal(_np(st.nu), f[f"s{n}_nu"])
+        assert np.array_equal(_np(st.kappa), f[f"s{n}_kappa"])
+        assert st.solver_iterations == int(f[f"s{n}_iters"]), n
 
 
 @pytest.mark.parametrize("limited", [False, True])
@@ -374,3 +415,10 @@ def test_step_rejects_an_output_that_aliases_the_input(hoc):
         t.zero_()
     with pytest.raises(ValueError, match="must not alias"):
         st._step_into(bufs, bufs)
+    other = st._new_state_buffers()
+    crossed = dict(other, u=bufs["v"])          # out.u is in.v
+    with pytest.raises(ValueError, match="must not alias"):
+        st._step_into(bufs, crossed)
+    twice = dict(other, v=other["u"])           # out.u and out.v one array
+    with pytest.raises(ValueError, match="must be distinct"):
+        st._step_into(bufs, twice)

@@ -112,6 +112,7 @@ PROTOTYPES = {
     "hoc_stepper_w_diag": ([VP, VP, VP, VP, VP], i32),
     "hoc_stepper_helmholtz_solve": ([VP, VP, VP, VP, VP, C.POINTER(i32), C.POINTER(i32), VP], i32),
     "hoc_thomas_batched": ([VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
+    "hoc_thomas_batched_noalloc": ([VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_eos_buoyancy": ([i32, f64, f64, f64, f64, f64, f64, VP, VP, VP, VP, i64, VP], i32),
     "hoc_thomas_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_stepper_tracer_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),

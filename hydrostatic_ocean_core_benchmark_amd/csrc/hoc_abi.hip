@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -395,6 +396,38 @@ int upload(T* dst, const T* src, long long n, const char* name) {
     int _rc = (expr);         \
     if (_rc != HOC_OK) return _rc; \
   } while (0)
+
+}  // namespace
+
+// Fixed device scratch of the no-allocation entry point: one copy per GPU,
+// held for the life of the process. In host code `scratch` names a host-side
+// stand-in, never the device address: host code uses only the per-device
+// pointer noalloc_scratch() looks up once per device.
+static __device__ __align__(256) double scratch[HOC_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call made with that device current and kept for later calls.
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(HOC_ERR_UNSUPPORTED, "no-alloc scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(HOC_ERR_DEVICE, "no-alloc scratch: null symbol address");
+    g_noalloc_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return HOC_OK;
+}
 
 // Fold the recorded event pairs into the running total.
 int prof_drain(hoc_stepper* h) {
@@ -2060,6 +2093,26 @@ int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
   k_thomas<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(ncol, (int)n, sub, diag, sup, rhs, x,
                                                           scratch);
   LAUNCH_CHECK("k_thomas");
+  return HOC_OK;
+}
+
+int hoc_thomas_batched_noalloc(const double* sub, const double* diag, const double* sup,
+                               const double* rhs, double* x, int64_t n, int64_t ncol,
+                               void* stream) {
+  if (!sub || !diag || !sup || !rhs || !x) return fail(HOC_ERR_INVALID, "null argument");
+  if (n < 1 || ncol < 1) return fail(HOC_ERR_INVALID, "n and ncol must be positive");
+  if (n > HOC_NOALLOC_SCRATCH_DOUBLES / ncol)
+    return fail(HOC_ERR_INVALID, "hoc_thomas_batched_noalloc: n * ncol = " +
+                                     std::to_string((long long)n * (long long)ncol) +
+                                     " exceeds the fixed scratch of " +
+                                     std::to_string((long long)HOC_NOALLOC_SCRATCH_DOUBLES) +
+                                     " doubles; use hoc_thomas_batched with a workspace");
+  double* cs = nullptr;
+  TRY(noalloc_scratch(&cs));
+  // k_thomas writes every c* it later reads, so the scratch needs no clearing
+  k_thomas<<<blocks(ncol), TPB, 0, (hipStream_t)stream>>>(ncol, (int)n, sub, diag, sup, rhs, x,
+                                                          cs);
+  LAUNCH_CHECK("k_thomas (no-alloc)");
   return HOC_OK;
 }
 

@@ -183,6 +183,20 @@ int hoc_thomas_batched(const double* sub, const double* diag, const double* sup,
                        const double* rhs, double* x, double* scratch, int64_t n, int64_t ncol,
                        void* stream);
 
+/* hoc_thomas_batched with no workspace argument and no device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). The c* sweep goes to a fixed device array of
+ * HOC_NOALLOC_SCRATCH_DOUBLES doubles (16 MiB), one per GPU, held until the
+ * process ends; n * ncol above that size is rejected (HOC_ERR_INVALID).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads) race. Same results as
+ * hoc_thomas_batched, bit for bit. */
+#define HOC_NOALLOC_SCRATCH_DOUBLES (1 << 21)
+int hoc_thomas_batched_noalloc(const double* sub, const double* diag, const double* sup,
+                               const double* rhs, double* x, int64_t n, int64_t ncol,
+                               void* stream);
+
 /* Adjoint of hoc_thomas_batched for x = A^-1 rhs (x from the forward solve):
  * lambda = A^-T g, rhs_bar = lambda, and the band gradients
  * d_sub[k] = -lambda[k] x[k-1], d_diag[k] = -lambda[k] x[k],

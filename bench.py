@@ -19,9 +19,10 @@ The JSON line also carries
                 the step's byte model, 72 B per layer cell) over its average
                 launch time (HIP events on the launch stream, inside the timed
                 region), against 8 TB/s; roofline_predictor the same for the
-                predictor (tied with it for the largest share of the step) with
-                the kernel's own compulsory bytes, 115 B per layer cell, over
-                10 further steps (no stage of the model is that kernel alone);
+                predictor (the next largest 3D kernel) with the bytes of its
+                stage of the model, stage B (56 B per layer cell; stage C also
+                holds back-substitution 1, a kernel of its own), over 10
+                further steps;
   step_model    the whole step against the stage model of the step's
                 compulsory traffic: 336 B per layer cell plus, per barotropic
                 solve, 64 B per column and B_it per column and iteration
@@ -43,9 +44,10 @@ The JSON line also carries
                 state, median of repeats) beside the H100 CUDA ms/step the
                 reference publishes for the same grid and scheme
                 (docs/32_hardware_ladder.md); context, not vs_baseline;
-  cpu_baseline  the NumPy oracle of the same step on one host core, timed on
-                a bounded sample (the 128 x 128 x 40 basin from the same
-                generator: 1 warm-up step, then 10 timed steps).
+  cpu_baseline  the NumPy oracle of the same step on one host core, on the
+                benchmark grid itself (512 x 512 x 40, same basin and initial
+                state), extrapolated from --cpu-baseline-steps steps (default 1,
+                about a minute).
 """
 
 from __future__ import annotations
@@ -68,24 +70,18 @@ HBM_PEAK_GBS = 8000.0
 # kernel: every array it must read or write touched once, fp64 = 8 B, masks
 # 1 B; interface arrays counted per layer cell (the (nz+1)/nz excess is
 # ignored). Stage figures follow the stage model of the step:
-#   k_tracers      T, S, u, v, w, K_h, dz3 read; T, S, b written     (72 B)
-#   k_closure      e, K_m, K_h, u, v, b, dz3 read; e, K_m, K_h written (80 B)
-#   k_predictor    u, v, the Picard iterate (u, v)_it, pgx, pgy, explicit
-#                  diffusion (2) and stress (2) terms, the Thomas factors sub,
-#                  den, c* read, masks m3, m3u, m3v (3 B); gh_u, gh_v written:
-#                  123 B, or 107 B in the first Picard pass (the iterate is the
-#                  state itself); 115 B per launch on average over the two
-#                  passes (the Coriolis partners of the state are formed from
-#                  u and v, whose neighbours are cached)
-#   k_momentum_explicit  u, v, b, w, K_m read; 6 explicit terms written (88 B)
-ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 115.0,
+#   k_tracers      stage E: T, S, u, v, K_h, dz3 read; T, S, b written  (72 B)
+#   k_closure      e, K_m, K_h, u, v, b, dz3 read; e, K_m, K_h written    (80 B)
+#   k_predictor    stage B: u, v, b, K_m, dz3 read; gh_u, gh_v written   (56 B)
+#   k_momentum_explicit  u, v, b, w, K_m read; 6 explicit terms written  (88 B)
+ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 56.0,
                       "k_momentum_explicit": 88.0}
 # Measured HBM bytes per launch at 512 x 512 x 40 (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
 # the summary is profiles/basin512x512x40_hbm_traffic_pmc.md). Re-measured
 # whenever the kernel changes.
-MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.899e9, "k_momentum_explicit": 1.397e9,
-                           "k_predictor": 2.088e9, "k_closure": 1.402e9}
+MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.305e9, "k_momentum_explicit": 1.071e9,
+                           "k_predictor": 1.828e9, "k_closure": 1.195e9}
 STEP_BYTES_PER_CELL = 336.0
 SOLVE_BYTES_PER_COLUMN = 64.0
 ITER_BYTES_PER_COLUMN = {"pcg_mg": 520.0, "pcg_jacobi": 120.0, "pcg_rbgs": 520.0,
@@ -120,6 +116,9 @@ def parse():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="step precision timed for the headline value (fp64: the plain step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-steps", type=int, default=1,
+                    help="oracle steps timed on the benchmark grid (about a minute each at "
+                         "512 x 512 x 40 on one host core)")
     ap.add_argument("--ladder-only", action="store_true",
                     help="time only the ladder workloads and print {'ladder': ...} (profiling)")
     ap.add_argument("--ladder", default=",".join(LADDER_DEFAULT),
@@ -139,31 +138,36 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(nz: int, solver: str) -> dict:
-    """The oracle on one core: 128 x 128 x nz basin, 1 warm-up + 10 timed steps."""
+def cpu_baseline(nx: int, nz: int, dx: float, dt: float, steps: int) -> dict:
+    """The oracle on one core on the benchmark's own grid: the same basin
+    (same generator, seed and initial state) and the same step, `steps` steps
+    from the initial state after set-up. NumPy's element-wise kernels are
+    single-threaded and nothing on the path calls BLAS, so one core does
+    the work; the rate is cells * steps / wall, labelled as taken from that
+    many steps."""
     import numpy as np
 
     from hydrostatic_ocean_core_benchmark_amd.cases import basin
     from oracle.stepper import OracleStepper, OState
-    p = basin(nx=128, nz=nz, solver=solver)
+    p = basin(nx=nx, nz=nz, dx=dx, dt=dt)
     st = OracleStepper(p.domain, p.physics, p.params, p.dt)
     g = p.domain.grid
     b = st.buoyancy_of(p.initial_T, p.initial_S) * p.domain.mask3
     s = OState(u=np.zeros(g.shape3d), v=np.zeros(g.shape3d), b=b, eta=np.zeros(g.shape),
                T=p.initial_T, S=p.initial_S)
-    s = st.step(s)
-    n = 10
     t0 = time.perf_counter()
-    for _ in range(n):
+    for _ in range(steps):
         s = st.step(s)
     wall = time.perf_counter() - t0
     cells = g.nx * g.ny * g.nz
-    return {"value": cells * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"NumPy oracle, 128x128x{nz} basin from the same generator ({solver}), "
-                      f"{n} steps "
-                      f"after 1 warm-up step ({wall:.1f} s; {platform.processor() or 'cpu'}, "
-                      f"numpy {np.__version__}, OMP_NUM_THREADS="
-                      f"{os.environ.get('OMP_NUM_THREADS', 'unset')})"}
+    return {"value": cells * steps / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy oracle of the same step on the benchmark grid itself "
+                      f"({g.nx}x{g.ny}x{g.nz} basin, same generator and initial state, "
+                      f"{p.params.solver}): extrapolated from {steps} step(s) from the initial "
+                      f"state, {wall:.1f} s; host {platform.processor() or 'cpu'}, "
+                      f"nproc {os.cpu_count()}, numpy {np.__version__}, OMP_NUM_THREADS="
+                      f"{os.environ.get('OMP_NUM_THREADS', 'unset')}, OPENBLAS_NUM_THREADS="
+                      f"{os.environ.get('OPENBLAS_NUM_THREADS', 'unset')}"}
 
 
 def _initial_fields(prob, st, torch):
@@ -488,7 +492,8 @@ def main() -> int:
         names = list(LADDER) if args.ladder == "all" else args.ladder.split(",")
         rec["ladder"] = reference_ladder(names)
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(args.nz, prob.params.solver)
+        rec["cpu_baseline"] = cpu_baseline(args.nx, args.nz, args.dx, args.dt,
+                                           args.cpu_baseline_steps)
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -141,10 +141,13 @@ struct hoc_stepper {
   double *tau_x = nullptr, *tau_y = nullptr, *tau_u = nullptr, *tau_v = nullptr;
   double *ustar2 = nullptr, *kappa_b = nullptr, *Hd = nullptr;
   double* dz3_ref = nullptr;  // resting thickness (moving coordinates)
+  uint8_t* cg_nw = nullptr;   // z-level column form: wet levels per column
+  double *cg_hb = nullptr, *cg_zs = nullptr;  // deepest wet thickness; zs[0..nz]
   // mixing (caller-owned)
   double *Km = nullptr, *Kh = nullptr;
   // coefficients
-  double *sub_m, *den_m, *cst_m, *sub_b, *den_b, *cst_b, *q, *Ku, *Kv, *inv_diag;
+  double *sub_m, *den_m, *cst_m, *q, *Ku, *Kv, *inv_diag;
+  double* cst_b;  // c* of tri_b, parked by the tracer kernel between its sweeps
   // step work arrays
   double *phi, *w, *pgx, *pgy, *avu, *avv, *dzu, *dzv, *stu, *stv;
   double *U0, *V0, *div0, *Ug, *Vg, *rhs;
@@ -429,6 +432,80 @@ int noalloc_scratch(double** out) {
   return HOC_OK;
 }
 
+// Column form of a static z-level geometry (see cg_dz in hoc_device.h): per
+// column the number of wet levels nw and the thickness hb of the deepest
+// wet cell, plus the uniform reference thickness dzr and the sequential
+// sums zs[k] of k reference thicknesses. Returns false when any stored
+// thickness, face thickness, mask or centre depth differs in a single bit
+// from what the column form gives (the kernels then read the full arrays).
+bool bits_equal(double a, double b) {
+  uint64_t x, y;
+  std::memcpy(&x, &a, 8);
+  std::memcpy(&y, &b, 8);
+  return x == y;
+}
+
+double face_rule_host(int rule, double a, double b) {
+  if (rule == 0) return (a <= b || a != a) ? a : b;
+  if (rule == 1) return 0.5 * (a + b);
+  const double s = a + b;
+  return s > 0.0 ? 2.0 * a * b / s : 0.0;
+}
+
+bool column_form(const Dims& d, int face_rule, const hoc_geometry* g, std::vector<uint8_t>& nw,
+                 std::vector<double>& hb, std::vector<double>& zs, double& dzr) {
+  const int nz = d.nz;
+  const long long n2 = d.n2;
+  if (nz > 255 || !g->mask3 || !g->dz3 || !g->dz3u || !g->dz3v || !g->z_centre || !g->mask3u ||
+      !g->mask3v || !g->mask_u || !g->mask_v)
+    return false;
+  nw.assign(n2, 0);
+  hb.assign(n2, 0.0);
+  dzr = 0.0;
+  for (long long c = 0; c < n2; ++c) {
+    int n = 0;
+    while (n < nz && g->mask3[(long long)n * n2 + c]) ++n;
+    for (int k = n; k < nz; ++k)
+      if (g->mask3[(long long)k * n2 + c]) return false;  // not a wet prefix
+    nw[c] = (uint8_t)n;
+    hb[c] = n > 0 ? g->dz3[(long long)(n - 1) * n2 + c] : 0.0;
+    if (n >= 2) {
+      const double top = g->dz3[c];
+      if (dzr == 0.0) dzr = top;
+      else if (!bits_equal(dzr, top)) return false;
+    }
+  }
+  if (dzr == 0.0) {  // no column with a full level: any value reproduces the arrays
+    for (long long c = 0; c < n2; ++c) dzr = std::max(dzr, hb[c]);
+  }
+  zs.assign(nz + 1, 0.0);
+  for (int k = 1; k <= nz; ++k) zs[k] = zs[k - 1] + dzr;
+  auto dz = [&](long long c, int k) {
+    const int n = nw[c];
+    return k < n - 1 ? dzr : (k == n - 1 ? hb[c] : 0.0);
+  };
+  for (int j = 0; j < d.ny; ++j)
+    for (int i = 0; i < d.nx; ++i) {
+      const long long c = (long long)j * d.nx + i;
+      const long long ce = (long long)j * d.nx + (i + 1 == d.nx ? 0 : i + 1);
+      const long long cn = (long long)(j + 1 == d.ny ? 0 : j + 1) * d.nx + i;
+      const int n = nw[c];
+      for (int k = 0; k < nz; ++k) {
+        const long long t = (long long)k * n2 + c;
+        if (!bits_equal(g->dz3[t], dz(c, k))) return false;
+        const double zc = k < n ? zs[k] + 0.5 * dz(c, k) : (n > 0 ? zs[n - 1] + hb[c] : 0.0);
+        if (!bits_equal(g->z_centre[t], zc)) return false;
+        const bool wu = g->mask_u[c] && k < n && k < nw[ce];
+        const bool wv = g->mask_v[c] && k < n && k < nw[cn];
+        if ((g->mask3u[t] != 0) != wu || (g->mask3v[t] != 0) != wv) return false;
+        const double fu = wu ? face_rule_host(face_rule, dz(c, k), dz(ce, k)) : 0.0;
+        const double fv = wv ? face_rule_host(face_rule, dz(c, k), dz(cn, k)) : 0.0;
+        if (!bits_equal(g->dz3u[t], fu) || !bits_equal(g->dz3v[t], fv)) return false;
+      }
+    }
+  return true;
+}
+
 // Fold the recorded event pairs into the running total.
 int prof_drain(hoc_stepper* h) {
   for (int i = 0; i < h->prof_used; ++i) {
@@ -464,12 +541,11 @@ int build_coefficients(hoc_stepper* h, hipStream_t s) {
   if (!h->Km || !h->Kh) return fail(HOC_ERR_INVALID, "bind K_m / K_h before building");
   const Dims d = h->d;
   { ProfScope _ps(h, "k_coefficients", s);
-  k_coefficients<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->Km, h->Kh, h->sub_m,
-                                              h->den_m, h->cst_m, h->sub_b, h->den_b,
-                                              h->cst_b, h->q); }
+  k_coefficients<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->Km, h->sub_m, h->den_m,
+                                              h->cst_m, h->q); }
   LAUNCH_CHECK("k_coefficients");
   { ProfScope _ps(h, "k_kukv", s);
-  k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, h->q, h->Ku, h->Kv); }
+  k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->q, h->Ku, h->Kv); }
   LAUNCH_CHECK("k_kukv");
   h->tridiagonal_solves += 1;
   if (h->cfg.scheme == 0) {
@@ -1475,8 +1551,6 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   A(h->sub_m, d.n3);
   A(h->den_m, d.n3);
   A(h->cst_m, d.n3);
-  A(h->sub_b, d.n3);
-  A(h->den_b, d.n3);
   A(h->cst_b, d.n3);
   A(h->q, d.n3);
   A(h->Ku, d.n2);
@@ -1727,6 +1801,27 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   geo.ustar2 = h->ustar2;
   geo.kappa_b = h->kappa_b;
   geo.H = h->Hd;
+  if (cfg->vcoord == 0) {
+    // static z-level geometry: read in column form wherever it reproduces the
+    // stored arrays bit for bit
+    std::vector<uint8_t> nw;
+    std::vector<double> hb, zs;
+    double dzr = 0.0;
+    if (column_form(d, cfg->face_rule, g, nw, hb, zs, dzr)) {
+      int crc = h->alloc(&h->cg_nw, d.n2);
+      crc = crc ? crc : h->alloc(&h->cg_hb, d.n2);
+      crc = crc ? crc : h->alloc(&h->cg_zs, d.nz + 1);
+      crc = crc ? crc : upload(h->cg_nw, nw.data(), d.n2, "nw");
+      crc = crc ? crc : upload(h->cg_hb, hb.data(), d.n2, "hb");
+      crc = crc ? crc : upload(h->cg_zs, zs.data(), d.nz + 1, "zs");
+      if (crc != HOC_OK) return cleanup(crc);
+      geo.nw = h->cg_nw;
+      geo.hb = h->cg_hb;
+      geo.zs = h->cg_zs;
+      h->p.zcol = 1;
+      h->p.dzr = dzr;
+    }
+  }
   if (cfg->scheme == 2) {
     k_face_depth<<<blocks(d.n2), TPB>>>(d, geo, h->Hu, h->Hv);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
@@ -1760,7 +1855,7 @@ int hoc_stepper_tke_step(hoc_stepper* h, const double* e_in, const double* u, co
   hipStream_t s = (hipStream_t)stream;
   const Dims d = h->d;
   if (h->p.mxl_recursive) {
-    k_closure_shear<<<blocks(d.ni), TPB, 0, s>>>(d, h->geo, u, v, b, h->s_n2, h->s_sh);
+    k_closure_shear<<<blocks(d.ni), TPB, 0, s>>>(d, h->p, h->geo, u, v, b, h->s_n2, h->s_sh);
     k_mixing_recursive<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e_in, h->s_n2, h->s_l,
                                                      h->s_le);
   } else {
@@ -1893,7 +1988,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
                                  h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
   const bool split = h->cfg.scheme == 2;
-  if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, in->u, in->v, h->U0, h->V0);
+  if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->U0, h->V0);
   if (theta) k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, in->eta, h->div0);
   const double* u_it = in->u;
   const double* v_it = in->v;
@@ -1901,9 +1996,16 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     double* gu = h->gh_u[it & 1];
     double* gv = h->gh_v[it & 1];
     { ProfScope _ps(h, "k_predictor", s);
-  k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
-                                             h->pgx, h->pgy, h->dzu, h->dzv, h->stu, h->stv,
-                                             h->sub_m, h->den_m, h->cst_m, gu, gv, h->Ug, h->Vg); }
+    if (p.zcol)
+      k_predictor_col<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
+                                                   h->pgx, h->pgy, h->dzu, h->dzv, h->stu,
+                                                   h->stv, h->sub_m, h->den_m, h->cst_m, gu, gv,
+                                                   h->Ug, h->Vg);
+    else
+      k_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
+                                               h->pgx, h->pgy, h->dzu, h->dzv, h->stu, h->stv,
+                                               h->sub_m, h->den_m, h->cst_m, gu, gv, h->Ug,
+                                               h->Vg); }
     LAUNCH_CHECK("k_predictor");
     h->tridiagonal_solves += 2;
     if (theta) {
@@ -1926,7 +2028,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   } else if (!theta) {
     HIP_TRY(hipMemcpyAsync(out->u, u_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(out->v, v_it, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-    k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, geo, out->u, out->v, h->U0, h->V0);
+    k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->U0, h->V0);
     k_eta_fb<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->U0, h->V0, out->eta);
     LAUNCH_CHECK("k_eta_fb");
   }
@@ -1934,7 +2036,7 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
   { ProfScope _ps(h, "k_tracers", s);
   launch_tracers_tiled(h->cfg.precision >= 1, s, d, p, geo, in->T, in->S, in->b, out->u, out->v,
-                       h->w, h->Kh, h->sub_b, h->den_b, h->cst_b, out->T, out->S, out->b); }
+                       h->w, h->Kh, h->cst_b, out->T, out->S, out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;

@@ -51,6 +51,118 @@ __device__ __forceinline__ double safe_inv(double x) {
 }
 __device__ __forceinline__ double mk(uint8_t m) { return m ? 1.0 : 0.0; }
 
+// Face thickness rule of the domain (libs/core/domain.py:98-110): 0 min,
+// 1 mean, 2 harmonic.
+__device__ __forceinline__ double face_value(int rule, double a, double b) {
+  if (rule == 0) return nmin(a, b);
+  if (rule == 1) return 0.5 * (a + b);
+  const double s = a + b;
+  return s > 0.0 ? 2.0 * a * b / s : 0.0;
+}
+
+// ------------------------------------- static z-level geometry by column
+// A z-level column is fixed by its number of wet levels nw and the
+// thickness hb of its deepest wet cell: levels fill from the surface with
+// the uniform reference thickness dzr, the deepest wet level may be partial
+// and every level below it is dry (libs/core/domain.py:114-131). The host
+// derives (nw, hb) from the full arrays and checks that every thickness,
+// face thickness, mask and centre depth below equals the stored value bit
+// for bit before the kernels use this form; the kernels then read one byte
+// and one double per column instead of five fp64 fields and three masks per
+// cell. zs[k] is the sequential sum of k reference thicknesses (np.cumsum's
+// order).
+__device__ __forceinline__ double cg_dz(double dzr, int nw, double hb, int k) {
+  return k < nw - 1 ? dzr : (k == nw - 1 ? hb : 0.0);
+}
+// Depth of interface k = 0..nz below the surface: the thicknesses above it
+// summed from the top in order (exact zeros add nothing below the bottom).
+__device__ __forceinline__ double cg_above(const double* zs, int nw, double hb, int k) {
+  if (k < nw) return zs[k];
+  return nw > 0 ? zs[nw - 1] + hb : 0.0;
+}
+// Depth of the centre of level k below the surface: the thickness above it
+// summed from the top, plus half its own (libs/core/domain.py:141-145).
+__device__ __forceinline__ double cg_zc(const double* zs, double dzr, int nw, double hb, int k) {
+  if (k < nw) return zs[k] + 0.5 * cg_dz(dzr, nw, hb, k);
+  return nw > 0 ? zs[nw - 1] + hb : 0.0;
+}
+// Centre spacing on interface k = 0..nz (libs/core/vertical_var.py:21-32).
+__device__ __forceinline__ double cg_dzi(double dzr, int nw, double hb, int k, int nz) {
+  if (k == 0) return cg_dz(dzr, nw, hb, 0);
+  if (k == nz) return cg_dz(dzr, nw, hb, nz - 1);
+  return 0.5 * (cg_dz(dzr, nw, hb, k - 1) + cg_dz(dzr, nw, hb, k));
+}
+// Thickness of the level-k face between a column (nw0, hb0) and its east or
+// north neighbour (nw1, hb1): the face rule where the face is wet (both
+// cells wet and the 2D face mask open), else 0 (mask3u * face, domain.py:138).
+__device__ __forceinline__ double cg_face(int rule, double dzr, bool open, int nw0, double hb0,
+                                          int nw1, double hb1, int k) {
+  if (!(open && k < nw0 && k < nw1)) return 0.0;
+  return face_value(rule, cg_dz(dzr, nw0, hb0, k), cg_dz(dzr, nw1, hb1, k));
+}
+
+// Vertical geometry of one column, from the stored arrays or, when the
+// geometry is in column form (StepParams::zcol, a uniform branch), from the
+// column's (nw, hb): the same bits either way.
+struct ColumnGeo {
+  const Geo& g;
+  long long c, n2;
+  int nz, nw;
+  bool col;
+  double hb, dzr;
+  __device__ __forceinline__ ColumnGeo(const Geo& geo, const StepParams& p, const Dims& d,
+                                       long long col_index)
+      : g(geo), c(col_index), n2(d.n2), nz(d.nz), nw(0), col(p.zcol != 0), hb(0.0),
+        dzr(p.dzr) {
+    if (col) {
+      nw = geo.nw[c];
+      hb = geo.hb[c];
+    }
+  }
+  __device__ __forceinline__ double dz(int k) const {
+    return col ? cg_dz(dzr, nw, hb, k) : g.dz3[(long long)k * n2 + c];
+  }
+  __device__ __forceinline__ double dzi(int k) const {  // k = 0..nz
+    return col ? cg_dzi(dzr, nw, hb, k, nz) : g.dzi[(long long)k * n2 + c];
+  }
+  __device__ __forceinline__ bool wet(int k) const {  // k = 0..nz-1
+    return col ? k < nw : g.m3[(long long)k * n2 + c] != 0;
+  }
+  __device__ __forceinline__ double zc(int k) const {
+    return col ? cg_zc(g.zs, dzr, nw, hb, k) : g.zc[(long long)k * n2 + c];
+  }
+};
+
+// Face thickness and wet flag of the east (axis 0) or north (axis 1) face
+// of one column at any level, from the stored arrays or the column form.
+struct FaceGeo {
+  const double* dzf;
+  const uint8_t* mf;
+  long long c, n2;
+  bool col, open;
+  int rule, nw0, nw1;
+  double hb0, hb1, dzr;
+  __device__ __forceinline__ FaceGeo(const Geo& geo, const StepParams& p, const Dims& d,
+                                     long long c0, long long c1, int axis)
+      : dzf(axis == 0 ? geo.dz3u : geo.dz3v), mf(axis == 0 ? geo.m3u : geo.m3v), c(c0),
+        n2(d.n2), col(p.zcol != 0), open(false), rule(p.face_rule), nw0(0), nw1(0), hb0(0.0),
+        hb1(0.0), dzr(p.dzr) {
+    if (col) {
+      open = (axis == 0 ? geo.mu2[c0] : geo.mv2[c0]) != 0;
+      nw0 = geo.nw[c0];
+      nw1 = geo.nw[c1];
+      hb0 = geo.hb[c0];
+      hb1 = geo.hb[c1];
+    }
+  }
+  __device__ __forceinline__ double dz(int k) const {
+    return col ? cg_face(rule, dzr, open, nw0, hb0, nw1, hb1, k) : dzf[(long long)k * n2 + c];
+  }
+  __device__ __forceinline__ bool wet(int k) const {
+    return col ? (open && k < nw0 && k < nw1) : mf[(long long)k * n2 + c] != 0;
+  }
+};
+
 // Tile of this workgroup in a 2D grid, remapped so that each of the 8 XCDs
 // (workgroups are dispatched to them round-robin by linear id) owns one
 // contiguous band of tile rows: neighbouring tiles then share their halo
@@ -152,6 +264,38 @@ __device__ __forceinline__ double explicit_diffusion_w(int k, int nz, double f0,
   const double out = (ft - fb) * inv;
   const double dr = drag * f1 * (bot_k ? 1.0 : 0.0);
   return (out - dr * inv) * (m1 ? 1.0 : 0.0);
+}
+
+// Row k of one implicit vertical-diffusion matrix (I - theta_v dt D[nu]) of
+// a column, factored for the Thomas sweep: sub, den = diag - sub c*_{k-1}
+// and c* = sup / den (libs/core/vertical_var.py:35-77). inv_dz, idzi_t and
+// idzi_b are 1/dz3 and 1/dzi above and below the cell.
+__device__ __forceinline__ void factor_row(const StepParams& p, int k, double inv_dz,
+                                           double nu_t, double nu_b, double idzi_t,
+                                           double idzi_b, bool wet, bool below,
+                                           double drag_term, double c_prev, double& sub_o,
+                                           double& den_o, double& cst_o) {
+  const double fac = p.thv_dt * inv_dz;
+  const double a_top = fac * nu_t * idzi_t;
+  const double a_bot = fac * nu_b * idzi_b;
+  double sub = -a_top, sup = -a_bot;
+  double diag = 1.0 + a_top + a_bot;
+  if (k == 0) {
+    diag = 1.0 + a_bot;
+    sub = 0.0;
+  }
+  if (wet && !below) {
+    diag = 1.0 + a_top + drag_term * inv_dz;
+    sup = 0.0;
+  }
+  if (!wet) {
+    diag = 1.0;
+    sub = 0.0;
+    sup = 0.0;
+  }
+  den_o = (k == 0) ? diag : diag - sub * c_prev;
+  cst_o = sup / den_o;
+  sub_o = sub;
 }
 
 // ------------------------------------------------------------------ EOS

@@ -45,7 +45,7 @@ __global__ void k_geometry(Dims d, StepParams p, const double* dz3, const uint8_
 __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* e_in, double* Km,
                           double* Kh, double* e_out, const double* s_n2, const double* s_sh,
                           const double* s_l, const double* s_le, double* s_c);
-__global__ void k_closure_shear(Dims d, Geo geo, const double* u, const double* v,
+__global__ void k_closure_shear(Dims d, StepParams p, Geo geo, const double* u, const double* v,
                                 const double* b, double* s_n2, double* s_sh);
 __global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* e_in,
                                    const double* s_n2, double* s_l, double* s_le);
@@ -57,9 +57,8 @@ void launch_closure_lengths(const Dims& d, const StepParams& p, const Geo& geo,
 // its staging does not fit one workgroup's LDS.
 int closure_lengths_prepare(int nz);
 __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* Km,
-                               const double* Kh, double* sub_m, double* den_m, double* cst_m,
-                               double* sub_b, double* den_b, double* cst_b, double* q);
-__global__ void k_kukv(Dims d, Geo geo, const double* q, double* Ku, double* Kv);
+                               double* sub_m, double* den_m, double* cst_m, double* q);
+__global__ void k_kukv(Dims d, StepParams p, Geo geo, const double* q, double* Ku, double* Kv);
 __global__ void k_phi(Dims d, StepParams p, Geo geo, const double* b, double* phi);
 __global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* u, const double* v,
                          double* w);
@@ -68,7 +67,7 @@ __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double*
                                     const double* w, const double* Km, double* pgx,
                                     double* pgy, double* avu, double* avv, double* dzu,
                                     double* dzv, double* stu, double* stv);
-__global__ void k_depth_sum(Dims d, Geo geo, const double* u, const double* v, double* U,
+__global__ void k_depth_sum(Dims d, StepParams p, Geo geo, const double* u, const double* v, double* U,
                             double* V);
 __global__ void k_div2(Dims d, StepParams p, Geo geo, const double* U, const double* V,
                        const double* eta, double* out);
@@ -103,13 +102,15 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* u, cons
                             const double* sub,
                             const double* den, const double* cst, double* gh_u, double* gh_v,
                             double* Ug, double* Vg);
+__global__ void k_predictor_col(Dims d, StepParams p, Geo geo, const double* u, const double* v,
+                                const double* eta, const double* u_it, const double* v_it,
+                                const double* pgx, const double* pgy, const double* dzu,
+                                const double* dzv, const double* stu, const double* stv,
+                                const double* sub, const double* den, const double* cst,
+                                double* gh_u, double* gh_v, double* Ug, double* Vg);
 __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* gh_u,
                           const double* gh_v, const double* q, const double* eta_n,
                           double* u_it, double* v_it);
-__global__ void k_tracers(Dims d, StepParams p, Geo geo, const double* T, const double* S,
-                          const double* b, const double* u, const double* v, const double* w,
-                          const double* Kh, const double* sub, const double* den,
-                          const double* cst, double* T_out, double* S_out, double* b_out);
 __global__ void k_thomas(long long ncol, int n, const double* sub, const double* diag,
                          const double* sup, const double* rhs, double* x, double* cs);
 __global__ void k_eos(long long n, StepParams p, const double* T, const double* S,
@@ -125,9 +126,8 @@ __global__ void k_momentum_advection(Dims d, StepParams p, Geo geo, const double
 // selects the fp32 advective fluxes (and EOS), else the fp64 plain step
 void launch_tracers_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
                           const double* T, const double* S, const double* b, const double* u,
-                          const double* v, const double* w, const double* Kh, const double* sub,
-                          const double* den, const double* cst, double* T_out, double* S_out,
-                          double* b_out);
+                          const double* v, const double* w, const double* Kh, double* cst,
+                          double* T_out, double* S_out, double* b_out);
 void launch_momentum_explicit_tiled(bool mixed, hipStream_t s, Dims d, StepParams p, Geo geo,
                                     const double* u, const double* v, const double* b,
                                     const double* phi, const double* w, const double* Km,

@@ -26,6 +26,8 @@ struct StepParams {
   int n_split, bt_mean;  // split_explicit: barotropic substeps, averaged transports
   int open_x, open_y;    // Flather open edges
   int vcoord, face_rule; // vertical coordinate (0 z-level, 1 z*, 2 sigma), face rule
+  int zcol;              // 1: the static z-level geometry is read in column form (Geo nw/hb/zs)
+  double dzr;            // uniform reference layer thickness of the column form
   double dx, dy, dz_nom, dt;
   double theta, theta_cor, theta_v;
   double g, f, rho0, nu_b, bottom_drag, N2;
@@ -64,6 +66,12 @@ struct Geo {
   const double *tau_u, *tau_v;          // (tau/rho0) * mask3[u|v][0]   [ny,nx]
   const double *ustar2, *kappa_b;       // surface u*^2, background kappa [ny,nx]
   const double *H;                      // depth [ny,nx] (open edges only, else null)
+  // z-level column form (when StepParams::zcol): wet levels per column, the
+  // thickness of each column's deepest wet cell, and zs[k] = the sequential
+  // sum of k reference thicknesses, k = 0..nz
+  const uint8_t *nw;                    // [ny,nx]
+  const double *hb;                     // [ny,nx]
+  const double *zs;                     // [nz+1]
 };
 
 }  // namespace hoc

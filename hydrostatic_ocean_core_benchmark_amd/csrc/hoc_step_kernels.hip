@@ -63,13 +63,6 @@ __global__ void k_moving_dz(Dims d, const uint8_t* __restrict__ m2, const double
   dzi[nz * n2 + c] = prev;
 }
 
-__device__ __forceinline__ double face_value(int rule, double a, double b) {
-  if (rule == 0) return nmin(a, b);
-  if (rule == 1) return 0.5 * (a + b);
-  const double s = a + b;
-  return s > 0.0 ? 2.0 * a * b / s : 0.0;
-}
-
 __global__ void k_moving_faces(Dims d, StepParams p, const uint8_t* __restrict__ m3u,
                                const uint8_t* __restrict__ m3v, const double* __restrict__ dz3,
                                double* __restrict__ dz3u, double* __restrict__ dz3v) {
@@ -93,12 +86,13 @@ __global__ void k_moving_faces(Dims d, StepParams p, const uint8_t* __restrict__
 //     s_c, d* in e_out), then back-substitution, floors, K_m and K_h.
 
 // N^2 and shear^2 on interface k of column c (closure.py:43-62,150-156).
-__device__ __forceinline__ void shear_n2(const Dims& d, const Geo& geo, const double* u,
-                                         const double* v, const double* b, long long c, int k,
-                                         double& n2k, double& sh) {
+__device__ __forceinline__ void shear_n2(const Dims& d, const StepParams& p, const Geo& geo,
+                                         const double* u, const double* v, const double* b,
+                                         long long c, int k, double& n2k, double& sh) {
   n2k = 0.0;
   sh = 0.0;
   if (k < 1 || k >= d.nz) return;
+  const ColumnGeo cg(geo, p, d, c);
   const long long n2 = d.n2;
   const int i = (int)(c % d.nx), j = (int)(c / d.nx);
   const long long cw = at2(d, j, wrap(i, -1, d.nx));
@@ -108,8 +102,8 @@ __device__ __forceinline__ void shear_n2(const Dims& d, const Geo& geo, const do
   const double vc_prev = 0.5 * (v[a + c] + v[a + cs]);
   const double uc = 0.5 * (u[o + c] + u[o + cw]);
   const double vc = 0.5 * (v[o + c] + v[o + cs]);
-  const double wet = mk(geo.m3[a + c]) * mk(geo.m3[o + c]);
-  const double idz = safe_inv(geo.dzi[o + c]);
+  const double wet = (cg.wet(k - 1) ? 1.0 : 0.0) * (cg.wet(k) ? 1.0 : 0.0);
+  const double idz = safe_inv(cg.dzi(k));
   const double du = (uc_prev - uc) * idz;
   const double dv = (vc_prev - vc) * idz;
   const double db = (b[a + c] - b[o + c]) * idz;
@@ -144,8 +138,8 @@ __global__ __launch_bounds__(MX_THREADS) void k_closure_lengths(
     const long long c = c0 + col;
     double n2k = 0.0, sh = 0.0, dz = 0.0;
     if (c < n2) {
-      shear_n2(d, geo, u, v, b, c, kk, n2k, sh);
-      dz = geo.dzi[(long long)kk * n2 + c];
+      shear_n2(d, p, geo, u, v, b, c, kk, n2k, sh);
+      dz = ColumnGeo(geo, p, d, c).dzi(kk);
       s_n2[(long long)kk * n2 + c] = n2k;
       s_sh[(long long)kk * n2 + c] = sh;
     }
@@ -171,15 +165,25 @@ __global__ __launch_bounds__(MX_THREADS) void k_closure_lengths(
   const long long c = c0 + col;
   if (c >= n2) return;
   const int lo = kbeg[col], hi = kend[col];
-  double h = 0.0;
-  for (int k = 0; k < nz; ++k) h = h + geo.dz3[(long long)k * n2 + c];
-  double zi = 0.0;  // depth of interface k: dz3[0] + ... + dz3[k-1], in that order
+  const ColumnGeo cg(geo, p, d, c);
+  // column depth and the depth of interface k: dz3[0] + ... + dz3[k-1], in
+  // that order
+  double h = 0.0, zi = 0.0;
+  if (cg.col) {
+    h = cg_above(geo.zs, cg.nw, cg.hb, nz);
+  } else {
+    for (int k = 0; k < nz; ++k) h = h + geo.dz3[(long long)k * n2 + c];
+  }
   int kz = 0;
   for (int k = k0; k <= nz; k += MX_TPC) {
-    for (; kz < k; ++kz) zi = zi + geo.dz3[(long long)kz * n2 + c];
+    if (cg.col) {
+      zi = cg_above(geo.zs, cg.nw, cg.hb, k);
+    } else {
+      for (; kz < k; ++kz) zi = zi + geo.dz3[(long long)kz * n2 + c];
+    }
     const long long o = (long long)k * n2 + c;
     const double wet =
-        (k >= 1 && k < nz) ? mk(geo.m3[o - n2]) * mk(geo.m3[o]) : 0.0;
+        (k >= 1 && k < nz) ? (cg.wet(k - 1) ? 1.0 : 0.0) * (cg.wet(k) ? 1.0 : 0.0) : 0.0;
     const double budget0 = nmax(e_in[o], p.e_min);
     // upward: interface k crosses k-1, ..., 0
     double l = 0.0, bud = budget0;
@@ -254,7 +258,7 @@ int closure_lengths_prepare(int nz) {
 
 // N^2 and shear^2 alone, one lane per interface cell (the recursive form
 // needs N^2 before its length recursion).
-__global__ void k_closure_shear(Dims d, Geo geo, const double* __restrict__ u,
+__global__ void k_closure_shear(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
                                 const double* __restrict__ v, const double* __restrict__ b,
                                 double* __restrict__ s_n2, double* __restrict__ s_sh) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -262,7 +266,7 @@ __global__ void k_closure_shear(Dims d, Geo geo, const double* __restrict__ u,
   const int k = (int)(t / d.n2);
   const long long c = t % d.n2;
   double n2k, sh;
-  shear_n2(d, geo, u, v, b, c, k, n2k, sh);
+  shear_n2(d, p, geo, u, v, b, c, k, n2k, sh);
   s_n2[t] = n2k;
   s_sh[t] = sh;
 }
@@ -276,26 +280,27 @@ __global__ void k_mixing_recursive(Dims d, StepParams p, Geo geo, const double* 
   if (c >= d.n2) return;
   const int nz = d.nz;
   const long long n2 = d.n2;
+  const ColumnGeo cg(geo, p, d, c);
   double h = 0.0;
-  for (int k = 0; k < nz; ++k) h = h + geo.dz3[k * n2 + c];
+  for (int k = 0; k < nz; ++k) h = h + cg.dz(k);
   double zi = 0.0;
   for (int k = 0; k <= nz; ++k) {
-    if (k > 0) zi = zi + geo.dz3[(k - 1) * n2 + c];
+    if (k > 0) zi = zi + cg.dz(k - 1);
     const double n2c = nmax(s_n2[k * n2 + c], p.n2_min);
     const double ek = nmax(e_in[k * n2 + c], p.e_min);
     double l = sqrt(2.0 * ek / n2c);
     const double zb = nmax(h - zi, 0.0);
     l = nmin(l, nmin(zi + p.z_0, zb + p.z_0));
-    if (k > 0) l = nmin(l, s_l[(k - 1) * n2 + c] + geo.dzi[k * n2 + c]);
+    if (k > 0) l = nmin(l, s_l[(k - 1) * n2 + c] + cg.dzi(k));
     s_l[k * n2 + c] = l;
   }
   for (int k = nz - 1; k >= 0; --k) {
-    const double l = nmin(s_l[k * n2 + c], s_l[(k + 1) * n2 + c] + geo.dzi[(k + 1) * n2 + c]);
+    const double l = nmin(s_l[k * n2 + c], s_l[(k + 1) * n2 + c] + cg.dzi(k + 1));
     s_l[k * n2 + c] = l;
   }
   for (int k = 0; k <= nz; ++k) {
     const double wet = (k >= 1 && k < nz)
-                           ? mk(geo.m3[(k - 1) * n2 + c]) * mk(geo.m3[k * n2 + c]) : 0.0;
+                           ? (cg.wet(k - 1) ? 1.0 : 0.0) * (cg.wet(k) ? 1.0 : 0.0) : 0.0;
     const double l = s_l[k * n2 + c] * wet;  // l_up = l_dn = l
     s_l[k * n2 + c] = nmax(p.l_min, sqrt(l * l));
     s_le[k * n2 + c] = nmax(p.l_min, nmin(l, l));
@@ -313,27 +318,33 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
   if (c >= d.n2) return;
   const int nz = d.nz;
   const long long n2 = d.n2;
-  const uint8_t* m3 = geo.m3;
+  const ColumnGeo cg(geo, p, d, c);
   const double kb = geo.kappa_b[c];
   const double dt = p.dt;
   double c_prev = 0.0, d_prev = 0.0;
   double rhs0 = 0.0;
+  // K_m at interfaces k-1, k, k+1 and the wet flag of the cell above
+  // interface k, carried down the column
+  double km_up = 0.0, km_0 = Km[c], km_dn = nz >= 1 ? Km[n2 + c] : 0.0;
+  bool wet_above = false;
   for (int k = 0; k <= nz; ++k) {
-    const double wet = (k >= 1 && k < nz) ? mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]) : 0.0;
+    const bool wet_below = k < nz && cg.wet(k);
+    const double wet = (k >= 1 && k < nz) ? (wet_above ? 1.0 : 0.0) * (wet_below ? 1.0 : 0.0)
+                                          : 0.0;
     const double ek = e_in[k * n2 + c];
     const double n2k = s_n2[k * n2 + c];
     const double l_eps = s_le[k * n2 + c];
-    const double kmo = Km[k * n2 + c];
+    const double kmo = km_0;
     const double kho = Kh[k * n2 + c];
     const double prod = (kmo - p.nu_b) * s_sh[k * n2 + c] - (kho - kb) * n2k;
     const double diss = p.c_eps * sqrt(nmax(ek, p.e_min)) / l_eps;
     double a_up = 0.0, a_dn = 0.0;
     if (k >= 1 && k < nz) {
-      const double idzi = safe_inv(geo.dzi[k * n2 + c]);
-      const double klay_up = 0.5 * (Km[(k - 1) * n2 + c] + kmo);
-      const double klay_dn = 0.5 * (kmo + Km[(k + 1) * n2 + c]);
-      a_up = dt * klay_up * safe_inv(geo.dz3[(k - 1) * n2 + c]) * idzi;
-      a_dn = dt * klay_dn * safe_inv(geo.dz3[k * n2 + c]) * idzi;
+      const double idzi = safe_inv(cg.dzi(k));
+      const double klay_up = 0.5 * (km_up + kmo);
+      const double klay_dn = 0.5 * (kmo + km_dn);
+      a_up = dt * klay_up * safe_inv(cg.dz(k - 1)) * idzi;
+      a_dn = dt * klay_dn * safe_inv(cg.dz(k)) * idzi;
     }
     double sub = -a_up, sup = -a_dn;
     double diag = 1.0 + a_up + a_dn + dt * diss;
@@ -345,7 +356,7 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
       rhs = p.e_min;
     }
     if (k == 0) {
-      rhs = m3[c] ? p.e_bb * geo.ustar2[c] : p.e_min;
+      rhs = wet_below ? p.e_bb * geo.ustar2[c] : p.e_min;
       rhs0 = rhs;
     }
     double cst, dst;
@@ -361,15 +372,22 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
     e_out[k * n2 + c] = dst;
     c_prev = cst;
     d_prev = dst;
+    km_up = km_0;
+    km_0 = km_dn;
+    km_dn = k + 2 <= nz ? Km[(k + 2) * n2 + c] : 0.0;
+    wet_above = wet_below;
   }
 
   // back-substitution, floors, eddy coefficients
   double x_next = 0.0;
+  bool wet_below = false;  // cell k (below interface k)
   for (int k = nz; k >= 0; --k) {
+    const bool wet_above = k >= 1 && cg.wet(k - 1);
     const double dst = e_out[k * n2 + c];
     const double x = (k == nz) ? dst : dst - s_c[k * n2 + c] * x_next;
     x_next = x;
-    const double wet = (k >= 1 && k < nz) ? mk(m3[(k - 1) * n2 + c]) * mk(m3[k * n2 + c]) : 0.0;
+    const double wet = (k >= 1 && k < nz) ? (wet_above ? 1.0 : 0.0) * (wet_below ? 1.0 : 0.0)
+                                          : 0.0;
     double en = nmax(x, p.e_min);
     en = wet > 0.0 ? en : p.e_min;
     if (k == 0) en = rhs0;
@@ -378,85 +396,46 @@ __global__ void k_closure(Dims d, StepParams p, Geo geo, const double* __restric
     e_out[k * n2 + c] = en;
     Km[k * n2 + c] = km;
     Kh[k * n2 + c] = kh;
+    wet_below = wet_above;
   }
 }
 
 // ------------------------------------------------ vertical coefficients
-// Row k of one implicit vertical-diffusion matrix (I - theta_v dt D[nu]) of
-// a column, factored for the Thomas sweep: sub, den = diag - sub c*_{k-1}
-// and c* = sup / den (libs/core/vertical_var.py:35-77). inv_dz, idzi_t and
-// idzi_b are 1/dz3 and 1/dzi above and below the cell.
-__device__ __forceinline__ void factor_row(const StepParams& p, int k, double inv_dz,
-                                           double nu_t, double nu_b, double idzi_t,
-                                           double idzi_b, bool wet, bool below,
-                                           double drag_term, double c_prev, double& sub_o,
-                                           double& den_o, double& cst_o) {
-  const double fac = p.thv_dt * inv_dz;
-  const double a_top = fac * nu_t * idzi_t;
-  const double a_bot = fac * nu_b * idzi_b;
-  double sub = -a_top, sup = -a_bot;
-  double diag = 1.0 + a_top + a_bot;
-  if (k == 0) {
-    diag = 1.0 + a_bot;
-    sub = 0.0;
-  }
-  if (wet && !below) {
-    diag = 1.0 + a_top + drag_term * inv_dz;
-    sup = 0.0;
-  }
-  if (!wet) {
-    diag = 1.0;
-    sub = 0.0;
-    sup = 0.0;
-  }
-  den_o = (k == 0) ? diag : diag - sub * c_prev;
-  cst_o = sup / den_o;
-  sub_o = sub;
-}
-
-// tri_m (viscosity Km, bottom drag) and tri_b (diffusivity Kh, no drag),
-// then q = A_m^-1 1 (libs/core/model3d_v05.py:141-153).
+// tri_m (viscosity Km, bottom drag), factored, then q = A_m^-1 1
+// (libs/core/model3d_v05.py:141-153); tri_b is factored by the tracer
+// kernels from the K_h they read anyway.
 __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* __restrict__ Km,
-                               const double* __restrict__ Kh, double* __restrict__ sub_m,
-                               double* __restrict__ den_m, double* __restrict__ cst_m,
-                               double* __restrict__ sub_b, double* __restrict__ den_b,
-                               double* __restrict__ cst_b, double* __restrict__ q) {
+                               double* __restrict__ sub_m, double* __restrict__ den_m,
+                               double* __restrict__ cst_m, double* __restrict__ q) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
   const int nz = d.nz;
   const long long n2 = d.n2;
-  // both factorizations and the forward sweep of q in one pass over the
+  // the factorization and the forward sweep of q in one pass over the
   // column, so the geometry is read once and tri_m is not read back
-  const double drag_b = p.thv_dt * 0.0;
-  double cm_prev = 0.0, cb_prev = 0.0, d_prev = 0.0;
-  double idzi_t = safe_inv(geo.dzi[c]);
-  double km_t = Km[c], kh_t = Kh[c];
-  bool wet = geo.m3[c] != 0;
+  const ColumnGeo cg(geo, p, d, c);
+  double cm_prev = 0.0, d_prev = 0.0;
+  double idzi_t = safe_inv(cg.dzi(0));
+  double km_t = Km[c];
+  bool wet = cg.wet(0);
   for (int k = 0; k < nz; ++k) {
     const long long o = k * n2 + c;
-    const double inv_dz = safe_inv(geo.dz3[o]);
-    const double idzi_b = safe_inv(geo.dzi[o + n2]);
-    const double km_b = Km[o + n2], kh_b = Kh[o + n2];
-    const bool below = (k + 1 < nz) && geo.m3[o + n2] != 0;
-    double sm, dm, cm, sb, db, cb;
+    const double inv_dz = safe_inv(cg.dz(k));
+    const double idzi_b = safe_inv(cg.dzi(k + 1));
+    const double km_b = Km[o + n2];
+    const bool below = (k + 1 < nz) && cg.wet(k + 1);
+    double sm, dm, cm;
     factor_row(p, k, inv_dz, km_t, km_b, idzi_t, idzi_b, wet, below, p.thv_dt_drag, cm_prev,
                sm, dm, cm);
-    factor_row(p, k, inv_dz, kh_t, kh_b, idzi_t, idzi_b, wet, below, drag_b, cb_prev, sb, db,
-               cb);
     sub_m[o] = sm;
     den_m[o] = dm;
     cst_m[o] = cm;
-    sub_b[o] = sb;
-    den_b[o] = db;
-    cst_b[o] = cb;
     const double dst = (k == 0) ? 1.0 / dm : (1.0 - sm * d_prev) / dm;
     q[o] = dst;
     d_prev = dst;
     cm_prev = cm;
-    cb_prev = cb;
     idzi_t = idzi_b;
     km_t = km_b;
-    kh_t = kh_b;
     wet = below;
   }
   double x_next = q[(nz - 1) * n2 + c];
@@ -471,19 +450,20 @@ __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* __re
 // Helmholtz face coefficients Ku = sum_k dz3u min(q, q_east) mask_u and Kv
 // (libs/core/model3d_v05.py:154-157), summed in k order like NumPy's
 // axis-0 reduction.
-__global__ void k_kukv(Dims d, Geo geo, const double* __restrict__ q, double* __restrict__ Ku,
-                       double* __restrict__ Kv) {
+__global__ void k_kukv(Dims d, StepParams p, Geo geo, const double* __restrict__ q,
+                       double* __restrict__ Ku, double* __restrict__ Kv) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
   const int i = (int)(c % d.nx), j = (int)(c / d.nx);
   const long long ce = at2(d, j, wrap(i, 1, d.nx));
   const long long cn = at2(d, wrap(j, 1, d.ny), i);
   const long long n2 = d.n2;
+  const FaceGeo fu(geo, p, d, c, ce, 0), fv(geo, p, d, c, cn, 1);
   double su = 0.0, sv = 0.0;
   for (int k = 0; k < d.nz; ++k) {
     const double qc = q[k * n2 + c];
-    const double tu = geo.dz3u[k * n2 + c] * nmin(qc, q[k * n2 + ce]);
-    const double tv = geo.dz3v[k * n2 + c] * nmin(qc, q[k * n2 + cn]);
+    const double tu = fu.dz(k) * nmin(qc, q[k * n2 + ce]);
+    const double tv = fv.dz(k) * nmin(qc, q[k * n2 + cn]);
     su = (k == 0) ? tu : su + tu;
     sv = (k == 0) ? tv : sv + tv;
   }
@@ -499,9 +479,10 @@ __global__ void k_phi(Dims d, StepParams p, Geo geo, const double* __restrict__ 
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
   const long long n2 = d.n2;
+  const ColumnGeo cg(geo, p, d, c);
   double above = 0.0;
   for (int k = 0; k < d.nz; ++k) {
-    const double dz = geo.dz3[k * n2 + c];
+    const double dz = cg.dz(k);
     const double bk = b[k * n2 + c];
     phi[k * n2 + c] = 0.5 * dz * bk + above;
     const double t = dz * bk;
@@ -510,7 +491,7 @@ __global__ void k_phi(Dims d, StepParams p, Geo geo, const double* __restrict__ 
   if (p.pgf_remove) {
     double tot = 0.0, wsum = 0.0;
     for (int k = 0; k < d.nz; ++k) {
-      const double dz = geo.dz3[k * n2 + c];
+      const double dz = cg.dz(k);
       const double t = dz * phi[k * n2 + c];
       tot = (k == 0) ? dz : tot + dz;
       wsum = (k == 0) ? t : wsum + t;
@@ -532,14 +513,17 @@ __global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* __restrict
   const long long cs = at2(d, wrap(j, -1, d.ny), i);
   const long long n2 = d.n2;
   const int nz = d.nz;
+  const ColumnGeo cg(geo, p, d, c);
+  const FaceGeo fe(geo, p, d, c, at2(d, j, wrap(i, 1, d.nx)), 0), fw(geo, p, d, cw, c, 0);
+  const FaceGeo fn(geo, p, d, c, at2(d, wrap(j, 1, d.ny), i), 1), fs(geo, p, d, cs, c, 1);
   double s = 0.0;
   for (int k = nz - 1; k >= 0; --k) {
     const long long o = k * n2;
-    const double U = geo.dz3u[o + c] * u[o + c];
-    const double Uw = geo.dz3u[o + cw] * u[o + cw];
-    const double V = geo.dz3v[o + c] * v[o + c];
-    const double Vs = geo.dz3v[o + cs] * v[o + cs];
-    const double dv = mk(geo.m3[o + c]) * ((U - Uw) / p.dx + (V - Vs) / p.dy);
+    const double U = fe.dz(k) * u[o + c];
+    const double Uw = fw.dz(k) * u[o + cw];
+    const double V = fn.dz(k) * v[o + c];
+    const double Vs = fs.dz(k) * v[o + cs];
+    const double dv = (cg.wet(k) ? 1.0 : 0.0) * ((U - Uw) / p.dx + (V - Vs) / p.dy);
     s = (k == nz - 1) ? dv : s + dv;
     w[o + c] = -s;
   }
@@ -841,16 +825,19 @@ __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double*
 }
 
 // Depth-integrated transports U = sum_k dz3u u, V = sum_k dz3v v (k order).
-__global__ void k_depth_sum(Dims d, Geo geo, const double* __restrict__ u,
+__global__ void k_depth_sum(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
                             const double* __restrict__ v, double* __restrict__ U,
                             double* __restrict__ V) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
   const long long n2 = d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const FaceGeo fu(geo, p, d, c, at2(d, j, wrap(i, 1, d.nx)), 0);
+  const FaceGeo fv(geo, p, d, c, at2(d, wrap(j, 1, d.ny), i), 1);
   double su = 0.0, sv = 0.0;
   for (int k = 0; k < d.nz; ++k) {
-    const double tu = geo.dz3u[k * n2 + c] * u[k * n2 + c];
-    const double tv = geo.dz3v[k * n2 + c] * v[k * n2 + c];
+    const double tu = fu.dz(k) * u[k * n2 + c];
+    const double tv = fv.dz(k) * v[k * n2 + c];
     su = (k == 0) ? tu : su + tu;
     sv = (k == 0) ? tv : sv + tv;
   }
@@ -1118,6 +1105,119 @@ __global__ void k_predictor(Dims d, StepParams p, Geo geo, const double* __restr
   Vg[c] = sv;
 }
 
+// Coriolis partners with the face masks given as flags: the same sums in
+// the same order as avg_v_to_u / avg_u_to_v.
+__device__ __forceinline__ double avg4(double a0, double a1, double a2, double a3, bool f0,
+                                       bool f1, bool f2, bool f3, bool own) {
+  const double w0 = f0 ? 1.0 : 0.0, w1 = f1 ? 1.0 : 0.0, w2 = f2 ? 1.0 : 0.0, w3 = f3 ? 1.0 : 0.0;
+  const double num = 0.0 + w0 * a0 + w1 * a1 + w2 * a2 + w3 * a3;
+  const double den = 0.0 + w0 + w1 + w2 + w3;
+  const double out = den > 0.0 ? num / den : 0.0;
+  return out * (own ? 1.0 : 0.0);
+}
+
+// k_predictor for a static z-level geometry in column form: the same
+// operations on the same operands in the same order (the same bits), with
+// the cell and face masks of the Coriolis averages and the face thicknesses
+// of the depth sums formed from the wet-level counts of the column and its
+// neighbours instead of read per cell. At most 128 VGPRs, so that a 512 x
+// 512 plane of columns is resident at once (4 waves per SIMD): the column
+// march is latency-bound.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_predictor_col(
+    Dims d, StepParams p, Geo geo, const double* __restrict__ u, const double* __restrict__ v,
+    const double* __restrict__ eta, const double* __restrict__ u_it,
+    const double* __restrict__ v_it, const double* __restrict__ pgx,
+    const double* __restrict__ pgy, const double* __restrict__ dzu,
+    const double* __restrict__ dzv, const double* __restrict__ stu,
+    const double* __restrict__ stv, const double* __restrict__ sub,
+    const double* __restrict__ den, const double* __restrict__ cst, double* __restrict__ gh_u,
+    double* __restrict__ gh_v, double* __restrict__ Ug, double* __restrict__ Vg) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const Nb q = neighbours(d, j, i);
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const bool split = p.scheme == SCHEME_SPLIT;
+  // a face mask at level k is k < its threshold: the 2D face mask and the
+  // shallower of the face's two cells
+  const int nwc = geo.nw[q.c], nwe = geo.nw[q.e], nwn = geo.nw[q.n], nwne = geo.nw[q.ne];
+  auto thr = [](uint8_t open, int a, int b) { return open ? (a < b ? a : b) : 0; };
+  const int tu_c = thr(geo.mu2[q.c], nwc, nwe), tu_w = thr(geo.mu2[q.w], geo.nw[q.w], nwc);
+  const int tu_n = thr(geo.mu2[q.n], nwn, nwne), tu_nw = thr(geo.mu2[q.nw], geo.nw[q.nw], nwn);
+  const int tv_c = thr(geo.mv2[q.c], nwc, nwn), tv_s = thr(geo.mv2[q.s], geo.nw[q.s], nwc);
+  const int tv_e = thr(geo.mv2[q.e], nwe, nwne), tv_se = thr(geo.mv2[q.se], geo.nw[q.se], nwe);
+  const double gx2 = mk(geo.mu2[c]) * (eta[q.e] - eta[c]) / p.dx;
+  const double gy2 = mk(geo.mv2[c]) * (eta[q.n] - eta[c]) / p.dy;
+  const bool first = u_it == u;
+  double du_prev = 0.0, dv_prev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long o = k * n2;
+    const long long t = o + c;
+    // Coriolis partners of the old state, and of the iterate in a second pass
+    const bool fv0 = k < tv_c, fv1 = k < tv_s, fv2 = k < tv_e, fv3 = k < tv_se;
+    const bool fu0 = k < tu_c, fu1 = k < tu_w, fu2 = k < tu_n, fu3 = k < tu_nw;
+    const double av_v = avg4(v[t], v[o + q.s], v[o + q.e], v[o + q.se], fv0, fv1, fv2, fv3, fu0);
+    const double av_u = avg4(u[t], u[o + q.w], u[o + q.n], u[o + q.nw], fu0, fu1, fu2, fu3, fv0);
+    const double av_v_it = first ? av_v
+                                 : avg4(v_it[t], v_it[o + q.s], v_it[o + q.e], v_it[o + q.se],
+                                        fv0, fv1, fv2, fv3, fu0);
+    const double av_u_it = first ? av_u
+                                 : avg4(u_it[t], u_it[o + q.w], u_it[o + q.n], u_it[o + q.nw],
+                                        fu0, fu1, fu2, fu3, fv0);
+    const double vcor = p.one_m_tc * av_v + p.theta_cor * av_v_it;
+    const double ucor = p.one_m_tc * av_u + p.theta_cor * av_u_it;
+    const double gxe = gx2 * (fu0 ? 1.0 : 0.0);
+    const double gye = gy2 * (fv0 ? 1.0 : 0.0);
+    // split_explicit leaves the surface gradient to the barotropic substeps
+    // (model3d_v05.py:431-432)
+    const double gu = split ? u[t] + p.dt * (p.f * vcor + pgx[t]) + p.c_dzd * dzu[t]
+                            : u[t] + p.dt * (p.f * vcor + pgx[t]) - p.c_gx * gxe + p.c_dzd * dzu[t];
+    const double gv = split ? v[t] + p.dt * (p.neg_f * ucor + pgy[t]) + p.c_dzd * dzv[t]
+                            : v[t] + p.dt * (p.neg_f * ucor + pgy[t]) - p.c_gx * gye +
+                                  p.c_dzd * dzv[t];
+    const double ru = gu + stu[t];
+    const double rv = gv + stv[t];
+    double du, dv;
+    if (k == 0) {
+      du = ru / den[t];
+      dv = rv / den[t];
+    } else {
+      du = (ru - sub[t] * du_prev) / den[t];
+      dv = (rv - sub[t] * dv_prev) / den[t];
+    }
+    gh_u[t] = du;
+    gh_v[t] = dv;
+    du_prev = du;
+    dv_prev = dv;
+  }
+  double xu = 0.0, xv = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long t = k * n2 + c;
+    if (k == nz - 1) {
+      xu = gh_u[t];
+      xv = gh_v[t];
+    } else {
+      xu = gh_u[t] - cst[t] * xu;
+      xv = gh_v[t] - cst[t] * xv;
+    }
+    const double mm = k < nwc ? 1.0 : 0.0;
+    gh_u[t] = xu * mm;
+    gh_v[t] = xv * mm;
+  }
+  const FaceGeo fu(geo, p, d, c, q.e, 0), fv(geo, p, d, c, q.n, 1);
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = k * n2 + c;
+    const double tu = fu.dz(k) * gh_u[t];
+    const double tv = fv.dz(k) * gh_v[t];
+    su = (k == 0) ? tu : su + tu;
+    sv = (k == 0) ? tv : sv + tv;
+  }
+  Ug[c] = su;
+  Vg[c] = sv;
+}
+
 // u_it = (gh_u - g dt th q Gx(eta_new) mask3u) mask3 (model3d_v05.py:404-406)
 __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* __restrict__ gh_u,
                           const double* __restrict__ gh_v, const double* __restrict__ q,
@@ -1131,9 +1231,21 @@ __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* __restric
   const long long cn = at2(d, wrap(j, 1, d.ny), i);
   const double gx = mk(geo.mu2[c]) * (eta_n[ce] - eta_n[c]) / p.dx;
   const double gy = mk(geo.mv2[c]) * (eta_n[cn] - eta_n[c]) / p.dy;
-  const double mm = mk(geo.m3[t]);
-  u_it[t] = (gh_u[t] - p.c_bs * q[t] * gx * mk(geo.m3u[t])) * mm;
-  v_it[t] = (gh_v[t] - p.c_bs * q[t] * gy * mk(geo.m3v[t])) * mm;
+  bool m3, m3u, m3v;
+  if (p.zcol) {
+    const int k = (int)(t / d.n2);
+    const int nwc = geo.nw[c];
+    m3 = k < nwc;
+    m3u = geo.mu2[c] && m3 && k < geo.nw[ce];
+    m3v = geo.mv2[c] && m3 && k < geo.nw[cn];
+  } else {
+    m3 = geo.m3[t] != 0;
+    m3u = geo.m3u[t] != 0;
+    m3v = geo.m3v[t] != 0;
+  }
+  const double mm = m3 ? 1.0 : 0.0;
+  u_it[t] = (gh_u[t] - p.c_bs * q[t] * gx * (m3u ? 1.0 : 0.0)) * mm;
+  v_it[t] = (gh_v[t] - p.c_bs * q[t] * gy * (m3v ? 1.0 : 0.0)) * mm;
 }
 
 // ------------------------------------------------------------- tracers
@@ -1181,69 +1293,6 @@ __device__ double tracer_advection(const Dims& d, const StepParams& p, const Geo
   };
   const double vdiv = (w[o + c] * cf(k) - w[o + n2 + c] * cf(k + 1)) * inv;
   return (hdiv + vdiv) * mk(M[q.c]);
-}
-
-// Right-hand side of one tracer at layer k (model3d_v05.py:466-479):
-// c + dt tend + (1-thv) dt Dz[c], plus theta_v dt flux / dz at the surface.
-__device__ double tracer_rhs(const Dims& d, const StepParams& p, const Geo& geo,
-                             const double* cc, const double* u, const double* v, const double* w,
-                             const double* Kh, double sflux, long long c, int k, const Nb& q,
-                             bool buoyancy_tracer) {
-  const long long o = (long long)k * d.n2;
-  double tend = 0.0;
-  if (buoyancy_tracer) tend = p.neg_N2 * (0.5 * (w[o + c] + w[o + d.n2 + c]));
-  if (p.advection != ADV_NONE) tend = tend - tracer_advection(d, p, geo, cc, u, v, w, c, k, q);
-  if (p.has_K_h) tend = tend + p.K_h * laplacian(p, cc, geo.m3u, geo.m3v, geo.m3, o, q);
-  const double dzd = explicit_diffusion(d, geo.m3, cc, Kh, geo.dz3, k == 0 ? sflux : 0.0, 0.0, c, k);
-  double r = cc[o + c] + p.dt * tend + p.c_dzd * dzd;
-  if (k == 0) r = r + p.thv_dt * sflux * safe_inv(geo.dz3[c]);
-  return r;
-}
-
-// Both tracers of a column solved on tri_b, then b from the EOS
-// (model3d_v05.py:514-533); buoyancy-only runs advance b with the -N^2 w
-// source instead. T_out/S_out (or b_out) hold d* between the sweeps.
-__global__ void k_tracers(Dims d, StepParams p, Geo geo, const double* __restrict__ T,
-                          const double* __restrict__ S, const double* __restrict__ b,
-                          const double* __restrict__ u, const double* __restrict__ v,
-                          const double* __restrict__ w, const double* __restrict__ Kh,
-                          const double* __restrict__ sub, const double* __restrict__ den,
-                          const double* __restrict__ cst, double* __restrict__ T_out,
-                          double* __restrict__ S_out, double* __restrict__ b_out) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.n2) return;
-  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
-  const Nb q = neighbours(d, j, i);
-  const long long n2 = d.n2;
-  const int nz = d.nz;
-  const bool ts = p.tracers_ts != 0;
-  const double* A = ts ? T : b;
-  double* Ao = ts ? T_out : b_out;
-  const double fa = ts ? p.flux_T : 0.0;
-  double da = 0.0, db = 0.0;
-  for (int k = 0; k < nz; ++k) {
-    const long long t = k * n2 + c;
-    const double ra = tracer_rhs(d, p, geo, A, u, v, w, Kh, fa, c, k, q, !ts);
-    da = (k == 0) ? ra / den[t] : (ra - sub[t] * da) / den[t];
-    Ao[t] = da;
-    if (ts) {
-      const double rb = tracer_rhs(d, p, geo, S, u, v, w, Kh, p.flux_S, c, k, q, false);
-      db = (k == 0) ? rb / den[t] : (rb - sub[t] * db) / den[t];
-      S_out[t] = db;
-    }
-  }
-  double xa = 0.0, xb = 0.0;
-  for (int k = nz - 1; k >= 0; --k) {
-    const long long t = k * n2 + c;
-    const double mm = mk(geo.m3[t]);
-    xa = (k == nz - 1) ? Ao[t] : Ao[t] - cst[t] * xa;
-    Ao[t] = xa * mm;
-    if (ts) {
-      xb = (k == nz - 1) ? S_out[t] : S_out[t] - cst[t] * xb;
-      S_out[t] = xb * mm;
-      b_out[t] = buoyancy(p, xa * mm, xb * mm, geo.zc[t]) * mm;
-    }
-  }
 }
 
 // ---------------------------------------------- standalone operator kernels

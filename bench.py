@@ -182,12 +182,17 @@ def _initial_fields(prob, st, torch):
                             dtype=torch.float64, device=st.device)}
 
 
-def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60) -> dict:
+FP32_PARTS = {"vcycle": 1, "tracer_flux": 2, "eos": 4, "momentum_flux": 8}
+
+
+def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60, per_part: bool = True) -> dict:
     """The plain fp64 step and the mixed step on the 1024 x 1024 x 60 basin
     (dx 5 km, dt 300 s, same generator), each timed over `steps` steps after
     one untimed warm-up step from the same initial state; per-field relative
     L2 of the mixed final state against the fp64 one (the fp64 GPU step is
-    the oracle's bit for bit on the parity cases)."""
+    the oracle's bit for bit on the parity cases). With per_part, each fp32
+    part alone as well (hoc_stepper_set_fp32_parts), so the error and the
+    time of the mixed step can be attributed to its parts."""
     import torch
 
     from hydrostatic_ocean_core_benchmark_amd.cases import basin
@@ -196,8 +201,14 @@ def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60) -> dict:
     g = prob.domain.grid
     cells = g.nx * g.ny * g.nz
     final, rows = {}, {}
-    for prec in ("fp64", "mixed", "mixed_fp32_momentum"):
+    runs = [("fp64", "fp64", None), ("mixed", "mixed", None),
+            ("mixed_fp32_momentum", "mixed_fp32_momentum", None)]
+    if per_part:
+        runs += [(f"only_{name}", "mixed", bit) for name, bit in FP32_PARTS.items()]
+    for label, prec, parts in runs:
         st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=prec)
+        if parts is not None:
+            st.set_fp32_parts(parts)
         bufs = [st._new_state_buffers(), st._new_state_buffers()]
         st._step_into(_initial_fields(prob, st, torch), bufs[1])  # warm-up
         st.reset_mixing()
@@ -210,44 +221,58 @@ def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60) -> dict:
             cur = bufs[n % 2]
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        final[prec] = {k: cur[k].clone() for k in ("eta", "T", "S", "u", "v", "b")}
-        rows[prec] = {"value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                      "solver_iterations": st.solver_iterations - it0,
-                      "solver_failures": st.solver_failures}
+        final[label] = {k: cur[k].clone() for k in ("eta", "T", "S", "u", "v", "b")}
+        rows[label] = {"value": cells * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                       "solver_iterations": st.solver_iterations - it0,
+                       "solver_failures": st.solver_failures}
+        st.close()
         del st, bufs, cur
         torch.cuda.empty_cache()
-    def err_of(prec):
-        return {k: float(torch.linalg.vector_norm(final[prec][k] - final["fp64"][k])
+
+    def err_of(label):
+        return {k: float(torch.linalg.vector_norm(final[label][k] - final["fp64"][k])
                          / torch.linalg.vector_norm(final["fp64"][k])) for k in final["fp64"]}
 
-    return {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 5 km, dt 300 s, "
-                        f"{steps} steps, polyTEOS10, up3_tvd, TKE, theta 0.5, pcg_mg "
-                        "(5 levels, 2 + 2 damped-Jacobi sweeps, omega 0.8) rtol 1e-12",
-            "unit": "cell-steps/s", "fp64": rows["fp64"], "mixed": rows["mixed"],
-            "speedup": rows["mixed"]["value"] / rows["fp64"]["value"],
-            "rel_l2_vs_fp64": err_of("mixed"),
-            "fp32_parts": "pcg_mg V-cycle preconditioner, tracer advective fluxes, EOS "
-                          "(anomaly form)",
-            "fp64_parts": "momentum advective fluxes, state updates, column (Thomas) solves, "
-                          "closure, outer PCG residual / dot products / update",
-            "mixed_fp32_momentum": {**rows["mixed_fp32_momentum"],
-                                    "speedup": rows["mixed_fp32_momentum"]["value"]
-                                    / rows["fp64"]["value"],
-                                    "rel_l2_vs_fp64": err_of("mixed_fp32_momentum"),
-                                    "note": "as mixed, with the momentum advective fluxes in "
-                                            "fp32 too"}}
+    out = {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 5 km, dt 300 s, "
+                       f"{steps} steps, polyTEOS10, up3_tvd, TKE, theta 0.5, pcg_mg "
+                       "(5 levels, 2 + 2 damped-Jacobi sweeps, omega 0.8) rtol 1e-12",
+           "unit": "cell-steps/s", "fp64": rows["fp64"], "mixed": rows["mixed"],
+           "speedup": rows["mixed"]["value"] / rows["fp64"]["value"],
+           "rel_l2_vs_fp64": err_of("mixed"),
+           "fp32_parts": "pcg_mg V-cycle preconditioner, tracer advective fluxes, EOS "
+                         "(anomaly form)",
+           "fp64_parts": "momentum advective fluxes, state updates, column (Thomas) solves, "
+                         "closure, outer PCG residual / dot products / update",
+           "mixed_fp32_momentum": {**rows["mixed_fp32_momentum"],
+                                   "speedup": rows["mixed_fp32_momentum"]["value"]
+                                   / rows["fp64"]["value"],
+                                   "rel_l2_vs_fp64": err_of("mixed_fp32_momentum"),
+                                   "note": "as mixed, with the momentum advective fluxes in "
+                                           "fp32 too"}}
+    if per_part:
+        out["per_part"] = {name: {"ms_per_step": round(rows[f"only_{name}"]["ms_per_step"], 3),
+                                  "speedup": rows[f"only_{name}"]["value"] / rows["fp64"]["value"],
+                                  "solver_iterations": rows[f"only_{name}"]["solver_iterations"],
+                                  "rel_l2_vs_fp64": err_of(f"only_{name}")}
+                           for name in FP32_PARTS}
+    return out
 
 
-def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30) -> dict:
+def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
+                  fd_directions: int = 16) -> dict:
     """dJ/dT0 and dJ/dkappa_b after `steps` steps of the 256 x 256 x 30 basin
     (same generator, dx 10 km, dt 600 s; kappa_b = 1e-5 (1 + 0.2 xi), xi
     seeded N(0, 1) per column), checkpoints every `every` steps: wall time of
     the gradient (forward sweep with checkpoints + reverse sweep) over the
-    wall time of the plain forward of the same steps, after one warm-up."""
+    wall time of the plain forward of the same steps, after one warm-up.
+    Then the stated gradient check: central differences with h = 1e-6 along
+    `fd_directions` seeded random directions of T0, each J from the plain
+    fp64 HIP forward of the same steps, against the adjoint's directional
+    derivative, for the workload's up3_tvd and for the unlimited up3."""
     import numpy as np
     import torch
 
-    from hydrostatic_ocean_core_benchmark_amd.adjoint import Adjoint
+    from hydrostatic_ocean_core_benchmark_amd.adjoint import Adjoint, directional_check
     from hydrostatic_ocean_core_benchmark_amd.cases import basin
     prob = basin(nx=nx, nz=nz)
     g = prob.domain.grid
@@ -259,16 +284,32 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30)
     adj.gradient(T0, S0, kb, n_steps=2, every=every)
     torch.cuda.synchronize()
     r = adj.gradient(T0, S0, kb, n_steps=steps, every=every)
-    return {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 10 km, dt 600 s, "
-                        f"{steps} steps, checkpoints every {every}, J = depth-weighted mean "
-                        "SSH^2 + 0.5 volume-mean face-weighted speed^2; gradients w.r.t. T0 "
-                        "and kappa_b(x, y)",
-            "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
-            "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
-            "hand_written": "Thomas (transposed sweep), Helmholtz (PCG on the adjoint rhs), "
-                            "tracer and momentum advection, Gaspar mixing lengths and the EOS "
-                            "(reverse kernels); the rest by autograd through the torch "
-                            "restatement"}
+    out = {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 10 km, dt 600 s, "
+                       f"{steps} steps, checkpoints every {every}, J = depth-weighted mean "
+                       "SSH^2 + 0.5 volume-mean face-weighted speed^2; gradients w.r.t. T0 "
+                       "and kappa_b(x, y)",
+           "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
+           "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
+           "hand_written": "Thomas (transposed sweep), Helmholtz (PCG on the adjoint rhs), "
+                           "tracer and momentum advection, Gaspar mixing lengths and the EOS "
+                           "(reverse kernels); the rest by autograd through the torch "
+                           "restatement"}
+    if fd_directions > 0:
+        checks = {}
+        for adv in ("up3_tvd", "up3"):
+            if adv != prob.params.advection:
+                prob.params.advection = adv
+                adj = Adjoint(prob.domain, prob.physics, prob.params, prob.dt)
+                r = adj.gradient(T0, S0, kb, n_steps=steps, every=every)
+            c = directional_check(adj, T0, S0, kb, r, n_dirs=fd_directions, h=1e-6)
+            checks[adv] = {k: c[k] for k in ("rel_diff_max", "rel_diff_median",
+                                             "grad_scaled_diff_max", "grad_scaled_diff_median",
+                                             "rel_diff")}
+        out["fd_check"] = {"directions": fd_directions, "h": 1e-6, "steps": steps,
+                           "metric": "rel_diff: |fd - ad| / |ad| of the directional derivative "
+                                     "along seeded N(0, 1) directions d of T0; grad_scaled_diff: "
+                                     "|fd - ad| / (|dJ/dT0| |d|)", **checks}
+    return out
 
 
 def ensemble_throughput(steps: int = 100) -> dict:

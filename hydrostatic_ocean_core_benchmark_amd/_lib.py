@@ -119,6 +119,8 @@ PROTOTYPES = {
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
     "hoc_stepper_set_sync_free": ([VP, i32], i32),
     "hoc_stepper_set_mg_tile": ([VP, i32], i32),
+    "hoc_stepper_geometry_form": ([VP, i32, C.POINTER(i32)], i32),
+    "hoc_stepper_set_fp32_parts": ([VP, i32], i32),
     "hoc_sw2d_create": ([VP, VP], i32),
     "hoc_sw2d_destroy": ([VP], i32),
     "hoc_sw2d_momentum": ([VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),

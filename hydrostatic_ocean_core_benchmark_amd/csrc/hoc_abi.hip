@@ -187,6 +187,8 @@ struct hoc_stepper {
   float *mgf_Ku[MAXLEV] = {}, *mgf_Kv[MAXLEV] = {}, *mgf_inv[MAXLEV] = {};
   float *mgf_x[MAXLEV] = {}, *mgf_t[MAXLEV] = {}, *mgf_b[MAXLEV] = {};
   bool force_multikernel = true;   // per-phase launches unless path 0 is asked for
+  // parts of a mixed-precision step run in fp32 (HOC_FP32_* bits); 0 for fp64
+  int fp32_parts = 0;
   bool coeffs_built = false;
   // host counters
   long long solver_rebuilds = 0, tridiagonal_solves = 0, eos_cells = 0;
@@ -329,8 +331,8 @@ constexpr TeosTerm TEOS_TERMS[] = {
 // polynomial in a = tt - tt_r, c = ss - ss_r (binomial re-expansion about
 // tt_r = T0 / 40, ss_r = sqrt((S0 + 32) 0.875 / 35.16504)); G[k][p][q] holds
 // the coefficient of a^p c^q, the constant terms go to ref[k] (fp64).
-void teos_anomaly_table(double T0, double S0, double ref[4], float G[4][7][7],
-                        double* ss_ref) {
+void teos_anomaly_table(double T0, double S0, double ref[4], double lin[4][2],
+                        float G[4][7][7], double* ss_ref) {
   const double tt_r = T0 * (1.0 / 40.0);
   const double ss_r = std::sqrt((S0 + 32.0) * (0.875 / 35.16504));
   *ss_ref = ss_r;
@@ -345,10 +347,15 @@ void teos_anomaly_table(double T0, double S0, double ref[4], float G[4][7][7],
       for (int q = 0; q <= t.i; ++q)
         acc[t.k][pp][q] += t.e * binom(t.j, pp) * std::pow(tt_r, t.j - pp) * binom(t.i, q) *
                            std::pow(ss_r, t.i - q);
+  // the constant and linear terms stay fp64 (ref, lin); the fp32 table holds
+  // the terms of degree >= 2, a fraction of the anomaly, so that its fp32
+  // rounding is a fraction of the anomaly's rounding
   for (int k = 0; k < 4; ++k) {
     ref[k] = acc[k][0][0];
+    lin[k][0] = acc[k][1][0];
+    lin[k][1] = acc[k][0][1];
     for (int pp = 0; pp < 7; ++pp)
-      for (int q = 0; q < 7; ++q) G[k][pp][q] = (pp == 0 && q == 0) ? 0.0f : (float)acc[k][pp][q];
+      for (int q = 0; q < 7; ++q) G[k][pp][q] = (pp + q < 2) ? 0.0f : (float)acc[k][pp][q];
   }
 }
 
@@ -630,7 +637,7 @@ int mg_build(hoc_stepper* h, const double* Ku, const double* Kv, const double* i
                                              h->mg_mask[l]);
     k_mg_diag<<<blocks(L.n), TPB, 0, s>>>(L, h->mg_inv[l]);
   }
-  if (h->cfg.precision >= 1) {
+  if (h->cfg.precision >= 1) {  // the fp32 levels exist for every mixed handle
     for (int l = 0; l < h->nlev; ++l) {
       const MgLevel& L = h->lev[l];
       MgLevelF& F = h->levf[l];
@@ -712,7 +719,7 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
   if (h->cfg.solver == SOLVER_PCG_RBGS) {
     k_mg_zero<<<blocks(h->d.n2), TPB, 0, s>>>(h->d.n2, z, h->sc);
     rbgs_sweep_pair(h, 0, r, z, false, true, s);
-  } else if (h->cfg.precision >= 1) {
+  } else if (h->fp32_parts & HOC_FP32_VCYCLE) {
     // the fp32 V-cycle: as the fused fp64 path below, every level's vectors
     // and coefficients in fp32; level 0 reads the fp64 residual r and writes
     // the fp64 z (the outer PCG stays fp64)
@@ -1514,6 +1521,10 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   d.ni = d.n2 * (cfg->nz + 1);
   derive_params(*cfg, h->p);
   h->mg_tile_shape = d.n2 <= 512LL * 512LL ? 2 : 0;
+  h->fp32_parts = cfg->precision == 0 ? 0
+                  : cfg->precision == 1 ? HOC_FP32_MIXED
+                                        : HOC_FP32_MIXED | HOC_FP32_MOMENTUM_FLUX;
+  h->p.eos_mixed = (h->fp32_parts & HOC_FP32_EOS) != 0;
   auto cleanup = [&](int rc) {
     delete h;
     return rc;
@@ -1525,7 +1536,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   } while (0)
   if (cfg->precision >= 1) {
     float G[4][7][7];
-    teos_anomaly_table(cfg->T0, cfg->S0, h->p.eos_ref, G, &h->p.ss_ref);
+    teos_anomaly_table(cfg->T0, cfg->S0, h->p.eos_ref, h->p.eos_lin, G, &h->p.ss_ref);
     float* dg = nullptr;
     A(dg, 4 * 7 * 7);
     if (upload(dg, &G[0][0][0], 4 * 7 * 7, "eos table") != HOC_OK) return cleanup(HOC_ERR_DEVICE);
@@ -1983,7 +1994,8 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
-  launch_momentum_explicit_tiled(h->cfg.precision == 2, s, d, p, geo, in->u, in->v, in->b, h->phi,
+  launch_momentum_explicit_tiled((h->fp32_parts & HOC_FP32_MOMENTUM_FLUX) != 0, s, d, p, geo,
+                                 in->u, in->v, in->b, h->phi,
                                  h->w, h->Km, h->pgx, h->pgy, nullptr, nullptr, h->dzu, h->dzv,
                                  h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
@@ -2035,7 +2047,8 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   { ProfScope _ps(h, "k_w_diag", s);
   k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
   { ProfScope _ps(h, "k_tracers", s);
-  launch_tracers_tiled(h->cfg.precision >= 1, s, d, p, geo, in->T, in->S, in->b, out->u, out->v,
+  launch_tracers_tiled((h->fp32_parts & HOC_FP32_TRACER_FLUX) != 0, s, d, p, geo, in->T, in->S,
+                       in->b, out->u, out->v,
                        h->w, h->Kh, h->cst_b, out->T, out->S, out->b); }
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
@@ -2081,6 +2094,27 @@ int hoc_stepper_set_mg_tile(hoc_stepper* h, int32_t shape) {
     return fail(HOC_ERR_INVALID, "mg tile shape must be 0 (32x16), 1 (64x8), 2 (16x16) or "
                                  "3 (64x16)");
   h->mg_tile_shape = shape;
+  return HOC_OK;
+}
+
+int hoc_stepper_geometry_form(hoc_stepper* h, int32_t column, int32_t* active) {
+  if (!h) return fail(HOC_ERR_INVALID, "null handle");
+  if (column == 1 && !h->cg_nw)
+    return fail(HOC_ERR_UNSUPPORTED, "no column form: moving vertical coordinate, or the stored "
+                                     "geometry is not a z-level column form");
+  if (column == 0 || column == 1) h->p.zcol = column;
+  else if (column >= 0) return fail(HOC_ERR_INVALID, "column must be -1 (query), 0 or 1");
+  if (active) *active = h->p.zcol;
+  return HOC_OK;
+}
+
+int hoc_stepper_set_fp32_parts(hoc_stepper* h, int32_t parts) {
+  if (!h) return fail(HOC_ERR_INVALID, "null handle");
+  if (h->cfg.precision == 0)
+    return fail(HOC_ERR_INVALID, "fp32 parts need a handle created with precision 1 or 2");
+  if (parts < 0 || parts > HOC_FP32_ALL) return fail(HOC_ERR_INVALID, "unknown fp32 part bits");
+  h->fp32_parts = parts;
+  h->p.eos_mixed = (parts & HOC_FP32_EOS) != 0;
   return HOC_OK;
 }
 

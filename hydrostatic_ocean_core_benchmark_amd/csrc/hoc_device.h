@@ -411,15 +411,21 @@ __device__ __forceinline__ double buoyancy_mixed(const StepParams& p, double T, 
     const double r0 = (((((R05 * zh + R04) * zh + R03) * zh + R02) * zh + R01) * zh + R00) * zh;
     const double ref = ((p.eos_ref[3] * zh + p.eos_ref[2]) * zh + p.eos_ref[1]) * zh +
                        p.eos_ref[0] + r0;
-    const float a = (float)(T - p.T0) * (1.0f / 40.0f);
-    const float ss = sqrtf((float)((S + 32.0) * (0.875 / 35.16504)));
-    const float c = (float)((S - p.S0) * (0.875 / 35.16504)) / (ss + (float)p.ss_ref);
+    // anomalies in fp64; the linear terms in fp64, the terms of degree >= 2
+    // (the fp32 table has zeros below degree 2) in fp32
+    const double a64 = (T - p.T0) * (1.0 / 40.0);
+    const double ss64 = sqrt((S + 32.0) * (0.875 / 35.16504));
+    const double c64 = ((S - p.S0) * (0.875 / 35.16504)) / (ss64 + p.ss_ref);
+    double lin = 0.0;
+    for (int k = 3; k >= 0; --k)
+      lin = lin * zh + (p.eos_lin[k][0] * a64 + p.eos_lin[k][1] * c64);
+    const float a = (float)a64, c = (float)c64;
     const float zf = (float)zh;
     const float* G = p.eos_g;
     const float d = ((teos_anomaly_poly(G, 3, 1, a, c) * zf + teos_anomaly_poly(G, 2, 2, a, c)) *
                          zf + teos_anomaly_poly(G, 1, 4, a, c)) * zf +
                     teos_anomaly_poly(G, 0, 6, a, c);
-    rp = (ref - p.rho0) + (double)d;
+    rp = (ref - p.rho0) + (lin + (double)d);
   }
   return p.neg_g * rp / p.rho0;
 }

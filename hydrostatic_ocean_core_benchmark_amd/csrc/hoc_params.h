@@ -27,6 +27,7 @@ struct StepParams {
   int open_x, open_y;    // Flather open edges
   int vcoord, face_rule; // vertical coordinate (0 z-level, 1 z*, 2 sigma), face rule
   int zcol;              // 1: the static z-level geometry is read in column form (Geo nw/hb/zs)
+  int eos_mixed;         // 1: the tracer kernels evaluate the EOS in the fp32 anomaly form
   double dzr;            // uniform reference layer thickness of the column form
   double dx, dy, dz_nom, dt;
   double theta, theta_cor, theta_v;
@@ -53,6 +54,7 @@ struct StepParams {
   // polyTEOS10 z-power-k coefficients at the reference point (fp64), eos_g
   // the device table [4][7][7] of the T/S anomaly polynomial (fp32)
   double eos_ref[4];
+  double eos_lin[4][2];   // the anomaly polynomial's linear terms (a, c) per z power, fp64
   double ss_ref;          // sqrt((S0 + 32) * 0.875 / 35.16504)
   const float* eos_g;
 };

@@ -416,10 +416,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_tile
     if (ts) {
       xb = (k == nz - 1) ? s_k : s_k - c_k * xb;
       S_out[tk] = xb * mm;
-      if constexpr (sizeof(R) == sizeof(double))
-        b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
-      else
+      if (p.eos_mixed)
         b_out[tk] = buoyancy_mixed(p, xa * mm, xb * mm, z_k) * mm;
+      else
+        b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
     }
   }
 }
@@ -736,10 +736,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tracers_col(
       xb = (k == nz - 1) ? s_k : s_k - c_k * xb;
       S_out[tk] = xb * mm;
       const double z_k = cg_zc(geo.zs, dzr, nwc, hbc, k);
-      if constexpr (sizeof(R) == sizeof(double))
-        b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
-      else
+      if (p.eos_mixed)
         b_out[tk] = buoyancy_mixed(p, xa * mm, xb * mm, z_k) * mm;
+      else
+        b_out[tk] = buoyancy(p, xa * mm, xb * mm, z_k) * mm;
     }
   }
 }

@@ -186,6 +186,23 @@ class HipStepper:
             device=self.device))
         _lib.check(self.lib.hoc_stepper_build_coefficients(self._h, self._stream()))
 
+    def set_fp32_parts(self, parts: int) -> None:
+        """Select the parts of a mixed step that run in fp32 (bits: 1 the
+        pcg_mg V-cycle, 2 the tracer advective fluxes, 4 the EOS, 8 the
+        momentum advective fluxes); needs precision "mixed" or
+        "mixed_fp32_momentum", whose defaults are 7 and 15."""
+        _lib.check(self.lib.hoc_stepper_set_fp32_parts(self._h, int(parts)))
+
+    def geometry_form(self, column: int | None = None) -> int:
+        """1 when the kernels form the static z-level geometry from per-column
+        wet-level counts, 0 when they read the stored arrays (same results).
+        column = 1 or 0 selects a form (1 raises NotImplementedError where the
+        column form does not exist: moving coordinates); None only queries."""
+        active = C.c_int32(-1)
+        _lib.check(self.lib.hoc_stepper_geometry_form(
+            self._h, -1 if column is None else int(column), C.byref(active)))
+        return int(active.value)
+
     # ------------------------------------------------------------ plumbing
     def close(self) -> None:
         """Destroy the library handle (its workspaces and multigrid levels)

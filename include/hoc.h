@@ -249,6 +249,29 @@ int hoc_stepper_set_sync_free(hoc_stepper* h, int32_t budget);
  * bits for every shape): 0 32x16 (default), 1 64x8, 2 16x16, 3 64x16 cells. */
 int hoc_stepper_set_mg_tile(hoc_stepper* h, int32_t shape);
 
+/* The parts of a mixed-precision step that run in fp32 (no reference
+ * counterpart; for attributing the mixed step's error and time to its
+ * parts). A handle created with precision 1 runs HOC_FP32_MIXED, with 2
+ * HOC_FP32_MIXED | HOC_FP32_MOMENTUM_FLUX; any subset may be selected on a
+ * handle created with precision 1 or 2 (0 gives the fp64 step's results). */
+#define HOC_FP32_VCYCLE 1         /* the pcg_mg V-cycle preconditioner */
+#define HOC_FP32_TRACER_FLUX 2    /* tracer advective fluxes and K_h gradients */
+#define HOC_FP32_EOS 4            /* the EOS (anomaly form) */
+#define HOC_FP32_MOMENTUM_FLUX 8  /* momentum advective fluxes and A_h gradients */
+#define HOC_FP32_MIXED (HOC_FP32_VCYCLE | HOC_FP32_TRACER_FLUX | HOC_FP32_EOS)
+#define HOC_FP32_ALL (HOC_FP32_MIXED | HOC_FP32_MOMENTUM_FLUX)
+int hoc_stepper_set_fp32_parts(hoc_stepper* h, int32_t parts);
+
+/* Geometry form of the kernels (no reference counterpart): for a static
+ * z-level grid the create call derives each column's wet-level count and
+ * deepest wet thickness and, when they reproduce every stored thickness,
+ * face thickness, mask and centre depth bit for bit, the kernels form those
+ * from the columns instead of reading them per cell. column = 1 selects that
+ * form (HOC_ERR_UNSUPPORTED when it is not available), 0 the stored arrays
+ * (same results), -1 only queries; *active (may be NULL) receives the form
+ * in use. */
+int hoc_stepper_geometry_form(hoc_stepper* h, int32_t column, int32_t* active);
+
 /* Adjoint of hoc_eos_buoyancy (no reference counterpart): g_T = g_b db/dT,
  * g_S = g_b db/dS at n points, the derivatives of the evaluated EOS
  * expression (forward-mode dual numbers through the same formula). */

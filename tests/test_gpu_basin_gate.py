@@ -47,6 +47,7 @@ def _gate(hoc, prob, n_steps):
     g = dom.grid
     ost = OracleStepper(dom, prob.physics, prob.params, prob.dt)
     st = hoc.HipStepper(dom, prob.physics, prob.params, prob.dt)
+    assert st.geometry_form() == 1  # the static z-level geometry in column form
     b0 = ost.buoyancy_of(prob.initial_T, prob.initial_S) * dom.mask3
     assert np.array_equal(_np(st.buoyancy_of(prob.initial_T, prob.initial_S)) * dom.mask3, b0)
     zero = np.zeros(g.shape3d)

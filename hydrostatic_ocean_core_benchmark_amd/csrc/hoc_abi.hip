@@ -2025,8 +2025,17 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       LAUNCH_CHECK("k_rhs_eta");
       int iters = 0, conv = 0;
       TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
-      { ProfScope _ps(h, "k_backsub", s);
-  k_backsub<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u, out->v); }
+      if (it + 1 == h->cfg.n_picard) {
+        // the last pass: back-substitution and w of the new velocity in one
+        // column pass (the tracers read w next)
+        ProfScope _ps(h, "k_backsub_w", s);
+        k_backsub_w<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u,
+                                                 out->v, h->w);
+      } else {
+        ProfScope _ps(h, "k_backsub", s);
+        k_backsub<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, gu, gv, h->q, out->eta, out->u,
+                                               out->v);
+      }
       LAUNCH_CHECK("k_backsub");
       u_it = out->u;
       v_it = out->v;
@@ -2044,8 +2053,10 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
     k_eta_fb<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->U0, h->V0, out->eta);
     LAUNCH_CHECK("k_eta_fb");
   }
-  { ProfScope _ps(h, "k_w_diag", s);
-  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w); }
+  if (!theta) {  // the theta scheme's last back-substitution formed w already
+    ProfScope _ps(h, "k_w_diag", s);
+    k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, out->u, out->v, h->w);
+  }
   { ProfScope _ps(h, "k_tracers", s);
   launch_tracers_tiled((h->fp32_parts & HOC_FP32_TRACER_FLUX) != 0, s, d, p, geo, in->T, in->S,
                        in->b, out->u, out->v,

@@ -108,6 +108,9 @@ __global__ void k_predictor_col(Dims d, StepParams p, Geo geo, const double* u, 
                                 const double* dzv, const double* stu, const double* stv,
                                 const double* sub, const double* den, const double* cst,
                                 double* gh_u, double* gh_v, double* Ug, double* Vg);
+__global__ void k_backsub_w(Dims d, StepParams p, Geo geo, const double* gh_u,
+                            const double* gh_v, const double* q, const double* eta_n, double* u,
+                            double* v, double* w);
 __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* gh_u,
                           const double* gh_v, const double* q, const double* eta_n,
                           double* u_it, double* v_it);

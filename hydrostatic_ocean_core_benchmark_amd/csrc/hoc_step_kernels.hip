@@ -1248,6 +1248,55 @@ __global__ void k_backsub(Dims d, StepParams p, Geo geo, const double* __restric
   v_it[t] = (gh_v[t] - p.c_bs * q[t] * gy * (m3v ? 1.0 : 0.0)) * mm;
 }
 
+// The last Picard pass's back-substitution fused with w of the new
+// velocity (model3d_v05.py:404-406 then 232-235), one lane per column
+// marching up from the bottom: u and v of the column are formed and stored
+// as k_backsub forms them; the west u and the south v that the transport
+// divergence needs are formed again from the same operands (the same bits),
+// so w is k_w_diag's of the stored fields without reading them back.
+__global__ void k_backsub_w(Dims d, StepParams p, Geo geo, const double* __restrict__ gh_u,
+                            const double* __restrict__ gh_v, const double* __restrict__ q,
+                            const double* __restrict__ eta_n, double* __restrict__ u,
+                            double* __restrict__ v, double* __restrict__ w) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  // surface-gradient factors of the column and of its west (u) and south (v)
+  // neighbours, as k_backsub forms them
+  const double gx_c = mk(geo.mu2[c]) * (eta_n[ce] - eta_n[c]) / p.dx;
+  const double gx_w = mk(geo.mu2[cw]) * (eta_n[c] - eta_n[cw]) / p.dx;
+  const double gy_c = mk(geo.mv2[c]) * (eta_n[cn] - eta_n[c]) / p.dy;
+  const double gy_s = mk(geo.mv2[cs]) * (eta_n[c] - eta_n[cs]) / p.dy;
+  const ColumnGeo gc(geo, p, d, c), gw(geo, p, d, cw), gs(geo, p, d, cs);
+  const FaceGeo fe(geo, p, d, c, ce, 0), fw(geo, p, d, cw, c, 0);
+  const FaceGeo fn(geo, p, d, c, cn, 1), fs(geo, p, d, cs, c, 1);
+  double sum = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long o = (long long)k * n2;
+    const double mc = gc.wet(k) ? 1.0 : 0.0;
+    const double uc = (gh_u[o + c] - p.c_bs * q[o + c] * gx_c * (fe.wet(k) ? 1.0 : 0.0)) * mc;
+    const double vc = (gh_v[o + c] - p.c_bs * q[o + c] * gy_c * (fn.wet(k) ? 1.0 : 0.0)) * mc;
+    const double uw = (gh_u[o + cw] - p.c_bs * q[o + cw] * gx_w * (fw.wet(k) ? 1.0 : 0.0)) *
+                      (gw.wet(k) ? 1.0 : 0.0);
+    const double vs = (gh_v[o + cs] - p.c_bs * q[o + cs] * gy_s * (fs.wet(k) ? 1.0 : 0.0)) *
+                      (gs.wet(k) ? 1.0 : 0.0);
+    u[o + c] = uc;
+    v[o + c] = vc;
+    const double U = fe.dz(k) * uc;
+    const double Uw = fw.dz(k) * uw;
+    const double V = fn.dz(k) * vc;
+    const double Vs = fs.dz(k) * vs;
+    const double dv = mc * ((U - Uw) / p.dx + (V - Vs) / p.dy);
+    sum = (k == nz - 1) ? dv : sum + dv;
+    w[o + c] = -sum;
+  }
+  w[(long long)nz * n2 + c] = 0.0;
+}
+
 // ------------------------------------------------------------- tracers
 // Flux-form tracer advection divergence at (k, j, i) with face thickness
 // (libs/core/model3d_v05.py:482-512).

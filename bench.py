@@ -110,6 +110,8 @@ def parse():
                     help="Helmholtz solver kind (default: the workload's)")
     ap.add_argument("--mg-levels", type=int, default=None, help="pcg_mg levels (default: the workload's)")
     ap.add_argument("--mg-sweeps", type=int, default=None, help="pcg_mg sweeps per side")
+    ap.add_argument("--mg-smoother", choices=("jacobi", "rbgs"), default=None,
+                    help="pcg_mg V-cycle smoother (default: the workload's)")
     ap.add_argument("--mg-tile", type=int, default=None,
                     help="fused V-cycle leg tile shape 0..3 (32x16, 64x8, 16x16, 64x16); "
                          "default: the library's")
@@ -351,10 +353,12 @@ LADDER = {
 LADDER_DEFAULT = tuple(LADDER)
 
 
-def reference_ladder(names, steps: int = 50, repeats: int = 3) -> dict:
+def reference_ladder(names, steps: int = 50, repeats: int = 5) -> dict:
     """The reference's hardware-ladder workloads on this GPU: per workload, a
     5-step warm-up, then `repeats` runs of `steps` steps from the initial
-    state, median ms/step, beside the published H100 CUDA figure."""
+    state (mixing coefficients reset too), median ms/step and its MAD, beside
+    the published H100 CUDA figure (the reference's protocol, RULES.md:61-98:
+    at least 5 repeats, median and MAD)."""
     import numpy as np
     import torch
     from hydrostatic_ocean_core_benchmark_amd.cases import ladder_lock_exchange
@@ -378,6 +382,7 @@ def reference_ladder(names, steps: int = 50, repeats: int = 3) -> dict:
             return cur
 
         run(5)
+        st.reset_mixing()
         torch.cuda.synchronize()
         setup_s = time.perf_counter() - t_setup
         walls, it0 = [], st.solver_iterations
@@ -387,11 +392,14 @@ def reference_ladder(names, steps: int = 50, repeats: int = 3) -> dict:
             fin = run(steps)
             torch.cuda.synchronize()
             walls.append(time.perf_counter() - t0)
+            st.reset_mixing()
         ms = 1e3 * float(np.median(walls)) / steps
+        mad = 1e3 * float(np.median(np.abs(np.asarray(walls) - np.median(walls)))) / steps
         cells = nx * nx * 30
         out["rows"].append({
             "workload": name, "grid": f"{nx}x{nx}x30", "scheme": prob.meta["scheme"],
             "cfl": cfl, "dt": prob.dt, "ms_per_step": round(ms, 3),
+            "mad_ms_per_step": round(mad, 4),
             "cell_steps_per_s": cells / (ms / 1e3),
             "solver_iterations_per_step": (st.solver_iterations - it0) / (repeats * steps),
             "solver_failures": st.solver_failures,
@@ -436,6 +444,8 @@ def main() -> int:
         prob.params.mg_levels = args.mg_levels
     if args.mg_sweeps is not None:
         prob.params.mg_sweeps = args.mg_sweeps
+    if args.mg_smoother is not None:
+        prob.params.mg_smoother = args.mg_smoother
     g = prob.domain.grid
     st = HipStepper(prob.domain, prob.physics, prob.params, prob.dt, precision=args.precision)
     st.solver_path = args.solver_path

@@ -37,7 +37,7 @@ class HocConfig(C.Structure):
         ("closure_tke", i32), ("mxl_recursive", i32),
         ("c_k", f64), ("c_eps", f64), ("pr_t", f64), ("kappa_vk", f64), ("z_0", f64),
         ("e_min", f64), ("n2_min", f64), ("l_min", f64), ("e_bb", f64),
-        ("precision", i32),
+        ("precision", i32), ("mg_smoother", i32),
     ]
 
 

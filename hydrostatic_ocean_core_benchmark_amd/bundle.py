@@ -164,7 +164,8 @@ def parse_bundle(path: str | Path) -> Bundle:
                           advection=s("advection", "none"), n_split=inum("n_split", 0),
                           solver=s("solver_kind", "pcg_jacobi"), rtol=num("rtol", 1e-12),
                           max_iter=inum("max_iter", 2000), mg_levels=inum("mg_levels", 5),
-                          mg_sweeps=inum("mg_sweeps", 2), mg_omega=num("mg_omega", 0.8))
+                          mg_sweeps=inum("mg_sweeps", 2), mg_omega=num("mg_omega", 0.8),
+                          mg_smoother=s("mg_smoother", "jacobi"))
     try:
         params.validate()
     except ValueError as exc:
@@ -243,7 +244,8 @@ def write_bundle(prefix: str | Path, domain: Domain, physics: PhysicsV05, params
         f"  n_picard = {params.n_picard}", f"  n_split = {params.n_split}",
         f"  solver_kind = '{params.solver}'", f"  rtol = {fd(params.rtol)}",
         f"  max_iter = {params.max_iter}", f"  mg_levels = {params.mg_levels}",
-        f"  mg_sweeps = {params.mg_sweeps}", f"  mg_omega = {fd(params.mg_omega)}", "/",
+        f"  mg_sweeps = {params.mg_sweeps}", f"  mg_omega = {fd(params.mg_omega)}",
+        f"  mg_smoother = '{params.mg_smoother}'", "/",
         "&case_nml", f"  case_name = '{case_name}'", "/",
         "&run_nml", f"  dt = {fd(dt)}", f"  t_final = {fd(dt * n_steps)}",
         f"  n_steps = {n_steps}", f"  n_repeat = {n_repeat}", f"  n_warmup = {n_warmup}",

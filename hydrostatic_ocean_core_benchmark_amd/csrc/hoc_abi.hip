@@ -391,6 +391,11 @@ int validate(const hoc_config& c) {
     return fail(HOC_ERR_INVALID, "mixed precision runs the theta scheme with pcg_mg only");
   if (c.precision >= 1 && c.mg_sweeps > MG_FUSED_MAX_SWEEPS)
     return fail(HOC_ERR_INVALID, "mixed precision needs mg_sweeps <= 3");
+  if (c.mg_smoother < 0 || c.mg_smoother > 1)
+    return fail(HOC_ERR_INVALID, "mg_smoother must be jacobi (0) or rbgs (1)");
+  if (c.mg_smoother == 1 && c.solver == SOLVER_PCG_MG &&
+      (c.precision != 0 || c.mg_sweeps > MG_FUSED_MAX_SWEEPS))
+    return fail(HOC_ERR_INVALID, "the rbgs pcg_mg smoother runs in fp64 with mg_sweeps <= 3");
   return HOC_OK;
 }
 
@@ -752,26 +757,41 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
     // runs in one workgroup in LDS
     const int S = h->cfg.mg_sweeps, last = h->nlev - 1;
     const double om = h->cfg.mg_omega;
+    const bool rb = h->cfg.mg_smoother == 1;
     auto level_b = [&](int l) -> const double* { return l == 0 ? r : h->mg_b[l]; };
     const int l0 = h->mg_cycle_l0 >= 0 ? h->mg_cycle_l0 : last;
-    for (int l = 0; l < l0; ++l)
-      launch_mg_down(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1], om, level_b(l),
-                     h->mg_x[l], h->mg_b[l + 1], h->sc);
+    for (int l = 0; l < l0; ++l) {
+      if (rb)
+        launch_mg_down_rb(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1], level_b(l),
+                          h->mg_x[l], h->mg_b[l + 1], h->sc);
+      else
+        launch_mg_down(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1], om, level_b(l),
+                       h->mg_x[l], h->mg_b[l + 1], h->sc);
+    }
     double* ec = l0 == 0 ? z : h->mg_x[l0];
     if (h->mg_cycle_l0 >= 0) {
       MgCycle C{};
       C.nl = last - l0 + 1;
+      C.rb = rb ? 1 : 0;
       for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
       const size_t lds = mg_cycle_layout(C);
       launch_mg_coarse_cycle(C, lds, om, S, level_b(l0), ec, h->sc, s);
+    } else if (rb) {
+      // the last level alone: S forward and S reverse sweeps from zero
+      k_mg_zero<<<blocks(h->lev[last].n), TPB, 0, s>>>(h->lev[last].n, ec, h->sc);
+      for (int k = 0; k < 2 * S; ++k) rbgs_sweep(h, last, level_b(last), ec, k >= S, s);
     } else {
       double* out = vcycle_jacobi(h, last, level_b(last), ec, h->mg_t[last], s);
       if (out != ec) return fail(HOC_ERR_DEVICE, "pcg_mg: coarse result left its buffer");
     }
     for (int l = l0 - 1; l >= 0; --l) {
       double* out = l == 0 ? z : h->mg_t[l];
-      launch_mg_up(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1].nx, om, h->mg_x[l], ec,
-                   level_b(l), out, h->sc);
+      if (rb)
+        launch_mg_up_rb(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1].nx, h->mg_x[l], ec,
+                        level_b(l), out, h->sc);
+      else
+        launch_mg_up(S, h->mg_tile_shape, s, h->lev[l], h->lev[l + 1].nx, om, h->mg_x[l], ec,
+                     level_b(l), out, h->sc);
       ec = out;
     }
   } else {
@@ -1746,6 +1766,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     if (l0 >= 0 && h->nlev - l0 <= MG_CYCLE_LEVELS) {
       MgCycle C{};
       C.nl = h->nlev - l0;
+      C.rb = cfg->mg_smoother == 1 ? 1 : 0;
       for (int l = 0; l < C.nl; ++l) C.lev[l] = h->lev[l0 + l];
       if (mg_cycle_prepare(mg_cycle_layout(C)) == 0) h->mg_cycle_l0 = l0;
       if (cfg->precision >= 1) {

@@ -362,6 +362,7 @@ struct MgCycleT {
   MgLevelT<R> lev[MG_CYCLE_LEVELS];
   int boff[MG_CYCLE_LEVELS], xoff[MG_CYCLE_LEVELS];  // LDS offsets (elements) of b, x_S
   int nl;
+  int rb;  // 1: red-black Gauss-Seidel sweeps (a third work vector in LDS), 0: damped Jacobi
 };
 using MgCycle = MgCycleT<double>;
 using MgCycleF = MgCycleT<float>;
@@ -372,6 +373,12 @@ int mg_cycle_prepare_f32(size_t lds);    // the same for the fp32 cycles
 void launch_mg_coarse_cycle(const MgCycle& C, size_t lds, double omega, int S, const double* b0,
                             double* out, const PcgScalars* sc, hipStream_t s);
 constexpr int MG_TILE_SHAPES = 4;  // fused-leg tile shapes (32x16, 64x8, 16x16, 64x16)
+// The pcg_mg V-cycle legs with red-black Gauss-Seidel smoothing (S = 1..3).
+void launch_mg_down_rb(int S, int shape, hipStream_t s, const MgLevel& L, const MgLevel& C,
+                       const double* b, double* xs_out, double* bc, const PcgScalars* sc);
+void launch_mg_up_rb(int S, int shape, hipStream_t s, const MgLevel& L, int cnx,
+                     const double* xs_in, const double* ec, const double* b, double* out,
+                     const PcgScalars* sc);
 void launch_mg_down(int S, int shape, hipStream_t s, const MgLevel& L, const MgLevel& C,
                     double omega, const double* b, double* xs_out, double* bc,
                     const PcgScalars* sc);

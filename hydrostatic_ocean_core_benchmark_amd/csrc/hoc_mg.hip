@@ -52,6 +52,16 @@ __device__ __forceinline__ R jacobi_vals(const MgLevelT<R>& L, R omega, long lon
   return x0 + omega * (L.inv_diag[c] * (b - ax)) * mkr<R>(L.mask[c]);
 }
 
+// Red-black Gauss-Seidel update at c: inv_diag (b + neighbour_sum(x)), the
+// neighbours' values unmasked (solvers_var.py:55-69), in the operand order
+// of mg_neighbour_sum.
+template <class R>
+__device__ __forceinline__ R rbgs_vals(const MgLevelT<R>& L, long long c, long long cw,
+                                       long long cs, R b, R x_e, R x_w, R x_n, R x_s) {
+  return L.inv_diag[c] * (b + (L.cx * (L.Ku[c] * x_e + L.Ku[cw] * x_w) +
+                               L.cy * (L.Kv[c] * x_n + L.Kv[cs] * x_s)));
+}
+
 struct Nbr {
   long long c, ce, cw, cn, cs;
 };
@@ -399,11 +409,130 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R 
   }
 }
 
+// ---------------------------------------- fused legs, RBGS smoothing
+// The pcg_mg V-cycle with S red-black Gauss-Seidel sweeps per side (red then
+// black on the way down, black then red on the way up; the wet mask after
+// each sweep, as rbgs_sweep): each half-sweep updates one colour from the
+// values of the previous half-sweep, evaluated on the tile plus the halo the
+// later half-sweeps still need (radius 1 per remaining half-sweep), with the
+// two LDS buffers alternating per half-sweep. Each thread takes a pair of
+// neighbouring cells along x, which on the even-sided levels these legs run
+// on (every level above the coarse cycle is coarsened, so both its sides
+// are even) holds one cell of each colour: one update per thread and
+// half-sweep, no lane idles on the other colour.
+
+// One half-sweep of colour `colour` on the region of halo h (H: the tile's
+// LDS halo), src -> dst; the wet mask applied after the sweep's second
+// colour (mask_all).
+template <int H, int TW, int TH>
+__device__ __forceinline__ void rb_half_sweep(const MgLevel& L, const double* __restrict__ b,
+                                              int i0, int j0, int h, int colour, bool mask_all,
+                                              const double* src, double* dst) {
+  constexpr int RW = TW + 2 * H;
+  const int w2 = (TW + 2 * h) / 2, hh = TH + 2 * h;
+  for (int t = threadIdx.x; t < w2 * hh; t += MG_THREADS) {
+    const int lj = H - h + t / w2;
+    const int li0 = H - h + 2 * (t % w2);
+    const int gj = mg_pmod(j0 - H + lj, L.ny);
+    const int gi0 = mg_pmod(i0 - H + li0, L.nx);
+    const int gi1 = wrap(gi0, 1, L.nx);
+    const int a = ((gi0 + gj) & 1) == colour ? 0 : 1;  // the pair's cell of this colour
+    const int gi = a ? gi1 : gi0, go = a ? gi0 : gi1;
+    const long long row = (long long)gj * L.nx;
+    const long long c = row + gi, co = row + go;
+    const long long cw = row + wrap(gi, -1, L.nx);
+    const long long cs = (long long)wrap(gj, -1, L.ny) * L.nx + gi;
+    const int q = lj * RW + li0 + a, qo = lj * RW + li0 + 1 - a;
+    const double v = rbgs_vals<double>(L, c, cw, cs, b[c], src[q + 1], src[q - 1], src[q + RW],
+                                       src[q - RW]);
+    const double vo = src[qo];
+    dst[q] = mask_all ? v * mkr<double>(L.mask[c]) : v;
+    dst[qo] = mask_all ? vo * mkr<double>(L.mask[co]) : vo;
+  }
+}
+
+// Way down: x_S = S sweeps from zero (written out for the way up) and
+// bc = restrict((b - A x_S) mask).
+template <int S, int TW = MG_TW, int TH = MG_TH>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_down_rb(MgLevel L, MgLevel C,
+                                                           const double* __restrict__ b,
+                                                           double* __restrict__ xs_out,
+                                                           double* __restrict__ bc,
+                                                           const PcgScalars* sc) {
+  if (finished(sc)) return;
+  constexpr int H = 2 * S + 1;  // 2S half-sweeps, then the residual's stencil
+  constexpr int RW = TW + 2 * H, RH = TH + 2 * H;
+  __shared__ double buf[2][RH * RW];
+  const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
+  for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) buf[0][t] = 0.0;
+  __syncthreads();
+  for (int ph = 1; ph <= 2 * S; ++ph) {  // red first
+    rb_half_sweep<H, TW, TH>(L, b, i0, j0, H - ph, (ph - 1) & 1, (ph & 1) == 0,
+                             buf[(ph - 1) & 1], buf[ph & 1]);
+    __syncthreads();
+  }
+  const double* xs = buf[0];  // 2S half-sweeps end in buf[0]
+  for (int t = threadIdx.x; t < TW * TH; t += MG_THREADS) {
+    const int gi = i0 + t % TW, gj = j0 + t / TW;
+    if (gi < L.nx && gj < L.ny)
+      xs_out[(long long)gj * L.nx + gi] = xs[(H + t / TW) * RW + H + t % TW];
+  }
+  constexpr int CW = TW / 2, CH = TH / 2;
+  for (int t = threadIdx.x; t < CW * CH; t += MG_THREADS) {
+    const int I = i0 / 2 + t % CW, J = j0 / 2 + t / CW;
+    if (I >= C.nx || J >= C.ny) continue;
+    double r[4];
+    for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
+      const int a = qd & 1, bq = qd >> 1;
+      const int gi = 2 * I + a, gj = 2 * J + bq;
+      const Nbr n = nbr_of(L, gi, gj);
+      const int q = (H + gj - j0) * RW + H + gi - i0;
+      const double ax = apply_vals<double>(L, n.c, n.ce, n.cw, n.cn, n.cs, xs[q], xs[q + 1],
+                                           xs[q - 1], xs[q + RW], xs[q - RW]);
+      r[qd] = (b[n.c] - ax) * mkr<double>(L.mask[n.c]);
+    }
+    bc[(long long)J * C.nx + I] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
+  }
+}
+
+// Way up: x = x_S + prolong(ec) mask, then S reversed sweeps; out = x.
+template <int S, int TW = MG_TW, int TH = MG_TH>
+__global__ __launch_bounds__(MG_THREADS) void k_mg_up_rb(MgLevel L, int cnx,
+                                                         const double* __restrict__ xs_in,
+                                                         const double* __restrict__ ec,
+                                                         const double* __restrict__ b,
+                                                         double* __restrict__ out,
+                                                         const PcgScalars* sc) {
+  if (finished(sc)) return;
+  constexpr int H = 2 * S;
+  constexpr int RW = TW + 2 * H, RH = TH + 2 * H;
+  __shared__ double buf[2][RH * RW];
+  const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
+  for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) {
+    const int gi = mg_pmod(i0 - H + t % RW, L.nx), gj = mg_pmod(j0 - H + t / RW, L.ny);
+    const long long c = (long long)gj * L.nx + gi;
+    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * mkr<double>(L.mask[c]);
+  }
+  __syncthreads();
+  for (int ph = 1; ph <= 2 * S; ++ph) {  // black first
+    rb_half_sweep<H, TW, TH>(L, b, i0, j0, H - ph, ph & 1, (ph & 1) == 0, buf[(ph - 1) & 1],
+                             buf[ph & 1]);
+    __syncthreads();
+  }
+  const double* x = buf[0];
+  for (int t = threadIdx.x; t < TW * TH; t += MG_THREADS) {
+    const int gi = i0 + t % TW, gj = j0 + t / TW;
+    if (gi < L.nx && gj < L.ny) out[(long long)gj * L.nx + gi] = x[(H + t / TW) * RW + H + t % TW];
+  }
+}
+
 // The coarse end of the pcg_mg V-cycle in one workgroup, every vector in
 // LDS: levels C.lev[0..nl) (the first with at most MG_COARSE_MAX cells, or
 // the last level alone), S sweeps from zero + residual restriction down, 2S
 // sweeps from zero on the last level, prolongation + S sweeps up; the same
-// arithmetic per cell as the tiled legs (jacobi_vals, apply_vals). b0 is
+// arithmetic per cell as the tiled legs (jacobi_vals or rbgs_vals,
+// apply_vals). With C.rb the sweeps are red-black Gauss-Seidel, reversed
+// (black first) on the way up and in the last level's second S. b0 is
 // level 0's right-hand side (staged in LDS), out receives level 0's result.
 template <class R, class RB, class RO>
 __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
@@ -415,16 +544,44 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   const int n0 = (int)C.lev[0].n;
   R* wA = cyc;
   R* wB = cyc + n0;
+  R* wC = cyc + 2 * n0;  // red-black sweeps only: the field between the two colours
   const int tid = threadIdx.x;
-  // one damped-Jacobi sweep on level L from src (zero if null) into dst
-  auto sweep = [&](const MgLevelT<R>& L, const R* b, const R* src, R* dst) {
+  // one sweep on level L from src (zero if null) into dst: damped Jacobi,
+  // or red-black Gauss-Seidel (red first unless reverse) through wC
+  auto sweep = [&](const MgLevelT<R>& L, const R* b, const R* src, R* dst, bool reverse) {
     const int n = (int)L.n;
+    if (!C.rb) {
+      for (int c = tid; c < n; c += MG_COARSE_THREADS) {
+        const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
+        dst[c] = src ? jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c],
+                                      src[nb.ce], src[nb.cw], src[nb.cn], src[nb.cs], b[c])
+                     : jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, R(0), R(0), R(0),
+                                      R(0), R(0), b[c]);
+      }
+      __syncthreads();
+      return;
+    }
+    const int first = reverse ? 1 : 0;
     for (int c = tid; c < n; c += MG_COARSE_THREADS) {
-      const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
-      dst[c] = src ? jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c], src[nb.ce],
-                                    src[nb.cw], src[nb.cn], src[nb.cs], b[c])
-                   : jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, R(0), R(0), R(0),
-                                    R(0), R(0), b[c]);
+      const int i = c % L.nx, j = c / L.nx;
+      R v = src ? src[c] : R(0);
+      if (((i + j) & 1) == first) {
+        const Nbr nb = nbr_of(L, i, j);
+        v = src ? rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], src[nb.ce], src[nb.cw], src[nb.cn],
+                               src[nb.cs])
+                : rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], R(0), R(0), R(0), R(0));
+      }
+      wC[c] = v;
+    }
+    __syncthreads();
+    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
+      const int i = c % L.nx, j = c / L.nx;
+      R v = wC[c];
+      if (((i + j) & 1) != first) {
+        const Nbr nb = nbr_of(L, i, j);
+        v = rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], wC[nb.ce], wC[nb.cw], wC[nb.cn], wC[nb.cs]);
+      }
+      dst[c] = v * mkr<R>(L.mask[c]);
     }
     __syncthreads();
   };
@@ -444,7 +601,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
     const R* src = nullptr;
     for (int k = 1; k <= S; ++k) {
       R* dst = (k == S) ? xs : ((k & 1) ? wA : wB);
-      sweep(L, b, src, dst);
+      sweep(L, b, src, dst, false);
       src = dst;
     }
     const MgLevelT<R>& Cl = C.lev[l + 1];
@@ -464,7 +621,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
     }
     __syncthreads();
   }
-  // the last level: 2S sweeps from zero
+  // the last level: 2S sweeps from zero (the second S reversed)
   R* e = nullptr;
   {
     const int l = C.nl - 1;
@@ -473,7 +630,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
     const R* src = nullptr;
     for (int k = 1; k <= 2 * S; ++k) {
       R* dst = (k & 1) ? wA : wB;
-      sweep(L, b, src, dst);
+      sweep(L, b, src, dst, k > S);
       src = dst;
       e = dst;
     }
@@ -495,7 +652,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
     const R* src = xp;
     for (int k = 1; k <= S; ++k) {
       R* dst = (k & 1) ? other : xp;
-      sweep(L, b, src, dst);
+      sweep(L, b, src, dst, true);
       src = dst;
       e = dst;
     }
@@ -506,7 +663,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
 template <class R>
 size_t mg_cycle_layout(MgCycleT<R>& C) {
   const int n0 = (int)C.lev[0].n;
-  int off = 2 * n0;
+  int off = (C.rb ? 3 : 2) * n0;
   for (int l = 0; l < C.nl; ++l) {
     C.boff[l] = off;
     off += (int)C.lev[l].n;
@@ -608,6 +765,43 @@ void launch_mg_up(int S, int shape, hipStream_t s, const MgLevel& L, int cnx, do
   else
     k_mg_up<2, double, double, double><<<grid, MG_THREADS, 0, s>>>(L, cnx, omega, xs_in, ec, b,
                                                                    out, sc);
+}
+
+void launch_mg_down_rb(int S, int shape, hipStream_t s, const MgLevel& L, const MgLevel& C,
+                       const double* b, double* xs_out, double* bc, const PcgScalars* sc) {
+  if (S != 2) shape = 0;
+  const dim3 grid = mg_tile_grid_shape(L, shape);
+  if (S == 1)
+    k_mg_down_rb<1><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+  else if (S == 3)
+    k_mg_down_rb<3><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+  else if (shape == 1)
+    k_mg_down_rb<2, 64, 8><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+  else if (shape == 2)
+    k_mg_down_rb<2, 16, 16><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+  else if (shape == 3)
+    k_mg_down_rb<2, 64, 16><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+  else
+    k_mg_down_rb<2><<<grid, MG_THREADS, 0, s>>>(L, C, b, xs_out, bc, sc);
+}
+
+void launch_mg_up_rb(int S, int shape, hipStream_t s, const MgLevel& L, int cnx,
+                     const double* xs_in, const double* ec, const double* b, double* out,
+                     const PcgScalars* sc) {
+  if (S != 2) shape = 0;
+  const dim3 grid = mg_tile_grid_shape(L, shape);
+  if (S == 1)
+    k_mg_up_rb<1><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
+  else if (S == 3)
+    k_mg_up_rb<3><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
+  else if (shape == 1)
+    k_mg_up_rb<2, 64, 8><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
+  else if (shape == 2)
+    k_mg_up_rb<2, 16, 16><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
+  else if (shape == 3)
+    k_mg_up_rb<2, 64, 16><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
+  else
+    k_mg_up_rb<2><<<grid, MG_THREADS, 0, s>>>(L, cnx, xs_in, ec, b, out, sc);
 }
 
 namespace {

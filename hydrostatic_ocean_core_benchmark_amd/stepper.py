@@ -68,6 +68,9 @@ class HipStepper:
                              "(expected fp64|mixed|mixed_fp32_momentum)")
         if precision != "fp64" and params.name == "theta" and params.solver != "pcg_mg":
             raise ValueError("mixed precision runs the theta scheme with pcg_mg only")
+        if (getattr(params, "mg_smoother", "jacobi") == "rbgs" and params.solver == "pcg_mg"
+                and (precision != "fp64" or int(getattr(params, "mg_sweeps", 2)) > 3)):
+            raise ValueError("the rbgs pcg_mg smoother runs in fp64 with mg_sweeps <= 3")
         self.precision = precision
         if params.name == "theta" and params.solver == "fft" and not domain.is_flat_periodic:
             raise ValueError(
@@ -128,6 +131,7 @@ class HipStepper:
         cfg.mg_levels = int(getattr(params, "mg_levels", 5))
         cfg.mg_sweeps = int(getattr(params, "mg_sweeps", 2))
         cfg.mg_omega = float(getattr(params, "mg_omega", 0.8))
+        cfg.mg_smoother = {"jacobi": 0, "rbgs": 1}[getattr(params, "mg_smoother", "jacobi")]
         cfg.precision = PRECISION[precision]
         cfg.g, cfg.f, cfg.rho0 = p.g, p.f, p.rho0
         cfg.nu_b, cfg.bottom_drag, cfg.N2 = p.nu, p.bottom_drag, p.N2

@@ -125,8 +125,11 @@ class SchemeParams:
     mg_levels: int = 5
     mg_sweeps: int = 2
     mg_omega: float = 0.8
+    mg_smoother: str = "jacobi"   # "jacobi" (damped) or "rbgs" (red-black Gauss-Seidel)
 
     def validate(self) -> None:
+        if self.mg_smoother not in ("jacobi", "rbgs"):
+            raise ValueError(f"unknown pcg_mg smoother '{self.mg_smoother}' (expected jacobi|rbgs)")
         if self.name not in SCHEMES:
             raise ValueError(f"unknown scheme '{self.name}' "
                              f"(expected fb|theta|split_explicit)")
@@ -187,6 +190,7 @@ def scheme_params_from_config(config) -> SchemeParams:
         n_picard=int(g("scheme.n_picard")), solver=str(g("solver.kind")),
         rtol=float(g("solver.rtol")), max_iter=int(g("solver.max_iter")),
         mg_levels=int(g("solver.mg_levels", 5)), mg_sweeps=int(g("solver.mg_sweeps", 2)),
-        mg_omega=float(g("solver.mg_omega", 0.8)))
+        mg_omega=float(g("solver.mg_omega", 0.8)),
+        mg_smoother=str(g("solver.mg_smoother", "jacobi")))
     params.validate()
     return params

@@ -45,7 +45,7 @@ typedef struct hoc_config {
   int32_t solver;          /* 0 pcg_jacobi, 1 pcg_rbgs, 2 rbgs, 3 multigrid, 4 pcg_mg */
   double rtol;
   int32_t max_iter;
-  int32_t mg_levels, mg_sweeps;  /* pcg_mg: levels and damped-Jacobi sweeps per side */
+  int32_t mg_levels, mg_sweeps;  /* pcg_mg: levels and smoothing sweeps per side */
   double mg_omega;               /* pcg_mg: Jacobi damping */
   int32_t n_split;               /* split_explicit: barotropic substeps per step (>= 1) */
   int32_t barotropic_mean;       /* split_explicit: 1 = mean of the substep transports */
@@ -64,6 +64,9 @@ typedef struct hoc_config {
                               preconditioner, the tracer advective fluxes and the EOS in
                               fp32, momentum fluxes, state updates and the outer PCG in
                               fp64; 2 as 1 with the momentum advective fluxes in fp32 */
+  int32_t mg_smoother;     /* pcg_mg smoother: 0 damped Jacobi (mg_omega), 1 red-black
+                              Gauss-Seidel (red then black down, black then red up; fp64,
+                              mg_sweeps <= 3) */
 } hoc_config;
 
 /* Static z-level geometry and surface forcing (host arrays). Built like
@@ -424,7 +427,7 @@ int hoc_m3_tracer(hoc_m3* h, int32_t which, const double* c, const double* u, co
 /* b = g (alpha_T (T - T0) - beta_S (S - S0)) (model3d.py:84-88). */
 int hoc_m3_eos(hoc_m3* h, const double* T, const double* S, double* b, void* stream);
 
-/* Model3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
+/* Stepper3DV05.integrate (libs/core/model3d_v05.py:535-538) on the device: n
  * steps from *a, ping-ponging between the caller's state sets a and b; the
  * final state is in a when n is even, in b when n is odd. */
 int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream);

@@ -4,8 +4,12 @@
 
 Kinds: pcg_jacobi, pcg_rbgs, rbgs, multigrid (the reference's RBGS V-cycle
 iterated as a solver) and pcg_mg (no reference counterpart: PCG whose
-preconditioner is one V-cycle of damped Jacobi on `mg_levels` levels; parity
-unpinned beyond agreeing with the other solvers' solutions).
+preconditioner is one V-cycle on `mg_levels` levels from a zero guess, with
+`mg_sweeps` damped-Jacobi sweeps per side or, with mg_smoother="rbgs",
+red-black Gauss-Seidel sweeps: red then black on the way down, black then
+red on the way up, so that the preconditioner stays symmetric). The pcg_mg
+V-cycle is pinned in tests/test_oracle_pins.py to a dense matrix recursion
+assembled independently of this code.
 """
 
 from __future__ import annotations
@@ -94,7 +98,7 @@ class HelmholtzVar:
 
     def __init__(self, nx, ny, dx, dy, coef, Ku, Kv, mask, kind="pcg_jacobi",
                  rtol=1e-12, max_iter=2000, n_pre=2, n_post=2, min_size=8,
-                 mg_levels=5, mg_sweeps=2, mg_omega=0.8):
+                 mg_levels=5, mg_sweeps=2, mg_omega=0.8, mg_smoother="jacobi"):
         if kind not in self.KINDS:
             raise ValueError(f"unknown variable-coefficient solver '{kind}' "
                              "(expected pcg_jacobi|pcg_rbgs|rbgs|multigrid|pcg_mg)")
@@ -102,6 +106,9 @@ class HelmholtzVar:
         self.rtol, self.max_iter = rtol, max_iter
         self.n_pre, self.n_post, self.min_size = n_pre, n_post, min_size
         self.mg_levels, self.mg_sweeps, self.mg_omega = mg_levels, mg_sweeps, mg_omega
+        if mg_smoother not in ("jacobi", "rbgs"):
+            raise ValueError(f"unknown pcg_mg smoother '{mg_smoother}' (expected jacobi|rbgs)")
+        self.mg_smoother = mg_smoother
         self.masks = colour_masks(ny, nx)
         self.op = VarOperator(nx, ny, dx, dy, coef, Ku, Kv, mask)
         self._build_levels()
@@ -146,15 +153,22 @@ class HelmholtzVar:
 
     def _mg_precond(self, b, lvl=0):
         op = self.levels[lvl]
+        rb = self.mg_smoother == "rbgs"
         x = np.zeros_like(b)
         for _ in range(self.mg_sweeps):
-            x = self._jacobi(x, b, op)
+            if rb:
+                rbgs_sweep(x, b, op, self.level_masks[lvl])
+            else:
+                x = self._jacobi(x, b, op)
         if lvl + 1 < len(self.levels):
             r = (b - op.apply(x)) * op.mask
             e = self._mg_precond(restrict(r), lvl + 1)
             x = x + prolong(e) * op.mask
         for _ in range(self.mg_sweeps):
-            x = self._jacobi(x, b, op)
+            if rb:
+                rbgs_sweep(x, b, op, self.level_masks[lvl], reverse=True)
+            else:
+                x = self._jacobi(x, b, op)
         return x
 
     def _precondition(self, r):

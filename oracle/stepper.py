@@ -108,7 +108,8 @@ class OracleStepper:
                     g.nx, g.ny, g.dx, g.dy, coef, self.Ku, self.Kv, d.mask, kind=pr.solver,
                     rtol=pr.rtol, max_iter=pr.max_iter,
                     mg_levels=getattr(pr, "mg_levels", 5), mg_sweeps=getattr(pr, "mg_sweeps", 2),
-                    mg_omega=getattr(pr, "mg_omega", 0.8))
+                    mg_omega=getattr(pr, "mg_omega", 0.8),
+                    mg_smoother=getattr(pr, "mg_smoother", "jacobi"))
             else:
                 self.solver.update(self.Ku, self.Kv)
                 self.solver_rebuilds += 1

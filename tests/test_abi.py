@@ -102,3 +102,9 @@ def test_stepper_precision_arguments_are_checked_before_any_device_use():
     for prec in ("mixed", "mixed_fp32_momentum"):
         with pytest.raises(ValueError, match="pcg_mg only"):
             HipStepper(dom, phys, params, dt, precision=prec)
+    params.solver, params.mg_smoother = "pcg_mg", "rbgs"
+    with pytest.raises(ValueError, match="rbgs pcg_mg smoother"):
+        HipStepper(dom, phys, params, dt, precision="mixed")
+    params.mg_sweeps = 4
+    with pytest.raises(ValueError, match="rbgs pcg_mg smoother"):
+        HipStepper(dom, phys, params, dt)

@@ -280,6 +280,37 @@ def test_pcg_mg_variants_match_oracle(hoc, shape, bc, levels, sweeps, omega):
     assert np.array_equal(_np(x), want), rel_l2(_np(x), want)
 
 
+@pytest.mark.parametrize("shape,bc,levels,sweeps,tile", [
+    ((512, 512), "closed", 5, 2, 0), ((512, 512), "closed", 5, 2, 1),
+    ((512, 512), "closed", 5, 2, 2), ((512, 512), "closed", 5, 2, 3),
+    ((512, 512), "closed", 5, 1, 2), ((512, 512), "periodic", 5, 3, 2),
+    ((300, 500), "closed", 2, 2, 0), ((200, 136), "periodic", 6, 2, 2),
+    ((64, 64), "closed", 1, 2, 0), ((96, 80), "periodic", 5, 2, 1)])
+def test_pcg_mg_rbgs_smoother_matches_oracle(hoc, shape, bc, levels, sweeps, tile):
+    """pcg_mg with the red-black Gauss-Seidel smoother: the fused legs at
+    1-3 sweeps per side and every tile shape (partial tiles at 96 x 80),
+    the one-workgroup coarse cycle (with odd periodic sides at 200 x 136 ->
+    17 x 25, and a 64 x 64 single level holding three fields in LDS), and a
+    coarse level too large for one workgroup (150 x 250, separate sweeps)."""
+    from hydrostatic_ocean_core_benchmark_amd import _lib
+    from hydrostatic_ocean_core_benchmark_amd.types import Physics3D, PhysicsV05, SchemeParams
+    from oracle.helmholtz import HelmholtzVar
+    grid, dom, mask, Ku, Kv, rhs = _random_helmholtz(shape, bc, seed=13)
+    ny, nx = shape
+    phys = PhysicsV05(base=Physics3D(g=9.80616, f=1e-4, H=4000.0))
+    params = SchemeParams(name="theta", solver="pcg_mg", rtol=1e-12, max_iter=5000,
+                          mg_levels=levels, mg_sweeps=sweeps, mg_smoother="rbgs")
+    st = hoc.HipStepper(dom, phys, params, 600.0)
+    _lib.check(st.lib.hoc_stepper_set_mg_tile(st._h, tile))
+    ref = HelmholtzVar(nx, ny, grid.dx, grid.dy, st.cfg.g * 0.25 * 600.0 ** 2, Ku, Kv, mask,
+                       kind="pcg_mg", rtol=1e-12, max_iter=5000, mg_levels=levels,
+                       mg_sweeps=sweeps, mg_smoother="rbgs")
+    want, rep = ref.solve(rhs)
+    x, iters, conv = st.helmholtz_solve(Ku, Kv, rhs)
+    assert conv == rep.converged and iters == rep.iterations, (iters, rep.iterations)
+    assert np.array_equal(_np(x), want), rel_l2(_np(x), want)
+
+
 @pytest.mark.parametrize("name", list(COMPOSED))
 def test_composed_step_matches_reference(hoc, name):
     """1-3 steps of the reference stepper (each fixture's own solver kind):

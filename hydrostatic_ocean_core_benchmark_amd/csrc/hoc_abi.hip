@@ -178,6 +178,7 @@ struct hoc_stepper {
   static constexpr int MAXLEV = 12;
   int nlev = 1;
   int mg_cycle_l0 = -1;  // first level of the one-workgroup coarse cycle, -1: none
+  bool mg_masks_built = false;  // the coarse levels' wet masks (static) are formed
   MgLevel lev[MAXLEV];
   double *mg_Ku[MAXLEV] = {}, *mg_Kv[MAXLEV] = {}, *mg_inv[MAXLEV] = {};
   uint8_t* mg_mask[MAXLEV] = {};
@@ -638,22 +639,35 @@ int mg_build(hoc_stepper* h, const double* Ku, const double* Kv, const double* i
     L.Kv = h->mg_Kv[l];
     L.mask = h->mg_mask[l];
     L.inv_diag = h->mg_inv[l];
-    k_mg_coarsen<<<blocks(L.n), TPB, 0, s>>>(h->lev[l - 1], L, h->mg_Ku[l], h->mg_Kv[l],
-                                             h->mg_mask[l]);
-    k_mg_diag<<<blocks(L.n), TPB, 0, s>>>(L, h->mg_inv[l]);
+    // the coarse masks follow from the static wet mask: formed on the first
+    // build only (with faces that k_mg_levels then forms again, equal bits)
+    if (!h->mg_masks_built)
+      k_mg_coarsen<<<blocks(L.n), TPB, 0, s>>>(h->lev[l - 1], L, h->mg_Ku[l], h->mg_Kv[l],
+                                               h->mg_mask[l]);
   }
-  if (h->cfg.precision >= 1) {  // the fp32 levels exist for every mixed handle
-    for (int l = 0; l < h->nlev; ++l) {
-      const MgLevel& L = h->lev[l];
+  h->mg_masks_built = true;
+  const bool f32 = h->cfg.precision >= 1;  // the fp32 levels exist for every mixed handle
+  MgBuildArgs A{};
+  if (h->nlev > MG_BUILD_LEVELS) return fail(HOC_ERR_INVALID, "too many multigrid levels");
+  A.nlev = h->nlev;
+  A.l_first = f32 ? 0 : 1;
+  long long nmax = 0;
+  for (int l = 0; l < h->nlev; ++l) {
+    A.lev[l] = h->lev[l];
+    A.Ku[l] = h->mg_Ku[l];
+    A.Kv[l] = h->mg_Kv[l];
+    A.inv[l] = h->mg_inv[l];
+    if (f32) {
       MgLevelF& F = h->levf[l];
-      F.Ku = h->mgf_Ku[l];
-      F.Kv = h->mgf_Kv[l];
-      F.inv_diag = h->mgf_inv[l];
-      F.mask = L.mask;
-      k_mg_to_f32<<<blocks(L.n), TPB, 0, s>>>(L.n, L.Ku, L.Kv, L.inv_diag, h->mgf_Ku[l],
-                                              h->mgf_Kv[l], h->mgf_inv[l]);
+      F.Ku = A.fKu[l] = h->mgf_Ku[l];
+      F.Kv = A.fKv[l] = h->mgf_Kv[l];
+      F.inv_diag = A.finv[l] = h->mgf_inv[l];
+      F.mask = h->lev[l].mask;
     }
+    if (l >= A.l_first) nmax = std::max(nmax, h->lev[l].n);
   }
+  if (A.nlev > A.l_first)
+    k_mg_levels<<<dim3((unsigned)blocks(nmax), (unsigned)(A.nlev - A.l_first)), TPB, 0, s>>>(A);
   LAUNCH_CHECK("multigrid levels");
   return HOC_OK;
 }

@@ -329,6 +329,18 @@ __global__ void k_pcg_totals(PcgArgs a, int which);
 __global__ void k_pcg_rz(PcgArgs a, int into_b);
 __global__ void k_mg_coarsen(MgLevel f, MgLevel c, double* cKu, double* cKv, uint8_t* cm);
 __global__ void k_mg_diag(MgLevel L, double* inv_diag);
+// Every coarse level's faces and 1/diag from level 0 in one launch (blockIdx.y
+// = level - l_first); the coarse masks, which do not change between solves,
+// must already be in lev[l].mask. With fKu set, fp32 copies of each level's
+// faces and 1/diag too (level 0 included when l_first = 0).
+constexpr int MG_BUILD_LEVELS = 12;
+struct MgBuildArgs {
+  MgLevel lev[MG_BUILD_LEVELS];
+  double *Ku[MG_BUILD_LEVELS], *Kv[MG_BUILD_LEVELS], *inv[MG_BUILD_LEVELS];
+  float *fKu[MG_BUILD_LEVELS], *fKv[MG_BUILD_LEVELS], *finv[MG_BUILD_LEVELS];
+  int nlev, l_first;
+};
+__global__ void k_mg_levels(MgBuildArgs A);
 __global__ void k_mg_zero(long long n, double* x, const PcgScalars* sc);
 constexpr int RB_TW = 32, RB_TH = 8;
 __global__ void k_mg_rbgs2(MgLevel L, int c0, const double* b, const double* xin, double* xout,
@@ -363,6 +375,10 @@ struct MgCycleT {
   int boff[MG_CYCLE_LEVELS], xoff[MG_CYCLE_LEVELS];  // LDS offsets (elements) of b, x_S
   int nl;
   int rb;  // 1: red-black Gauss-Seidel sweeps (a third work vector in LDS), 0: damped Jacobi
+  // 1: the levels' Ku, Kv, 1/diag (at coff, elements) and masks (at moff, bytes) are
+  // copied into LDS too, when they fit beside the vectors
+  int stage;
+  int coff[MG_CYCLE_LEVELS], moff[MG_CYCLE_LEVELS];
 };
 using MgCycle = MgCycleT<double>;
 using MgCycleF = MgCycleT<float>;

@@ -25,22 +25,32 @@ __device__ __forceinline__ R mkr(uint8_t m) {
 // A x at c from the values at c and its four neighbours, x masked first
 // (solvers_var.py:47-53); ce/cw/cn/cs are the neighbours' indices. In the
 // level's type R throughout (for R = double the reference's arithmetic).
+// The same from the operands themselves: the faces' Ku[c], Ku[cw], Kv[c],
+// Kv[cs] and the five cells' masks (as 0/1 values of R).
+template <class R>
+__device__ __forceinline__ R apply_from(R coef, R dx2, R dy2, R ku_c, R ku_w, R kv_c, R kv_s,
+                                        R mc, R me, R mw, R mn, R ms, R x0, R x_e, R x_w, R x_n,
+                                        R x_s) {
+  const R xc = x0 * mc;
+  const R xe = x_e * me;
+  const R xw = x_w * mw;
+  const R xn = x_n * mn;
+  const R xs = x_s * ms;
+  const R fx = ku_c * (xe - xc);
+  const R fxw = ku_w * (xc - xw);
+  const R fy = kv_c * (xn - xc);
+  const R fys = kv_s * (xc - xs);
+  const R lap = (fx - fxw) / dx2 + (fy - fys) / dy2;
+  return (xc - coef * lap) * mc + xc * (R(1) - mc);
+}
+
 template <class R>
 __device__ __forceinline__ R apply_vals(const MgLevelT<R>& L, long long c, long long ce,
                                         long long cw, long long cn, long long cs, R x0, R x_e,
                                         R x_w, R x_n, R x_s) {
-  const R mc = mkr<R>(L.mask[c]);
-  const R xc = x0 * mc;
-  const R xe = x_e * mkr<R>(L.mask[ce]);
-  const R xw = x_w * mkr<R>(L.mask[cw]);
-  const R xn = x_n * mkr<R>(L.mask[cn]);
-  const R xs = x_s * mkr<R>(L.mask[cs]);
-  const R fx = L.Ku[c] * (xe - xc);
-  const R fxw = L.Ku[cw] * (xc - xw);
-  const R fy = L.Kv[c] * (xn - xc);
-  const R fys = L.Kv[cs] * (xc - xs);
-  const R lap = (fx - fxw) / L.dx2 + (fy - fys) / L.dy2;
-  return (xc - L.coef * lap) * mc + xc * (R(1) - mc);
+  return apply_from<R>(L.coef, L.dx2, L.dy2, L.Ku[c], L.Ku[cw], L.Kv[c], L.Kv[cs],
+                       mkr<R>(L.mask[c]), mkr<R>(L.mask[ce]), mkr<R>(L.mask[cw]),
+                       mkr<R>(L.mask[cn]), mkr<R>(L.mask[cs]), x0, x_e, x_w, x_n, x_s);
 }
 
 // Damped Jacobi at c: x + omega (inv_diag (b - A x)) mask.
@@ -56,10 +66,16 @@ __device__ __forceinline__ R jacobi_vals(const MgLevelT<R>& L, R omega, long lon
 // neighbours' values unmasked (solvers_var.py:55-69), in the operand order
 // of mg_neighbour_sum.
 template <class R>
+__device__ __forceinline__ R rbgs_from(R inv, R b, R cx, R cy, R ku_c, R ku_w, R kv_c, R kv_s,
+                                       R x_e, R x_w, R x_n, R x_s) {
+  return inv * (b + (cx * (ku_c * x_e + ku_w * x_w) + cy * (kv_c * x_n + kv_s * x_s)));
+}
+
+template <class R>
 __device__ __forceinline__ R rbgs_vals(const MgLevelT<R>& L, long long c, long long cw,
                                        long long cs, R b, R x_e, R x_w, R x_n, R x_s) {
-  return L.inv_diag[c] * (b + (L.cx * (L.Ku[c] * x_e + L.Ku[cw] * x_w) +
-                               L.cy * (L.Kv[c] * x_n + L.Kv[cs] * x_s)));
+  return rbgs_from<R>(L.inv_diag[c], b, L.cx, L.cy, L.Ku[c], L.Ku[cw], L.Kv[c], L.Kv[cs], x_e,
+                      x_w, x_n, x_s);
 }
 
 struct Nbr {
@@ -127,6 +143,90 @@ __global__ void k_mg_diag(MgLevel L, double* __restrict__ inv_diag) {
   double dg = 1.0 + L.cx * (L.Ku[c] + L.Ku[cw]) + L.cy * (L.Kv[c] + L.Kv[cs]);
   dg = L.mask[c] ? dg : 1.0;
   inv_diag[c] = 1.0 / dg;
+}
+
+// The face coefficient of a level `depth` levels below the fine one: the
+// fine faces v[0], v[stride], ..., v[(2^depth - 1) stride] along the coarse
+// cell's edge, averaged pairwise as k_mg_coarsen's recursion does, level by
+// level (0.5 (left + right) at each node of the balanced tree).
+__device__ __forceinline__ double mean_tree(const double* __restrict__ v, long long stride,
+                                            int depth) {
+  double st[MG_BUILD_LEVELS];
+  const int len = 1 << depth;
+  for (int q = 0; q < len; ++q) {
+    double x = v[(long long)q * stride];
+    int lv = 0;
+    for (int qq = q; qq & 1; qq >>= 1) x = 0.5 * (st[lv++] + x);
+    st[lv] = x;
+  }
+  return st[depth];
+}
+
+// The same tree for a depth known at compile time: every load issued at
+// once, then the combine levels.
+template <int D>
+__device__ __forceinline__ double mean_tree_d(const double* __restrict__ v, long long stride) {
+  constexpr int N = 1 << D;
+  double x[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) x[q] = v[(long long)q * stride];
+#pragma unroll
+  for (int w = N; w > 1; w >>= 1)
+#pragma unroll
+    for (int q = 0; q < w / 2; ++q) x[q] = 0.5 * (x[2 * q] + x[2 * q + 1]);
+  return x[0];
+}
+
+__device__ __forceinline__ double mean_tree_any(const double* __restrict__ v, long long stride,
+                                                int depth) {
+  switch (depth) {
+    case 1: return mean_tree_d<1>(v, stride);
+    case 2: return mean_tree_d<2>(v, stride);
+    case 3: return mean_tree_d<3>(v, stride);
+    case 4: return mean_tree_d<4>(v, stride);
+    case 5: return mean_tree_d<5>(v, stride);
+    default: return mean_tree(v, stride, depth);
+  }
+}
+
+// Level l's Ku at (J, I) is the tree over the level-0 Ku of column
+// (I + 1) 2^l - 1, rows J 2^l .. (J + 1) 2^l - 1; its Kv the tree over the
+// level-0 Kv of row (J + 1) 2^l - 1, columns I 2^l .. (I + 1) 2^l - 1. Every
+// level halves even sides, so level 0 is exactly 2^l times as wide.
+__global__ void k_mg_levels(MgBuildArgs A) {
+  const int l = (int)blockIdx.y + A.l_first;
+  if (l >= A.nlev) return;
+  const MgLevel& L = A.lev[l];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.n) return;
+  if (l == 0) {
+    A.fKu[0][t] = (float)L.Ku[t];
+    A.fKv[0][t] = (float)L.Kv[t];
+    A.finv[0][t] = (float)L.inv_diag[t];
+    return;
+  }
+  const MgLevel& F = A.lev[0];
+  const int I = (int)(t % L.nx), J = (int)(t / L.nx);
+  const long long span = 1ll << l;
+  auto ku_at = [&](int jj, int ii) {
+    return mean_tree_any(F.Ku + jj * span * F.nx + (ii + 1) * span - 1, F.nx, l);
+  };
+  auto kv_at = [&](int jj, int ii) {
+    return mean_tree_any(F.Kv + ((jj + 1) * span - 1) * F.nx + ii * span, 1, l);
+  };
+  const double ku = ku_at(J, I), kv = kv_at(J, I);
+  const double kuw = ku_at(J, wrap(I, -1, L.nx)), kvs = kv_at(wrap(J, -1, L.ny), I);
+  double dg = 1.0 + L.cx * (ku + kuw) + L.cy * (kv + kvs);
+  dg = L.mask[t] ? dg : 1.0;
+  const double inv = 1.0 / dg;
+  A.Ku[l][t] = ku;
+  A.Kv[l][t] = kv;
+  A.inv[l][t] = inv;
+  if (A.fKu[l]) {
+    A.fKu[l][t] = (float)ku;
+    A.fKv[l][t] = (float)kv;
+    A.finv[l][t] = (float)inv;
+  }
 }
 
 __global__ void k_mg_zero(long long n, double* __restrict__ x, const PcgScalars* sc) {
@@ -415,39 +515,59 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up(MgLevelT<R> L, int cnx, R 
 // each sweep, as rbgs_sweep): each half-sweep updates one colour from the
 // values of the previous half-sweep, evaluated on the tile plus the halo the
 // later half-sweeps still need (radius 1 per remaining half-sweep), with the
-// two LDS buffers alternating per half-sweep. Each thread takes a pair of
-// neighbouring cells along x, which on the even-sided levels these legs run
-// on (every level above the coarse cycle is coarsened, so both its sides
-// are even) holds one cell of each colour: one update per thread and
-// half-sweep, no lane idles on the other colour.
+// two LDS buffers alternating per half-sweep. The level's coefficients and
+// right-hand side over the tile's region are staged in LDS once, so the
+// half-sweeps read no global memory. Each thread takes a pair of
+// neighbouring cells along x, one of each colour: one update per thread and
+// half-sweep, no lane idles on the other colour. These legs run on the
+// levels above the coarse cycle, which are all coarsened and so have even
+// sides: a cell's colour is then the parity of its unwrapped coordinates.
 
-// One half-sweep of colour `colour` on the region of halo h (H: the tile's
-// LDS halo), src -> dst; the wet mask applied after the sweep's second
-// colour (mask_all).
+// A tile region's copy of the level: face coefficients, 1/diag, right-hand
+// side, wet mask (0/1).
+template <int RN>
+struct RbStage {
+  double ku[RN], kv[RN], inv[RN], b[RN], m[RN];
+};
+
+// Stage the region of halo H around the tile at (i0, j0), positions wrapping
+// like the operator.
 template <int H, int TW, int TH>
-__device__ __forceinline__ void rb_half_sweep(const MgLevel& L, const double* __restrict__ b,
-                                              int i0, int j0, int h, int colour, bool mask_all,
-                                              const double* src, double* dst) {
+__device__ __forceinline__ void rb_stage(RbStage<(TW + 2 * H) * (TH + 2 * H)>& st,
+                                         const MgLevel& L, const double* __restrict__ b, int i0,
+                                         int j0) {
+  constexpr int RW = TW + 2 * H, RN = RW * (TH + 2 * H);
+  for (int t = threadIdx.x; t < RN; t += MG_THREADS) {
+    const int gi = mg_pmod(i0 - H + t % RW, L.nx), gj = mg_pmod(j0 - H + t / RW, L.ny);
+    const long long c = (long long)gj * L.nx + gi;
+    st.ku[t] = L.Ku[c];
+    st.kv[t] = L.Kv[c];
+    st.inv[t] = L.inv_diag[c];
+    st.b[t] = b[c];
+    st.m[t] = mkr<double>(L.mask[c]);
+  }
+}
+
+// One half-sweep of colour `colour` on the region of halo h, src -> dst; the
+// wet mask applied after the sweep's second colour (mask_all).
+template <int H, int TW, int TH>
+__device__ __forceinline__ void rb_half_sweep(const RbStage<(TW + 2 * H) * (TH + 2 * H)>& st,
+                                              double cx, double cy, int i0, int j0, int h,
+                                              int colour, bool mask_all, const double* src,
+                                              double* dst) {
   constexpr int RW = TW + 2 * H;
   const int w2 = (TW + 2 * h) / 2, hh = TH + 2 * h;
   for (int t = threadIdx.x; t < w2 * hh; t += MG_THREADS) {
     const int lj = H - h + t / w2;
     const int li0 = H - h + 2 * (t % w2);
-    const int gj = mg_pmod(j0 - H + lj, L.ny);
-    const int gi0 = mg_pmod(i0 - H + li0, L.nx);
-    const int gi1 = wrap(gi0, 1, L.nx);
-    const int a = ((gi0 + gj) & 1) == colour ? 0 : 1;  // the pair's cell of this colour
-    const int gi = a ? gi1 : gi0, go = a ? gi0 : gi1;
-    const long long row = (long long)gj * L.nx;
-    const long long c = row + gi, co = row + go;
-    const long long cw = row + wrap(gi, -1, L.nx);
-    const long long cs = (long long)wrap(gj, -1, L.ny) * L.nx + gi;
+    const int a = ((i0 + li0 + j0 + lj) & 1) == colour ? 0 : 1;  // the pair's cell of this colour
     const int q = lj * RW + li0 + a, qo = lj * RW + li0 + 1 - a;
-    const double v = rbgs_vals<double>(L, c, cw, cs, b[c], src[q + 1], src[q - 1], src[q + RW],
-                                       src[q - RW]);
+    const double v = rbgs_from<double>(st.inv[q], st.b[q], cx, cy, st.ku[q], st.ku[q - 1],
+                                       st.kv[q], st.kv[q - RW], src[q + 1], src[q - 1],
+                                       src[q + RW], src[q - RW]);
     const double vo = src[qo];
-    dst[q] = mask_all ? v * mkr<double>(L.mask[c]) : v;
-    dst[qo] = mask_all ? vo * mkr<double>(L.mask[co]) : vo;
+    dst[q] = mask_all ? v * st.m[q] : v;
+    dst[qo] = mask_all ? vo * st.m[qo] : vo;
   }
 }
 
@@ -461,13 +581,15 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_down_rb(MgLevel L, MgLevel C,
                                                            const PcgScalars* sc) {
   if (finished(sc)) return;
   constexpr int H = 2 * S + 1;  // 2S half-sweeps, then the residual's stencil
-  constexpr int RW = TW + 2 * H, RH = TH + 2 * H;
-  __shared__ double buf[2][RH * RW];
+  constexpr int RW = TW + 2 * H, RN = RW * (TH + 2 * H);
+  __shared__ RbStage<RN> st;
+  __shared__ double buf[2][RN];
   const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
-  for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) buf[0][t] = 0.0;
+  rb_stage<H, TW, TH>(st, L, b, i0, j0);
+  for (int t = threadIdx.x; t < RN; t += MG_THREADS) buf[0][t] = 0.0;
   __syncthreads();
   for (int ph = 1; ph <= 2 * S; ++ph) {  // red first
-    rb_half_sweep<H, TW, TH>(L, b, i0, j0, H - ph, (ph - 1) & 1, (ph & 1) == 0,
+    rb_half_sweep<H, TW, TH>(st, L.cx, L.cy, i0, j0, H - ph, (ph - 1) & 1, (ph & 1) == 0,
                              buf[(ph - 1) & 1], buf[ph & 1]);
     __syncthreads();
   }
@@ -483,13 +605,13 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_down_rb(MgLevel L, MgLevel C,
     if (I >= C.nx || J >= C.ny) continue;
     double r[4];
     for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
-      const int a = qd & 1, bq = qd >> 1;
-      const int gi = 2 * I + a, gj = 2 * J + bq;
-      const Nbr n = nbr_of(L, gi, gj);
+      const int gi = 2 * I + (qd & 1), gj = 2 * J + (qd >> 1);
       const int q = (H + gj - j0) * RW + H + gi - i0;
-      const double ax = apply_vals<double>(L, n.c, n.ce, n.cw, n.cn, n.cs, xs[q], xs[q + 1],
-                                           xs[q - 1], xs[q + RW], xs[q - RW]);
-      r[qd] = (b[n.c] - ax) * mkr<double>(L.mask[n.c]);
+      const double ax = apply_from<double>(
+          L.coef, L.dx2, L.dy2, st.ku[q], st.ku[q - 1], st.kv[q], st.kv[q - RW], st.m[q],
+          st.m[q + 1], st.m[q - 1], st.m[q + RW], st.m[q - RW], xs[q], xs[q + 1], xs[q - 1],
+          xs[q + RW], xs[q - RW]);
+      r[qd] = (st.b[q] - ax) * st.m[q];
     }
     bc[(long long)J * C.nx + I] = 0.25 * (r[0] + r[1] + r[2] + r[3]);
   }
@@ -505,18 +627,20 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up_rb(MgLevel L, int cnx,
                                                          const PcgScalars* sc) {
   if (finished(sc)) return;
   constexpr int H = 2 * S;
-  constexpr int RW = TW + 2 * H, RH = TH + 2 * H;
-  __shared__ double buf[2][RH * RW];
+  constexpr int RW = TW + 2 * H, RN = RW * (TH + 2 * H);
+  __shared__ RbStage<RN> st;
+  __shared__ double buf[2][RN];
   const int i0 = blockIdx.x * TW, j0 = blockIdx.y * TH;
-  for (int t = threadIdx.x; t < RW * RH; t += MG_THREADS) {
+  rb_stage<H, TW, TH>(st, L, b, i0, j0);
+  for (int t = threadIdx.x; t < RN; t += MG_THREADS) {  // st.m[t] was written by this thread
     const int gi = mg_pmod(i0 - H + t % RW, L.nx), gj = mg_pmod(j0 - H + t / RW, L.ny);
     const long long c = (long long)gj * L.nx + gi;
-    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * mkr<double>(L.mask[c]);
+    buf[0][t] = xs_in[c] + ec[(long long)(gj / 2) * cnx + gi / 2] * st.m[t];
   }
   __syncthreads();
   for (int ph = 1; ph <= 2 * S; ++ph) {  // black first
-    rb_half_sweep<H, TW, TH>(L, b, i0, j0, H - ph, ph & 1, (ph & 1) == 0, buf[(ph - 1) & 1],
-                             buf[ph & 1]);
+    rb_half_sweep<H, TW, TH>(st, L.cx, L.cy, i0, j0, H - ph, ph & 1, (ph & 1) == 0,
+                             buf[(ph - 1) & 1], buf[ph & 1]);
     __syncthreads();
   }
   const double* x = buf[0];
@@ -542,6 +666,17 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   extern __shared__ __align__(16) unsigned char cyc_raw[];
   R* cyc = reinterpret_cast<R*>(cyc_raw);
   const int n0 = (int)C.lev[0].n;
+  // level l with its coefficients read from the LDS copies when staged
+  auto lev = [&](int l) -> MgLevelT<R> {
+    MgLevelT<R> L = C.lev[l];
+    if (C.stage) {
+      L.Ku = cyc + C.coff[l];
+      L.Kv = cyc + C.coff[l] + L.n;
+      L.inv_diag = cyc + C.coff[l] + 2 * L.n;
+      L.mask = cyc_raw + C.moff[l];
+    }
+    return L;
+  };
   R* wA = cyc;
   R* wB = cyc + n0;
   R* wC = cyc + 2 * n0;  // red-black sweeps only: the field between the two colours
@@ -586,16 +721,30 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
     __syncthreads();
   };
   // level 0's right-hand side, staged (the restriction of the levels below
-  // overwrites nothing of it)
+  // overwrites nothing of it), and the levels' coefficients when they fit
   {
     R* st = cyc + C.boff[0];
     for (int c = tid; c < n0; c += MG_COARSE_THREADS) st[c] = (R)b0[c];
+    if (C.stage) {
+      for (int l = 0; l < C.nl; ++l) {
+        const MgLevelT<R>& G = C.lev[l];
+        const int n = (int)G.n;
+        R* k = cyc + C.coff[l];
+        unsigned char* m = cyc_raw + C.moff[l];
+        for (int c = tid; c < n; c += MG_COARSE_THREADS) {
+          k[c] = G.Ku[c];
+          k[n + c] = G.Kv[c];
+          k[2 * n + c] = G.inv_diag[c];
+          m[c] = G.mask[c];
+        }
+      }
+    }
     __syncthreads();
   }
   auto level_b = [&](int l) -> const R* { return cyc + C.boff[l]; };
   // down: x_S into XS[l], restricted residual into B[l + 1]
   for (int l = 0; l + 1 < C.nl; ++l) {
-    const MgLevelT<R>& L = C.lev[l];
+    const MgLevelT<R> L = lev(l);
     const R* b = level_b(l);
     R* xs = cyc + C.xoff[l];
     const R* src = nullptr;
@@ -604,7 +753,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
       sweep(L, b, src, dst, false);
       src = dst;
     }
-    const MgLevelT<R>& Cl = C.lev[l + 1];
+    const MgLevelT<R>& Cl = C.lev[l + 1];  // sizes only
     R* bc = cyc + C.boff[l + 1];
     const int nc = (int)Cl.n;
     for (int t = tid; t < nc; t += MG_COARSE_THREADS) {
@@ -625,7 +774,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   R* e = nullptr;
   {
     const int l = C.nl - 1;
-    const MgLevelT<R>& L = C.lev[l];
+    const MgLevelT<R> L = lev(l);
     const R* b = level_b(l);
     const R* src = nullptr;
     for (int k = 1; k <= 2 * S; ++k) {
@@ -637,7 +786,7 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   }
   // up: x = XS[l] + prolong(e) mask, then S sweeps
   for (int l = C.nl - 2; l >= 0; --l) {
-    const MgLevelT<R>& L = C.lev[l];
+    const MgLevelT<R> L = lev(l);
     const R* b = level_b(l);
     const R* xs = cyc + C.xoff[l];
     const int cnx = C.lev[l + 1].nx;
@@ -670,7 +819,19 @@ size_t mg_cycle_layout(MgCycleT<R>& C) {
     C.xoff[l] = off;
     off += (int)C.lev[l].n;
   }
-  return (size_t)off * sizeof(R);
+  const size_t vectors = (size_t)off * sizeof(R);
+  for (int l = 0; l < C.nl; ++l) {
+    C.coff[l] = off;
+    off += 3 * (int)C.lev[l].n;
+  }
+  size_t bytes = (size_t)off * sizeof(R);
+  for (int l = 0; l < C.nl; ++l) {
+    C.moff[l] = (int)bytes;
+    bytes += (size_t)C.lev[l].n;
+  }
+  bytes = (bytes + 15) / 16 * 16;
+  C.stage = bytes <= MG_CYCLE_LDS ? 1 : 0;
+  return C.stage ? bytes : vectors;
 }
 template size_t mg_cycle_layout<double>(MgCycle&);
 template size_t mg_cycle_layout<float>(MgCycleF&);

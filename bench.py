@@ -321,7 +321,7 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
     return out
 
 
-def ensemble_throughput(steps: int = 100) -> dict:
+def ensemble_throughput(steps: int = 100, mg_smoother: str = "rbgs") -> dict:
     """The 64-member ensemble (16 each at 128^2, 192^2, 256^2, 384^2, 30
     levels, per-member seeds, T noise N(0, 0.01)): leg tile shapes by the
     autotuner, solve budgets calibrated, one HIP graph per two steps;
@@ -329,7 +329,7 @@ def ensemble_throughput(steps: int = 100) -> dict:
     member after member with polled solves."""
     from hydrostatic_ocean_core_benchmark_amd.ensemble import Ensemble, benchmark_members
     t0 = time.perf_counter()
-    e = Ensemble(benchmark_members())
+    e = Ensemble(benchmark_members(mg_smoother=mg_smoother))
     setup = time.perf_counter() - t0
     tiles = e.autotune()
     budgets = e.calibrate()
@@ -339,7 +339,8 @@ def ensemble_throughput(steps: int = 100) -> dict:
     eager = e.run_eager(steps)
     return {"workload": "64 members: 16 each at 128x128, 192x192, 256x256, 384x384, 30 levels, "
                         "GRF basins from per-member seeds, T noise N(0, 0.01 K), fp64, "
-                        f"{steps} steps, one HIP graph replay per 2 steps",
+                        f"pcg_mg ({mg_smoother} smoother), {steps} steps, one HIP graph "
+                        "replay per 2 steps",
             "unit": "cell-steps/s", "value": r.cell_steps_per_s, "cells": r.cells,
             "wall_s": round(r.wall_s, 3), "solver_failures": r.solver_failures,
             "eager_sequential_value": r.cells * steps / eager,

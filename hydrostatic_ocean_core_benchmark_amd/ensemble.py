@@ -254,16 +254,19 @@ class Ensemble:
 
 
 def benchmark_members(sizes=(128, 192, 256, 384), per_size: int = 16, nz: int = 30,
-                      base_seed: int = 20261204, sigma_T: float = 0.01):
+                      base_seed: int = 20261204, sigma_T: float = 0.01,
+                      mg_smoother: str = "rbgs"):
     """The ensemble of the benchmark: `per_size` members at each size, 30
     levels, bathymetry and stratification generated like the 512 x 512
     basin (dx 10 km, dt 600 s) from per-member seeds, T perturbed by
-    N(0, sigma_T)."""
+    N(0, sigma_T); pcg_mg with the given V-cycle smoother."""
     from .cases import basin
     out = []
     idx = 0
     for n in sizes:
         for _ in range(per_size):
-            out.append(basin(nx=n, nz=nz, seed=base_seed + idx, sigma_T=sigma_T))
+            prob = basin(nx=n, nz=nz, seed=base_seed + idx, sigma_T=sigma_T)
+            prob.params.mg_smoother = mg_smoother
+            out.append(prob)
             idx += 1
     return out

@@ -1971,7 +1971,10 @@ int split_barotropic(hoc_stepper* h, const double* eta, const double* gh_u, cons
   return HOC_OK;
 }
 
-int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream) {
+// One step; w_ready: h->w already holds w of in->u, in->v (the previous step's
+// output on a static geometry), so the step does not form it again.
+static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream,
+                     bool w_ready) {
   if (!h || !in || !out) return fail(HOC_ERR_INVALID, "null argument");
   if (!in->u || !in->v || !in->b || !in->eta || !out->u || !out->v || !out->b || !out->eta)
     return fail(HOC_ERR_INVALID, "state arrays u, v, b, eta are required");
@@ -2026,8 +2029,10 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   }
   { ProfScope _ps(h, "k_phi", s);
   k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->b, h->phi); }
-  { ProfScope _ps(h, "k_w_diag", s);
-  k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w); }
+  if (!w_ready) {
+    ProfScope _ps(h, "k_w_diag", s);
+    k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w);
+  }
   { ProfScope _ps(h, "k_momentum_explicit", s);
   launch_momentum_explicit_tiled((h->fp32_parts & HOC_FP32_MOMENTUM_FLUX) != 0, s, d, p, geo,
                                  in->u, in->v, in->b, h->phi,
@@ -2100,6 +2105,10 @@ int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;
   return HOC_OK;
+}
+
+int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream) {
+  return step_impl(h, in, out, stream, false);
 }
 
 int hoc_stepper_z_centre(hoc_stepper* h, double* out, void* stream) {
@@ -2374,9 +2383,14 @@ int hoc_stepper_momentum_advection_adjoint(hoc_stepper* h, const double* u, cons
 int hoc_stepper_step_n(hoc_stepper* h, int32_t n, hoc_state* a, hoc_state* b, void* stream) {
   if (!h || !a || !b) return fail(HOC_ERR_INVALID, "null argument");
   if (n < 0) return fail(HOC_ERR_INVALID, "n must be >= 0");
+  // every step leaves h->w = w of its output velocity (the theta scheme's
+  // last back-substitution or the closing k_w_diag), which is the next
+  // step's input: on a static geometry the next step takes it as it is
+  const bool reuse_w = h->cfg.vcoord == 0;
   for (int32_t k = 0; k < n; ++k) {
-    const int rc = (k % 2 == 0) ? hoc_stepper_step(h, a, b, stream)
-                                : hoc_stepper_step(h, b, a, stream);
+    const bool w_ready = reuse_w && k > 0;
+    const int rc = (k % 2 == 0) ? step_impl(h, a, b, stream, w_ready)
+                                : step_impl(h, b, a, stream, w_ready);
     if (rc != HOC_OK) return rc;
   }
   return HOC_OK;

@@ -276,6 +276,16 @@ class HipStepper:
         if self.physics.closure != "tke":
             fout["e"] = fin.get("e")
 
+    def _step_n_into(self, a: dict, b: dict, n: int) -> dict:
+        """n steps from a, ping-ponging between the buffer sets a and b in one
+        library call (hoc_stepper_step_n: from the second step on, w of the
+        input is the one the previous step formed); returns the set holding
+        the final state."""
+        sa, sb = self._hoc_state(a), self._hoc_state(b)
+        _lib.check(self.lib.hoc_stepper_step_n(self._h, int(n), C.byref(sa), C.byref(sb),
+                                               self._stream()))
+        return a if n % 2 == 0 else b
+
     # ---------------------------------------------------------------- API
     def step(self, s) -> State3D:
         """libs/core/model3d_v05.py:337-415"""
@@ -286,18 +296,21 @@ class HipStepper:
                        t=s.t + self.dt, T=fout.get("T"), S=fout.get("S"), e=fout.get("e"))
 
     def integrate(self, state, n_steps: int) -> State3D:
-        """libs/core/model3d_v05.py:535-538, ping-ponging two buffer sets."""
+        """libs/core/model3d_v05.py:535-538: the input copied into the first of
+        two buffer sets, then hoc_stepper_step_n ping-ponging between them."""
         if n_steps <= 0:
             return state
         fin = self._inputs(state)
         if not self._bufs:
             self._bufs = [self._new_state_buffers(), self._new_state_buffers()]
+        a, b = self._bufs
+        for k, buf in a.items():
+            buf.copy_(fin[k])
+        cur = dict(self._step_n_into(a, b, n_steps))
+        if self.physics.closure != "tke":
+            cur["e"] = fin.get("e")
         t = state.t
-        cur = fin
-        for n in range(n_steps):
-            out = self._bufs[n % 2]
-            self._step_into(cur, out)
-            cur = out
+        for _ in range(n_steps):
             t = t + self.dt
         res = {k: (v.clone() if isinstance(v, torch.Tensor) else v) for k, v in cur.items()}
         return State3D(u=res["u"], v=res["v"], b=res["b"], eta=res["eta"], t=t,

@@ -655,77 +655,128 @@ __global__ __launch_bounds__(MG_THREADS) void k_mg_up_rb(MgLevel L, int cnx,
 // the last level alone), S sweeps from zero + residual restriction down, 2S
 // sweeps from zero on the last level, prolongation + S sweeps up; the same
 // arithmetic per cell as the tiled legs (jacobi_vals or rbgs_vals,
-// apply_vals). With C.rb the sweeps are red-black Gauss-Seidel, reversed
-// (black first) on the way up and in the last level's second S. b0 is
-// level 0's right-hand side (staged in LDS), out receives level 0's result.
-template <class R, class RB, class RO>
-__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
-    MgCycleT<R> C, R omega, int S, const RB* __restrict__ b0, RO* __restrict__ out,
-    const PcgScalars* sc) {
-  if (finished(sc)) return;
-  extern __shared__ __align__(16) unsigned char cyc_raw[];
+// apply_vals, from their operands). With C.rb the sweeps are red-black
+// Gauss-Seidel, reversed (black first) on the way up and in the last level's
+// second S. b0 is level 0's right-hand side (staged in LDS), out receives
+// level 0's result.
+
+// One level's operands inside the cycle: its coefficients from their LDS
+// copies (STAGED) or from the level's arrays, never a mix, so that the
+// compiler addresses the copies as LDS.
+template <class R, bool STAGED>
+struct CycLevel {
+  int nx, ny, n;
+  R coef, cx, cy, dx2, dy2;
+  const R *ku, *kv, *inv;
+  const uint8_t* m;
+};
+
+// One cell of a level with its neighbours and colour (the parity of i + j).
+struct CycCell {
+  int c, ce, cw, cn, cs, par;
+};
+
+template <class R, bool STAGED>
+__device__ __forceinline__ CycCell cyc_cell(const CycLevel<R, STAGED>& L, int c) {
+  const int i = c % L.nx, j = c / L.nx;
+  return CycCell{c, j * L.nx + wrap(i, 1, L.nx), j * L.nx + wrap(i, -1, L.nx),
+                 wrap(j, 1, L.ny) * L.nx + i, wrap(j, -1, L.ny) * L.nx + i, (i + j) & 1};
+}
+
+template <class R, bool STAGED>
+__device__ __forceinline__ R cyc_apply(const CycLevel<R, STAGED>& L, int c, int ce, int cw,
+                                       int cn, int cs, R x0, R x_e, R x_w, R x_n, R x_s) {
+  return apply_from<R>(L.coef, L.dx2, L.dy2, L.ku[c], L.ku[cw], L.kv[c], L.kv[cs],
+                       mkr<R>(L.m[c]), mkr<R>(L.m[ce]), mkr<R>(L.m[cw]), mkr<R>(L.m[cn]),
+                       mkr<R>(L.m[cs]), x0, x_e, x_w, x_n, x_s);
+}
+
+template <class R, class RB, class RO, bool STAGED>
+__device__ __forceinline__ void coarse_cycle(const MgCycleT<R>& C, R omega, int S,
+                                             const RB* __restrict__ b0, RO* __restrict__ out,
+                                             unsigned char* cyc_raw) {
   R* cyc = reinterpret_cast<R*>(cyc_raw);
   const int n0 = (int)C.lev[0].n;
-  // level l with its coefficients read from the LDS copies when staged
-  auto lev = [&](int l) -> MgLevelT<R> {
-    MgLevelT<R> L = C.lev[l];
-    if (C.stage) {
-      L.Ku = cyc + C.coff[l];
-      L.Kv = cyc + C.coff[l] + L.n;
-      L.inv_diag = cyc + C.coff[l] + 2 * L.n;
-      L.mask = cyc_raw + C.moff[l];
+  const int tid = threadIdx.x;
+  auto lev = [&](int l) {
+    const MgLevelT<R>& G = C.lev[l];
+    CycLevel<R, STAGED> L;
+    L.nx = G.nx;
+    L.ny = G.ny;
+    L.n = (int)G.n;
+    L.coef = G.coef;
+    L.cx = G.cx;
+    L.cy = G.cy;
+    L.dx2 = G.dx2;
+    L.dy2 = G.dy2;
+    if (STAGED) {
+      L.ku = cyc + C.coff[l];
+      L.kv = cyc + C.coff[l] + L.n;
+      L.inv = cyc + C.coff[l] + 2 * L.n;
+      L.m = cyc_raw + C.moff[l];
+    } else {
+      L.ku = G.Ku;
+      L.kv = G.Kv;
+      L.inv = G.inv_diag;
+      L.m = G.mask;
     }
     return L;
   };
   R* wA = cyc;
   R* wB = cyc + n0;
   R* wC = cyc + 2 * n0;  // red-black sweeps only: the field between the two colours
-  const int tid = threadIdx.x;
-  // one sweep on level L from src (zero if null) into dst: damped Jacobi,
-  // or red-black Gauss-Seidel (red first unless reverse) through wC
-  auto sweep = [&](const MgLevelT<R>& L, const R* b, const R* src, R* dst, bool reverse) {
-    const int n = (int)L.n;
+  // one sweep on level L from src (zero when `zero`) into dst: damped
+  // Jacobi, or red-black Gauss-Seidel (red first unless reverse) through wC.
+  // K0 is the thread's first cell (tid), formed once per visit of the
+  // level; the others (levels of more than MG_COARSE_THREADS cells) are
+  // formed as they come.
+  auto for_cells = [&](const CycLevel<R, STAGED>& L, const CycCell& K0, auto&& f) {
+    if (tid < L.n) f(K0);
+    for (int c = tid + MG_COARSE_THREADS; c < L.n; c += MG_COARSE_THREADS) f(cyc_cell(L, c));
+  };
+  auto sweep = [&](const CycLevel<R, STAGED>& L, const CycCell& K0, const R* b, const R* src,
+                   bool zero, R* dst, bool reverse) {
     if (!C.rb) {
-      for (int c = tid; c < n; c += MG_COARSE_THREADS) {
-        const Nbr nb = nbr_of(L, c % L.nx, c / L.nx);
-        dst[c] = src ? jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, src[c],
-                                      src[nb.ce], src[nb.cw], src[nb.cn], src[nb.cs], b[c])
-                     : jacobi_vals<R>(L, omega, c, nb.ce, nb.cw, nb.cn, nb.cs, R(0), R(0), R(0),
-                                      R(0), R(0), b[c]);
-      }
+      for_cells(L, K0, [&](const CycCell& K) {
+        const int c = K.c;
+        const R x0 = zero ? R(0) : src[c], xe = zero ? R(0) : src[K.ce];
+        const R xw = zero ? R(0) : src[K.cw], xn = zero ? R(0) : src[K.cn];
+        const R xs = zero ? R(0) : src[K.cs];
+        const R ax = cyc_apply(L, c, K.ce, K.cw, K.cn, K.cs, x0, xe, xw, xn, xs);
+        dst[c] = x0 + omega * (L.inv[c] * (b[c] - ax)) * mkr<R>(L.m[c]);
+      });
       __syncthreads();
       return;
     }
     const int first = reverse ? 1 : 0;
-    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
-      const int i = c % L.nx, j = c / L.nx;
-      R v = src ? src[c] : R(0);
-      if (((i + j) & 1) == first) {
-        const Nbr nb = nbr_of(L, i, j);
-        v = src ? rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], src[nb.ce], src[nb.cw], src[nb.cn],
-                               src[nb.cs])
-                : rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], R(0), R(0), R(0), R(0));
+    for_cells(L, K0, [&](const CycCell& K) {
+      const int c = K.c;
+      R v = zero ? R(0) : src[c];
+      if (K.par == first) {
+        const R xe = zero ? R(0) : src[K.ce], xw = zero ? R(0) : src[K.cw];
+        const R xn = zero ? R(0) : src[K.cn], xs = zero ? R(0) : src[K.cs];
+        v = rbgs_from<R>(L.inv[c], b[c], L.cx, L.cy, L.ku[c], L.ku[K.cw], L.kv[c], L.kv[K.cs],
+                         xe, xw, xn, xs);
       }
       wC[c] = v;
-    }
+    });
     __syncthreads();
-    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
-      const int i = c % L.nx, j = c / L.nx;
+    for_cells(L, K0, [&](const CycCell& K) {
+      const int c = K.c;
       R v = wC[c];
-      if (((i + j) & 1) != first) {
-        const Nbr nb = nbr_of(L, i, j);
-        v = rbgs_vals<R>(L, c, nb.cw, nb.cs, b[c], wC[nb.ce], wC[nb.cw], wC[nb.cn], wC[nb.cs]);
-      }
-      dst[c] = v * mkr<R>(L.mask[c]);
-    }
+      if (K.par != first)
+        v = rbgs_from<R>(L.inv[c], b[c], L.cx, L.cy, L.ku[c], L.ku[K.cw], L.kv[c], L.kv[K.cs],
+                         wC[K.ce], wC[K.cw], wC[K.cn], wC[K.cs]);
+      dst[c] = v * mkr<R>(L.m[c]);
+    });
     __syncthreads();
   };
   // level 0's right-hand side, staged (the restriction of the levels below
-  // overwrites nothing of it), and the levels' coefficients when they fit
+  // overwrites nothing of it), and the levels' coefficients when staged
   {
     R* st = cyc + C.boff[0];
     for (int c = tid; c < n0; c += MG_COARSE_THREADS) st[c] = (R)b0[c];
-    if (C.stage) {
+    if (STAGED) {
       for (int l = 0; l < C.nl; ++l) {
         const MgLevelT<R>& G = C.lev[l];
         const int n = (int)G.n;
@@ -744,13 +795,14 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
   auto level_b = [&](int l) -> const R* { return cyc + C.boff[l]; };
   // down: x_S into XS[l], restricted residual into B[l + 1]
   for (int l = 0; l + 1 < C.nl; ++l) {
-    const MgLevelT<R> L = lev(l);
+    const CycLevel<R, STAGED> L = lev(l);
+    const CycCell K0 = cyc_cell(L, tid);
     const R* b = level_b(l);
     R* xs = cyc + C.xoff[l];
-    const R* src = nullptr;
+    const R* src = wA;
     for (int k = 1; k <= S; ++k) {
       R* dst = (k == S) ? xs : ((k & 1) ? wA : wB);
-      sweep(L, b, src, dst, false);
+      sweep(L, K0, b, src, k == 1, dst, false);
       src = dst;
     }
     const MgLevelT<R>& Cl = C.lev[l + 1];  // sizes only
@@ -761,52 +813,66 @@ __global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
       R r[4];
       for (int qd = 0; qd < 4; ++qd) {  // [0::2,0::2], [0::2,1::2], [1::2,0::2], [1::2,1::2]
         const int gi = 2 * I + (qd & 1), gj = 2 * J + (qd >> 1);
-        const Nbr nb = nbr_of(L, gi, gj);
-        const R ax = apply_vals<R>(L, nb.c, nb.ce, nb.cw, nb.cn, nb.cs, xs[nb.c], xs[nb.ce],
-                                   xs[nb.cw], xs[nb.cn], xs[nb.cs]);
-        r[qd] = (b[nb.c] - ax) * mkr<R>(L.mask[nb.c]);
+        const int c = gj * L.nx + gi;
+        const int ce = gj * L.nx + wrap(gi, 1, L.nx), cw = gj * L.nx + wrap(gi, -1, L.nx);
+        const int cn = wrap(gj, 1, L.ny) * L.nx + gi, cs = wrap(gj, -1, L.ny) * L.nx + gi;
+        const R ax = cyc_apply(L, c, ce, cw, cn, cs, xs[c], xs[ce], xs[cw], xs[cn], xs[cs]);
+        r[qd] = (b[c] - ax) * mkr<R>(L.m[c]);
       }
       bc[t] = R(0.25) * (r[0] + r[1] + r[2] + r[3]);
     }
     __syncthreads();
   }
   // the last level: 2S sweeps from zero (the second S reversed)
-  R* e = nullptr;
+  R* e = wA;
   {
     const int l = C.nl - 1;
-    const MgLevelT<R> L = lev(l);
+    const CycLevel<R, STAGED> L = lev(l);
+    const CycCell K0 = cyc_cell(L, tid);
     const R* b = level_b(l);
-    const R* src = nullptr;
+    const R* src = wB;
     for (int k = 1; k <= 2 * S; ++k) {
       R* dst = (k & 1) ? wA : wB;
-      sweep(L, b, src, dst, k > S);
+      sweep(L, K0, b, src, k == 1, dst, k > S);
       src = dst;
       e = dst;
     }
   }
   // up: x = XS[l] + prolong(e) mask, then S sweeps
   for (int l = C.nl - 2; l >= 0; --l) {
-    const MgLevelT<R> L = lev(l);
+    const CycLevel<R, STAGED> L = lev(l);
+    const CycCell K0 = cyc_cell(L, tid);
     const R* b = level_b(l);
     const R* xs = cyc + C.xoff[l];
     const int cnx = C.lev[l + 1].nx;
     R* xp = (e == wA) ? wB : wA;
     R* other = (e == wA) ? wA : wB;
-    const int n = (int)L.n;
-    for (int c = tid; c < n; c += MG_COARSE_THREADS) {
+    for (int c = tid; c < L.n; c += MG_COARSE_THREADS) {
       const int i = c % L.nx, j = c / L.nx;
-      xp[c] = xs[c] + e[(j / 2) * cnx + i / 2] * mkr<R>(L.mask[c]);
+      xp[c] = xs[c] + e[(j / 2) * cnx + i / 2] * mkr<R>(L.m[c]);
     }
     __syncthreads();
     const R* src = xp;
     for (int k = 1; k <= S; ++k) {
       R* dst = (k & 1) ? other : xp;
-      sweep(L, b, src, dst, true);
+      sweep(L, K0, b, src, false, dst, true);
       src = dst;
       e = dst;
     }
   }
   for (int c = tid; c < n0; c += MG_COARSE_THREADS) out[c] = (RO)e[c];
+}
+
+template <class R, class RB, class RO>
+__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_cycle(
+    MgCycleT<R> C, R omega, int S, const RB* __restrict__ b0, RO* __restrict__ out,
+    const PcgScalars* sc) {
+  if (finished(sc)) return;
+  extern __shared__ __align__(16) unsigned char cyc_raw[];
+  if (C.stage)
+    coarse_cycle<R, RB, RO, true>(C, omega, S, b0, out, cyc_raw);
+  else
+    coarse_cycle<R, RB, RO, false>(C, omega, S, b0, out, cyc_raw);
 }
 
 template <class R>

@@ -754,6 +754,13 @@ template <class R>
 struct MomLds {
   R u[SY][SX], v[SY][SX];               // velocities, 2-cell halo
   uint8_t mu[SY][SX], mv[SY][SX];       // mask3u, mask3v, 2-cell halo
+};
+
+// What only the cell pass reads (one buffer: level k + 1 is stored after the
+// barrier that ends level k and before the face pass of level k + 1, whose
+// closing barrier orders it before the cell pass).
+template <class R>
+struct MomPLds {
   double phi[TY + 1][TX + 1];           // potential, +1 east / north
   double b[TY + 1][TX + 1];
   double zc[TY + 1][TX + 1];
@@ -841,15 +848,19 @@ __device__ __forceinline__ void mom_store(MomLds<RT>& L, const MomRegs<RT>& R, i
       L.mv[ly][lx] = R.mv[q];
     }
   }
+}
+
+template <class RT>
+__device__ __forceinline__ void mom_store_p(MomPLds<RT>& P, const MomRegs<RT>& R, int tid) {
 #pragma unroll
   for (int q = 0; q < MS_P; ++q) {
     const int t = tid + q * TILE_THREADS;
     if (t < (TY + 1) * (TX + 1)) {
       const int lx = t % (TX + 1), ly = t / (TX + 1);
-      L.phi[ly][lx] = R.phi[q];
-      L.b[ly][lx] = R.b[q];
-      L.zc[ly][lx] = R.zc[q];
-      L.w[ly][lx] = R.w[q];
+      P.phi[ly][lx] = R.phi[q];
+      P.b[ly][lx] = R.b[q];
+      P.zc[ly][lx] = R.zc[q];
+      P.w[ly][lx] = R.w[q];
     }
   }
 }
@@ -890,8 +901,7 @@ __device__ __forceinline__ void mom_load_c(MomRegsC<RT>& R, const MomOffs& F, lo
 
 template <class RT>
 __device__ __forceinline__ void mom_store_c(MomLds<RT>& L, const MomRegsC<RT>& R,
-                                            const ColTile& CG, const double* zs, double dzr,
-                                            int kk, int tid) {
+                                            const ColTile& CG, int kk, int tid) {
 #pragma unroll
   for (int q = 0; q < MS_H; ++q) {
     const int t = tid + q * TILE_THREADS;
@@ -903,15 +913,21 @@ __device__ __forceinline__ void mom_store_c(MomLds<RT>& L, const MomRegsC<RT>& R
       L.mv[ly][lx] = col_wet_v(CG, ly, lx, kk);
     }
   }
+}
+
+template <class RT>
+__device__ __forceinline__ void mom_store_p_c(MomPLds<RT>& P, const MomRegsC<RT>& R,
+                                              const ColTile& CG, const double* zs, double dzr,
+                                              int kk, int tid) {
 #pragma unroll
   for (int q = 0; q < MS_P; ++q) {
     const int t = tid + q * TILE_THREADS;
     if (t < (TY + 1) * (TX + 1)) {
       const int lx = t % (TX + 1), ly = t / (TX + 1);
-      L.phi[ly][lx] = R.phi[q];
-      L.b[ly][lx] = R.b[q];
-      L.zc[ly][lx] = cg_zc(zs, dzr, CG.nw[ly + HALO][lx + HALO], CG.hb[ly + HALO][lx + HALO], kk);
-      L.w[ly][lx] = R.w[q];
+      P.phi[ly][lx] = R.phi[q];
+      P.b[ly][lx] = R.b[q];
+      P.zc[ly][lx] = cg_zc(zs, dzr, CG.nw[ly + HALO][lx + HALO], CG.hb[ly + HALO][lx + HALO], kk);
+      P.w[ly][lx] = R.w[q];
     }
   }
 }
@@ -943,13 +959,15 @@ __device__ __forceinline__ R mom_flux_z(int adv, int kk, int nz, R wb, R far_up,
 // static z-level geometry in column form (masks, face thicknesses and
 // centre depths formed from the tile's column geometry, the same bits).
 template <class R, bool ZC>
-__global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
+__global__ __launch_bounds__(TILE_THREADS)
+__attribute__((amdgpu_waves_per_eu(ZC ? 3 : 1))) void k_momentum_explicit_tiled(
     Dims d, StepParams p, Geo geo, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ b, const double* __restrict__ phi, const double* __restrict__ w,
     const double* __restrict__ Km, double* __restrict__ pgx, double* __restrict__ pgy,
     double* __restrict__ avu, double* __restrict__ avv, double* __restrict__ dzu,
     double* __restrict__ dzv, double* __restrict__ stu, double* __restrict__ stv) {
   __shared__ MomLds<R> LB[2];
+  __shared__ MomPLds<R> P;
   __shared__ MomFaces<R> F;
   __shared__ std::conditional_t<ZC, ColTile, NoTile> CG;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -976,7 +994,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     mom_load_c(RG, MO, 0, n2, u, v, phi, b, w);
     col_tile_load(CG, geo, tid, i0, j0, nx, ny);
     __syncthreads();
-    mom_store_c(LB[0], RG, CG, geo.zs, p.dzr, 0, tid);
+    mom_store_c(LB[0], RG, CG, 0, tid);
   } else {
     mom_load(RG, MO, 0, n2, u, v, phi, b, w, geo);
     mom_store(LB[0], RG, tid);
@@ -1049,6 +1067,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2;
     const MomLds<R>& L = LB[k & 1];
+    // level k's cell-pass inputs, loaded during level k - 1 (before the
+    // loads of level k + 1 reuse the registers)
+    if constexpr (ZC) mom_store_p_c(P, RG, CG, geo.zs, p.dzr, k, tid);
+    else mom_store_p(P, RG, tid);
     if (k + 1 < nz) {
       if constexpr (ZC) mom_load_c(RG, MO, o + n2, n2, u, v, phi, b, w);
       else mom_load(RG, MO, o + n2, n2, u, v, phi, b, w, geo);
@@ -1057,7 +1079,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
     // (ly, lx), and the centre depth of tile point (ly, lx) of the +1 tile
     auto MUf = [&](int ly, int lx) -> bool { return L.mu[ly][lx] != 0; };
     auto MVf = [&](int ly, int lx) -> bool { return L.mv[ly][lx] != 0; };
-    auto ZCf = [&](int ly, int lx) -> double { return L.zc[ly][lx]; };
+    auto ZCf = [&](int ly, int lx) -> double { return P.zc[ly][lx]; };
     // the own column's viscosity below level k, issued before the face pass
     const double km_k1 = active ? Km[o + n2 + c] : 0.0;
     {
@@ -1123,20 +1145,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
         {
           const double zr = ZCf(ty, tx + 1);
           const double zf = 0.5 * (zc + zr);
-          const double left = L.phi[ty][tx] + L.b[ty][tx] * (zf - zc);
-          const double right = L.phi[ty][tx + 1] + L.b[ty][tx + 1] * (zf - zr);
+          const double left = P.phi[ty][tx] + P.b[ty][tx] * (zf - zc);
+          const double right = P.phi[ty][tx + 1] + P.b[ty][tx + 1] * (zf - zr);
           gx = mu_c * (right - left) / p.dx;
         }
         {
           const double zr = ZCf(ty + 1, tx);
           const double zf = 0.5 * (zc + zr);
-          const double left = L.phi[ty][tx] + L.b[ty][tx] * (zf - zc);
-          const double right = L.phi[ty + 1][tx] + L.b[ty + 1][tx] * (zf - zr);
+          const double left = P.phi[ty][tx] + P.b[ty][tx] * (zf - zc);
+          const double right = P.phi[ty + 1][tx] + P.b[ty + 1][tx] * (zf - zr);
           gy = mv_c * (right - left) / p.dy;
         }
       } else {
-        gx = mu_c * (L.phi[ty][tx + 1] - L.phi[ty][tx]) / p.dx * mu_c;
-        gy = mv_c * (L.phi[ty + 1][tx] - L.phi[ty][tx]) / p.dy * mv_c;
+        gx = mu_c * (P.phi[ty][tx + 1] - P.phi[ty][tx]) / p.dx * mu_c;
+        gy = mv_c * (P.phi[ty + 1][tx] - P.phi[ty][tx]) / p.dy * mv_c;
       }
       pgx[t] = gx;
       pgy[t] = gy;
@@ -1186,8 +1208,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
         const R by_ = (F.fvy[ty + 1][tx] - F.fvy[ty][tx]) / R(p.dy);
         const R bx_ = (F.fvx[ty][tx + 1] - F.fvx[ty][tx]) / R(p.dx);
         // vertical fluxes: the top one is the bottom one of the level above
-        const R wbu = R(0.5) * (L.w[ty][tx] + L.w[ty][tx + 1]);
-        const R wbv = R(0.5) * (L.w[ty][tx] + L.w[ty + 1][tx]);
+        const R wbu = R(0.5) * (P.w[ty][tx] + P.w[ty][tx + 1]);
+        const R wbv = R(0.5) * (P.w[ty][tx] + P.w[ty + 1][tx]);
         const R fzu_bot = mom_flux_z<R>(adv, k + 1, nz, wbu, R(uw5[1]), R(uw5[2]), R(uw5[3]),
                                         R(uw5[4]), muw[1], muw[4]);
         const R fzv_bot = mom_flux_z<R>(adv, k + 1, nz, wbv, R(vw5[1]), R(vw5[2]), R(vw5[3]),
@@ -1237,7 +1259,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_momentum_explicit_tiled(
       m3w[2] = k2 < nz && own_m3(k2);
     }
     if (k + 1 < nz) {
-      if constexpr (ZC) mom_store_c(LB[(k + 1) & 1], RG, CG, geo.zs, p.dzr, k + 1, tid);
+      if constexpr (ZC) mom_store_c(LB[(k + 1) & 1], RG, CG, k + 1, tid);
       else mom_store(LB[(k + 1) & 1], RG, tid);
     }
     __syncthreads();

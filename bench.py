@@ -80,7 +80,7 @@ ALG_BYTES_PER_CELL = {"k_tracers": 72.0, "k_closure": 80.0, "k_predictor": 56.0,
 # WRITE_SIZE, scaled by a 1 GiB copy of known size: fetch x 2.0, write x 1.0;
 # the summary is profiles/basin512x512x40_hbm_traffic_pmc.md). Re-measured
 # whenever the kernel changes.
-MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.305e9, "k_momentum_explicit": 1.071e9,
+MEASURED_TRAFFIC_512x40 = {"k_tracers": 1.305e9, "k_momentum_explicit": 1.121e9,
                            "k_predictor": 1.828e9, "k_closure": 1.195e9}
 STEP_BYTES_PER_CELL = 336.0
 SOLVE_BYTES_PER_COLUMN = 64.0

@@ -268,7 +268,7 @@ def mixed_comparison(steps: int, nx: int = 1024, nz: int = 60, per_part: bool = 
 
 
 def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
-                  fd_directions: int = 16) -> dict:
+                  fd_directions: int = 16, mg_smoother: str = "jacobi") -> dict:
     """dJ/dT0 and dJ/dkappa_b after `steps` steps of the 256 x 256 x 30 basin
     (same generator, dx 10 km, dt 600 s; kappa_b = 1e-5 (1 + 0.2 xi), xi
     seeded N(0, 1) per column), checkpoints every `every` steps: wall time of
@@ -284,6 +284,7 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
     from hydrostatic_ocean_core_benchmark_amd.adjoint import Adjoint, directional_check
     from hydrostatic_ocean_core_benchmark_amd.cases import basin
     prob = basin(nx=nx, nz=nz)
+    prob.params.mg_smoother = mg_smoother
     g = prob.domain.grid
     xi = np.random.default_rng(20261203).standard_normal(g.shape)
     kb = torch.as_tensor(1e-5 * (1.0 + 0.2 * xi), device="cuda")
@@ -294,7 +295,8 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
     torch.cuda.synchronize()
     r = adj.gradient(T0, S0, kb, n_steps=steps, every=every)
     out = {"workload": f"closed GRF basin {g.nx}x{g.ny}x{g.nz}, dx 10 km, dt 600 s, "
-                       f"{steps} steps, checkpoints every {every}, J = depth-weighted mean "
+                       f"pcg_mg ({mg_smoother} smoother), {steps} steps, checkpoints every "
+                       f"{every}, J = depth-weighted mean "
                        "SSH^2 + 0.5 volume-mean face-weighted speed^2; gradients w.r.t. T0 "
                        "and kappa_b(x, y)",
            "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),

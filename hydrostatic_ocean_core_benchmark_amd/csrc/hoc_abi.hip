@@ -2027,20 +2027,19 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   } else if (in->e && out->e && in->e != out->e) {
     HIP_TRY(hipMemcpyAsync(out->e, in->e, d.ni * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  { ProfScope _ps(h, "k_phi", s);
-  k_phi<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->b, h->phi); }
-  if (!w_ready) {
-    ProfScope _ps(h, "k_w_diag", s);
-    k_w_diag<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->w);
-  }
+  const bool split = h->cfg.scheme == 2;
+  // phi, the transports of the state (theta, split_explicit) and, unless the
+  // previous step formed it, w of the state: one column pass
+  { ProfScope _ps(h, "k_prologue", s);
+  k_prologue<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->b, h->phi,
+                                          (theta || split) ? h->U0 : nullptr, h->V0,
+                                          w_ready ? nullptr : h->w); }
   { ProfScope _ps(h, "k_momentum_explicit", s);
   launch_momentum_explicit_tiled((h->fp32_parts & HOC_FP32_MOMENTUM_FLUX) != 0, s, d, p, geo,
                                  in->u, in->v, in->b, h->phi,
                                  h->w, h->Km, h->pgx, h->pgy, nullptr, nullptr, h->dzu, h->dzv,
                                  h->stu, h->stv); }
   LAUNCH_CHECK("explicit momentum");
-  const bool split = h->cfg.scheme == 2;
-  if (theta || split) k_depth_sum<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, h->U0, h->V0);
   if (theta) k_div2<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->U0, h->V0, in->eta, h->div0);
   const double* u_it = in->u;
   const double* v_it = in->v;

@@ -60,6 +60,9 @@ __global__ void k_coefficients(Dims d, StepParams p, Geo geo, const double* Km,
                                double* sub_m, double* den_m, double* cst_m, double* q);
 __global__ void k_kukv(Dims d, StepParams p, Geo geo, const double* q, double* Ku, double* Kv);
 __global__ void k_phi(Dims d, StepParams p, Geo geo, const double* b, double* phi);
+// k_phi, k_depth_sum (U0 set) and k_w_diag (w set) in one column pass
+__global__ void k_prologue(Dims d, StepParams p, Geo geo, const double* u, const double* v,
+                           const double* b, double* phi, double* U0, double* V0, double* w);
 __global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* u, const double* v,
                          double* w);
 __global__ void k_momentum_explicit(Dims d, StepParams p, Geo geo, const double* u,

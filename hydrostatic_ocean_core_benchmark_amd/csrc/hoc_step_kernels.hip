@@ -530,6 +530,71 @@ __global__ void k_w_diag(Dims d, StepParams p, Geo geo, const double* __restrict
   w[(long long)nz * n2 + c] = 0.0;
 }
 
+// ------------------------------------------------- the step's prologue
+// k_phi, k_depth_sum and k_w_diag of one column in one pass over it (the
+// same operations in the same order, so the same bits): the top-down march
+// forms phi from b and the transports U0 = sum dz3u u, V0 = sum dz3v v
+// (when U0 is set), then the bottom-up march forms w (when w is set) from u
+// and the west u, v and the south v, the column's own velocities in cache.
+__global__ void k_prologue(Dims d, StepParams p, Geo geo, const double* __restrict__ u,
+                           const double* __restrict__ v, const double* __restrict__ b,
+                           double* __restrict__ phi, double* __restrict__ U0,
+                           double* __restrict__ V0, double* __restrict__ w) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cn = at2(d, wrap(j, 1, d.ny), i);
+  const ColumnGeo cg(geo, p, d, c);
+  const FaceGeo fe(geo, p, d, c, ce, 0), fn(geo, p, d, c, cn, 1);
+  double above = 0.0, su = 0.0, sv = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = k * n2 + c;
+    const double dz = cg.dz(k);
+    const double bk = b[t];
+    phi[t] = 0.5 * dz * bk + above;
+    const double tb = dz * bk;
+    above = (k == 0) ? tb : above + tb;
+    if (U0) {
+      const double tu = fe.dz(k) * u[t];
+      const double tv = fn.dz(k) * v[t];
+      su = (k == 0) ? tu : su + tu;
+      sv = (k == 0) ? tv : sv + tv;
+    }
+  }
+  if (U0) {
+    U0[c] = su;
+    V0[c] = sv;
+  }
+  if (p.pgf_remove) {
+    double tot = 0.0, wsum = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      const double dz = cg.dz(k);
+      const double t = dz * phi[k * n2 + c];
+      tot = (k == 0) ? dz : tot + dz;
+      wsum = (k == 0) ? t : wsum + t;
+    }
+    const double mean = wsum * safe_inv(tot);
+    for (int k = 0; k < nz; ++k) phi[k * n2 + c] = phi[k * n2 + c] - mean;
+  }
+  if (!w) return;
+  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
+  const FaceGeo fw(geo, p, d, cw, c, 0), fs(geo, p, d, cs, c, 1);
+  double sw = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long o = k * n2;
+    const double U = fe.dz(k) * u[o + c];
+    const double Uw = fw.dz(k) * u[o + cw];
+    const double V = fn.dz(k) * v[o + c];
+    const double Vs = fs.dz(k) * v[o + cs];
+    const double dv = (cg.wet(k) ? 1.0 : 0.0) * ((U - Uw) / p.dx + (V - Vs) / p.dy);
+    sw = (k == nz - 1) ? dv : sw + dv;
+    w[o + c] = -sw;
+  }
+  w[(long long)nz * n2 + c] = 0.0;
+}
+
 // ------------------------------------------------ explicit momentum terms
 struct Nb {  // periodic neighbour offsets of a column (a plane has < 2^31 columns)
   int c, e, w, ee, ww, n, s, nn, ss, ne, nw, se, sw;

@@ -133,6 +133,9 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=60,
                     help="steps of the adjoint:forward measurement on the 256 x 256 x 30 "
                          "basin (0: skip it)")
+    ap.add_argument("--adjoint-fd-directions", type=int, default=16,
+                    help="directions of the central-difference gradient check after the "
+                         "adjoint timing (0: skip the check)")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the plain-vs-mixed comparison on the 1024 x 1024 x 60 "
                          "basin (0: skip it)")
@@ -554,7 +557,8 @@ def main() -> int:
     if args.ensemble_steps > 0:
         rec["ensemble"] = ensemble_throughput(args.ensemble_steps)
     if args.adjoint_steps > 0:
-        rec["adjoint"] = adjoint_ratio(args.adjoint_steps)
+        rec["adjoint"] = adjoint_ratio(args.adjoint_steps,
+                                       fd_directions=args.adjoint_fd_directions)
     if args.ladder != "none":
         names = list(LADDER) if args.ladder == "all" else args.ladder.split(",")
         rec["ladder"] = reference_ladder(names)

@@ -179,6 +179,8 @@ struct hoc_stepper {
   int nlev = 1;
   int mg_cycle_l0 = -1;  // first level of the one-workgroup coarse cycle, -1: none
   bool mg_masks_built = false;  // the coarse levels' wet masks (static) are formed
+  long long coeff_gen = 0;       // rebuilds of Ku, Kv, 1/diag (build_coefficients)
+  long long mg_gen = -1;         // coeff_gen the multigrid levels were built from
   MgLevel lev[MAXLEV];
   double *mg_Ku[MAXLEV] = {}, *mg_Kv[MAXLEV] = {}, *mg_inv[MAXLEV] = {};
   uint8_t* mg_mask[MAXLEV] = {};
@@ -561,6 +563,7 @@ int build_coefficients(hoc_stepper* h, hipStream_t s) {
   k_kukv<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, h->q, h->Ku, h->Kv); }
   LAUNCH_CHECK("k_kukv");
   h->tridiagonal_solves += 1;
+  h->coeff_gen += 1;  // Ku, Kv and 1/diag are new: the multigrid levels follow them
   if (h->cfg.scheme == 0) {
     HelmOp op{h->Ku, h->Kv, h->m2, h->p.helm_coef, h->p.helm_cx, h->p.helm_cy, h->p.dx2,
               h->p.dy2};
@@ -607,11 +610,25 @@ PcgArgs pcg_args(hoc_stepper* h, const HelmOp& op, const double* inv_diag, const
   return a;
 }
 
-// Read back (iterations, converged) after a solve; synchronises.
-int pcg_finish(hoc_stepper* h, hipStream_t s, bool persistent, int* iters_out, int* conv_out) {
+// Poll of a per-phase solve between batches: it, iters, done, converged
+// (the end of PcgScalars) into host_flag[0..3] in one copy; synchronises.
+// Returns done.
+int poll_done(hoc_stepper* h, hipStream_t s, int* done) {
   HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->it, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h->host_flag + 4, h->bar_err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  *done = h->host_flag[2];
+  return HOC_OK;
+}
+
+// Read back (iterations, converged) after a solve; synchronises, except
+// after a poll that saw the solve done (polled: host_flag holds them).
+int pcg_finish(hoc_stepper* h, hipStream_t s, bool persistent, int* iters_out, int* conv_out,
+               bool polled = false) {
+  if (!polled || persistent) {
+    HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->it, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->host_flag + 4, h->bar_err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
   if (persistent && h->host_flag[4]) {
     (void)hipMemset(h->bar_err, 0, sizeof(int));
     (void)hipMemset(h->bar, 0, 2 * sizeof(unsigned));
@@ -829,7 +846,14 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
   if (np == 0) return fail(HOC_ERR_INVALID, "empty grid");
   const int kind = h->cfg.solver;
   if (kind != SOLVER_PCG_JACOBI) {
-    TRY(mg_build(h, Ku, Kv, inv_diag, s));
+    // the levels of the step's own coefficients are built once per rebuild
+    // of them (both Picard passes solve with the same Ku, Kv); any other
+    // operands are built for this solve alone
+    const bool own = Ku == h->Ku && Kv == h->Kv && inv_diag == h->inv_diag;
+    if (!own || h->mg_gen != h->coeff_gen) {
+      TRY(mg_build(h, Ku, Kv, inv_diag, s));
+      h->mg_gen = own ? h->coeff_gen : -1;
+    }
     const int max_iter = h->p.max_iter;
     k_pcg_init<<<np, PCG_WG, 0, s>>>(a);   // r = rhs * mask, x = 0, ||rhs||^2 parts
     k_mg_start<<<1, PCG_WG, 0, s>>>(a);
@@ -848,13 +872,13 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
           k_mg_check<<<1, PCG_WG, 0, s>>>(ar, it);
         }
         LAUNCH_CHECK("stationary iteration");
-        HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (h->host_flag[0]) break;
+        int done = 0;
+        TRY(poll_done(h, s, &done));
+        if (done) break;
         if (it > max_iter) return fail(HOC_ERR_DEVICE, "solver did not report completion");
         batch = std::min(2 * batch, 64);  // slow stationary convergence: fewer polls
       }
-      return pcg_finish(h, s, false, iters_out, conv_out);
+      return pcg_finish(h, s, false, iters_out, conv_out, true);
     }
     // PCG with the RBGS or multigrid preconditioner
     TRY(precondition(h, h->r, h->z, s));
@@ -895,13 +919,13 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
         }
       }
       LAUNCH_CHECK("pcg iteration");
-      HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      if (h->host_flag[0]) break;
+      int done = 0;
+      TRY(poll_done(h, s, &done));
+      if (done) break;
       if (it > max_iter + 1) return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
       batch = 4;
     }
-    return pcg_finish(h, s, false, iters_out, conv_out);
+    return pcg_finish(h, s, false, iters_out, conv_out, true);
   }
   if (h->pcg_parts > 0 && !h->force_multikernel) {
     a.sc = h->sc;
@@ -930,14 +954,14 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
       }
     }
     LAUNCH_CHECK("pcg iteration");
-    HIP_TRY(hipMemcpyAsync(h->host_flag, &h->sc->done, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (h->host_flag[0]) break;
+    int done = 0;
+    TRY(poll_done(h, s, &done));
+    if (done) break;
     if (it > max_iter + 1)
       return fail(HOC_ERR_DEVICE, "Helmholtz solve did not report completion");
     batch = 4;
   }
-  return pcg_finish(h, s, false, iters_out, conv_out);
+  return pcg_finish(h, s, false, iters_out, conv_out, true);
 }
 
 }  // namespace

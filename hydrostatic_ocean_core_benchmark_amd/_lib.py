@@ -90,6 +90,16 @@ class HocState(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in ("u", "v", "b", "eta", "T", "S", "e")]
 
 
+TAPE_MAX_PICARD = 4
+
+
+class HocStepTape(C.Structure):
+    _fields_ = [("q", C.c_void_p), ("Ku", C.c_void_p), ("Kv", C.c_void_p),
+                ("w_old", C.c_void_p), ("w", C.c_void_p),
+                ("gh_u", C.c_void_p * TAPE_MAX_PICARD), ("gh_v", C.c_void_p * TAPE_MAX_PICARD),
+                ("eta", C.c_void_p * TAPE_MAX_PICARD)]
+
+
 class HocCounters(C.Structure):
     _fields_ = [(name, i64) for name in ("solver_iterations", "solver_failures",
                                          "solver_rebuilds", "tridiagonal_solves",
@@ -117,6 +127,8 @@ PROTOTYPES = {
     "hoc_thomas_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, i64, i64, VP], i32),
     "hoc_stepper_tracer_advection_adjoint": ([VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP], i32),
     "hoc_stepper_step_n": ([VP, i32, VP, VP, VP], i32),
+    "hoc_stepper_step_tape": ([VP, C.POINTER(HocState), C.POINTER(HocState),
+                               C.POINTER(HocStepTape), VP], i32),
     "hoc_stepper_set_sync_free": ([VP, i32], i32),
     "hoc_stepper_set_mg_tile": ([VP, i32], i32),
     "hoc_stepper_geometry_form": ([VP, i32, C.POINTER(i32)], i32),

@@ -143,6 +143,49 @@ def laplacian(a, dx, dy, mask_u, mask_v, mask):
     return div(gradx(a, dx, mask_u), grady(a, dy, mask_v), dx, dy, mask)
 
 
+# ------------------------------------- transposes of the C-grid operators
+# east and north are rolls by -1, west and south by +1: east^T = west and
+# north^T = south. Each function below is the adjoint of the operator of the
+# same name without the suffix, at the same masks.
+def gradx_T(X, dx, mask_u):
+    m = mask_u * X
+    return (west(m) - m) / dx
+
+
+def grady_T(Y, dy, mask_v):
+    m = mask_v * Y
+    return (south(m) - m) / dy
+
+
+def div_T(D, dx, dy, mask):
+    m = mask * D
+    return (m - east(m)) / dx, (m - north(m)) / dy
+
+
+def _mean_weight(G, den, mask_out):
+    pos = den > 0.0
+    return torch.where(pos, G * mask_out / torch.where(pos, den, 1.0), 0.0)
+
+
+def avg_v_to_u_T(G, mask_v, mask_u):
+    ms = south(mask_v)
+    me, mes = east(mask_v), east(ms)
+    H = _mean_weight(G, mask_v + ms + me + mes, mask_u)
+    return mask_v * H + north(ms * H) + west(me * H) + north(west(mes * H))
+
+
+def avg_u_to_v_T(G, mask_u, mask_v):
+    mw = west(mask_u)
+    mn, mnw = north(mask_u), north(mw)
+    H = _mean_weight(G, mask_u + mw + mn + mnw, mask_v)
+    return mask_u * H + east(mw * H) + south(mn * H) + south(east(mnw * H))
+
+
+def laplacian_T(G, dx, dy, mask_u, mask_v, mask):
+    gu, gv = div_T(G, dx, dy, mask)
+    return gradx_T(gu, dx, mask_u) + grady_T(gv, dy, mask_v)
+
+
 # ---------------------------------------------------- kappa = 1/3 faces
 def superbee(r):
     return nmax(0.0, nmax(nmin(2.0 * r, 1.0), nmin(r, 2.0)))
@@ -309,6 +352,19 @@ def thomas_torch(sub, diag, sup, rhs):
     for k in range(n - 2, -1, -1):
         x[k] = ds[k] - cs[k] * x[k + 1]
     return torch.stack(x, dim=0)
+
+
+def thomas_adjoint_torch(sub, diag, sup, x, g):
+    """lambda = A^-T g for the tridiagonal A (sub[0] and sup[-1] unused) and
+    the band gradients of -lambda^T (dA) x: diag -lambda_k x_k, sub
+    -lambda_k x_{k-1}, sup -lambda_k x_{k+1}."""
+    z = torch.zeros_like(diag[:1])
+    lam = thomas_torch(torch.cat([z, sup[:-1]], dim=0), diag,
+                       torch.cat([sub[1:], z], dim=0), g)
+    g_diag = -lam * x
+    g_sub = torch.cat([z, -lam[1:] * x[:-1]], dim=0)
+    g_sup = torch.cat([-lam[:-1] * x[1:], z], dim=0)
+    return lam, g_sub, g_diag, g_sup
 
 
 class _ThomasHip(torch.autograd.Function):
@@ -784,8 +840,11 @@ class DiffStep:
         return e_new, K_m, K_h
 
     # --------------------------------------------------------------- step
-    def step(self, s: DiffState, kb: torch.Tensor) -> DiffState:
-        """libs/core/model3d_v05.py:337-415, theta branch."""
+    def step(self, s: DiffState, kb: torch.Tensor, tape: dict | None = None) -> DiffState:
+        """libs/core/model3d_v05.py:337-415, theta branch. With a dict `tape`
+        the intermediates the reverse step reads are stored in it (q, Ku, Kv,
+        w of the old velocity, the masked predictor solutions and the free
+        surface of every Picard pass, w of the new velocity)."""
         d, p, g, pr = self.domain, self.physics, self.grid, self.params
         dt, th, tc, thv = self.dt, pr.theta, pr.theta_cor, pr.theta_v
         m3, m3u, m3v = self.mask3, self.mask3u, self.mask3v
@@ -800,6 +859,8 @@ class DiffStep:
         qv = nmin(q, north(q))
         Ku = torch.sum(self.dz3u * qu, dim=0) * self.mask_u
         Kv = torch.sum(self.dz3v * qv, dim=0) * self.mask_v
+        if tape is not None:
+            tape.update(q=q, Ku=Ku, Kv=Kv, gh_u=[], gh_v=[], eta=[])
 
         pgx, pgy = self.pressure_gradients(s.b)
         av_prev_v = avg_v_to_u(s.v, m3v, m3u)
@@ -807,6 +868,8 @@ class DiffStep:
         dzu_diff = self.apply_diffusion(s.u, Km, self.dz3u, self.tau_u, p.bottom_drag)
         dzv_diff = self.apply_diffusion(s.v, Km, self.dz3v, self.tau_v, p.bottom_drag)
         w_old = self.w_diag(s.u, s.v)
+        if tape is not None:
+            tape["w_old"] = w_old
         if self.kernels == "hip":
             au, av = _MomentumAdvectionHip.apply(s.u, s.v, w_old, self.owner)
         else:
@@ -841,10 +904,16 @@ class DiffStep:
                         g.dx, g.dy, self.mask)
             rhs = s.eta - dt * ((1.0 - th) * div_old + th * div_g)
             eta_new = self.helmholtz(Ku, Kv, rhs)
+            if tape is not None:
+                tape["gh_u"].append(gh_u)
+                tape["gh_v"].append(gh_v)
+                tape["eta"].append(eta_new)
             u_it = gh_u - p.g * dt * th * q * gradx(eta_new, g.dx, self.mask_u)[None] * m3u
             v_it = gh_v - p.g * dt * th * q * grady(eta_new, g.dy, self.mask_v)[None] * m3v
             u_it, v_it = u_it * m3, v_it * m3
         w = self.w_diag(u_it, v_it)
+        if tape is not None:
+            tape["w"] = w
         # tracers (model3d_v05.py:466-533)
         flux_T = torch.full(g.shape, p.q_heat / (p.rho0 * p.cp), dtype=F64, device=self.device)
         flux_S = torch.full(g.shape, p.q_salt, dtype=F64, device=self.device)
@@ -865,6 +934,278 @@ class DiffStep:
         top = rhs[0] + thv * dt * surf_flux * safe_inv(self.dz3[0])
         rhs = torch.cat([top[None], rhs[1:]], dim=0)
         return self.thomas(*tri_b, rhs) * self.mask3
+
+    # ------------------------------------------------ hand-written reverse
+    # Transposed operators of the step's pieces. kernels="hip" runs the HIP
+    # reverse kernels; kernels="torch" takes the vector-Jacobian products of
+    # the torch formulas (the reference the HIP kernels are checked against).
+    @staticmethod
+    def _vjp(fn, ins, grads):
+        with torch.enable_grad():
+            xs = [t.detach().requires_grad_() for t in ins]
+            outs = fn(*xs)
+            outs = outs if isinstance(outs, tuple) else (outs,)
+            pairs = [(o, g) for o, g in zip(outs, grads) if g is not None]
+            got = torch.autograd.grad([o for o, _ in pairs], xs, [g for _, g in pairs],
+                                      allow_unused=True)
+        return [torch.zeros_like(x) if gi is None else gi for x, gi in zip(xs, got)]
+
+    def thomas_T(self, tri, x, g):
+        """(lambda, d_sub, d_diag, d_sup) of x = A^-1 rhs for the adjoint g of x."""
+        sub, diag, sup = (t.contiguous() for t in tri)
+        if self.kernels != "hip":
+            return thomas_adjoint_torch(sub, diag, sup, x, g)
+        st = self.owner
+        x, g = x.contiguous(), g.contiguous()
+        n = diag.shape[0]
+        lam, d_sub, d_diag, d_sup, scratch = (torch.empty_like(x) for _ in range(5))
+        _lib.check(st.lib.hoc_thomas_adjoint(_ptr(sub), _ptr(diag), _ptr(sup), _ptr(x), _ptr(g),
+                                             _ptr(lam), _ptr(d_sub), _ptr(d_diag), _ptr(d_sup),
+                                             _ptr(scratch), n, diag.numel() // n, st._stream()))
+        return lam, d_sub, d_diag, d_sup
+
+    def tracer_advection_T(self, c, u, v, w, g):
+        if self.kernels != "hip":
+            return self._vjp(self.tracer_advection, [c, u, v, w], [g])
+        st = self.owner
+        c, u, v, w, g = (t.contiguous() for t in (c, u, v, w, g))
+        gc, gu, gv, gw = (torch.empty_like(t) for t in (c, u, v, w))
+        _lib.check(st.lib.hoc_stepper_tracer_advection_adjoint(
+            st._h, _ptr(c), _ptr(u), _ptr(v), _ptr(w), _ptr(g), _ptr(gc), _ptr(gu), _ptr(gv),
+            _ptr(gw), st._stream()))
+        return gc, gu, gv, gw
+
+    def momentum_advection_T(self, u, v, w, g_au, g_av):
+        if self.kernels != "hip":
+            g, pr = self.grid, self.params
+            fn = lambda a, b, c: momentum_advection_up3(a, b, c, g.dx, g.dy, g.dz, self.mask3u,
+                                                        self.mask3v, pr.advection == "up3_tvd")
+            return self._vjp(fn, [u, v, w], [g_au, g_av])
+        st = self.owner
+        u, v, w, g_au, g_av = (t.contiguous() for t in (u, v, w, g_au, g_av))
+        gu, gv, gw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+        _lib.check(st.lib.hoc_stepper_momentum_advection_adjoint(
+            st._h, _ptr(u), _ptr(v), _ptr(w), _ptr(g_au), _ptr(g_av), _ptr(gu), _ptr(gv),
+            _ptr(gw), st._stream()))
+        return gu, gv, gw
+
+    def buoyancy_T(self, T, S, g):
+        if self.kernels != "hip":
+            return self._vjp(self.buoyancy, [T, S], [g])
+        p, st = self.physics, self.owner
+        T, S, g = T.contiguous(), S.contiguous(), g.contiguous()
+        gT, gS = torch.empty_like(T), torch.empty_like(S)
+        _lib.check(st.lib.hoc_eos_buoyancy_adjoint(
+            _EOS_KIND[p.eos], p.g, p.rho0, p.alpha_T, p.beta_S, p.T0, p.S0, _ptr(T), _ptr(S),
+            _ptr(self.zc), _ptr(g), _ptr(gT), _ptr(gS), T.numel(), st._stream()))
+        return gT, gS
+
+    def helmholtz_solve(self, Ku, Kv, rhs):
+        if self.kernels == "hip":
+            return self.helm.solve(Ku.contiguous(), Kv.contiguous(), rhs.contiguous())
+        return self.helm.solve_dense(Ku, Kv, rhs)
+
+    def diffusion_coeffs_T(self, g_sub, g_diag, g_sup, dz3, theta_v):
+        """Adjoint of diffusion_coeffs with respect to nu (vertical_var.py:35-77):
+        the rows' a_top and a_bot terms with the bottom and dry overrides."""
+        nz = dz3.shape[0]
+        dzi = self.interface_thickness(dz3)
+        fac = theta_v * self.dt * safe_inv(dz3)
+        wet = self.mask3 > 0.0
+        bot = self.bottom
+        g_sub = torch.where(wet, g_sub, 0.0)
+        g_diag = torch.where(wet, g_diag, 0.0)
+        g_sup = torch.where(wet & ~bot, g_sup, 0.0)
+        ga_top = torch.cat([torch.where(bot[:1], g_diag[:1], 0.0), g_diag[1:] - g_sub[1:]], dim=0)
+        ga_bot = torch.where(bot, 0.0, g_diag - g_sup)
+        g_nu = torch.zeros((nz + 1,) + dz3.shape[1:], dtype=dz3.dtype, device=dz3.device)
+        g_nu[:nz] += fac * safe_inv(dzi[:nz]) * ga_top
+        g_nu[1:] += fac * safe_inv(dzi[1:]) * ga_bot
+        return g_nu
+
+    def apply_diffusion_T(self, g_out, field, nu, dz3, drag):
+        """Adjoint of apply_diffusion (vertical_var.py:80-106) with respect to
+        the field and nu (the surface flux is a constant)."""
+        nz = field.shape[0]
+        dzi = self.interface_thickness(dz3)
+        inv = safe_inv(dz3)
+        oi = g_out * (self.mask3 > 0) * inv
+        g_inner = torch.where(self.bottom[:-1], 0.0, oi[1:] - oi[:-1])
+        c = nu[1:nz] * safe_inv(dzi[1:nz])
+        g_f = (-drag) * oi * self.bottom
+        g_f = g_f + torch.cat([c * g_inner, torch.zeros_like(field[:1])], dim=0)
+        g_f = g_f - torch.cat([torch.zeros_like(field[:1]), c * g_inner], dim=0)
+        g_nu = torch.zeros_like(nu)
+        g_nu[1:nz] = (field[:-1] - field[1:]) * safe_inv(dzi[1:nz]) * g_inner
+        return g_f, g_nu
+
+    def w_diag_T(self, g_w):
+        """Adjoint of w_diag: w_k = -sum_{j >= k} div_j, then the transposed
+        divergence of the face transports."""
+        g = self.grid
+        nz = g_w.shape[0] - 1
+        g_div = -torch.cumsum(g_w[:nz], dim=0)
+        gU, gV = div_T(g_div, g.dx, g.dy, self.mask3)
+        return self.dz3u * gU, self.dz3v * gV
+
+    def pressure_gradients_T(self, g_pgx, g_pgy):
+        """Adjoint of pressure_gradients with respect to b."""
+        g, p = self.grid, self.physics
+        dz3 = self.dz3
+        if not p.pgf_correction:
+            g_phi = (gradx_T(g_pgx * self.mask3u, g.dx, self.mask3u)
+                     + grady_T(g_pgy * self.mask3v, g.dy, self.mask3v))
+            g_b = torch.zeros_like(g_pgx)
+        else:
+            zc = self.zc
+            g_phi = torch.zeros_like(g_pgx)
+            g_b = torch.zeros_like(g_pgx)
+            for dim, fmask, delta, G in ((-1, self.mask3u, g.dx, g_pgx),
+                                         (-2, self.mask3v, g.dy, g_pgy)):
+                P = fmask * G / delta
+                z_r = torch.roll(zc, -1, dims=dim)
+                z_face = 0.5 * (zc + z_r)
+                g_phi = g_phi + torch.roll(P, 1, dims=dim) - P
+                g_b = g_b + torch.roll(P * (z_face - z_r), 1, dims=dim) - P * (z_face - zc)
+        if p.pgf == "remove":
+            total = torch.sum(dz3, dim=0, keepdim=True)
+            g_phi = g_phi - dz3 * safe_inv(total) * torch.sum(g_phi, dim=0, keepdim=True)
+        below = torch.flip(torch.cumsum(torch.flip(g_phi, [0]), dim=0), [0]) - g_phi
+        return g_b + 0.5 * dz3 * g_phi + dz3 * below
+
+    def face_coefficients_T(self, q, g_Ku, g_Kv):
+        """Adjoint of K_u = sum dz3u min(q, q_E) mask_u (and K_v) w.r.t. q."""
+        out = torch.zeros_like(q)
+        for G, dz, fm, fwd, back in ((g_Ku, self.dz3u, self.mask_u, east, west),
+                                     (g_Kv, self.dz3v, self.mask_v, north, south)):
+            Q = dz * (G * fm)[None]
+            keep = (q <= fwd(q)) | (q != q)
+            out = out + torch.where(keep, Q, 0.0) + back(torch.where(keep, 0.0, Q))
+        return out
+
+    def _tracer_T(self, c, u, v, w, Kh, tri_b, x, g_x):
+        """Adjoint of _advance_tracer: (g_c, g_u, g_v, g_w, g_Kh)."""
+        g, p, dt, thv = self.grid, self.physics, self.dt, self.params.theta_v
+        lam, d_sub, d_diag, d_sup = self.thomas_T(tri_b, x, g_x * self.mask3)
+        g_Kh = self.diffusion_coeffs_T(d_sub, d_diag, d_sup, self.dz3, thv)
+        g_c, g_u, g_v, g_w = self.tracer_advection_T(c, u, v, w, -dt * lam)
+        g_c = g_c + lam
+        if p.K_h > 0.0:
+            g_c = g_c + dt * p.K_h * laplacian_T(lam, g.dx, g.dy, self.mask3u, self.mask3v,
+                                                  self.mask3)
+        d_f, d_nu = self.apply_diffusion_T((1.0 - thv) * dt * lam, c, Kh, self.dz3, 0.0)
+        return g_c + d_f, g_u, g_v, g_w, g_Kh + d_nu
+
+    def reverse(self, s: DiffState, kb: torch.Tensor, y: DiffState, tape: dict, adj: dict):
+        """Hand-written reverse of step(s, kb) -> y: from the adjoints `adj`
+        of y's fields (None for none) the adjoints of s's fields and of kb,
+        with the forward's intermediates from `tape` (step(..., tape=)). The
+        closure (tke_step) alone is differentiated by autograd."""
+        d, p, g, pr = self.domain, self.physics, self.grid, self.params
+        dt, th, tc, thv = self.dt, pr.theta, pr.theta_cor, pr.theta_v
+        m3, m3u, m3v = self.mask3, self.mask3u, self.mask3v
+        dx, dy = g.dx, g.dy
+        zero3 = torch.zeros_like(s.u)
+        A = lambda k, like: adj[k] if adj.get(k) is not None else torch.zeros_like(like)
+        Km, Kh, q, Ku, Kv = y.Km, y.Kh, tape["q"], tape["Ku"], tape["Kv"]
+        g_Km = A("Km", Km).clone()
+        g_Kh = A("Kh", Kh).clone()
+        # the EOS of the new tracers
+        g_T, g_S = A("T", y.T), A("S", y.S)
+        if adj.get("b") is not None:
+            dT, dS = self.buoyancy_T(y.T, y.S, adj["b"] * m3)
+            g_T, g_S = g_T + dT, g_S + dS
+        # tracers (model3d_v05.py:466-533)
+        tri_b = self.diffusion_coeffs(Kh, self.dz3, thv, 0.0)
+        w = tape["w"]
+        gT_in, gu1, gv1, gw, dKh = self._tracer_T(s.T, y.u, y.v, w, Kh, tri_b, y.T, g_T)
+        gS_in, gu2, gv2, gw2, dKh2 = self._tracer_T(s.S, y.u, y.v, w, Kh, tri_b, y.S, g_S)
+        g_Kh = g_Kh + dKh + dKh2
+        gwu, gwv = self.w_diag_T(gw + gw2)
+        g_ui = A("u", y.u) + gu1 + gu2 + gwu
+        g_vi = A("v", y.v) + gv1 + gv2 + gwv
+        # Picard passes in reverse (model3d_v05.py:387-408)
+        tri_m = self.diffusion_coeffs(Km, self.dz3, thv, p.bottom_drag)
+        cH = p.g * dt * th
+        g_u, g_v, g_b = zero3.clone(), zero3.clone(), zero3.clone()
+        g_eta = torch.zeros_like(s.eta)
+        g_eta_p = A("eta", y.eta)
+        g_q = torch.zeros_like(q)
+        g_Ku, g_Kv = torch.zeros_like(Ku), torch.zeros_like(Kv)
+        g_div_old = torch.zeros_like(s.eta)
+        band = [zero3.clone(), zero3.clone(), zero3.clone()]
+        L_u, L_v = zero3.clone(), zero3.clone()
+        cor_v, cor_u = zero3.clone(), zero3.clone()
+        for it in range(pr.n_picard - 1, -1, -1):
+            eta_it, ghu, ghv = tape["eta"][it], tape["gh_u"][it], tape["gh_v"][it]
+            Au, Av = g_ui * m3, g_vi * m3
+            gxe = gradx(eta_it, dx, self.mask_u)[None] * m3u
+            gye = grady(eta_it, dy, self.mask_v)[None] * m3v
+            g_q = g_q - cH * (gxe * Au + gye * Av)
+            X = torch.sum(-cH * q * m3u * Au, dim=0)
+            Y = torch.sum(-cH * q * m3v * Av, dim=0)
+            ge = g_eta_p + gradx_T(X, dx, self.mask_u) + grady_T(Y, dy, self.mask_v)
+            lam_eta = self.helmholtz_solve(Ku, Kv, ge)
+            dKu, dKv = self.helm.coefficient_gradients(eta_it, lam_eta)
+            g_Ku, g_Kv = g_Ku + dKu, g_Kv + dKv
+            g_eta = g_eta + lam_eta
+            g_div_old = g_div_old - dt * (1.0 - th) * lam_eta
+            GU, GV = div_T(-dt * th * lam_eta, dx, dy, self.mask)
+            lu, a1, a2, a3 = self.thomas_T(tri_m, ghu, (Au + self.dz3u * GU[None]) * m3)
+            lv, b1, b2, b3 = self.thomas_T(tri_m, ghv, (Av + self.dz3v * GV[None]) * m3)
+            band = [band[0] + a1 + b1, band[1] + a2 + b2, band[2] + a3 + b3]
+            L_u, L_v = L_u + lu, L_v + lv
+            gvc, guc = dt * p.f * lu, -dt * p.f * lv
+            cor_v, cor_u = cor_v + gvc, cor_u + guc
+            g_eta_p = torch.zeros_like(s.eta)
+            if it > 0:   # the previous pass's velocity enters this pass's Coriolis terms
+                g_ui = tc * avg_u_to_v_T(guc, m3u, m3v)
+                g_vi = tc * avg_v_to_u_T(gvc, m3v, m3u)
+            else:
+                g_u = g_u + tc * avg_u_to_v_T(guc, m3u, m3v)
+                g_v = g_v + tc * avg_v_to_u_T(gvc, m3v, m3u)
+        # the step's explicit terms, all linear in the L's
+        g_u = g_u + (1.0 - tc) * avg_u_to_v_T(cor_u, m3u, m3v) + L_u
+        g_v = g_v + (1.0 - tc) * avg_v_to_u_T(cor_v, m3v, m3u) + L_v
+        g_b = g_b + self.pressure_gradients_T(dt * L_u, dt * L_v)
+        ge_x = torch.sum(m3u * (-p.g * dt * (1.0 - th)) * L_u, dim=0)
+        ge_y = torch.sum(m3v * (-p.g * dt * (1.0 - th)) * L_v, dim=0)
+        g_eta = g_eta + gradx_T(ge_x, dx, self.mask_u) + grady_T(ge_y, dy, self.mask_v)
+        du, dKm_u = self.apply_diffusion_T((1.0 - thv) * dt * L_u, s.u, Km, self.dz3u,
+                                           p.bottom_drag)
+        dv, dKm_v = self.apply_diffusion_T((1.0 - thv) * dt * L_v, s.v, Km, self.dz3v,
+                                           p.bottom_drag)
+        g_u, g_v, g_Km = g_u + du, g_v + dv, g_Km + dKm_u + dKm_v
+        gex_u, gex_v = dt * L_u, dt * L_v
+        if p.A_h > 0.0:
+            g_u = g_u + p.A_h * laplacian_T(gex_u, dx, dy, m3u, m3v, m3u)
+            g_v = g_v + p.A_h * laplacian_T(gex_v, dx, dy, m3u, m3v, m3v)
+        du, dv, dw = self.momentum_advection_T(s.u, s.v, tape["w_old"], -(m3u * gex_u),
+                                               -(m3v * gex_v))
+        dwu, dwv = self.w_diag_T(dw)
+        g_u, g_v = g_u + du + dwu, g_v + dv + dwv
+        GU, GV = div_T(g_div_old, dx, dy, self.mask)
+        g_u = g_u + self.dz3u * GU[None]
+        g_v = g_v + self.dz3v * GV[None]
+        # q = A_m^-1 1 and the face coefficients K_u, K_v (model3d_v05.py:150-167)
+        g_q = g_q + self.face_coefficients_T(q, g_Ku, g_Kv)
+        _, c1, c2, c3 = self.thomas_T(tri_m, q, g_q)
+        band = [band[0] + c1, band[1] + c2, band[2] + c3]
+        g_Km = g_Km + self.diffusion_coeffs_T(*band, self.dz3, thv)
+        # the closure (closure.py:141-205) by autograd
+        out = {"u": g_u, "v": g_v, "b": g_b, "eta": g_eta, "T": gT_in, "S": gS_in,
+               "e": None, "Km": None, "Kh": None}
+        g_kb = torch.zeros_like(kb)
+        if p.closure == "tke":
+            ins = [s.e, s.u, s.v, s.b, s.Km, s.Kh, kb]
+            got = self._vjp(self.tke_step, ins, [adj.get("e"), g_Km, g_Kh])
+            for k, gk in zip(("e", "u", "v", "b", "Km", "Kh"), got[:6]):
+                out[k] = gk if out[k] is None else out[k] + gk
+            g_kb = got[6]
+        else:
+            out["Km"], out["Kh"] = g_Km, g_Kh
+            out["e"] = adj.get("e")
+        return out, g_kb
 
     # ------------------------------------------------------- initial state
     def initial_state(self, T0, S0, kb) -> DiffState:
@@ -912,12 +1253,18 @@ class Adjoint:
     over checkpoints: the plain HIP step runs forward keeping the full state
     (u, v, b, eta, T, S, e, K_m, K_h) every `every` steps; the reverse sweep
     recomputes each window's states with the HIP step, then back-propagates
-    through the differentiable step one step at a time."""
+    one step at a time: reverse="hand" (default) recomputes the window with
+    hoc_stepper_step_tape and runs DiffStep.reverse on each step's tape;
+    reverse="autograd" differentiates the torch restatement of each step."""
 
-    def __init__(self, domain, physics, params, dt, device="cuda:0", kernels="hip"):
+    def __init__(self, domain, physics, params, dt, device="cuda:0", kernels="hip",
+                 reverse="hand"):
+        if reverse not in ("hand", "autograd"):
+            raise ValueError("reverse must be hand or autograd")
         self.domain, self.physics, self.params, self.dt = domain, physics, params, float(dt)
         self.device = torch.device(device)
         self.kernels = kernels
+        self.reverse = reverse
         self.ds = DiffStep(domain, physics, params, dt, device=device, kernels=kernels)
 
     def _hip(self, kb_np):
@@ -940,6 +1287,33 @@ class Adjoint:
         _lib.check(st.lib.hoc_stepper_build_coefficients(st._h, st._stream()))
         return State3D(u=c.u.clone(), v=c.v.clone(), b=c.b.clone(), eta=c.eta.clone(), t=0.0,
                        T=c.T.clone(), S=c.S.clone(), e=None if c.e is None else c.e.clone())
+
+    def _reverse_window(self, st, c0: DiffState, start: int, stop: int, kb, adj: dict):
+        """Steps start..stop-1 recomputed from their checkpoint c0 with the
+        tape, then reversed last to first; `adj` (the adjoint of the state
+        after step stop-1) becomes that of the state before step start.
+        Returns this window's share of dJ/dkappa_b."""
+        ds = self.ds
+        s = self._restore(st, c0)
+        fin = st._inputs(s)
+        states, tapes = [c0], []
+        for _ in range(start, stop):
+            fout = st._new_state_buffers()
+            tape = st._new_tape()
+            st._step_tape_into(fin, fout, tape)
+            tapes.append(tape)
+            states.append(DiffState(u=fout["u"], v=fout["v"], b=fout["b"], eta=fout["eta"],
+                                    T=fout["T"], S=fout["S"], e=fout.get("e"),
+                                    Km=st.nu.clone(), Kh=st.kappa.clone()))
+            fin = fout
+        g_kb = torch.zeros_like(kb)
+        for i in range(stop - start - 1, -1, -1):
+            out, gk = ds.reverse(states[i], kb, states[i + 1], tapes[i], adj)
+            adj.clear()
+            adj.update(out)
+            g_kb += gk
+            states[i + 1] = tapes[i] = None
+        return g_kb
 
     def J_forward(self, T0, S0, kb, n_steps):
         """J after n steps of the plain HIP step (for finite differences)."""
@@ -989,6 +1363,9 @@ class Adjoint:
         # reverse sweep, window by window
         for start in sorted(ckpt, reverse=True):
             stop = min(start + every, n_steps)
+            if self.reverse == "hand":
+                g_kb += self._reverse_window(st, ckpt[start], start, stop, kb, adj)
+                continue
             states = [ckpt[start]]
             if stop - start > 1:
                 s = self._restore(st, ckpt[start])

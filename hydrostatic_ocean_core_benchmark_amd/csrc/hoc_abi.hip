@@ -1998,8 +1998,19 @@ int split_barotropic(hoc_stepper* h, const double* eta, const double* gh_u, cons
 // One step; w_ready: h->w already holds w of in->u, in->v (the previous step's
 // output on a static geometry), so the step does not form it again.
 static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream,
-                     bool w_ready) {
+                     bool w_ready, const hoc_step_tape* tape = nullptr) {
   if (!h || !in || !out) return fail(HOC_ERR_INVALID, "null argument");
+  if (tape) {
+    if (h->cfg.scheme != 0 || h->cfg.vcoord != 0)
+      return fail(HOC_ERR_UNSUPPORTED, "the step tape covers the theta scheme on z-level grids");
+    if (h->cfg.n_picard > HOC_TAPE_MAX_PICARD)
+      return fail(HOC_ERR_UNSUPPORTED, "the step tape holds at most HOC_TAPE_MAX_PICARD passes");
+    if (!tape->q || !tape->Ku || !tape->Kv || !tape->w_old || !tape->w)
+      return fail(HOC_ERR_INVALID, "tape arrays q, Ku, Kv, w_old, w are required");
+    for (int it = 0; it < h->cfg.n_picard; ++it)
+      if (!tape->gh_u[it] || !tape->gh_v[it] || !tape->eta[it])
+        return fail(HOC_ERR_INVALID, "tape arrays gh_u, gh_v, eta are required for every pass");
+  }
   if (!in->u || !in->v || !in->b || !in->eta || !out->u || !out->v || !out->b || !out->eta)
     return fail(HOC_ERR_INVALID, "state arrays u, v, b, eta are required");
   const bool ts = h->cfg.tracers_ts != 0;
@@ -2058,6 +2069,8 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   k_prologue<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->b, h->phi,
                                           (theta || split) ? h->U0 : nullptr, h->V0,
                                           w_ready ? nullptr : h->w); }
+  if (tape)
+    HIP_TRY(hipMemcpyAsync(tape->w_old, h->w, d.ni * sizeof(double), hipMemcpyDeviceToDevice, s));
   { ProfScope _ps(h, "k_momentum_explicit", s);
   launch_momentum_explicit_tiled((h->fp32_parts & HOC_FP32_MOMENTUM_FLUX) != 0, s, d, p, geo,
                                  in->u, in->v, in->b, h->phi,
@@ -2068,8 +2081,8 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   const double* u_it = in->u;
   const double* v_it = in->v;
   for (int it = 0; it < h->cfg.n_picard; ++it) {
-    double* gu = h->gh_u[it & 1];
-    double* gv = h->gh_v[it & 1];
+    double* gu = tape ? tape->gh_u[it] : h->gh_u[it & 1];
+    double* gv = tape ? tape->gh_v[it] : h->gh_v[it & 1];
     { ProfScope _ps(h, "k_predictor", s);
     if (p.zcol)
       k_predictor_col<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->u, in->v, in->eta, u_it, v_it,
@@ -2088,6 +2101,9 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       LAUNCH_CHECK("k_rhs_eta");
       int iters = 0, conv = 0;
       TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
+      if (tape)
+        HIP_TRY(hipMemcpyAsync(tape->eta[it], out->eta, d.n2 * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
       if (it + 1 == h->cfg.n_picard) {
         // the last pass: back-substitution and w of the new velocity in one
         // column pass (the tracers read w next)
@@ -2127,11 +2143,23 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
   LAUNCH_CHECK("k_tracers");
   h->tridiagonal_solves += ts ? 2 : 1;
   if (ts) h->eos_cells += d.n3;
+  if (tape) {
+    HIP_TRY(hipMemcpyAsync(tape->q, h->q, d.n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(tape->Ku, h->Ku, d.n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(tape->Kv, h->Kv, d.n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(tape->w, h->w, d.ni * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
   return HOC_OK;
 }
 
 int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream) {
   return step_impl(h, in, out, stream, false);
+}
+
+int hoc_stepper_step_tape(hoc_stepper* h, const hoc_state* in, hoc_state* out,
+                          const hoc_step_tape* tape, void* stream) {
+  if (!tape) return fail(HOC_ERR_INVALID, "null tape");
+  return step_impl(h, in, out, stream, false, tape);
 }
 
 int hoc_stepper_z_centre(hoc_stepper* h, double* out, void* stream) {

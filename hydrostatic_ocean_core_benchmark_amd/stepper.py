@@ -276,6 +276,36 @@ class HipStepper:
         if self.physics.closure != "tke":
             fout["e"] = fin.get("e")
 
+    def _new_tape(self) -> dict:
+        """Device arrays of one step's tape (hoc_step_tape)."""
+        g = self.grid
+        opts = dict(dtype=torch.float64, device=self.device)
+        shape_i = (g.nz + 1,) + g.shape
+        n = self.params.n_picard
+        return {"q": torch.empty(g.shape3d, **opts), "Ku": torch.empty(g.shape, **opts),
+                "Kv": torch.empty(g.shape, **opts), "w_old": torch.empty(shape_i, **opts),
+                "w": torch.empty(shape_i, **opts),
+                "gh_u": [torch.empty(g.shape3d, **opts) for _ in range(n)],
+                "gh_v": [torch.empty(g.shape3d, **opts) for _ in range(n)],
+                "eta": [torch.empty(g.shape, **opts) for _ in range(n)]}
+
+    def _step_tape_into(self, fin: dict, fout: dict, tape: dict) -> None:
+        """One step fin -> fout (hoc_stepper_step_tape) writing `tape`."""
+        if self.params.n_picard > _lib.TAPE_MAX_PICARD:
+            raise ValueError(f"the step tape holds at most {_lib.TAPE_MAX_PICARD} Picard passes")
+        t = _lib.HocStepTape()
+        for k in ("q", "Ku", "Kv", "w_old", "w"):
+            setattr(t, k, tape[k].data_ptr())
+        for k in ("gh_u", "gh_v", "eta"):
+            arr = getattr(t, k)
+            for i, buf in enumerate(tape[k]):
+                arr[i] = buf.data_ptr()
+        sin, sout = self._hoc_state(fin), self._hoc_state(fout)
+        _lib.check(self.lib.hoc_stepper_step_tape(self._h, C.byref(sin), C.byref(sout),
+                                                  C.byref(t), self._stream()))
+        if self.physics.closure != "tke":
+            fout["e"] = fin.get("e")
+
     def _step_n_into(self, a: dict, b: dict, n: int) -> dict:
         """n steps from a, ping-ponging between the buffer sets a and b in one
         library call (hoc_stepper_step_n: from the second step on, w of the

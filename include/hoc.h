@@ -122,6 +122,29 @@ int hoc_stepper_build_coefficients(hoc_stepper* h, void* stream);
  * Helmholtz solve to poll convergence. */
 int hoc_stepper_step(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* stream);
 
+/* The intermediates of one step that its reverse (the adjoint sweep) reads,
+ * caller-owned device arrays: q = A_m^-1 1 and the face coefficients K_u,
+ * K_v of the step's mixing (model3d_v05.py:150-167), w of the input and of
+ * the output velocity (:232-235), and per Picard pass the masked predictor
+ * solutions gh_u, gh_v and the free surface it solved for (:387-408). */
+#define HOC_TAPE_MAX_PICARD 4
+typedef struct hoc_step_tape {
+  double* q;                          /* [nz][ny][nx] */
+  double* Ku;                         /* [ny][nx] */
+  double* Kv;                         /* [ny][nx] */
+  double* w_old;                      /* [nz+1][ny][nx] */
+  double* w;                          /* [nz+1][ny][nx] */
+  double* gh_u[HOC_TAPE_MAX_PICARD];  /* [nz][ny][nx], passes 0..n_picard-1 */
+  double* gh_v[HOC_TAPE_MAX_PICARD];
+  double* eta[HOC_TAPE_MAX_PICARD];   /* [ny][nx] */
+} hoc_step_tape;
+
+/* hoc_stepper_step that also writes the tape (no reference counterpart; the
+ * forward half of the checkpointed adjoint). The same results as
+ * hoc_stepper_step, bit for bit. Theta scheme, z-level. */
+int hoc_stepper_step_tape(hoc_stepper* h, const hoc_state* in, hoc_state* out,
+                          const hoc_step_tape* tape, void* stream);
+
 /* The current depth of the cell centres [nz][ny][nx] (device, written on
  * `stream`): fixed for z-level, rebuilt from eta every step for z* and sigma
  * (the reference's self.z_centre, model3d_v05.py:128-139). */

@@ -252,6 +252,31 @@ def test_hand_written_adjoint_equals_autograd_and_checkpointing(adj):
     assert _rel(r2.dJ_dkappa, gk) < 1e-8, _rel(r2.dJ_dkappa, gk)
     assert _rel(r2.dJ_dT0, r5.dJ_dT0) < 1e-12
     assert _rel(r2.dJ_dkappa, r5.dJ_dkappa) < 1e-12
+    # the hand-written reverse step (the default) against autograd through
+    # the torch restatement with the same HIP operators
+    ag = adj.Adjoint(prob.domain, prob.physics, prob.params, prob.dt, kernels="hip",
+                     reverse="autograd").gradient(T0, S0, kb, n_steps=5, every=2)
+    assert _rel(r2.dJ_dT0, ag.dJ_dT0) < 1e-10, _rel(r2.dJ_dT0, ag.dJ_dT0)
+    assert _rel(r2.dJ_dkappa, ag.dJ_dkappa) < 1e-10, _rel(r2.dJ_dkappa, ag.dJ_dkappa)
+
+
+def test_hand_reverse_other_branches_equal_autograd(adj):
+    """The other branches of the hand-written reverse step (no closure,
+    unlimited up3, pgf remove, bottom drag, no A_h) against autograd through
+    the torch restatement: 24 x 24 x 30 basin, 4 steps, checkpoints every 2."""
+    prob = _basin(24, nz=30, advection="up3", closure="none")
+    prob.physics = dataclasses.replace(
+        prob.physics, pgf="remove",
+        base=dataclasses.replace(prob.physics.base, A_h=0.0, bottom_drag=2e-3))
+    kb = _kappa(prob)
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    run = lambda how: adj.Adjoint(prob.domain, prob.physics, prob.params, prob.dt,
+                                  kernels="hip", reverse=how).gradient(T0, S0, kb, n_steps=4,
+                                                                       every=2)
+    hand, ag = run("hand"), run("autograd")
+    assert _rel(hand.dJ_dT0, ag.dJ_dT0) < 1e-10, _rel(hand.dJ_dT0, ag.dJ_dT0)
+    assert _rel(hand.dJ_dkappa, ag.dJ_dkappa) < 1e-10, _rel(hand.dJ_dkappa, ag.dJ_dkappa)
 
 
 def test_adjoint_against_finite_differences(adj):

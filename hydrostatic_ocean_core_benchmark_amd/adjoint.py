@@ -1096,11 +1096,77 @@ class DiffStep:
         d_f, d_nu = self.apply_diffusion_T((1.0 - thv) * dt * lam, c, Kh, self.dz3, 0.0)
         return g_c + d_f, g_u, g_v, g_w, g_Kh + d_nu
 
-    def reverse(self, s: DiffState, kb: torch.Tensor, y: DiffState, tape: dict, adj: dict):
+    def reverse(self, s: DiffState, kb: torch.Tensor, y: DiffState, tape: dict, adj: dict,
+                fused: bool | None = None):
         """Hand-written reverse of step(s, kb) -> y: from the adjoints `adj`
         of y's fields (None for none) the adjoints of s's fields and of kb,
-        with the forward's intermediates from `tape` (step(..., tape=)). The
-        closure (tke_step) alone is differentiated by autograd."""
+        with the forward's intermediates from `tape` (step(..., tape=) or
+        hoc_stepper_step_tape). fused (the default with kernels="hip"): the
+        step after its closure reversed by hoc_stepper_step_adjoint's
+        kernels; otherwise by the transposed operators below. The closure
+        (tke_step) is differentiated by autograd either way."""
+        if fused is None:
+            fused = self.kernels == "hip"
+        if fused:
+            return self._reverse_fused(s, kb, y, tape, adj)
+        return self._reverse_ops(s, kb, y, tape, adj)
+
+    def _closure_T(self, s: DiffState, kb, out: dict, g_e, g_Km, g_Kh):
+        """The closure's share of the reverse step added into `out`; returns
+        dJ/dkappa_b of the step."""
+        if self.physics.closure == "tke":
+            ins = [s.e, s.u, s.v, s.b, s.Km, s.Kh, kb]
+            got = self._vjp(self.tke_step, ins, [g_e, g_Km, g_Kh])
+            for k, gk in zip(("e", "u", "v", "b", "Km", "Kh"), got[:6]):
+                out[k] = gk if out.get(k) is None else out[k] + gk
+            return got[6]
+        out["Km"], out["Kh"], out["e"] = g_Km, g_Kh, g_e
+        return torch.zeros_like(kb)
+
+    def _reverse_fused(self, s: DiffState, kb, y: DiffState, tape: dict, adj: dict):
+        st = self.owner
+        g = self.grid
+        opts = dict(dtype=F64, device=self.device)
+        gx = {k: torch.empty(g.shape3d, **opts) for k in ("u", "v", "b", "T", "S")}
+        gx["eta"] = torch.empty(g.shape, **opts)
+        shape_i = (g.nz + 1,) + g.shape
+        g_Km, g_Kh = torch.empty(shape_i, **opts), torch.empty(shape_i, **opts)
+        ptr = lambda t: None if t is None else t.contiguous().data_ptr()
+        hx, hy, hgy, hgx = _lib.HocState(), _lib.HocState(), _lib.HocState(), _lib.HocState()
+        keep = []
+        for name in ("u", "v", "T", "S"):
+            a, b = getattr(s, name).contiguous(), getattr(y, name).contiguous()
+            keep += [a, b]
+            setattr(hx, name, a.data_ptr())
+            setattr(hy, name, b.data_ptr())
+        for name in ("u", "v", "b", "eta", "T", "S"):
+            t = adj.get(name)
+            if t is not None:
+                t = t.contiguous()
+                keep.append(t)
+            setattr(hgy, name, ptr(t))
+            setattr(hgx, name, gx[name].data_ptr())
+        gy_Km = None if adj.get("Km") is None else adj["Km"].contiguous()
+        gy_Kh = None if adj.get("Kh") is None else adj["Kh"].contiguous()
+        t = _lib.HocStepTape()
+        for k in ("q", "Ku", "Kv", "w_old", "w"):
+            setattr(t, k, tape[k].data_ptr())
+        for k in ("gh_u", "gh_v", "eta"):
+            arr = getattr(t, k)
+            for i, buf in enumerate(tape[k]):
+                arr[i] = buf.data_ptr()
+        Km, Kh = y.Km.contiguous(), y.Kh.contiguous()
+        _lib.check(st.lib.hoc_stepper_step_adjoint(
+            st._h, C.byref(hx), C.byref(hy), Km.data_ptr(), Kh.data_ptr(), C.byref(t),
+            C.byref(hgy), ptr(gy_Km), ptr(gy_Kh), C.byref(hgx), g_Km.data_ptr(),
+            g_Kh.data_ptr(), st._stream()))
+        out = dict(gx)
+        out.update(e=None, Km=None, Kh=None)
+        g_kb = self._closure_T(s, kb, out, adj.get("e"), g_Km, g_Kh)
+        return out, g_kb
+
+    def _reverse_ops(self, s: DiffState, kb: torch.Tensor, y: DiffState, tape: dict,
+                     adj: dict):
         d, p, g, pr = self.domain, self.physics, self.grid, self.params
         dt, th, tc, thv = self.dt, pr.theta, pr.theta_cor, pr.theta_v
         m3, m3u, m3v = self.mask3, self.mask3u, self.mask3v
@@ -1195,16 +1261,7 @@ class DiffStep:
         # the closure (closure.py:141-205) by autograd
         out = {"u": g_u, "v": g_v, "b": g_b, "eta": g_eta, "T": gT_in, "S": gS_in,
                "e": None, "Km": None, "Kh": None}
-        g_kb = torch.zeros_like(kb)
-        if p.closure == "tke":
-            ins = [s.e, s.u, s.v, s.b, s.Km, s.Kh, kb]
-            got = self._vjp(self.tke_step, ins, [adj.get("e"), g_Km, g_Kh])
-            for k, gk in zip(("e", "u", "v", "b", "Km", "Kh"), got[:6]):
-                out[k] = gk if out[k] is None else out[k] + gk
-            g_kb = got[6]
-        else:
-            out["Km"], out["Kh"] = g_Km, g_Kh
-            out["e"] = adj.get("e")
+        g_kb = self._closure_T(s, kb, out, adj.get("e"), g_Km, g_Kh)
         return out, g_kb
 
     # ------------------------------------------------------- initial state

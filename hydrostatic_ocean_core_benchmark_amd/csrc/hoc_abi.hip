@@ -157,6 +157,11 @@ struct hoc_stepper {
   long long barotropic_substeps = 0;
   double *gh_u[2], *gh_v[2];
   double *s_n2, *s_sh, *s_l, *s_le, *s_c;
+  // reverse-step workspace (the first hoc_stepper_step_adjoint allocates it)
+  bool rv_ready = false;
+  double *rv_lamT, *rv_lamS, *rv_cs, *rv_gui, *rv_gvi, *rv_Lu, *rv_Lv, *rv_lu, *rv_lv, *rv_gq;
+  double *rv_gw, *rv_gw0;
+  double *rv_X, *rv_Y, *rv_ge, *rv_leta, *rv_GU, *rv_GV, *rv_gdiv, *rv_inv, *rv_gKu, *rv_gKv;
   // Helmholtz
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
   unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
@@ -2160,6 +2165,106 @@ int hoc_stepper_step_tape(hoc_stepper* h, const hoc_state* in, hoc_state* out,
                           const hoc_step_tape* tape, void* stream) {
   if (!tape) return fail(HOC_ERR_INVALID, "null tape");
   return step_impl(h, in, out, stream, false, tape);
+}
+
+int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state* y,
+                             const double* Km, const double* Kh, const hoc_step_tape* tape,
+                             const hoc_state* gy, const double* gy_Km, const double* gy_Kh,
+                             hoc_state* gx, double* g_Km, double* g_Kh, void* stream) {
+  if (!h || !x || !y || !Km || !Kh || !tape || !gy || !gx || !g_Km || !g_Kh)
+    return fail(HOC_ERR_INVALID, "null argument");
+  const hoc_config& cfg = h->cfg;
+  const StepParams& p = h->p;
+  if (cfg.scheme != 0 || cfg.vcoord != 0)
+    return fail(HOC_ERR_UNSUPPORTED, "the reverse step covers the theta scheme on z-level grids");
+  if (p.open_x || p.open_y)
+    return fail(HOC_ERR_UNSUPPORTED, "the reverse step covers closed or periodic edges");
+  if (p.advection != ADV_UP3 && p.advection != ADV_UP3_TVD)
+    return fail(HOC_ERR_UNSUPPORTED, "the reverse step covers advection up3 and up3_tvd");
+  if (!cfg.tracers_ts) return fail(HOC_ERR_UNSUPPORTED, "the reverse step needs tracers T and S");
+  if (h->fp32_parts) return fail(HOC_ERR_UNSUPPORTED, "the reverse step is the fp64 step's");
+  if (h->d.nx < 4 || h->d.ny < 4)
+    return fail(HOC_ERR_UNSUPPORTED, "the reverse step needs nx, ny >= 4");
+  if (cfg.n_picard < 1 || cfg.n_picard > HOC_TAPE_MAX_PICARD)
+    return fail(HOC_ERR_UNSUPPORTED, "the reverse step covers 1..HOC_TAPE_MAX_PICARD passes");
+  if (!x->u || !x->v || !x->T || !x->S || !y->u || !y->v || !y->T || !y->S)
+    return fail(HOC_ERR_INVALID, "x needs u, v, T, S and y needs u, v, T, S");
+  if (!gx->u || !gx->v || !gx->b || !gx->eta || !gx->T || !gx->S)
+    return fail(HOC_ERR_INVALID, "gx needs u, v, b, eta, T, S");
+  if (!tape->q || !tape->Ku || !tape->Kv || !tape->w_old || !tape->w)
+    return fail(HOC_ERR_INVALID, "tape arrays q, Ku, Kv, w_old, w are required");
+  for (int it = 0; it < cfg.n_picard; ++it)
+    if (!tape->gh_u[it] || !tape->gh_v[it] || !tape->eta[it])
+      return fail(HOC_ERR_INVALID, "tape arrays gh_u, gh_v, eta are required for every pass");
+  const Dims d = h->d;
+  const Geo& geo = h->geo;
+  hipStream_t s = (hipStream_t)stream;
+  if (!h->rv_ready) {
+    for (double** b : {&h->rv_lamT, &h->rv_lamS, &h->rv_cs, &h->rv_gui, &h->rv_gvi, &h->rv_Lu,
+                       &h->rv_Lv, &h->rv_lu, &h->rv_lv, &h->rv_gq})
+      TRY(h->alloc(b, d.n3));
+    TRY(h->alloc(&h->rv_gw, d.ni));
+    TRY(h->alloc(&h->rv_gw0, d.ni));
+    for (double** b : {&h->rv_X, &h->rv_Y, &h->rv_ge, &h->rv_leta, &h->rv_GU, &h->rv_GV,
+                       &h->rv_gdiv, &h->rv_inv, &h->rv_gKu, &h->rv_gKv})
+      TRY(h->alloc(b, d.n2));
+    h->rv_ready = true;
+  }
+  auto init = [&](double* dst, const double* src, long long n) -> int {
+    if (src)
+      HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else
+      k_fill<<<blocks(n), TPB, 0, s>>>(n, 0.0, dst);
+    return HOC_OK;
+  };
+  TRY(init(g_Km, gy_Km, d.ni));
+  TRY(init(g_Kh, gy_Kh, d.ni));
+  for (double* b : {h->rv_gq, h->rv_Lu, h->rv_Lv, gx->u, gx->v}) TRY(init(b, nullptr, d.n3));
+  for (double* b : {gx->eta, h->rv_gdiv, h->rv_gKu, h->rv_gKv}) TRY(init(b, nullptr, d.n2));
+  // tracers and the new velocity's w (model3d_v05.py:466-533)
+  k_rev_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->T, y->S, gy->T, gy->S,
+                                             gy->b, Kh, h->rv_lamT, h->rv_lamS, h->rv_cs, gx->T,
+                                             gx->S, g_Kh);
+  k_rev_tracer_stencil<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->u, y->v, tape->w,
+                                                    h->rv_lamT, h->rv_lamS, gy->u, gy->v, gx->T,
+                                                    gx->S, h->rv_gui, h->rv_gvi, h->rv_gw);
+  LAUNCH_CHECK("reverse tracers");
+  // Picard passes in reverse (model3d_v05.py:387-408)
+  HelmOp op{tape->Ku, tape->Kv, h->m2, p.helm_coef, p.helm_cx, p.helm_cy, p.dx2, p.dy2};
+  k_helm_diag<<<blocks(d.n2), TPB, 0, s>>>(d, op, h->rv_inv);
+  const int last = cfg.n_picard - 1;
+  for (int it = last; it >= 0; --it) {
+    k_rev_backsub<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, it == last ? h->rv_gw : nullptr,
+                                               tape->q, tape->eta[it], h->rv_gui, h->rv_gvi,
+                                               h->rv_gq, h->rv_X, h->rv_Y);
+    k_rev_eta_rhs<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, it == last ? gy->eta : nullptr,
+                                               h->rv_X, h->rv_Y, h->rv_ge);
+    LAUNCH_CHECK("reverse back-substitution");
+    int iters = 0, conv = 0;
+    TRY(helmholtz_solve(h, tape->Ku, tape->Kv, h->rv_inv, h->rv_ge, h->rv_leta, s, &iters, &conv));
+    k_rev_eta_post<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, tape->eta[it], h->rv_leta, h->rv_gKu,
+                                                h->rv_gKv, gx->eta, h->rv_gdiv, h->rv_GU,
+                                                h->rv_GV);
+    k_rev_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, Km, h->rv_gui, h->rv_gvi, h->rv_GU,
+                                                 h->rv_GV, tape->gh_u[it], tape->gh_v[it],
+                                                 h->rv_lu, h->rv_lv, h->rv_cs, h->rv_Lu,
+                                                 h->rv_Lv, g_Km);
+    k_rev_coriolis<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, h->rv_lu, h->rv_lv, h->rv_Lu,
+                                                h->rv_Lv, it == 0 ? 1 : 0,
+                                                it == 0 ? gx->u : h->rv_gui,
+                                                it == 0 ? gx->v : h->rv_gvi);
+    LAUNCH_CHECK("reverse predictor");
+  }
+  // the explicit terms of the old state, its w and q (model3d_v05.py:150-167,349-386)
+  k_rev_explicit<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->u, x->v, Km, h->rv_Lu, h->rv_Lv,
+                                              h->rv_gdiv, gx->u, gx->v, gx->b, gx->eta, g_Km);
+  k_rev_momentum_adv<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, x->u, x->v, tape->w_old, h->rv_Lu,
+                                                  h->rv_Lv, gx->u, gx->v, h->rv_gw0);
+  k_rev_wold_q<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, h->rv_gw0, Km, tape->q, h->rv_gKu,
+                                            h->rv_gKv, h->rv_gq, h->rv_lu, h->rv_cs, gx->u, gx->v,
+                                            g_Km);
+  LAUNCH_CHECK("reverse explicit terms");
+  return HOC_OK;
 }
 
 int hoc_stepper_z_centre(hoc_stepper* h, double* out, void* stream) {

@@ -139,16 +139,19 @@ __device__ __forceinline__ int pwrap(int a, int n) {
 // Gradients of L = sum(G * adv(c, u, v, w)) for one cell (k, j, i): gc at
 // the cell, gu / gv at its east / north face, gw at its top interface (and
 // the bottom interface nz, which carries no flux, for k = nz - 1).
-__global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
-                                           const double* __restrict__ cc,
-                                           const double* __restrict__ u,
-                                           const double* __restrict__ v,
-                                           const double* __restrict__ w,
-                                           const double* __restrict__ G,
-                                           double* __restrict__ gc, double* __restrict__ gu,
-                                           double* __restrict__ gv, double* __restrict__ gw) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.n3) return;
+// G(q) returns the adjoint of the advection term at layer index q. Results:
+// gc_out at the cell, gu_out / gv_out at its east / north face, gw_out at its
+// top interface k.
+template <class GA>
+__device__ __forceinline__ void tracer_adv_adjoint_point(const Dims& d, const StepParams& p,
+                                                         const Geo& geo,
+                                                         const double* __restrict__ cc,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ v,
+                                                         const double* __restrict__ w, GA G,
+                                                         long long t, double& gc_out,
+                                                         double& gu_out, double& gv_out,
+                                                         double& gw_out) {
   const long long n2 = d.n2;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int k = (int)(t / n2);
@@ -160,7 +163,7 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
   // Ghat = G mask / dz at a cell of this level
   auto ghat = [&](long long cell, long long off) -> double {
     const long long q = off + cell;
-    return G[q] * mk(geo.m3[q]) * safe_inv(geo.dz3[q]);
+    return G(q) * mk(geo.m3[q]) * safe_inv(geo.dz3[q]);
   };
   const double* C = cc + o;
   const uint8_t* M = geo.m3 + o;
@@ -175,7 +178,7 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
                                      lim, dp);
     const double fbar = (ghat(f, o) - ghat(fe, o)) / p.dx;
     acc += fbar * geo.dz3u[o + f] * vel * dp[1 - s];
-    if (s == 0) gu[t] = fbar * geo.dz3u[o + f] * face;
+    if (s == 0) gu_out = fbar * geo.dz3u[o + f] * face;
   }
   // ---- y-faces
   for (int s = -2; s <= 1; ++s) {
@@ -187,7 +190,7 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
                                      lim, dp);
     const double fbar = (ghat(f, o) - ghat(fn, o)) / p.dy;
     acc += fbar * geo.dz3v[o + f] * vel * dp[1 - s];
-    if (s == 0) gv[t] = fbar * geo.dz3v[o + f] * face;
+    if (s == 0) gv_out = fbar * geo.dz3v[o + f] * face;
   }
   // ---- vertical: interface kk (1..nz-1) reads layers kk-2..kk+1 as
   // kappa_face(cm = far_dn, c0 = below, c1 = above, c2 = far_up)
@@ -208,15 +211,34 @@ __global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
     const double wb = zbar(kk) * w[a];
     // this layer is far_dn (kk = k - 1), below (kk = k), above (k + 1), far_up (k + 2)
     acc += wb * dp[kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3];
-    if (kk == k) gw[t] = zbar(kk) * face;
+    if (kk == k) gw_out = zbar(kk) * face;
   }
   if (k == 0) {  // the surface interface carries w[0] c[0]
     const double gz0 = ghat(c, 0);
     acc += gz0 * w[c];
-    gw[c] = gz0 * cc[c];
+    gw_out = gz0 * cc[c];
   }
-  if (k == nz - 1) gw[(long long)nz * n2 + c] = 0.0;
-  gc[t] = acc;
+  gc_out = acc;
+}
+
+__global__ void k_tracer_advection_adjoint(Dims d, StepParams p, Geo geo,
+                                           const double* __restrict__ cc,
+                                           const double* __restrict__ u,
+                                           const double* __restrict__ v,
+                                           const double* __restrict__ w,
+                                           const double* __restrict__ G,
+                                           double* __restrict__ gc, double* __restrict__ gu,
+                                           double* __restrict__ gv, double* __restrict__ gw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  double a = 0.0, b = 0.0, e = 0.0, f = 0.0;
+  tracer_adv_adjoint_point(d, p, geo, cc, u, v, w, [&](long long q) { return G[q]; }, t, a, b,
+                           e, f);
+  gc[t] = a;
+  gu[t] = b;
+  gv[t] = e;
+  gw[t] = f;
+  if (t / d.n2 == d.nz - 1) gw[(long long)d.nz * d.n2 + t % d.n2] = 0.0;
 }
 
 }  // namespace hoc
@@ -426,16 +448,17 @@ namespace hoc {
 // and i+1); the v equation is the transpose of that layout. Every flux
 // enters its two neighbours' divergences with opposite signs, which gives
 // the flux adjoints (Fbar) the gather reads.
-__global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
-                                             const double* __restrict__ u,
-                                             const double* __restrict__ v,
-                                             const double* __restrict__ w,
-                                             const double* __restrict__ Gu,
-                                             const double* __restrict__ Gv,
-                                             double* __restrict__ gu, double* __restrict__ gv,
-                                             double* __restrict__ gw) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.n3) return;
+// Gu(q), Gv(q) return the adjoints of the two advection terms at layer
+// index q; results gU, gV at the point and gW at its top interface.
+template <class GUA, class GVA>
+__device__ __forceinline__ void momentum_adv_adjoint_point(const Dims& d, const StepParams& p,
+                                                           const Geo& geo,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ v,
+                                                           const double* __restrict__ w,
+                                                           GUA Gu, GVA Gv, long long t,
+                                                           double& gU_out, double& gV_out,
+                                                           double& gW_out) {
   const long long n2 = d.n2;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int k = (int)(t / n2);
@@ -450,8 +473,8 @@ __global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
   auto V = [&](int jj, int ii) { return v[o + at(jj, ii)]; };
   auto MU = [&](int jj, int ii) { return geo.m3u[o + at(jj, ii)] != 0; };
   auto MV = [&](int jj, int ii) { return geo.m3v[o + at(jj, ii)] != 0; };
-  auto GU = [&](int jj, int ii) { return Gu[o + at(jj, ii)]; };
-  auto GV = [&](int jj, int ii) { return Gv[o + at(jj, ii)]; };
+  auto GU = [&](int jj, int ii) { return Gu(o + at(jj, ii)); };
+  auto GV = [&](int jj, int ii) { return Gv(o + at(jj, ii)); };
   double gU = 0.0, gV = 0.0, dp[4];
   // ---------------- u equation
   for (int s = -1; s <= 2; ++s) {  // x flux at centre cc = i + s
@@ -521,10 +544,10 @@ __global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
     const int role = kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3;
     const double wbu = 0.5 * (w[a + c] + w[a + ce]);
     zface(u, geo.m3u, kk, c, wbu, dp);
-    gU += (Gu[a + c] - Gu[a - n2 + c]) / p.dz_nom * wbu * dp[role];
+    gU += (Gu(a + c) - Gu(a - n2 + c)) / p.dz_nom * wbu * dp[role];
     const double wbv = 0.5 * (w[a + c] + w[a + cn]);
     zface(v, geo.m3v, kk, c, wbv, dp);
-    gV += (Gv[a + c] - Gv[a - n2 + c]) / p.dz_nom * wbv * dp[role];
+    gV += (Gv(a + c) - Gv(a - n2 + c)) / p.dz_nom * wbv * dp[role];
   }
   double gW = 0.0;
   if (k >= 1) {  // interface k: w enters wb of the u columns i-1, i and v rows j-1, j
@@ -533,17 +556,609 @@ __global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
       const long long cu = at(j, i + s), cue = at(j, i + s + 1);
       const double wbu = 0.5 * (w[a + cu] + w[a + cue]);
       const double fu = zface(u, geo.m3u, k, cu, wbu, dp);
-      gW += (Gu[a + cu] - Gu[a - n2 + cu]) / p.dz_nom * fu * 0.5;
+      gW += (Gu(a + cu) - Gu(a - n2 + cu)) / p.dz_nom * fu * 0.5;
       const long long cv = at(j + s, i), cvn = at(j + s + 1, i);
       const double wbv = 0.5 * (w[a + cv] + w[a + cvn]);
       const double fv = zface(v, geo.m3v, k, cv, wbv, dp);
-      gW += (Gv[a + cv] - Gv[a - n2 + cv]) / p.dz_nom * fv * 0.5;
+      gW += (Gv(a + cv) - Gv(a - n2 + cv)) / p.dz_nom * fv * 0.5;
     }
   }
-  gu[t] = gU;
-  gv[t] = gV;
-  gw[t] = gW;
-  if (k == nz - 1) gw[(long long)nz * n2 + c] = 0.0;
+  gU_out = gU;
+  gV_out = gV;
+  gW_out = gW;
+}
+
+__global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo,
+                                             const double* __restrict__ u,
+                                             const double* __restrict__ v,
+                                             const double* __restrict__ w,
+                                             const double* __restrict__ Gu,
+                                             const double* __restrict__ Gv,
+                                             double* __restrict__ gu, double* __restrict__ gv,
+                                             double* __restrict__ gw) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  momentum_adv_adjoint_point(d, p, geo, u, v, w, [&](long long q) { return Gu[q]; },
+                             [&](long long q) { return Gv[q]; }, t, a, b, c);
+  gu[t] = a;
+  gv[t] = b;
+  gw[t] = c;
+  if (t / d.n2 == d.nz - 1) gw[(long long)d.nz * d.n2 + t % d.n2] = 0.0;
+}
+
+}  // namespace hoc
+
+namespace hoc {
+
+// ================================================================ reverse step
+// The reverse of the theta step after its closure (model3d_v05.py:337-415,
+// 466-533) for hoc_stepper_step_adjoint: tracers, the new velocity's w, the
+// Picard passes in reverse order with the adjoint Helmholtz solves, the
+// step's explicit terms, q and the face coefficients. Column kernels own a
+// column (i, j) and march k; stencil kernels own one layer point. The
+// stored geometry arrays are read (m3, m3u, m3v, dz3, dz3u, dz3v, zc).
+
+// Row k of diffusion_coeffs (vertical_var.py:35-77) of one column over
+// thickness dz and interface coefficient nu, with the factors ft, fb of
+// nu_k in a_top and of nu_{k+1} in a_bot; wet and bottom flags of the cell.
+struct DiffRow {
+  double sub, diag, sup, ft, fb;
+  bool wet, bot;
+};
+
+__device__ __forceinline__ DiffRow diff_row(const Dims& d, const Geo& geo, long long c, int k,
+                                            const double* __restrict__ nu,
+                                            const double* __restrict__ dz, double thv_dt,
+                                            double thv_dt_drag) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const long long t = (long long)k * n2 + c;
+  DiffRow r;
+  r.wet = geo.m3[t] != 0;
+  r.bot = r.wet && (k == nz - 1 || geo.m3[t + n2] == 0);
+  const double dzk = dz[t];
+  const double inv = safe_inv(dzk);
+  const double fac = thv_dt * inv;
+  const double dzi_t = k == 0 ? dzk : 0.5 * (dz[t - n2] + dzk);
+  const double dzi_b = k == nz - 1 ? dzk : 0.5 * (dzk + dz[t + n2]);
+  r.ft = fac * safe_inv(dzi_t);
+  r.fb = fac * safe_inv(dzi_b);
+  const double a_top = r.ft * nu[t], a_bot = r.fb * nu[t + n2];
+  if (!r.wet) {
+    r.sub = 0.0;
+    r.diag = 1.0;
+    r.sup = 0.0;
+  } else {
+    r.sub = k == 0 ? 0.0 : -a_top;
+    if (r.bot) {
+      r.diag = 1.0 + a_top + thv_dt_drag * inv;
+      r.sup = 0.0;
+    } else {
+      r.diag = k == 0 ? 1.0 + a_bot : 1.0 + a_top + a_bot;
+      r.sup = -a_bot;
+    }
+  }
+  return r;
+}
+
+// lambda_r = A^-T g_r for one column's matrix A (rows diff_row) and NR
+// right-hand sides (g(k, out) fills out[0..NR-1]); lam[r] receives lambda_r,
+// cs is sweep scratch (all stride n2). Then the band gradients of
+// -lambda_r^T (dA) x_r (x_r the forward solutions), summed over r, are turned
+// into the adjoint of nu and added to gnu (vertical_var.py:35-77 reversed:
+// the bottom row's sup and the dry rows carry no nu).
+template <int NR, class GF>
+__device__ void rev_column_solve(const Dims& d, const Geo& geo, long long c,
+                                 const double* __restrict__ nu, const double* __restrict__ dz,
+                                 double thv_dt, double thv_dt_drag, GF g,
+                                 const double* const* x, double* const* lam,
+                                 double* __restrict__ cs, double* __restrict__ gnu) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  DiffRow r = diff_row(d, geo, c, 0, nu, dz, thv_dt, thv_dt_drag);
+  double cp = 0.0, sup_prev = 0.0, dp[NR];
+  for (int q = 0; q < NR; ++q) dp[q] = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = (long long)k * n2 + c;
+    DiffRow rn{};
+    double supT = 0.0;  // A^T: sub_T[k] = sup[k-1], sup_T[k] = sub[k+1]
+    if (k + 1 < nz) {
+      rn = diff_row(d, geo, c, k + 1, nu, dz, thv_dt, thv_dt_drag);
+      supT = rn.sub;
+    }
+    const double den = k == 0 ? r.diag : r.diag - sup_prev * cp;
+    const double cc = supT / den;
+    double gk[NR];
+    g(k, gk);
+    for (int q = 0; q < NR; ++q) {
+      const double dd = k == 0 ? gk[q] / den : (gk[q] - sup_prev * dp[q]) / den;
+      lam[q][t] = dd;
+      dp[q] = dd;
+    }
+    cs[t] = cc;
+    cp = cc;
+    sup_prev = r.sup;
+    r = rn;
+  }
+  double ln[NR];
+  for (int q = 0; q < NR; ++q) ln[q] = lam[q][(long long)(nz - 1) * n2 + c];
+  for (int k = nz - 2; k >= 0; --k) {
+    const long long t = (long long)k * n2 + c;
+    for (int q = 0; q < NR; ++q) {
+      ln[q] = lam[q][t] - cs[t] * ln[q];
+      lam[q][t] = ln[q];
+    }
+  }
+  for (int k = 0; k < nz; ++k) {
+    const long long t = (long long)k * n2 + c;
+    const DiffRow rk = diff_row(d, geo, c, k, nu, dz, thv_dt, thv_dt_drag);
+    if (!rk.wet) continue;
+    double gd = 0.0, gs = 0.0, gp = 0.0;
+    for (int q = 0; q < NR; ++q) {
+      const double l = lam[q][t];
+      gd -= l * x[q][t];
+      if (k > 0) gs -= l * x[q][t - n2];
+      if (k + 1 < nz && !rk.bot) gp -= l * x[q][t + n2];
+    }
+    const double ga_top = k >= 1 ? gd - gs : (rk.bot ? gd : 0.0);
+    const double ga_bot = rk.bot ? 0.0 : gd - gp;
+    gnu[t] += rk.ft * ga_top;
+    gnu[t + n2] += rk.fb * ga_bot;
+  }
+}
+
+// Adjoint of apply_diffusion (vertical_var.py:80-106, cell masks, the
+// surface flux a constant) for one column: with go(k) the adjoint of the
+// output, adds dL/dfield to gf and dL/dnu to gnu.
+template <class OF>
+__device__ void rev_apply_diffusion(const Dims& d, const Geo& geo, long long c,
+                                    const double* __restrict__ f, const double* __restrict__ nu,
+                                    const double* __restrict__ dz, double drag, OF go,
+                                    double* __restrict__ gf, double* __restrict__ gnu) {
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  double oi_prev = 0.0, pend = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const long long t = (long long)k * n2 + c;
+    const bool wet = geo.m3[t] != 0;
+    const bool bot = wet && (k == nz - 1 || geo.m3[t + n2] == 0);
+    const double oi = wet ? go(k) * safe_inv(dz[t]) : 0.0;
+    double add = bot ? (-drag) * oi : 0.0;
+    if (k >= 1) {  // interface k between cells k-1 and k (no flux below a bottom cell)
+      const bool bot_above = geo.m3[t - n2] != 0 && !wet;
+      const double gi = bot_above ? 0.0 : oi - oi_prev;
+      const double rdzi = safe_inv(0.5 * (dz[t - n2] + dz[t]));
+      const double cf = nu[t] * rdzi;
+      gf[t - n2] += pend + cf * gi;
+      add -= cf * gi;
+      gnu[t] += (f[t - n2] - f[t]) * rdzi * gi;
+    }
+    pend = add;
+    oi_prev = oi;
+  }
+  gf[(long long)(nz - 1) * n2 + c] += pend;
+}
+
+__global__ void k_fill(long long n, double value, double* __restrict__ a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = value;
+}
+
+// Tracers (model3d_v05.py:466-533) reversed per column for T and S at once:
+// the adjoint of T_new, S_new (plus that of b_new through the EOS, dual
+// numbers of buoyancy_t), masked, solved with tri_b^T; lambda_T, lambda_S
+// out; gT0 = lambda + (1 - theta_v) dt D^T lambda (the explicit diffusion's
+// adjoint); the nu adjoints of tri_b and of that diffusion added to gKh.
+__global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* __restrict__ T0,
+                              const double* __restrict__ S0, const double* __restrict__ T1,
+                              const double* __restrict__ S1, const double* __restrict__ gT1,
+                              const double* __restrict__ gS1, const double* __restrict__ gb1,
+                              const double* __restrict__ Kh, double* __restrict__ lamT,
+                              double* __restrict__ lamS, double* __restrict__ cs,
+                              double* __restrict__ gT0, double* __restrict__ gS0,
+                              double* __restrict__ gKh) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  auto g = [&](int k, double* out) {
+    const long long t = (long long)k * n2 + c;
+    const double m = mk(geo.m3[t]);
+    double a = gT1 ? gT1[t] : 0.0, b = gS1 ? gS1[t] : 0.0;
+    if (gb1 && geo.m3[t]) {
+      const Dual zz(geo.zc[t]);
+      const Dual dT = buoyancy_t<Dual>(p, Dual(T1[t], 1.0), Dual(S1[t]), zz);
+      const Dual dS = buoyancy_t<Dual>(p, Dual(T1[t]), Dual(S1[t], 1.0), zz);
+      const double gb = gb1[t] * m;
+      a += gb * dT.d;
+      b += gb * dS.d;
+    }
+    out[0] = a * m;
+    out[1] = b * m;
+  };
+  const double* xs[2] = {T1, S1};
+  double* ls[2] = {lamT, lamS};
+  rev_column_solve<2>(d, geo, c, Kh, geo.dz3, p.thv_dt, 0.0, g, xs, ls, cs, gKh);
+  for (int k = 0; k < d.nz; ++k) {
+    const long long t = (long long)k * n2 + c;
+    gT0[t] = lamT[t];
+    gS0[t] = lamS[t];
+  }
+  rev_apply_diffusion(d, geo, c, T0, Kh, geo.dz3, 0.0,
+                      [&](int k) { return p.c_dzd * lamT[(long long)k * n2 + c]; }, gT0, gKh);
+  rev_apply_diffusion(d, geo, c, S0, Kh, geo.dz3, 0.0,
+                      [&](int k) { return p.c_dzd * lamS[(long long)k * n2 + c]; }, gS0, gKh);
+}
+
+// Adjoint at point t of laplacian(., mask_u, mask_v, mask) (operators_masked.py:
+// 64-68) applied to G: gradx^T and grady^T of div^T.
+__device__ __forceinline__ double lap_T_point(const Dims& d, const StepParams& p,
+                                              const uint8_t* __restrict__ mu,
+                                              const uint8_t* __restrict__ mv,
+                                              const uint8_t* __restrict__ m,
+                                              const double* __restrict__ G, long long t) {
+  const long long n2 = d.n2;
+  const long long o = (t / n2) * n2, c = t % n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long e = o + at2(d, j, wrap(i, 1, d.nx)), w = o + at2(d, j, wrap(i, -1, d.nx));
+  const long long n = o + at2(d, wrap(j, 1, d.ny), i), s = o + at2(d, wrap(j, -1, d.ny), i);
+  const double mg = mk(m[t]) * G[t];
+  const double Uc = (mg - mk(m[e]) * G[e]) / p.dx, Uw = (mk(m[w]) * G[w] - mg) / p.dx;
+  const double Vc = (mg - mk(m[n]) * G[n]) / p.dy, Vs = (mk(m[s]) * G[s] - mg) / p.dy;
+  return (mk(mu[w]) * Uw - mk(mu[t]) * Uc) / p.dx + (mk(mv[s]) * Vs - mk(mv[t]) * Vc) / p.dy;
+}
+
+// Tracer advection (gathered, hand-written reverse of adv) and K_h
+// Laplacian reversed at one point for T and S: adds to gT0, gS0; the new
+// velocity's adjoint gu1 = gu_y + the flux-velocity parts (gv1 likewise);
+// gw1 the adjoint of the new w (both tracers).
+__global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double* __restrict__ T0,
+                                     const double* __restrict__ S0,
+                                     const double* __restrict__ u1,
+                                     const double* __restrict__ v1,
+                                     const double* __restrict__ w1,
+                                     const double* __restrict__ lamT,
+                                     const double* __restrict__ lamS,
+                                     const double* __restrict__ gu_y,
+                                     const double* __restrict__ gv_y, double* __restrict__ gT0,
+                                     double* __restrict__ gS0, double* __restrict__ gu1,
+                                     double* __restrict__ gv1, double* __restrict__ gw1) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const double ndt = -p.dt;
+  double cT, uT, vT, wT, cS, uS, vS, wS;
+  tracer_adv_adjoint_point(d, p, geo, T0, u1, v1, w1, [&](long long q) { return ndt * lamT[q]; },
+                           t, cT, uT, vT, wT);
+  tracer_adv_adjoint_point(d, p, geo, S0, u1, v1, w1, [&](long long q) { return ndt * lamS[q]; },
+                           t, cS, uS, vS, wS);
+  if (p.has_K_h) {
+    const double dk = p.dt * p.K_h;
+    cT += dk * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3, lamT, t);
+    cS += dk * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3, lamS, t);
+  }
+  gT0[t] += cT;
+  gS0[t] += cS;
+  gu1[t] = (gu_y ? gu_y[t] : 0.0) + uT + uS;
+  gv1[t] = (gv_y ? gv_y[t] : 0.0) + vT + vS;
+  gw1[t] = wT + wS;
+  if (t / d.n2 == d.nz - 1) gw1[(long long)d.nz * d.n2 + t % d.n2] = 0.0;
+}
+
+// Back-substitution u_it = (gh_u - g dt th q Gx(eta) mask3u) mask3 reversed
+// for one column (model3d_v05.py:404-406); with gw (the last pass) the
+// adjoint of w = w_diag(u_it, v_it) is added first (w_k = -sum_{j>=k} div_j,
+// div of the face transports). gui, gvi are replaced by the adjoints of
+// the masked predictor solutions' pre-mask expression (A = gui mask3); q's
+// adjoint is added to gq; X, Y (columns) are the sums the free surface's
+// adjoint takes through Gx^T, Gy^T.
+__global__ void k_rev_backsub(Dims d, StepParams p, Geo geo, const double* __restrict__ gw,
+                              const double* __restrict__ q, const double* __restrict__ eta,
+                              double* __restrict__ gui, double* __restrict__ gvi,
+                              double* __restrict__ gq, double* __restrict__ X,
+                              double* __restrict__ Y) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cn = at2(d, wrap(j, 1, d.ny), i);
+  const double gx2 = mk(geo.mu2[c]) * (eta[ce] - eta[c]) / p.dx;
+  const double gy2 = mk(geo.mv2[c]) * (eta[cn] - eta[c]) / p.dy;
+  double pc = 0.0, pe = 0.0, pn = 0.0, xs = 0.0, ys = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    const long long o = (long long)k * n2, t = o + c;
+    double ui = gui[t], vi = gvi[t];
+    if (gw) {
+      pc -= gw[t];
+      pe -= gw[o + ce];
+      pn -= gw[o + cn];
+      const double mc = mk(geo.m3[t]) * pc;
+      ui += geo.dz3u[t] * ((mc - mk(geo.m3[o + ce]) * pe) / p.dx);
+      vi += geo.dz3v[t] * ((mc - mk(geo.m3[o + cn]) * pn) / p.dy);
+    }
+    const double m = mk(geo.m3[t]);
+    const double Au = ui * m, Av = vi * m;
+    const double mu = mk(geo.m3u[t]), mv = mk(geo.m3v[t]);
+    gq[t] -= p.c_bs * (gx2 * mu * Au + gy2 * mv * Av);
+    xs += -p.c_bs * q[t] * mu * Au;
+    ys += -p.c_bs * q[t] * mv * Av;
+    gui[t] = Au;
+    gvi[t] = Av;
+  }
+  X[c] = xs;
+  Y[c] = ys;
+}
+
+// The right-hand side of the adjoint Helmholtz solve of one pass:
+// ge = g_eta (the output's, last pass only) + Gx^T X + Gy^T Y.
+__global__ void k_rev_eta_rhs(Dims d, StepParams p, Geo geo, const double* __restrict__ geta,
+                              const double* __restrict__ X, const double* __restrict__ Y,
+                              double* __restrict__ ge) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
+  const double mxc = mk(geo.mu2[c]) * X[c], myc = mk(geo.mv2[c]) * Y[c];
+  ge[c] = (geta ? geta[c] : 0.0) + (mk(geo.mu2[cw]) * X[cw] - mxc) / p.dx +
+          (mk(geo.mv2[cs]) * Y[cs] - myc) / p.dy;
+}
+
+// After the adjoint solve lam = A^-1 ge of one pass: the face coefficients'
+// adjoints -lam^T (dA/dK) x (x the pass's free surface), the old free
+// surface's and old divergence's shares of rhs = eta - dt((1 - th) div_old
+// + th div_g), and the face values GU, GV of div^T(-dt th lam).
+__global__ void k_rev_eta_post(Dims d, StepParams p, Geo geo, const double* __restrict__ x,
+                               const double* __restrict__ lam, double* __restrict__ gKu,
+                               double* __restrict__ gKv, double* __restrict__ geta0,
+                               double* __restrict__ gdiv, double* __restrict__ GU,
+                               double* __restrict__ GV) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cn = at2(d, wrap(j, 1, d.ny), i);
+  const double mc = mk(geo.m2[c]), me = mk(geo.m2[ce]), mn = mk(geo.m2[cn]);
+  const double xc = x[c] * mc, xe = x[ce] * me, xn = x[cn] * mn;
+  const double lc = lam[c] * mc, le = lam[ce] * me, ln = lam[cn] * mn;
+  gKu[c] += p.helm_coef * (xe - xc) * (lc - le) / p.dx2;
+  gKv[c] += p.helm_coef * (xn - xc) * (lc - ln) / p.dy2;
+  geta0[c] += lam[c];
+  gdiv[c] -= p.dt * p.one_m_th * lam[c];
+  const double dc = -p.dt * p.th * lam[c];
+  const double de = -p.dt * p.th * lam[ce], dn = -p.dt * p.th * lam[cn];
+  GU[c] = (mc * dc - me * de) / p.dx;
+  GV[c] = (mc * dc - mn * dn) / p.dy;
+}
+
+// The predictor's two solves gh = A_m^-1 (g + stress) reversed for one
+// column (model3d_v05.py:394-399): the adjoint of gh (A + dz3[u|v] G[U|V]),
+// masked, solved with tri_m^T into lu, lv; tri_m's nu adjoint added to gKm;
+// Lu += lu, Lv += lv (the sums over passes).
+__global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* __restrict__ Km,
+                                const double* __restrict__ Au, const double* __restrict__ Av,
+                                const double* __restrict__ GU, const double* __restrict__ GV,
+                                const double* __restrict__ ghu, const double* __restrict__ ghv,
+                                double* __restrict__ lu, double* __restrict__ lv,
+                                double* __restrict__ cs, double* __restrict__ Lu,
+                                double* __restrict__ Lv, double* __restrict__ gKm) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const double gUc = GU[c], gVc = GV[c];
+  auto g = [&](int k, double* out) {
+    const long long t = (long long)k * n2 + c;
+    const double m = mk(geo.m3[t]);
+    out[0] = (Au[t] + geo.dz3u[t] * gUc) * m;
+    out[1] = (Av[t] + geo.dz3v[t] * gVc) * m;
+  };
+  const double* xs[2] = {ghu, ghv};
+  double* ls[2] = {lu, lv};
+  rev_column_solve<2>(d, geo, c, Km, geo.dz3, p.thv_dt, p.thv_dt_drag, g, xs, ls, cs, gKm);
+  for (int k = 0; k < d.nz; ++k) {
+    const long long t = (long long)k * n2 + c;
+    Lu[t] += lu[t];
+    Lv[t] += lv[t];
+  }
+}
+
+// Coriolis terms reversed at one point (operators_masked.py:45-61
+// transposed): the u point c takes part in the v-point means of c, e, s,
+// se; the v point in the u-point means of c, n, w, nw. With first == 0
+// (a pass after the first) the previous iterate's adjoints are written:
+// gu = tc Au2v^T(-dt f lv), gv = tc Av2u^T(dt f lu); with first != 0 the
+// old state's shares of both passes are added: gu += Au2v^T(-dt f (tc lv +
+// (1 - tc) Lv)), gv += Av2u^T(dt f (tc lu + (1 - tc) Lu)).
+__global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* __restrict__ lu,
+                               const double* __restrict__ lv, const double* __restrict__ Lu,
+                               const double* __restrict__ Lv, int first,
+                               double* __restrict__ gu, double* __restrict__ gv) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long n2 = d.n2;
+  const long long o = (t / n2) * n2, c = t % n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  auto at = [&](int dj, int di) { return o + at2(d, wrap(j, dj, d.ny), wrap(i, di, d.nx)); };
+  const uint8_t* mu = geo.m3u;
+  const uint8_t* mv = geo.m3v;
+  auto Gv = [&](long long q) {  // the v-point adjoint of u_cor
+    return first ? -p.dt * p.f * (p.theta_cor * lv[q] + p.one_m_tc * Lv[q]) : -p.dt * p.f * lv[q];
+  };
+  auto Gu = [&](long long q) {  // the u-point adjoint of v_cor
+    return first ? p.dt * p.f * (p.theta_cor * lu[q] + p.one_m_tc * Lu[q]) : p.dt * p.f * lu[q];
+  };
+  // H'(q) at a v point q: G mask_v / (mu(q) + mu(w q) + mu(n q) + mu(nw q))
+  auto Hv = [&](int dj, int di) {
+    const long long q = at(dj, di);
+    const double den = mk(mu[q]) + mk(mu[at(dj, di - 1)]) + mk(mu[at(dj + 1, di)]) +
+                       mk(mu[at(dj + 1, di - 1)]);
+    return den > 0.0 ? Gv(q) * mk(mv[q]) / den : 0.0;
+  };
+  // H(q) at a u point q: G mask_u / (mv(q) + mv(s q) + mv(e q) + mv(se q))
+  auto Hu = [&](int dj, int di) {
+    const long long q = at(dj, di);
+    const double den = mk(mv[q]) + mk(mv[at(dj - 1, di)]) + mk(mv[at(dj, di + 1)]) +
+                       mk(mv[at(dj - 1, di + 1)]);
+    return den > 0.0 ? Gu(q) * mk(mu[q]) / den : 0.0;
+  };
+  const double su = mk(mu[t]) * (Hv(0, 0) + Hv(0, 1) + Hv(-1, 0) + Hv(-1, 1));
+  const double sv = mk(mv[t]) * (Hu(0, 0) + Hu(1, 0) + Hu(0, -1) + Hu(1, -1));
+  if (first) {
+    gu[t] += su;
+    gv[t] += sv;
+  } else {
+    gu[t] = p.theta_cor * su;
+    gv[t] = p.theta_cor * sv;
+  }
+}
+
+// The step's explicit terms reversed for one column, from Lu, Lv (the sums
+// of the predictor solves' adjoints over the passes, model3d_v05.py:
+// 349-386): gu0 += Lu + dz3u div^T(gdiv) + A_h dt Lap^T(Lu) and the
+// explicit diffusion's field adjoint (gKm takes its nu adjoint), gv0
+// likewise; geta0 += Gx^T of the surface-gradient term; gb0 = the pressure
+// gradients' adjoint (model3d_v05.py:195-230).
+__global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* __restrict__ u0,
+                               const double* __restrict__ v0, const double* __restrict__ Km,
+                               const double* __restrict__ Lu, const double* __restrict__ Lv,
+                               const double* __restrict__ gdiv, double* __restrict__ gu0,
+                               double* __restrict__ gv0, double* __restrict__ gb0,
+                               double* __restrict__ geta0, double* __restrict__ gKm) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
+  const double Dc = mk(geo.m2[c]) * gdiv[c];
+  const double DUx = (Dc - mk(geo.m2[ce]) * gdiv[ce]) / p.dx;
+  const double DVy = (Dc - mk(geo.m2[cn]) * gdiv[cn]) / p.dy;
+  double xc = 0.0, xw = 0.0, yc = 0.0, ys = 0.0;
+  const double dtA = p.dt * p.A_h;
+  for (int k = 0; k < nz; ++k) {
+    const long long o = (long long)k * n2, t = o + c;
+    double au = Lu[t] + geo.dz3u[t] * DUx;
+    double av = Lv[t] + geo.dz3v[t] * DVy;
+    if (p.has_A_h) {
+      au += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3u, Lu, t);
+      av += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3v, Lv, t);
+    }
+    gu0[t] += au;
+    gv0[t] += av;
+    xc += mk(geo.m3u[t]) * (-p.c_gx) * Lu[t];
+    xw += mk(geo.m3u[o + cw]) * (-p.c_gx) * Lu[o + cw];
+    yc += mk(geo.m3v[t]) * (-p.c_gx) * Lv[t];
+    ys += mk(geo.m3v[o + cs]) * (-p.c_gx) * Lv[o + cs];
+  }
+  geta0[c] += (mk(geo.mu2[cw]) * xw - mk(geo.mu2[c]) * xc) / p.dx +
+              (mk(geo.mv2[cs]) * ys - mk(geo.mv2[c]) * yc) / p.dy;
+  rev_apply_diffusion(d, geo, c, u0, Km, geo.dz3u, p.bottom_drag,
+                      [&](int k) { return p.c_dzd * Lu[(long long)k * n2 + c]; }, gu0, gKm);
+  rev_apply_diffusion(d, geo, c, v0, Km, geo.dz3v, p.bottom_drag,
+                      [&](int k) { return p.c_dzd * Lv[(long long)k * n2 + c]; }, gv0, gKm);
+  // pressure gradients: phi's adjoint and b's direct share per level, then
+  // phi = dz3 b / 2 + sum_{j<k} dz3 b reversed from the bottom up
+  auto level = [&](int k, double& gphi, double& gbd) {
+    const long long o = (long long)k * n2, t = o + c;
+    const double Pc = mk(geo.m3u[t]) * (p.dt * Lu[t]) / p.dx;
+    const double Pw = mk(geo.m3u[o + cw]) * (p.dt * Lu[o + cw]) / p.dx;
+    const double Qc = mk(geo.m3v[t]) * (p.dt * Lv[t]) / p.dy;
+    const double Qs = mk(geo.m3v[o + cs]) * (p.dt * Lv[o + cs]) / p.dy;
+    gphi = (Pw - Pc) + (Qs - Qc);
+    gbd = 0.0;
+    if (p.pgf_correction) {
+      const double zc = geo.zc[t];
+      const double zfx = 0.5 * (zc + geo.zc[o + ce]), zfw = 0.5 * (geo.zc[o + cw] + zc);
+      const double zfy = 0.5 * (zc + geo.zc[o + cn]), zfs = 0.5 * (geo.zc[o + cs] + zc);
+      gbd = Pw * (zfw - zc) - Pc * (zfx - zc) + Qs * (zfs - zc) - Qc * (zfy - zc);
+    }
+  };
+  double rm = 0.0;
+  if (p.pgf_remove) {
+    double tot = 0.0, sg = 0.0;
+    for (int k = 0; k < nz; ++k) {
+      double gphi, gbd;
+      level(k, gphi, gbd);
+      tot += geo.dz3[(long long)k * n2 + c];
+      sg += gphi;
+    }
+    rm = safe_inv(tot) * sg;
+  }
+  double below = 0.0;
+  for (int k = nz - 1; k >= 0; --k) {
+    const long long t = (long long)k * n2 + c;
+    double gphi, gbd;
+    level(k, gphi, gbd);
+    const double dz = geo.dz3[t];
+    if (p.pgf_remove) gphi -= dz * rm;
+    gb0[t] = gbd + 0.5 * dz * gphi + dz * below;
+    below += gphi;
+  }
+}
+
+// Momentum advection of the old state reversed at one point with the
+// adjoints -dt m3[u|v] L[u|v] of the advection terms (gathered reverse);
+// adds to gu0, gv0 and writes the adjoint of the old w.
+__global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* __restrict__ u0,
+                                   const double* __restrict__ v0,
+                                   const double* __restrict__ w0,
+                                   const double* __restrict__ Lu,
+                                   const double* __restrict__ Lv, double* __restrict__ gu0,
+                                   double* __restrict__ gv0, double* __restrict__ gw0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  double a = 0.0, b = 0.0, e = 0.0;
+  momentum_adv_adjoint_point(
+      d, p, geo, u0, v0, w0, [&](long long q) { return -(mk(geo.m3u[q]) * (p.dt * Lu[q])); },
+      [&](long long q) { return -(mk(geo.m3v[q]) * (p.dt * Lv[q])); }, t, a, b, e);
+  gu0[t] += a;
+  gv0[t] += b;
+  gw0[t] = e;
+  if (t / d.n2 == d.nz - 1) gw0[(long long)d.nz * d.n2 + t % d.n2] = 0.0;
+}
+
+// Per column: the old w's adjoint into gu0, gv0 (w_diag reversed), then
+// q = A_m^-1 1 reversed: q's adjoint gains the face coefficients' shares
+// (K_u = sum dz3u min(q, q_E) mask_u, model3d_v05.py:150-167, the minimum's
+// kept operand differentiated), is solved with tri_m^T and tri_m's nu
+// adjoint is added to gKm.
+__global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* __restrict__ gw0,
+                             const double* __restrict__ Km, const double* __restrict__ q,
+                             const double* __restrict__ gKu, const double* __restrict__ gKv,
+                             double* __restrict__ gq, double* __restrict__ lam,
+                             double* __restrict__ cs, double* __restrict__ gu0,
+                             double* __restrict__ gv0, double* __restrict__ gKm) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
+  const long long cn = at2(d, wrap(j, 1, d.ny), i), cs2 = at2(d, wrap(j, -1, d.ny), i);
+  const double Gxc = gKu[c] * mk(geo.mu2[c]), Gxw = gKu[cw] * mk(geo.mu2[cw]);
+  const double Gyc = gKv[c] * mk(geo.mv2[c]), Gys = gKv[cs2] * mk(geo.mv2[cs2]);
+  double pc = 0.0, pe = 0.0, pn = 0.0;
+  for (int k = 0; k < d.nz; ++k) {
+    const long long o = (long long)k * n2, t = o + c;
+    pc -= gw0[t];
+    pe -= gw0[o + ce];
+    pn -= gw0[o + cn];
+    const double mc = mk(geo.m3[t]) * pc;
+    gu0[t] += geo.dz3u[t] * ((mc - mk(geo.m3[o + ce]) * pe) / p.dx);
+    gv0[t] += geo.dz3v[t] * ((mc - mk(geo.m3[o + cn]) * pn) / p.dy);
+    const double qc = q[t];
+    double a = gq[t];
+    const bool kx = qc <= q[o + ce] || qc != qc;           // min(q, q_E) keeps q at c
+    const bool kxw = q[o + cw] <= qc || q[o + cw] != q[o + cw];  // ... keeps q_W at w
+    const bool ky = qc <= q[o + cn] || qc != qc;
+    const bool kys = q[o + cs2] <= qc || q[o + cs2] != q[o + cs2];
+    if (kx) a += geo.dz3u[t] * Gxc;
+    if (!kxw) a += geo.dz3u[o + cw] * Gxw;
+    if (ky) a += geo.dz3v[t] * Gyc;
+    if (!kys) a += geo.dz3v[o + cs2] * Gys;
+    gq[t] = a;
+  }
+  auto g = [&](int k, double* out) { out[0] = gq[(long long)k * n2 + c]; };
+  const double* xs[1] = {q};
+  double* ls[1] = {lam};
+  rev_column_solve<1>(d, geo, c, Km, geo.dz3, p.thv_dt, p.thv_dt_drag, g, xs, ls, cs, gKm);
 }
 
 }  // namespace hoc

@@ -167,6 +167,44 @@ __global__ void k_momentum_advection_adjoint(Dims d, StepParams p, Geo geo, cons
 __global__ void k_eos_adjoint(long long n, StepParams p, const double* T, const double* S,
                               const double* z, const double* g_b, double* g_T, double* g_S);
 
+// the reverse step (hoc_stepper_step_adjoint)
+__global__ void k_fill(long long n, double value, double* a);
+__global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* T0, const double* S0,
+                              const double* T1, const double* S1, const double* gT1,
+                              const double* gS1, const double* gb1, const double* Kh,
+                              double* lamT, double* lamS, double* cs, double* gT0, double* gS0,
+                              double* gKh);
+__global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double* T0,
+                                     const double* S0, const double* u1, const double* v1,
+                                     const double* w1, const double* lamT, const double* lamS,
+                                     const double* gu_y, const double* gv_y, double* gT0,
+                                     double* gS0, double* gu1, double* gv1, double* gw1);
+__global__ void k_rev_backsub(Dims d, StepParams p, Geo geo, const double* gw, const double* q,
+                              const double* eta, double* gui, double* gvi, double* gq, double* X,
+                              double* Y);
+__global__ void k_rev_eta_rhs(Dims d, StepParams p, Geo geo, const double* geta, const double* X,
+                              const double* Y, double* ge);
+__global__ void k_rev_eta_post(Dims d, StepParams p, Geo geo, const double* x, const double* lam,
+                               double* gKu, double* gKv, double* geta0, double* gdiv, double* GU,
+                               double* GV);
+__global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* Km, const double* Au,
+                                const double* Av, const double* GU, const double* GV,
+                                const double* ghu, const double* ghv, double* lu, double* lv,
+                                double* cs, double* Lu, double* Lv, double* gKm);
+__global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* lu, const double* lv,
+                               const double* Lu, const double* Lv, int first, double* gu,
+                               double* gv);
+__global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* u0, const double* v0,
+                               const double* Km, const double* Lu, const double* Lv,
+                               const double* gdiv, double* gu0, double* gv0, double* gb0,
+                               double* geta0, double* gKm);
+__global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* u0,
+                                   const double* v0, const double* w0, const double* Lu,
+                                   const double* Lv, double* gu0, double* gv0, double* gw0);
+__global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* gw0, const double* Km,
+                             const double* q, const double* gKu, const double* gKv, double* gq,
+                             double* lam, double* cs, double* gu0, double* gv0, double* gKm);
+
 // hoc_sw2d.hip: the 2D shallow-water model (libs/core/schemes.py:97-177).
 // Scalars formed on the host in Python's left-to-right order.
 struct Sw2d {

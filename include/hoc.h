@@ -145,6 +145,25 @@ typedef struct hoc_step_tape {
 int hoc_stepper_step_tape(hoc_stepper* h, const hoc_state* in, hoc_state* out,
                           const hoc_step_tape* tape, void* stream);
 
+/* The reverse of one step after its closure (no reference counterpart: the
+ * reference has no adjoint; the step reversed is Stepper3DV05.step,
+ * model3d_v05.py:337-415, theta branch). x is the step's input state (u, v,
+ * T, S read), y its output (u, v, T, S read), Km, Kh ([nz+1][ny][nx]) the
+ * mixing coefficients the step used (the closure's output) and tape the
+ * step's tape (hoc_stepper_step_tape). gy holds the adjoints of y's fields
+ * (u, v, b, eta, T, S; a NULL field counts as zero) and gy_Km, gy_Kh those
+ * of Km, Kh as state of the next step (NULL: zero). Writes gx's u, v, b,
+ * eta, T, S: the adjoints of x's fields through everything of the step
+ * except its closure, and g_Km, g_Kh: the total adjoints of the step's Km,
+ * Kh (gy's plus the step's uses), which the closure's reverse takes. The
+ * Helmholtz solves of the reverse use the handle's solver on the tape's
+ * K_u, K_v (the masked operator is symmetric). Theta scheme, z-level,
+ * closed or periodic edges, up3 / up3_tvd, tracers T and S, fp64. */
+int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state* y,
+                             const double* Km, const double* Kh, const hoc_step_tape* tape,
+                             const hoc_state* gy, const double* gy_Km, const double* gy_Kh,
+                             hoc_state* gx, double* g_Km, double* g_Kh, void* stream);
+
 /* The current depth of the cell centres [nz][ny][nx] (device, written on
  * `stream`): fixed for z-level, rebuilt from eta every step for z* and sigma
  * (the reference's self.z_centre, model3d_v05.py:128-139). */

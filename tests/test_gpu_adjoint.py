@@ -260,6 +260,56 @@ def test_hand_written_adjoint_equals_autograd_and_checkpointing(adj):
     assert _rel(r2.dJ_dkappa, ag.dJ_dkappa) < 1e-10, _rel(r2.dJ_dkappa, ag.dJ_dkappa)
 
 
+@pytest.mark.parametrize("variant", ["basin", "no_closure_drag_remove"])
+def test_fused_reverse_step_matches_transposed_operators(adj, variant):
+    """hoc_stepper_step_adjoint (the fused reverse kernels) against the
+    reverse step written with the transposed torch operators, one step of
+    a 24 x 24 x 30 basin after 3 steps, seeded adjoints of every output
+    field: every input adjoint and dJ/dkappa_b within 1e-11 (both sides
+    share the Helmholtz solves; the rest differs in summation order)."""
+    from hydrostatic_ocean_core_benchmark_amd.types import State3D
+    prob = _basin(24, nz=30)
+    if variant != "basin":
+        prob = _basin(24, nz=30, advection="up3", closure="none")
+        prob.physics = dataclasses.replace(
+            prob.physics, pgf="remove",
+            base=dataclasses.replace(prob.physics.base, A_h=0.0, bottom_drag=2e-3))
+    kb = _kappa(prob)
+    a = adj.Adjoint(prob.domain, prob.physics, prob.params, prob.dt, kernels="hip")
+    st = a._hip(kb.cpu().numpy())
+    ds = a.ds
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    s0 = ds.initial_state(T0, S0, kb)
+    x = st.integrate(State3D(u=s0.u, v=s0.v, b=s0.b, eta=s0.eta, t=0.0, T=s0.T, S=s0.S,
+                             e=s0.e), 3)
+    c = a._capture(st, x)
+    fin, fout, tape = st._inputs(x), st._new_state_buffers(), st._new_tape()
+    st._step_tape_into(fin, fout, tape)
+    y = adj.DiffState(u=fout["u"], v=fout["v"], b=fout["b"], eta=fout["eta"], T=fout["T"],
+                      S=fout["S"], e=fout.get("e"), Km=st.nu.clone(), Kh=st.kappa.clone())
+    rng = np.random.default_rng(5)
+    seeds = {}
+    for k in adj.DiffState.FIELDS:
+        t = getattr(y, k)
+        seeds[k] = (None if t is None else torch.as_tensor(rng.standard_normal(t.shape),
+                                                           device="cuda") * float(t.abs().mean()))
+    f_out, f_kb = ds.reverse(c, kb, y, tape, dict(seeds), fused=True)
+    o_out, o_kb = ds.reverse(c, kb, y, tape, dict(seeds), fused=False)
+    def close(a, b, name):
+        if float(b.abs().max()) == 0.0:   # e.g. K_m of a clipped closure: exactly zero
+            assert float(a.abs().max()) == 0.0, name
+        else:
+            assert _rel(a, b) < 1e-11, (name, _rel(a, b))
+
+    for k in adj.DiffState.FIELDS:
+        if o_out.get(k) is None:
+            assert f_out.get(k) is None, k
+            continue
+        close(f_out[k], o_out[k], k)
+    close(f_kb, o_kb, "kappa_b")
+
+
 def test_hand_reverse_other_branches_equal_autograd(adj):
     """The other branches of the hand-written reverse step (no closure,
     unlimited up3, pgf remove, bottom drag, no A_h) against autograd through

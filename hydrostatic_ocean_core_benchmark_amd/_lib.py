@@ -132,6 +132,7 @@ PROTOTYPES = {
     "hoc_stepper_step_adjoint": ([VP, C.POINTER(HocState), C.POINTER(HocState), VP, VP,
                                   C.POINTER(HocStepTape), C.POINTER(HocState), VP, VP,
                                   C.POINTER(HocState), VP, VP, VP], i32),
+    "hoc_stepper_closure_adjoint": ([VP] * 19, i32),
     "hoc_stepper_set_sync_free": ([VP, i32], i32),
     "hoc_stepper_set_mg_tile": ([VP, i32], i32),
     "hoc_stepper_geometry_form": ([VP, i32, C.POINTER(i32)], i32),

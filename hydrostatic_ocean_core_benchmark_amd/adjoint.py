@@ -1111,10 +1111,27 @@ class DiffStep:
             return self._reverse_fused(s, kb, y, tape, adj)
         return self._reverse_ops(s, kb, y, tape, adj)
 
-    def _closure_T(self, s: DiffState, kb, out: dict, g_e, g_Km, g_Kh):
+    def _closure_T(self, s: DiffState, kb, out: dict, g_e, g_Km, g_Kh, fused: bool = False):
         """The closure's share of the reverse step added into `out`; returns
-        dJ/dkappa_b of the step."""
-        if self.physics.closure == "tke":
+        dJ/dkappa_b of the step. fused: hoc_stepper_closure_adjoint (TKE with
+        integral lengths); otherwise autograd through tke_step."""
+        p = self.physics
+        if fused and p.closure == "tke" and p.mxl == "integral":
+            st = self.owner
+            ptr = lambda t: t.data_ptr()
+            ins = [t.contiguous() for t in (s.e, s.u, s.v, s.b, s.Km, s.Kh, kb, g_Km, g_Kh)]
+            ge = None if g_e is None else g_e.contiguous()
+            for k in ("u", "v", "b"):
+                out[k] = out[k].contiguous()
+            g_e0, g_Km0, g_Kh0 = (torch.empty_like(s.e) for _ in range(3))
+            g_kb = torch.zeros_like(kb)
+            _lib.check(st.lib.hoc_stepper_closure_adjoint(
+                st._h, *(ptr(t) for t in ins[:7]), None if ge is None else ptr(ge),
+                ptr(ins[7]), ptr(ins[8]), ptr(g_e0), ptr(out["u"]), ptr(out["v"]),
+                ptr(out["b"]), ptr(g_Km0), ptr(g_Kh0), ptr(g_kb), st._stream()))
+            out["e"], out["Km"], out["Kh"] = g_e0, g_Km0, g_Kh0
+            return g_kb
+        if p.closure == "tke":
             ins = [s.e, s.u, s.v, s.b, s.Km, s.Kh, kb]
             got = self._vjp(self.tke_step, ins, [g_e, g_Km, g_Kh])
             for k, gk in zip(("e", "u", "v", "b", "Km", "Kh"), got[:6]):
@@ -1162,7 +1179,7 @@ class DiffStep:
             g_Kh.data_ptr(), st._stream()))
         out = dict(gx)
         out.update(e=None, Km=None, Kh=None)
-        g_kb = self._closure_T(s, kb, out, adj.get("e"), g_Km, g_Kh)
+        g_kb = self._closure_T(s, kb, out, adj.get("e"), g_Km, g_Kh, fused=True)
         return out, g_kb
 
     def _reverse_ops(self, s: DiffState, kb: torch.Tensor, y: DiffState, tape: dict,

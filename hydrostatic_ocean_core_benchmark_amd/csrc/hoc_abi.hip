@@ -162,6 +162,9 @@ struct hoc_stepper {
   double *rv_lamT, *rv_lamS, *rv_cs, *rv_gui, *rv_gvi, *rv_Lu, *rv_Lv, *rv_lu, *rv_lv, *rv_gq;
   double *rv_gw, *rv_gw0;
   double *rv_X, *rv_Y, *rv_ge, *rv_leta, *rv_GU, *rv_GV, *rv_gdiv, *rv_inv, *rv_gKu, *rv_gKv;
+  // closure-reverse workspace (the first hoc_stepper_closure_adjoint allocates it)
+  bool cr_ready = false;
+  double *cr_n2, *cr_lu, *cr_ld, *cr_x, *cr_cs, *cr_lam, *cr_gl, *cr_n2b, *cr_guc, *cr_gvc;
   // Helmholtz
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
   unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
@@ -2264,6 +2267,38 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
                                             h->rv_gKv, h->rv_gq, h->rv_lu, h->rv_cs, gx->u, gx->v,
                                             g_Km);
   LAUNCH_CHECK("reverse explicit terms");
+  return HOC_OK;
+}
+
+int hoc_stepper_closure_adjoint(hoc_stepper* h, const double* e, const double* u,
+                                const double* v, const double* b, const double* Km,
+                                const double* Kh, const double* kappa_b, const double* g_e_new,
+                                const double* g_Km_new, const double* g_Kh_new, double* g_e,
+                                double* g_u, double* g_v, double* g_b, double* g_Km,
+                                double* g_Kh, double* g_kappa_b, void* stream) {
+  if (!h || !e || !u || !v || !b || !Km || !Kh || !kappa_b || !g_Km_new || !g_Kh_new || !g_e ||
+      !g_u || !g_v || !g_b || !g_Km || !g_Kh || !g_kappa_b)
+    return fail(HOC_ERR_INVALID, "null argument");
+  if (!h->cfg.closure_tke || h->p.mxl_recursive)
+    return fail(HOC_ERR_UNSUPPORTED, "the closure reverse covers the TKE closure with integral lengths");
+  if (h->cfg.vcoord != 0) return fail(HOC_ERR_UNSUPPORTED, "the closure reverse covers z-level grids");
+  const Dims d = h->d;
+  hipStream_t s = (hipStream_t)stream;
+  if (!h->cr_ready) {
+    for (double** q : {&h->cr_n2, &h->cr_lu, &h->cr_ld, &h->cr_x, &h->cr_cs, &h->cr_lam,
+                       &h->cr_gl, &h->cr_n2b})
+      TRY(h->alloc(q, d.ni));
+    TRY(h->alloc(&h->cr_guc, d.n3));
+    TRY(h->alloc(&h->cr_gvc, d.n3));
+    h->cr_ready = true;
+  }
+  k_rev_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e, u, v, b, Km, Kh, kappa_b,
+                                             g_e_new, g_Km_new, g_Kh_new, h->cr_n2, h->cr_lu,
+                                             h->cr_ld, h->cr_x, h->cr_cs, h->cr_lam, h->cr_gl,
+                                             h->cr_n2b, g_e, h->cr_guc, h->cr_gvc, g_b, g_Km,
+                                             g_Kh, g_kappa_b);
+  k_rev_closure_uv<<<blocks(d.n3), TPB, 0, s>>>(d, h->cr_guc, h->cr_gvc, g_u, g_v);
+  LAUNCH_CHECK("k_rev_closure");
   return HOC_OK;
 }
 

@@ -1162,3 +1162,311 @@ __global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* __rest
 }
 
 }  // namespace hoc
+
+namespace hoc {
+
+// =========================================================== closure reverse
+// tke_step (closure.py:141-205, integral mixing lengths) reversed per column:
+// the forward closure is recomputed (N^2 and shear^2 on the interfaces, the
+// Gaspar lengths with the march stopping once its budget is spent, the
+// implicit interface tridiagonal and its solve), then reversed from the
+// outputs' adjoints (e after its floors, K_m, K_h = (K_m - nu_b)/Pr + kappa_b):
+// K_m's factors, the floors, the transposed solve and its band gradients,
+// dissipation, production, the length maxima/minima and the reverse march
+// (march_step_adjoint) of every interface, N^2 and shear^2 back to b and to
+// the centred velocities (guc, gvc; k_rev_closure_uv spreads them to u, v).
+// A march step taken with a spent budget moves neither l nor the budget and
+// carries a zero adjoint, so the march and its reverse stop there too.
+// Scratch (stride n2, nz + 1 levels): sn2 N^2, slu / sld the lengths, sx
+// the solve, scs sweep coefficients, slam the adjoint right-hand side and
+// lambda, sgl the adjoint of l_k, sn2b the adjoint of N^2.
+__global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __restrict__ e,
+                              const double* __restrict__ u, const double* __restrict__ v,
+                              const double* __restrict__ b, const double* __restrict__ Km0,
+                              const double* __restrict__ Kh0, const double* __restrict__ kb,
+                              const double* __restrict__ ge1, const double* __restrict__ gKm1,
+                              const double* __restrict__ gKh1, double* __restrict__ sn2,
+                              double* __restrict__ slu, double* __restrict__ sld,
+                              double* __restrict__ sx, double* __restrict__ scs,
+                              double* __restrict__ slam, double* __restrict__ sgl,
+                              double* __restrict__ sn2b, double* __restrict__ ge0,
+                              double* __restrict__ guc, double* __restrict__ gvc,
+                              double* __restrict__ gb0, double* __restrict__ gKm0,
+                              double* __restrict__ gKh0, double* __restrict__ gkb) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const long long n2 = d.n2;
+  const int nz = d.nz;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
+  const double* dz3 = geo.dz3;
+  auto T = [&](int k) { return (long long)k * n2 + c; };
+  auto dzi = [&](int k) {
+    if (k == 0) return dz3[c];
+    if (k == nz) return dz3[T(nz - 1)];
+    return 0.5 * (dz3[T(k - 1)] + dz3[T(k)]);
+  };
+  auto wet_i = [&](int k) {
+    return (k >= 1 && k < nz) ? mk(geo.m3[T(k - 1)]) * mk(geo.m3[T(k)]) : 0.0;
+  };
+  auto uc = [&](int k) { const long long o = (long long)k * n2; return 0.5 * (u[o + c] + u[o + cw]); };
+  auto vc = [&](int k) { const long long o = (long long)k * n2; return 0.5 * (v[o + c] + v[o + cs]); };
+  auto dssqrt = [](double x) -> double { return x > 0.0 ? 0.5 / ::sqrt(x) : 0.0; };
+  auto ssqrt = [](double x) -> double { return x > 0.0 ? ::sqrt(x) : 0.0; };
+  double H = 0.0;
+  for (int k = 0; k < nz; ++k) H = H + dz3[T(k)];
+  const double kbc = kb[c];
+  // ---- forward: N^2, shear^2 (recomputed where needed), lengths
+  for (int k = 0; k <= nz; ++k) {
+    double n2k = 0.0;
+    if (k >= 1 && k < nz) n2k = (b[T(k - 1)] - b[T(k)]) * safe_inv(dzi(k)) * wet_i(k);
+    sn2[T(k)] = n2k;
+    sn2b[T(k)] = 0.0;
+  }
+  auto sh2_at = [&](int k) {
+    if (k < 1 || k >= nz) return 0.0;
+    const double idz = safe_inv(dzi(k));
+    const double du = (uc(k - 1) - uc(k)) * idz, dv = (vc(k - 1) - vc(k)) * idz;
+    return (du * du + dv * dv) * wet_i(k);
+  };
+  double zs = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    if (k > 0) zs = zs + dz3[T(k - 1)];
+    const double ek = nmax(e[T(k)], p.e_min);
+    double l = 0.0, bud = ek;
+    for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
+      const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
+      l = o.l;
+      bud = o.b;
+    }
+    slu[T(k)] = nmin(l, zs + p.z_0) * wet_i(k);
+    l = 0.0;
+    bud = ek;
+    for (int kk = k + 1; kk <= nz && bud > 0.0; ++kk) {
+      const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
+      l = o.l;
+      bud = o.b;
+    }
+    sld[T(k)] = nmin(l, nmax(H - zs, 0.0) + p.z_0) * wet_i(k);
+  }
+  // rows of the interface tridiagonal (closure.py:176-196)
+  auto row = [&](int k, double& sub, double& diag, double& sup, double& rhs) {
+    if (!(wet_i(k) > 0.0)) {
+      sub = 0.0;
+      diag = 1.0;
+      sup = 0.0;
+      rhs = k == 0 ? (geo.m3[c] ? p.e_bb * geo.ustar2[c] : p.e_min) : p.e_min;
+      return;
+    }
+    const double lu = slu[T(k)], ld = sld[T(k)];
+    const double l_eps = nmax(p.l_min, nmin(lu, ld));
+    const double ek = e[T(k)];
+    const double diss = p.c_eps * ssqrt(nmax(ek, p.e_min)) / l_eps;
+    const double prod = (Km0[T(k)] - p.nu_b) * sh2_at(k) - (Kh0[T(k)] - kbc) * sn2[T(k)];
+    const double klu = 0.5 * (Km0[T(k - 1)] + Km0[T(k)]), kld = 0.5 * (Km0[T(k)] + Km0[T(k + 1)]);
+    const double idzi = safe_inv(dzi(k));
+    const double a_up = p.dt * klu * safe_inv(dz3[T(k - 1)]) * idzi;
+    const double a_dn = p.dt * kld * safe_inv(dz3[T(k)]) * idzi;
+    sub = -a_up;
+    sup = -a_dn;
+    diag = 1.0 + a_up + a_dn + p.dt * diss;
+    rhs = ek + p.dt * prod;
+  };
+  {
+    double cp = 0.0, dp = 0.0;
+    for (int k = 0; k <= nz; ++k) {
+      double sb, dg, sp, rh;
+      row(k, sb, dg, sp, rh);
+      const double den = k == 0 ? dg : dg - sb * cp;
+      cp = sp / den;
+      dp = k == 0 ? rh / den : (rh - sb * dp) / den;
+      scs[T(k)] = cp;
+      sx[T(k)] = dp;
+    }
+    double xn = sx[T(nz)];
+    for (int k = nz - 1; k >= 0; --k) {
+      xn = sx[T(k)] - scs[T(k)] * xn;
+      sx[T(k)] = xn;
+    }
+  }
+  // ---- reverse: the outputs' adjoints back to the solve's
+  double gkb_acc = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    const long long t = T(k);
+    const double gK = gKm1[t] + gKh1[t] / p.pr_t;
+    gkb_acc += gKh1[t];
+    const double w = wet_i(k);
+    const double x = sx[t];
+    double e_new;
+    if (k == 0) {
+      double sb, dg, sp, rh;
+      row(0, sb, dg, sp, rh);
+      e_new = rh;
+    } else {
+      e_new = w > 0.0 ? nmax(x, p.e_min) : p.e_min;
+    }
+    const double lu = slu[t], ld = sld[t];
+    const double l_k = nmax(p.l_min, ssqrt(lu * ld));
+    sgl[t] = p.c_k * ssqrt(e_new) * w * gK;
+    const double ge_new = (ge1 ? ge1[t] : 0.0) + p.c_k * l_k * w * gK * dssqrt(e_new);
+    slam[t] = (k >= 1 && w > 0.0 && (x >= p.e_min || x != x)) ? ge_new : 0.0;
+    ge0[t] = 0.0;
+    gKm0[t] = 0.0;
+    gKh0[t] = 0.0;
+  }
+  for (int k = 0; k < nz; ++k) {
+    guc[T(k)] = 0.0;
+    gvc[T(k)] = 0.0;
+  }
+  {  // lambda = A^-T g: sub_T[k] = sup[k-1], sup_T[k] = sub[k+1]
+    double cp = 0.0, dp = 0.0, sup_prev = 0.0;
+    double sb, dg, sp, rh, sbn = 0.0, dgn = 1.0, spn = 0.0, rhn = 0.0;
+    row(0, sb, dg, sp, rh);
+    for (int k = 0; k <= nz; ++k) {
+      double supT = 0.0;
+      if (k < nz) {
+        row(k + 1, sbn, dgn, spn, rhn);
+        supT = sbn;
+      }
+      const double den = k == 0 ? dg : dg - sup_prev * cp;
+      const double cc = supT / den;
+      const double dd = k == 0 ? slam[T(k)] / den : (slam[T(k)] - sup_prev * dp) / den;
+      scs[T(k)] = cc;
+      slam[T(k)] = dd;
+      cp = cc;
+      dp = dd;
+      sup_prev = sp;
+      sb = sbn;
+      dg = dgn;
+      sp = spn;
+      rh = rhn;
+    }
+    double ln = slam[T(nz)];
+    for (int k = nz - 1; k >= 0; --k) {
+      ln = slam[T(k)] - scs[T(k)] * ln;
+      slam[T(k)] = ln;
+    }
+  }
+  // per interface: the row's band and right-hand-side adjoints, then the
+  // lengths and their reverse marches
+  zs = 0.0;
+  for (int k = 0; k <= nz; ++k) {
+    const long long t = T(k);
+    if (k > 0) zs = zs + dz3[T(k - 1)];
+    const double w = wet_i(k);
+    const double lu = slu[t], ld = sld[t];
+    double gl_eps = 0.0;
+    const double ek = e[t];
+    double gek = 0.0;
+    if (w > 0.0) {
+      const double lam = slam[t];
+      const double xk = sx[t];
+      const double gd = -lam * xk, gs = -lam * sx[t - n2], gp = -lam * sx[t + n2];
+      const double ga_up = gd - gs, ga_dn = gd - gp, gdiss = p.dt * gd;
+      const double idzi = safe_inv(dzi(k));
+      const double glu = p.dt * safe_inv(dz3[T(k - 1)]) * idzi * ga_up;
+      const double gld = p.dt * safe_inv(dz3[t]) * idzi * ga_dn;
+      gKm0[t - n2] += 0.5 * glu;
+      gKm0[t] += 0.5 * glu + 0.5 * gld;
+      gKm0[t + n2] += 0.5 * gld;
+      gek += lam;
+      const double gprod = p.dt * lam;
+      const double n2k = sn2[t];
+      // shear^2 at k and its adjoint to the centred velocities
+      const double idz = idzi;
+      const double du = (uc(k - 1) - uc(k)) * idz, dv = (vc(k - 1) - vc(k)) * idz;
+      const double sh2 = (du * du + dv * dv) * w;
+      gKm0[t] += sh2 * gprod;
+      const double gsh = (Km0[t] - p.nu_b) * gprod;
+      gKh0[t] += -n2k * gprod;
+      gkb_acc += n2k * gprod;
+      sn2b[t] += -(Kh0[t] - kbc) * gprod;
+      const double gdu = 2.0 * du * w * gsh * idz, gdv = 2.0 * dv * w * gsh * idz;
+      guc[t - n2] += gdu;
+      guc[t] -= gdu;
+      gvc[t - n2] += gdv;
+      gvc[t] -= gdv;
+      // dissipation c_eps sqrt(max(e, e_min)) / l_eps
+      const double E = nmax(ek, p.e_min);
+      const double l_eps = nmax(p.l_min, nmin(lu, ld));
+      const double diss = p.c_eps * ssqrt(E) / l_eps;
+      if (ek >= p.e_min || ek != ek) gek += gdiss * p.c_eps * dssqrt(E) / l_eps;
+      gl_eps = -gdiss * diss / l_eps;
+    }
+    // l_eps = max(l_min, min(l_up, l_dn)), l_k = max(l_min, sqrt(l_up l_dn))
+    double glu = 0.0, gld = 0.0;
+    {
+      const double X = nmin(lu, ld);
+      const double gX = (X <= p.l_min) ? 0.0 : gl_eps;
+      if (lu <= ld || lu != lu) glu += gX;
+      else gld += gX;
+      const double P = lu * ld;
+      const double gY = (ssqrt(P) <= p.l_min) ? 0.0 : sgl[t];
+      const double gP = gY * dssqrt(P);
+      glu += gP * ld;
+      gld += gP * lu;
+    }
+    // reverse marches of interface k (closure.py:65-103)
+    const double ek0 = nmax(ek, p.e_min);
+    double gb0_acc = 0.0;
+    for (int dir = 0; dir < 2; ++dir) {
+      const double gout = (dir == 0 ? glu : gld) * w;
+      if (gout == 0.0) continue;
+      const double cap = (dir == 0 ? zs : nmax(H - zs, 0.0)) + p.z_0;
+      // the march forward to its end, then each crossing recomputed from the
+      // start for its reverse (the marches here stop within a few crossings)
+      int nstep = 0;
+      double l = 0.0, bud = ek0;
+      for (int kk = dir == 0 ? k - 1 : k + 1; (dir == 0 ? kk >= 0 : kk <= nz) && bud > 0.0;
+           kk += dir == 0 ? -1 : 1) {
+        const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
+        l = o.l;
+        bud = o.b;
+        ++nstep;
+      }
+      double lb = (l <= cap || l != l) ? gout : 0.0, bb = 0.0;
+      for (int s = nstep; s >= 1; --s) {
+        double ls = 0.0, bs = ek0;
+        int kk = dir == 0 ? k - 1 : k + 1;
+        for (int r = 1; r < s; ++r) {
+          const MarchIn o = march_step(ls, bs, nmax(sn2[T(kk)], 0.0), dzi(kk));
+          ls = o.l;
+          bs = o.b;
+          kk += dir == 0 ? -1 : 1;
+        }
+        const double n2r = sn2[T(kk)];
+        double lb_in, bb_in, nb = 0.0;
+        march_step_adjoint(ls, bs, nmax(n2r, 0.0), dzi(kk), lb, bb, lb_in, bb_in, nb);
+        lb = lb_in;
+        bb = bb_in;
+        if (n2r >= 0.0 || n2r != n2r) sn2b[T(kk)] += nb;
+      }
+      gb0_acc += bb;
+    }
+    if (ek >= p.e_min || ek != ek) gek += gb0_acc;
+    ge0[t] += gek;
+  }
+  gkb[c] += gkb_acc;
+  // N^2 = (b[k-1] - b[k]) / dzi wet back to b
+  for (int k = 1; k < nz; ++k) {
+    const double gdb = sn2b[T(k)] * wet_i(k) * safe_inv(dzi(k));
+    gb0[T(k - 1)] += gdb;
+    gb0[T(k)] -= gdb;
+  }
+}
+
+// The centred velocities' adjoints (uc = (u + u_W) / 2, vc = (v + v_S) / 2)
+// added to u and v.
+__global__ void k_rev_closure_uv(Dims d, const double* __restrict__ guc,
+                                 const double* __restrict__ gvc, double* __restrict__ gu,
+                                 double* __restrict__ gv) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long o = (t / d.n2) * d.n2, c = t % d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long te = o + at2(d, j, wrap(i, 1, d.nx)), tn = o + at2(d, wrap(j, 1, d.ny), i);
+  gu[t] += 0.5 * guc[t] + 0.5 * guc[te];
+  gv[t] += 0.5 * gvc[t] + 0.5 * gvc[tn];
+}
+
+}  // namespace hoc

@@ -201,6 +201,15 @@ __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* u0, 
 __global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* u0,
                                    const double* v0, const double* w0, const double* Lu,
                                    const double* Lv, double* gu0, double* gv0, double* gw0);
+__global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* e, const double* u,
+                              const double* v, const double* b, const double* Km0,
+                              const double* Kh0, const double* kb, const double* ge1,
+                              const double* gKm1, const double* gKh1, double* sn2, double* slu,
+                              double* sld, double* sx, double* scs, double* slam, double* sgl,
+                              double* sn2b, double* ge0, double* guc, double* gvc, double* gb0,
+                              double* gKm0, double* gKh0, double* gkb);
+__global__ void k_rev_closure_uv(Dims d, const double* guc, const double* gvc, double* gu,
+                                 double* gv);
 __global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* gw0, const double* Km,
                              const double* q, const double* gKu, const double* gKv, double* gq,
                              double* lam, double* cs, double* gu0, double* gv0, double* gKm);

@@ -164,6 +164,22 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
                              const hoc_state* gy, const double* gy_Km, const double* gy_Kh,
                              hoc_state* gx, double* g_Km, double* g_Kh, void* stream);
 
+/* The reverse of the step's closure, tke_step with integral mixing lengths
+ * (no reference counterpart; closure.py:141-205 reversed). Inputs: the
+ * step's input e ([nz+1][ny][nx]), u, v, b ([nz][ny][nx]), the mixing
+ * coefficients Km, Kh it started from and the background kappa_b ([ny][nx]);
+ * the adjoints of its outputs e_new (g_e_new, NULL: zero), K_m and K_h
+ * (g_Km_new, g_Kh_new; hoc_stepper_step_adjoint's g_Km, g_Kh). Writes g_e,
+ * g_Km, g_Kh (the adjoints of e, Km, Kh) and adds to g_u, g_v, g_b and
+ * g_kappa_b. Branches of maxima, minima and the march held at their forward
+ * values (a tie differentiates the first operand). */
+int hoc_stepper_closure_adjoint(hoc_stepper* h, const double* e, const double* u,
+                                const double* v, const double* b, const double* Km,
+                                const double* Kh, const double* kappa_b, const double* g_e_new,
+                                const double* g_Km_new, const double* g_Kh_new, double* g_e,
+                                double* g_u, double* g_v, double* g_b, double* g_Km,
+                                double* g_Kh, double* g_kappa_b, void* stream);
+
 /* The current depth of the cell centres [nz][ny][nx] (device, written on
  * `stream`): fixed for z-level, rebuilt from eta every step for z* and sigma
  * (the reference's self.z_centre, model3d_v05.py:128-139). */

@@ -1217,11 +1217,16 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
   for (int k = 0; k < nz; ++k) H = H + dz3[T(k)];
   const double kbc = kb[c];
   // ---- forward: N^2, shear^2 (recomputed where needed), lengths
+  // crossings of interfaces below the last one with N^2 > 0 or a thickness
+  // (the dry ones under the bottom) change neither l nor the budget and
+  // carry a zero adjoint: the marches end at `hi`
+  int hi = 0;
   for (int k = 0; k <= nz; ++k) {
     double n2k = 0.0;
     if (k >= 1 && k < nz) n2k = (b[T(k - 1)] - b[T(k)]) * safe_inv(dzi(k)) * wet_i(k);
     sn2[T(k)] = n2k;
     sn2b[T(k)] = 0.0;
+    if (nmax(n2k, 0.0) != 0.0 || dzi(k) != 0.0) hi = k + 1;
   }
   auto sh2_at = [&](int k) {
     if (k < 1 || k >= nz) return 0.0;
@@ -1232,6 +1237,11 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
   double zs = 0.0;
   for (int k = 0; k <= nz; ++k) {
     if (k > 0) zs = zs + dz3[T(k - 1)];
+    if (!(wet_i(k) > 0.0)) {  // the lengths carry the factor wet
+      slu[T(k)] = 0.0;
+      sld[T(k)] = 0.0;
+      continue;
+    }
     const double ek = nmax(e[T(k)], p.e_min);
     double l = 0.0, bud = ek;
     for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
@@ -1242,7 +1252,7 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
     slu[T(k)] = nmin(l, zs + p.z_0) * wet_i(k);
     l = 0.0;
     bud = ek;
-    for (int kk = k + 1; kk <= nz && bud > 0.0; ++kk) {
+    for (int kk = k + 1; kk < hi && bud > 0.0; ++kk) {
       const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
       l = o.l;
       bud = o.b;
@@ -1417,7 +1427,7 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       // start for its reverse (the marches here stop within a few crossings)
       int nstep = 0;
       double l = 0.0, bud = ek0;
-      for (int kk = dir == 0 ? k - 1 : k + 1; (dir == 0 ? kk >= 0 : kk <= nz) && bud > 0.0;
+      for (int kk = dir == 0 ? k - 1 : k + 1; (dir == 0 ? kk >= 0 : kk < hi) && bud > 0.0;
            kk += dir == 0 ? -1 : 1) {
         const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
         l = o.l;

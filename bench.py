@@ -136,6 +136,9 @@ def parse():
     ap.add_argument("--adjoint-fd-directions", type=int, default=16,
                     help="directions of the central-difference gradient check after the "
                          "adjoint timing (0: skip the check)")
+    ap.add_argument("--adjoint-mg-smoother", choices=("jacobi", "rbgs"), default="jacobi",
+                    help="pcg_mg V-cycle smoother of the adjoint measurement's forward and "
+                         "reverse solves")
     ap.add_argument("--mixed-steps", type=int, default=50,
                     help="steps of the plain-vs-mixed comparison on the 1024 x 1024 x 60 "
                          "basin (0: skip it)")
@@ -304,10 +307,16 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
                        "and kappa_b(x, y)",
            "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
            "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
-           "hand_written": "Thomas (transposed sweep), Helmholtz (PCG on the adjoint rhs), "
-                           "tracer and momentum advection, Gaspar mixing lengths and the EOS "
-                           "(reverse kernels); the rest by autograd through the torch "
-                           "restatement"}
+           "reverse": "window recompute (hoc_stepper_step_tape) on a second stream under the "
+                      "reverse of the next window; recompute and adjoint solves sync-free, "
+                      f"{r.solve_budget} iterations per solve"
+                      + (" (a solve outran it: timed again polled)" if r.redone_polled else ""),
+           "hand_written": "the whole step: hoc_stepper_step_adjoint (tracers with the EOS, "
+                           "advection, Picard passes with adjoint Helmholtz PCG solves, "
+                           "Coriolis, pressure gradients, explicit diffusion, q and the face "
+                           "coefficients) and hoc_stepper_closure_adjoint (TKE closure with "
+                           "the reverse Gaspar march) on the tapes of hoc_stepper_step_tape; "
+                           "no autograd"}
     if fd_directions > 0:
         checks = {}
         for adv in ("up3_tvd", "up3"):
@@ -558,7 +567,8 @@ def main() -> int:
         rec["ensemble"] = ensemble_throughput(args.ensemble_steps)
     if args.adjoint_steps > 0:
         rec["adjoint"] = adjoint_ratio(args.adjoint_steps,
-                                       fd_directions=args.adjoint_fd_directions)
+                                       fd_directions=args.adjoint_fd_directions,
+                                       mg_smoother=args.adjoint_mg_smoother)
     if args.ladder != "none":
         names = list(LADDER) if args.ladder == "all" else args.ladder.split(",")
         rec["ladder"] = reference_ladder(names)

@@ -1319,6 +1319,8 @@ class GradientResult:
     gradient_s: float     # forward with checkpoints + reverse sweep
     n_steps: int
     every: int
+    solve_budget: int = 0     # sync-free iterations per recompute solve (0: polled)
+    redone_polled: bool = False
 
 
 class Adjoint:
@@ -1339,6 +1341,11 @@ class Adjoint:
         self.device = torch.device(device)
         self.kernels = kernels
         self.reverse = reverse
+        # the window recomputes and the adjoint solves run sync-free with a
+        # budget from the forward sweep's iteration counts (the polled results
+        # when every solve converges within it; otherwise the gradient is
+        # redone with polled solves)
+        self.sync_free_recompute = True
         self.ds = DiffStep(domain, physics, params, dt, device=device, kernels=kernels)
 
     def _hip(self, kb_np):
@@ -1362,31 +1369,82 @@ class Adjoint:
         return State3D(u=c.u.clone(), v=c.v.clone(), b=c.b.clone(), eta=c.eta.clone(), t=0.0,
                        T=c.T.clone(), S=c.S.clone(), e=None if c.e is None else c.e.clone())
 
-    def _reverse_window(self, st, c0: DiffState, start: int, stop: int, kb, adj: dict):
-        """Steps start..stop-1 recomputed from their checkpoint c0 with the
-        tape, then reversed last to first; `adj` (the adjoint of the state
-        after step stop-1) becomes that of the state before step start.
-        Returns this window's share of dJ/dkappa_b."""
-        ds = self.ds
-        s = self._restore(st, c0)
-        fin = st._inputs(s)
+    @staticmethod
+    def _tape_step(st, fin: dict):
+        """One HIP step from the device fields fin with its tape: (the state
+        after it with the mixing coefficients the step used, the tape)."""
+        fout = st._new_state_buffers()
+        tape = st._new_tape()
+        st._step_tape_into(fin, fout, tape)
+        return (DiffState(u=fout["u"], v=fout["v"], b=fout["b"], eta=fout["eta"], T=fout["T"],
+                          S=fout["S"], e=fout.get("e"), Km=st.nu.clone(), Kh=st.kappa.clone()),
+                tape)
+
+    def _recompute(self, st, c0: DiffState, start: int, stop: int, side, holder: list):
+        """Generator: steps start..stop-1 recomputed with their tapes from the
+        checkpoint c0 on the stream `side`, one step per resumption; the
+        states and tapes go to holder[0]."""
         states, tapes = [c0], []
+        holder.append((states, tapes))
+        with torch.cuda.stream(side):
+            fin = st._inputs(self._restore(st, c0))
+        yield
         for _ in range(start, stop):
-            fout = st._new_state_buffers()
-            tape = st._new_tape()
-            st._step_tape_into(fin, fout, tape)
+            with torch.cuda.stream(side):
+                y, tape = self._tape_step(st, fin)
+            states.append(y)
             tapes.append(tape)
-            states.append(DiffState(u=fout["u"], v=fout["v"], b=fout["b"], eta=fout["eta"],
-                                    T=fout["T"], S=fout["S"], e=fout.get("e"),
-                                    Km=st.nu.clone(), Kh=st.kappa.clone()))
-            fin = fout
+            fin = {k: getattr(y, k) for k in ("u", "v", "b", "eta", "T", "S", "e")}
+            yield
+
+    def _reverse_sweep(self, st, ckpt: dict, kept, n_steps: int, every: int, kb, adj: dict):
+        """The windows reversed last to first (the last from the forward
+        sweep's tapes `kept`); while a window is reversed on the current
+        stream, the one before it is recomputed on a second stream (its own
+        handle `st`, step by step between the reverse steps), so the two
+        sequences of small kernels share the device. The reverse of a window
+        waits for the event that ends its recompute; tensors made on the
+        second stream are recorded on the first before use. `adj` becomes
+        the adjoint of the state before step 0; returns dJ/dkappa_b."""
+        main = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(main)
+        starts = sorted(ckpt, reverse=True)
         g_kb = torch.zeros_like(kb)
-        for i in range(stop - start - 1, -1, -1):
-            out, gk = ds.reverse(states[i], kb, states[i + 1], tapes[i], adj)
-            adj.clear()
-            adj.update(out)
-            g_kb += gk
-            states[i + 1] = tapes[i] = None
+        window, ev = kept, None
+        for idx, start in enumerate(starts):
+            stop = min(start + every, n_steps)
+            if ev is not None:
+                main.wait_event(ev)
+                for y in window[0][1:]:
+                    for t in y.tensors():
+                        t.record_stream(main)
+                for tape in window[1]:
+                    for k, t in tape.items():
+                        for tt in (t if isinstance(t, list) else [t]):
+                            tt.record_stream(main)
+            states, tapes = window
+            holder, rec = [], None
+            if idx + 1 < len(starts):
+                nxt = starts[idx + 1]
+                rec = self._recompute(st, ckpt[nxt], nxt, min(nxt + every, n_steps), side,
+                                      holder)
+                next(rec)
+            for i in range(stop - start - 1, -1, -1):
+                out, gk = self.ds.reverse(states[i], kb, states[i + 1], tapes[i], adj)
+                adj.clear()
+                adj.update(out)
+                g_kb += gk
+                states[i + 1] = tapes[i] = None
+                if rec is not None:
+                    next(rec, None)
+            if rec is not None:
+                for _ in rec:
+                    pass
+                window = holder[0]
+                ev = torch.cuda.Event()
+                ev.record(side)
+        main.wait_stream(side)
         return g_kb
 
     def J_forward(self, T0, S0, kb, n_steps):
@@ -1418,13 +1476,29 @@ class Adjoint:
 
         st.reset_mixing()
         t0 = time.perf_counter()
-        # forward sweep with checkpoints
+        # forward sweep with checkpoints; with the hand-written reverse the
+        # last window's steps keep their tapes (that window needs no recompute)
         ckpt = {}
+        last = ((n_steps - 1) // every) * every
+        kept = None
         s = State3D(u=s0.u, v=s0.v, b=s0.b, eta=s0.eta, t=0.0, T=s0.T, S=s0.S, e=s0.e)
+        iters_max = 0
         for n in range(n_steps):
+            it0 = st.solver_iterations
             if n % every == 0:
                 ckpt[n] = self._capture(st, s)
-            s = st.step(s)
+            if self.reverse == "hand" and n >= last:
+                if kept is None:
+                    kept = ([ckpt[last]], [])
+                    fin = st._inputs(s)
+                y, tape = self._tape_step(st, fin)
+                kept[0].append(y)
+                kept[1].append(tape)
+                fin = {k: getattr(y, k) for k in ("u", "v", "b", "eta", "T", "S", "e")}
+                s = State3D(u=y.u, v=y.v, b=y.b, eta=y.eta, t=0.0, T=y.T, S=y.S, e=y.e)
+            else:
+                s = st.step(s)
+            iters_max = max(iters_max, st.solver_iterations - it0)
         final = self._capture(st, s)
         # adjoint of J
         fin = [final.u.requires_grad_(), final.v.requires_grad_(), final.eta.requires_grad_()]
@@ -1435,11 +1509,32 @@ class Adjoint:
         adj.update(u=gu, v=gv, eta=geta)
         g_kb = torch.zeros_like(kb)
         # reverse sweep, window by window
-        for start in sorted(ckpt, reverse=True):
+        budget = 0
+        if self.reverse == "hand":
+            if self.sync_free_recompute and self.params.solver in ("pcg_mg", "pcg_rbgs"):
+                budget = -(-iters_max // max(1, self.params.n_picard)) + 2
+            own = ds.owner
+            f0, fa0 = st.solver_failures, own.solver_failures
+            _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, budget))
+            # the adjoint solves (same operator, other right-hand sides) get
+            # two more iterations of room
+            _lib.check(own.lib.hoc_stepper_set_sync_free(own._h, budget + 2 if budget else 0))
+            try:
+                g_kb += self._reverse_sweep(st, ckpt, kept, n_steps, every, kb, adj)
+            finally:
+                _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, 0))
+                _lib.check(own.lib.hoc_stepper_set_sync_free(own._h, 0))
+            if st.solver_failures > f0 or own.solver_failures > fa0:
+                # a solve outran its budget: the gradient again with polled solves
+                self.sync_free_recompute = False
+                try:
+                    r = self.gradient(T0, S0, kb, n_steps=n_steps, every=every)
+                finally:
+                    self.sync_free_recompute = True
+                r.redone_polled = True
+                return r
+        for start in (sorted(ckpt, reverse=True) if self.reverse != "hand" else ()):
             stop = min(start + every, n_steps)
-            if self.reverse == "hand":
-                g_kb += self._reverse_window(st, ckpt[start], start, stop, kb, adj)
-                continue
             states = [ckpt[start]]
             if stop - start > 1:
                 s = self._restore(st, ckpt[start])
@@ -1485,7 +1580,7 @@ class Adjoint:
         gradient_s = time.perf_counter() - t0
         return GradientResult(J=float(J.detach()), dJ_dT0=gT.detach(), dJ_dkappa=g_kb.detach(),
                               forward_s=forward_s, gradient_s=gradient_s, n_steps=n_steps,
-                              every=every)
+                              every=every, solve_budget=budget)
 
 
 def directional_check(adj: Adjoint, T0, S0, kb, result: GradientResult, n_dirs: int = 16,

@@ -165,6 +165,8 @@ struct hoc_stepper {
   // closure-reverse workspace (the first hoc_stepper_closure_adjoint allocates it)
   bool cr_ready = false;
   double *cr_n2, *cr_lu, *cr_ld, *cr_x, *cr_cs, *cr_lam, *cr_gl, *cr_n2b, *cr_guc, *cr_gvc;
+  double *cr_zs, *cr_du, *cr_dv, *cr_row;
+  int* cr_hi;
   // Helmholtz
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
   unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
@@ -2222,8 +2224,8 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
   };
   TRY(init(g_Km, gy_Km, d.ni));
   TRY(init(g_Kh, gy_Kh, d.ni));
-  for (double* b : {h->rv_gq, h->rv_Lu, h->rv_Lv, gx->u, gx->v}) TRY(init(b, nullptr, d.n3));
-  for (double* b : {gx->eta, h->rv_gdiv, h->rv_gKu, h->rv_gKv}) TRY(init(b, nullptr, d.n2));
+  // the other accumulators (gq, Lu, Lv, gx->u, gx->v, gx->eta, gdiv, gKu,
+  // gKv) are written by their first writer in the last Picard pass's reverse
   // tracers and the new velocity's w (model3d_v05.py:466-533)
   k_rev_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->T, y->S, gy->T, gy->S,
                                              gy->b, Kh, h->rv_lamT, h->rv_lamS, h->rv_cs, gx->T,
@@ -2237,19 +2239,20 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
   k_helm_diag<<<blocks(d.n2), TPB, 0, s>>>(d, op, h->rv_inv);
   const int last = cfg.n_picard - 1;
   for (int it = last; it >= 0; --it) {
+    const int first = it == last ? 1 : 0;
     k_rev_backsub<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, it == last ? h->rv_gw : nullptr,
-                                               tape->q, tape->eta[it], h->rv_gui, h->rv_gvi,
-                                               h->rv_gq, h->rv_X, h->rv_Y);
+                                               tape->q, tape->eta[it], first, h->rv_gui,
+                                               h->rv_gvi, h->rv_gq, h->rv_X, h->rv_Y);
     k_rev_eta_rhs<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, it == last ? gy->eta : nullptr,
                                                h->rv_X, h->rv_Y, h->rv_ge);
     LAUNCH_CHECK("reverse back-substitution");
     int iters = 0, conv = 0;
     TRY(helmholtz_solve(h, tape->Ku, tape->Kv, h->rv_inv, h->rv_ge, h->rv_leta, s, &iters, &conv));
-    k_rev_eta_post<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, tape->eta[it], h->rv_leta, h->rv_gKu,
-                                                h->rv_gKv, gx->eta, h->rv_gdiv, h->rv_GU,
-                                                h->rv_GV);
+    k_rev_eta_post<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, tape->eta[it], h->rv_leta, first,
+                                                h->rv_gKu, h->rv_gKv, gx->eta, h->rv_gdiv,
+                                                h->rv_GU, h->rv_GV);
     k_rev_predictor<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, Km, h->rv_gui, h->rv_gvi, h->rv_GU,
-                                                 h->rv_GV, tape->gh_u[it], tape->gh_v[it],
+                                                 h->rv_GV, tape->gh_u[it], tape->gh_v[it], first,
                                                  h->rv_lu, h->rv_lv, h->rv_cs, h->rv_Lu,
                                                  h->rv_Lv, g_Km);
     k_rev_coriolis<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, h->rv_lu, h->rv_lv, h->rv_Lu,
@@ -2286,17 +2289,25 @@ int hoc_stepper_closure_adjoint(hoc_stepper* h, const double* e, const double* u
   hipStream_t s = (hipStream_t)stream;
   if (!h->cr_ready) {
     for (double** q : {&h->cr_n2, &h->cr_lu, &h->cr_ld, &h->cr_x, &h->cr_cs, &h->cr_lam,
-                       &h->cr_gl, &h->cr_n2b})
+                       &h->cr_gl, &h->cr_n2b, &h->cr_zs, &h->cr_du, &h->cr_dv})
       TRY(h->alloc(q, d.ni));
+    TRY(h->alloc(&h->cr_row, 4 * d.ni));
+    TRY(h->alloc(&h->cr_hi, d.n2));
     TRY(h->alloc(&h->cr_guc, d.n3));
     TRY(h->alloc(&h->cr_gvc, d.n3));
     h->cr_ready = true;
   }
-  k_rev_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e, u, v, b, Km, Kh, kappa_b,
-                                             g_e_new, g_Km_new, g_Kh_new, h->cr_n2, h->cr_lu,
-                                             h->cr_ld, h->cr_x, h->cr_cs, h->cr_lam, h->cr_gl,
-                                             h->cr_n2b, g_e, h->cr_guc, h->cr_gvc, g_b, g_Km,
-                                             g_Kh, g_kappa_b);
+  k_rev_closure_col0<<<blocks(d.n2), TPB, 0, s>>>(d, h->geo, b, h->cr_n2, h->cr_n2b, h->cr_zs,
+                                                  h->cr_hi);
+  k_rev_closure_rows<<<blocks(d.ni), TPB, 0, s>>>(d, h->p, h->geo, e, u, v, Km, Kh, kappa_b,
+                                                  h->cr_n2, h->cr_zs, h->cr_hi, h->cr_du,
+                                                  h->cr_dv, h->cr_lu, h->cr_ld, h->cr_row);
+  k_rev_closure<<<blocks(d.n2), TPB, 0, s>>>(d, h->p, h->geo, e, Km, Kh, kappa_b, g_e_new,
+                                             g_Km_new, g_Kh_new, h->cr_n2, h->cr_zs, h->cr_hi,
+                                             h->cr_du, h->cr_dv, h->cr_lu, h->cr_ld, h->cr_row,
+                                             h->cr_x, h->cr_cs, h->cr_lam, h->cr_gl, h->cr_n2b,
+                                             g_e, h->cr_guc, h->cr_gvc, g_b, g_Km, g_Kh,
+                                             g_kappa_b);
   k_rev_closure_uv<<<blocks(d.n3), TPB, 0, s>>>(d, h->cr_guc, h->cr_gvc, g_u, g_v);
   LAUNCH_CHECK("k_rev_closure");
   return HOC_OK;

@@ -418,6 +418,33 @@ __device__ __forceinline__ Dual sqrt(Dual a) {
   return Dual(r, r > 0.0 ? a.d / (2.0 * r) : 0.0);
 }
 
+// Two derivatives at once (with respect to T and S): one evaluation of the
+// EOS expression gives both partials.
+struct Dual2 {
+  double v, a, b;
+  __device__ Dual2() : v(0.0), a(0.0), b(0.0) {}
+  __device__ Dual2(double x) : v(x), a(0.0), b(0.0) {}
+  __device__ Dual2(double x, double da, double db) : v(x), a(da), b(db) {}
+};
+__device__ __forceinline__ Dual2 operator+(Dual2 x, Dual2 y) {
+  return Dual2(x.v + y.v, x.a + y.a, x.b + y.b);
+}
+__device__ __forceinline__ Dual2 operator-(Dual2 x, Dual2 y) {
+  return Dual2(x.v - y.v, x.a - y.a, x.b - y.b);
+}
+__device__ __forceinline__ Dual2 operator-(Dual2 x) { return Dual2(-x.v, -x.a, -x.b); }
+__device__ __forceinline__ Dual2 operator*(Dual2 x, Dual2 y) {
+  return Dual2(x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b);
+}
+__device__ __forceinline__ Dual2 operator/(Dual2 x, Dual2 y) {
+  const double q = x.v / y.v;
+  return Dual2(q, (x.a - q * y.a) / y.v, (x.b - q * y.b) / y.v);
+}
+__device__ __forceinline__ Dual2 sqrt(Dual2 x) {
+  const double r = ::sqrt(x.v);
+  return Dual2(r, r > 0.0 ? x.a / (2.0 * r) : 0.0, r > 0.0 ? x.b / (2.0 * r) : 0.0);
+}
+
 // dL/dT, dL/dS of L = sum(g_b * b(T, S, z)).
 __global__ void k_eos_adjoint(long long n, StepParams p, const double* __restrict__ T,
                               const double* __restrict__ S, const double* __restrict__ z,
@@ -766,12 +793,11 @@ __global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* __res
     const double m = mk(geo.m3[t]);
     double a = gT1 ? gT1[t] : 0.0, b = gS1 ? gS1[t] : 0.0;
     if (gb1 && geo.m3[t]) {
-      const Dual zz(geo.zc[t]);
-      const Dual dT = buoyancy_t<Dual>(p, Dual(T1[t], 1.0), Dual(S1[t]), zz);
-      const Dual dS = buoyancy_t<Dual>(p, Dual(T1[t]), Dual(S1[t], 1.0), zz);
+      const Dual2 db = buoyancy_t<Dual2>(p, Dual2(T1[t], 1.0, 0.0), Dual2(S1[t], 0.0, 1.0),
+                                         Dual2(geo.zc[t]));
       const double gb = gb1[t] * m;
-      a += gb * dT.d;
-      b += gb * dS.d;
+      a += gb * db.a;
+      b += gb * db.b;
     }
     out[0] = a * m;
     out[1] = b * m;
@@ -853,7 +879,7 @@ __global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double
 // adjoint takes through Gx^T, Gy^T.
 __global__ void k_rev_backsub(Dims d, StepParams p, Geo geo, const double* __restrict__ gw,
                               const double* __restrict__ q, const double* __restrict__ eta,
-                              double* __restrict__ gui, double* __restrict__ gvi,
+                              int first, double* __restrict__ gui, double* __restrict__ gvi,
                               double* __restrict__ gq, double* __restrict__ X,
                               double* __restrict__ Y) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -878,7 +904,8 @@ __global__ void k_rev_backsub(Dims d, StepParams p, Geo geo, const double* __res
     const double m = mk(geo.m3[t]);
     const double Au = ui * m, Av = vi * m;
     const double mu = mk(geo.m3u[t]), mv = mk(geo.m3v[t]);
-    gq[t] -= p.c_bs * (gx2 * mu * Au + gy2 * mv * Av);
+    const double dq = -(p.c_bs * (gx2 * mu * Au + gy2 * mv * Av));
+    gq[t] = first ? dq : gq[t] + dq;
     xs += -p.c_bs * q[t] * mu * Au;
     ys += -p.c_bs * q[t] * mv * Av;
     gui[t] = Au;
@@ -907,7 +934,7 @@ __global__ void k_rev_eta_rhs(Dims d, StepParams p, Geo geo, const double* __res
 // surface's and old divergence's shares of rhs = eta - dt((1 - th) div_old
 // + th div_g), and the face values GU, GV of div^T(-dt th lam).
 __global__ void k_rev_eta_post(Dims d, StepParams p, Geo geo, const double* __restrict__ x,
-                               const double* __restrict__ lam, double* __restrict__ gKu,
+                               const double* __restrict__ lam, int first, double* __restrict__ gKu,
                                double* __restrict__ gKv, double* __restrict__ geta0,
                                double* __restrict__ gdiv, double* __restrict__ GU,
                                double* __restrict__ GV) {
@@ -918,10 +945,13 @@ __global__ void k_rev_eta_post(Dims d, StepParams p, Geo geo, const double* __re
   const double mc = mk(geo.m2[c]), me = mk(geo.m2[ce]), mn = mk(geo.m2[cn]);
   const double xc = x[c] * mc, xe = x[ce] * me, xn = x[cn] * mn;
   const double lc = lam[c] * mc, le = lam[ce] * me, ln = lam[cn] * mn;
-  gKu[c] += p.helm_coef * (xe - xc) * (lc - le) / p.dx2;
-  gKv[c] += p.helm_coef * (xn - xc) * (lc - ln) / p.dy2;
-  geta0[c] += lam[c];
-  gdiv[c] -= p.dt * p.one_m_th * lam[c];
+  const double ku = p.helm_coef * (xe - xc) * (lc - le) / p.dx2;
+  const double kv = p.helm_coef * (xn - xc) * (lc - ln) / p.dy2;
+  const double dv = -(p.dt * p.one_m_th * lam[c]);
+  gKu[c] = first ? ku : gKu[c] + ku;
+  gKv[c] = first ? kv : gKv[c] + kv;
+  geta0[c] = first ? lam[c] : geta0[c] + lam[c];
+  gdiv[c] = first ? dv : gdiv[c] + dv;
   const double dc = -p.dt * p.th * lam[c];
   const double de = -p.dt * p.th * lam[ce], dn = -p.dt * p.th * lam[cn];
   GU[c] = (mc * dc - me * de) / p.dx;
@@ -936,7 +966,7 @@ __global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* __r
                                 const double* __restrict__ Au, const double* __restrict__ Av,
                                 const double* __restrict__ GU, const double* __restrict__ GV,
                                 const double* __restrict__ ghu, const double* __restrict__ ghv,
-                                double* __restrict__ lu, double* __restrict__ lv,
+                                int first, double* __restrict__ lu, double* __restrict__ lv,
                                 double* __restrict__ cs, double* __restrict__ Lu,
                                 double* __restrict__ Lv, double* __restrict__ gKm) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -954,8 +984,8 @@ __global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* __r
   rev_column_solve<2>(d, geo, c, Km, geo.dz3, p.thv_dt, p.thv_dt_drag, g, xs, ls, cs, gKm);
   for (int k = 0; k < d.nz; ++k) {
     const long long t = (long long)k * n2 + c;
-    Lu[t] += lu[t];
-    Lv[t] += lv[t];
+    Lu[t] = first ? lu[t] : Lu[t] + lu[t];
+    Lv[t] = first ? lv[t] : Lv[t] + lv[t];
   }
 }
 
@@ -964,8 +994,9 @@ __global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* __r
 // se; the v point in the u-point means of c, n, w, nw. With first == 0
 // (a pass after the first) the previous iterate's adjoints are written:
 // gu = tc Au2v^T(-dt f lv), gv = tc Av2u^T(dt f lu); with first != 0 the
-// old state's shares of both passes are added: gu += Au2v^T(-dt f (tc lv +
-// (1 - tc) Lv)), gv += Av2u^T(dt f (tc lu + (1 - tc) Lu)).
+// old state's shares of both passes are written (the first contributions
+// to its velocity adjoints): gu = Au2v^T(-dt f (tc lv + (1 - tc) Lv)),
+// gv = Av2u^T(dt f (tc lu + (1 - tc) Lu)).
 __global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* __restrict__ lu,
                                const double* __restrict__ lv, const double* __restrict__ Lu,
                                const double* __restrict__ Lv, int first,
@@ -1000,9 +1031,9 @@ __global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* __re
   };
   const double su = mk(mu[t]) * (Hv(0, 0) + Hv(0, 1) + Hv(-1, 0) + Hv(-1, 1));
   const double sv = mk(mv[t]) * (Hu(0, 0) + Hu(1, 0) + Hu(0, -1) + Hu(1, -1));
-  if (first) {
-    gu[t] += su;
-    gv[t] += sv;
+  if (first) {  // the first writer of the old state's velocity adjoints
+    gu[t] = su;
+    gv[t] = sv;
   } else {
     gu[t] = p.theta_cor * su;
     gv[t] = p.theta_cor * sv;
@@ -1166,27 +1197,150 @@ __global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* __rest
 namespace hoc {
 
 // =========================================================== closure reverse
-// tke_step (closure.py:141-205, integral mixing lengths) reversed per column:
-// the forward closure is recomputed (N^2 and shear^2 on the interfaces, the
-// Gaspar lengths with the march stopping once its budget is spent, the
-// implicit interface tridiagonal and its solve), then reversed from the
-// outputs' adjoints (e after its floors, K_m, K_h = (K_m - nu_b)/Pr + kappa_b):
-// K_m's factors, the floors, the transposed solve and its band gradients,
-// dissipation, production, the length maxima/minima and the reverse march
-// (march_step_adjoint) of every interface, N^2 and shear^2 back to b and to
-// the centred velocities (guc, gvc; k_rev_closure_uv spreads them to u, v).
-// A march step taken with a spent budget moves neither l nor the budget and
-// carries a zero adjoint, so the march and its reverse stop there too.
-// Scratch (stride n2, nz + 1 levels): sn2 N^2, slu / sld the lengths, sx
-// the solve, scs sweep coefficients, slam the adjoint right-hand side and
+// tke_step (closure.py:141-205, integral mixing lengths) reversed, in three
+// launches over the columns' interfaces (layout [nz+1][ny][nx]):
+//   k_rev_closure_col0  per column: N^2 on the interfaces (sn2), the depth
+//       of each interface below the surface (szs, sequential sums) and the
+//       end of the column's live interfaces (shi): crossings below the last
+//       interface with N^2 > 0 or a thickness (the dry ones under the bottom)
+//       change neither l nor the budget and carry a zero adjoint, so every
+//       march and its reverse stop there;
+//   k_rev_closure_rows  per interface: the centred-velocity differences
+//       (sdu, sdv), the Gaspar lengths (slu, sld; the march stops once its
+//       budget is spent) and the row of the implicit interface tridiagonal
+//       (srow: sub, diag, sup, rhs);
+//   k_rev_closure       per column: the solve (sx, before the floors), the
+//       outputs' adjoints (e after its floors, K_m, K_h = (K_m - nu_b)/Pr +
+//       kappa_b) back through the floors and K_m's factors, the transposed
+//       solve and its band gradients, dissipation, production, the length
+//       maxima/minima and the reverse march (march_step_adjoint) of every
+//       interface, then N^2 back to b; the centred velocities' adjoints go
+//       to guc, gvc (k_rev_closure_uv spreads them to u, v).
+// Scratch: scs sweep coefficients, slam the adjoint right-hand side and
 // lambda, sgl the adjoint of l_k, sn2b the adjoint of N^2.
+
+struct ClosureCol {
+  const Dims& d;
+  const Geo& geo;
+  long long c;
+  __device__ long long T(int k) const { return (long long)k * d.n2 + c; }
+  __device__ double dzi(int k) const {
+    if (k == 0) return geo.dz3[c];
+    if (k == d.nz) return geo.dz3[T(d.nz - 1)];
+    return 0.5 * (geo.dz3[T(k - 1)] + geo.dz3[T(k)]);
+  }
+  __device__ double wet(int k) const {
+    return (k >= 1 && k < d.nz) ? mk(geo.m3[T(k - 1)]) * mk(geo.m3[T(k)]) : 0.0;
+  }
+};
+
+__device__ __forceinline__ double rc_ssqrt(double x) { return x > 0.0 ? ::sqrt(x) : 0.0; }
+__device__ __forceinline__ double rc_dssqrt(double x) { return x > 0.0 ? 0.5 / ::sqrt(x) : 0.0; }
+
+__global__ void k_rev_closure_col0(Dims d, Geo geo, const double* __restrict__ b,
+                                   double* __restrict__ sn2, double* __restrict__ sn2b,
+                                   double* __restrict__ szs, int* __restrict__ shi) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.n2) return;
+  const ClosureCol C{d, geo, c};
+  int hi = 0;
+  double zs = 0.0;
+  for (int k = 0; k <= d.nz; ++k) {
+    if (k > 0) zs = zs + geo.dz3[C.T(k - 1)];
+    szs[C.T(k)] = zs;
+    double n2k = 0.0;
+    const double dz = C.dzi(k);
+    if (k >= 1 && k < d.nz) n2k = (b[C.T(k - 1)] - b[C.T(k)]) * safe_inv(dz) * C.wet(k);
+    sn2[C.T(k)] = n2k;
+    sn2b[C.T(k)] = 0.0;
+    if (nmax(n2k, 0.0) != 0.0 || dz != 0.0) hi = k + 1;
+  }
+  shi[c] = hi;
+}
+
+__global__ void k_rev_closure_rows(Dims d, StepParams p, Geo geo, const double* __restrict__ e,
+                                   const double* __restrict__ u, const double* __restrict__ v,
+                                   const double* __restrict__ Km0,
+                                   const double* __restrict__ Kh0,
+                                   const double* __restrict__ kb,
+                                   const double* __restrict__ sn2,
+                                   const double* __restrict__ szs, const int* __restrict__ shi,
+                                   double* __restrict__ sdu, double* __restrict__ sdv,
+                                   double* __restrict__ slu, double* __restrict__ sld,
+                                   double* __restrict__ srow) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.ni) return;
+  const long long n2 = d.n2, ni = d.ni;
+  const int nz = d.nz;
+  const int k = (int)(t / n2);
+  const long long c = t % n2;
+  const ClosureCol C{d, geo, c};
+  const double w = C.wet(k);
+  double* sub = srow;
+  double* diag = srow + ni;
+  double* sup = srow + 2 * ni;
+  double* rhs = srow + 3 * ni;
+  if (!(w > 0.0)) {  // lengths carry the factor wet; boundary rows are fixed
+    sdu[t] = 0.0;
+    sdv[t] = 0.0;
+    slu[t] = 0.0;
+    sld[t] = 0.0;
+    sub[t] = 0.0;
+    diag[t] = 1.0;
+    sup[t] = 0.0;
+    rhs[t] = k == 0 ? (geo.m3[c] ? p.e_bb * geo.ustar2[c] : p.e_min) : p.e_min;
+    return;
+  }
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
+  const long long a = (long long)(k - 1) * n2, o = (long long)k * n2;
+  const double idzi = safe_inv(C.dzi(k));
+  const double du = (0.5 * (u[a + c] + u[a + cw]) - 0.5 * (u[o + c] + u[o + cw])) * idzi;
+  const double dv = (0.5 * (v[a + c] + v[a + cs]) - 0.5 * (v[o + c] + v[o + cs])) * idzi;
+  sdu[t] = du;
+  sdv[t] = dv;
+  const int hi = shi[c];
+  const double zs = szs[t], H = szs[(long long)nz * n2 + c];
+  const double ek = e[t];
+  const double e0 = nmax(ek, p.e_min);
+  double l = 0.0, bud = e0;
+  for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
+    const MarchIn m = march_step(l, bud, nmax(sn2[C.T(kk)], 0.0), C.dzi(kk));
+    l = m.l;
+    bud = m.b;
+  }
+  const double lu = nmin(l, zs + p.z_0) * w;
+  l = 0.0;
+  bud = e0;
+  for (int kk = k + 1; kk < hi && bud > 0.0; ++kk) {
+    const MarchIn m = march_step(l, bud, nmax(sn2[C.T(kk)], 0.0), C.dzi(kk));
+    l = m.l;
+    bud = m.b;
+  }
+  const double ld = nmin(l, nmax(H - zs, 0.0) + p.z_0) * w;
+  slu[t] = lu;
+  sld[t] = ld;
+  const double l_eps = nmax(p.l_min, nmin(lu, ld));
+  const double diss = p.c_eps * rc_ssqrt(e0) / l_eps;
+  const double sh2 = (du * du + dv * dv) * w;
+  const double prod = (Km0[t] - p.nu_b) * sh2 - (Kh0[t] - kb[c]) * sn2[t];
+  const double klu = 0.5 * (Km0[t - n2] + Km0[t]), kld = 0.5 * (Km0[t] + Km0[t + n2]);
+  const double a_up = p.dt * klu * safe_inv(geo.dz3[t - n2]) * idzi;
+  const double a_dn = p.dt * kld * safe_inv(geo.dz3[t]) * idzi;
+  sub[t] = -a_up;
+  sup[t] = -a_dn;
+  diag[t] = 1.0 + a_up + a_dn + p.dt * diss;
+  rhs[t] = ek + p.dt * prod;
+}
+
 __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __restrict__ e,
-                              const double* __restrict__ u, const double* __restrict__ v,
-                              const double* __restrict__ b, const double* __restrict__ Km0,
-                              const double* __restrict__ Kh0, const double* __restrict__ kb,
-                              const double* __restrict__ ge1, const double* __restrict__ gKm1,
-                              const double* __restrict__ gKh1, double* __restrict__ sn2,
-                              double* __restrict__ slu, double* __restrict__ sld,
+                              const double* __restrict__ Km0, const double* __restrict__ Kh0,
+                              const double* __restrict__ kb, const double* __restrict__ ge1,
+                              const double* __restrict__ gKm1, const double* __restrict__ gKh1,
+                              const double* __restrict__ sn2, const double* __restrict__ szs,
+                              const int* __restrict__ shi, const double* __restrict__ sdu,
+                              const double* __restrict__ sdv, const double* __restrict__ slu,
+                              const double* __restrict__ sld, const double* __restrict__ srow,
                               double* __restrict__ sx, double* __restrict__ scs,
                               double* __restrict__ slam, double* __restrict__ sgl,
                               double* __restrict__ sn2b, double* __restrict__ ge0,
@@ -1195,103 +1349,27 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
                               double* __restrict__ gKh0, double* __restrict__ gkb) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
-  const long long n2 = d.n2;
+  const long long n2 = d.n2, ni = d.ni;
   const int nz = d.nz;
-  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
-  const long long cw = at2(d, j, wrap(i, -1, d.nx)), cs = at2(d, wrap(j, -1, d.ny), i);
-  const double* dz3 = geo.dz3;
-  auto T = [&](int k) { return (long long)k * n2 + c; };
-  auto dzi = [&](int k) {
-    if (k == 0) return dz3[c];
-    if (k == nz) return dz3[T(nz - 1)];
-    return 0.5 * (dz3[T(k - 1)] + dz3[T(k)]);
-  };
-  auto wet_i = [&](int k) {
-    return (k >= 1 && k < nz) ? mk(geo.m3[T(k - 1)]) * mk(geo.m3[T(k)]) : 0.0;
-  };
-  auto uc = [&](int k) { const long long o = (long long)k * n2; return 0.5 * (u[o + c] + u[o + cw]); };
-  auto vc = [&](int k) { const long long o = (long long)k * n2; return 0.5 * (v[o + c] + v[o + cs]); };
-  auto dssqrt = [](double x) -> double { return x > 0.0 ? 0.5 / ::sqrt(x) : 0.0; };
-  auto ssqrt = [](double x) -> double { return x > 0.0 ? ::sqrt(x) : 0.0; };
-  double H = 0.0;
-  for (int k = 0; k < nz; ++k) H = H + dz3[T(k)];
+  const ClosureCol C{d, geo, c};
+  auto T = [&](int k) { return C.T(k); };
+  const double* sub = srow;
+  const double* diag = srow + ni;
+  const double* sup = srow + 2 * ni;
+  const double* rhs = srow + 3 * ni;
   const double kbc = kb[c];
-  // ---- forward: N^2, shear^2 (recomputed where needed), lengths
-  // crossings of interfaces below the last one with N^2 > 0 or a thickness
-  // (the dry ones under the bottom) change neither l nor the budget and
-  // carry a zero adjoint: the marches end at `hi`
-  int hi = 0;
-  for (int k = 0; k <= nz; ++k) {
-    double n2k = 0.0;
-    if (k >= 1 && k < nz) n2k = (b[T(k - 1)] - b[T(k)]) * safe_inv(dzi(k)) * wet_i(k);
-    sn2[T(k)] = n2k;
-    sn2b[T(k)] = 0.0;
-    if (nmax(n2k, 0.0) != 0.0 || dzi(k) != 0.0) hi = k + 1;
-  }
-  auto sh2_at = [&](int k) {
-    if (k < 1 || k >= nz) return 0.0;
-    const double idz = safe_inv(dzi(k));
-    const double du = (uc(k - 1) - uc(k)) * idz, dv = (vc(k - 1) - vc(k)) * idz;
-    return (du * du + dv * dv) * wet_i(k);
-  };
-  double zs = 0.0;
-  for (int k = 0; k <= nz; ++k) {
-    if (k > 0) zs = zs + dz3[T(k - 1)];
-    if (!(wet_i(k) > 0.0)) {  // the lengths carry the factor wet
-      slu[T(k)] = 0.0;
-      sld[T(k)] = 0.0;
-      continue;
-    }
-    const double ek = nmax(e[T(k)], p.e_min);
-    double l = 0.0, bud = ek;
-    for (int kk = k - 1; kk >= 0 && bud > 0.0; --kk) {
-      const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
-      l = o.l;
-      bud = o.b;
-    }
-    slu[T(k)] = nmin(l, zs + p.z_0) * wet_i(k);
-    l = 0.0;
-    bud = ek;
-    for (int kk = k + 1; kk < hi && bud > 0.0; ++kk) {
-      const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
-      l = o.l;
-      bud = o.b;
-    }
-    sld[T(k)] = nmin(l, nmax(H - zs, 0.0) + p.z_0) * wet_i(k);
-  }
-  // rows of the interface tridiagonal (closure.py:176-196)
-  auto row = [&](int k, double& sub, double& diag, double& sup, double& rhs) {
-    if (!(wet_i(k) > 0.0)) {
-      sub = 0.0;
-      diag = 1.0;
-      sup = 0.0;
-      rhs = k == 0 ? (geo.m3[c] ? p.e_bb * geo.ustar2[c] : p.e_min) : p.e_min;
-      return;
-    }
-    const double lu = slu[T(k)], ld = sld[T(k)];
-    const double l_eps = nmax(p.l_min, nmin(lu, ld));
-    const double ek = e[T(k)];
-    const double diss = p.c_eps * ssqrt(nmax(ek, p.e_min)) / l_eps;
-    const double prod = (Km0[T(k)] - p.nu_b) * sh2_at(k) - (Kh0[T(k)] - kbc) * sn2[T(k)];
-    const double klu = 0.5 * (Km0[T(k - 1)] + Km0[T(k)]), kld = 0.5 * (Km0[T(k)] + Km0[T(k + 1)]);
-    const double idzi = safe_inv(dzi(k));
-    const double a_up = p.dt * klu * safe_inv(dz3[T(k - 1)]) * idzi;
-    const double a_dn = p.dt * kld * safe_inv(dz3[T(k)]) * idzi;
-    sub = -a_up;
-    sup = -a_dn;
-    diag = 1.0 + a_up + a_dn + p.dt * diss;
-    rhs = ek + p.dt * prod;
-  };
+  const int hi = shi[c];
+  const double H = szs[T(nz)];
+  // the forward solve of the interface tridiagonal
   {
     double cp = 0.0, dp = 0.0;
     for (int k = 0; k <= nz; ++k) {
-      double sb, dg, sp, rh;
-      row(k, sb, dg, sp, rh);
-      const double den = k == 0 ? dg : dg - sb * cp;
-      cp = sp / den;
-      dp = k == 0 ? rh / den : (rh - sb * dp) / den;
-      scs[T(k)] = cp;
-      sx[T(k)] = dp;
+      const long long t = T(k);
+      const double den = k == 0 ? diag[t] : diag[t] - sub[t] * cp;
+      cp = sup[t] / den;
+      dp = k == 0 ? rhs[t] / den : (rhs[t] - sub[t] * dp) / den;
+      scs[t] = cp;
+      sx[t] = dp;
     }
     double xn = sx[T(nz)];
     for (int k = nz - 1; k >= 0; --k) {
@@ -1299,26 +1377,19 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       sx[T(k)] = xn;
     }
   }
-  // ---- reverse: the outputs' adjoints back to the solve's
+  // the outputs' adjoints back to the solve's
   double gkb_acc = 0.0;
   for (int k = 0; k <= nz; ++k) {
     const long long t = T(k);
     const double gK = gKm1[t] + gKh1[t] / p.pr_t;
     gkb_acc += gKh1[t];
-    const double w = wet_i(k);
+    const double w = C.wet(k);
     const double x = sx[t];
-    double e_new;
-    if (k == 0) {
-      double sb, dg, sp, rh;
-      row(0, sb, dg, sp, rh);
-      e_new = rh;
-    } else {
-      e_new = w > 0.0 ? nmax(x, p.e_min) : p.e_min;
-    }
+    const double e_new = k == 0 ? rhs[t] : (w > 0.0 ? nmax(x, p.e_min) : p.e_min);
     const double lu = slu[t], ld = sld[t];
-    const double l_k = nmax(p.l_min, ssqrt(lu * ld));
-    sgl[t] = p.c_k * ssqrt(e_new) * w * gK;
-    const double ge_new = (ge1 ? ge1[t] : 0.0) + p.c_k * l_k * w * gK * dssqrt(e_new);
+    const double l_k = nmax(p.l_min, rc_ssqrt(lu * ld));
+    sgl[t] = p.c_k * rc_ssqrt(e_new) * w * gK;
+    const double ge_new = (ge1 ? ge1[t] : 0.0) + p.c_k * l_k * w * gK * rc_dssqrt(e_new);
     slam[t] = (k >= 1 && w > 0.0 && (x >= p.e_min || x != x)) ? ge_new : 0.0;
     ge0[t] = 0.0;
     gKm0[t] = 0.0;
@@ -1329,27 +1400,17 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
     gvc[T(k)] = 0.0;
   }
   {  // lambda = A^-T g: sub_T[k] = sup[k-1], sup_T[k] = sub[k+1]
-    double cp = 0.0, dp = 0.0, sup_prev = 0.0;
-    double sb, dg, sp, rh, sbn = 0.0, dgn = 1.0, spn = 0.0, rhn = 0.0;
-    row(0, sb, dg, sp, rh);
+    double cp = 0.0, dp = 0.0;
     for (int k = 0; k <= nz; ++k) {
-      double supT = 0.0;
-      if (k < nz) {
-        row(k + 1, sbn, dgn, spn, rhn);
-        supT = sbn;
-      }
-      const double den = k == 0 ? dg : dg - sup_prev * cp;
+      const long long t = T(k);
+      const double supT = k < nz ? sub[t + n2] : 0.0;
+      const double den = k == 0 ? diag[t] : diag[t] - sup[t - n2] * cp;
       const double cc = supT / den;
-      const double dd = k == 0 ? slam[T(k)] / den : (slam[T(k)] - sup_prev * dp) / den;
-      scs[T(k)] = cc;
-      slam[T(k)] = dd;
+      const double dd = k == 0 ? slam[t] / den : (slam[t] - sup[t - n2] * dp) / den;
+      scs[t] = cc;
+      slam[t] = dd;
       cp = cc;
       dp = dd;
-      sup_prev = sp;
-      sb = sbn;
-      dg = dgn;
-      sp = spn;
-      rh = rhn;
     }
     double ln = slam[T(nz)];
     for (int k = nz - 1; k >= 0; --k) {
@@ -1359,39 +1420,34 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
   }
   // per interface: the row's band and right-hand-side adjoints, then the
   // lengths and their reverse marches
-  zs = 0.0;
   for (int k = 0; k <= nz; ++k) {
     const long long t = T(k);
-    if (k > 0) zs = zs + dz3[T(k - 1)];
-    const double w = wet_i(k);
+    const double w = C.wet(k);
     const double lu = slu[t], ld = sld[t];
     double gl_eps = 0.0;
     const double ek = e[t];
     double gek = 0.0;
     if (w > 0.0) {
       const double lam = slam[t];
-      const double xk = sx[t];
-      const double gd = -lam * xk, gs = -lam * sx[t - n2], gp = -lam * sx[t + n2];
+      const double gd = -lam * sx[t], gs = -lam * sx[t - n2], gp = -lam * sx[t + n2];
       const double ga_up = gd - gs, ga_dn = gd - gp, gdiss = p.dt * gd;
-      const double idzi = safe_inv(dzi(k));
-      const double glu = p.dt * safe_inv(dz3[T(k - 1)]) * idzi * ga_up;
-      const double gld = p.dt * safe_inv(dz3[t]) * idzi * ga_dn;
+      const double idzi = safe_inv(C.dzi(k));
+      const double glu = p.dt * safe_inv(geo.dz3[t - n2]) * idzi * ga_up;
+      const double gld = p.dt * safe_inv(geo.dz3[t]) * idzi * ga_dn;
       gKm0[t - n2] += 0.5 * glu;
       gKm0[t] += 0.5 * glu + 0.5 * gld;
       gKm0[t + n2] += 0.5 * gld;
       gek += lam;
       const double gprod = p.dt * lam;
       const double n2k = sn2[t];
-      // shear^2 at k and its adjoint to the centred velocities
-      const double idz = idzi;
-      const double du = (uc(k - 1) - uc(k)) * idz, dv = (vc(k - 1) - vc(k)) * idz;
+      const double du = sdu[t], dv = sdv[t];
       const double sh2 = (du * du + dv * dv) * w;
       gKm0[t] += sh2 * gprod;
       const double gsh = (Km0[t] - p.nu_b) * gprod;
       gKh0[t] += -n2k * gprod;
       gkb_acc += n2k * gprod;
       sn2b[t] += -(Kh0[t] - kbc) * gprod;
-      const double gdu = 2.0 * du * w * gsh * idz, gdv = 2.0 * dv * w * gsh * idz;
+      const double gdu = 2.0 * du * w * gsh * idzi, gdv = 2.0 * dv * w * gsh * idzi;
       guc[t - n2] += gdu;
       guc[t] -= gdu;
       gvc[t - n2] += gdv;
@@ -1399,8 +1455,8 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       // dissipation c_eps sqrt(max(e, e_min)) / l_eps
       const double E = nmax(ek, p.e_min);
       const double l_eps = nmax(p.l_min, nmin(lu, ld));
-      const double diss = p.c_eps * ssqrt(E) / l_eps;
-      if (ek >= p.e_min || ek != ek) gek += gdiss * p.c_eps * dssqrt(E) / l_eps;
+      const double diss = p.c_eps * rc_ssqrt(E) / l_eps;
+      if (ek >= p.e_min || ek != ek) gek += gdiss * p.c_eps * rc_dssqrt(E) / l_eps;
       gl_eps = -gdiss * diss / l_eps;
     }
     // l_eps = max(l_min, min(l_up, l_dn)), l_k = max(l_min, sqrt(l_up l_dn))
@@ -1411,13 +1467,14 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       if (lu <= ld || lu != lu) glu += gX;
       else gld += gX;
       const double P = lu * ld;
-      const double gY = (ssqrt(P) <= p.l_min) ? 0.0 : sgl[t];
-      const double gP = gY * dssqrt(P);
+      const double gY = (rc_ssqrt(P) <= p.l_min) ? 0.0 : sgl[t];
+      const double gP = gY * rc_dssqrt(P);
       glu += gP * ld;
       gld += gP * lu;
     }
     // reverse marches of interface k (closure.py:65-103)
     const double ek0 = nmax(ek, p.e_min);
+    const double zs = szs[t];
     double gb0_acc = 0.0;
     for (int dir = 0; dir < 2; ++dir) {
       const double gout = (dir == 0 ? glu : gld) * w;
@@ -1429,7 +1486,7 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       double l = 0.0, bud = ek0;
       for (int kk = dir == 0 ? k - 1 : k + 1; (dir == 0 ? kk >= 0 : kk < hi) && bud > 0.0;
            kk += dir == 0 ? -1 : 1) {
-        const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), dzi(kk));
+        const MarchIn o = march_step(l, bud, nmax(sn2[T(kk)], 0.0), C.dzi(kk));
         l = o.l;
         bud = o.b;
         ++nstep;
@@ -1439,14 +1496,14 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
         double ls = 0.0, bs = ek0;
         int kk = dir == 0 ? k - 1 : k + 1;
         for (int r = 1; r < s; ++r) {
-          const MarchIn o = march_step(ls, bs, nmax(sn2[T(kk)], 0.0), dzi(kk));
+          const MarchIn o = march_step(ls, bs, nmax(sn2[T(kk)], 0.0), C.dzi(kk));
           ls = o.l;
           bs = o.b;
           kk += dir == 0 ? -1 : 1;
         }
         const double n2r = sn2[T(kk)];
         double lb_in, bb_in, nb = 0.0;
-        march_step_adjoint(ls, bs, nmax(n2r, 0.0), dzi(kk), lb, bb, lb_in, bb_in, nb);
+        march_step_adjoint(ls, bs, nmax(n2r, 0.0), C.dzi(kk), lb, bb, lb_in, bb_in, nb);
         lb = lb_in;
         bb = bb_in;
         if (n2r >= 0.0 || n2r != n2r) sn2b[T(kk)] += nb;
@@ -1459,7 +1516,7 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
   gkb[c] += gkb_acc;
   // N^2 = (b[k-1] - b[k]) / dzi wet back to b
   for (int k = 1; k < nz; ++k) {
-    const double gdb = sn2b[T(k)] * wet_i(k) * safe_inv(dzi(k));
+    const double gdb = sn2b[T(k)] * C.wet(k) * safe_inv(C.dzi(k));
     gb0[T(k - 1)] += gdb;
     gb0[T(k)] -= gdb;
   }

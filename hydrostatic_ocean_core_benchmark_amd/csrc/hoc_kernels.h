@@ -180,17 +180,17 @@ __global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double
                                      const double* gu_y, const double* gv_y, double* gT0,
                                      double* gS0, double* gu1, double* gv1, double* gw1);
 __global__ void k_rev_backsub(Dims d, StepParams p, Geo geo, const double* gw, const double* q,
-                              const double* eta, double* gui, double* gvi, double* gq, double* X,
-                              double* Y);
+                              const double* eta, int first, double* gui, double* gvi, double* gq,
+                              double* X, double* Y);
 __global__ void k_rev_eta_rhs(Dims d, StepParams p, Geo geo, const double* geta, const double* X,
                               const double* Y, double* ge);
 __global__ void k_rev_eta_post(Dims d, StepParams p, Geo geo, const double* x, const double* lam,
-                               double* gKu, double* gKv, double* geta0, double* gdiv, double* GU,
-                               double* GV);
+                               int first, double* gKu, double* gKv, double* geta0, double* gdiv,
+                               double* GU, double* GV);
 __global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* Km, const double* Au,
                                 const double* Av, const double* GU, const double* GV,
-                                const double* ghu, const double* ghv, double* lu, double* lv,
-                                double* cs, double* Lu, double* Lv, double* gKm);
+                                const double* ghu, const double* ghv, int first, double* lu,
+                                double* lv, double* cs, double* Lu, double* Lv, double* gKm);
 __global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* lu, const double* lv,
                                const double* Lu, const double* Lv, int first, double* gu,
                                double* gv);
@@ -201,13 +201,21 @@ __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* u0, 
 __global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* u0,
                                    const double* v0, const double* w0, const double* Lu,
                                    const double* Lv, double* gu0, double* gv0, double* gw0);
-__global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* e, const double* u,
-                              const double* v, const double* b, const double* Km0,
+__global__ void k_rev_closure_col0(Dims d, Geo geo, const double* b, double* sn2, double* sn2b,
+                                   double* szs, int* shi);
+__global__ void k_rev_closure_rows(Dims d, StepParams p, Geo geo, const double* e,
+                                   const double* u, const double* v, const double* Km0,
+                                   const double* Kh0, const double* kb, const double* sn2,
+                                   const double* szs, const int* shi, double* sdu, double* sdv,
+                                   double* slu, double* sld, double* srow);
+__global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* e, const double* Km0,
                               const double* Kh0, const double* kb, const double* ge1,
-                              const double* gKm1, const double* gKh1, double* sn2, double* slu,
-                              double* sld, double* sx, double* scs, double* slam, double* sgl,
-                              double* sn2b, double* ge0, double* guc, double* gvc, double* gb0,
-                              double* gKm0, double* gKh0, double* gkb);
+                              const double* gKm1, const double* gKh1, const double* sn2,
+                              const double* szs, const int* shi, const double* sdu,
+                              const double* sdv, const double* slu, const double* sld,
+                              const double* srow, double* sx, double* scs, double* slam,
+                              double* sgl, double* sn2b, double* ge0, double* guc, double* gvc,
+                              double* gb0, double* gKm0, double* gKh0, double* gkb);
 __global__ void k_rev_closure_uv(Dims d, const double* guc, const double* gvc, double* gu,
                                  double* gv);
 __global__ void k_rev_wold_q(Dims d, StepParams p, Geo geo, const double* gw0, const double* Km,

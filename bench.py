@@ -308,8 +308,9 @@ def adjoint_ratio(steps: int = 60, every: int = 10, nx: int = 256, nz: int = 30,
            "forward_ms": round(r.forward_s * 1e3, 2), "gradient_ms": round(r.gradient_s * 1e3, 1),
            "adjoint_forward_ratio": round(r.gradient_s / r.forward_s, 2), "J": r.J,
            "reverse": "window recompute (hoc_stepper_step_tape) on a second stream under the "
-                      "reverse of the next window; recompute and adjoint solves sync-free, "
-                      f"{r.solve_budget} iterations per solve"
+                      "reverse of the next window; sync-free solves: recompute "
+                      f"{r.solve_budget} iterations, adjoint (pcg_mg with the rbgs smoother) "
+                      f"{r.adjoint_budget}"
                       + (" (a solve outran it: timed again polled)" if r.redone_polled else ""),
            "hand_written": "the whole step: hoc_stepper_step_adjoint (tracers with the EOS, "
                            "advection, Picard passes with adjoint Helmholtz PCG solves, "

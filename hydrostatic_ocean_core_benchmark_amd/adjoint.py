@@ -635,7 +635,15 @@ class DiffStep:
         self.owner = None
         if kernels == "hip":
             from .stepper import HipStepper
-            self.owner = HipStepper(domain, physics, params, dt, device=device)
+            # the handle of the reverse kernels and of the adjoint Helmholtz
+            # solves; with pcg_mg those use the red-black Gauss-Seidel
+            # smoother (9 iterations to the same tolerance where damped Jacobi
+            # takes 10-11 on the 256^2 x 30 basin: any convergent
+            # preconditioner gives the adjoint system's solution to rtol)
+            rparams = params
+            if params.solver == "pcg_mg":
+                rparams = dataclasses.replace(params, mg_smoother="rbgs")
+            self.owner = HipStepper(domain, physics, rparams, dt, device=device)
         self.helm = _HelmholtzOperator(self.owner, self.mask, coef, g.dx, g.dy)
 
     # ------------------------------------------------------ operators
@@ -1321,6 +1329,7 @@ class GradientResult:
     n_steps: int
     every: int
     solve_budget: int = 0     # sync-free iterations per recompute solve (0: polled)
+    adjoint_budget: int = 0   # sync-free iterations per adjoint solve (0: polled)
     redone_polled: bool = False
 
 
@@ -1342,11 +1351,12 @@ class Adjoint:
         self.device = torch.device(device)
         self.kernels = kernels
         self.reverse = reverse
-        # the window recomputes and the adjoint solves run sync-free with a
-        # budget from the forward sweep's iteration counts (the polled results
-        # when every solve converges within it; otherwise the gradient is
-        # redone with polled solves)
+        # the window recomputes run sync-free with a budget from the forward
+        # sweep's iteration counts, the adjoint solves with one from the
+        # first reverse step (the polled results when every solve converges
+        # within its budget; otherwise the gradient is redone polled)
         self.sync_free_recompute = True
+        self.adjoint_budget = 0
         self.ds = DiffStep(domain, physics, params, dt, device=device, kernels=kernels)
 
     def _hip(self, kb_np):
@@ -1398,7 +1408,8 @@ class Adjoint:
             fin = {k: getattr(y, k) for k in ("u", "v", "b", "eta", "T", "S", "e")}
             yield
 
-    def _reverse_sweep(self, st, ckpt: dict, kept, n_steps: int, every: int, kb, adj: dict):
+    def _reverse_sweep(self, st, ckpt: dict, kept, n_steps: int, every: int, kb, adj: dict,
+                       sync_free: bool = False):
         """The windows reversed last to first (the last from the forward
         sweep's tapes `kept`); while a window is reversed on the current
         stream, the one before it is recomputed on a second stream (its own
@@ -1406,8 +1417,13 @@ class Adjoint:
         sequences of small kernels share the device. The reverse of a window
         waits for the event that ends its recompute; tensors made on the
         second stream are recorded on the first before use. `adj` becomes
-        the adjoint of the state before step 0; returns dJ/dkappa_b."""
+        the adjoint of the state before step 0; returns dJ/dkappa_b. With
+        sync_free the adjoint solves after the first reverse step (polled)
+        run sync-free with one iteration more than that step's largest
+        solve."""
         main = torch.cuda.current_stream(self.device)
+        own = self.ds.owner
+        calibrate, it_a = sync_free, own.solver_iterations
         side = torch.cuda.Stream(device=self.device)
         side.wait_stream(main)
         starts = sorted(ckpt, reverse=True)
@@ -1436,6 +1452,11 @@ class Adjoint:
                 adj.clear()
                 adj.update(out)
                 g_kb += gk
+                if calibrate:
+                    calibrate = False
+                    per = (own.solver_iterations - it_a) / max(1, self.params.n_picard)
+                    self.adjoint_budget = math.ceil(per) + 1
+                    _lib.check(own.lib.hoc_stepper_set_sync_free(own._h, self.adjoint_budget))
                 states[i + 1] = tapes[i] = None
                 if rec is not None:
                     next(rec, None)
@@ -1468,12 +1489,20 @@ class Adjoint:
         # timed plain forward (the adjoint:forward denominator)
         with torch.no_grad():
             s0 = ds.initial_state(T0, S0, kb)
+        it_f = st.solver_iterations
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         s = State3D(u=s0.u, v=s0.v, b=s0.b, eta=s0.eta, t=0.0, T=s0.T, S=s0.S, e=s0.e)
         s = st.integrate(s, n_steps)
         torch.cuda.synchronize(self.device)
         forward_s = time.perf_counter() - t0
+        n_solves = max(1, n_steps * max(1, self.params.n_picard))
+        sync_free = (self.reverse == "hand" and self.sync_free_recompute
+                     and self.params.solver in ("pcg_mg", "pcg_rbgs"))
+        f0 = st.solver_failures
+        if sync_free:  # the checkpointed sweep repeats the timed forward's solves
+            fwd_budget = math.ceil((st.solver_iterations - it_f) / n_solves) + 2
+            _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, fwd_budget))
 
         st.reset_mixing()
         t0 = time.perf_counter()
@@ -1519,16 +1548,14 @@ class Adjoint:
         # reverse sweep, window by window
         budget = 0
         if self.reverse == "hand":
-            if self.sync_free_recompute and self.params.solver in ("pcg_mg", "pcg_rbgs"):
+            if sync_free:
                 budget = math.ceil(per_solve) + 2
             own = ds.owner
-            f0, fa0 = st.solver_failures, own.solver_failures
+            fa0 = own.solver_failures
             _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, budget))
-            # the adjoint solves (same operator, other right-hand sides; measured
-            # on the 256^2 x 30 basin: 10-11 iterations against the forward's 10)
-            _lib.check(own.lib.hoc_stepper_set_sync_free(own._h, budget))
             try:
-                g_kb += self._reverse_sweep(st, ckpt, kept, n_steps, every, kb, adj)
+                g_kb += self._reverse_sweep(st, ckpt, kept, n_steps, every, kb, adj,
+                                            sync_free=budget > 0)
             finally:
                 _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, 0))
                 _lib.check(own.lib.hoc_stepper_set_sync_free(own._h, 0))
@@ -1588,7 +1615,8 @@ class Adjoint:
         gradient_s = time.perf_counter() - t0
         return GradientResult(J=float(J.detach()), dJ_dT0=gT.detach(), dJ_dkappa=g_kb.detach(),
                               forward_s=forward_s, gradient_s=gradient_s, n_steps=n_steps,
-                              every=every, solve_budget=budget)
+                              every=every, solve_budget=budget,
+                              adjoint_budget=self.adjoint_budget if budget else 0)
 
 
 def directional_check(adj: Adjoint, T0, S0, kb, result: GradientResult, n_dirs: int = 16,

@@ -1,40 +1,49 @@
-"""Adjoint of the fp64 step: a differentiable restatement of
-Stepper3DV05.step (libs/core/model3d_v05.py:337-415, theta scheme) in torch
-operations on the device, three operators of it behind torch.autograd
-Functions with hand-written adjoints, and the checkpointed gradient of a
-scalar of the state after n steps.
+"""Adjoint of the fp64 step (Stepper3DV05.step, libs/core/model3d_v05.py:
+337-415, theta scheme) and the checkpointed gradient of a scalar of the
+state after n steps.
 
-Hand-written adjoints (kernels="hip"):
-  equation of state      forward: the HIP EOS; adjoint: dual numbers through
-                         the same expression (hoc_eos_buoyancy_adjoint);
-  mixing lengths         forward: the Gaspar budget march on the device
-                         (hoc_gaspar_lengths); adjoint: the reverse march per
-                         interface (hoc_gaspar_lengths_adjoint);
-  momentum advection     forward: the HIP operator; adjoint: the gathered
-                         reverse kernel (hoc_stepper_momentum_advection_adjoint);
-  Thomas solves          forward: the HIP batched Thomas solve; adjoint: the
-                         transposed solve on the device (hoc_thomas_adjoint);
-  Helmholtz solve        forward: the stepper's PCG (hoc_stepper_helmholtz_solve);
-                         adjoint: the same PCG on the adjoint right-hand side
-                         (the masked operator is symmetric) plus the face
-                         coefficient gradients of -lambda^T (dA/dK) x;
-  tracer advection       forward: the HIP operator; adjoint: the gathered
-                         reverse kernel (hoc_stepper_tracer_advection_adjoint).
-Everything else of the step (the rest of the closure, Coriolis,
-pressure gradients, the Picard loop) is differentiated by autograd through
-its torch operations. kernels="torch" replaces the six Functions by torch
-operations (the momentum advection formula, the EOS polynomial, the march, a sequential Thomas sweep, a dense solve of the
-Helmholtz operator, the advection formula), so autograd differentiates the whole step:
-the reference gradient the hand-written adjoints are checked against, on
-grids small enough for the dense solve.
+The reverse step runs in hand-written HIP kernels (hoc_adjoint.hip) behind
+three C-ABI entries: hoc_stepper_step_tape (the step, bit for bit, writing
+the intermediates the reverse reads), hoc_stepper_step_adjoint (everything
+after the closure: tracers with the EOS, advection, the Picard passes with
+adjoint Helmholtz PCG solves, Coriolis, pressure gradients, explicit
+diffusion, q and the face coefficients) and hoc_stepper_closure_adjoint (the
+TKE closure with integral lengths). DiffStep.reverse drives them
+(fused=True, the default with kernels="hip").
+
+DiffStep also restates the step in torch operations, in the oracle's
+operator order; DiffStep.reverse(..., fused=False) is the same reverse
+written with transposed torch operators (the reference the fused kernels
+are checked against). Six operators of the restatement sit behind
+torch.autograd Functions with hand-written adjoints, for torch.autograd.grad
+through DiffStep.step:
+  equation of state      hoc_eos_buoyancy / hoc_eos_buoyancy_adjoint (dual
+                         numbers through the same expression);
+  mixing lengths         hoc_gaspar_lengths / hoc_gaspar_lengths_adjoint (the
+                         reverse march per interface);
+  momentum advection     hoc_stepper_momentum_advection(_adjoint);
+  Thomas solves          hoc_thomas_batched / hoc_thomas_adjoint;
+  Helmholtz solve        the stepper's PCG on the right-hand side and on the
+                         adjoint one (the masked operator is symmetric) plus the
+                         face coefficient gradients of -lambda^T (dA/dK) x;
+  tracer advection       hoc_stepper_tracer_advection(_adjoint).
+kernels="torch" replaces them by torch operations (the formulas, a
+sequential Thomas sweep, a dense Helmholtz solve), so autograd
+differentiates the whole step: the reference for the hand-written adjoints
+on grids small enough for the dense solve.
+
+Adjoint.gradient: the plain HIP step forward with checkpoints, then the
+windows reversed last to first, each earlier window recomputed with its
+tapes on a second stream while the next is reversed (see _reverse_sweep).
 
 Derivative conventions: NumPy's maximum/minimum keep the first operand on a
 tie, and the branch kept is the one differentiated (nmax, nmin below and in
-the advection kernel); sqrt has derivative 0 at 0 (ssqrt), where the
-reference only ever uses its value through a selected other branch.
+the kernels); sqrt has derivative 0 at 0 (ssqrt), where the reference only
+ever uses its value through a selected other branch.
 
 Scope: theta scheme, z-level, closed or periodic edges, advection up3 or
-up3_tvd, closure tke (integral or recursive mixing length) or none, any EOS.
+up3_tvd, closure tke (integral or recursive mixing length; the recursive
+form's closure reverse by autograd through tke_step) or none, any EOS.
 """
 
 from __future__ import annotations

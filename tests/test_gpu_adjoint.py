@@ -260,7 +260,8 @@ def test_hand_written_adjoint_equals_autograd_and_checkpointing(adj):
     assert _rel(r2.dJ_dkappa, ag.dJ_dkappa) < 1e-10, _rel(r2.dJ_dkappa, ag.dJ_dkappa)
 
 
-@pytest.mark.parametrize("variant", ["basin", "no_closure_drag_remove", "lock_exchange"])
+@pytest.mark.parametrize("variant", ["basin", "no_closure_drag_remove", "lock_exchange",
+                                     "lock_exchange_recursive"])
 def test_fused_reverse_step_matches_transposed_operators(adj, variant):
     """hoc_stepper_step_adjoint and hoc_stepper_closure_adjoint (the fused
     reverse kernels) against the reverse step written with the transposed
@@ -272,12 +273,15 @@ def test_fused_reverse_step_matches_transposed_operators(adj, variant):
     exchange (wind-driven TKE above the floor, so the closure's implicit
     solve is reversed too; its v adjoint is ~1e-9 against ~1 for u, in a
     4-row periodic channel with v ~ 0, and carries cancellation: 1e-9
-    there)."""
+    there); the lock exchange with recursive mixing lengths, whose closure
+    the fused path leaves to autograd through tke_step."""
     from hydrostatic_ocean_core_benchmark_amd.types import State3D
     prob = _basin(24, nz=30)
-    if variant == "lock_exchange":
+    if variant.startswith("lock_exchange"):
         from hydrostatic_ocean_core_benchmark_amd.cases import lock_exchange
         prob = lock_exchange(nx=32, nz=20)
+        if variant.endswith("recursive"):
+            prob.physics = dataclasses.replace(prob.physics, mxl="recursive")
     elif variant != "basin":
         prob = _basin(24, nz=30, advection="up3", closure="none")
         prob.physics = dataclasses.replace(
@@ -305,7 +309,7 @@ def test_fused_reverse_step_matches_transposed_operators(adj, variant):
                                                            device="cuda") * float(t.abs().mean()))
     f_out, f_kb = ds.reverse(c, kb, y, tape, dict(seeds), fused=True)
     o_out, o_kb = ds.reverse(c, kb, y, tape, dict(seeds), fused=False)
-    tol = 1e-9 if variant == "lock_exchange" else 1e-11
+    tol = 1e-9 if variant.startswith("lock_exchange") else 1e-11
 
     def close(a, b, name):
         if float(b.abs().max()) == 0.0:   # e.g. K_m of a clipped closure: exactly zero

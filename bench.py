@@ -530,6 +530,23 @@ def main() -> int:
         roofline2 = roofline_of(args.kernel2, *st.profile_read())
         st.profile_kernel(None)
     n_solves = prob.params.n_picard if prob.params.name == "theta" else 0
+    # the Helmholtz solves' share of the step: HIP events around each solve
+    # (its span on the device timeline, host polls included) over 10 steps
+    solver_share = None
+    if n_solves:
+        st.profile_kernel("helmholtz_solve")
+        i1 = st.solver_iterations
+        for _ in range(10):
+            one_step()
+        torch.cuda.synchronize()
+        n_sv, sv_ms = st.profile_read()
+        i2 = st.solver_iterations
+        st.profile_kernel(None)
+        if n_sv > 0:
+            per_step = sv_ms / 10
+            solver_share = {"ms_per_step": round(per_step, 4),
+                            "share_of_step": round(per_step / (1e3 * wall / args.steps), 3),
+                            "solves_per_step": n_sv / 10, "iterations_per_step": (i2 - i1) / 10}
     solve_bytes = g.nx * g.ny * (SOLVE_BYTES_PER_COLUMN * n_solves
                                  + ITER_BYTES_PER_COLUMN[prob.params.solver] * iters / args.steps)
     step_bytes = STEP_BYTES_PER_CELL * cells + solve_bytes
@@ -557,6 +574,7 @@ def main() -> int:
                    "solver_failures": fails, "state_finite": finite},
         "roofline": roofline,
         "step_model": step_model,
+        "helmholtz_solves": solver_share,
     }
     if roofline2 is not None:
         rec[f"roofline_{args.kernel2.removeprefix('k_')}"] = roofline2

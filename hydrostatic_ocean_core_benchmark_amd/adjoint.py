@@ -650,7 +650,7 @@ class DiffStep:
             # takes 10-11 on the 256^2 x 30 basin: any convergent
             # preconditioner gives the adjoint system's solution to rtol)
             rparams = params
-            if params.solver == "pcg_mg":
+            if params.solver == "pcg_mg" and 1 <= getattr(params, "mg_sweeps", 2) <= 3:
                 rparams = dataclasses.replace(params, mg_smoother="rbgs")
             self.owner = HipStepper(domain, physics, rparams, dt, device=device)
         self.helm = _HelmholtzOperator(self.owner, self.mask, coef, g.dx, g.dy)

@@ -2110,7 +2110,10 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       k_rhs_eta<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->div0, h->Ug, h->Vg, h->rhs);
       LAUNCH_CHECK("k_rhs_eta");
       int iters = 0, conv = 0;
-      TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
+      {  // the solve's span on the device timeline, host polls included
+        ProfScope _ps(h, "helmholtz_solve", s);
+        TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
+      }
       if (tape)
         HIP_TRY(hipMemcpyAsync(tape->eta[it], out->eta, d.n2 * sizeof(double),
                                hipMemcpyDeviceToDevice, s));

@@ -1366,6 +1366,7 @@ class Adjoint:
         # within its budget; otherwise the gradient is redone polled)
         self.sync_free_recompute = True
         self.adjoint_budget = 0
+        self._work_stream = None
         self.ds = DiffStep(domain, physics, params, dt, device=device, kernels=kernels)
 
     def _hip(self, kb_np):
@@ -1490,6 +1491,22 @@ class Adjoint:
                                ds.mask))
 
     def gradient(self, T0, S0, kb, n_steps=60, every=10) -> GradientResult:
+        """The gradient on a stream of its own (the library replays its
+        sync-free solves as HIP graphs, which the legacy default stream
+        cannot capture); the current stream waits for nothing else."""
+        cur = torch.cuda.current_stream(self.device)
+        if cur.cuda_stream != 0:
+            return self._gradient(T0, S0, kb, n_steps, every)
+        if self._work_stream is None:
+            self._work_stream = torch.cuda.Stream(device=self.device)
+        work = self._work_stream
+        work.wait_stream(cur)
+        with torch.cuda.stream(work):
+            r = self._gradient(T0, S0, kb, n_steps, every)
+        cur.wait_stream(work)
+        return r
+
+    def _gradient(self, T0, S0, kb, n_steps=60, every=10) -> GradientResult:
         ds = self.ds
         T0 = T0.detach().clone()
         kb = kb.detach().clone()
@@ -1572,7 +1589,7 @@ class Adjoint:
                 # a solve outran its budget: the gradient again with polled solves
                 self.sync_free_recompute = False
                 try:
-                    r = self.gradient(T0, S0, kb, n_steps=n_steps, every=every)
+                    r = self._gradient(T0, S0, kb, n_steps=n_steps, every=every)
                 finally:
                     self.sync_free_recompute = True
                 r.redone_polled = True

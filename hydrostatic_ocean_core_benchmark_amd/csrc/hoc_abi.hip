@@ -162,6 +162,7 @@ struct hoc_stepper {
   double *rv_lamT, *rv_lamS, *rv_cs, *rv_gui, *rv_gvi, *rv_Lu, *rv_Lv, *rv_lu, *rv_lv, *rv_gq;
   double *rv_gw, *rv_gw0;
   double *rv_X, *rv_Y, *rv_ge, *rv_leta, *rv_GU, *rv_GV, *rv_gdiv, *rv_inv, *rv_gKu, *rv_gKv;
+  double *rv_Ku, *rv_Kv;  // the tape's K_u, K_v at fixed addresses (graph-replayed solves)
   // closure-reverse workspace (the first hoc_stepper_closure_adjoint allocates it)
   bool cr_ready = false;
   double *cr_n2, *cr_lu, *cr_ld, *cr_x, *cr_cs, *cr_lam, *cr_gl, *cr_n2b, *cr_guc, *cr_gvc;
@@ -180,6 +181,13 @@ struct hoc_stepper {
   Reducer red;
   int last_iters = 8;
   int sync_free_budget = 0;
+  // sync-free solves replayed as one HIP graph (captured on first use, again
+  // when any operand pointer, the budget or the leg tile shape changes)
+  hipGraph_t sf_graph = nullptr;
+  hipGraphExec_t sf_exec = nullptr;
+  const void* sf_key[5] = {};
+  int sf_key_budget = 0, sf_key_tile = -1;
+  double* sf_x = nullptr;  // fixed solution buffer of the step's graph-replayed solves
   int mg_tile_shape = 0;     // fused V-cycle leg tile shape (0..MG_TILE_SHAPES-1)
                              // default by size: 16x16 up to 512x512 columns
                              // (5.50 vs 5.66 ms/step at 512x512x40), else 32x16
@@ -221,6 +229,8 @@ struct hoc_stepper {
     return HOC_OK;
   }
   ~hoc_stepper() {
+    if (sf_exec) (void)hipGraphExecDestroy(sf_exec);
+    if (sf_graph) (void)hipGraphDestroy(sf_graph);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void* v : allocs) (void)hipFree(v);
     if (host_flag) (void)hipHostFree(host_flag);
@@ -844,6 +854,48 @@ int precondition(hoc_stepper* h, const double* r, double* z, hipStream_t s) {
   return HOC_OK;
 }
 
+// The sync-free PCG of pcg_mg / pcg_rbgs: a fixed launch sequence with no
+// host read-back (capturable in a HIP graph). Every phase returns at once
+// after the device `done` flag is set, and a solve that has not converged
+// within the budget counts as a failure.
+int pcg_sync_free(hoc_stepper* h, const PcgArgs& a, unsigned np, hipStream_t s, int* iters_out,
+                  int* conv_out) {
+  const int max_iter = h->p.max_iter;
+  k_pcg_init<<<np, PCG_WG, 0, s>>>(a);   // r = rhs * mask, x = 0, ||rhs||^2 parts
+  k_mg_start<<<1, PCG_WG, 0, s>>>(a);
+  TRY(precondition(h, h->r, h->z, s));
+  k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 1);
+  const int n_it = std::min(h->sync_free_budget, max_iter + 1);
+  for (int it = 1; it <= n_it; ++it) {
+    if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, it == 1 ? 1 : 2);
+    k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it);
+    if (it <= max_iter) {
+      if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, 0);
+      k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 0);
+      TRY(precondition(h, h->r, h->z, s));
+      k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 0);
+    }
+  }
+  k_pcg_budget_check<<<1, 64, 0, s>>>(a, n_it);
+  LAUNCH_CHECK("pcg iteration (sync-free)");
+  if (iters_out) *iters_out = -1;
+  if (conv_out) *conv_out = -1;
+  return HOC_OK;
+}
+
+// Whether a sync-free solve on stream s may be replayed from the handle's
+// graph: not on the legacy default stream, not while the caller captures
+// the stream itself (its own graph then holds the launches), not while a
+// kernel is being profiled by name.
+int sf_graph_usable(hoc_stepper* h, hipStream_t s, bool* ok) {
+  *ok = false;
+  if (s == nullptr || !h->prof_name.empty()) return HOC_OK;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(s, &st));
+  *ok = st == hipStreamCaptureStatusNone;
+  return HOC_OK;
+}
+
 // PCG-Jacobi on A = I - coef D[K G .]; x is the output (device). Path 0: one
 // cooperative launch; path 1: one launch per phase, the host polling the
 // device `done` flag between batches of iterations.
@@ -865,6 +917,43 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
       h->mg_gen = own ? h->coeff_gen : -1;
     }
     const int max_iter = h->p.max_iter;
+    if (h->sync_free_budget > 0 && kind != SOLVER_RBGS && kind != SOLVER_MULTIGRID) {
+      bool replay = false;
+      TRY(sf_graph_usable(h, s, &replay));
+      if (!replay) return pcg_sync_free(h, a, np, s, iters_out, conv_out);
+      const void* key[5] = {Ku, Kv, inv_diag, rhs, x};
+      bool same = h->sf_exec != nullptr && h->sf_key_budget == h->sync_free_budget &&
+                  h->sf_key_tile == h->mg_tile_shape;
+      for (int q = 0; q < 5 && same; ++q) same = h->sf_key[q] == key[q];
+      if (!same) {
+        if (h->sf_exec) (void)hipGraphExecDestroy(h->sf_exec);
+        if (h->sf_graph) (void)hipGraphDestroy(h->sf_graph);
+        h->sf_exec = nullptr;
+        h->sf_graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        const int rc = pcg_sync_free(h, a, np, s, iters_out, conv_out);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != HOC_OK || e != hipSuccess || !g) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc != HOC_OK ? rc : fail(HOC_ERR_DEVICE, "capture of the sync-free solve failed");
+        }
+        const hipError_t ie = hipGraphInstantiate(&h->sf_exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          h->sf_exec = nullptr;
+          return fail(HOC_ERR_DEVICE, "instantiation of the sync-free solve failed");
+        }
+        h->sf_graph = g;
+        for (int q = 0; q < 5; ++q) h->sf_key[q] = key[q];
+        h->sf_key_budget = h->sync_free_budget;
+        h->sf_key_tile = h->mg_tile_shape;
+      }
+      HIP_TRY(hipGraphLaunch(h->sf_exec, s));
+      if (iters_out) *iters_out = -1;
+      if (conv_out) *conv_out = -1;
+      return HOC_OK;
+    }
     k_pcg_init<<<np, PCG_WG, 0, s>>>(a);   // r = rhs * mask, x = 0, ||rhs||^2 parts
     k_mg_start<<<1, PCG_WG, 0, s>>>(a);
     LAUNCH_CHECK("solver init");
@@ -893,28 +982,6 @@ int helmholtz_solve(hoc_stepper* h, const double* Ku, const double* Kv, const do
     // PCG with the RBGS or multigrid preconditioner
     TRY(precondition(h, h->r, h->z, s));
     k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 1);
-    if (h->sync_free_budget > 0) {
-      // a fixed launch sequence with no host read-back (capturable in a HIP
-      // graph): every phase returns at once after the device `done` flag is
-      // set, and a solve that has not converged within the budget counts
-      // as a failure
-      const int n_it = std::min(h->sync_free_budget, max_iter + 1);
-      for (int it = 1; it <= n_it; ++it) {
-        if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, it == 1 ? 1 : 2);
-        k_pcg_ap<<<np, PCG_WG, 0, s>>>(a, it);
-        if (it <= max_iter) {
-          if (a.central) k_pcg_totals<<<1, PCG_WG, 0, s>>>(a, 0);
-          k_pcg_update<<<np, PCG_WG, 0, s>>>(a, it, 0);
-          TRY(precondition(h, h->r, h->z, s));
-          k_pcg_rz<<<np, PCG_WG, 0, s>>>(a, 0);
-        }
-      }
-      k_pcg_budget_check<<<1, 64, 0, s>>>(a, n_it);
-      LAUNCH_CHECK("pcg iteration (sync-free)");
-      if (iters_out) *iters_out = -1;
-      if (conv_out) *conv_out = -1;
-      return HOC_OK;
-    }
     int it = 1;
     int batch = std::max(1, std::min(h->last_iters + 1, max_iter + 1));
     while (true) {
@@ -2110,10 +2177,21 @@ static int step_impl(hoc_stepper* h, const hoc_state* in, hoc_state* out, void* 
       k_rhs_eta<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, in->eta, h->div0, h->Ug, h->Vg, h->rhs);
       LAUNCH_CHECK("k_rhs_eta");
       int iters = 0, conv = 0;
+      // a graph-replayed sync-free solve writes a fixed buffer (the graph's
+      // operands stay the same from step to step), copied to the state
+      double* xs = out->eta;
+      bool replay = false;
+      if (h->sync_free_budget > 0) TRY(sf_graph_usable(h, s, &replay));
+      if (replay) {
+        if (!h->sf_x) TRY(h->alloc(&h->sf_x, d.n2));
+        xs = h->sf_x;
+      }
       {  // the solve's span on the device timeline, host polls included
         ProfScope _ps(h, "helmholtz_solve", s);
-        TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, out->eta, s, &iters, &conv));
+        TRY(helmholtz_solve(h, h->Ku, h->Kv, h->inv_diag, h->rhs, xs, s, &iters, &conv));
       }
+      if (xs != out->eta)
+        HIP_TRY(hipMemcpyAsync(out->eta, xs, d.n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
       if (tape)
         HIP_TRY(hipMemcpyAsync(tape->eta[it], out->eta, d.n2 * sizeof(double),
                                hipMemcpyDeviceToDevice, s));
@@ -2214,7 +2292,7 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
     TRY(h->alloc(&h->rv_gw, d.ni));
     TRY(h->alloc(&h->rv_gw0, d.ni));
     for (double** b : {&h->rv_X, &h->rv_Y, &h->rv_ge, &h->rv_leta, &h->rv_GU, &h->rv_GV,
-                       &h->rv_gdiv, &h->rv_inv, &h->rv_gKu, &h->rv_gKv})
+                       &h->rv_gdiv, &h->rv_inv, &h->rv_gKu, &h->rv_gKv, &h->rv_Ku, &h->rv_Kv})
       TRY(h->alloc(b, d.n2));
     h->rv_ready = true;
   }
@@ -2238,7 +2316,9 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
                                                     gx->S, h->rv_gui, h->rv_gvi, h->rv_gw);
   LAUNCH_CHECK("reverse tracers");
   // Picard passes in reverse (model3d_v05.py:387-408)
-  HelmOp op{tape->Ku, tape->Kv, h->m2, p.helm_coef, p.helm_cx, p.helm_cy, p.dx2, p.dy2};
+  HIP_TRY(hipMemcpyAsync(h->rv_Ku, tape->Ku, d.n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemcpyAsync(h->rv_Kv, tape->Kv, d.n2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HelmOp op{h->rv_Ku, h->rv_Kv, h->m2, p.helm_coef, p.helm_cx, p.helm_cy, p.dx2, p.dy2};
   k_helm_diag<<<blocks(d.n2), TPB, 0, s>>>(d, op, h->rv_inv);
   const int last = cfg.n_picard - 1;
   for (int it = last; it >= 0; --it) {
@@ -2250,7 +2330,7 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
                                                h->rv_X, h->rv_Y, h->rv_ge);
     LAUNCH_CHECK("reverse back-substitution");
     int iters = 0, conv = 0;
-    TRY(helmholtz_solve(h, tape->Ku, tape->Kv, h->rv_inv, h->rv_ge, h->rv_leta, s, &iters, &conv));
+    TRY(helmholtz_solve(h, h->rv_Ku, h->rv_Kv, h->rv_inv, h->rv_ge, h->rv_leta, s, &iters, &conv));
     k_rev_eta_post<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, tape->eta[it], h->rv_leta, first,
                                                 h->rv_gKu, h->rv_gKv, gx->eta, h->rv_gdiv,
                                                 h->rv_GU, h->rv_GV);

@@ -1367,6 +1367,7 @@ class Adjoint:
         self.sync_free_recompute = True
         self.adjoint_budget = 0
         self._work_stream = None
+        self._budget_override = 0   # tests: a recompute budget forced from the host
         self.ds = DiffStep(domain, physics, params, dt, device=device, kernels=kernels)
 
     def _hip(self, kb_np):
@@ -1576,6 +1577,8 @@ class Adjoint:
         if self.reverse == "hand":
             if sync_free:
                 budget = math.ceil(per_solve) + 2
+                if self._budget_override:
+                    budget = self._budget_override
             own = ds.owner
             fa0 = own.solver_failures
             _lib.check(st.lib.hoc_stepper_set_sync_free(st._h, budget))

@@ -325,6 +325,25 @@ def test_fused_reverse_step_matches_transposed_operators(adj, variant):
     close(f_kb, o_kb, "kappa_b")
 
 
+def test_sync_free_budget_overrun_redoes_the_gradient_polled(adj):
+    """A recompute solve that outruns its sync-free budget (forced from the
+    host: one iteration, so every such solve counts as unconverged while
+    every kernel exits normally) makes the gradient run again with polled
+    solves: flagged, and equal to the gradient of the regular run."""
+    prob = _basin(24, nz=30)
+    kb = _kappa(prob)
+    T0 = torch.as_tensor(prob.initial_T, device="cuda")
+    S0 = torch.as_tensor(prob.initial_S, device="cuda")
+    a = adj.Adjoint(prob.domain, prob.physics, prob.params, prob.dt, kernels="hip")
+    ref = a.gradient(T0, S0, kb, n_steps=4, every=2)
+    assert not ref.redone_polled and ref.solve_budget > 1
+    a._budget_override = 1
+    r = a.gradient(T0, S0, kb, n_steps=4, every=2)
+    assert r.redone_polled
+    assert _rel(r.dJ_dT0, ref.dJ_dT0) < 1e-12, _rel(r.dJ_dT0, ref.dJ_dT0)
+    assert _rel(r.dJ_dkappa, ref.dJ_dkappa) < 1e-12
+
+
 def test_hand_reverse_other_branches_equal_autograd(adj):
     """The other branches of the hand-written reverse step (no closure,
     unlimited up3, pgf remove, bottom drag, no A_h) against autograd through

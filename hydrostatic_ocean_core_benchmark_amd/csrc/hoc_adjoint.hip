@@ -717,10 +717,18 @@ __device__ void rev_column_solve(const Dims& d, const Geo& geo, long long c,
       lam[q][t] = ln[q];
     }
   }
+  // interface k takes row k-1's a_bot term, then row k's a_top term; each
+  // interface is read and written once (the a_bot term carried over)
+  bool carry = false;
+  double bot_term = 0.0;
   for (int k = 0; k < nz; ++k) {
     const long long t = (long long)k * n2 + c;
     const DiffRow rk = diff_row(d, geo, c, k, nu, dz, thv_dt, thv_dt_drag);
-    if (!rk.wet) continue;
+    if (!rk.wet) {
+      if (carry) gnu[t] = gnu[t] + bot_term;
+      carry = false;
+      continue;
+    }
     double gd = 0.0, gs = 0.0, gp = 0.0;
     for (int q = 0; q < NR; ++q) {
       const double l = lam[q][t];
@@ -730,8 +738,15 @@ __device__ void rev_column_solve(const Dims& d, const Geo& geo, long long c,
     }
     const double ga_top = k >= 1 ? gd - gs : (rk.bot ? gd : 0.0);
     const double ga_bot = rk.bot ? 0.0 : gd - gp;
-    gnu[t] += rk.ft * ga_top;
-    gnu[t + n2] += rk.fb * ga_bot;
+    double v = gnu[t];
+    if (carry) v = v + bot_term;
+    gnu[t] = v + rk.ft * ga_top;
+    carry = true;
+    bot_term = rk.fb * ga_bot;
+  }
+  if (carry) {
+    const long long t = (long long)nz * n2 + c;
+    gnu[t] = gnu[t] + bot_term;
   }
 }
 
@@ -1391,13 +1406,6 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
     sgl[t] = p.c_k * rc_ssqrt(e_new) * w * gK;
     const double ge_new = (ge1 ? ge1[t] : 0.0) + p.c_k * l_k * w * gK * rc_dssqrt(e_new);
     slam[t] = (k >= 1 && w > 0.0 && (x >= p.e_min || x != x)) ? ge_new : 0.0;
-    ge0[t] = 0.0;
-    gKm0[t] = 0.0;
-    gKh0[t] = 0.0;
-  }
-  for (int k = 0; k < nz; ++k) {
-    guc[T(k)] = 0.0;
-    gvc[T(k)] = 0.0;
   }
   {  // lambda = A^-T g: sub_T[k] = sup[k-1], sup_T[k] = sub[k+1]
     double cp = 0.0, dp = 0.0;
@@ -1419,7 +1427,12 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
     }
   }
   // per interface: the row's band and right-hand-side adjoints, then the
-  // lengths and their reverse marches
+  // lengths and their reverse marches. The adjoints of K_m (interfaces
+  // k-1, k, k+1) and of the centred velocities (layers k-1, k) are carried
+  // in registers and each address is written once, the terms added in the
+  // same order as a running sum per address would add them.
+  double km_prev = 0.0, km_cur = 0.0;  // K_m adjoint of interfaces k-1 and k so far
+  double uc_prev = 0.0, vc_prev = 0.0;  // centred-velocity adjoints of layer k-1 so far
   for (int k = 0; k <= nz; ++k) {
     const long long t = T(k);
     const double w = C.wet(k);
@@ -1427,6 +1440,7 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
     double gl_eps = 0.0;
     const double ek = e[t];
     double gek = 0.0;
+    double km_next = 0.0, uc_cur = 0.0, vc_cur = 0.0, gkh = 0.0;
     if (w > 0.0) {
       const double lam = slam[t];
       const double gd = -lam * sx[t], gs = -lam * sx[t - n2], gp = -lam * sx[t + n2];
@@ -1434,24 +1448,24 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       const double idzi = safe_inv(C.dzi(k));
       const double glu = p.dt * safe_inv(geo.dz3[t - n2]) * idzi * ga_up;
       const double gld = p.dt * safe_inv(geo.dz3[t]) * idzi * ga_dn;
-      gKm0[t - n2] += 0.5 * glu;
-      gKm0[t] += 0.5 * glu + 0.5 * gld;
-      gKm0[t + n2] += 0.5 * gld;
+      km_prev += 0.5 * glu;
+      km_cur += 0.5 * glu + 0.5 * gld;
+      km_next += 0.5 * gld;
       gek += lam;
       const double gprod = p.dt * lam;
       const double n2k = sn2[t];
       const double du = sdu[t], dv = sdv[t];
       const double sh2 = (du * du + dv * dv) * w;
-      gKm0[t] += sh2 * gprod;
+      km_cur += sh2 * gprod;
       const double gsh = (Km0[t] - p.nu_b) * gprod;
-      gKh0[t] += -n2k * gprod;
+      gkh += -n2k * gprod;
       gkb_acc += n2k * gprod;
       sn2b[t] += -(Kh0[t] - kbc) * gprod;
       const double gdu = 2.0 * du * w * gsh * idzi, gdv = 2.0 * dv * w * gsh * idzi;
-      guc[t - n2] += gdu;
-      guc[t] -= gdu;
-      gvc[t - n2] += gdv;
-      gvc[t] -= gdv;
+      uc_prev += gdu;
+      uc_cur -= gdu;
+      vc_prev += gdv;
+      vc_cur -= gdv;
       // dissipation c_eps sqrt(max(e, e_min)) / l_eps
       const double E = nmax(ek, p.e_min);
       const double l_eps = nmax(p.l_min, nmin(lu, ld));
@@ -1511,8 +1525,19 @@ __global__ void k_rev_closure(Dims d, StepParams p, Geo geo, const double* __res
       gb0_acc += bb;
     }
     if (ek >= p.e_min || ek != ek) gek += gb0_acc;
-    ge0[t] += gek;
+    ge0[t] = 0.0 + gek;
+    gKh0[t] = gkh;
+    if (k >= 1) {  // interface k-1 and layer k-1 have all their terms
+      gKm0[t - n2] = km_prev;
+      guc[t - n2] = uc_prev;
+      gvc[t - n2] = vc_prev;
+    }
+    km_prev = km_cur;
+    km_cur = km_next;
+    uc_prev = uc_cur;
+    vc_prev = vc_cur;
   }
+  gKm0[T(nz)] = km_prev;
   gkb[c] += gkb_acc;
   // N^2 = (b[k-1] - b[k]) / dzi wet back to b
   for (int k = 1; k < nz; ++k) {

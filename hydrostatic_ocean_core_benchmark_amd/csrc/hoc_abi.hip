@@ -170,7 +170,7 @@ struct hoc_stepper {
   int* cr_hi;
   // Helmholtz
   double *r, *z, *pp, *pp2, *Ap, *part_a, *part_b, *part_c;
-  unsigned* bar = nullptr;      // persistent-solver grid barrier [2]
+  unsigned* bar = nullptr;      // persistent-solver grid barrier [GRID_BAR_WORDS]
   int* bar_err = nullptr;
   int pcg_sub = 1;        // subtrees per full 8192-element chunk
   int pcg_parts_all = 0;  // workgroups of one PCG phase
@@ -651,7 +651,7 @@ int pcg_finish(hoc_stepper* h, hipStream_t s, bool persistent, int* iters_out, i
   }
   if (persistent && h->host_flag[4]) {
     (void)hipMemset(h->bar_err, 0, sizeof(int));
-    (void)hipMemset(h->bar, 0, 2 * sizeof(unsigned));
+    (void)hipMemset(h->bar, 0, GRID_BAR_WORDS * sizeof(unsigned));
     return fail(HOC_ERR_DEVICE, "persistent Helmholtz solve: grid barrier timed out");
   }
   // PcgScalars ends with it, iters, done, converged
@@ -1746,7 +1746,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
   A(h->pp, d.n2);
   A(h->pp2, d.n2);
   A(h->Ap, d.n2);
-  A(h->bar, 2);
+  A(h->bar, GRID_BAR_WORDS);
   A(h->bar_err, 1);
   A(h->sc, 1);
   A(h->dcount, 4);
@@ -1922,7 +1922,7 @@ int hoc_stepper_create(const hoc_config* cfg, const hoc_geometry* g, hoc_stepper
     rc = rc ? rc : upload(h->dz3_ref, g->dz3, d.n3, "dz3");
   }
   if (rc != HOC_OK) return cleanup(rc);
-  if (hipMemset(h->bar, 0, 2 * sizeof(unsigned)) != hipSuccess ||
+  if (hipMemset(h->bar, 0, GRID_BAR_WORDS * sizeof(unsigned)) != hipSuccess ||
       hipMemset(h->bar_err, 0, sizeof(int)) != hipSuccess)
     return cleanup(fail(HOC_ERR_DEVICE, "hipMemset failed"));
   if (hipMemset(h->dcount, 0, 4 * sizeof(long long)) != hipSuccess)

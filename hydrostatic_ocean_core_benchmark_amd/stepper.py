@@ -223,10 +223,12 @@ class HipStepper:
     def __exit__(self, *exc) -> None:
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter shutdown the exit hook has already closed live
-        # handles; never call into the runtime from the finaliser then
-        if not sys.is_finalizing():
+        # handles; never call into the runtime from the finaliser then.
+        # is_finalizing is bound at class creation because module teardown
+        # sets this module's globals (sys among them) to None first.
+        if not _finalizing():
             self.close()
 
     def _stream(self) -> C.c_void_p:

@@ -108,3 +108,17 @@ def test_stepper_precision_arguments_are_checked_before_any_device_use():
     params.mg_sweeps = 4
     with pytest.raises(ValueError, match="rbgs pcg_mg smoother"):
         HipStepper(dom, phys, params, dt)
+
+
+@pytest.mark.parametrize("module,cls", [("stepper", "HipStepper"), ("model2d", "Helmholtz2D"),
+                                        ("model2d", "SWStepper"), ("model3d_v02", "Stepper3D")])
+def test_finaliser_survives_module_teardown(monkeypatch, module, cls):
+    """At interpreter exit, module teardown sets a module's globals to None
+    before its last objects are collected; a finaliser that looked up
+    `sys` there raised AttributeError. The finalisers bind
+    sys.is_finalizing when the class is created."""
+    import importlib
+    mod = importlib.import_module(f"hydrostatic_ocean_core_benchmark_amd.{module}")
+    obj = object.__new__(getattr(mod, cls))   # no library handle
+    monkeypatch.setattr(mod, "sys", None)
+    obj.__del__()

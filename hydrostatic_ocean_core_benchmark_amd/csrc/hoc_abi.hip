@@ -2311,7 +2311,8 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
   k_rev_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->T, y->S, gy->T, gy->S,
                                              gy->b, Kh, h->rv_lamT, h->rv_lamS, h->rv_cs, gx->T,
                                              gx->S, g_Kh);
-  k_rev_tracer_stencil<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->u, y->v, tape->w,
+  k_rev_tracer_stencil<<<dim3((d.nx + RTS_TX - 1) / RTS_TX, (d.ny + RTS_TY - 1) / RTS_TY, d.nz),
+                         RTS_TX * RTS_TY, 0, s>>>(d, p, geo, x->T, x->S, y->u, y->v, tape->w,
                                                     h->rv_lamT, h->rv_lamS, gy->u, gy->v, gx->T,
                                                     gx->S, h->rv_gui, h->rv_gvi, h->rv_gw);
   LAUNCH_CHECK("reverse tracers");

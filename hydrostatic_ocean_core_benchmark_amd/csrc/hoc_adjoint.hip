@@ -849,29 +849,151 @@ __device__ __forceinline__ double lap_T_point(const Dims& d, const StepParams& p
   return (mk(mu[w]) * Uw - mk(mu[t]) * Uc) / p.dx + (mk(mv[s]) * Vs - mk(mv[t]) * Vc) / p.dy;
 }
 
-// Tracer advection (gathered, hand-written reverse of adv) and K_h
-// Laplacian reversed at one point for T and S: adds to gT0, gS0; the new
-// velocity's adjoint gu1 = gu_y + the flux-velocity parts (gv1 likewise);
-// gw1 the adjoint of the new w (both tracers).
-__global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double* __restrict__ T0,
-                                     const double* __restrict__ S0,
-                                     const double* __restrict__ u1,
-                                     const double* __restrict__ v1,
-                                     const double* __restrict__ w1,
-                                     const double* __restrict__ lamT,
-                                     const double* __restrict__ lamS,
-                                     const double* __restrict__ gu_y,
-                                     const double* __restrict__ gv_y, double* __restrict__ gT0,
-                                     double* __restrict__ gS0, double* __restrict__ gu1,
-                                     double* __restrict__ gv1, double* __restrict__ gw1) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.n3) return;
+// Tracer advection reversed on a RTS_TX x RTS_TY tile of one level: the
+// same sums as tracer_adv_adjoint_point, with every face evaluated once.
+// The tile's tracer and mask (3-cell halo) and Ghat = G mask / dz (2-cell
+// halo) are staged in LDS; each x- and y-face the tile's cells read has its
+// four products (fbar dz3 vel) dp[m] and its velocity adjoint fbar dz3 face
+// formed once into LDS; each cell then adds its faces' products in the
+// order of the gathered form (x faces s = -2..1, y faces, the vertical
+// interfaces, the surface), so the result is that form's bit for bit
+// (every product is rounded on its own). The vertical part stays per cell.
+constexpr int RTS_CW = RTS_TX + 6, RTS_CH = RTS_TY + 6;  // tracer and mask, halo 3
+constexpr int RTS_GW = RTS_TX + 4, RTS_GH = RTS_TY + 4;  // Ghat, halo 2
+constexpr int RTS_XF = RTS_TX + 3, RTS_YF = RTS_TY + 3;  // faces i0-2 .. i0+TX (j likewise)
+struct RtsLds {
+  double C[RTS_CH][RTS_CW];
+  double gh[RTS_GH][RTS_GW];
+  double X[RTS_TY][RTS_XF][5];
+  double Y[RTS_YF][RTS_TX][5];
+  uint8_t M[RTS_CH][RTS_CW];
+};
+
+template <class GA>
+__device__ __forceinline__ void tracer_adv_adjoint_tile(const Dims& d, const StepParams& p,
+                                                        const Geo& geo,
+                                                        const double* __restrict__ cc,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ v,
+                                                        const double* __restrict__ w, GA G,
+                                                        RtsLds& L, double& gc_out,
+                                                        double& gu_out, double& gv_out,
+                                                        double& gw_out) {
+  const long long n2 = d.n2;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RTS_TX, j0 = blockIdx.y * RTS_TY, k = blockIdx.z;
+  const long long o = (long long)k * n2;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  auto col = [&](int jj, int ii) -> long long { return (long long)pwrap(jj, ny) * nx + pwrap(ii, nx); };
+  auto ghat = [&](long long q) -> double { return G(q) * mk(geo.m3[q]) * safe_inv(geo.dz3[q]); };
+  for (int q = tid; q < RTS_CH * RTS_CW; q += RTS_TX * RTS_TY) {
+    const int qy = q / RTS_CW, qx = q % RTS_CW;
+    const long long cell = o + col(j0 - 3 + qy, i0 - 3 + qx);
+    L.C[qy][qx] = cc[cell];
+    L.M[qy][qx] = geo.m3[cell];
+  }
+  for (int q = tid; q < RTS_GH * RTS_GW; q += RTS_TX * RTS_TY) {
+    const int qy = q / RTS_GW, qx = q % RTS_GW;
+    L.gh[qy][qx] = ghat(o + col(j0 - 2 + qy, i0 - 2 + qx));
+  }
+  __syncthreads();
+  // x-faces: face fi is the east face of column i0 - 2 + fi in row j0 + ty
+  for (int q = tid; q < RTS_TY * RTS_XF; q += RTS_TX * RTS_TY) {
+    const int ty = q / RTS_XF, fi = q % RTS_XF, cy = ty + 3;
+    const long long f = o + col(j0 + ty, i0 - 2 + fi);
+    const double vel = u[f];
+    double dp[4];
+    const double face = kappa_face_d(L.C[cy][fi], L.C[cy][fi + 1], L.C[cy][fi + 2],
+                                     L.C[cy][fi + 3], vel, L.M[cy][fi] != 0,
+                                     L.M[cy][fi + 3] != 0, lim, dp);
+    const double fbar = (L.gh[ty + 2][fi] - L.gh[ty + 2][fi + 1]) / p.dx;
+    const double a = fbar * geo.dz3u[f] * vel;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.X[ty][fi][m] = a * dp[m];
+    L.X[ty][fi][4] = fbar * geo.dz3u[f] * face;
+  }
+  // y-faces: face row fj is the north face of row j0 - 2 + fj in column i0 + tx
+  for (int q = tid; q < RTS_YF * RTS_TX; q += RTS_TX * RTS_TY) {
+    const int fj = q / RTS_TX, tx = q % RTS_TX, cx = tx + 3;
+    const long long f = o + col(j0 - 2 + fj, i0 + tx);
+    const double vel = v[f];
+    double dp[4];
+    const double face = kappa_face_d(L.C[fj][cx], L.C[fj + 1][cx], L.C[fj + 2][cx],
+                                     L.C[fj + 3][cx], vel, L.M[fj][cx] != 0,
+                                     L.M[fj + 3][cx] != 0, lim, dp);
+    const double fbar = (L.gh[fj][tx + 2] - L.gh[fj + 1][tx + 2]) / p.dy;
+    const double a = fbar * geo.dz3v[f] * vel;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.Y[fj][tx][m] = a * dp[m];
+    L.Y[fj][tx][4] = fbar * geo.dz3v[f] * face;
+  }
+  __syncthreads();
+  const int tx = tid % RTS_TX, ty = tid / RTS_TX;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i >= nx || j >= ny) return;
+  const long long c = (long long)j * nx + i;
+  double acc = 0.0;
+#pragma unroll
+  for (int s = -2; s <= 1; ++s) acc += L.X[ty][tx + s + 2][1 - s];
+  gu_out = L.X[ty][tx + 2][4];
+#pragma unroll
+  for (int s = -2; s <= 1; ++s) acc += L.Y[ty + s + 2][tx][1 - s];
+  gv_out = L.Y[ty + 2][tx][4];
+  const double gh_k = L.gh[ty + 2][tx + 2];
+  auto ghat_at = [&](int kk) -> double {
+    return kk == k ? gh_k : ghat((long long)kk * n2 + c);
+  };
+  auto zbar = [&](int kk) -> double {
+    const double below = ghat_at(kk);
+    const double above = kk >= 1 ? ghat_at(kk - 1) : 0.0;
+    return below - above;
+  };
+  for (int kk = k - 1; kk <= k + 2; ++kk) {
+    if (kk < 1 || kk > nz - 1) continue;
+    const long long a = (long long)kk * n2 + c;
+    const double far_up = kk >= 2 ? cc[a - 2 * n2] : 0.0;
+    const double far_dn = kk <= nz - 2 ? cc[a + n2] : 0.0;
+    const bool hu = kk >= 2 && geo.m3[a - 2 * n2] != 0;
+    const bool hd = kk <= nz - 2 && geo.m3[a + n2] != 0;
+    double dp[4];
+    const double face = kappa_face_d(far_dn, cc[a], cc[a - n2], far_up, w[a], hd, hu, lim, dp);
+    const double zb = zbar(kk);
+    const double wb = zb * w[a];
+    acc += wb * dp[kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3];
+    if (kk == k) gw_out = zb * face;
+  }
+  if (k == 0) {
+    acc += gh_k * w[c];
+    gw_out = gh_k * cc[c];
+  }
+  gc_out = acc;
+}
+
+// Tracer advection (hand-written reverse of adv) and K_h Laplacian reversed
+// for T and S on one tile of one level (grid: tiles x tiles x nz, RTS_TX x
+// RTS_TY threads): adds to gT0, gS0; the new velocity's adjoint gu1 = gu_y +
+// the flux-velocity parts (gv1 likewise); gw1 the adjoint of the new w
+// (both tracers).
+__global__ __launch_bounds__(RTS_TX* RTS_TY) void k_rev_tracer_stencil(
+    Dims d, StepParams p, Geo geo, const double* __restrict__ T0, const double* __restrict__ S0,
+    const double* __restrict__ u1, const double* __restrict__ v1, const double* __restrict__ w1,
+    const double* __restrict__ lamT, const double* __restrict__ lamS,
+    const double* __restrict__ gu_y, const double* __restrict__ gv_y, double* __restrict__ gT0,
+    double* __restrict__ gS0, double* __restrict__ gu1, double* __restrict__ gv1,
+    double* __restrict__ gw1) {
+  __shared__ RtsLds L;
   const double ndt = -p.dt;
-  double cT, uT, vT, wT, cS, uS, vS, wS;
-  tracer_adv_adjoint_point(d, p, geo, T0, u1, v1, w1, [&](long long q) { return ndt * lamT[q]; },
-                           t, cT, uT, vT, wT);
-  tracer_adv_adjoint_point(d, p, geo, S0, u1, v1, w1, [&](long long q) { return ndt * lamS[q]; },
-                           t, cS, uS, vS, wS);
+  double cT = 0.0, uT = 0.0, vT = 0.0, wT = 0.0, cS = 0.0, uS = 0.0, vS = 0.0, wS = 0.0;
+  tracer_adv_adjoint_tile(d, p, geo, T0, u1, v1, w1, [&](long long q) { return ndt * lamT[q]; },
+                          L, cT, uT, vT, wT);
+  __syncthreads();
+  tracer_adv_adjoint_tile(d, p, geo, S0, u1, v1, w1, [&](long long q) { return ndt * lamS[q]; },
+                          L, cS, uS, vS, wS);
+  const int i = blockIdx.x * RTS_TX + (int)threadIdx.x % RTS_TX;
+  const int j = blockIdx.y * RTS_TY + (int)threadIdx.x / RTS_TX;
+  if (i >= d.nx || j >= d.ny) return;
+  const long long t = (long long)blockIdx.z * d.n2 + (long long)j * d.nx + i;
   if (p.has_K_h) {
     const double dk = p.dt * p.K_h;
     cT += dk * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3, lamT, t);

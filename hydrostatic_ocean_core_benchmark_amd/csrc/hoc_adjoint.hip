@@ -1262,25 +1262,181 @@ __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* __re
   }
 }
 
-// Momentum advection of the old state reversed at one point with the
-// adjoints -dt m3[u|v] L[u|v] of the advection terms (gathered reverse);
-// adds to gu0, gv0 and writes the adjoint of the old w.
-__global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* __restrict__ u0,
-                                   const double* __restrict__ v0,
-                                   const double* __restrict__ w0,
-                                   const double* __restrict__ Lu,
-                                   const double* __restrict__ Lv, double* __restrict__ gu0,
-                                   double* __restrict__ gv0, double* __restrict__ gw0) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.n3) return;
-  double a = 0.0, b = 0.0, e = 0.0;
-  momentum_adv_adjoint_point(
-      d, p, geo, u0, v0, w0, [&](long long q) { return -(mk(geo.m3u[q]) * (p.dt * Lu[q])); },
-      [&](long long q) { return -(mk(geo.m3v[q]) * (p.dt * Lv[q])); }, t, a, b, e);
-  gu0[t] += a;
-  gv0[t] += b;
-  gw0[t] = e;
-  if (t / d.n2 == d.nz - 1) gw0[(long long)d.nz * d.n2 + t % d.n2] = 0.0;
+// Momentum advection of the old state reversed on a RMA_TX x RMA_TY tile of
+// one level with the adjoints -dt m3[u|v] L[u|v] of the advection terms:
+// the sums of momentum_adv_adjoint_point with every horizontal face
+// evaluated once. u, v, their masks (3-cell halo) and the two adjoints
+// (2-cell halo) are staged in LDS. Phase 1 forms the u equation's faces (A:
+// x fluxes at centres, B: y fluxes at corners) into LDS, phase 2 the v
+// equation's (C: y fluxes at centres, D: x fluxes at corners) in the same
+// space; each face holds its four products (fbar wb) dp[m] and the product
+// the other velocity's gather takes. Each point adds them in the gathered
+// form's order for each accumulator (gU: A, B, then D's corner share; gV:
+// B's corner share, C, D), then the vertical part as before, so the result
+// is that form's bit for bit (every product is rounded on its own).
+constexpr int RMA_SW = RMA_TX + 6, RMA_SH = RMA_TY + 6;  // u, v, masks: halo 3
+constexpr int RMA_GW = RMA_TX + 4, RMA_GH = RMA_TY + 4;  // adjoints: halo 2
+struct RmaLds {
+  double U[RMA_SH][RMA_SW], V[RMA_SH][RMA_SW];
+  double GU[RMA_GH][RMA_GW], GV[RMA_GH][RMA_GW];
+  union {
+    struct {
+      double A[RMA_TY][RMA_TX + 3][5];      // rows j0.., centres i0-1 .. i0+TX+1
+      double B[RMA_TY + 3][RMA_TX + 1][5];  // corner rows j0-2 .., columns i0-1 ..
+    } p1;
+    struct {
+      double C[RMA_TY + 3][RMA_TX][5];      // centre rows j0-1 .., columns i0 ..
+      double D[RMA_TY + 1][RMA_TX + 3][5];  // rows j0-1 .., corner columns i0-2 ..
+    } p2;
+  } f;
+  uint8_t MU[RMA_SH][RMA_SW], MV[RMA_SH][RMA_SW];
+};
+
+__global__ __launch_bounds__(RMA_TX* RMA_TY) void k_rev_momentum_adv(
+    Dims d, StepParams p, Geo geo, const double* __restrict__ u0, const double* __restrict__ v0,
+    const double* __restrict__ w0, const double* __restrict__ Lu, const double* __restrict__ Lv,
+    double* __restrict__ gu0, double* __restrict__ gv0, double* __restrict__ gw0) {
+  __shared__ RmaLds L;
+  const long long n2 = d.n2;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RMA_TX, j0 = blockIdx.y * RMA_TY, k = blockIdx.z;
+  const long long o = (long long)k * n2;
+  const bool lim = p.advection == ADV_UP3_TVD;
+  auto col = [&](int jj, int ii) -> long long { return (long long)pwrap(jj, ny) * nx + pwrap(ii, nx); };
+  auto Gu = [&](long long q) { return -(mk(geo.m3u[q]) * (p.dt * Lu[q])); };
+  auto Gv = [&](long long q) { return -(mk(geo.m3v[q]) * (p.dt * Lv[q])); };
+  constexpr int NT = RMA_TX * RMA_TY;
+  for (int q = tid; q < RMA_SH * RMA_SW; q += NT) {
+    const int qy = q / RMA_SW, qx = q % RMA_SW;
+    const long long cell = o + col(j0 - 3 + qy, i0 - 3 + qx);
+    L.U[qy][qx] = u0[cell];
+    L.V[qy][qx] = v0[cell];
+    L.MU[qy][qx] = geo.m3u[cell];
+    L.MV[qy][qx] = geo.m3v[cell];
+  }
+  for (int q = tid; q < RMA_GH * RMA_GW; q += NT) {
+    const int qy = q / RMA_GW, qx = q % RMA_GW;
+    const long long cell = o + col(j0 - 2 + qy, i0 - 2 + qx);
+    L.GU[qy][qx] = Gu(cell);
+    L.GV[qy][qx] = Gv(cell);
+  }
+  __syncthreads();
+  double dp[4];
+  // ---- phase 1: the u equation's faces
+  for (int q = tid; q < RMA_TY * (RMA_TX + 3); q += NT) {  // A: x flux at centre i0 - 1 + a
+    const int ty = q / (RMA_TX + 3), a = q % (RMA_TX + 3), ry = ty + 3;
+    const double ub = 0.5 * (L.U[ry][a + 1] + L.U[ry][a + 2]);
+    const double uc = kappa_face_d(L.U[ry][a], L.U[ry][a + 1], L.U[ry][a + 2], L.U[ry][a + 3],
+                                   ub, L.MU[ry][a] != 0, L.MU[ry][a + 3] != 0, lim, dp);
+    const double fbar = (L.GU[ty + 2][a] - L.GU[ty + 2][a + 1]) / p.dx;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.f.p1.A[ty][a][m] = fbar * ub * dp[m];
+    L.f.p1.A[ty][a][4] = fbar * 0.5 * uc;
+  }
+  for (int q = tid; q < (RMA_TY + 3) * (RMA_TX + 1); q += NT) {  // B: y flux at corner row
+    const int b = q / (RMA_TX + 1), e = q % (RMA_TX + 1), cx = e + 2;  // j0 - 2 + b, column i0 - 1 + e
+    const double vb = 0.5 * (L.V[b + 1][cx] + L.V[b + 1][cx + 1]);
+    const double uy = kappa_face_d(L.U[b][cx], L.U[b + 1][cx], L.U[b + 2][cx], L.U[b + 3][cx],
+                                   vb, L.MU[b][cx] != 0, L.MU[b + 3][cx] != 0, lim, dp);
+    const double fbar = (L.GU[b][e + 1] - L.GU[b + 1][e + 1]) / p.dy;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.f.p1.B[b][e][m] = fbar * vb * dp[m];
+    L.f.p1.B[b][e][4] = fbar * uy * 0.5;
+  }
+  __syncthreads();
+  const int tx = tid % RMA_TX, ty = tid / RMA_TX;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool inside = i < nx && j < ny;
+  double gU = 0.0, gV = 0.0;
+  if (inside) {
+#pragma unroll
+    for (int s = -1; s <= 2; ++s) {
+      gU += L.f.p1.A[ty][tx + s + 1][2 - s];
+      if (s == 0 || s == 1) gU += L.f.p1.A[ty][tx + s + 1][4];
+    }
+#pragma unroll
+    for (int s = -2; s <= 1; ++s) gU += L.f.p1.B[ty + s + 2][tx + 1][1 - s];
+#pragma unroll
+    for (int s = -1; s <= 0; ++s) gV += L.f.p1.B[ty + 2][tx + 1 + s][4];
+  }
+  __syncthreads();
+  // ---- phase 2: the v equation's faces
+  for (int q = tid; q < (RMA_TY + 3) * RMA_TX; q += NT) {  // C: y flux at centre row j0 - 1 + cr
+    const int cr = q / RMA_TX, cx = q % RMA_TX + 3;
+    const double vb = 0.5 * (L.V[cr + 1][cx] + L.V[cr + 2][cx]);
+    const double vc = kappa_face_d(L.V[cr][cx], L.V[cr + 1][cx], L.V[cr + 2][cx],
+                                   L.V[cr + 3][cx], vb, L.MV[cr][cx] != 0,
+                                   L.MV[cr + 3][cx] != 0, lim, dp);
+    const double fbar = (L.GV[cr][cx - 1] - L.GV[cr + 1][cx - 1]) / p.dy;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.f.p2.C[cr][cx - 3][m] = fbar * vb * dp[m];
+    L.f.p2.C[cr][cx - 3][4] = fbar * 0.5 * vc;
+  }
+  for (int q = tid; q < (RMA_TY + 1) * (RMA_TX + 3); q += NT) {  // D: x flux at corner column
+    const int r = q / (RMA_TX + 3), a = q % (RMA_TX + 3);  // row j0 - 1 + r, column i0 - 2 + a
+    const double ub = 0.5 * (L.U[r + 2][a + 1] + L.U[r + 3][a + 1]);
+    const double vx = kappa_face_d(L.V[r + 2][a], L.V[r + 2][a + 1], L.V[r + 2][a + 2],
+                                   L.V[r + 2][a + 3], ub, L.MV[r + 2][a] != 0,
+                                   L.MV[r + 2][a + 3] != 0, lim, dp);
+    const double fbar = (L.GV[r + 1][a] - L.GV[r + 1][a + 1]) / p.dx;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) L.f.p2.D[r][a][m] = fbar * ub * dp[m];
+    L.f.p2.D[r][a][4] = fbar * vx * 0.5;
+  }
+  __syncthreads();
+  if (!inside) return;
+#pragma unroll
+  for (int s = -1; s <= 2; ++s) {
+    gV += L.f.p2.C[ty + s + 1][tx][2 - s];
+    if (s == 0 || s == 1) gV += L.f.p2.C[ty + s + 1][tx][4];
+  }
+#pragma unroll
+  for (int s = -2; s <= 1; ++s) gV += L.f.p2.D[ty + 1][tx + s + 2][1 - s];
+#pragma unroll
+  for (int s = -1; s <= 0; ++s) gU += L.f.p2.D[ty + 1 + s][tx + 2][4];
+  // ---- vertical fluxes (interfaces 1..nz-1 carry them), per point
+  const long long c = (long long)j * nx + i;
+  auto zface = [&](const double* f, const uint8_t* m3, int kk, long long cl, double wb,
+                   double* pd) -> double {
+    const long long a = (long long)kk * n2 + cl;
+    const double far_up = kk >= 2 ? f[a - 2 * n2] : 0.0;
+    const double far_dn = kk <= nz - 2 ? f[a + n2] : 0.0;
+    const bool hu = kk >= 2 && m3[a - 2 * n2] != 0;
+    const bool hd = kk <= nz - 2 && m3[a + n2] != 0;
+    return kappa_face_d(far_dn, f[a], f[a - n2], far_up, wb, hd, hu, lim, pd);
+  };
+  const long long ce = col(j, i + 1), cn = col(j + 1, i);
+  for (int kk = k - 1; kk <= k + 2; ++kk) {
+    if (kk < 1 || kk > nz - 1) continue;
+    const long long a = (long long)kk * n2;
+    const int role = kk == k - 1 ? 0 : kk == k ? 1 : kk == k + 1 ? 2 : 3;
+    const double wbu = 0.5 * (w0[a + c] + w0[a + ce]);
+    zface(u0, geo.m3u, kk, c, wbu, dp);
+    gU += (Gu(a + c) - Gu(a - n2 + c)) / p.dz_nom * wbu * dp[role];
+    const double wbv = 0.5 * (w0[a + c] + w0[a + cn]);
+    zface(v0, geo.m3v, kk, c, wbv, dp);
+    gV += (Gv(a + c) - Gv(a - n2 + c)) / p.dz_nom * wbv * dp[role];
+  }
+  double gW = 0.0;
+  if (k >= 1) {  // interface k: w enters wb of the u columns i-1, i and v rows j-1, j
+    const long long a = (long long)k * n2;
+    for (int s = -1; s <= 0; ++s) {
+      const long long cu = col(j, i + s), cue = col(j, i + s + 1);
+      const double wbu = 0.5 * (w0[a + cu] + w0[a + cue]);
+      const double fu = zface(u0, geo.m3u, k, cu, wbu, dp);
+      gW += (Gu(a + cu) - Gu(a - n2 + cu)) / p.dz_nom * fu * 0.5;
+      const long long cv = col(j + s, i), cvn = col(j + s + 1, i);
+      const double wbv = 0.5 * (w0[a + cv] + w0[a + cvn]);
+      const double fv = zface(v0, geo.m3v, k, cv, wbv, dp);
+      gW += (Gv(a + cv) - Gv(a - n2 + cv)) / p.dz_nom * fv * 0.5;
+    }
+  }
+  const long long t = o + c;
+  gu0[t] += gU;
+  gv0[t] += gV;
+  gw0[t] = gW;
+  if (k == nz - 1) gw0[(long long)nz * n2 + c] = 0.0;
 }
 
 // Per column: the old w's adjoint into gu0, gv0 (w_diag reversed), then

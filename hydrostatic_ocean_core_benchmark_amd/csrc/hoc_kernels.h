@@ -174,8 +174,9 @@ __global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* T0, c
                               const double* gS1, const double* gb1, const double* Kh,
                               double* lamT, double* lamS, double* cs, double* gT0, double* gS0,
                               double* gKh);
-// k_rev_tracer_stencil: one workgroup per RTS_TX x RTS_TY tile of one level
+// k_rev_tracer_stencil and k_rev_momentum_adv: one workgroup per tile of one level
 constexpr int RTS_TX = 32, RTS_TY = 8;
+constexpr int RMA_TX = 32, RMA_TY = 8;
 __global__ void k_rev_tracer_stencil(Dims d, StepParams p, Geo geo, const double* T0,
                                      const double* S0, const double* u1, const double* v1,
                                      const double* w1, const double* lamT, const double* lamS,

@@ -2308,9 +2308,10 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
   // the other accumulators (gq, Lu, Lv, gx->u, gx->v, gx->eta, gdiv, gKu,
   // gKv) are written by their first writer in the last Picard pass's reverse
   // tracers and the new velocity's w (model3d_v05.py:466-533)
-  k_rev_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->T, y->S, gy->T, gy->S,
-                                             gy->b, Kh, h->rv_lamT, h->rv_lamS, h->rv_cs, gx->T,
-                                             gx->S, g_Kh);
+  k_rev_tracer_rhs<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, y->T, y->S, gy->T, gy->S, gy->b,
+                                                h->rv_lamT, h->rv_lamS);
+  k_rev_tracers<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->T, x->S, y->T, y->S, Kh, h->rv_lamT,
+                                             h->rv_lamS, h->rv_cs, gx->T, gx->S, g_Kh);
   k_rev_tracer_stencil<<<dim3((d.nx + RTS_TX - 1) / RTS_TX, (d.ny + RTS_TY - 1) / RTS_TY, d.nz),
                          RTS_TX * RTS_TY, 0, s>>>(d, p, geo, x->T, x->S, y->u, y->v, tape->w,
                                                     h->rv_lamT, h->rv_lamS, gy->u, gy->v, gx->T,

@@ -787,35 +787,51 @@ __global__ void k_fill(long long n, double value, double* __restrict__ a) {
   if (t < n) a[t] = value;
 }
 
+// Right-hand sides of the reversed tracer solve, one cell per thread: the
+// adjoint of T_new, S_new plus that of b_new through the EOS (dual numbers
+// of buoyancy_t give both partials in one evaluation), masked, into rT, rS.
+__global__ void k_rev_tracer_rhs(Dims d, StepParams p, Geo geo, const double* __restrict__ T1,
+                                 const double* __restrict__ S1,
+                                 const double* __restrict__ gT1,
+                                 const double* __restrict__ gS1,
+                                 const double* __restrict__ gb1, double* __restrict__ rT,
+                                 double* __restrict__ rS) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const double m = mk(geo.m3[t]);
+  double a = gT1 ? gT1[t] : 0.0, b = gS1 ? gS1[t] : 0.0;
+  if (gb1 && geo.m3[t]) {
+    const Dual2 db = buoyancy_t<Dual2>(p, Dual2(T1[t], 1.0, 0.0), Dual2(S1[t], 0.0, 1.0),
+                                       Dual2(geo.zc[t]));
+    const double gb = gb1[t] * m;
+    a += gb * db.a;
+    b += gb * db.b;
+  }
+  rT[t] = a * m;
+  rS[t] = b * m;
+}
+
 // Tracers (model3d_v05.py:466-533) reversed per column for T and S at once:
-// the adjoint of T_new, S_new (plus that of b_new through the EOS, dual
-// numbers of buoyancy_t), masked, solved with tri_b^T; lambda_T, lambda_S
-// out; gT0 = lambda + (1 - theta_v) dt D^T lambda (the explicit diffusion's
-// adjoint); the nu adjoints of tri_b and of that diffusion added to gKh.
+// the right-hand sides k_rev_tracer_rhs left in lamT, lamS solved with
+// tri_b^T in place; lambda_T, lambda_S out; gT0 = lambda + (1 - theta_v) dt
+// D^T lambda (the explicit diffusion's adjoint); the nu adjoints of tri_b
+// and of that diffusion added to gKh.
 __global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* __restrict__ T0,
                               const double* __restrict__ S0, const double* __restrict__ T1,
-                              const double* __restrict__ S1, const double* __restrict__ gT1,
-                              const double* __restrict__ gS1, const double* __restrict__ gb1,
-                              const double* __restrict__ Kh, double* __restrict__ lamT,
+                              const double* __restrict__ S1, const double* __restrict__ Kh,
+                              double* __restrict__ lamT,
                               double* __restrict__ lamS, double* __restrict__ cs,
                               double* __restrict__ gT0, double* __restrict__ gS0,
                               double* __restrict__ gKh) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.n2) return;
   const long long n2 = d.n2;
+  // right-hand sides from k_rev_tracer_rhs; the solve overwrites them level
+  // by level after reading them
   auto g = [&](int k, double* out) {
     const long long t = (long long)k * n2 + c;
-    const double m = mk(geo.m3[t]);
-    double a = gT1 ? gT1[t] : 0.0, b = gS1 ? gS1[t] : 0.0;
-    if (gb1 && geo.m3[t]) {
-      const Dual2 db = buoyancy_t<Dual2>(p, Dual2(T1[t], 1.0, 0.0), Dual2(S1[t], 0.0, 1.0),
-                                         Dual2(geo.zc[t]));
-      const double gb = gb1[t] * m;
-      a += gb * db.a;
-      b += gb * db.b;
-    }
-    out[0] = a * m;
-    out[1] = b * m;
+    out[0] = lamT[t];
+    out[1] = lamS[t];
   };
   const double* xs[2] = {T1, S1};
   double* ls[2] = {lamT, lamS};

@@ -169,9 +169,11 @@ __global__ void k_eos_adjoint(long long n, StepParams p, const double* T, const 
 
 // the reverse step (hoc_stepper_step_adjoint)
 __global__ void k_fill(long long n, double value, double* a);
+__global__ void k_rev_tracer_rhs(Dims d, StepParams p, Geo geo, const double* T1,
+                                 const double* S1, const double* gT1, const double* gS1,
+                                 const double* gb1, double* rT, double* rS);
 __global__ void k_rev_tracers(Dims d, StepParams p, Geo geo, const double* T0, const double* S0,
-                              const double* T1, const double* S1, const double* gT1,
-                              const double* gS1, const double* gb1, const double* Kh,
+                              const double* T1, const double* S1, const double* Kh,
                               double* lamT, double* lamS, double* cs, double* gT0, double* gS0,
                               double* gKh);
 // k_rev_tracer_stencil and k_rev_momentum_adv: one workgroup per tile of one level

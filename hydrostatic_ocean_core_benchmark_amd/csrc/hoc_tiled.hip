@@ -12,7 +12,16 @@
 
 namespace hoc {
 
-constexpr int TX = 32, TY = 8, HALO = 2;
+// Tile shape: 32 x 8 (HOC_TILE_TX, HOC_TILE_TY override it at build time;
+// every shape with TX * TY = 256 gives the same bits).
+#ifndef HOC_TILE_TX
+#define HOC_TILE_TX 32
+#endif
+#ifndef HOC_TILE_TY
+#define HOC_TILE_TY 8
+#endif
+constexpr int TX = HOC_TILE_TX, TY = HOC_TILE_TY, HALO = 2;
+static_assert(TX * TY == 256 && TX + TY <= 256, "tile of 256 threads");
 constexpr int SX = TX + 2 * HALO, SY = TY + 2 * HALO;
 constexpr int TILE_THREADS = TX * TY;
 

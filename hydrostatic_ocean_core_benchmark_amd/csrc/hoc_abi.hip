@@ -2347,8 +2347,10 @@ int hoc_stepper_step_adjoint(hoc_stepper* h, const hoc_state* x, const hoc_state
     LAUNCH_CHECK("reverse predictor");
   }
   // the explicit terms of the old state, its w and q (model3d_v05.py:150-167,349-386)
+  k_rev_explicit_cells<<<blocks(d.n3), TPB, 0, s>>>(d, p, geo, h->rv_Lu, h->rv_Lv, h->rv_gdiv,
+                                                    gx->u, gx->v);
   k_rev_explicit<<<blocks(d.n2), TPB, 0, s>>>(d, p, geo, x->u, x->v, Km, h->rv_Lu, h->rv_Lv,
-                                              h->rv_gdiv, gx->u, gx->v, gx->b, gx->eta, g_Km);
+                                              gx->u, gx->v, gx->b, gx->eta, g_Km);
   k_rev_momentum_adv<<<dim3((d.nx + RMA_TX - 1) / RMA_TX, (d.ny + RMA_TY - 1) / RMA_TY, d.nz),
                        RMA_TX * RMA_TY, 0, s>>>(d, p, geo, x->u, x->v, tape->w_old, h->rv_Lu,
                                                 h->rv_Lv, gx->u, gx->v, h->rv_gw0);

@@ -1193,16 +1193,42 @@ __global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* __re
   }
 }
 
-// The step's explicit terms reversed for one column, from Lu, Lv (the sums
-// of the predictor solves' adjoints over the passes, model3d_v05.py:
-// 349-386): gu0 += Lu + dz3u div^T(gdiv) + A_h dt Lap^T(Lu) and the
-// explicit diffusion's field adjoint (gKm takes its nu adjoint), gv0
-// likewise; geta0 += Gx^T of the surface-gradient term; gb0 = the pressure
-// gradients' adjoint (model3d_v05.py:195-230).
+// The step's explicit terms reversed, from Lu, Lv (the sums of the
+// predictor solves' adjoints over the passes, model3d_v05.py:349-386), one
+// cell per thread: gu0 += Lu + dz3u div^T(gdiv) + A_h dt Lap^T(Lu), gv0
+// likewise. k_rev_explicit then adds the column parts.
+__global__ void k_rev_explicit_cells(Dims d, StepParams p, Geo geo, const double* __restrict__ Lu,
+                                     const double* __restrict__ Lv,
+                                     const double* __restrict__ gdiv, double* __restrict__ gu0,
+                                     double* __restrict__ gv0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.n3) return;
+  const long long c = t % d.n2;
+  const int i = (int)(c % d.nx), j = (int)(c / d.nx);
+  const long long ce = at2(d, j, wrap(i, 1, d.nx)), cn = at2(d, wrap(j, 1, d.ny), i);
+  const double Dc = mk(geo.m2[c]) * gdiv[c];
+  const double DUx = (Dc - mk(geo.m2[ce]) * gdiv[ce]) / p.dx;
+  const double DVy = (Dc - mk(geo.m2[cn]) * gdiv[cn]) / p.dy;
+  const double dtA = p.dt * p.A_h;
+  double au = Lu[t] + geo.dz3u[t] * DUx;
+  double av = Lv[t] + geo.dz3v[t] * DVy;
+  if (p.has_A_h) {
+    au += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3u, Lu, t);
+    av += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3v, Lv, t);
+  }
+  gu0[t] += au;
+  gv0[t] += av;
+}
+
+// The column parts of the explicit terms reversed, after
+// k_rev_explicit_cells: the explicit diffusion's field adjoint added to
+// gu0, gv0 (gKm takes its nu adjoint); geta0 += Gx^T of the
+// surface-gradient term; gb0 = the pressure gradients' adjoint
+// (model3d_v05.py:195-230).
 __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* __restrict__ u0,
                                const double* __restrict__ v0, const double* __restrict__ Km,
                                const double* __restrict__ Lu, const double* __restrict__ Lv,
-                               const double* __restrict__ gdiv, double* __restrict__ gu0,
+                               double* __restrict__ gu0,
                                double* __restrict__ gv0, double* __restrict__ gb0,
                                double* __restrict__ geta0, double* __restrict__ gKm) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1212,21 +1238,9 @@ __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* __re
   const int i = (int)(c % d.nx), j = (int)(c / d.nx);
   const long long ce = at2(d, j, wrap(i, 1, d.nx)), cw = at2(d, j, wrap(i, -1, d.nx));
   const long long cn = at2(d, wrap(j, 1, d.ny), i), cs = at2(d, wrap(j, -1, d.ny), i);
-  const double Dc = mk(geo.m2[c]) * gdiv[c];
-  const double DUx = (Dc - mk(geo.m2[ce]) * gdiv[ce]) / p.dx;
-  const double DVy = (Dc - mk(geo.m2[cn]) * gdiv[cn]) / p.dy;
   double xc = 0.0, xw = 0.0, yc = 0.0, ys = 0.0;
-  const double dtA = p.dt * p.A_h;
   for (int k = 0; k < nz; ++k) {
     const long long o = (long long)k * n2, t = o + c;
-    double au = Lu[t] + geo.dz3u[t] * DUx;
-    double av = Lv[t] + geo.dz3v[t] * DVy;
-    if (p.has_A_h) {
-      au += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3u, Lu, t);
-      av += dtA * lap_T_point(d, p, geo.m3u, geo.m3v, geo.m3v, Lv, t);
-    }
-    gu0[t] += au;
-    gv0[t] += av;
     xc += mk(geo.m3u[t]) * (-p.c_gx) * Lu[t];
     xw += mk(geo.m3u[o + cw]) * (-p.c_gx) * Lu[o + cw];
     yc += mk(geo.m3v[t]) * (-p.c_gx) * Lv[t];

@@ -199,10 +199,13 @@ __global__ void k_rev_predictor(Dims d, StepParams p, Geo geo, const double* Km,
 __global__ void k_rev_coriolis(Dims d, StepParams p, Geo geo, const double* lu, const double* lv,
                                const double* Lu, const double* Lv, int first, double* gu,
                                double* gv);
+__global__ void k_rev_explicit_cells(Dims d, StepParams p, Geo geo, const double* Lu,
+                                     const double* Lv, const double* gdiv, double* gu0,
+                                     double* gv0);
 __global__ void k_rev_explicit(Dims d, StepParams p, Geo geo, const double* u0, const double* v0,
                                const double* Km, const double* Lu, const double* Lv,
-                               const double* gdiv, double* gu0, double* gv0, double* gb0,
-                               double* geta0, double* gKm);
+                               double* gu0, double* gv0, double* gb0, double* geta0,
+                               double* gKm);
 __global__ void k_rev_momentum_adv(Dims d, StepParams p, Geo geo, const double* u0,
                                    const double* v0, const double* w0, const double* Lu,
                                    const double* Lv, double* gu0, double* gv0, double* gw0);
